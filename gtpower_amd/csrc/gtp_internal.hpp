@@ -1,0 +1,112 @@
+// Internal layouts shared by the host runtime (gtp_api.cpp) and the gfx950
+// kernels (gtp_kernels.hip).
+//
+// Device event store: one 32-bit word per event, per-net CSR order,
+//   word = (time << 2) | value,  time < 2^30 ticks, value in {0,1,2,3}.
+// Times compare as words (the value bits are below the time bits), so a
+// lower_bound on time is a lower_bound on words.  EV_INF is past every event.
+//
+// Per-cycle results are accumulated as signed 64-bit fixed point: a term x is
+// added as llrint(x / quantum), which makes the sums exact, order-free and
+// therefore bit-reproducible.  Per-gate sums are float-float (hi + lo)
+// compensated accumulations.
+#pragma once
+#include <stdint.h>
+
+namespace gtp {
+
+constexpr uint32_t EV_TIME_BITS = 30;
+constexpr uint32_t EV_INF = 0xFFFFFFFFu;       // time field 2^30 - 1
+constexpr uint32_t EV_TIME_INF = 0x3FFFFFFFu;
+constexpr int MAXP_SMALL = 4;                   // kernel variant for cells with <= 4 pins
+constexpr int MAXP_LARGE = 6;                   // GTP_MAX_CELL_PINS
+constexpr int MAX_CELLS_LDS = 64;               // cell types staged in LDS
+constexpr int ARC_FRAC_BITS = 20;               // related-pin comparison grid: 2^-20 tick
+constexpr int64_t ARC_NONE = INT64_MIN;
+
+// Per-cycle accumulator components (LDS bins and partials).
+enum { BIN_INT = 0, BIN_GINT = 1, BIN_SW = 2, BIN_GSW = 3, BIN_LD = 4, BIN_LW = 5, N_BIN = 6 };
+// Per-gate partial components (float-float).
+enum { PG_INT = 0, PG_GINT = 1, PG_SW = 2, PG_GSW = 3, PG_LEAK = 4, N_PG = 5 };
+
+// One cell type as the power kernel sees it.
+struct CellDev {
+  int32_t n_pin, n_in;
+  uint32_t clock_mask;
+  int32_t state_bits;        // n_pin when some internal power is `when`-conditioned, else 0
+  int32_t leak_off;          // into the LDS leakage table (2^n_pin floats)
+  int32_t n_slot;            // energy slots per gate of this cell
+  int32_t slot_off[6];       // energy slot base of each pin
+};
+
+// One energy slot of a cell: which table to interpolate at which pin's slew.
+struct SlotDesc {
+  int32_t table;             // -1: the slot's energy is 0
+  int16_t slew_pin;          // cell pin whose slew is the table's slew axis
+  int8_t slew_rf;            // 0 rise, 1 fall
+  int8_t cap_pin;            // cell pin whose load cap is the cap axis; -1 = 0.0
+  int32_t list_off;          // default-energy slots: mean over list entries
+  int32_t list_cnt;          // 0 = plain slot
+};
+struct ListEntry {
+  int32_t table;
+  int16_t slew_pin;
+  int8_t slew_rf;
+  int8_t cap_pin;
+};
+
+struct TableDev {             // offsets into the staged float table data
+  int32_t n1, n2, var1, var2;
+  int32_t axis1, axis2, values;
+};
+
+// Arguments of the power kernel.
+struct PowerArgs {
+  // gates in schedule order
+  const int32_t* order;          // [n_sched] gate index
+  int64_t n_sched;
+  const int32_t* gate_cell;      // [G]
+  const int64_t* gate_pin_off;   // [G+1]
+  const int64_t* gate_arc_off;   // [G+1]
+  const int64_t* gate_energy_off;// [G]
+  // pins
+  const int32_t* pin_net;        // [P]
+  const float* pin_gk;           // [P] 2 / (slew_rise + slew_fall): glitch ratio factor (1/s)
+  float time_scale;              // seconds per tick
+  const float* pin_csw;          // [P] C_load * V^2 of output pins
+  const int8_t* pin_default;     // [P]
+  // per-gate energies and arcs
+  const float* energy;           // [sum slots]
+  const int64_t* arc_q;          // [sum arcs] delay in 2^-20 ticks, ARC_NONE = no arc
+  // event store
+  const int64_t* net_off;        // [N+1]
+  const uint32_t* ev;            // [E]
+  const int32_t* tile_idx;       // [N * (n_tile + 1)] first event index >= tile start (net-relative)
+  int32_t n_tile;
+  int32_t tile_cycles;
+  int64_t T;                     // ticks per cycle
+  int64_t t_max;
+  int64_t n_cycle;
+  float inv_T;
+  // work decomposition
+  int32_t n_range;               // gate ranges per tile (all classes)
+  int32_t range_gates;           // gates per range; order[] is padded with -1 to whole ranges
+  int32_t xcd_remap;             // 1: remap blockIdx so each XCD takes a contiguous block of work
+  // cell tables (staged into LDS)
+  const CellDev* cells;
+  int32_t n_cell;
+  const float* leak;             // [n_leak] leakage per gate state (W)
+  const long long* leak_q;       // [n_leak] the same in leakage quanta
+  int32_t n_leak;
+  // fixed-point scales: term / quantum
+  float inv_q_energy;
+  float inv_q_leak;
+  // outputs
+  float* pg_hi;                  // [N_PG][n_tile][n_sched]
+  float* pg_lo;
+  uint32_t* pg_tog;              // [n_tile][n_sched]
+  uint32_t* pg_glitch;           // [n_tile][n_sched]
+  long long* cyc_part;           // [n_tile][n_range][N_BIN][tile_cycles]
+};
+
+}  // namespace gtp

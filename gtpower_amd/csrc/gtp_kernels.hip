@@ -1,0 +1,625 @@
+// gfx950 kernels of the time-based power engine.
+//
+//   pack_events_kernel     int64 time + u8 value -> one 32-bit event word (device event store)
+//   tile_index_kernel      per (net, tile): first event at or after the tile start
+//   energy_kernel          Liberty table interpolation per gate energy slot, tables staged in LDS
+//   power_tile_kernel<P>   fused per-gate event merge over one cycle tile: toggle and glitch
+//                          filtering, internal / switching / leakage evaluation, per-cycle
+//                          binning in LDS (int64 fixed point) and per-gate float-float sums
+//   cycle_reduce_kernel    per-cycle sum over gate ranges -> FP64 waveforms
+//   gate_reduce_kernel     per-gate sum over tiles -> FP64 per-gate power
+//   total_kernel           FP64 design and class totals
+//
+// Reference counterparts: kernel1AllPowerCalculationTimeRangePartitionedByCycleUnit
+// (power/cuda/CudaPower.cu:317-584), kernel2MergeAllPowerResult (:586-606), the
+// CPU algorithm they restate (power/TimeBasedPower.cc:703-901), and the device
+// LUTs (util/cuda/OneDimensionalLUT.cuh:62-86, TwoDimensionalLUT.cuh:113-164).
+#include <hip/hip_runtime.h>
+
+#include "gtp_internal.hpp"
+
+namespace gtp {
+
+__device__ __forceinline__ uint32_t ev_time(uint32_t w) { return w >> 2; }
+__device__ __forceinline__ int ev_val(uint32_t w) { return (int)(w & 3u); }
+
+// power/PowerUtils.cc:246-256 getNToggle and :213-236 getNRise, times two.
+__device__ __forceinline__ int toggle_x2(int p, int c) {
+  return p == c ? 0 : ((p == 2 || c == 2) ? 1 : 2);
+}
+__device__ __forceinline__ int rise_x2(int p, int c) {
+  if (c == 1) return p == 0 ? 2 : (p == 2 ? 1 : 0);
+  return (c == 2 && p == 0) ? 1 : 0;
+}
+
+// floor(t / T) for t < 2^30: a float estimate, then integer correction.
+__device__ __forceinline__ uint32_t cycle_of(uint32_t t, uint32_t T, float invT) {
+  uint32_t q = (uint32_t)((float)t * invT);
+  int32_t r = (int32_t)(t - q * T);
+  while (r < 0) { --q; r += (int32_t)T; }
+  while (r >= (int32_t)T) { ++q; r -= (int32_t)T; }
+  return q;
+}
+
+// Compensated accumulation (TwoSum): hi + lo carries ~48 significant bits.
+__device__ __forceinline__ void ff_add(float& hi, float& lo, float x) {
+  float s = hi + x;
+  float bp = s - hi;
+  float err = (hi - (s - bp)) + (x - bp);
+  hi = s;
+  lo += err;
+}
+// hi + lo += a * b with the product's rounding error kept (fma).
+__device__ __forceinline__ void ff_add_prod(float& hi, float& lo, float a, float b) {
+  float p = a * b;
+  float e = fmaf(a, b, -p);
+  ff_add(hi, lo, p);
+  lo += e;
+}
+
+__device__ __forceinline__ long long quant(float x, float inv_q) {
+  return __float2ll_rn(x * inv_q);
+}
+
+// ---------------------------------------------------------------------------
+// Event store
+// ---------------------------------------------------------------------------
+__global__ void pack_events_kernel(const int64_t* __restrict__ t, const uint8_t* __restrict__ v,
+                                   uint32_t* __restrict__ out, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) out[i] = ((uint32_t)t[i] << 2) | (uint32_t)(v[i] & 3u);
+}
+
+__global__ void tile_index_kernel(const int64_t* __restrict__ net_off, const uint32_t* __restrict__ ev,
+                                  int64_t n_net, int32_t n_tile, int64_t tile_ticks,
+                                  int32_t* __restrict__ out) {
+  const int64_t per = (int64_t)n_tile + 1;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n_net * per; i += stride) {
+    const int64_t n = i / per;
+    const int32_t k = (int32_t)(i - n * per);
+    const int64_t b = net_off[n];
+    const int32_t cnt = (int32_t)(net_off[n + 1] - b);
+    int32_t lo = 0, hi = cnt;
+    if (k == n_tile) {
+      lo = cnt;
+    } else {
+      const int64_t tk = (int64_t)k * tile_ticks;
+      const uint32_t key = tk >= (int64_t)EV_TIME_INF ? EV_INF : (uint32_t)(tk << 2);
+      while (lo < hi) {
+        int32_t mid = (lo + hi) >> 1;
+        if (ev[b + mid] < key) lo = mid + 1; else hi = mid;
+      }
+    }
+    out[i] = lo;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Liberty table interpolation (FP32), tables staged in LDS
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int find_index_f(const float* ax, int n, float x) {
+  // liberty/TableModel.cc:1465-1488 findValueIndex
+  if (n <= 1 || x <= ax[0]) return 0;
+  if (x >= ax[n - 1]) return n - 2;
+  int lo = -1, hi = n;
+  while (hi - lo > 1) {
+    int mid = (hi + lo) >> 1;
+    if (x >= ax[mid]) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ float interp_f(const float* D, const TableDev& tb, float slew, float cap) {
+  // liberty/TableModel.cc:858-1062 Table1/Table2::findValue
+  const float x1 = tb.var1 == 0 ? slew : cap;
+  const float x2 = tb.var2 == 0 ? slew : cap;
+  const float* a1 = D + tb.axis1;
+  const float* v = D + tb.values;
+  if (tb.n2 == 0) {
+    if (tb.n1 == 1) return v[0];
+    int i = find_index_f(a1, tb.n1, x1);
+    float d = (x1 - a1[i]) / (a1[i + 1] - a1[i]);
+    return (1.f - d) * v[i] + d * v[i + 1];
+  }
+  const float* a2 = D + tb.axis2;
+  const int n1 = tb.n1, n2 = tb.n2;
+  if (n1 == 1 && n2 == 1) return v[0];
+  if (n1 == 1) {
+    int j = find_index_f(a2, n2, x2);
+    float d = (x2 - a2[j]) / (a2[j + 1] - a2[j]);
+    return (1.f - d) * v[j] + d * v[j + 1];
+  }
+  if (n2 == 1) {
+    int i = find_index_f(a1, n1, x1);
+    float d = (x1 - a1[i]) / (a1[i + 1] - a1[i]);
+    return (1.f - d) * v[i] + d * v[i + 1];
+  }
+  int i = find_index_f(a1, n1, x1), j = find_index_f(a2, n2, x2);
+  float d1 = (x1 - a1[i]) / (a1[i + 1] - a1[i]);
+  float d2 = (x2 - a2[j]) / (a2[j + 1] - a2[j]);
+  return (1.f - d1) * (1.f - d2) * v[i * n2 + j] + d1 * (1.f - d2) * v[(i + 1) * n2 + j] +
+         d1 * d2 * v[(i + 1) * n2 + j + 1] + (1.f - d1) * d2 * v[i * n2 + j + 1];
+}
+
+struct EnergyArgs {
+  const int32_t* gate_cell;
+  const int64_t* gate_pin_off;
+  const int64_t* gate_energy_off;
+  int64_t n_gate;
+  const float* pin_slew;      // [2P]
+  const float* pin_cap;       // [P]
+  const CellDev* cells;
+  const int32_t* cell_slot_desc_off;  // [n_cell] into descs
+  const SlotDesc* descs;
+  const ListEntry* list;
+  const TableDev* tables;
+  int32_t n_table;
+  const float* table_data;
+  int32_t n_table_data;
+  float* energy;
+  unsigned int* max_abs_bits;
+};
+
+__global__ void energy_kernel(EnergyArgs a) {
+  extern __shared__ float s_tab[];
+  for (int i = threadIdx.x; i < a.n_table_data; i += blockDim.x) s_tab[i] = a.table_data[i];
+  __syncthreads();
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float local_max = 0.f;
+  if (g < a.n_gate) {
+    const int c = a.gate_cell[g];
+    const CellDev cd = a.cells[c];
+    const int64_t pb = a.gate_pin_off[g];
+    const int64_t eb = a.gate_energy_off[g];
+    const SlotDesc* ds = a.descs + a.cell_slot_desc_off[c];
+    for (int s = 0; s < cd.n_slot; ++s) {
+      const SlotDesc d = ds[s];
+      float e = 0.f;
+      if (d.list_cnt > 0) {
+        // mean over (related input, rf, table) entries with a finite slew
+        // (power/TimeBasedPower.cc:511-555)
+        float tot = 0.f;
+        int n = 0;
+        for (int k = 0; k < d.list_cnt; ++k) {
+          const ListEntry le = a.list[d.list_off + k];
+          const float slew = a.pin_slew[2 * (pb + le.slew_pin) + le.slew_rf];
+          if (!isfinite(slew)) continue;
+          const float cap = le.cap_pin >= 0 ? a.pin_cap[pb + le.cap_pin] : 0.f;
+          if (le.table >= 0) tot += interp_f(s_tab, a.tables[le.table], slew, cap);
+          ++n;
+        }
+        e = n ? tot / (float)n : 0.f;
+      } else if (d.table >= 0) {
+        const float slew = a.pin_slew[2 * (pb + d.slew_pin) + d.slew_rf];
+        if (isfinite(slew)) {  // delayInf(slew) -> 0 (TimeBasedPower.cc:367-370, :488)
+          const float cap = d.cap_pin >= 0 ? a.pin_cap[pb + d.cap_pin] : 0.f;
+          e = interp_f(s_tab, a.tables[d.table], slew, cap);
+        }
+      }
+      a.energy[eb + s] = e;
+      local_max = fmaxf(local_max, fabsf(e));
+    }
+  }
+  // wave max, then one atomic per wave (positive floats order like their bits)
+  for (int off = 32; off > 0; off >>= 1) local_max = fmaxf(local_max, __shfl_xor(local_max, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(a.max_abs_bits, __float_as_uint(local_max));
+}
+
+// ---------------------------------------------------------------------------
+// The fused power kernel
+// ---------------------------------------------------------------------------
+template <int MAXP>
+__global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t range_begin,
+                                                          int32_t range_count) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int TC = a.tile_cycles;
+  long long* bins = reinterpret_cast<long long*>(smem_raw);
+  long long* s_leakq = bins + (size_t)N_BIN * TC;
+  float* s_leak = reinterpret_cast<float*>(s_leakq + a.n_leak);
+  CellDev* s_cells = reinterpret_cast<CellDev*>(s_leak + a.n_leak + (a.n_leak & 1));
+  __shared__ long long s_base;
+
+  // work item -> (tile, range); XCD-aware: each XCD takes a contiguous block of
+  // tile-major work items, so the gates of one tile share that XCD's L2.
+  const int64_t n_work = (int64_t)a.n_tile * range_count;
+  int64_t w = blockIdx.x;
+  if (a.xcd_remap) {
+    const int64_t per = (n_work + 7) / 8;
+    w = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  }
+  if (w >= n_work) return;
+  const int tile = (int)(w / range_count);
+  const int range = range_begin + (int)(w - (int64_t)tile * range_count);
+
+  for (int i = threadIdx.x; i < N_BIN * TC; i += blockDim.x) bins[i] = 0;
+  for (int i = threadIdx.x; i < a.n_leak; i += blockDim.x) {
+    s_leak[i] = a.leak[i];
+    s_leakq[i] = a.leak_q[i];
+  }
+  for (int i = threadIdx.x; i < a.n_cell; i += blockDim.x) s_cells[i] = a.cells[i];
+  if (threadIdx.x == 0) s_base = 0;
+  __syncthreads();
+
+  const uint32_t T = (uint32_t)a.T;
+  const float invT = a.inv_T;
+  const int64_t tc0 = (int64_t)tile * TC;
+  const int tcn = (int)min((int64_t)TC, a.n_cycle - tc0);
+  const bool last_tile = tile == a.n_tile - 1;
+  const uint32_t tstart = (uint32_t)(tc0 * (int64_t)T);
+  const uint32_t tend = last_tile ? EV_TIME_INF : (uint32_t)((tc0 + TC) * (int64_t)T);
+  const uint32_t lend = last_tile ? (uint32_t)a.t_max : tend;
+  const uint32_t t_max = (uint32_t)a.t_max;
+  const int64_t tile_stride = (int64_t)a.n_tile + 1;
+
+  long long base_acc = 0;
+  const int64_t gi_begin = (int64_t)range * a.range_gates;
+  const int64_t gi_end = min(a.n_sched, gi_begin + a.range_gates);
+  for (int64_t gi = gi_begin + threadIdx.x; gi < gi_end; gi += blockDim.x) {
+    const int g = a.order[gi];
+    if (g < 0) continue;  // padding to a whole range
+    const CellDev cd = s_cells[a.gate_cell[g]];
+    const int n = cd.n_pin, nin = cd.n_in;
+    const int64_t pb = a.gate_pin_off[g];
+    const int64_t eb = a.gate_energy_off[g];
+    const int64_t ab = a.gate_arc_off[g];
+    const int sbits = cd.state_bits;
+
+    const uint32_t* evp[MAXP];
+    int32_t cnt[MAXP], idx[MAXP];
+    uint32_t w0[MAXP], w1[MAXP], w2[MAXP];
+    uint32_t c0[MAXP], c1[MAXP];
+    uint32_t lt[MAXP], lc[MAXP];
+    int cv[MAXP], jl[MAXP], rfl[MAXP];
+    bool lF[MAXP];
+    float gk[MAXP], csw[MAXP];
+    uint32_t sb = 0;
+#pragma unroll
+    for (int p = 0; p < MAXP; ++p) {
+      evp[p] = a.ev;
+      cnt[p] = 0;
+      idx[p] = 0;
+      w0[p] = w1[p] = w2[p] = EV_INF;
+      c0[p] = c1[p] = 0;
+      lt[p] = 0;
+      lc[p] = 0xFFFFFFFFu;
+      cv[p] = 2;
+      jl[p] = -1;
+      rfl[p] = 1;
+      lF[p] = false;
+      gk[p] = 0.f;
+      csw[p] = 0.f;
+      if (p < n) {
+        const int32_t net = a.pin_net[pb + p];
+        gk[p] = a.pin_gk[pb + p] * a.time_scale;
+        csw[p] = a.pin_csw[pb + p];
+        int64_t nb = 0;
+        if (net >= 0) {
+          nb = a.net_off[net];
+          cnt[p] = (int32_t)(a.net_off[net + 1] - nb);
+        }
+        if (cnt[p] == 0) {
+          cv[p] = a.pin_default[pb + p];
+        } else {
+          evp[p] = a.ev + nb;
+          const int32_t i0 = a.tile_idx[(int64_t)net * tile_stride + tile];
+          const bool isclk = (cd.clock_mask >> p) & 1u;
+          if (i0 == 0) {
+            const uint32_t e0 = evp[p][0];
+            if (tile == 0 && ev_time(e0) == 0) {
+              // the reference consumes a time-0 event as the initial state
+              // (power/TimeBasedPower.cc:782-787)
+              cv[p] = ev_val(e0);
+              idx[p] = 1;
+              lt[p] = 0;
+              lc[p] = 0;
+            } else {
+              cv[p] = 2;  // unknown initial state
+              idx[p] = 0;
+              lt[p] = ev_time(e0);
+              lc[p] = cycle_of(lt[p], T, invT);
+            }
+          } else {
+            const uint32_t el = evp[p][i0 - 1];
+            cv[p] = ev_val(el);
+            lt[p] = ev_time(el);
+            lc[p] = cycle_of(lt[p], T, invT);
+            idx[p] = i0;
+            if (i0 - 1 >= 1) {
+              rfl[p] = rise_x2(ev_val(evp[p][i0 - 2]), cv[p]) > 0 ? 0 : 1;
+              // position of event i0-1 in its cycle group; it is the group's
+              // last event because event i0 lies in a later cycle
+              int j = 0;
+              const uint32_t cstart = lc[p] * T;
+              for (int e = i0 - 2; e >= 1 && ev_time(evp[p][e]) >= cstart; --e) ++j;
+              jl[p] = j;
+              lF[p] = !isclk && (j & 1);
+            }
+          }
+          const int32_t i = idx[p];
+          w0[p] = i < cnt[p] ? evp[p][i] : EV_INF;
+          w1[p] = i + 1 < cnt[p] ? evp[p][i + 1] : EV_INF;
+          w2[p] = i + 2 < cnt[p] ? evp[p][i + 2] : EV_INF;
+          if (w0[p] != EV_INF) c0[p] = cycle_of(ev_time(w0[p]), T, invT);
+          if (w1[p] != EV_INF) c1[p] = cycle_of(ev_time(w1[p]), T, invT);
+        }
+      }
+      if (p < n && cv[p] == 1) sb |= 1u << p;
+    }
+    const int loff = cd.leak_off;
+    base_acc += s_leakq[loff + sb];
+
+    float acc_hi[N_PG], acc_lo[N_PG];
+#pragma unroll
+    for (int k = 0; k < N_PG; ++k) acc_hi[k] = acc_lo[k] = 0.f;
+    uint32_t tog = 0, glc = 0;
+    int pc = -1;  // pending cycle (tile-relative), -1 none
+    float p_int = 0.f, p_gint = 0.f, p_sw = 0.f, p_gsw = 0.f, p_lw = 0.f;
+    long long p_ld = 0;
+    uint32_t prev_t = tstart;
+
+    auto flush_pending = [&]() {
+      if (pc >= 0 && pc < tcn) {
+        if (p_int != 0.f) atomicAdd((unsigned long long*)&bins[BIN_INT * TC + pc], (unsigned long long)quant(p_int, a.inv_q_energy));
+        if (p_gint != 0.f) atomicAdd((unsigned long long*)&bins[BIN_GINT * TC + pc], (unsigned long long)quant(p_gint, a.inv_q_energy));
+        if (p_sw != 0.f) atomicAdd((unsigned long long*)&bins[BIN_SW * TC + pc], (unsigned long long)quant(p_sw, a.inv_q_energy));
+        if (p_gsw != 0.f) atomicAdd((unsigned long long*)&bins[BIN_GSW * TC + pc], (unsigned long long)quant(p_gsw, a.inv_q_energy));
+        if (p_ld != 0) atomicAdd((unsigned long long*)&bins[BIN_LD * TC + pc], (unsigned long long)p_ld);
+        if (p_lw != 0.f) atomicAdd((unsigned long long*)&bins[BIN_LW * TC + pc], (unsigned long long)quant(p_lw, a.inv_q_leak));
+      }
+      p_int = p_gint = p_sw = p_gsw = p_lw = 0.f;
+      p_ld = 0;
+    };
+
+    for (;;) {
+      uint32_t wm = w0[0];
+#pragma unroll
+      for (int p = 1; p < MAXP; ++p) wm = min(wm, w0[p]);
+      const uint32_t t = ev_time(wm);
+      if (t >= tend) break;
+      uint32_t c = 0;
+      bool trig[MAXP];
+      uint32_t sa = sb;
+#pragma unroll
+      for (int p = 0; p < MAXP; ++p) {
+        trig[p] = ev_time(w0[p]) == t;
+        if (trig[p]) {
+          c = c0[p];
+          sa = (sa & ~(1u << p)) | ((ev_val(w0[p]) == 1 ? 1u : 0u) << p);
+        }
+      }
+      const int cr = (int)((int64_t)c - tc0);
+      if (cr != pc) {
+        flush_pending();
+        pc = cr;
+      }
+      // leakage of the previous state over [prev_t, t) (TimeBasedPower.cc:831-843)
+      const float Lb = s_leak[loff + sb];
+      ff_add_prod(acc_hi[PG_LEAK], acc_lo[PG_LEAK], Lb, (float)(t - prev_t));
+      if (sa != sb) {
+        const float La = s_leak[loff + sa];
+        p_ld += s_leakq[loff + sa] - s_leakq[loff + sb];
+        const uint32_t cend = min((c + 1) * T, t_max);
+        p_lw += (La - Lb) * ((float)(cend - t) * invT);
+      }
+      // internal and switching energy of every pin changing at t (:845-883)
+      float ri = 0.f, rg = 0.f, rs = 0.f, gs = 0.f;
+      bool Fi[MAXP];
+      int ji[MAXP];
+#pragma unroll
+      for (int p = 0; p < MAXP; ++p) {
+        Fi[p] = false;
+        ji[p] = -1;
+        if (!trig[p]) continue;
+        const int pv = cv[p], nv = ev_val(w0[p]);
+        const int tx2 = toggle_x2(pv, nv);
+        const int r2 = rise_x2(pv, nv);
+        const int rf = r2 > 0 ? 0 : 1;
+        tog += (uint32_t)tx2;
+        // clock-cycle glitch flag of this event (TimeBasedPower.cc:589-637)
+        // and its scaling ratio from flagged neighbours (:303-339)
+        float ratio = -1.f;
+        const int i = idx[p];
+        if (i >= 1) {
+          const int j = (jl[p] >= 0 && lc[p] == c) ? jl[p] + 1 : 0;
+          ji[p] = j;
+          const uint32_t cend = (c + 1) * T;
+          const bool same1 = w1[p] != EV_INF && ev_time(w1[p]) < cend;
+          const bool isclk = (cd.clock_mask >> p) & 1u;
+          const bool F = !isclk && (same1 || (j & 1));
+          Fi[p] = F;
+          if (F) {
+            int64_t pw = lF[p] ? (int64_t)(t - lt[p]) : -1;
+            if (w1[p] != EV_INF) {
+              const int j1 = same1 ? j + 1 : 0;
+              const bool same2 = w2[p] != EV_INF && ev_time(w2[p]) < (c1[p] + 1) * T;
+              if (same2 || (j1 & 1)) pw = max(pw, (int64_t)(ev_time(w1[p]) - t));
+            }
+            if (pw > 0) {
+              const float r = (float)pw * gk[p];
+              ratio = fminf(1.f, r * r);
+            }
+          }
+        }
+        if (ratio > 0.f) ++glc;
+        float E;
+        if (p < nin) {
+          E = a.energy[eb + cd.slot_off[p] + ((rf << sbits) | (sbits ? (int)sb : 0))];
+        } else {
+          // related input: the latest event before t whose arc-delayed arrival is
+          // closest to t, within the same cycle (TimeBasedPower.cc:412-473)
+          const int o = p - nin;
+          int rel = -1, relrf = 0;
+          int64_t best = (int64_t)a.t_max << ARC_FRAC_BITS;
+#pragma unroll
+          for (int q = 0; q < MAXP; ++q) {
+            if (q >= nin || cnt[q] == 0) continue;
+            const int64_t d = a.arc_q[ab + (((int64_t)o * nin + q) * 2 + rfl[q]) * 2 + rf];
+            if (d == ARC_NONE) continue;
+            int64_t diff = (((int64_t)t - (int64_t)lt[q]) << ARC_FRAC_BITS) - d;
+            diff = diff < 0 ? -diff : diff;
+            if (diff < best && lc[q] == c) {
+              best = diff;
+              rel = q;
+              relrf = rfl[q];
+            }
+          }
+          const int64_t so = eb + cd.slot_off[p];
+          if (rel >= 0)
+            E = a.energy[so + ((((rel * 2 + relrf) * 2 + rf) << sbits) | (sbits ? (int)sb : 0))];
+          else
+            E = a.energy[so + ((nin * 4) << sbits) + rf];
+          if (r2 > 0) {
+            const float es = 0.5f * (float)r2 * csw[p];
+            if (ratio > 0.f) gs += ratio * es; else rs += es;
+          }
+        }
+        E *= 0.5f * (float)tx2;
+        if (ratio > 0.f) rg += ratio * E; else ri += E;
+      }
+      ff_add(acc_hi[PG_INT], acc_lo[PG_INT], ri);
+      ff_add(acc_hi[PG_GINT], acc_lo[PG_GINT], rg);
+      ff_add(acc_hi[PG_SW], acc_lo[PG_SW], rs);
+      ff_add(acc_hi[PG_GSW], acc_lo[PG_GSW], gs);
+      p_int += ri;
+      p_gint += rg;
+      p_sw += rs;
+      p_gsw += gs;
+      // consume the events at t
+#pragma unroll
+      for (int p = 0; p < MAXP; ++p) {
+        if (!trig[p]) continue;
+        const int nv = ev_val(w0[p]);
+        rfl[p] = (idx[p] == 0) ? 1 : (rise_x2(cv[p], nv) > 0 ? 0 : 1);
+        lF[p] = Fi[p];
+        jl[p] = ji[p];
+        lt[p] = t;
+        lc[p] = c;
+        cv[p] = nv;
+        const int32_t i = ++idx[p];
+        w0[p] = w1[p];
+        c0[p] = c1[p];
+        w1[p] = w2[p];
+        if (w1[p] != EV_INF) c1[p] = cycle_of(ev_time(w1[p]), T, invT);
+        w2[p] = i + 2 < cnt[p] ? evp[p][i + 2] : EV_INF;
+      }
+      sb = sa;
+      prev_t = t;
+    }
+    flush_pending();
+    if (lend > prev_t) ff_add_prod(acc_hi[PG_LEAK], acc_lo[PG_LEAK], s_leak[loff + sb], (float)(lend - prev_t));
+    const int64_t slot = (int64_t)tile * a.n_sched + gi;
+    const int64_t plane = (int64_t)a.n_tile * a.n_sched;
+#pragma unroll
+    for (int k = 0; k < N_PG; ++k) {
+      a.pg_hi[k * plane + slot] = acc_hi[k];
+      a.pg_lo[k * plane + slot] = acc_lo[k];
+    }
+    a.pg_tog[slot] = tog;
+    a.pg_glitch[slot] = glc;
+  }
+  // leakage at the tile start, summed over this workgroup's gates
+  for (int off = 32; off > 0; off >>= 1) base_acc += __shfl_xor(base_acc, off);
+  if ((threadIdx.x & 63) == 0) atomicAdd((unsigned long long*)&s_base, (unsigned long long)base_acc);
+  __syncthreads();
+
+  // flush: S_c = base + sum of leakage steps in earlier cycles of the tile
+  if (threadIdx.x == 0) {
+    long long run = s_base;
+    for (int ci = 0; ci < TC; ++ci) {
+      const long long d = bins[BIN_LD * TC + ci];
+      bins[BIN_LD * TC + ci] = run;
+      run += d;
+    }
+  }
+  __syncthreads();
+  long long* out = a.cyc_part + ((int64_t)tile * a.n_range + range) * N_BIN * TC;
+  for (int i = threadIdx.x; i < N_BIN * TC; i += blockDim.x) out[i] = bins[i];
+}
+
+template __global__ void power_tile_kernel<MAXP_SMALL>(PowerArgs, int32_t, int32_t);
+template __global__ void power_tile_kernel<MAXP_LARGE>(PowerArgs, int32_t, int32_t);
+
+// ---------------------------------------------------------------------------
+// Reductions
+// ---------------------------------------------------------------------------
+__global__ void cycle_reduce_kernel(const long long* __restrict__ part, int32_t n_range,
+                                    int32_t tile_cycles, int64_t n_cycle, int64_t T, int64_t t_max,
+                                    double q_energy, double q_leak, double inv_period,
+                                    double* __restrict__ out /* [5][C] */) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cycle) return;
+  const int64_t tile = c / tile_cycles;
+  const int ci = (int)(c - tile * tile_cycles);
+  long long s[N_BIN] = {0, 0, 0, 0, 0, 0};
+  for (int r = 0; r < n_range; ++r) {
+    const long long* p = part + ((tile * n_range + r) * N_BIN) * tile_cycles + ci;
+#pragma unroll
+    for (int k = 0; k < N_BIN; ++k) s[k] += p[(int64_t)k * tile_cycles];
+  }
+  const int64_t cend = min((c + 1) * T, t_max);
+  const double frac = (double)(cend - c * T) / (double)T;
+  out[0 * n_cycle + c] = ((double)s[BIN_LD] * frac + (double)s[BIN_LW]) * q_leak;
+  out[1 * n_cycle + c] = (double)(s[BIN_INT] + s[BIN_GINT]) * q_energy * inv_period;
+  out[2 * n_cycle + c] = (double)s[BIN_GINT] * q_energy * inv_period;
+  out[3 * n_cycle + c] = (double)(s[BIN_SW] + s[BIN_GSW]) * q_energy * inv_period;
+  out[4 * n_cycle + c] = (double)s[BIN_GSW] * q_energy * inv_period;
+}
+
+__global__ void gate_reduce_kernel(const float* __restrict__ hi, const float* __restrict__ lo,
+                                   const uint32_t* __restrict__ tog, const uint32_t* __restrict__ glc,
+                                   const int32_t* __restrict__ order, int64_t n_sched, int32_t n_tile,
+                                   int64_t n_gate, double norm_energy, double norm_leak,
+                                   double* __restrict__ gate_power, unsigned long long* __restrict__ gate_tog,
+                                   unsigned long long* __restrict__ gate_glitch) {
+  const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= n_sched) return;
+  const int g = order[gi];
+  if (g < 0) return;
+  const int64_t plane = (int64_t)n_tile * n_sched;
+#pragma unroll
+  for (int k = 0; k < N_PG; ++k) {
+    float h = 0.f, l = 0.f;
+    for (int t = 0; t < n_tile; ++t) {
+      ff_add(h, l, hi[k * plane + (int64_t)t * n_sched + gi]);
+      l += lo[k * plane + (int64_t)t * n_sched + gi];
+    }
+    const double v = (double)h + (double)l;
+    gate_power[(int64_t)k * n_gate + g] = v * (k == PG_LEAK ? norm_leak : norm_energy);
+  }
+  unsigned long long tg = 0, gg = 0;
+  for (int t = 0; t < n_tile; ++t) {
+    tg += tog[(int64_t)t * n_sched + gi];
+    gg += glc[(int64_t)t * n_sched + gi];
+  }
+  gate_tog[g] = tg;
+  gate_glitch[g] = gg;
+}
+
+// FP64 design and class totals, one workgroup, fixed summation order.
+__global__ void total_kernel(const double* __restrict__ gate_power, const int32_t* __restrict__ gate_class,
+                             int64_t n_gate, double* __restrict__ out /* [5] + [5 classes][5] */) {
+  __shared__ double s[256];
+  for (int k = 0; k < 5; ++k) {
+    for (int cls = -1; cls < 5; ++cls) {
+      double v = 0.0;
+      for (int64_t g = threadIdx.x; g < n_gate; g += blockDim.x)
+        if (cls < 0 || gate_class[g] == cls) v += gate_power[(int64_t)k * n_gate + g];
+      s[threadIdx.x] = v;
+      __syncthreads();
+      for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) {
+        if (cls < 0) out[k] = s[0];
+        else out[5 + cls * 5 + k] = s[0];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+}  // namespace gtp

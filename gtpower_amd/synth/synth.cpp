@@ -1,0 +1,693 @@
+// Seeded synthetic designs and activity traces (test/benchmark inputs).
+//
+// Design: gates are placed on `depth` logic levels; a gate on level l takes its
+// first input from level l-1 and the others from the three levels below, so the
+// netlist is a DAG in gate-index order.  Flip-flops sit on levels like other
+// gates (D from below, CLK from the clock net), which keeps the whole trace
+// simulatable level by level.  Load capacitance = wire cap + input pin caps;
+// slews and arc delays come from the cells' NLDM tables (liberty/TableModel.cc
+// interpolation semantics).
+//
+// Trace: primary inputs toggle once per cycle with probability p at a fixed
+// per-input offset; every gate output follows an inertial-delay simulation with
+// its arc delays (pulses narrower than the gate delay are swallowed), so only
+// reconvergent paths whose delays differ by more than a gate delay produce
+// glitches.  Every net starts
+// with an event at time 0 (the VCD initial value).  Flip-flop outputs start at
+// X and load D (as seen just before the edge) on every rising clock edge.
+#include "synth.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+inline uint64_t splitmix64(uint64_t& x) {
+  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+inline uint64_t hash2(uint64_t a, uint64_t b) {
+  uint64_t x = a ^ (b * 0xD1B54A32D192ED03ull + 0x632BE59BD9B4E019ull);
+  return splitmix64(x);
+}
+
+// xoshiro256** (Blackman & Vigna), seeded through splitmix64.
+struct Rng {
+  uint64_t s[4];
+  explicit Rng(uint64_t seed) {
+    uint64_t x = seed;
+    for (auto& v : s) v = splitmix64(x);
+  }
+  static inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+  uint64_t next() {
+    const uint64_t result = rotl(s[1] * 5, 7) * 9;
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return result;
+  }
+  double uniform() { return (next() >> 11) * 0x1.0p-53; }
+  uint64_t below(uint64_t n) { return n ? next() % n : 0; }
+  double normal() {
+    double u1 = uniform(), u2 = uniform();
+    if (u1 < 1e-300) u1 = 1e-300;
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+  }
+};
+
+template <class F>
+void parallel_for(int64_t n, int threads, F f) {
+  if (n <= 0) return;
+  if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+  if (threads == 1 || n < 64) {
+    for (int64_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int64_t> next{0};
+  const int64_t chunk = std::max<int64_t>(1, n / (threads * 16));
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t) {
+    pool.emplace_back([&] {
+      for (;;) {
+        int64_t b = next.fetch_add(chunk);
+        if (b >= n) break;
+        int64_t e = std::min(n, b + chunk);
+        for (int64_t i = b; i < e; ++i) f(i);
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+}
+
+// Liberty index search and linear/bilinear interpolation with boundary-segment
+// extrapolation (liberty/TableModel.cc:1465-1488, :858-1062), in double.
+int find_index(const double* ax, int n, double x) {
+  if (n <= 1 || x <= ax[0]) return 0;
+  if (x >= ax[n - 1]) return n - 2;
+  int lo = -1, hi = n;
+  while (hi - lo > 1) {
+    int mid = (hi + lo) >> 1;
+    if (x >= ax[mid]) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+double interp(const std::vector<gtp_table>& tables, const std::vector<double>& data, int t,
+              double slew, double cap) {
+  const gtp_table& tb = tables[t];
+  double x1 = tb.var1 == GTP_AXIS_SLEW ? slew : cap;
+  double x2 = tb.var2 == GTP_AXIS_SLEW ? slew : cap;
+  const double* a1 = &data[tb.axis1];
+  const double* v = &data[tb.values];
+  if (tb.n2 == 0) {
+    if (tb.n1 == 1) return v[0];
+    int i = find_index(a1, tb.n1, x1);
+    double d = (x1 - a1[i]) / (a1[i + 1] - a1[i]);
+    return (1 - d) * v[i] + d * v[i + 1];
+  }
+  const double* a2 = &data[tb.axis2];
+  int n1 = tb.n1, n2 = tb.n2;
+  if (n1 == 1 && n2 == 1) return v[0];
+  if (n1 == 1) {
+    int j = find_index(a2, n2, x2);
+    double d = (x2 - a2[j]) / (a2[j + 1] - a2[j]);
+    return (1 - d) * v[j] + d * v[j + 1];
+  }
+  if (n2 == 1) {
+    int i = find_index(a1, n1, x1);
+    double d = (x1 - a1[i]) / (a1[i + 1] - a1[i]);
+    return (1 - d) * v[i] + d * v[(i + 1)];
+  }
+  int i = find_index(a1, n1, x1), j = find_index(a2, n2, x2);
+  double d1 = (x1 - a1[i]) / (a1[i + 1] - a1[i]);
+  double d2 = (x2 - a2[j]) / (a2[j + 1] - a2[j]);
+  return (1 - d1) * (1 - d2) * v[i * n2 + j] + d1 * (1 - d2) * v[(i + 1) * n2 + j] +
+         d1 * d2 * v[(i + 1) * n2 + j + 1] + (1 - d1) * d2 * v[i * n2 + j + 1];
+}
+
+// 3-valued evaluation of a truth table over n inputs (values 0, 1, 2 = X, 3 = Z
+// treated as X): the output is known when every completion of the unknown
+// inputs agrees.
+inline uint8_t eval3(uint64_t func, int n, const uint8_t* v) {
+  uint32_t known = 0, unk = 0;
+  for (int k = 0; k < n; ++k) {
+    if (v[k] == 1) known |= 1u << k;
+    else if (v[k] != 0) unk |= 1u << k;
+  }
+  if (!unk) return (func >> known) & 1;
+  int seen0 = 0, seen1 = 0;
+  // enumerate subsets of unk
+  uint32_t sub = 0;
+  for (;;) {
+    if ((func >> (known | sub)) & 1) seen1 = 1; else seen0 = 1;
+    if (seen0 && seen1) return 2;
+    if (sub == unk) break;
+    sub = (sub - unk) & unk;
+  }
+  return seen1 ? 1 : 0;
+}
+
+inline bool is_rise(uint8_t prev, uint8_t cur) {
+  // getNRise > 0 (power/PowerUtils.cc:213-236)
+  return (cur == 1 && (prev == 0 || prev == 2)) || (cur == 2 && prev == 0);
+}
+
+}  // namespace
+
+struct gtps_design {
+  int32_t n_cell = 0;
+  std::vector<int32_t> cell_n_pin, cell_n_in;
+  std::vector<gtp_table> tables;
+  std::vector<double> table_data;
+  std::vector<int32_t> arc_tables;
+  std::vector<uint64_t> cell_func;
+  int64_t G = 0, P = 0, N = 0, n_pi = 0;
+  int64_t T = 0;
+  double time_scale = 0, pi_slew = 0;
+  int32_t dff_cell = -1;
+  int threads = 0;
+  std::vector<int32_t> gate_cell, gate_level;
+  std::vector<int64_t> gate_pin_offset, gate_arc_offset, level_start;
+  std::vector<double> arc_delay;
+  std::vector<int32_t> arc_ticks;  // parallel to arc_delay, -1 = no arc
+  std::vector<int32_t> pin_net;
+  std::vector<double> pin_slew, pin_load_cap, pin_voltage;
+  std::vector<int8_t> pin_default_state;
+  std::vector<int32_t> net_driver;
+  int arc(int cell, int in, int fr, int to, int k) const {
+    return arc_tables[((((size_t)cell * GTP_MAX_CELL_PINS + in) * 2 + fr) * 2 + to) * 2 + k];
+  }
+};
+
+struct gtps_trace {
+  int64_t N = 0, t_max = 0;
+  double time_scale = 0, clk_period = 0;
+  std::vector<int64_t> off;
+  std::vector<int64_t> time;
+  std::vector<uint8_t> value;
+};
+
+extern "C" {
+
+const char* gtps_last_error(void) { return g_err.c_str(); }
+
+int32_t gtps_design_create(const gtp_library* lib, const int32_t* arc_tables,
+                           const uint64_t* cell_func, const double* cell_pin_cap,
+                           const gtps_design_params* p, gtps_design** out) {
+  try {
+    if (!lib || !arc_tables || !cell_func || !cell_pin_cap || !p || !out || p->n_gate <= 0 ||
+        p->depth <= 0 || p->n_mix <= 0 || p->ticks_per_cycle <= 1) {
+      g_err = "gtps_design_create: bad arguments";
+      return GTP_ERR_ARG;
+    }
+    auto* d = new gtps_design();
+    d->n_cell = lib->n_cell;
+    d->cell_n_pin.assign(lib->cell_n_pin, lib->cell_n_pin + lib->n_cell);
+    d->cell_n_in.assign(lib->cell_n_in, lib->cell_n_in + lib->n_cell);
+    d->tables.assign(lib->tables, lib->tables + lib->n_table);
+    size_t data_len = 0;
+    for (auto& t : d->tables) {
+      size_t end = (size_t)t.values + (size_t)t.n1 * (t.n2 ? t.n2 : 1);
+      data_len = std::max(data_len, end);
+    }
+    d->table_data.assign(lib->table_data, lib->table_data + data_len);
+    d->arc_tables.assign(arc_tables, arc_tables + (size_t)lib->n_cell * GTP_MAX_CELL_PINS * 8);
+    d->cell_func.assign(cell_func, cell_func + lib->n_cell);
+    d->G = p->n_gate;
+    d->T = p->ticks_per_cycle;
+    d->time_scale = p->time_scale;
+    d->pi_slew = p->pi_slew;
+    d->dff_cell = p->dff_cell;
+    d->threads = p->threads;
+    const int L = p->depth;
+    d->n_pi = p->n_pi > 0 ? p->n_pi : std::max<int64_t>(8, d->G / L);
+    const int64_t G = d->G;
+    Rng rng(hash2(p->seed, 0x6e65746cull));
+
+    // cells: choose DFF positions exactly, the rest from the mix.
+    int64_t n_dff = (p->dff_cell >= 0) ? (int64_t)std::llround(p->dff_fraction * (double)G) : 0;
+    std::vector<uint8_t> is_dff(G, 0);
+    for (int64_t i = 0; i < n_dff; ++i) is_dff[i] = 1;
+    for (int64_t i = G - 1; i > 0; --i) std::swap(is_dff[i], is_dff[rng.below(i + 1)]);
+    double wsum = 0;
+    for (int k = 0; k < p->n_mix; ++k) wsum += p->mix_weights[k];
+    d->gate_cell.resize(G);
+    d->gate_level.resize(G);
+    d->level_start.assign(L + 2, 0);
+    for (int64_t g = 0; g < G; ++g) {
+      int lvl = 1 + (int)((g * L) / G);
+      d->gate_level[g] = lvl;
+      if (is_dff[g]) {
+        d->gate_cell[g] = p->dff_cell;
+      } else {
+        double u = rng.uniform() * wsum;
+        int k = 0;
+        while (k + 1 < p->n_mix && u >= p->mix_weights[k]) { u -= p->mix_weights[k]; ++k; }
+        d->gate_cell[g] = p->mix_cells[k];
+      }
+    }
+    for (int l = 0; l <= L + 1; ++l) d->level_start[l] = 0;
+    for (int64_t g = 0; g < G; ++g) d->level_start[d->gate_level[g] + 1]++;
+    for (int l = 1; l <= L + 1; ++l) d->level_start[l] += d->level_start[l - 1];
+    // level_start[l] = first gate of level l (levels 1..L); level_start[1] == 0.
+
+    // nets: 0 clock, 1..n_pi primary inputs, n_pi + 1 + g output of gate g
+    const int64_t N = 1 + d->n_pi + G;
+    d->N = N;
+    d->net_driver.assign(N, -1);
+    for (int64_t g = 0; g < G; ++g) d->net_driver[1 + d->n_pi + g] = (int32_t)g;
+    std::vector<int32_t> cap(N, 0);
+    std::vector<std::vector<int32_t>> open(L + 1);  // open nets per level (level 0 = PIs)
+    std::vector<std::vector<int32_t>> all(L + 1);
+    for (int64_t i = 0; i < d->n_pi; ++i) {
+      int32_t n = (int32_t)(1 + i);
+      cap[n] = p->fanout_min + (int)rng.below(p->fanout_max - p->fanout_min + 1);
+      open[0].push_back(n);
+      all[0].push_back(n);
+    }
+    // pins
+    d->gate_pin_offset.resize(G + 1);
+    d->gate_arc_offset.resize(G + 1);
+    d->gate_pin_offset[0] = 0;
+    d->gate_arc_offset[0] = 0;
+    for (int64_t g = 0; g < G; ++g) {
+      int c = d->gate_cell[g];
+      d->gate_pin_offset[g + 1] = d->gate_pin_offset[g] + d->cell_n_pin[c];
+      int n_out = d->cell_n_pin[c] - d->cell_n_in[c];
+      d->gate_arc_offset[g + 1] = d->gate_arc_offset[g] + (int64_t)n_out * d->cell_n_in[c] * 4;
+    }
+    d->P = d->gate_pin_offset[G];
+    d->pin_net.assign(d->P, -1);
+    auto pick = [&](int level, const int32_t* avoid, int n_avoid) -> int32_t {
+      for (int attempt = 0; attempt < 12; ++attempt) {
+        int32_t n;
+        auto& op = open[level];
+        size_t j = 0;
+        bool from_open = !op.empty() && attempt < 8;
+        if (from_open) {
+          j = rng.below(op.size());
+          n = op[j];
+        } else {
+          n = all[level][rng.below(all[level].size())];
+        }
+        bool dup = false;
+        for (int k = 0; k < n_avoid; ++k) dup |= (avoid[k] == n);
+        if (dup) continue;
+        if (from_open && --cap[n] == 0) {
+          op[j] = op.back();
+          op.pop_back();
+        }
+        return n;
+      }
+      return all[level][rng.below(all[level].size())];
+    };
+    for (int64_t g = 0; g < G; ++g) {
+      int c = d->gate_cell[g];
+      int lvl = d->gate_level[g];
+      int nin = d->cell_n_in[c];
+      int64_t pb = d->gate_pin_offset[g];
+      int32_t chosen[GTP_MAX_CELL_PINS];
+      if (c == p->dff_cell) {
+        d->pin_net[pb + 0] = 0;  // CLK
+        int lv = lvl - 1 - (int)rng.below(std::min(lvl, 4));
+        d->pin_net[pb + 1] = pick(lv, nullptr, 0);
+      } else {
+        for (int k = 0; k < nin; ++k) {
+          int lv = lvl - 1 - (int)rng.below(std::min(lvl, 4));
+          chosen[k] = pick(lv, chosen, k);
+          d->pin_net[pb + k] = chosen[k];
+        }
+      }
+      int32_t on = (int32_t)(1 + d->n_pi + g);
+      d->pin_net[pb + nin] = on;  // single output
+      cap[on] = p->fanout_min + (int)rng.below(p->fanout_max - p->fanout_min + 1);
+      open[lvl].push_back(on);
+      all[lvl].push_back(on);
+    }
+
+    // load capacitance per net: wire + input pin caps
+    std::vector<double> net_cap(N, 0.0);
+    for (int64_t n = 0; n < N; ++n) {
+      Rng r(hash2(p->seed ^ 0x77697265ull, (uint64_t)n));
+      net_cap[n] = p->wire_cap_min + (p->wire_cap_max - p->wire_cap_min) * r.uniform();
+    }
+    for (int64_t g = 0; g < G; ++g) {
+      int c = d->gate_cell[g];
+      for (int k = 0; k < d->cell_n_in[c]; ++k) {
+        int32_t n = d->pin_net[d->gate_pin_offset[g] + k];
+        net_cap[n] += cell_pin_cap[(size_t)c * GTP_MAX_CELL_PINS + k];
+      }
+    }
+    // slews and arc delays, level by level
+    std::vector<double> net_slew(2 * N, p->pi_slew);
+    d->pin_slew.assign(2 * d->P, p->pi_slew);
+    d->pin_load_cap.assign(d->P, 0.0);
+    d->pin_voltage.assign(d->P, 1.8);
+    d->pin_default_state.assign(d->P, 2);
+    d->arc_delay.assign(d->gate_arc_offset[G], std::nan(""));
+    d->arc_ticks.assign(d->gate_arc_offset[G], -1);
+    for (int l = 1; l <= L; ++l) {
+      int64_t g0 = d->level_start[l], g1 = d->level_start[l + 1];
+      parallel_for(g1 - g0, d->threads, [&](int64_t i) {
+        int64_t g = g0 + i;
+        int c = d->gate_cell[g];
+        int nin = d->cell_n_in[c];
+        int64_t pb = d->gate_pin_offset[g];
+        int32_t on = d->pin_net[pb + nin];
+        double load = net_cap[on];
+        double out_slew[2] = {0, 0};
+        bool any[2] = {false, false};
+        for (int k = 0; k < nin; ++k) {
+          int32_t n = d->pin_net[pb + k];
+          d->pin_slew[2 * (pb + k)] = net_slew[2 * n];
+          d->pin_slew[2 * (pb + k) + 1] = net_slew[2 * n + 1];
+          for (int fr = 0; fr < 2; ++fr)
+            for (int to = 0; to < 2; ++to) {
+              int dt = d->arc(c, k, fr, to, 0), st = d->arc(c, k, fr, to, 1);
+              if (dt < 0) continue;
+              double s_in = net_slew[2 * n + fr];
+              double delay = interp(d->tables, d->table_data, dt, s_in, load);
+              double slew = interp(d->tables, d->table_data, st, s_in, load);
+              int64_t a = d->gate_arc_offset[g] + ((int64_t)k * 2 + fr) * 2 + to;
+              d->arc_delay[a] = delay;
+              d->arc_ticks[a] = (int32_t)std::max<int64_t>(1, std::llround(delay / d->time_scale));
+              if (!any[to] || slew > out_slew[to]) out_slew[to] = slew;
+              any[to] = true;
+            }
+        }
+        for (int to = 0; to < 2; ++to) {
+          double s = any[to] ? out_slew[to] : p->pi_slew;
+          net_slew[2 * on + to] = s;
+          d->pin_slew[2 * (pb + nin) + to] = s;
+        }
+        d->pin_load_cap[pb + nin] = load;
+      });
+    }
+    *out = d;
+    return GTP_OK;
+  } catch (const std::bad_alloc&) {
+    g_err = "gtps_design_create: out of memory";
+    return GTP_ERR_OOM;
+  }
+}
+
+void gtps_design_free(gtps_design* d) { delete d; }
+
+void gtps_design_sizes(const gtps_design* d, int64_t* s) {
+  s[0] = d->G;
+  s[1] = d->P;
+  s[2] = d->N;
+  s[3] = (int64_t)d->arc_delay.size();
+  s[4] = d->n_pi;
+}
+
+void gtps_design_view(const gtps_design* d, gtp_design* v) {
+  v->n_gate = d->G;
+  v->n_pin = d->P;
+  v->n_net = d->N;
+  v->gate_cell = d->gate_cell.data();
+  v->gate_pin_offset = d->gate_pin_offset.data();
+  v->gate_arc_offset = d->gate_arc_offset.data();
+  v->arc_delay = d->arc_delay.data();
+  v->pin_net = d->pin_net.data();
+  v->pin_slew = d->pin_slew.data();
+  v->pin_load_cap = d->pin_load_cap.data();
+  v->pin_voltage = d->pin_voltage.data();
+  v->pin_default_state = d->pin_default_state.data();
+}
+
+const int32_t* gtps_net_driver(const gtps_design* d) { return d->net_driver.data(); }
+const int32_t* gtps_gate_level(const gtps_design* d) { return d->gate_level.data(); }
+
+namespace {
+
+struct Wave {
+  std::vector<int64_t> t;
+  std::vector<uint8_t> v;
+  void push(int64_t tt, uint8_t vv) {
+    t.push_back(tt);
+    v.push_back(vv);
+  }
+  // Inertial-delay transaction issued at time `now` for time `tau` (reject
+  // limit = the delay): every pending transaction (time > now) is dropped, so a
+  // pulse narrower than the gate delay never reaches the output; a pending
+  // change to the same value keeps its earlier time.
+  void schedule(int64_t now, int64_t tau, uint8_t val) {
+    int64_t keep_t = -1;
+    while (t.size() > 1 && t.back() > now) {
+      if (t.back() < tau && v.back() == val && keep_t < 0) keep_t = t.back();
+      t.pop_back();
+      v.pop_back();
+    }
+    if (v.back() != val) push(keep_t >= 0 ? keep_t : tau, val);
+  }
+};
+
+// Stretched-time map for burst cycles: within each block of 10 cycles, cycle
+// offset h(block) is a burst of `m` times the rate.  s -> (cycle, fraction).
+struct BurstMap {
+  double m;
+  uint64_t seed;
+  bool on;
+  int offset(int64_t b) const { return (int)(hash2(seed, (uint64_t)b) % 10); }
+  double block_len() const { return on ? 9.0 + m : 10.0; }
+  void map(double s, int64_t& cyc, double& frac) const {
+    if (!on) {
+      cyc = (int64_t)s;
+      frac = s - (double)cyc;
+      return;
+    }
+    int64_t b = (int64_t)(s / block_len());
+    double r = s - (double)b * block_len();
+    int o = offset(b);
+    if (r < o) {
+      cyc = 10 * b + (int64_t)r;
+      frac = r - std::floor(r);
+    } else if (r < o + m) {
+      cyc = 10 * b + o;
+      frac = (r - o) / m;
+    } else {
+      double q = r - o - m;
+      cyc = 10 * b + o + 1 + (int64_t)q;
+      frac = q - std::floor(q);
+    }
+  }
+};
+
+}  // namespace
+
+int32_t gtps_trace_create(const gtps_design* d, const gtps_trace_params* p, gtps_trace** out) {
+  try {
+    if (!d || !p || !out || p->n_cycle <= 0) {
+      g_err = "gtps_trace_create: bad arguments";
+      return GTP_ERR_ARG;
+    }
+    const int64_t N = d->N, T = d->T, C = p->n_cycle;
+    const int64_t t_max = C * T;
+    int threads = p->threads ? p->threads : d->threads;
+    std::vector<Wave> w(N);
+    // clock: 1 at 0, falls at k*T + T/2, rises at (k+1)*T
+    {
+      Wave& ck = w[0];
+      ck.t.reserve(2 * C + 1);
+      ck.v.reserve(2 * C + 1);
+      ck.push(0, 1);
+      for (int64_t k = 0; k < C; ++k) {
+        ck.push(k * T + T / 2, 0);
+        if ((k + 1) * T < t_max) ck.push((k + 1) * T, 1);
+      }
+    }
+    if (p->mode == 1) {
+      // independent random activity on every non-clock net
+      std::vector<double> rate(N, 0.0);
+      std::vector<uint8_t> hot(N, 0);
+      double s_hot = 0, s_cold = 0;
+      for (int64_t n = 1; n < N; ++n) {
+        Rng r(hash2(p->seed ^ 0x72617465ull, (uint64_t)n));
+        rate[n] = std::exp(p->lognorm_mu + p->lognorm_sigma * r.normal());
+        hot[n] = r.uniform() < p->hot_fraction;
+        (hot[n] ? s_hot : s_cold) += rate[n];
+      }
+      if (p->hot_fraction > 0 && s_hot > 0 && p->hot_share > 0 && p->hot_share < 1) {
+        double a = p->hot_share / (1 - p->hot_share) * s_cold / s_hot;
+        for (int64_t n = 1; n < N; ++n)
+          if (hot[n]) rate[n] *= a;
+      }
+      double cap = p->max_rate > 0 ? p->max_rate : 1e30;
+      BurstMap bm{p->burst_mult, p->seed ^ 0x62757273ull, p->burst_fraction > 0 && p->burst_mult > 1};
+      const double len = bm.on ? (double)C / 10.0 * bm.block_len() : (double)C;
+      parallel_for(N - 1, threads, [&](int64_t i) {
+        int64_t n = 1 + i;
+        Rng r(hash2(p->seed, (uint64_t)n));
+        double lam = std::min(rate[n], cap);
+        Wave& wv = w[n];
+        uint8_t val = (uint8_t)(r.next() & 1);
+        wv.push(0, val);
+        if (lam <= 0) return;
+        double s = 0;
+        for (;;) {
+          double u = r.uniform();
+          s += -std::log(1.0 - u) / lam;
+          if (s >= len) break;
+          int64_t cyc;
+          double frac;
+          bm.map(s, cyc, frac);
+          if (cyc >= C) break;
+          int64_t tt = cyc * T + (int64_t)(frac * (double)T);
+          if (tt <= wv.t.back()) continue;
+          val ^= 1;
+          wv.push(tt, val);
+        }
+      });
+    } else {
+      // primary inputs
+      parallel_for(d->n_pi, threads, [&](int64_t i) {
+        int64_t n = 1 + i;
+        Rng r(hash2(p->seed, (uint64_t)n));
+        int64_t off = T / 20 + (int64_t)r.below((uint64_t)std::max<int64_t>(1, T / 10));
+        uint8_t val = (uint8_t)(r.next() & 1);
+        Wave& wv = w[n];
+        wv.push(0, val);
+        for (int64_t k = 0; k < C; ++k) {
+          if (r.uniform() < p->toggle_p) {
+            int64_t tt = k * T + off;
+            if (tt >= t_max) break;
+            val ^= 1;
+            wv.push(tt, val);
+          }
+        }
+      });
+      // gates, level by level
+      const int L = (int)d->level_start.size() - 2;
+      for (int l = 1; l <= L; ++l) {
+        int64_t g0 = d->level_start[l], g1 = d->level_start[l + 1];
+        parallel_for(g1 - g0, threads, [&](int64_t i) {
+          int64_t g = g0 + i;
+          int c = d->gate_cell[g];
+          int nin = d->cell_n_in[c];
+          int64_t pb = d->gate_pin_offset[g];
+          int64_t ab = d->gate_arc_offset[g];
+          Wave& o = w[d->pin_net[pb + nin]];
+          auto ticks = [&](int k, int fr, int to) { return d->arc_ticks[ab + ((int64_t)k * 2 + fr) * 2 + to]; };
+          if (c == d->dff_cell) {
+            const Wave& ck = w[d->pin_net[pb + 0]];
+            const Wave& dd = w[d->pin_net[pb + 1]];
+            o.push(0, 2);
+            size_t jd = 0;
+            for (size_t j = 1; j < ck.t.size(); ++j) {
+              if (!(ck.v[j] == 1 && ck.v[j - 1] != 1)) continue;
+              int64_t te = ck.t[j];
+              while (jd + 1 < dd.t.size() && dd.t[jd + 1] < te) ++jd;
+              uint8_t nv = dd.v[jd];
+              int64_t dl;
+              if (nv == 1) dl = ticks(0, 0, 0);
+              else if (nv == 0) dl = ticks(0, 0, 1);
+              else dl = std::max(ticks(0, 0, 0), ticks(0, 0, 1));
+              if (dl < 1) dl = 1;
+              int64_t tau = te + dl;
+              if (tau < t_max) o.schedule(te, tau, nv);
+            }
+            return;
+          }
+          const Wave* in[GTP_MAX_CELL_PINS];
+          size_t pos[GTP_MAX_CELL_PINS];
+          uint8_t cur[GTP_MAX_CELL_PINS], prv[GTP_MAX_CELL_PINS];
+          for (int k = 0; k < nin; ++k) {
+            in[k] = &w[d->pin_net[pb + k]];
+            cur[k] = in[k]->v[0];
+            pos[k] = 1;
+          }
+          const uint64_t f = d->cell_func[c];
+          o.push(0, eval3(f, nin, cur));
+          int64_t dmax[2] = {1, 1};
+          for (int k = 0; k < nin; ++k)
+            for (int fr = 0; fr < 2; ++fr)
+              for (int to = 0; to < 2; ++to) dmax[to] = std::max<int64_t>(dmax[to], ticks(k, fr, to));
+          for (;;) {
+            int64_t t = INT64_MAX;
+            for (int k = 0; k < nin; ++k)
+              if (pos[k] < in[k]->t.size()) t = std::min(t, in[k]->t[pos[k]]);
+            if (t == INT64_MAX) break;
+            uint32_t changed = 0;
+            for (int k = 0; k < nin; ++k) {
+              prv[k] = cur[k];
+              if (pos[k] < in[k]->t.size() && in[k]->t[pos[k]] == t) {
+                cur[k] = in[k]->v[pos[k]++];
+                changed |= 1u << k;
+              }
+            }
+            uint8_t nv = eval3(f, nin, cur);
+            int64_t dl = 0;
+            for (int k = 0; k < nin; ++k) {
+              if (!((changed >> k) & 1)) continue;
+              int fr = is_rise(prv[k], cur[k]) ? 0 : 1;
+              for (int to = 0; to < 2; ++to) {
+                if ((nv == 1 && to == 1) || (nv == 0 && to == 0)) continue;
+                dl = std::max<int64_t>(dl, ticks(k, fr, to));
+              }
+            }
+            if (dl <= 0) dl = (nv == 1) ? dmax[0] : (nv == 0 ? dmax[1] : std::max(dmax[0], dmax[1]));
+            int64_t tau = t + dl;
+            o.schedule(t, tau, nv);
+          }
+          while (o.t.size() > 1 && o.t.back() >= t_max) { o.t.pop_back(); o.v.pop_back(); }
+          o.t.shrink_to_fit();
+          o.v.shrink_to_fit();
+        });
+      }
+    }
+    auto* tr = new gtps_trace();
+    tr->N = N;
+    tr->t_max = t_max;
+    tr->time_scale = d->time_scale;
+    tr->clk_period = (double)T * d->time_scale;
+    tr->off.resize(N + 1);
+    tr->off[0] = 0;
+    for (int64_t n = 0; n < N; ++n) tr->off[n + 1] = tr->off[n] + (int64_t)w[n].t.size();
+    const int64_t E = tr->off[N];
+    tr->time.resize(E);
+    tr->value.resize(E);
+    parallel_for(N, threads, [&](int64_t n) {
+      std::copy(w[n].t.begin(), w[n].t.end(), tr->time.begin() + tr->off[n]);
+      std::copy(w[n].v.begin(), w[n].v.end(), tr->value.begin() + tr->off[n]);
+      std::vector<int64_t>().swap(w[n].t);
+      std::vector<uint8_t>().swap(w[n].v);
+    });
+    *out = tr;
+    return GTP_OK;
+  } catch (const std::bad_alloc&) {
+    g_err = "gtps_trace_create: out of memory";
+    return GTP_ERR_OOM;
+  }
+}
+
+void gtps_trace_free(gtps_trace* t) { delete t; }
+
+void gtps_trace_view(const gtps_trace* t, gtp_trace* v) {
+  v->n_net = t->N;
+  v->net_event_offset = t->off.data();
+  v->event_time = t->time.data();
+  v->event_value = t->value.data();
+  v->t_max = t->t_max;
+  v->time_scale = t->time_scale;
+  v->clk_period = t->clk_period;
+}
+
+int64_t gtps_trace_events(const gtps_trace* t) { return t->off.empty() ? 0 : t->off.back(); }
+
+}  // extern "C"

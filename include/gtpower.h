@@ -1,0 +1,248 @@
+/*
+ * gtpower.h -- C ABI of the MI355X time-based (per-cycle) power engine.
+ *
+ * This is the drop-in boundary for GTPower's time-based power hot path.  In the
+ * reference the path is reached through the gtpower::Power virtual interface
+ *   virtual void Power::power(const Corner*, PowerResult& total, ...)   power/Power.hh:80-87
+ *   virtual void Power::printCellRes() const                            power/Power.hh:109
+ * which CudaPower overrides (power/cuda/CudaPower.hh:24-33) and Sta::makePower
+ * selects (search/Sta.cc:502-509).  Its inputs are implicit OpenSTA state
+ * (netlist, slews, load caps, arc delays, Liberty tables, VCD waveforms).  Here
+ * every input is an explicit, caller-owned flat array, and every output goes to
+ * caller-owned buffers.
+ *
+ * Conventions
+ *   - Plain C types only; no exceptions cross the ABI; functions return a
+ *     gtp_status (0 = ok, < 0 = error) and never call exit().  The text of the
+ *     last error of a context is returned by gtp_last_error().
+ *   - The caller owns every input and output array; the library copies what it
+ *     keeps and owns its device memory.
+ *   - One context per host thread.  A context is bound to one HIP device.
+ *   - Units are SI: seconds, farads, volts, joules, watts.  Event times are
+ *     integer ticks of `time_scale` seconds (the VCD $timescale).
+ *   - Logic values are 0, 1, 2 (X) and 3 (Z), as the reference's VCD value map
+ *     (power/Power.hh:114).
+ */
+#ifndef GTPOWER_H
+#define GTPOWER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GTP_ABI_VERSION 1
+
+typedef int32_t gtp_status;
+enum {
+  GTP_OK = 0,
+  GTP_ERR_ARG = -1,         /* malformed input (sizes, offsets, values) */
+  GTP_ERR_DEVICE = -2,      /* a HIP runtime call failed */
+  GTP_ERR_OOM = -3,         /* device or host allocation failed, or the event budget is exceeded */
+  GTP_ERR_STATE = -4,       /* call order violated (e.g. run before set_trace) */
+  GTP_ERR_UNSUPPORTED = -5, /* valid input outside what this build implements */
+  GTP_ERR_LIMIT = -6        /* a size limit of this build is exceeded */
+};
+
+/* Power components, in the column order of the reference's cell report
+ * (power/cuda/CudaPower.cu:3134-3165): regular internal (glitch excluded),
+ * glitch internal, regular switching (glitch excluded), glitch switching,
+ * leakage. */
+enum { GTP_INTERNAL = 0, GTP_GLITCH_INTERNAL = 1, GTP_SWITCHING = 2,
+       GTP_GLITCH_SWITCHING = 3, GTP_LEAKAGE = 4, GTP_N_COMPONENT = 5 };
+
+/* Per-cycle waveforms, in the section order of the reference's per-cycle file
+ * (include/sta/PowerClass.hh:123-149): leakage, internal (glitch included),
+ * glitch internal, switching (glitch included), glitch switching. */
+enum { GTP_WAVE_LEAKAGE = 0, GTP_WAVE_INTERNAL = 1, GTP_WAVE_GLITCH_INTERNAL = 2,
+       GTP_WAVE_SWITCHING = 3, GTP_WAVE_GLITCH_SWITCHING = 4, GTP_N_WAVE = 5 };
+
+/* Instance classes for the grouped totals of report_power
+ * (power/cuda/CudaPower.cu:3037-3132 finalizeResult). */
+enum { GTP_CLASS_COMBINATIONAL = 0, GTP_CLASS_SEQUENTIAL = 1, GTP_CLASS_CLOCK = 2,
+       GTP_CLASS_MACRO = 3, GTP_CLASS_PAD = 4, GTP_N_CLASS = 5 };
+
+/* Options.  Mirrors the power flags of the reference CLI
+ * (app/StaMain.cc:71-249, util/GlobalConfig.hh:16-46) that apply to this path. */
+typedef struct {
+  int32_t device;            /* HIP device ordinal; -1 = the current device */
+  int32_t tile_cycles;       /* cycles per work tile; 0 = chosen by the autotuner */
+  int32_t block_lanes;       /* lanes (gates) per workgroup pass; 0 = autotuned */
+  int32_t gates_per_wg;      /* gates a workgroup walks per tile; 0 = autotuned */
+  int32_t split_target;      /* target events per work unit for the density split; 0 = default */
+  int32_t autotune;          /* 1 = time candidate schedules on the loaded trace */
+  int64_t max_event_num;     /* device event budget (-max_event_num); 0 = no limit */
+  int32_t host_threads;      /* host worker threads (-multi_thread_number); 0 = all cores */
+  int32_t verbose;           /* 0 quiet, 1 timings to stderr */
+} gtp_options;
+
+/* A Liberty NLDM table, 1-D (n2 == 0) or 2-D, row-major values[i1 * n2 + i2].
+ * Offsets index gtp_library.table_data.  var1/var2 name what each axis is
+ * (reference: liberty/InternalPower.cc:178-229 axisValue). */
+enum { GTP_AXIS_SLEW = 0, GTP_AXIS_CAP = 1 };
+typedef struct {
+  int32_t n1, n2;
+  int32_t var1, var2;
+  int64_t axis1, axis2, values;
+} gtp_table;
+
+/* One Liberty internal_power group of a cell pin.  Groups of a (cell, pin) are
+ * listed in Liberty order: the first one whose `when` holds is used
+ * (reference: power/TimeBasedPower.cc:373-378, :489-502).
+ * when_truth bit s is set when the condition holds for gate state index s,
+ * the state index being sum over cell pins p of (value_p == 1) << p
+ * (reference: power/PowerUtils.cc:39-48, :155-187). */
+typedef struct {
+  int32_t cell;
+  int32_t pin;           /* cell pin index: inputs first, then outputs */
+  int32_t related_pin;   /* cell pin index of the related input; -1 = none */
+  int32_t rise_table;    /* index into tables[]; -1 = none */
+  int32_t fall_table;
+  int32_t pad_;
+  uint64_t when_truth;
+} gtp_internal_power;
+
+/* The cell library, flattened.  Pins of a cell are ordered inputs first, then
+ * outputs, as power/TimeBasedPower.cc:223-256 orders them.  Cells may have at
+ * most GTP_MAX_CELL_PINS pins. */
+#define GTP_MAX_CELL_PINS 6
+typedef struct {
+  int32_t n_cell;
+  const int32_t* cell_n_pin;        /* [n_cell] */
+  const int32_t* cell_n_in;         /* [n_cell] */
+  const int32_t* cell_class;        /* [n_cell] GTP_CLASS_* */
+  const uint32_t* cell_clock_mask;  /* [n_cell] bit p: pin p is a clock pin (no glitch flags) */
+  /* State-indexed leakage (reference getLeakagePower, power/TimeBasedPower.cc:258-301):
+   * cell c uses leak_values[cell_leak_offset[c] .. +2^n_pin) when the cell has
+   * leakage_power groups, else its default below.  */
+  const int64_t* cell_leak_offset;  /* [n_cell + 1] */
+  const double* leak_values;
+  const double* cell_leak_default;  /* [n_cell] cell_leakage_power, or 0 when absent */
+  int32_t n_table;
+  const gtp_table* tables;
+  const double* table_data;
+  int32_t n_ipower;
+  const gtp_internal_power* ipowers;
+} gtp_library;
+
+/* The design: gates (leaf instances), their pins and the pins' nets.
+ * Pins of gate g are pins[gate_pin_offset[g] .. gate_pin_offset[g+1]) in the
+ * cell's pin order.  Arc delays of gate g are
+ *   arc_delay[gate_arc_offset[g] + ((o * n_in + i) * 2 + from_rf) * 2 + to_rf]
+ * for output o (0-based among outputs), input i, rf 0 = rise, 1 = fall, in
+ * seconds; NaN = no such timing arc (reference: Gate::cellArcDelay,
+ * power/cuda/Gate.cuh:344-348, and graph_->gateEdgeArc,
+ * power/TimeBasedPower.cc:432-446). */
+typedef struct {
+  int64_t n_gate;
+  int64_t n_pin;
+  int64_t n_net;
+  const int32_t* gate_cell;         /* [n_gate] */
+  const int64_t* gate_pin_offset;   /* [n_gate + 1] */
+  const int64_t* gate_arc_offset;   /* [n_gate + 1] */
+  const double* arc_delay;
+  const int32_t* pin_net;           /* [n_pin] net index, -1 = no activity */
+  const double* pin_slew;           /* [2 * n_pin] rise, fall slew per pin (s); +inf allowed */
+  const double* pin_load_cap;       /* [n_pin] total load capacitance seen by an output pin (F) */
+  const double* pin_voltage;        /* [n_pin] supply voltage of an output pin (V) */
+  const int8_t* pin_default_state;  /* [n_pin] state of a pin without activity (0,1,2) */
+} gtp_design;
+
+/* One activity trace as per-net event lists (compressed sparse rows).
+ * Events of net n are [net_event_offset[n], net_event_offset[n+1]) in strictly
+ * increasing time.  As the reference's VCD model guarantees
+ * (power/Vcd.cc:203-216), consecutive events of a net carry different values
+ * and a net whose first change is after 0 starts with an X at time 0. */
+typedef struct {
+  int64_t n_net;
+  const int64_t* net_event_offset;  /* [n_net + 1] */
+  const int64_t* event_time;        /* [n_event] ticks */
+  const uint8_t* event_value;       /* [n_event] 0,1,2,3 */
+  int64_t t_max;                    /* end of the trace (ticks); the VCD's last time */
+  double time_scale;                /* seconds per tick */
+  double clk_period;                /* seconds */
+} gtp_trace;
+
+/* Results.  Every pointer may be NULL when that output is not wanted. */
+typedef struct {
+  double* gate_power;        /* [GTP_N_COMPONENT * n_gate], component-major */
+  double* cycle_power;       /* [GTP_N_WAVE * n_cycle], waveform-major */
+  double* total;             /* [GTP_N_COMPONENT] design totals */
+  double* class_total;       /* [GTP_N_CLASS * GTP_N_COMPONENT] */
+  uint64_t* gate_toggle_x2;  /* [n_gate] sum over pin events of 2 * n_toggle */
+  uint64_t* gate_glitch;     /* [n_gate] pin events with a glitch scaling ratio > 0 */
+  int64_t n_cycle;           /* out: ceil(t_max / ticks_per_cycle) */
+  double kernel_ms;          /* out: device time of the power evaluation */
+  double total_ms;           /* out: device time of the whole run call */
+} gtp_result;
+
+typedef struct gtp_ctx gtp_ctx;
+
+/* Fill `opt` with defaults. */
+void gtp_options_init(gtp_options* opt);
+
+/* Create a context on opt->device.  Replaces CudaPower's constructor plus
+ * utils::cuda::setDevice (util/cuda/DeviceSetting.cu:6-15). */
+gtp_status gtp_create(const gtp_options* opt, gtp_ctx** out);
+void gtp_destroy(gtp_ctx* ctx);
+const char* gtp_last_error(const gtp_ctx* ctx);
+/* Error text of the last failed gtp_create on this thread. */
+const char* gtp_create_error(void);
+
+/* Load the cell library and the design.  Replaces CudaPower::initGateData
+ * (power/cuda/CudaPower.cu:1601-1816) and the LUT construction of
+ * getInputInternalPower / getOutputInternalPower (:2347-2624). */
+gtp_status gtp_set_design(gtp_ctx* ctx, const gtp_library* lib, const gtp_design* design);
+
+/* Load one activity trace and build the device event store.  Replaces
+ * getGateWaveformRangeMultiThread + copyWaveformToDeviceSide
+ * (power/cuda/CudaPower.cu:1874-2130) and setPeriodAndInitPerCyclePower
+ * (:2656-2701). */
+gtp_status gtp_set_trace(gtp_ctx* ctx, const gtp_trace* trace);
+
+/* Evaluate per-gate and per-cycle power of the loaded trace.  Replaces
+ * CudaPower::power (power/cuda/CudaPower.cu:1227-1337): schedule,
+ * kernel1AllPowerCalculationTimeRangePartitionedByCycleUnit (:317-584),
+ * kernel2MergeAllPowerResult (:586-606), recordResult and finalizeResult
+ * (:2996-3132). */
+gtp_status gtp_run(gtp_ctx* ctx, gtp_result* res);
+
+/* Evaluate several traces of one design in one persistent launch.  traces[k]
+ * and results[k] pair up.  No counterpart in the reference (new method). */
+gtp_status gtp_run_batch(gtp_ctx* ctx, int32_t n_trace, const gtp_trace* traces,
+                         gtp_result* results);
+
+/* Schedule of the last run: tile length, lanes, workgroups, units. */
+typedef struct {
+  int32_t tile_cycles, block_lanes, gates_per_wg, split_target;
+  int64_t n_unit, n_workgroup, n_tile, n_event, n_pin_event;
+  double autotune_ms;
+} gtp_schedule_info;
+gtp_status gtp_get_schedule(const gtp_ctx* ctx, gtp_schedule_info* info);
+
+/* The reference's event-density heuristic for cycles per thread
+ * (Gate::setNCyclePerThreadByEventCount, power/cuda/Gate.cuh:354-376), for
+ * n gates with n_event[g] events each over n_cycle cycles. */
+gtp_status gtp_reference_cycles_per_thread(int64_t n, const int64_t* n_event, int64_t n_cycle,
+                                           int64_t e_target, int64_t parallelism_floor,
+                                           int64_t* out_ncpt);
+
+/* Writers for the reference's two result files.
+ * gtp_write_cell_power: `cuda_cell_power.txt` (power/cuda/CudaPower.cu:3134-3165),
+ *   one row per gate: name, internal, glitch internal, switching, glitch
+ *   switching, leakage, total, cell name; tab separated, 6 significant digits.
+ * gtp_write_per_cycle: the per-cycle waveform file
+ *   (include/sta/PowerClass.hh:123-149) for one top instance.
+ * Names are NUL-separated strings, one per gate (resp. cell type). */
+gtp_status gtp_write_cell_power(const char* path, int64_t n_gate, const double* gate_power,
+                                const char* gate_names, const int32_t* gate_cell,
+                                const char* cell_names);
+gtp_status gtp_write_per_cycle(const char* path, const char* top_name, int64_t n_cycle,
+                               const double* cycle_power);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GTPOWER_H */
