@@ -1,0 +1,874 @@
+// Host runtime of the power engine behind the C ABI of include/gtpower.h.
+//
+// Replaces CudaPower's host orchestration (power/cuda/CudaPower.cu:1227-1337):
+// flattening of gates, pins, tables and delays (initGateData :1601-1816), the
+// LUT construction (:2347-2624), the event store (:1874-2130), the schedule
+// (:2718-2849), the launches (:2851-2957) and the result merge (:2959-3132).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "gtp_internal.hpp"
+#include "gtpower.h"
+
+#define GTP_API extern "C" __attribute__((visibility("default")))
+
+using namespace gtp;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+template <class T>
+struct DArr {
+  T* p = nullptr;
+  size_t n = 0;
+  DArr() = default;
+  DArr(const DArr&) = delete;
+  DArr& operator=(const DArr&) = delete;
+  ~DArr() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    if (count == n && p) return hipSuccess;
+    release();
+    if (count == 0) return hipSuccess;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
+    n = count;
+    return hipSuccess;
+  }
+  hipError_t upload(const T* src, size_t count, hipStream_t s) {
+    hipError_t e = alloc(count);
+    if (e != hipSuccess || count == 0) return e;
+    return hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s);
+  }
+};
+
+template <class F>
+void parallel_for(int64_t n, int threads, F f) {
+  if (n <= 0) return;
+  if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+  threads = (int)std::min<int64_t>(threads, std::max<int64_t>(1, n / 1024));
+  if (threads <= 1) {
+    for (int64_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int64_t> next{0};
+  const int64_t chunk = std::max<int64_t>(1, n / (threads * 16));
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t)
+    pool.emplace_back([&] {
+      for (;;) {
+        int64_t b = next.fetch_add(chunk);
+        if (b >= n) break;
+        int64_t e = std::min(n, b + chunk);
+        for (int64_t i = b; i < e; ++i) f(i);
+      }
+    });
+  for (auto& th : pool) th.join();
+}
+
+struct HostCell {
+  int n_pin = 0, n_in = 0, cls = 0, sbits = 0;
+  uint32_t clock_mask = 0;
+  std::vector<float> leak;   // 2^n_pin
+  std::vector<std::vector<int>> groups;  // internal power groups per pin
+};
+
+}  // namespace
+
+struct gtp_ctx {
+  gtp_options opt{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int threads = 0;
+
+  // library
+  std::vector<HostCell> cells;
+  std::vector<CellDev> cell_dev;
+  std::vector<float> leak_f;     // concatenated per-cell leakage tables
+  double leak_max = 0;
+  DArr<CellDev> d_cells;
+  DArr<float> d_leak;
+  DArr<long long> d_leak_q;
+
+  // design
+  bool have_design = false;
+  int64_t G = 0, P = 0, N = 0;
+  std::vector<int32_t> gate_cell;
+  std::vector<int64_t> gate_pin_off, gate_arc_off;
+  std::vector<int32_t> pin_net;
+  std::vector<double> arc_delay;
+  double csw_max = 0;
+  float energy_max = 0;
+  DArr<int32_t> d_gate_cell, d_gate_class;
+  DArr<int64_t> d_gate_pin_off, d_gate_arc_off, d_gate_energy_off;
+  DArr<int32_t> d_pin_net;
+  DArr<float> d_pin_gk, d_pin_csw;
+  DArr<int8_t> d_pin_default;
+  DArr<float> d_energy;
+  DArr<int64_t> d_arc_q;
+  double arc_q_scale = 0;  // time scale the arc_q table was built for
+
+  // trace
+  bool have_trace = false;
+  int64_t E = 0, T = 0, C = 0, t_max = 0, P_ev = 0;
+  double scale = 0, period = 0;
+  std::vector<int64_t> gate_work;
+  DArr<int64_t> d_net_off;
+  DArr<uint32_t> d_ev;
+  DArr<int32_t> d_tile_idx;
+  int32_t tile_idx_tc = 0;
+
+  // schedule
+  std::vector<int32_t> order;  // padded, class A then class B
+  int32_t range_gates = 0, n_range_a = 0, n_range_b = 0;
+  DArr<int32_t> d_order;
+  gtp_schedule_info sched{};
+
+  // scratch and results
+  DArr<float> d_pg_hi, d_pg_lo;
+  DArr<uint32_t> d_pg_tog, d_pg_glc;
+  DArr<long long> d_cyc_part;
+  DArr<double> d_cycle, d_gate, d_total;
+  DArr<unsigned long long> d_gtog, d_gglc;
+  DArr<unsigned int> d_maxbits;
+
+  gtp_status fail(gtp_status st, const std::string& m) {
+    err = m;
+    return st;
+  }
+};
+
+#define HIPCHK(ctx, expr)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return (ctx)->fail(e_ == hipErrorOutOfMemory ? GTP_ERR_OOM : GTP_ERR_DEVICE,          \
+                         std::string(#expr) + ": " + hipGetErrorString(e_));                \
+  } while (0)
+
+namespace {
+
+uint64_t full_truth(int n) { return n >= 6 ? ~0ull : ((1ull << (1 << n)) - 1ull); }
+
+// Resolve the cell library into per-cell slot descriptors.  The choice of
+// internal_power group depends on (cell, pin, related pin, rf, state) only, so
+// it is made here once; the energy kernel then interpolates per gate.
+gtp_status build_library(gtp_ctx* x, const gtp_library* L, std::vector<SlotDesc>& descs,
+                         std::vector<int32_t>& desc_off, std::vector<ListEntry>& list,
+                         std::vector<TableDev>& tables, std::vector<float>& tdata) {
+  if (!L || L->n_cell <= 0 || !L->cell_n_pin || !L->cell_n_in || !L->cell_class ||
+      !L->cell_clock_mask || !L->cell_leak_offset || !L->cell_leak_default)
+    return x->fail(GTP_ERR_ARG, "gtp_library: missing arrays");
+  if (L->n_cell > MAX_CELLS_LDS) return x->fail(GTP_ERR_LIMIT, "gtp_library: more than 64 cell types");
+  if (L->n_table < 0 || (L->n_table > 0 && (!L->tables || !L->table_data)))
+    return x->fail(GTP_ERR_ARG, "gtp_library: tables");
+  // tables -> float data
+  size_t data_len = 0;
+  for (int t = 0; t < L->n_table; ++t) {
+    const gtp_table& tb = L->tables[t];
+    if (tb.n1 <= 0 || tb.n2 < 0 || tb.axis1 < 0 || tb.values < 0 || (tb.n2 > 0 && tb.axis2 < 0))
+      return x->fail(GTP_ERR_ARG, "gtp_library: malformed table " + std::to_string(t));
+    data_len = std::max(data_len, (size_t)tb.values + (size_t)tb.n1 * (tb.n2 ? tb.n2 : 1));
+    data_len = std::max(data_len, (size_t)tb.axis1 + tb.n1);
+    if (tb.n2) data_len = std::max(data_len, (size_t)tb.axis2 + tb.n2);
+  }
+  if (data_len > 12000) return x->fail(GTP_ERR_LIMIT, "gtp_library: table data exceeds 12000 values (LDS staging)");
+  tdata.resize(std::max<size_t>(data_len, 1));
+  for (size_t i = 0; i < data_len; ++i) tdata[i] = (float)L->table_data[i];
+  tables.resize(L->n_table);
+  for (int t = 0; t < L->n_table; ++t) {
+    const gtp_table& tb = L->tables[t];
+    tables[t] = TableDev{tb.n1, tb.n2, tb.var1, tb.var2, (int32_t)tb.axis1, (int32_t)tb.axis2, (int32_t)tb.values};
+  }
+  x->cells.assign(L->n_cell, HostCell{});
+  for (int c = 0; c < L->n_cell; ++c) {
+    HostCell& hc = x->cells[c];
+    hc.n_pin = L->cell_n_pin[c];
+    hc.n_in = L->cell_n_in[c];
+    hc.cls = L->cell_class[c];
+    hc.clock_mask = L->cell_clock_mask[c];
+    if (hc.n_pin <= 0 || hc.n_pin > GTP_MAX_CELL_PINS || hc.n_in < 0 || hc.n_in > hc.n_pin)
+      return x->fail(GTP_ERR_LIMIT, "gtp_library: cell " + std::to_string(c) + " pin count outside 1..6");
+    if (hc.cls < 0 || hc.cls >= GTP_N_CLASS) return x->fail(GTP_ERR_ARG, "gtp_library: cell class");
+    const int64_t o0 = L->cell_leak_offset[c], o1 = L->cell_leak_offset[c + 1];
+    const int ns = 1 << hc.n_pin;
+    hc.leak.assign(ns, (float)L->cell_leak_default[c]);
+    if (o1 != o0) {
+      if (o1 - o0 != ns || !L->leak_values) return x->fail(GTP_ERR_ARG, "gtp_library: leakage table size");
+      for (int s = 0; s < ns; ++s) hc.leak[s] = (float)L->leak_values[o0 + s];
+    }
+    hc.groups.assign(hc.n_pin, {});
+  }
+  for (int k = 0; k < L->n_ipower; ++k) {
+    const gtp_internal_power& ip = L->ipowers[k];
+    if (ip.cell < 0 || ip.cell >= L->n_cell) return x->fail(GTP_ERR_ARG, "gtp_library: ipower cell");
+    HostCell& hc = x->cells[ip.cell];
+    if (ip.pin < 0 || ip.pin >= hc.n_pin || ip.related_pin >= hc.n_in || ip.rise_table >= L->n_table ||
+        ip.fall_table >= L->n_table)
+      return x->fail(GTP_ERR_ARG, "gtp_library: ipower " + std::to_string(k));
+    hc.groups[ip.pin].push_back(k);
+    const uint64_t full = full_truth(hc.n_pin);
+    if ((ip.when_truth & full) != full) hc.sbits = hc.n_pin;
+  }
+  // slots
+  x->cell_dev.assign(L->n_cell, CellDev{});
+  x->leak_f.clear();
+  x->leak_max = 0;
+  desc_off.assign(L->n_cell, 0);
+  descs.clear();
+  list.clear();
+  for (int c = 0; c < L->n_cell; ++c) {
+    HostCell& hc = x->cells[c];
+    CellDev& cd = x->cell_dev[c];
+    cd.n_pin = hc.n_pin;
+    cd.n_in = hc.n_in;
+    cd.clock_mask = hc.clock_mask;
+    cd.state_bits = hc.sbits;
+    cd.leak_off = (int32_t)x->leak_f.size();
+    for (float v : hc.leak) {
+      x->leak_f.push_back(v);
+      x->leak_max = std::max(x->leak_max, (double)std::fabs(v));
+    }
+    desc_off[c] = (int32_t)descs.size();
+    const int sb = hc.sbits, nst = 1 << sb;
+    int off = 0;
+    for (int p = 0; p < hc.n_pin; ++p) {
+      cd.slot_off[p] = off;
+      if (p < hc.n_in) {
+        // findInputInternalVal (power/TimeBasedPower.cc:341-381)
+        for (int rf = 0; rf < 2; ++rf)
+          for (int st = 0; st < nst; ++st) {
+            SlotDesc d{-1, (int16_t)p, (int8_t)rf, (int8_t)-1, 0, 0};
+            for (int k : hc.groups[p]) {
+              const gtp_internal_power& ip = L->ipowers[k];
+              // without conditional groups (sb == 0) every group holds in every state
+              if ((ip.when_truth >> st) & 1ull) {
+                d.table = rf == 0 ? ip.rise_table : ip.fall_table;
+                break;
+              }
+            }
+            descs.push_back(d);
+          }
+        off += 2 * nst;
+      } else {
+        // findOutputInternalVal (:383-509): related input q, its edge, this edge, state
+        for (int q = 0; q < hc.n_in; ++q)
+          for (int frf = 0; frf < 2; ++frf)
+            for (int trf = 0; trf < 2; ++trf)
+              for (int st = 0; st < nst; ++st) {
+                SlotDesc d{-1, (int16_t)q, (int8_t)frf, (int8_t)p, 0, 0};
+                for (int k : hc.groups[p]) {
+                  const gtp_internal_power& ip = L->ipowers[k];
+                  if (ip.related_pin != q) continue;
+                  if (!((ip.when_truth >> st) & 1ull)) continue;
+                  d.table = trf == 0 ? ip.rise_table : ip.fall_table;
+                  break;
+                }
+                descs.push_back(d);
+              }
+        off += hc.n_in * 4 * nst;
+        // getDefaultOutputPinInternalEnergyVal (:511-555): mean over related
+        // inputs x {rise, fall} x their groups, `when` ignored
+        for (int trf = 0; trf < 2; ++trf) {
+          SlotDesc d{-1, 0, 0, (int8_t)p, (int32_t)list.size(), 0};
+          for (int i = 0; i < hc.n_in; ++i)
+            for (int frf = 0; frf < 2; ++frf)
+              for (int k : hc.groups[p]) {
+                const gtp_internal_power& ip = L->ipowers[k];
+                if (ip.related_pin != i) continue;
+                list.push_back(ListEntry{trf == 0 ? ip.rise_table : ip.fall_table, (int16_t)i, (int8_t)frf, (int8_t)p});
+                ++d.list_cnt;
+              }
+          // a default slot with no entries is 0: keep it plain with table -1
+          if (d.list_cnt == 0) d.list_off = 0;
+          descs.push_back(d);
+        }
+        off += 2;
+      }
+    }
+    cd.n_slot = off;
+  }
+  return GTP_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+GTP_API void gtp_options_init(gtp_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->device = -1;
+  o->autotune = 0;
+}
+
+GTP_API const char* gtp_create_error(void) { return g_create_error.c_str(); }
+
+GTP_API gtp_status gtp_create(const gtp_options* opt, gtp_ctx** out) {
+  if (!out) {
+    g_create_error = "gtp_create: null output pointer";
+    return GTP_ERR_ARG;
+  }
+  *out = nullptr;
+  auto* x = new (std::nothrow) gtp_ctx();
+  if (!x) {
+    g_create_error = "gtp_create: out of host memory";
+    return GTP_ERR_OOM;
+  }
+  if (opt) x->opt = *opt; else gtp_options_init(&x->opt);
+  int n_dev = 0;
+  hipError_t e = hipGetDeviceCount(&n_dev);
+  if (e != hipSuccess || n_dev <= 0) {
+    g_create_error = std::string("gtp_create: no HIP device: ") + hipGetErrorString(e);
+    delete x;
+    return GTP_ERR_DEVICE;
+  }
+  int dev = x->opt.device;
+  if (dev < 0) {
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  }
+  if (dev >= n_dev) {
+    g_create_error = "gtp_create: device ordinal out of range";
+    delete x;
+    return GTP_ERR_ARG;
+  }
+  x->device = dev;
+  if ((e = hipSetDevice(dev)) != hipSuccess || (e = hipStreamCreate(&x->stream)) != hipSuccess) {
+    g_create_error = std::string("gtp_create: ") + hipGetErrorString(e);
+    delete x;
+    return GTP_ERR_DEVICE;
+  }
+  x->threads = x->opt.host_threads > 0 ? x->opt.host_threads
+                                       : (int)std::max(1u, std::thread::hardware_concurrency());
+  *out = x;
+  return GTP_OK;
+}
+
+GTP_API void gtp_destroy(gtp_ctx* x) {
+  if (!x) return;
+  (void)hipSetDevice(x->device);
+  if (x->stream) {
+    (void)hipStreamSynchronize(x->stream);
+    (void)hipStreamDestroy(x->stream);
+  }
+  delete x;
+}
+
+GTP_API const char* gtp_last_error(const gtp_ctx* x) { return x ? x->err.c_str() : "null context"; }
+
+// ---------------------------------------------------------------------------
+GTP_API gtp_status gtp_set_design(gtp_ctx* x, const gtp_library* L, const gtp_design* d) {
+  if (!x) return GTP_ERR_ARG;
+  HIPCHK(x, hipSetDevice(x->device));
+  x->have_design = false;
+  x->have_trace = false;
+  std::vector<SlotDesc> descs;
+  std::vector<int32_t> desc_off;
+  std::vector<ListEntry> list;
+  std::vector<TableDev> tables;
+  std::vector<float> tdata;
+  gtp_status st = build_library(x, L, descs, desc_off, list, tables, tdata);
+  if (st != GTP_OK) return st;
+  if (!d || d->n_gate <= 0 || d->n_pin <= 0 || d->n_net < 0 || !d->gate_cell || !d->gate_pin_offset ||
+      !d->gate_arc_offset || !d->pin_net || !d->pin_slew || !d->pin_load_cap || !d->pin_voltage ||
+      !d->pin_default_state)
+    return x->fail(GTP_ERR_ARG, "gtp_design: missing arrays");
+  if (d->n_gate >= INT32_MAX || d->n_pin >= INT32_MAX || d->n_net >= INT32_MAX)
+    return x->fail(GTP_ERR_LIMIT, "gtp_design: more than 2^31 gates, pins or nets");
+  const int64_t G = d->n_gate, P = d->n_pin, N = d->n_net;
+  if (d->gate_pin_offset[0] != 0 || d->gate_arc_offset[0] != 0)
+    return x->fail(GTP_ERR_ARG, "gtp_design: offsets must start at 0");
+  std::vector<int64_t> eoff(G);
+  int64_t eacc = 0;
+  for (int64_t g = 0; g < G; ++g) {
+    const int c = d->gate_cell[g];
+    if (c < 0 || c >= L->n_cell) return x->fail(GTP_ERR_ARG, "gtp_design: gate_cell out of range at " + std::to_string(g));
+    const HostCell& hc = x->cells[c];
+    if (d->gate_pin_offset[g + 1] - d->gate_pin_offset[g] != hc.n_pin)
+      return x->fail(GTP_ERR_ARG, "gtp_design: pin count of gate " + std::to_string(g) + " differs from its cell");
+    if (d->gate_arc_offset[g + 1] - d->gate_arc_offset[g] != (int64_t)(hc.n_pin - hc.n_in) * hc.n_in * 4)
+      return x->fail(GTP_ERR_ARG, "gtp_design: arc count of gate " + std::to_string(g));
+    eoff[g] = eacc;
+    eacc += x->cell_dev[c].n_slot;
+  }
+  if (d->gate_pin_offset[G] != P) return x->fail(GTP_ERR_ARG, "gtp_design: gate_pin_offset[n_gate] != n_pin");
+  const int64_t A = d->gate_arc_offset[G];
+  if (A > 0 && !d->arc_delay) return x->fail(GTP_ERR_ARG, "gtp_design: arc_delay missing");
+  std::vector<float> gk(P), csw(P, 0.f), slew(2 * P), cap(P);
+  std::vector<int32_t> gclass(G);
+  double csw_max = 0;
+  for (int64_t g = 0; g < G; ++g) {
+    const HostCell& hc = x->cells[d->gate_cell[g]];
+    gclass[g] = hc.cls;
+    for (int p = 0; p < hc.n_pin; ++p) {
+      const int64_t q = d->gate_pin_offset[g] + p;
+      if (p >= hc.n_in) {
+        const double v = d->pin_load_cap[q] * d->pin_voltage[q] * d->pin_voltage[q];
+        csw[q] = (float)v;
+        csw_max = std::max(csw_max, std::fabs(v));
+      }
+    }
+  }
+  for (int64_t q = 0; q < P; ++q) {
+    if (d->pin_net[q] < -1 || d->pin_net[q] >= N) return x->fail(GTP_ERR_ARG, "gtp_design: pin_net out of range at " + std::to_string(q));
+    if (d->pin_default_state[q] < 0 || d->pin_default_state[q] > 3) return x->fail(GTP_ERR_ARG, "gtp_design: pin_default_state");
+    const double sr = d->pin_slew[2 * q], sf = d->pin_slew[2 * q + 1];
+    if (std::isnan(sr) || std::isnan(sf)) return x->fail(GTP_ERR_ARG, "gtp_design: NaN slew at pin " + std::to_string(q));
+    const double sum = sr + sf;
+    // glitch scaling ratio min(1, (2 w / (s_r + s_f))^2) (power/PowerUtils.cc:285-294);
+    // a zero slew sum gives ratio 1, an infinite one gives 0 (no glitch).
+    gk[q] = sum == 0 ? INFINITY : (float)(2.0 / sum);
+    slew[2 * q] = (float)sr;
+    slew[2 * q + 1] = (float)sf;
+    cap[q] = (float)d->pin_load_cap[q];
+  }
+  x->G = G;
+  x->P = P;
+  x->N = N;
+  x->gate_cell.assign(d->gate_cell, d->gate_cell + G);
+  x->gate_pin_off.assign(d->gate_pin_offset, d->gate_pin_offset + G + 1);
+  x->gate_arc_off.assign(d->gate_arc_offset, d->gate_arc_offset + G + 1);
+  x->pin_net.assign(d->pin_net, d->pin_net + P);
+  x->arc_delay.assign(d->arc_delay, d->arc_delay + A);
+  x->csw_max = csw_max;
+  x->arc_q_scale = 0;
+  hipStream_t s = x->stream;
+  HIPCHK(x, x->d_cells.upload(x->cell_dev.data(), x->cell_dev.size(), s));
+  HIPCHK(x, x->d_leak.upload(x->leak_f.data(), x->leak_f.size(), s));
+  HIPCHK(x, x->d_leak_q.alloc(x->leak_f.size()));
+  HIPCHK(x, x->d_gate_cell.upload(d->gate_cell, G, s));
+  HIPCHK(x, x->d_gate_class.upload(gclass.data(), G, s));
+  HIPCHK(x, x->d_gate_pin_off.upload(d->gate_pin_offset, G + 1, s));
+  HIPCHK(x, x->d_gate_arc_off.upload(d->gate_arc_offset, G + 1, s));
+  HIPCHK(x, x->d_gate_energy_off.upload(eoff.data(), G, s));
+  HIPCHK(x, x->d_pin_net.upload(d->pin_net, P, s));
+  HIPCHK(x, x->d_pin_gk.upload(gk.data(), P, s));
+  HIPCHK(x, x->d_pin_csw.upload(csw.data(), P, s));
+  HIPCHK(x, x->d_pin_default.upload(d->pin_default_state, P, s));
+  // energies: LDS-staged table interpolation per gate slot
+  DArr<float> d_slew, d_cap, d_tdata;
+  DArr<SlotDesc> d_descs;
+  DArr<ListEntry> d_list;
+  DArr<TableDev> d_tables;
+  DArr<int32_t> d_desc_off;
+  HIPCHK(x, d_slew.upload(slew.data(), slew.size(), s));
+  HIPCHK(x, d_cap.upload(cap.data(), cap.size(), s));
+  HIPCHK(x, d_tdata.upload(tdata.data(), tdata.size(), s));
+  HIPCHK(x, d_descs.upload(descs.data(), descs.size(), s));
+  if (!list.empty()) HIPCHK(x, d_list.upload(list.data(), list.size(), s));
+  if (!tables.empty()) HIPCHK(x, d_tables.upload(tables.data(), tables.size(), s));
+  HIPCHK(x, d_desc_off.upload(desc_off.data(), desc_off.size(), s));
+  HIPCHK(x, x->d_energy.alloc(std::max<int64_t>(eacc, 1)));
+  HIPCHK(x, x->d_maxbits.alloc(1));
+  HIPCHK(x, hipMemsetAsync(x->d_maxbits.p, 0, sizeof(unsigned int), s));
+  EnergyArgs ea{};
+  ea.gate_cell = x->d_gate_cell.p;
+  ea.gate_pin_off = x->d_gate_pin_off.p;
+  ea.gate_energy_off = x->d_gate_energy_off.p;
+  ea.n_gate = G;
+  ea.pin_slew = d_slew.p;
+  ea.pin_cap = d_cap.p;
+  ea.cells = x->d_cells.p;
+  ea.cell_slot_desc_off = d_desc_off.p;
+  ea.descs = d_descs.p;
+  ea.list = d_list.p;
+  ea.tables = d_tables.p;
+  ea.n_table = (int32_t)tables.size();
+  ea.table_data = d_tdata.p;
+  ea.n_table_data = (int32_t)tdata.size();
+  ea.energy = x->d_energy.p;
+  ea.max_abs_bits = x->d_maxbits.p;
+  HIPCHK(x, launch_energy(ea, s));
+  unsigned int bits = 0;
+  HIPCHK(x, hipMemcpyAsync(&bits, x->d_maxbits.p, sizeof(bits), hipMemcpyDeviceToHost, s));
+  HIPCHK(x, hipStreamSynchronize(s));
+  std::memcpy(&x->energy_max, &bits, sizeof(float));
+  x->have_design = true;
+  return GTP_OK;
+}
+
+// ---------------------------------------------------------------------------
+GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
+  if (!x) return GTP_ERR_ARG;
+  if (!x->have_design) return x->fail(GTP_ERR_STATE, "gtp_set_trace: call gtp_set_design first");
+  HIPCHK(x, hipSetDevice(x->device));
+  x->have_trace = false;
+  if (!tr || !tr->net_event_offset || tr->n_net != x->N)
+    return x->fail(GTP_ERR_ARG, "gtp_trace: n_net must equal the design's n_net");
+  if (!(tr->time_scale > 0) || !(tr->clk_period > 0) || tr->t_max <= 0)
+    return x->fail(GTP_ERR_ARG, "gtp_trace: time_scale, clk_period and t_max must be positive");
+  if (tr->t_max >= (int64_t)EV_TIME_INF)
+    return x->fail(GTP_ERR_UNSUPPORTED, "gtp_trace: t_max must be below 2^30 - 1 ticks");
+  const int64_t N = x->N;
+  const int64_t* off = tr->net_event_offset;
+  if (off[0] != 0) return x->fail(GTP_ERR_ARG, "gtp_trace: net_event_offset[0] != 0");
+  for (int64_t n = 0; n < N; ++n)
+    if (off[n + 1] < off[n]) return x->fail(GTP_ERR_ARG, "gtp_trace: net_event_offset decreases at " + std::to_string(n));
+  const int64_t E = off[N];
+  if (E > 0 && (!tr->event_time || !tr->event_value)) return x->fail(GTP_ERR_ARG, "gtp_trace: event arrays missing");
+  if (x->opt.max_event_num > 0 && E > x->opt.max_event_num)
+    return x->fail(GTP_ERR_OOM, "gtp_trace: " + std::to_string(E) + " events exceed max_event_num");
+  // validation: strictly increasing times within [0, t_max], values 0..3
+  std::atomic<int64_t> bad{-1};
+  parallel_for(N, x->threads, [&](int64_t n) {
+    int64_t prev = -1;
+    for (int64_t e = off[n]; e < off[n + 1]; ++e) {
+      const int64_t t = tr->event_time[e];
+      if (t <= prev || t > tr->t_max || tr->event_value[e] > 3) {
+        int64_t expect = -1;
+        bad.compare_exchange_strong(expect, n);
+        return;
+      }
+      prev = t;
+    }
+  });
+  if (bad.load() >= 0)
+    return x->fail(GTP_ERR_ARG, "gtp_trace: events of net " + std::to_string(bad.load()) +
+                                    " are not strictly increasing in [0, t_max] or carry a value > 3");
+  double ratio = tr->clk_period / tr->time_scale;
+  int64_t T = (int64_t)std::ceil(ratio);  // power/PowerUtils.cc:27-32
+  if (std::fabs(ratio - std::round(ratio)) < 1e-9 * std::max(1.0, ratio)) T = (int64_t)std::llround(ratio);
+  if (T < 1 || T >= (int64_t)EV_TIME_INF) return x->fail(GTP_ERR_UNSUPPORTED, "gtp_trace: ticks per cycle out of range");
+  x->T = T;
+  x->t_max = tr->t_max;
+  x->C = (tr->t_max + T - 1) / T;
+  x->scale = tr->time_scale;
+  x->period = tr->clk_period;
+  x->E = E;
+  hipStream_t s = x->stream;
+  HIPCHK(x, x->d_net_off.upload(off, N + 1, s));
+  HIPCHK(x, x->d_ev.alloc(std::max<int64_t>(E, 1)));
+  // chunked upload + pack into the 32-bit event store
+  {
+    const int64_t CH = 1 << 26;
+    DArr<int64_t> st;
+    DArr<uint8_t> sv;
+    HIPCHK(x, st.alloc(std::min<int64_t>(CH, std::max<int64_t>(E, 1))));
+    HIPCHK(x, sv.alloc(std::min<int64_t>(CH, std::max<int64_t>(E, 1))));
+    for (int64_t b = 0; b < E; b += CH) {
+      const int64_t n = std::min(CH, E - b);
+      HIPCHK(x, hipMemcpyAsync(st.p, tr->event_time + b, n * sizeof(int64_t), hipMemcpyHostToDevice, s));
+      HIPCHK(x, hipMemcpyAsync(sv.p, tr->event_value + b, n, hipMemcpyHostToDevice, s));
+      HIPCHK(x, launch_pack(st.p, sv.p, x->d_ev.p + b, n, s));
+    }
+    HIPCHK(x, hipStreamSynchronize(s));
+  }
+  x->d_tile_idx.release();
+  x->tile_idx_tc = 0;
+  // arc delays on the 2^-20 tick grid of the related-pin comparison
+  if (x->arc_q_scale != tr->time_scale) {
+    std::vector<int64_t> aq(x->arc_delay.size());
+    const double f = (double)(1 << ARC_FRAC_BITS) / tr->time_scale;
+    for (size_t i = 0; i < aq.size(); ++i) {
+      const double dl = x->arc_delay[i];
+      aq[i] = std::isnan(dl) ? ARC_NONE : (int64_t)std::llround(dl * f);
+    }
+    HIPCHK(x, x->d_arc_q.upload(aq.data(), std::max<size_t>(aq.size(), 1), s));
+    HIPCHK(x, hipStreamSynchronize(s));
+    x->arc_q_scale = tr->time_scale;
+  }
+  // per-gate work (pin events) for the schedule
+  x->gate_work.assign(x->G, 0);
+  int64_t pev = 0;
+  for (int64_t g = 0; g < x->G; ++g) {
+    int64_t w = 0;
+    for (int64_t q = x->gate_pin_off[g]; q < x->gate_pin_off[g + 1]; ++q) {
+      const int32_t net = x->pin_net[q];
+      if (net >= 0) w += off[net + 1] - off[net];
+    }
+    x->gate_work[g] = w;
+    pev += w;
+  }
+  x->P_ev = pev;
+  x->order.clear();
+  x->have_trace = true;
+  return GTP_OK;
+}
+
+namespace {
+
+// Schedule: gates sorted by work (pin events), heaviest first, split into the
+// <= 4-pin and the 5-6-pin kernel classes, each padded to whole ranges.
+void build_order(gtp_ctx* x, int32_t range_gates) {
+  std::vector<int32_t> a, b;
+  for (int64_t g = 0; g < x->G; ++g) {
+    if (x->cells[x->gate_cell[g]].n_pin <= MAXP_SMALL) a.push_back((int32_t)g);
+    else b.push_back((int32_t)g);
+  }
+  auto by_work = [&](int32_t p, int32_t q) {
+    if (x->gate_work[p] != x->gate_work[q]) return x->gate_work[p] > x->gate_work[q];
+    return p < q;
+  };
+  std::sort(a.begin(), a.end(), by_work);
+  std::sort(b.begin(), b.end(), by_work);
+  x->range_gates = range_gates;
+  x->n_range_a = (int32_t)((a.size() + range_gates - 1) / range_gates);
+  x->n_range_b = (int32_t)((b.size() + range_gates - 1) / range_gates);
+  x->order.assign((size_t)(x->n_range_a + x->n_range_b) * range_gates, -1);
+  std::copy(a.begin(), a.end(), x->order.begin());
+  std::copy(b.begin(), b.end(), x->order.begin() + (size_t)x->n_range_a * range_gates);
+}
+
+}  // namespace
+
+GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
+  if (!x) return GTP_ERR_ARG;
+  if (!x->have_trace) return x->fail(GTP_ERR_STATE, "gtp_run: call gtp_set_trace first");
+  if (!res) return x->fail(GTP_ERR_ARG, "gtp_run: null result");
+  HIPCHK(x, hipSetDevice(x->device));
+  hipStream_t s = x->stream;
+  hipEvent_t ev0, ev1, ev2;
+  HIPCHK(x, hipEventCreate(&ev0));
+  HIPCHK(x, hipEventCreate(&ev1));
+  HIPCHK(x, hipEventCreate(&ev2));
+  struct EvGuard {
+    hipEvent_t a, b, c;
+    ~EvGuard() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+      (void)hipEventDestroy(c);
+    }
+  } evg{ev0, ev1, ev2};
+  HIPCHK(x, hipEventRecord(ev0, s));
+
+  // --- schedule parameters
+  int64_t TC = x->opt.tile_cycles > 0 ? x->opt.tile_cycles : 512;
+  if (x->T < (1 << 24)) TC = std::min<int64_t>(TC, std::max<int64_t>(1, ((1 << 24) - 1) / x->T));
+  TC = std::max<int64_t>(1, std::min<int64_t>(TC, x->C));
+  TC = std::min<int64_t>(TC, 4096);
+  const int32_t n_tile = (int32_t)((x->C + TC - 1) / TC);
+  const int32_t RG = x->opt.gates_per_wg > 0 ? x->opt.gates_per_wg : 4096;
+  const int block = x->opt.block_lanes > 0 ? std::min(1024, std::max(64, (x->opt.block_lanes + 63) / 64 * 64)) : 256;
+  if (x->order.empty() || x->range_gates != RG) {
+    build_order(x, RG);
+    HIPCHK(x, x->d_order.upload(x->order.data(), x->order.size(), s));
+  }
+  const int64_t n_sched = (int64_t)x->order.size();
+  const int32_t n_range = x->n_range_a + x->n_range_b;
+  if (x->tile_idx_tc != TC || !x->d_tile_idx.p) {
+    HIPCHK(x, x->d_tile_idx.alloc((size_t)x->N * (n_tile + 1) + 1));
+    HIPCHK(x, launch_tile_index(x->d_net_off.p, x->d_ev.p, x->N, n_tile, TC * x->T, x->d_tile_idx.p, s));
+    x->tile_idx_tc = (int32_t)TC;
+  }
+  // --- fixed-point quanta (per-cycle sums stay below 2^62 quanta)
+  const double two62 = 4611686018427387904.0;
+  double emax = std::max((double)x->energy_max, x->csw_max);
+  if (!(emax > 0)) emax = 1e-30;
+  const double q_e_req = emax * 2.0 * (double)std::max<int64_t>(x->P_ev, 1) / two62;
+  const float inv_q_e = (float)(1.0 / q_e_req);
+  const double q_e = 1.0 / (double)inv_q_e;
+  double lmax = x->leak_max > 0 ? x->leak_max : 1e-30;
+  const double q_l_req = lmax * 4.0 * (double)(x->G + std::max<int64_t>(x->P_ev, 1)) / two62;
+  const float inv_q_l = (float)(1.0 / q_l_req);
+  const double q_l = 1.0 / (double)inv_q_l;
+  {
+    std::vector<long long> lq(x->leak_f.size());
+    for (size_t i = 0; i < lq.size(); ++i) lq[i] = std::llrint((double)x->leak_f[i] * (double)inv_q_l);
+    HIPCHK(x, hipMemcpyAsync(x->d_leak_q.p, lq.data(), lq.size() * sizeof(long long), hipMemcpyHostToDevice, s));
+    HIPCHK(x, hipStreamSynchronize(s));
+  }
+  // --- scratch
+  HIPCHK(x, x->d_pg_hi.alloc((size_t)N_PG * n_tile * n_sched));
+  HIPCHK(x, x->d_pg_lo.alloc((size_t)N_PG * n_tile * n_sched));
+  HIPCHK(x, x->d_pg_tog.alloc((size_t)n_tile * n_sched));
+  HIPCHK(x, x->d_pg_glc.alloc((size_t)n_tile * n_sched));
+  HIPCHK(x, x->d_cyc_part.alloc((size_t)n_tile * n_range * N_BIN * TC));
+  HIPCHK(x, x->d_cycle.alloc((size_t)GTP_N_WAVE * x->C));
+  HIPCHK(x, x->d_gate.alloc((size_t)GTP_N_COMPONENT * x->G));
+  HIPCHK(x, x->d_total.alloc(5 + 25));
+  HIPCHK(x, x->d_gtog.alloc(x->G));
+  HIPCHK(x, x->d_gglc.alloc(x->G));
+
+  PowerArgs a{};
+  a.order = x->d_order.p;
+  a.n_sched = n_sched;
+  a.gate_cell = x->d_gate_cell.p;
+  a.gate_pin_off = x->d_gate_pin_off.p;
+  a.gate_arc_off = x->d_gate_arc_off.p;
+  a.gate_energy_off = x->d_gate_energy_off.p;
+  a.pin_net = x->d_pin_net.p;
+  a.pin_gk = x->d_pin_gk.p;
+  a.time_scale = (float)x->scale;
+  a.pin_csw = x->d_pin_csw.p;
+  a.pin_default = x->d_pin_default.p;
+  a.energy = x->d_energy.p;
+  a.arc_q = x->d_arc_q.p;
+  a.net_off = x->d_net_off.p;
+  a.ev = x->d_ev.p;
+  a.tile_idx = x->d_tile_idx.p;
+  a.n_tile = n_tile;
+  a.tile_cycles = (int32_t)TC;
+  a.T = x->T;
+  a.t_max = x->t_max;
+  a.n_cycle = x->C;
+  a.inv_T = (float)(1.0 / (double)x->T);
+  a.n_range = n_range;
+  a.range_gates = RG;
+  a.xcd_remap = 1;
+  a.cells = x->d_cells.p;
+  a.n_cell = (int32_t)x->cell_dev.size();
+  a.leak = x->d_leak.p;
+  a.leak_q = x->d_leak_q.p;
+  a.n_leak = (int32_t)x->leak_f.size();
+  a.inv_q_energy = inv_q_e;
+  a.inv_q_leak = inv_q_l;
+  a.pg_hi = x->d_pg_hi.p;
+  a.pg_lo = x->d_pg_lo.p;
+  a.pg_tog = x->d_pg_tog.p;
+  a.pg_glitch = x->d_pg_glc.p;
+  a.cyc_part = x->d_cyc_part.p;
+  const size_t lds = (size_t)N_BIN * TC * 8 + (size_t)a.n_leak * 8 + (size_t)(a.n_leak + (a.n_leak & 1)) * 4 +
+                     (size_t)a.n_cell * sizeof(CellDev);
+  if (lds > 160 * 1024) return x->fail(GTP_ERR_LIMIT, "gtp_run: tile needs more LDS than a workgroup has");
+  HIPCHK(x, hipEventRecord(ev1, s));
+  if (x->n_range_a > 0) {
+    const int64_t nw = (int64_t)n_tile * x->n_range_a;
+    HIPCHK(x, launch_power(MAXP_SMALL, (nw + 7) / 8 * 8, block, lds, s, a, 0, x->n_range_a));
+  }
+  if (x->n_range_b > 0) {
+    const int64_t nw = (int64_t)n_tile * x->n_range_b;
+    HIPCHK(x, launch_power(MAXP_LARGE, (nw + 7) / 8 * 8, block, lds, s, a, x->n_range_a, x->n_range_b));
+  }
+  HIPCHK(x, hipEventRecord(ev2, s));
+  HIPCHK(x, launch_cycle_reduce(x->d_cyc_part.p, n_range, (int32_t)TC, x->C, x->T, x->t_max, q_e, q_l,
+                                1.0 / x->period, x->d_cycle.p, s));
+  HIPCHK(x, launch_gate_reduce(x->d_pg_hi.p, x->d_pg_lo.p, x->d_pg_tog.p, x->d_pg_glc.p, x->d_order.p, n_sched,
+                               n_tile, x->G, 1.0 / ((double)x->t_max * x->scale), 1.0 / (double)x->t_max,
+                               x->d_gate.p, x->d_gtog.p, x->d_gglc.p, s));
+  HIPCHK(x, launch_total(x->d_gate.p, x->d_gate_class.p, x->G, x->d_total.p, s));
+  if (res->cycle_power)
+    HIPCHK(x, hipMemcpyAsync(res->cycle_power, x->d_cycle.p, sizeof(double) * GTP_N_WAVE * x->C, hipMemcpyDeviceToHost, s));
+  if (res->gate_power)
+    HIPCHK(x, hipMemcpyAsync(res->gate_power, x->d_gate.p, sizeof(double) * GTP_N_COMPONENT * x->G, hipMemcpyDeviceToHost, s));
+  if (res->gate_toggle_x2)
+    HIPCHK(x, hipMemcpyAsync(res->gate_toggle_x2, x->d_gtog.p, sizeof(uint64_t) * x->G, hipMemcpyDeviceToHost, s));
+  if (res->gate_glitch)
+    HIPCHK(x, hipMemcpyAsync(res->gate_glitch, x->d_gglc.p, sizeof(uint64_t) * x->G, hipMemcpyDeviceToHost, s));
+  double tot[30];
+  HIPCHK(x, hipMemcpyAsync(tot, x->d_total.p, sizeof(tot), hipMemcpyDeviceToHost, s));
+  hipEvent_t ev3;
+  HIPCHK(x, hipEventCreate(&ev3));
+  HIPCHK(x, hipEventRecord(ev3, s));
+  HIPCHK(x, hipStreamSynchronize(s));
+  float k_ms = 0, t_ms = 0;
+  HIPCHK(x, hipEventElapsedTime(&k_ms, ev1, ev2));
+  HIPCHK(x, hipEventElapsedTime(&t_ms, ev0, ev3));
+  (void)hipEventDestroy(ev3);
+  if (res->total) std::memcpy(res->total, tot, sizeof(double) * 5);
+  if (res->class_total) std::memcpy(res->class_total, tot + 5, sizeof(double) * 25);
+  res->n_cycle = x->C;
+  res->kernel_ms = k_ms;
+  res->total_ms = t_ms;
+  gtp_schedule_info& si = x->sched;
+  si.tile_cycles = (int32_t)TC;
+  si.block_lanes = block;
+  si.gates_per_wg = RG;
+  si.split_target = 0;
+  si.n_unit = (int64_t)n_tile * x->G;
+  si.n_workgroup = (int64_t)n_tile * n_range;
+  si.n_tile = n_tile;
+  si.n_event = x->E;
+  si.n_pin_event = x->P_ev;
+  si.autotune_ms = 0;
+  return GTP_OK;
+}
+
+GTP_API gtp_status gtp_run_batch(gtp_ctx* x, int32_t n_trace, const gtp_trace* traces, gtp_result* results) {
+  if (!x) return GTP_ERR_ARG;
+  (void)n_trace;
+  (void)traces;
+  (void)results;
+  return x->fail(GTP_ERR_UNSUPPORTED, "gtp_run_batch: not built yet");
+}
+
+GTP_API gtp_status gtp_get_schedule(const gtp_ctx* x, gtp_schedule_info* info) {
+  if (!x || !info) return GTP_ERR_ARG;
+  *info = x->sched;
+  return GTP_OK;
+}
+
+// power/cuda/Gate.cuh:354-376 setNCyclePerThreadByEventCount, per gate.
+GTP_API gtp_status gtp_reference_cycles_per_thread(int64_t n, const int64_t* n_event, int64_t n_cycle,
+                                                   int64_t e_target, int64_t parallelism_floor,
+                                                   int64_t* out) {
+  if (n < 0 || (n > 0 && (!n_event || !out)) || n_cycle <= 0 || e_target <= 0 || parallelism_floor <= 0)
+    return GTP_ERR_ARG;
+  int64_t n_cycle_target = (n_cycle + parallelism_floor - 1) / parallelism_floor;
+  n_cycle_target = std::max<int64_t>(n_cycle_target, 1);
+  const double rho_floor = (double)e_target / (double)n_cycle_target;
+  for (int64_t i = 0; i < n; ++i) {
+    if (n_event[i] < 0) return GTP_ERR_ARG;
+    const double rho = (double)n_event[i] / (double)n_cycle;
+    int64_t v = (int64_t)std::llround((double)e_target / (rho + rho_floor));
+    out[i] = std::max<int64_t>(1, std::min(v, n_cycle));
+  }
+  return GTP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Writers (power/cuda/CudaPower.cu:3134-3165, include/sta/PowerClass.hh:123-149)
+// ---------------------------------------------------------------------------
+GTP_API gtp_status gtp_write_cell_power(const char* path, int64_t n_gate, const double* gp,
+                                        const char* gate_names, const int32_t* gate_cell,
+                                        const char* cell_names) {
+  if (!path || n_gate < 0 || (n_gate > 0 && !gp)) return GTP_ERR_ARG;
+  std::vector<const char*> cells;
+  if (cell_names)
+    for (const char* c = cell_names; *c; c += std::strlen(c) + 1) cells.push_back(c);
+  std::ofstream f(path, std::ios::out);
+  if (!f) return GTP_ERR_ARG;
+  f << "cell\tInternal_Power\tGlitch_Internal_Power\tSwitching_Power\tGlitch_Switching_Power\tLeakage_Power\tTotal_Power\tStandard_Cell\n";
+  const char* name = gate_names;
+  for (int64_t g = 0; g < n_gate; ++g) {
+    const float v[5] = {(float)gp[0 * n_gate + g], (float)gp[1 * n_gate + g], (float)gp[2 * n_gate + g],
+                        (float)gp[3 * n_gate + g], (float)gp[4 * n_gate + g]};
+    if (name) {
+      f << name;
+      name += std::strlen(name) + 1;
+    } else {
+      f << "g" << g;
+    }
+    f << '\t' << v[0] << '\t' << v[1] << '\t' << v[2] << '\t' << v[3] << '\t' << v[4] << '\t'
+      << v[0] + v[1] + v[2] + v[3] + v[4] << '\t';
+    if (gate_cell && gate_cell[g] >= 0 && (size_t)gate_cell[g] < cells.size()) f << cells[gate_cell[g]];
+    f << '\n';
+  }
+  return f ? GTP_OK : GTP_ERR_ARG;
+}
+
+GTP_API gtp_status gtp_write_per_cycle(const char* path, const char* top_name, int64_t n_cycle,
+                                       const double* cp) {
+  if (!path || n_cycle < 0 || (n_cycle > 0 && !cp)) return GTP_ERR_ARG;
+  std::ofstream f(path, std::ios::out);
+  if (!f) return GTP_ERR_ARG;
+  static const char* titles[5] = {"Per cycle leakage power: ", "Per cycle internal power: ",
+                                  "Per cycle glitch internal power: ", "Per cycle switching power: ",
+                                  "Per cycle glitch switching power: "};
+  for (int w = 0; w < 5; ++w) {
+    f << titles[w] << std::endl;
+    f << (top_name ? top_name : "") << '\t';
+    for (int64_t c = 0; c < n_cycle; ++c) f << (float)cp[(int64_t)w * n_cycle + c] << '\t';
+    f << std::endl;
+  }
+  f << "End of per cycle power" << std::endl;
+  return f ? GTP_OK : GTP_ERR_ARG;
+}

@@ -13,6 +13,8 @@
 #pragma once
 #include <stdint.h>
 
+#include <hip/hip_runtime.h>
+
 namespace gtp {
 
 constexpr uint32_t EV_TIME_BITS = 30;
@@ -108,5 +110,42 @@ struct PowerArgs {
   uint32_t* pg_glitch;           // [n_tile][n_sched]
   long long* cyc_part;           // [n_tile][n_range][N_BIN][tile_cycles]
 };
+
+// Arguments of the energy (table interpolation) kernel.
+struct EnergyArgs {
+  const int32_t* gate_cell;
+  const int64_t* gate_pin_off;
+  const int64_t* gate_energy_off;
+  int64_t n_gate;
+  const float* pin_slew;      // [2P]
+  const float* pin_cap;       // [P]
+  const CellDev* cells;
+  const int32_t* cell_slot_desc_off;  // [n_cell] into descs
+  const SlotDesc* descs;
+  const ListEntry* list;
+  const TableDev* tables;
+  int32_t n_table;
+  const float* table_data;
+  int32_t n_table_data;
+  float* energy;
+  unsigned int* max_abs_bits;
+};
+
+// Launch wrappers, defined with the kernels.
+hipError_t launch_pack(const int64_t* t, const uint8_t* v, uint32_t* out, int64_t n, hipStream_t s);
+hipError_t launch_tile_index(const int64_t* net_off, const uint32_t* ev, int64_t n_net, int32_t n_tile,
+                             int64_t tile_ticks, int32_t* out, hipStream_t s);
+hipError_t launch_energy(const EnergyArgs& a, hipStream_t s);
+hipError_t launch_power(int maxp, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,
+                        int32_t range_begin, int32_t range_count);
+hipError_t launch_cycle_reduce(const long long* part, int32_t n_range, int32_t tile_cycles, int64_t n_cycle,
+                               int64_t T, int64_t t_max, double q_energy, double q_leak, double inv_period,
+                               double* out, hipStream_t s);
+hipError_t launch_gate_reduce(const float* hi, const float* lo, const uint32_t* tog, const uint32_t* glc,
+                              const int32_t* order, int64_t n_sched, int32_t n_tile, int64_t n_gate,
+                              double norm_energy, double norm_leak, double* gate_power,
+                              unsigned long long* gate_tog, unsigned long long* gate_glitch, hipStream_t s);
+hipError_t launch_total(const double* gate_power, const int32_t* gate_class, int64_t n_gate, double* out,
+                        hipStream_t s);
 
 }  // namespace gtp

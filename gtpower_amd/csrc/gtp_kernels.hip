@@ -16,6 +16,8 @@
 // LUTs (util/cuda/OneDimensionalLUT.cuh:62-86, TwoDimensionalLUT.cuh:113-164).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "gtp_internal.hpp"
 
 namespace gtp {
@@ -144,24 +146,7 @@ __device__ float interp_f(const float* D, const TableDev& tb, float slew, float 
          d1 * d2 * v[(i + 1) * n2 + j + 1] + (1.f - d1) * d2 * v[i * n2 + j + 1];
 }
 
-struct EnergyArgs {
-  const int32_t* gate_cell;
-  const int64_t* gate_pin_off;
-  const int64_t* gate_energy_off;
-  int64_t n_gate;
-  const float* pin_slew;      // [2P]
-  const float* pin_cap;       // [P]
-  const CellDev* cells;
-  const int32_t* cell_slot_desc_off;  // [n_cell] into descs
-  const SlotDesc* descs;
-  const ListEntry* list;
-  const TableDev* tables;
-  int32_t n_table;
-  const float* table_data;
-  int32_t n_table_data;
-  float* energy;
-  unsigned int* max_abs_bits;
-};
+
 
 __global__ void energy_kernel(EnergyArgs a) {
   extern __shared__ float s_tab[];
@@ -620,6 +605,68 @@ __global__ void total_kernel(const double* __restrict__ gate_power, const int32_
       __syncthreads();
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launch wrappers (the API translation unit launches through these)
+// ---------------------------------------------------------------------------
+hipError_t launch_pack(const int64_t* t, const uint8_t* v, uint32_t* out, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+  pack_events_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(t, v, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_index(const int64_t* net_off, const uint32_t* ev, int64_t n_net, int32_t n_tile,
+                             int64_t tile_ticks, int32_t* out, hipStream_t s) {
+  const int64_t n = n_net * ((int64_t)n_tile + 1);
+  if (n <= 0) return hipSuccess;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+  tile_index_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(net_off, ev, n_net, n_tile, tile_ticks, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy(const EnergyArgs& a, hipStream_t s) {
+  if (a.n_gate <= 0) return hipSuccess;
+  const int64_t blocks = (a.n_gate + 255) / 256;
+  energy_kernel<<<dim3((unsigned)blocks), dim3(256), sizeof(float) * a.n_table_data, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_power(int maxp, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,
+                        int32_t range_begin, int32_t range_count) {
+  if (grid <= 0) return hipSuccess;
+  if (maxp <= MAXP_SMALL)
+    power_tile_kernel<MAXP_SMALL><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count);
+  else
+    power_tile_kernel<MAXP_LARGE><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count);
+  return hipGetLastError();
+}
+
+hipError_t launch_cycle_reduce(const long long* part, int32_t n_range, int32_t tile_cycles, int64_t n_cycle,
+                               int64_t T, int64_t t_max, double q_energy, double q_leak, double inv_period,
+                               double* out, hipStream_t s) {
+  if (n_cycle <= 0) return hipSuccess;
+  cycle_reduce_kernel<<<dim3((unsigned)((n_cycle + 255) / 256)), dim3(256), 0, s>>>(
+      part, n_range, tile_cycles, n_cycle, T, t_max, q_energy, q_leak, inv_period, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gate_reduce(const float* hi, const float* lo, const uint32_t* tog, const uint32_t* glc,
+                              const int32_t* order, int64_t n_sched, int32_t n_tile, int64_t n_gate,
+                              double norm_energy, double norm_leak, double* gate_power,
+                              unsigned long long* gate_tog, unsigned long long* gate_glitch, hipStream_t s) {
+  if (n_sched <= 0) return hipSuccess;
+  gate_reduce_kernel<<<dim3((unsigned)((n_sched + 255) / 256)), dim3(256), 0, s>>>(
+      hi, lo, tog, glc, order, n_sched, n_tile, n_gate, norm_energy, norm_leak, gate_power, gate_tog,
+      gate_glitch);
+  return hipGetLastError();
+}
+
+hipError_t launch_total(const double* gate_power, const int32_t* gate_class, int64_t n_gate, double* out,
+                        hipStream_t s) {
+  total_kernel<<<dim3(1), dim3(256), 0, s>>>(gate_power, gate_class, n_gate, out);
+  return hipGetLastError();
 }
 
 }  // namespace gtp

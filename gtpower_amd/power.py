@@ -1,0 +1,157 @@
+"""Host-side mirror of the reference's power interface over the C ABI.
+
+The reference selects a power backend in Sta::makePower (search/Sta.cc:502-509)
+and calls ``power(corner, total, sequential, combinational, clock, macro, pad)``
+then ``printCellRes()`` (power/Power.hh:80-109).  ``HipPower`` offers the same
+two operations on explicit inputs:
+
+    engine = HipPower(options)
+    engine.set_design(library, design)        # initGateData + LUT construction
+    engine.read_power_activities(trace)       # read_power_activities -vcd (event store)
+    res = engine.power()                      # report_power: totals, classes, waveforms
+    engine.print_cell_res(path, names)        # cuda_cell_power.txt
+    engine.print_per_cycle_results(path, top) # cuda_per_cycle_waveform.txt
+
+There is no CPU fallback: every call goes to libgtpower_hip.so, and a missing
+library or device raises.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+
+@dataclass
+class PowerResult:
+    """Totals and waveforms of one run (reference: include/sta/PowerClass.hh:95-161)."""
+    gate_power: np.ndarray          # [5, G]: internal, glitch internal, switching, glitch switching, leakage
+    cycle_power: np.ndarray         # [5, C]: leakage, internal, glitch internal, switching, glitch switching
+    total: np.ndarray               # [5]
+    class_total: np.ndarray         # [5 classes, 5]
+    gate_toggle_x2: np.ndarray      # [G]
+    gate_glitch: np.ndarray         # [G]
+    n_cycle: int
+    kernel_ms: float
+    total_ms: float
+    schedule: dict = field(default_factory=dict)
+
+    @property
+    def total_power(self):
+        return float(self.total.sum())
+
+
+def options(**kw):
+    o = abi.GtpOptions()
+    lib = abi.load_engine()
+    lib.gtp_options_init(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class HipPower:
+    """One engine context bound to one HIP device."""
+
+    def __init__(self, opts=None, **kw):
+        self._lib = abi.load_engine()
+        self._opts = opts if opts is not None else options(**kw)
+        h = C.c_void_p()
+        st = self._lib.gtp_create(C.byref(self._opts), C.byref(h))
+        if st != 0:
+            raise abi.GtpError(st, "gtp_create", self._lib.gtp_create_error().decode())
+        self._h = h
+        self._keep = []
+        self.n_gate = 0
+
+    def _check(self, st, where):
+        if st != 0:
+            raise abi.GtpError(st, where, self._lib.gtp_last_error(self._h).decode())
+
+    def set_design(self, library, design):
+        self._keep = [library, design]
+        lv = library.as_abi()
+        dv = design.as_abi()
+        self._check(self._lib.gtp_set_design(self._h, C.byref(lv), C.byref(dv)), "gtp_set_design")
+        self.n_gate = dv.n_gate
+        self.library = library
+        self.design = design
+
+    def read_power_activities(self, trace):
+        tv = trace.as_abi()
+        self._check(self._lib.gtp_set_trace(self._h, C.byref(tv)), "gtp_set_trace")
+        self.trace = trace
+
+    def power(self, want=("gate", "cycle", "counts")):
+        G = self.n_gate
+        T = self.trace.ticks_per_cycle
+        Cn = -(-self.trace.t_max // T)
+        gp = np.zeros(5 * G) if "gate" in want else None
+        cp = np.zeros(5 * Cn) if "cycle" in want else None
+        tot = np.zeros(5)
+        cls = np.zeros(25)
+        tg = np.zeros(G, np.uint64) if "counts" in want else None
+        gl = np.zeros(G, np.uint64) if "counts" in want else None
+        r = abi.GtpResult(abi.ptr(gp, C.c_double), abi.ptr(cp, C.c_double), abi.ptr(tot, C.c_double),
+                          abi.ptr(cls, C.c_double), abi.ptr(tg, C.c_uint64), abi.ptr(gl, C.c_uint64),
+                          0, 0.0, 0.0)
+        self._check(self._lib.gtp_run(self._h, C.byref(r)), "gtp_run")
+        si = abi.GtpScheduleInfo()
+        self._check(self._lib.gtp_get_schedule(self._h, C.byref(si)), "gtp_get_schedule")
+        sched = {k: getattr(si, k) for k, _ in abi.GtpScheduleInfo._fields_}
+        self.last = PowerResult(gp.reshape(5, G) if gp is not None else None,
+                                cp.reshape(5, Cn) if cp is not None else None, tot, cls.reshape(5, 5),
+                                tg, gl, r.n_cycle, r.kernel_ms, r.total_ms, sched)
+        return self.last
+
+    def print_cell_res(self, path, gate_names=None):
+        """Write cuda_cell_power.txt (power/cuda/CudaPower.cu:3134-3165)."""
+        res = self.last
+        gp = np.ascontiguousarray(res.gate_power.reshape(-1))
+        names = None
+        if gate_names is not None:
+            names = b"".join(n.encode() + b"\0" for n in gate_names)
+        cells = b"".join(n.encode() + b"\0" for n in self.library.full_names) + b"\0"
+        gc = np.ascontiguousarray(self.design.gate_cell, dtype=np.int32)
+        st = self._lib.gtp_write_cell_power(str(path).encode(), self.n_gate, abi.ptr(gp, C.c_double), names,
+                                            abi.ptr(gc, C.c_int32), cells)
+        if st != 0:
+            raise abi.GtpError(st, "gtp_write_cell_power")
+
+    def print_per_cycle_results(self, path, top_name="top"):
+        """Write the per-cycle waveform file (include/sta/PowerClass.hh:123-149)."""
+        cp = np.ascontiguousarray(self.last.cycle_power.reshape(-1))
+        st = self._lib.gtp_write_per_cycle(str(path).encode(), top_name.encode(), self.last.n_cycle,
+                                           abi.ptr(cp, C.c_double))
+        if st != 0:
+            raise abi.GtpError(st, "gtp_write_per_cycle")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.gtp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def reference_cycles_per_thread(n_event, n_cycle, e_target=8, parallelism_floor=512):
+    """The reference's per-gate cycles-per-thread (power/cuda/Gate.cuh:354-376)."""
+    lib = abi.load_engine()
+    ne = np.ascontiguousarray(n_event, dtype=np.int64)
+    out = np.zeros(len(ne), dtype=np.int64)
+    st = lib.gtp_reference_cycles_per_thread(len(ne), abi.ptr(ne, C.c_int64), n_cycle, e_target,
+                                             parallelism_floor, abi.ptr(out, C.c_int64))
+    if st != 0:
+        raise abi.GtpError(st, "gtp_reference_cycles_per_thread")
+    return out

@@ -17,6 +17,9 @@ from .library import CellLibrary
 MASTER_SEED = 0x47545057
 
 COMB_MIX = ("inv_1", "buf_1", "nand2_1", "nor2_1", "xor2_1", "a21oi_1", "mux2_1")
+# Share of each combinational cell in the random netlists (XOR/MUX kept rare,
+# as in synthesized logic, so that activity does not grow without bound with depth).
+COMB_WEIGHTS = (0.20, 0.15, 0.25, 0.15, 0.05, 0.10, 0.10)
 
 
 class DesignParams(C.Structure):
@@ -210,11 +213,11 @@ class Design:
 
 def make_design(n_gate, dff_fraction=0.10, depth=30, fanout=(1, 4), wire_cap_ff=(1.0, 10.0),
                 clock_ns=10.0, time_scale=1e-12, seed=MASTER_SEED, n_pi=0, threads=0,
-                library=None, pi_slew_ns=0.05):
+                library=None, pi_slew_ns=0.05, mix_weights=COMB_WEIGHTS):
     library = library or CellLibrary()
     lib = _setup(abi.load_synth())
     mix = np.array([library.index(c) for c in COMB_MIX], dtype=np.int32)
-    w = np.ones(len(mix), dtype=np.float64)
+    w = np.asarray(mix_weights, dtype=np.float64)
     arcs = library.arc_table_index()
     funcs = np.array([c["func"] or 0 for c in library.cells], dtype=np.uint64)
     pin_cap = np.zeros((len(library.cells), abi.MAX_CELL_PINS), dtype=np.float64)
@@ -263,3 +266,54 @@ def workload(name, scale_gates=None, scale_cycles=None, threads=0):
     d = make_design(seed=seed, threads=threads, **dk)
     t = d.trace(seed=seed + 17, threads=threads, **tk)
     return d, t
+
+
+class ArrayDesign:
+    """A design held in numpy arrays (hand-built test designs)."""
+
+    def __init__(self, library, gate_cell, pin_net, pin_slew, pin_load_cap, arc_delay,
+                 n_net, pin_voltage=None, pin_default_state=None):
+        self.library = library
+        self.gate_cell = np.ascontiguousarray(gate_cell, dtype=np.int32)
+        G = len(self.gate_cell)
+        npin = library.cell_n_pin[self.gate_cell].astype(np.int64)
+        nin = library.cell_n_in[self.gate_cell].astype(np.int64)
+        self.gate_pin_offset = np.zeros(G + 1, np.int64)
+        np.cumsum(npin, out=self.gate_pin_offset[1:])
+        self.gate_arc_offset = np.zeros(G + 1, np.int64)
+        np.cumsum((npin - nin) * nin * 4, out=self.gate_arc_offset[1:])
+        P = int(self.gate_pin_offset[-1])
+        self.pin_net = np.ascontiguousarray(pin_net, dtype=np.int32)
+        self.pin_slew = np.ascontiguousarray(pin_slew, dtype=np.float64).reshape(-1)
+        self.pin_load_cap = np.ascontiguousarray(pin_load_cap, dtype=np.float64)
+        self.arc_delay = np.ascontiguousarray(arc_delay, dtype=np.float64).reshape(-1)
+        self.pin_voltage = np.ascontiguousarray(
+            pin_voltage if pin_voltage is not None else np.full(P, library.voltage), dtype=np.float64)
+        self.pin_default_state = np.ascontiguousarray(
+            pin_default_state if pin_default_state is not None else np.full(P, 2), dtype=np.int8)
+        assert len(self.pin_net) == P and len(self.pin_slew) == 2 * P and len(self.pin_load_cap) == P
+        assert len(self.arc_delay) == self.gate_arc_offset[-1]
+        self.n_gate, self.n_pin, self.n_net = G, P, int(n_net)
+
+    def as_abi(self):
+        v = abi.GtpDesign()
+        v.n_gate, v.n_pin, v.n_net = self.n_gate, self.n_pin, self.n_net
+        v.gate_cell = abi.ptr(self.gate_cell, C.c_int32)
+        v.gate_pin_offset = abi.ptr(self.gate_pin_offset, C.c_int64)
+        v.gate_arc_offset = abi.ptr(self.gate_arc_offset, C.c_int64)
+        v.arc_delay = abi.ptr(self.arc_delay, C.c_double) if len(self.arc_delay) else None
+        v.pin_net = abi.ptr(self.pin_net, C.c_int32)
+        v.pin_slew = abi.ptr(self.pin_slew, C.c_double)
+        v.pin_load_cap = abi.ptr(self.pin_load_cap, C.c_double)
+        v.pin_voltage = abi.ptr(self.pin_voltage, C.c_double)
+        v.pin_default_state = abi.ptr(self.pin_default_state, C.c_int8)
+        return v
+
+
+def trace_from_lists(waves, t_max, time_scale=1e-12, clk_period=1e-9):
+    """An ArrayTrace from per-net [(time, value), ...] lists."""
+    off = np.zeros(len(waves) + 1, np.int64)
+    off[1:] = np.cumsum([len(w) for w in waves])
+    t = np.array([e[0] for w in waves for e in w], dtype=np.int64)
+    v = np.array([e[1] for w in waves for e in w], dtype=np.uint8)
+    return ArrayTrace(off, t, v, t_max, time_scale, clk_period)
