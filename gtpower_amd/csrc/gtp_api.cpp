@@ -98,6 +98,9 @@ struct gtp_ctx {
   gtp_options opt{};
   int device = 0;
   hipStream_t stream = nullptr;
+  static constexpr int N_SIDE = 3;
+  hipStream_t side[N_SIDE] = {nullptr, nullptr, nullptr};  // concurrent per-cell launches
+  hipEvent_t fork = nullptr, join[N_SIDE] = {nullptr, nullptr, nullptr};
   std::string err;
   int threads = 0;
 
@@ -125,7 +128,7 @@ struct gtp_ctx {
   DArr<float> d_pin_gk, d_pin_csw;
   DArr<int8_t> d_pin_default;
   DArr<float> d_energy;
-  DArr<int64_t> d_arc_q;
+  DArr<int32_t> d_arc_q;
   double arc_q_scale = 0;  // time scale the arc_q table was built for
 
   // trace
@@ -135,20 +138,31 @@ struct gtp_ctx {
   std::vector<int64_t> gate_work;
   DArr<int64_t> d_net_off;
   DArr<uint32_t> d_ev;
+  DArr<uint32_t> d_aux;          // glitch pulse width per event
+  DArr<int64_t> d_chunk_off;     // annotation blocks per net (prefix)
+  int64_t n_chunk = 0;
   DArr<int32_t> d_tile_idx;
   int32_t tile_idx_tc = 0;
 
-  // schedule
-  std::vector<int32_t> order;  // padded, class A then class B
-  int32_t range_gates = 0, n_range_a = 0, n_range_b = 0;
+  // schedule: ranges of range_gates gates; a range holds gates of one
+  // specialised cell type, or generic gates
+  struct Launch {
+    int cell;          // cell type for the specialised kernel, -1 = generic
+    int np;            // pins (specialised) or MAXP of the generic kernel
+    int32_t range_begin, range_count;
+  };
+  std::vector<int32_t> order;
+  std::vector<Launch> launches;
+  int32_t range_gates = 0, n_range = 0;
+  int64_t order_T = 0;
   DArr<int32_t> d_order;
   gtp_schedule_info sched{};
 
   // scratch and results
-  DArr<float> d_pg_hi, d_pg_lo;
+  DArr<long long> d_pg;
   DArr<uint32_t> d_pg_tog, d_pg_glc;
   DArr<long long> d_cyc_part;
-  DArr<double> d_cycle, d_gate, d_total;
+  DArr<double> d_cycle, d_gate, d_total, d_total_part;
   DArr<unsigned long long> d_gtog, d_gglc;
   DArr<unsigned int> d_maxbits;
 
@@ -352,10 +366,19 @@ GTP_API gtp_status gtp_create(const gtp_options* opt, gtp_ctx** out) {
     return GTP_ERR_ARG;
   }
   x->device = dev;
-  if ((e = hipSetDevice(dev)) != hipSuccess || (e = hipStreamCreate(&x->stream)) != hipSuccess) {
+  if ((e = hipSetDevice(dev)) != hipSuccess || (e = hipStreamCreate(&x->stream)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&x->fork, hipEventDisableTiming)) != hipSuccess) {
     g_create_error = std::string("gtp_create: ") + hipGetErrorString(e);
     delete x;
     return GTP_ERR_DEVICE;
+  }
+  for (int k = 0; k < gtp_ctx::N_SIDE; ++k) {
+    if ((e = hipStreamCreate(&x->side[k])) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&x->join[k], hipEventDisableTiming)) != hipSuccess) {
+      g_create_error = std::string("gtp_create: ") + hipGetErrorString(e);
+      gtp_destroy(x);
+      return GTP_ERR_DEVICE;
+    }
   }
   x->threads = x->opt.host_threads > 0 ? x->opt.host_threads
                                        : (int)std::max(1u, std::thread::hardware_concurrency());
@@ -370,6 +393,14 @@ GTP_API void gtp_destroy(gtp_ctx* x) {
     (void)hipStreamSynchronize(x->stream);
     (void)hipStreamDestroy(x->stream);
   }
+  for (int k = 0; k < gtp_ctx::N_SIDE; ++k) {
+    if (x->side[k]) {
+      (void)hipStreamSynchronize(x->side[k]);
+      (void)hipStreamDestroy(x->side[k]);
+    }
+    if (x->join[k]) (void)hipEventDestroy(x->join[k]);
+  }
+  if (x->fork) (void)hipEventDestroy(x->fork);
   delete x;
 }
 
@@ -574,13 +605,29 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
   }
   x->d_tile_idx.release();
   x->tile_idx_tc = 0;
+  {
+    // 256-event annotation blocks per net
+    std::vector<int64_t> co(N + 1, 0);
+    for (int64_t n = 0; n < N; ++n) co[n + 1] = co[n] + (off[n + 1] - off[n] + 255) / 256;
+    x->n_chunk = co[N];
+    HIPCHK(x, x->d_chunk_off.upload(co.data(), co.size(), s));
+    HIPCHK(x, x->d_aux.alloc(std::max<int64_t>(E, 1)));
+    HIPCHK(x, hipStreamSynchronize(s));
+  }
   // arc delays on the 2^-20 tick grid of the related-pin comparison
   if (x->arc_q_scale != tr->time_scale) {
-    std::vector<int64_t> aq(x->arc_delay.size());
+    std::vector<int32_t> aq(x->arc_delay.size());
     const double f = (double)(1 << ARC_FRAC_BITS) / tr->time_scale;
     for (size_t i = 0; i < aq.size(); ++i) {
       const double dl = x->arc_delay[i];
-      aq[i] = std::isnan(dl) ? ARC_NONE : (int64_t)std::llround(dl * f);
+      if (std::isnan(dl)) {
+        aq[i] = ARC_NONE;
+        continue;
+      }
+      const double q = std::round(dl * f);
+      if (!(std::fabs(q) < 2147483000.0))
+        return x->fail(GTP_ERR_LIMIT, "gtp_trace: an arc delay exceeds 2^15 ticks (the 2^-16-tick comparison grid)");
+      aq[i] = (int32_t)q;
     }
     HIPCHK(x, x->d_arc_q.upload(aq.data(), std::max<size_t>(aq.size(), 1), s));
     HIPCHK(x, hipStreamSynchronize(s));
@@ -606,26 +653,44 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
 
 namespace {
 
-// Schedule: gates sorted by work (pin events), heaviest first, split into the
-// <= 4-pin and the 5-6-pin kernel classes, each padded to whole ranges.
+// Schedule: gates grouped by kernel (one group per single-output cell type
+// without `when` conditions, one generic group), each group sorted by work (pin
+// events), heaviest first, and padded with -1 to whole ranges.
 void build_order(gtp_ctx* x, int32_t range_gates) {
-  std::vector<int32_t> a, b;
+  const int nc = (int)x->cells.size();
+  std::vector<std::vector<int32_t>> by_cell(nc);
+  std::vector<int32_t> generic;
+  int generic_maxp = 0;
   for (int64_t g = 0; g < x->G; ++g) {
-    if (x->cells[x->gate_cell[g]].n_pin <= MAXP_SMALL) a.push_back((int32_t)g);
-    else b.push_back((int32_t)g);
+    const HostCell& hc = x->cells[x->gate_cell[g]];
+    // the specialised kernel compares arrivals on 32 bits: cycles below 2^15 ticks
+    const bool fast = hc.sbits == 0 && hc.n_pin - hc.n_in == 1 && hc.n_pin >= 2 && hc.n_pin <= 6 &&
+                      x->T < 32768;
+    if (fast) by_cell[x->gate_cell[g]].push_back((int32_t)g);
+    else {
+      generic.push_back((int32_t)g);
+      generic_maxp = std::max(generic_maxp, hc.n_pin);
+    }
   }
   auto by_work = [&](int32_t p, int32_t q) {
     if (x->gate_work[p] != x->gate_work[q]) return x->gate_work[p] > x->gate_work[q];
     return p < q;
   };
-  std::sort(a.begin(), a.end(), by_work);
-  std::sort(b.begin(), b.end(), by_work);
+  x->order.clear();
+  x->launches.clear();
   x->range_gates = range_gates;
-  x->n_range_a = (int32_t)((a.size() + range_gates - 1) / range_gates);
-  x->n_range_b = (int32_t)((b.size() + range_gates - 1) / range_gates);
-  x->order.assign((size_t)(x->n_range_a + x->n_range_b) * range_gates, -1);
-  std::copy(a.begin(), a.end(), x->order.begin());
-  std::copy(b.begin(), b.end(), x->order.begin() + (size_t)x->n_range_a * range_gates);
+  auto add = [&](std::vector<int32_t>& v, int cell, int np) {
+    if (v.empty()) return;
+    std::sort(v.begin(), v.end(), by_work);
+    const int32_t nr = (int32_t)((v.size() + range_gates - 1) / range_gates);
+    const int32_t rb = (int32_t)(x->order.size() / range_gates);
+    x->order.insert(x->order.end(), v.begin(), v.end());
+    x->order.resize((size_t)(rb + nr) * range_gates, -1);
+    x->launches.push_back(gtp_ctx::Launch{cell, np, rb, nr});
+  };
+  for (int c = 0; c < nc; ++c) add(by_cell[c], c, x->cells[c].n_pin);
+  add(generic, -1, generic_maxp <= MAXP_SMALL ? MAXP_SMALL : MAXP_LARGE);
+  x->n_range = (int32_t)(x->order.size() / range_gates);
 }
 
 }  // namespace
@@ -651,35 +716,48 @@ GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
   HIPCHK(x, hipEventRecord(ev0, s));
 
   // --- schedule parameters
-  int64_t TC = x->opt.tile_cycles > 0 ? x->opt.tile_cycles : 512;
+  int64_t TC = x->opt.tile_cycles > 0 ? x->opt.tile_cycles : 128;
   if (x->T < (1 << 24)) TC = std::min<int64_t>(TC, std::max<int64_t>(1, ((1 << 24) - 1) / x->T));
   TC = std::max<int64_t>(1, std::min<int64_t>(TC, x->C));
   TC = std::min<int64_t>(TC, 4096);
   const int32_t n_tile = (int32_t)((x->C + TC - 1) / TC);
-  const int32_t RG = x->opt.gates_per_wg > 0 ? x->opt.gates_per_wg : 4096;
-  const int block = x->opt.block_lanes > 0 ? std::min(1024, std::max(64, (x->opt.block_lanes + 63) / 64 * 64)) : 256;
-  if (x->order.empty() || x->range_gates != RG) {
+  // gates per range: enough workgroups to fill the GPU several times over
+  int32_t RG = x->opt.gates_per_wg;
+  if (RG <= 0) {
+    const double want = (double)x->G * n_tile / 8192.0;
+    RG = 256;
+    while (RG < 4096 && RG * 2 <= want) RG *= 2;
+  }
+  // the power kernel is compiled for at most 256 lanes per workgroup
+  const int block = x->opt.block_lanes > 0 ? std::min(256, std::max(64, (x->opt.block_lanes + 63) / 64 * 64)) : 256;
+  if (x->order.empty() || x->range_gates != RG || x->order_T != x->T) {
+    x->order_T = x->T;
     build_order(x, RG);
     HIPCHK(x, x->d_order.upload(x->order.data(), x->order.size(), s));
   }
   const int64_t n_sched = (int64_t)x->order.size();
-  const int32_t n_range = x->n_range_a + x->n_range_b;
+  const int32_t n_range = x->n_range;
   if (x->tile_idx_tc != TC || !x->d_tile_idx.p) {
     HIPCHK(x, x->d_tile_idx.alloc((size_t)x->N * (n_tile + 1) + 1));
     HIPCHK(x, launch_tile_index(x->d_net_off.p, x->d_ev.p, x->N, n_tile, TC * x->T, x->d_tile_idx.p, s));
     x->tile_idx_tc = (int32_t)TC;
   }
-  // --- fixed-point quanta (per-cycle sums stay below 2^62 quanta)
-  const double two62 = 4611686018427387904.0;
-  double emax = std::max((double)x->energy_max, x->csw_max);
+  // --- fixed-point quanta: powers of two with one term <= 2^30 quanta, so a
+  // term converts through int32 and every sum is exact in int64 (a cycle
+  // would need 2^33 terms, a gate's leakage 2^33 ticks of trace, to overflow)
+  double emax = std::max((double)x->energy_max * 1.000001, x->csw_max);
   if (!(emax > 0)) emax = 1e-30;
-  const double q_e_req = emax * 2.0 * (double)std::max<int64_t>(x->P_ev, 1) / two62;
-  const float inv_q_e = (float)(1.0 / q_e_req);
-  const double q_e = 1.0 / (double)inv_q_e;
-  double lmax = x->leak_max > 0 ? x->leak_max : 1e-30;
-  const double q_l_req = lmax * 4.0 * (double)(x->G + std::max<int64_t>(x->P_ev, 1)) / two62;
-  const float inv_q_l = (float)(1.0 / q_l_req);
-  const double q_l = 1.0 / (double)inv_q_l;
+  const int ke = (int)std::floor(30.0 - std::log2(emax));
+  const float inv_q_e = std::ldexp(1.0f, ke);
+  const double q_e = std::ldexp(1.0, -ke);
+  const double lmax = x->leak_max > 0 ? x->leak_max * 1.000001 : 1e-30;
+  const int kl = (int)std::floor(30.0 - std::log2(lmax));
+  const float inv_q_l = std::ldexp(1.0f, kl);
+  const double q_l = std::ldexp(1.0, -kl);
+  if (!std::isfinite(inv_q_e) || !std::isfinite(inv_q_l) || inv_q_e == 0.f || inv_q_l == 0.f)
+    return x->fail(GTP_ERR_UNSUPPORTED, "gtp_run: energy or leakage magnitudes outside the FP32 range");
+  if ((double)x->t_max * 2147483648.0 >= 9.2e18)
+    return x->fail(GTP_ERR_LIMIT, "gtp_run: trace too long for the int64 leakage sums");
   {
     std::vector<long long> lq(x->leak_f.size());
     for (size_t i = 0; i < lq.size(); ++i) lq[i] = std::llrint((double)x->leak_f[i] * (double)inv_q_l);
@@ -687,14 +765,14 @@ GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
     HIPCHK(x, hipStreamSynchronize(s));
   }
   // --- scratch
-  HIPCHK(x, x->d_pg_hi.alloc((size_t)N_PG * n_tile * n_sched));
-  HIPCHK(x, x->d_pg_lo.alloc((size_t)N_PG * n_tile * n_sched));
+  HIPCHK(x, x->d_pg.alloc((size_t)N_PG * n_tile * n_sched));
   HIPCHK(x, x->d_pg_tog.alloc((size_t)n_tile * n_sched));
   HIPCHK(x, x->d_pg_glc.alloc((size_t)n_tile * n_sched));
   HIPCHK(x, x->d_cyc_part.alloc((size_t)n_tile * n_range * N_BIN * TC));
   HIPCHK(x, x->d_cycle.alloc((size_t)GTP_N_WAVE * x->C));
   HIPCHK(x, x->d_gate.alloc((size_t)GTP_N_COMPONENT * x->G));
   HIPCHK(x, x->d_total.alloc(5 + 25));
+  HIPCHK(x, x->d_total_part.alloc((size_t)TOTAL_BLOCKS * 30));
   HIPCHK(x, x->d_gtog.alloc(x->G));
   HIPCHK(x, x->d_gglc.alloc(x->G));
 
@@ -714,6 +792,7 @@ GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
   a.arc_q = x->d_arc_q.p;
   a.net_off = x->d_net_off.p;
   a.ev = x->d_ev.p;
+  a.aux = x->d_aux.p;
   a.tile_idx = x->d_tile_idx.p;
   a.n_tile = n_tile;
   a.tile_cycles = (int32_t)TC;
@@ -731,30 +810,51 @@ GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
   a.n_leak = (int32_t)x->leak_f.size();
   a.inv_q_energy = inv_q_e;
   a.inv_q_leak = inv_q_l;
-  a.pg_hi = x->d_pg_hi.p;
-  a.pg_lo = x->d_pg_lo.p;
+  a.pg = x->d_pg.p;
   a.pg_tog = x->d_pg_tog.p;
   a.pg_glitch = x->d_pg_glc.p;
   a.cyc_part = x->d_cyc_part.p;
   const size_t lds = (size_t)N_BIN * TC * 8 + (size_t)a.n_leak * 8 + (size_t)(a.n_leak + (a.n_leak & 1)) * 4 +
                      (size_t)a.n_cell * sizeof(CellDev);
   if (lds > 160 * 1024) return x->fail(GTP_ERR_LIMIT, "gtp_run: tile needs more LDS than a workgroup has");
+  const size_t lds_cell = (size_t)N_BIN * TC * 8 * BIN_COPIES;
+  if (lds_cell > 160 * 1024) return x->fail(GTP_ERR_LIMIT, "gtp_run: tile needs more LDS than a workgroup has");
   HIPCHK(x, hipEventRecord(ev1, s));
-  if (x->n_range_a > 0) {
-    const int64_t nw = (int64_t)n_tile * x->n_range_a;
-    HIPCHK(x, launch_power(MAXP_SMALL, (nw + 7) / 8 * 8, block, lds, s, a, 0, x->n_range_a));
+  HIPCHK(x, launch_glitch_annotate(x->d_net_off.p, x->d_ev.p, x->d_chunk_off.p, x->N, x->n_chunk,
+                                   (uint32_t)x->T, a.inv_T, x->d_aux.p, s));
+  // the launches of the cell groups write disjoint outputs: run them
+  // concurrently on the main and the side streams (all default priority)
+  HIPCHK(x, hipEventRecord(x->fork, s));
+  for (int k = 0; k < gtp_ctx::N_SIDE; ++k) HIPCHK(x, hipStreamWaitEvent(x->side[k], x->fork, 0));
+  {
+    std::vector<size_t> idx(x->launches.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](size_t p, size_t q) {
+      return x->launches[p].range_count > x->launches[q].range_count;
+    });
+    int k = 0;
+    for (size_t li : idx) {
+      const gtp_ctx::Launch& L = x->launches[li];
+      hipStream_t ls = (k == 0) ? s : x->side[k - 1];
+      k = (k + 1) % (gtp_ctx::N_SIDE + 1);
+      const int64_t nw = (int64_t)n_tile * L.range_count;
+      if (L.cell >= 0)
+        HIPCHK(x, launch_power_cell(L.np, (nw + 7) / 8 * 8, block, lds_cell, ls, a, L.range_begin, L.range_count, L.cell));
+      else
+        HIPCHK(x, launch_power(L.np, (nw + 7) / 8 * 8, block, lds, ls, a, L.range_begin, L.range_count));
+    }
   }
-  if (x->n_range_b > 0) {
-    const int64_t nw = (int64_t)n_tile * x->n_range_b;
-    HIPCHK(x, launch_power(MAXP_LARGE, (nw + 7) / 8 * 8, block, lds, s, a, x->n_range_a, x->n_range_b));
+  for (int k = 0; k < gtp_ctx::N_SIDE; ++k) {
+    HIPCHK(x, hipEventRecord(x->join[k], x->side[k]));
+    HIPCHK(x, hipStreamWaitEvent(s, x->join[k], 0));
   }
   HIPCHK(x, hipEventRecord(ev2, s));
   HIPCHK(x, launch_cycle_reduce(x->d_cyc_part.p, n_range, (int32_t)TC, x->C, x->T, x->t_max, q_e, q_l,
                                 1.0 / x->period, x->d_cycle.p, s));
-  HIPCHK(x, launch_gate_reduce(x->d_pg_hi.p, x->d_pg_lo.p, x->d_pg_tog.p, x->d_pg_glc.p, x->d_order.p, n_sched,
-                               n_tile, x->G, 1.0 / ((double)x->t_max * x->scale), 1.0 / (double)x->t_max,
+  HIPCHK(x, launch_gate_reduce(x->d_pg.p, x->d_pg_tog.p, x->d_pg_glc.p, x->d_order.p, n_sched,
+                               n_tile, x->G, q_e / ((double)x->t_max * x->scale), q_l / (double)x->t_max,
                                x->d_gate.p, x->d_gtog.p, x->d_gglc.p, s));
-  HIPCHK(x, launch_total(x->d_gate.p, x->d_gate_class.p, x->G, x->d_total.p, s));
+  HIPCHK(x, launch_total(x->d_gate.p, x->d_gate_class.p, x->G, x->d_total_part.p, x->d_total.p, s));
   if (res->cycle_power)
     HIPCHK(x, hipMemcpyAsync(res->cycle_power, x->d_cycle.p, sizeof(double) * GTP_N_WAVE * x->C, hipMemcpyDeviceToHost, s));
   if (res->gate_power)
@@ -782,9 +882,9 @@ GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
   si.tile_cycles = (int32_t)TC;
   si.block_lanes = block;
   si.gates_per_wg = RG;
-  si.split_target = 0;
   si.n_unit = (int64_t)n_tile * x->G;
   si.n_workgroup = (int64_t)n_tile * n_range;
+  si.split_target = (int32_t)x->launches.size();
   si.n_tile = n_tile;
   si.n_event = x->E;
   si.n_pin_event = x->P_ev;

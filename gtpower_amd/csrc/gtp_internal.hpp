@@ -6,10 +6,11 @@
 // Times compare as words (the value bits are below the time bits), so a
 // lower_bound on time is a lower_bound on words.  EV_INF is past every event.
 //
-// Per-cycle results are accumulated as signed 64-bit fixed point: a term x is
-// added as llrint(x / quantum), which makes the sums exact, order-free and
-// therefore bit-reproducible.  Per-gate sums are float-float (hi + lo)
-// compensated accumulations.
+// Results are accumulated as signed 64-bit fixed point: every FP32-evaluated
+// term x is added as rint(x / quantum), with a power-of-two quantum such that
+// one term is at most 2^30 quanta.  The sums are then exact integer sums:
+// order-free and bit-reproducible.  Per-gate leakage is the exact integer
+// sum of (state leakage in quanta) * (ticks in that state).
 #pragma once
 #include <stdint.h>
 
@@ -23,8 +24,8 @@ constexpr uint32_t EV_TIME_INF = 0x3FFFFFFFu;
 constexpr int MAXP_SMALL = 4;                   // kernel variant for cells with <= 4 pins
 constexpr int MAXP_LARGE = 6;                   // GTP_MAX_CELL_PINS
 constexpr int MAX_CELLS_LDS = 64;               // cell types staged in LDS
-constexpr int ARC_FRAC_BITS = 20;               // related-pin comparison grid: 2^-20 tick
-constexpr int64_t ARC_NONE = INT64_MIN;
+constexpr int ARC_FRAC_BITS = 16;               // related-pin comparison grid: 2^-16 tick
+constexpr int32_t ARC_NONE = INT32_MIN;         // arc delays are int32 on that grid (< 2^15 ticks)
 
 // Per-cycle accumulator components (LDS bins and partials).
 enum { BIN_INT = 0, BIN_GINT = 1, BIN_SW = 2, BIN_GSW = 3, BIN_LD = 4, BIN_LW = 5, N_BIN = 6 };
@@ -79,10 +80,11 @@ struct PowerArgs {
   const int8_t* pin_default;     // [P]
   // per-gate energies and arcs
   const float* energy;           // [sum slots]
-  const int64_t* arc_q;          // [sum arcs] delay in 2^-20 ticks, ARC_NONE = no arc
+  const int32_t* arc_q;          // [sum arcs] delay in 2^-16 ticks, ARC_NONE = no arc
   // event store
   const int64_t* net_off;        // [N+1]
   const uint32_t* ev;            // [E]
+  const uint32_t* aux;           // [E] glitch pulse width of each event (0 = not a glitch)
   const int32_t* tile_idx;       // [N * (n_tile + 1)] first event index >= tile start (net-relative)
   int32_t n_tile;
   int32_t tile_cycles;
@@ -100,12 +102,11 @@ struct PowerArgs {
   const float* leak;             // [n_leak] leakage per gate state (W)
   const long long* leak_q;       // [n_leak] the same in leakage quanta
   int32_t n_leak;
-  // fixed-point scales: term / quantum
+  // fixed-point scales (powers of two): term / quantum
   float inv_q_energy;
   float inv_q_leak;
   // outputs
-  float* pg_hi;                  // [N_PG][n_tile][n_sched]
-  float* pg_lo;
+  long long* pg;                 // [N_PG][n_tile][n_sched] per-gate sums in quanta (leakage: quanta * ticks)
   uint32_t* pg_tog;              // [n_tile][n_sched]
   uint32_t* pg_glitch;           // [n_tile][n_sched]
   long long* cyc_part;           // [n_tile][n_range][N_BIN][tile_cycles]
@@ -141,11 +142,18 @@ hipError_t launch_power(int maxp, int64_t grid, int block, size_t lds, hipStream
 hipError_t launch_cycle_reduce(const long long* part, int32_t n_range, int32_t tile_cycles, int64_t n_cycle,
                                int64_t T, int64_t t_max, double q_energy, double q_leak, double inv_period,
                                double* out, hipStream_t s);
-hipError_t launch_gate_reduce(const float* hi, const float* lo, const uint32_t* tog, const uint32_t* glc,
+hipError_t launch_gate_reduce(const long long* pg, const uint32_t* tog, const uint32_t* glc,
                               const int32_t* order, int64_t n_sched, int32_t n_tile, int64_t n_gate,
                               double norm_energy, double norm_leak, double* gate_power,
                               unsigned long long* gate_tog, unsigned long long* gate_glitch, hipStream_t s);
-hipError_t launch_total(const double* gate_power, const int32_t* gate_class, int64_t n_gate, double* out,
-                        hipStream_t s);
+constexpr int TOTAL_BLOCKS = 256;
+constexpr int BIN_COPIES = 4;  // lane-interleaved copies of the per-cycle bins (specialised kernel)
+hipError_t launch_glitch_annotate(const int64_t* net_off, const uint32_t* ev, const int64_t* chunk_off,
+                                  int64_t n_net, int64_t n_chunk, uint32_t T, float invT, uint32_t* aux,
+                                  hipStream_t s);
+hipError_t launch_total(const double* gate_power, const int32_t* gate_class, int64_t n_gate, double* part,
+                        double* out, hipStream_t s);
+hipError_t launch_power_cell(int np, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,
+                             int32_t range_begin, int32_t range_count, int32_t cell);
 
 }  // namespace gtp

@@ -5,10 +5,12 @@
 //   energy_kernel          Liberty table interpolation per gate energy slot, tables staged in LDS
 //   power_tile_kernel<P>   fused per-gate event merge over one cycle tile: toggle and glitch
 //                          filtering, internal / switching / leakage evaluation, per-cycle
-//                          binning in LDS (int64 fixed point) and per-gate float-float sums
+//                          binning in LDS and per-gate sums, all int64 fixed point
 //   cycle_reduce_kernel    per-cycle sum over gate ranges -> FP64 waveforms
-//   gate_reduce_kernel     per-gate sum over tiles -> FP64 per-gate power
-//   total_kernel           FP64 design and class totals
+//   gate_reduce_kernel     per-gate sum over tiles (int64) -> FP64 per-gate power
+//   glitch_annotate_kernel clock-cycle glitch flags and pulse widths, once per net event
+//   power_cell_kernel<NP>  the fused kernel specialised per single-output cell type without `when`
+//   total_*_kernel         FP64 design and class totals
 //
 // Reference counterparts: kernel1AllPowerCalculationTimeRangePartitionedByCycleUnit
 // (power/cuda/CudaPower.cu:317-584), kernel2MergeAllPowerResult (:586-606), the
@@ -61,6 +63,11 @@ __device__ __forceinline__ void ff_add_prod(float& hi, float& lo, float a, float
 
 __device__ __forceinline__ long long quant(float x, float inv_q) {
   return __float2ll_rn(x * inv_q);
+}
+// One term in quanta: |x * inv_q| <= 2^30 by the choice of quantum, so a
+// 32-bit conversion is exact.
+__device__ __forceinline__ long long qterm(float x, float inv_q) {
+  return (long long)(int)rintf(x * inv_q);
 }
 
 // ---------------------------------------------------------------------------
@@ -441,9 +448,9 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
 #pragma unroll
           for (int q = 0; q < MAXP; ++q) {
             if (q >= nin || cnt[q] == 0) continue;
-            const int64_t d = a.arc_q[ab + (((int64_t)o * nin + q) * 2 + rfl[q]) * 2 + rf];
+            const int32_t d = a.arc_q[ab + (((int64_t)o * nin + q) * 2 + rfl[q]) * 2 + rf];
             if (d == ARC_NONE) continue;
-            int64_t diff = (((int64_t)t - (int64_t)lt[q]) << ARC_FRAC_BITS) - d;
+            int64_t diff = (((int64_t)t - (int64_t)lt[q]) << ARC_FRAC_BITS) - (int64_t)d;
             diff = diff < 0 ? -diff : diff;
             if (diff < best && lc[q] == c) {
               best = diff;
@@ -499,8 +506,8 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
     const int64_t plane = (int64_t)a.n_tile * a.n_sched;
 #pragma unroll
     for (int k = 0; k < N_PG; ++k) {
-      a.pg_hi[k * plane + slot] = acc_hi[k];
-      a.pg_lo[k * plane + slot] = acc_lo[k];
+      const float iq = k == PG_LEAK ? a.inv_q_leak : a.inv_q_energy;  // powers of two: exact scaling
+      a.pg[k * plane + slot] = __float2ll_rn(acc_hi[k] * iq) + __float2ll_rn(acc_lo[k] * iq);
     }
     a.pg_tog[slot] = tog;
     a.pg_glitch[slot] = glc;
@@ -522,6 +529,390 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
   __syncthreads();
   long long* out = a.cyc_part + ((int64_t)tile * a.n_range + range) * N_BIN * TC;
   for (int i = threadIdx.x; i < N_BIN * TC; i += blockDim.x) out[i] = bins[i];
+}
+
+
+// ---------------------------------------------------------------------------
+// Glitch filtering, once per net event (power/TimeBasedPower.cc:589-637 and
+// :303-339): aux[e] = the pulse width (ticks) that scales event e when its
+// clock-cycle glitch flag is set, else 0.  A flagged event always has a flagged
+// neighbour in its own cycle group, so a flagged event has aux > 0.  One block
+// takes 256 consecutive events of one net; chunk_off[n] is the first block of
+// net n.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int group_pos(const uint32_t* ev, int64_t i, uint32_t cstart) {
+  // number of events before i in the same cycle group (event 0 is never grouped)
+  int j = 0;
+  for (int64_t k = i - 1; k >= 1 && ev_time(ev[k]) >= cstart; --k) ++j;
+  return j;
+}
+
+__global__ __launch_bounds__(256) void glitch_annotate_kernel(const int64_t* __restrict__ net_off,
+                                                              const uint32_t* __restrict__ ev,
+                                                              const int64_t* __restrict__ chunk_off,
+                                                              int64_t n_net, uint32_t T, float invT,
+                                                              uint32_t* __restrict__ aux) {
+  __shared__ int64_t s_net;
+  const int64_t chunk = blockIdx.x;
+  if (threadIdx.x == 0) {
+    int64_t lo = 0, hi = n_net;  // last n with chunk_off[n] <= chunk
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (chunk_off[mid] <= chunk) lo = mid; else hi = mid;
+    }
+    s_net = lo;
+  }
+  __syncthreads();
+  const int64_t n = s_net;
+  const int64_t b = net_off[n];
+  const int64_t cnt = net_off[n + 1] - b;
+  const int64_t i = (chunk - chunk_off[n]) * 256 + threadIdx.x;
+  if (i >= cnt) return;
+  const uint32_t* e = ev + b;
+  uint32_t out = 0;
+  if (i >= 1) {
+    const uint32_t t = ev_time(e[i]);
+    const uint32_t c = cycle_of(t, T, invT);
+    const uint32_t cs = c * T, ce = cs + T;
+    const int j = group_pos(e, i, cs);
+    const bool has1 = i + 1 < cnt;
+    const uint32_t t1 = has1 ? ev_time(e[i + 1]) : 0;
+    const bool last = !has1 || t1 >= ce;
+    if (!last || (j & 1)) {
+      int64_t pw = -1;
+      if (i - 1 >= 1) {
+        bool fprev;
+        if (j > 0) {
+          fprev = true;  // same group and not its last event
+        } else {
+          const uint32_t tp = ev_time(e[i - 1]);
+          const uint32_t cps = cycle_of(tp, T, invT) * T;
+          fprev = group_pos(e, i - 1, cps) & 1;  // last of its group: flagged iff at an odd position
+        }
+        if (fprev) pw = t - ev_time(e[i - 1]);
+      }
+      if (has1) {
+        bool fnext;
+        const bool has2 = i + 2 < cnt;
+        const uint32_t t2 = has2 ? ev_time(e[i + 2]) : 0;
+        if (!last) {
+          const bool last1 = !has2 || t2 >= ce;
+          fnext = !last1 || ((j + 1) & 1);
+        } else {
+          const uint32_t c1e = (cycle_of(t1, T, invT) + 1) * T;
+          fnext = has2 && t2 < c1e;
+        }
+        if (fnext) pw = max(pw, (int64_t)(t1 - t));
+      }
+      out = pw > 0 ? (uint32_t)pw : 0u;
+    }
+  }
+  aux[b + i] = out;
+}
+
+// ---------------------------------------------------------------------------
+// Fused power kernel specialised for one single-output cell type with NP pins
+// (NP - 1 inputs, output = pin NP - 1) whose internal power has no `when`
+// condition.  Every workgroup takes gates of one cell type, so the pin loops
+// are compile-time, the per-gate energies and arc delays live in registers and
+// the cell's leakage table in LDS.  Glitch flags come from the annotation
+// pass.  Per-cycle sums go to BIN_COPIES lane-interleaved copies of the LDS
+// bins so that lanes working on the same cycle do not serialise on one address.
+// Semantics are those of power_tile_kernel.
+// ---------------------------------------------------------------------------
+template <int NP>
+__global__ __launch_bounds__(256) void power_cell_kernel(PowerArgs a, int32_t range_begin,
+                                                          int32_t range_count, int32_t cell) {
+  constexpr int NIN = NP - 1;
+  constexpr int OUT = NP - 1;
+  constexpr int NS = 6 * NIN + 2;
+  constexpr int NSTATE = 1 << NP;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ float s_leak[NSTATE];
+  __shared__ long long s_leakq[NSTATE];
+  __shared__ long long s_base;
+  const int TC = a.tile_cycles;
+  const int BW = N_BIN * TC;  // one copy of the bins
+  long long* bins = reinterpret_cast<long long*>(smem_raw);
+
+  const int64_t n_work = (int64_t)a.n_tile * range_count;
+  int64_t w = blockIdx.x;
+  if (a.xcd_remap) {
+    const int64_t per = (n_work + 7) / 8;
+    w = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  }
+  if (w >= n_work) return;
+  const int tile = (int)(w / range_count);
+  const int range = range_begin + (int)(w - (int64_t)tile * range_count);
+
+  const CellDev cd = a.cells[cell];
+  const uint32_t clk = cd.clock_mask;
+  for (int i = threadIdx.x; i < BW * BIN_COPIES; i += blockDim.x) bins[i] = 0;
+  for (int i = threadIdx.x; i < NSTATE; i += blockDim.x) {
+    s_leak[i] = a.leak[cd.leak_off + i];
+    s_leakq[i] = a.leak_q[cd.leak_off + i];
+  }
+  if (threadIdx.x == 0) s_base = 0;
+  __syncthreads();
+  long long* mybins = bins + (threadIdx.x % BIN_COPIES) * BW;
+
+  const uint32_t T = (uint32_t)a.T;
+  const float invT = a.inv_T;
+  const int64_t tc0 = (int64_t)tile * TC;
+  const int tcn = (int)min((int64_t)TC, a.n_cycle - tc0);
+  const bool last_tile = tile == a.n_tile - 1;
+  const uint32_t tstart = (uint32_t)(tc0 * (int64_t)T);
+  const uint32_t tend = last_tile ? EV_TIME_INF : (uint32_t)((tc0 + TC) * (int64_t)T);
+  const uint32_t lend = last_tile ? (uint32_t)a.t_max : tend;
+  const uint32_t t_max = (uint32_t)a.t_max;
+  const int64_t tile_stride = (int64_t)a.n_tile + 1;
+  const float inv_qe = a.inv_q_energy, inv_ql = a.inv_q_leak;
+  const int64_t aux_delta = a.aux - a.ev;
+
+  long long base_acc = 0;
+  const int64_t gi_begin = (int64_t)range * a.range_gates;
+  const int64_t gi_end = min(a.n_sched, gi_begin + a.range_gates);
+  for (int64_t gi = gi_begin + threadIdx.x; gi < gi_end; gi += blockDim.x) {
+    const int g = a.order[gi];
+    if (g < 0) continue;
+    const int64_t pb = a.gate_pin_off[g];
+    const int64_t eb = a.gate_energy_off[g];
+    const int64_t ab = a.gate_arc_off[g];
+    // per-gate energies and arc delays, kept in registers: the empty asm
+    // statements stop the compiler from turning the selections below into a
+    // dynamically indexed (scratch) array
+    float E[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      E[k] = a.energy[eb + k];
+      asm volatile("" : "+v"(E[k]));
+    }
+    int32_t D[4 * NIN];
+#pragma unroll
+    for (int k = 0; k < 4 * NIN; ++k) {
+      D[k] = a.arc_q[ab + k];
+      asm volatile("" : "+v"(D[k]));
+    }
+
+    const uint32_t* cur[NP];   // next unconsumed event
+    const uint32_t* endp[NP];
+    uint32_t w0[NP], w1[NP];   // words of the next two events
+    uint32_t x0[NP], x1[NP];   // their glitch pulse widths
+    uint32_t lt[NP];           // time of the last consumed event (for the related-pin search)
+    int cv[NP], rfl[NP];       // current value; edge of the last consumed event (0 rise, 1 fall)
+    bool first[NP];            // no event consumed yet and event 0 is next
+    float gk[NP];
+    float csw = 0.f;
+    uint32_t sb = 0;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      cur[p] = a.ev;
+      endp[p] = a.ev;
+      w0[p] = w1[p] = EV_INF;
+      x0[p] = x1[p] = 0;
+      lt[p] = 0;
+      cv[p] = 2;
+      rfl[p] = 1;
+      first[p] = false;
+      const int32_t net = a.pin_net[pb + p];
+      gk[p] = a.pin_gk[pb + p] * a.time_scale;
+      if (p == OUT) csw = a.pin_csw[pb + p];
+      int64_t nb = 0, cn = 0;
+      if (net >= 0) {
+        nb = a.net_off[net];
+        cn = a.net_off[net + 1] - nb;
+      }
+      if (cn == 0) {
+        cv[p] = a.pin_default[pb + p];
+      } else {
+        const uint32_t* evp = a.ev + nb;
+        endp[p] = evp + cn;
+        const int32_t i0 = a.tile_idx[(int64_t)net * tile_stride + tile];
+        int32_t i = i0;
+        if (i0 == 0) {
+          const uint32_t e0 = evp[0];
+          if (tile == 0 && ev_time(e0) == 0) {
+            cv[p] = ev_val(e0);  // time-0 event is the initial state (TimeBasedPower.cc:782-787)
+            i = 1;
+          } else {
+            cv[p] = 2;
+            lt[p] = ev_time(e0);
+            first[p] = true;
+          }
+        } else {
+          const uint32_t el = evp[i0 - 1];
+          cv[p] = ev_val(el);
+          lt[p] = ev_time(el);
+          if (i0 - 1 >= 1) rfl[p] = rise_x2(ev_val(evp[i0 - 2]), cv[p]) > 0 ? 0 : 1;
+        }
+        cur[p] = evp + i;
+        if (cur[p] < endp[p]) {
+          w0[p] = cur[p][0];
+          x0[p] = cur[p][aux_delta];
+        }
+        if (cur[p] + 1 < endp[p]) {
+          w1[p] = cur[p][1];
+          x1[p] = cur[p][aux_delta + 1];
+        }
+      }
+      if (cv[p] == 1) sb |= 1u << p;
+    }
+    base_acc += s_leakq[sb];
+
+    long long acc[N_PG];     // per-gate sums in quanta (leakage: quanta * ticks)
+#pragma unroll
+    for (int k = 0; k < N_PG; ++k) acc[k] = 0;
+    uint32_t tog = 0, glc = 0;
+    int pc = -1;
+    long long pq[N_BIN];     // this gate's sums for the pending cycle pc
+#pragma unroll
+    for (int k = 0; k < N_BIN; ++k) pq[k] = 0;
+    uint32_t prev_t = tstart;
+
+    auto flush = [&]() {
+      if (pc >= 0 && pc < tcn) {
+#pragma unroll
+        for (int k = 0; k < N_BIN; ++k)
+          if (pq[k] != 0) atomicAdd((unsigned long long*)&mybins[k * TC + pc], (unsigned long long)pq[k]);
+      }
+      acc[PG_INT] += pq[BIN_INT];
+      acc[PG_GINT] += pq[BIN_GINT];
+      acc[PG_SW] += pq[BIN_SW];
+      acc[PG_GSW] += pq[BIN_GSW];
+#pragma unroll
+      for (int k = 0; k < N_BIN; ++k) pq[k] = 0;
+    };
+
+    for (;;) {
+      uint32_t wm = w0[0];
+#pragma unroll
+      for (int p = 1; p < NP; ++p) wm = min(wm, w0[p]);
+      const uint32_t t = ev_time(wm);
+      if (t >= tend) break;
+      const uint32_t c = cycle_of(t, T, invT);
+      const uint32_t cstart = c * T, cend = cstart + T;
+      const int cr = (int)((int64_t)c - tc0);
+      if (cr != pc) {
+        flush();
+        pc = cr;
+      }
+      bool trig[NP];
+      uint32_t sa = sb;
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        trig[p] = ev_time(w0[p]) == t;
+        if (trig[p]) sa = (sa & ~(1u << p)) | ((ev_val(w0[p]) == 1 ? 1u : 0u) << p);
+      }
+      // leakage of the previous state over [prev_t, t), exact in quanta * ticks
+      acc[PG_LEAK] += (long long)(int)s_leakq[sb] * (long long)(t - prev_t);
+      if (sa != sb) {
+        pq[BIN_LD] += s_leakq[sa] - s_leakq[sb];
+        pq[BIN_LW] += qterm((s_leak[sa] - s_leak[sb]) * ((float)(min(cend, t_max) - t) * invT), inv_ql);
+      }
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        if (!trig[p]) continue;
+        const int pv = cv[p], nv = ev_val(w0[p]);
+        const int tx2 = toggle_x2(pv, nv);
+        const int r2 = rise_x2(pv, nv);
+        const int rf = r2 > 0 ? 0 : 1;
+        tog += (uint32_t)tx2;
+        float ratio = -1.f;
+        if (!((clk >> p) & 1u)) {
+          const uint32_t pw = x0[p];
+          if (pw) {
+            const float r = (float)pw * gk[p];
+            ratio = fminf(1.f, r * r);
+          }
+        }
+        if (ratio > 0.f) ++glc;
+        float En;
+        if (p < NIN) {
+          En = rf ? E[2 * p + 1] : E[2 * p];
+        } else {
+          // related input: latest event before t in the same cycle whose
+          // arc-delayed arrival is closest to t (TimeBasedPower.cc:412-473)
+          int32_t best = INT32_MAX;  // every same-cycle candidate is closer than t_max
+          En = rf ? E[6 * NIN + 1] : E[6 * NIN];  // default when none qualifies
+#pragma unroll
+          for (int q = 0; q < NIN; ++q) {
+            const bool hi_rf = rfl[q] != 0, lo_rf = rf != 0;
+            const int32_t d01 = lo_rf ? D[q * 4 + 1] : D[q * 4 + 0];
+            const int32_t d23 = lo_rf ? D[q * 4 + 3] : D[q * 4 + 2];
+            const int32_t d = hi_rf ? d23 : d01;
+            const float e01 = lo_rf ? E[2 * NIN + q * 4 + 1] : E[2 * NIN + q * 4 + 0];
+            const float e23 = lo_rf ? E[2 * NIN + q * 4 + 3] : E[2 * NIN + q * 4 + 2];
+            const float eq = hi_rf ? e23 : e01;
+            const bool ok = endp[q] != a.ev && d != ARC_NONE && lt[q] >= cstart && lt[q] < cend;
+            // within one cycle t - lt < T < 2^15, so the grid difference fits 32 bits
+            int32_t diff = (int32_t)((t - lt[q]) << ARC_FRAC_BITS) - d;
+            diff = diff < 0 ? -diff : diff;
+            if (ok && diff < best) {
+              best = diff;
+              En = eq;
+            }
+          }
+          if (r2 > 0) {
+            const float es = 0.5f * (float)r2 * csw;
+            if (ratio > 0.f) pq[BIN_GSW] += qterm(ratio * es, inv_qe);
+            else pq[BIN_SW] += qterm(es, inv_qe);
+          }
+        }
+        const float e = En * (0.5f * (float)tx2);
+        if (ratio > 0.f) pq[BIN_GINT] += qterm(ratio * e, inv_qe);
+        else pq[BIN_INT] += qterm(e, inv_qe);
+      }
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        if (!trig[p]) continue;
+        const int nv = ev_val(w0[p]);
+        rfl[p] = first[p] ? 1 : (rise_x2(cv[p], nv) > 0 ? 0 : 1);
+        first[p] = false;
+        lt[p] = t;
+        cv[p] = nv;
+        ++cur[p];
+        w0[p] = w1[p];
+        x0[p] = x1[p];
+        if (cur[p] + 1 < endp[p]) {
+          w1[p] = cur[p][1];
+          x1[p] = cur[p][aux_delta + 1];
+        } else {
+          w1[p] = EV_INF;
+        }
+      }
+      sb = sa;
+      prev_t = t;
+    }
+    if (lend > prev_t) acc[PG_LEAK] += (long long)(int)s_leakq[sb] * (long long)(lend - prev_t);
+    flush();
+    const int64_t slot = (int64_t)tile * a.n_sched + gi;
+    const int64_t plane = (int64_t)a.n_tile * a.n_sched;
+#pragma unroll
+    for (int k = 0; k < N_PG; ++k) a.pg[k * plane + slot] = acc[k];
+    a.pg_tog[slot] = tog;
+    a.pg_glitch[slot] = glc;
+  }
+  for (int off = 32; off > 0; off >>= 1) base_acc += __shfl_xor(base_acc, off);
+  if ((threadIdx.x & 63) == 0) atomicAdd((unsigned long long*)&s_base, (unsigned long long)base_acc);
+  __syncthreads();
+  // fold the bin copies into copy 0, then S_c = base + earlier leakage steps
+  for (int i = threadIdx.x; i < BW; i += blockDim.x) {
+    long long v = bins[i];
+    for (int k = 1; k < BIN_COPIES; ++k) v += bins[k * BW + i];
+    bins[i] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long run = s_base;
+    for (int ci = 0; ci < TC; ++ci) {
+      const long long d = bins[BIN_LD * TC + ci];
+      bins[BIN_LD * TC + ci] = run;
+      run += d;
+    }
+  }
+  __syncthreads();
+  long long* out = a.cyc_part + ((int64_t)tile * a.n_range + range) * N_BIN * TC;
+  for (int i = threadIdx.x; i < BW; i += blockDim.x) out[i] = bins[i];
 }
 
 template __global__ void power_tile_kernel<MAXP_SMALL>(PowerArgs, int32_t, int32_t);
@@ -553,11 +944,11 @@ __global__ void cycle_reduce_kernel(const long long* __restrict__ part, int32_t 
   out[4 * n_cycle + c] = (double)s[BIN_GSW] * q_energy * inv_period;
 }
 
-__global__ void gate_reduce_kernel(const float* __restrict__ hi, const float* __restrict__ lo,
-                                   const uint32_t* __restrict__ tog, const uint32_t* __restrict__ glc,
-                                   const int32_t* __restrict__ order, int64_t n_sched, int32_t n_tile,
-                                   int64_t n_gate, double norm_energy, double norm_leak,
-                                   double* __restrict__ gate_power, unsigned long long* __restrict__ gate_tog,
+__global__ void gate_reduce_kernel(const long long* __restrict__ pg, const uint32_t* __restrict__ tog,
+                                   const uint32_t* __restrict__ glc, const int32_t* __restrict__ order,
+                                   int64_t n_sched, int32_t n_tile, int64_t n_gate, double norm_energy,
+                                   double norm_leak, double* __restrict__ gate_power,
+                                   unsigned long long* __restrict__ gate_tog,
                                    unsigned long long* __restrict__ gate_glitch) {
   const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gi >= n_sched) return;
@@ -566,13 +957,9 @@ __global__ void gate_reduce_kernel(const float* __restrict__ hi, const float* __
   const int64_t plane = (int64_t)n_tile * n_sched;
 #pragma unroll
   for (int k = 0; k < N_PG; ++k) {
-    float h = 0.f, l = 0.f;
-    for (int t = 0; t < n_tile; ++t) {
-      ff_add(h, l, hi[k * plane + (int64_t)t * n_sched + gi]);
-      l += lo[k * plane + (int64_t)t * n_sched + gi];
-    }
-    const double v = (double)h + (double)l;
-    gate_power[(int64_t)k * n_gate + g] = v * (k == PG_LEAK ? norm_leak : norm_energy);
+    long long v = 0;
+    for (int t = 0; t < n_tile; ++t) v += pg[k * plane + (int64_t)t * n_sched + gi];
+    gate_power[(int64_t)k * n_gate + g] = (double)v * (k == PG_LEAK ? norm_leak : norm_energy);
   }
   unsigned long long tg = 0, gg = 0;
   for (int t = 0; t < n_tile; ++t) {
@@ -583,28 +970,41 @@ __global__ void gate_reduce_kernel(const float* __restrict__ hi, const float* __
   gate_glitch[g] = gg;
 }
 
-// FP64 design and class totals, one workgroup, fixed summation order.
-__global__ void total_kernel(const double* __restrict__ gate_power, const int32_t* __restrict__ gate_class,
-                             int64_t n_gate, double* __restrict__ out /* [5] + [5 classes][5] */) {
+// FP64 design and class totals: per-block partials, then one fixed-order pass.
+__global__ void total_partial_kernel(const double* __restrict__ gate_power, const int32_t* __restrict__ gate_class,
+                                     int64_t n_gate, double* __restrict__ part /* [blocks][30] */) {
   __shared__ double s[256];
-  for (int k = 0; k < 5; ++k) {
-    for (int cls = -1; cls < 5; ++cls) {
-      double v = 0.0;
-      for (int64_t g = threadIdx.x; g < n_gate; g += blockDim.x)
-        if (cls < 0 || gate_class[g] == cls) v += gate_power[(int64_t)k * n_gate + g];
-      s[threadIdx.x] = v;
-      __syncthreads();
-      for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
-        __syncthreads();
-      }
-      if (threadIdx.x == 0) {
-        if (cls < 0) out[k] = s[0];
-        else out[5 + cls * 5 + k] = s[0];
-      }
-      __syncthreads();
+  double v[30];
+#pragma unroll
+  for (int k = 0; k < 30; ++k) v[k] = 0.0;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n_gate; g += (int64_t)gridDim.x * blockDim.x) {
+    const int cls = gate_class[g];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const double x = gate_power[(int64_t)k * n_gate + g];
+      v[k] += x;
+#pragma unroll
+      for (int c = 0; c < 5; ++c)
+        if (cls == c) v[5 + c * 5 + k] += x;
     }
   }
+  for (int k = 0; k < 30; ++k) {
+    s[threadIdx.x] = v[k];
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x * 30 + k] = s[0];
+    __syncthreads();
+  }
+}
+
+__global__ void total_final_kernel(const double* __restrict__ part, int n_part, double* __restrict__ out) {
+  if (threadIdx.x >= 30) return;
+  double v = 0.0;
+  for (int b = 0; b < n_part; ++b) v += part[b * 30 + threadIdx.x];
+  out[threadIdx.x] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -652,20 +1052,43 @@ hipError_t launch_cycle_reduce(const long long* part, int32_t n_range, int32_t t
   return hipGetLastError();
 }
 
-hipError_t launch_gate_reduce(const float* hi, const float* lo, const uint32_t* tog, const uint32_t* glc,
+hipError_t launch_gate_reduce(const long long* pg, const uint32_t* tog, const uint32_t* glc,
                               const int32_t* order, int64_t n_sched, int32_t n_tile, int64_t n_gate,
                               double norm_energy, double norm_leak, double* gate_power,
                               unsigned long long* gate_tog, unsigned long long* gate_glitch, hipStream_t s) {
   if (n_sched <= 0) return hipSuccess;
   gate_reduce_kernel<<<dim3((unsigned)((n_sched + 255) / 256)), dim3(256), 0, s>>>(
-      hi, lo, tog, glc, order, n_sched, n_tile, n_gate, norm_energy, norm_leak, gate_power, gate_tog,
+      pg, tog, glc, order, n_sched, n_tile, n_gate, norm_energy, norm_leak, gate_power, gate_tog,
       gate_glitch);
   return hipGetLastError();
 }
 
-hipError_t launch_total(const double* gate_power, const int32_t* gate_class, int64_t n_gate, double* out,
-                        hipStream_t s) {
-  total_kernel<<<dim3(1), dim3(256), 0, s>>>(gate_power, gate_class, n_gate, out);
+hipError_t launch_total(const double* gate_power, const int32_t* gate_class, int64_t n_gate, double* part,
+                        double* out, hipStream_t s) {
+  total_partial_kernel<<<dim3(TOTAL_BLOCKS), dim3(256), 0, s>>>(gate_power, gate_class, n_gate, part);
+  total_final_kernel<<<dim3(1), dim3(64), 0, s>>>(part, TOTAL_BLOCKS, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_glitch_annotate(const int64_t* net_off, const uint32_t* ev, const int64_t* chunk_off,
+                                  int64_t n_net, int64_t n_chunk, uint32_t T, float invT, uint32_t* aux,
+                                  hipStream_t s) {
+  if (n_chunk <= 0) return hipSuccess;
+  glitch_annotate_kernel<<<dim3((unsigned)n_chunk), dim3(256), 0, s>>>(net_off, ev, chunk_off, n_net, T, invT, aux);
+  return hipGetLastError();
+}
+
+hipError_t launch_power_cell(int np, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,
+                             int32_t range_begin, int32_t range_count, int32_t cell) {
+  if (grid <= 0) return hipSuccess;
+  switch (np) {
+    case 2: power_cell_kernel<2><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count, cell); break;
+    case 3: power_cell_kernel<3><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count, cell); break;
+    case 4: power_cell_kernel<4><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count, cell); break;
+    case 5: power_cell_kernel<5><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count, cell); break;
+    case 6: power_cell_kernel<6><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count, cell); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -10,7 +10,7 @@
 //     overlap / T, which equals the reference's float-second split when
 //     period / time_scale is an integer;
 //   * the related-pin arrival comparison |t_out - d - t_in| is done exactly, on
-//     ticks scaled by 2^20 with the arc delay rounded to that grid, so that the
+//     ticks scaled by 2^16 with the arc delay rounded to that grid, so that the
 //     engine can reproduce the same choice bit for bit.
 //   * a per-cycle contribution whose cycle index is >= n_cycle (an event at
 //     t_max when t_max is a multiple of T) is dropped instead of written one
@@ -28,7 +28,7 @@
 namespace {
 
 constexpr int RISE = 0, FALL = 1;
-constexpr int ARC_FRAC_BITS = 20;
+constexpr int ARC_FRAC_BITS = 16;
 
 // power/PowerUtils.cc:246-256 getNToggle, times two.
 inline int toggle_x2(int prev, int cur) {
