@@ -106,3 +106,77 @@ def test_trace_end_on_cycle_boundary_and_partial_cycle():
     res, ref = run_both(d, tr2, tile_cycles=16)
     compare(res, ref, "partial last cycle")
     assert res.n_cycle == 50
+
+
+# ---------------------------------------------------------------------------
+# hand-built edge cases
+# ---------------------------------------------------------------------------
+def _mux_design(lib, nets, n_net):
+    """Two gates: a mux2_1 and an a21oi_1 on the given pin nets."""
+    cm, ca = lib.index("mux2_1"), lib.index("a21oi_1")
+    P = 8
+    slew = np.linspace(30e-12, 90e-12, 2 * P)
+    cap = np.array([0, 0, 0, 6e-15, 0, 0, 0, 3e-15])
+    arcs = np.full(2 * 12, np.nan)
+    rng = np.random.default_rng(0)
+    for g, (cell, nin) in enumerate(((cm, 3), (ca, 3))):
+        for i in range(nin):
+            for fr in range(2):
+                for to in range(2):
+                    if lib.cells[cell]["arcs"].get((0, i, fr, to)):
+                        arcs[g * 12 + (i * 2 + fr) * 2 + to] = rng.uniform(20e-12, 80e-12)
+    return S.ArrayDesign(lib, [cm, ca], nets, slew, cap, arcs, n_net)
+
+
+@pytest.mark.parametrize("case", ["x_and_z", "late_first_event", "shared_net", "unconnected", "coincident"])
+def test_hand_cases(library, case):
+    print(f"[parity] hand case {case}", flush=True)
+    lib = library
+    T = 1000
+    if case == "x_and_z":
+        waves = [[(0, 2), (150, 0), (900, 1), (1300, 3), (2100, 2), (2200, 0)],
+                 [(0, 1), (400, 0), (420, 1), (2500, 2)],
+                 [(0, 0), (700, 1), (710, 0), (730, 1), (3300, 0)],
+                 [(0, 2), (190, 0), (950, 1), (1350, 2), (2250, 0)],
+                 [(0, 0), (2700, 1)]]
+        nets = [0, 1, 2, 3, 1, 2, 4, 4]
+    elif case == "late_first_event":
+        waves = [[(300, 1), (1500, 0)], [(0, 1)], [(1200, 1), (1250, 0), (1260, 1)], [(350, 0), (1600, 1)],
+                 [(2000, 1)]]
+        nets = [0, 1, 2, 3, 0, 2, 1, 4]
+    elif case == "shared_net":
+        waves = [[(0, 0), (100, 1), (180, 0), (1100, 1)], [(0, 1), (300, 0)], [(0, 0), (130, 1), (200, 0)],
+                 [(0, 1), (150, 0), (1150, 1)], []]
+        nets = [0, 0, 1, 2, 0, 0, 0, 3]
+    elif case == "unconnected":
+        waves = [[(0, 0), (100, 1), (1100, 0)], [(0, 1), (130, 0), (1160, 1)], [], [], []]
+        nets = [0, -1, -1, 1, 0, -1, 4, 4]
+    else:  # coincident events on several pins, in and out at one time
+        waves = [[(0, 0), (500, 1), (1500, 0)], [(0, 1), (500, 0), (1500, 1)], [(0, 0), (500, 1), (2500, 0)],
+                 [(0, 1), (500, 0), (540, 1), (1500, 0)], [(0, 0), (540, 1)]]
+        nets = [0, 1, 2, 3, 2, 1, 0, 4]
+    d = _mux_design(lib, nets, len(waves))
+    tr = S.trace_from_lists(waves, 4000, 1e-12, T * 1e-12)
+    res, ref = run_both(d, tr, tile_cycles=2)
+    compare(res, ref, case)
+
+
+def test_when_conditioned_library_uses_the_generic_kernel(library):
+    """A library whose nand2 input power depends on the state (a `when`
+    condition) is evaluated by the generic kernel with state-indexed energies."""
+    print("[parity] when-conditioned library", flush=True)
+    from gtpower_amd.library import CellLibrary
+    lib = CellLibrary()
+    c = lib.index("nand2_1")
+    ipa = lib.ipower_array
+    # nand2 pin A: power only when B is 1 (states 2, 3, 6, 7)
+    k = [i for i in range(len(ipa)) if ipa[i]["cell"] == c and ipa[i]["pin"] == 0][0]
+    ipa[k]["when_truth"] = sum(1 << s for s in (2, 3, 6, 7))
+    # nand2 output, related A: only when Y was 1 before (states 4..7)
+    k2 = [i for i in range(len(ipa)) if ipa[i]["cell"] == c and ipa[i]["pin"] == 2 and ipa[i]["related_pin"] == 0][0]
+    ipa[k2]["when_truth"] = sum(1 << s for s in range(4, 8))
+    lib.ipowers = [tuple(r) for r in ipa[["cell", "pin", "related_pin", "rise_table", "fall_table", "when_truth"]]]
+    d = S.make_design(400, dff_fraction=0.1, clock_ns=2.0, seed=41, library=lib)
+    tr = d.trace(300, toggle_p=0.3, seed=42)
+    res, ref = run_both(d, tr)
+    compare(res, ref, "when")
