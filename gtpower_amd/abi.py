@@ -87,6 +87,16 @@ class GtpScheduleInfo(C.Structure):
                 ("autotune_ms", C.c_double)]
 
 
+class GtpMlpConfig(C.Structure):
+    _fields_ = [("device", C.c_int32), ("n_in", C.c_int32), ("n_hidden", C.c_int32),
+                ("pad_", C.c_int32), ("seed", C.c_uint64)]
+
+
+class GtpTrainConfig(C.Structure):
+    _fields_ = [("batch", C.c_int32), ("epochs", C.c_int32), ("lr", C.c_float),
+                ("pad_", C.c_int32), ("seed", C.c_uint64)]
+
+
 class GtpError(RuntimeError):
     def __init__(self, status, where, text=""):
         self.status = status
@@ -132,9 +142,31 @@ def load_engine():
                                                     C.c_int64, p_i64]
     lib.gtp_write_cell_power.argtypes = [C.c_char_p, C.c_int64, p_f64, C.c_char_p, p_i32, C.c_char_p]
     lib.gtp_write_per_cycle.argtypes = [C.c_char_p, C.c_char_p, C.c_int64, p_f64]
+    lib.gtp_run_tiles.argtypes = [C.c_void_p, C.c_int32, C.c_int32, p_i32, p_f64, p_f64]
+    p_f32 = C.POINTER(C.c_float)
+    lib.gtp_mlp_create.argtypes = [C.POINTER(GtpMlpConfig), C.POINTER(C.c_void_p)]
+    lib.gtp_mlp_destroy.argtypes = [C.c_void_p]
+    lib.gtp_mlp_destroy.restype = None
+    lib.gtp_mlp_last_error.argtypes = [C.c_void_p]
+    lib.gtp_mlp_last_error.restype = C.c_char_p
+    lib.gtp_mlp_param_count.argtypes = [C.c_void_p]
+    lib.gtp_mlp_param_count.restype = C.c_int64
+    lib.gtp_mlp_get_params.argtypes = [C.c_void_p, p_f32, p_f32]
+    lib.gtp_mlp_set_params.argtypes = [C.c_void_p, p_f32, p_f32]
+    lib.gtp_mlp_gradients.argtypes = [C.c_void_p, p_u32, p_f32, C.c_int32, p_f32, p_f32]
+    lib.gtp_mlp_train.argtypes = [C.c_void_p, p_u32, p_f32, C.c_int64, C.POINTER(GtpTrainConfig), p_f32]
+    lib.gtp_mlp_predict.argtypes = [C.c_void_p, p_u32, C.c_int64, p_f32]
+    lib.gtp_mlp_predict_trace.argtypes = [C.c_void_p, C.c_void_p, p_i32, C.c_int64, C.c_int64, p_f32, p_f64]
+    lib.gtp_feature_correlation.argtypes = [C.c_void_p, p_f64, p_u8, p_f32]
+    lib.gtp_toggle_features.argtypes = [C.c_void_p, C.c_int32, p_i32, C.c_int64, C.c_int64, p_u32]
+    lib.gtp_mlp_gemm_test.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                      C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, p_f32]
     for name in ("gtp_create", "gtp_set_design", "gtp_set_trace", "gtp_run", "gtp_run_batch",
                  "gtp_get_schedule", "gtp_reference_cycles_per_thread", "gtp_write_cell_power",
-                 "gtp_write_per_cycle"):
+                 "gtp_write_per_cycle", "gtp_run_tiles", "gtp_mlp_create", "gtp_mlp_get_params",
+                 "gtp_mlp_set_params", "gtp_mlp_gradients", "gtp_mlp_train", "gtp_mlp_predict",
+                 "gtp_mlp_predict_trace", "gtp_feature_correlation", "gtp_toggle_features",
+                 "gtp_mlp_gemm_test"):
         getattr(lib, name).restype = C.c_int32
     _ENGINE = lib
     return lib

@@ -143,6 +143,9 @@ struct gtp_ctx {
   int64_t n_chunk = 0;
   DArr<int32_t> d_tile_idx;
   int32_t tile_idx_tc = 0;
+  std::vector<int64_t> h_net_off;  // host copy of the event offsets
+  bool aux_valid = false;          // d_aux holds the annotation of this trace
+  DArr<int32_t> d_tile_list;
 
   // schedule: ranges of range_gates gates; a range holds gates of one
   // specialised cell type, or generic gates
@@ -587,6 +590,8 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
   x->E = E;
   hipStream_t s = x->stream;
   HIPCHK(x, x->d_net_off.upload(off, N + 1, s));
+  x->h_net_off.assign(off, off + N + 1);
+  x->aux_valid = false;
   HIPCHK(x, x->d_ev.alloc(std::max<int64_t>(E, 1)));
   // chunked upload + pack into the 32-bit event store
   {
@@ -695,31 +700,30 @@ void build_order(gtp_ctx* x, int32_t range_gates) {
 
 }  // namespace
 
-GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
-  if (!x) return GTP_ERR_ARG;
-  if (!x->have_trace) return x->fail(GTP_ERR_STATE, "gtp_run: call gtp_set_trace first");
-  if (!res) return x->fail(GTP_ERR_ARG, "gtp_run: null result");
-  HIPCHK(x, hipSetDevice(x->device));
-  hipStream_t s = x->stream;
-  hipEvent_t ev0, ev1, ev2;
-  HIPCHK(x, hipEventCreate(&ev0));
-  HIPCHK(x, hipEventCreate(&ev1));
-  HIPCHK(x, hipEventCreate(&ev2));
-  struct EvGuard {
-    hipEvent_t a, b, c;
-    ~EvGuard() {
-      (void)hipEventDestroy(a);
-      (void)hipEventDestroy(b);
-      (void)hipEventDestroy(c);
-    }
-  } evg{ev0, ev1, ev2};
-  HIPCHK(x, hipEventRecord(ev0, s));
+namespace {
 
-  // --- schedule parameters
-  int64_t TC = x->opt.tile_cycles > 0 ? x->opt.tile_cycles : 128;
+// Everything a power evaluation needs besides the launches themselves.
+struct RunPlan {
+  int64_t TC = 0;
+  int32_t n_tile = 0, RG = 0, n_range = 0;
+  int block = 256;
+  int64_t n_sched = 0;
+  double q_e = 0, q_l = 0;
+  size_t lds = 0, lds_cell = 0;
+  PowerArgs a{};
+};
+
+// Schedule parameters, order, tile index, fixed-point quanta and scratch.
+// tc_req > 0 asks for exactly that tile length.
+gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, RunPlan* rp) {
+  hipStream_t s = x->stream;
+  int64_t TC = tc_req > 0 ? tc_req : (x->opt.tile_cycles > 0 ? x->opt.tile_cycles : 128);
   if (x->T < (1 << 24)) TC = std::min<int64_t>(TC, std::max<int64_t>(1, ((1 << 24) - 1) / x->T));
   TC = std::max<int64_t>(1, std::min<int64_t>(TC, x->C));
   TC = std::min<int64_t>(TC, 4096);
+  if (tc_req > 0 && TC != tc_req)
+    return x->fail(GTP_ERR_ARG, "tile_cycles " + std::to_string(tc_req) + " outside what this trace allows (" +
+                                    std::to_string(TC) + ")");
   const int32_t n_tile = (int32_t)((x->C + TC - 1) / TC);
   // gates per range: enough workgroups to fill the GPU several times over
   int32_t RG = x->opt.gates_per_wg;
@@ -742,18 +746,16 @@ GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
     HIPCHK(x, launch_tile_index(x->d_net_off.p, x->d_ev.p, x->N, n_tile, TC * x->T, x->d_tile_idx.p, s));
     x->tile_idx_tc = (int32_t)TC;
   }
-  // --- fixed-point quanta: powers of two with one term <= 2^30 quanta, so a
-  // term converts through int32 and every sum is exact in int64 (a cycle
-  // would need 2^33 terms, a gate's leakage 2^33 ticks of trace, to overflow)
+  // fixed-point quanta: powers of two with one term <= 2^30 quanta, so a term
+  // converts through int32 and every sum is exact in int64 (a cycle would
+  // need 2^33 terms, a gate's leakage 2^33 ticks of trace, to overflow)
   double emax = std::max((double)x->energy_max * 1.000001, x->csw_max);
   if (!(emax > 0)) emax = 1e-30;
   const int ke = (int)std::floor(30.0 - std::log2(emax));
   const float inv_q_e = std::ldexp(1.0f, ke);
-  const double q_e = std::ldexp(1.0, -ke);
   const double lmax = x->leak_max > 0 ? x->leak_max * 1.000001 : 1e-30;
   const int kl = (int)std::floor(30.0 - std::log2(lmax));
   const float inv_q_l = std::ldexp(1.0f, kl);
-  const double q_l = std::ldexp(1.0, -kl);
   if (!std::isfinite(inv_q_e) || !std::isfinite(inv_q_l) || inv_q_e == 0.f || inv_q_l == 0.f)
     return x->fail(GTP_ERR_UNSUPPORTED, "gtp_run: energy or leakage magnitudes outside the FP32 range");
   if ((double)x->t_max * 2147483648.0 >= 9.2e18)
@@ -764,7 +766,7 @@ GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
     HIPCHK(x, hipMemcpyAsync(x->d_leak_q.p, lq.data(), lq.size() * sizeof(long long), hipMemcpyHostToDevice, s));
     HIPCHK(x, hipStreamSynchronize(s));
   }
-  // --- scratch
+  // scratch
   HIPCHK(x, x->d_pg.alloc((size_t)N_PG * n_tile * n_sched));
   HIPCHK(x, x->d_pg_tog.alloc((size_t)n_tile * n_sched));
   HIPCHK(x, x->d_pg_glc.alloc((size_t)n_tile * n_sched));
@@ -776,7 +778,8 @@ GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
   HIPCHK(x, x->d_gtog.alloc(x->G));
   HIPCHK(x, x->d_gglc.alloc(x->G));
 
-  PowerArgs a{};
+  PowerArgs& a = rp->a;
+  a = PowerArgs{};
   a.order = x->d_order.p;
   a.n_sched = n_sched;
   a.gate_cell = x->d_gate_cell.p;
@@ -803,6 +806,8 @@ GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
   a.n_range = n_range;
   a.range_gates = RG;
   a.xcd_remap = 1;
+  a.tile_list = nullptr;
+  a.n_work_tiles = n_tile;
   a.cells = x->d_cells.p;
   a.n_cell = (int32_t)x->cell_dev.size();
   a.leak = x->d_leak.p;
@@ -814,46 +819,87 @@ GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
   a.pg_tog = x->d_pg_tog.p;
   a.pg_glitch = x->d_pg_glc.p;
   a.cyc_part = x->d_cyc_part.p;
-  const size_t lds = (size_t)N_BIN * TC * 8 + (size_t)a.n_leak * 8 + (size_t)(a.n_leak + (a.n_leak & 1)) * 4 +
-                     (size_t)a.n_cell * sizeof(CellDev);
-  if (lds > 160 * 1024) return x->fail(GTP_ERR_LIMIT, "gtp_run: tile needs more LDS than a workgroup has");
-  const size_t lds_cell = (size_t)N_BIN * TC * 8 * BIN_COPIES;
-  if (lds_cell > 160 * 1024) return x->fail(GTP_ERR_LIMIT, "gtp_run: tile needs more LDS than a workgroup has");
-  HIPCHK(x, hipEventRecord(ev1, s));
-  HIPCHK(x, launch_glitch_annotate(x->d_net_off.p, x->d_ev.p, x->d_chunk_off.p, x->N, x->n_chunk,
-                                   (uint32_t)x->T, a.inv_T, x->d_aux.p, s));
-  // the launches of the cell groups write disjoint outputs: run them
-  // concurrently on the main and the side streams (all default priority)
+  rp->lds = (size_t)N_BIN * TC * 8 + (size_t)a.n_leak * 8 + (size_t)(a.n_leak + (a.n_leak & 1)) * 4 +
+            (size_t)a.n_cell * sizeof(CellDev);
+  if (rp->lds > 160 * 1024) return x->fail(GTP_ERR_LIMIT, "gtp_run: tile needs more LDS than a workgroup has");
+  rp->lds_cell = (size_t)N_BIN * TC * 8 * BIN_COPIES;
+  if (rp->lds_cell > 160 * 1024) return x->fail(GTP_ERR_LIMIT, "gtp_run: tile needs more LDS than a workgroup has");
+  rp->TC = TC;
+  rp->n_tile = n_tile;
+  rp->RG = RG;
+  rp->n_range = n_range;
+  rp->block = block;
+  rp->n_sched = n_sched;
+  rp->q_e = std::ldexp(1.0, -ke);
+  rp->q_l = std::ldexp(1.0, -kl);
+  return GTP_OK;
+}
+
+// The per-cell-group power launches over n_work_tiles tiles (rp.a), run
+// concurrently on the main and the side streams (all default priority): the
+// launches write disjoint outputs.
+gtp_status launch_groups(gtp_ctx* x, const RunPlan& rp) {
+  hipStream_t s = x->stream;
   HIPCHK(x, hipEventRecord(x->fork, s));
   for (int k = 0; k < gtp_ctx::N_SIDE; ++k) HIPCHK(x, hipStreamWaitEvent(x->side[k], x->fork, 0));
-  {
-    std::vector<size_t> idx(x->launches.size());
-    std::iota(idx.begin(), idx.end(), 0);
-    std::sort(idx.begin(), idx.end(), [&](size_t p, size_t q) {
-      return x->launches[p].range_count > x->launches[q].range_count;
-    });
-    int k = 0;
-    for (size_t li : idx) {
-      const gtp_ctx::Launch& L = x->launches[li];
-      hipStream_t ls = (k == 0) ? s : x->side[k - 1];
-      k = (k + 1) % (gtp_ctx::N_SIDE + 1);
-      const int64_t nw = (int64_t)n_tile * L.range_count;
-      if (L.cell >= 0)
-        HIPCHK(x, launch_power_cell(L.np, (nw + 7) / 8 * 8, block, lds_cell, ls, a, L.range_begin, L.range_count, L.cell));
-      else
-        HIPCHK(x, launch_power(L.np, (nw + 7) / 8 * 8, block, lds, ls, a, L.range_begin, L.range_count));
-    }
+  std::vector<size_t> idx(x->launches.size());
+  std::iota(idx.begin(), idx.end(), 0);
+  std::sort(idx.begin(), idx.end(), [&](size_t p, size_t q) {
+    return x->launches[p].range_count > x->launches[q].range_count;
+  });
+  int k = 0;
+  for (size_t li : idx) {
+    const gtp_ctx::Launch& L = x->launches[li];
+    hipStream_t ls = (k == 0) ? s : x->side[k - 1];
+    k = (k + 1) % (gtp_ctx::N_SIDE + 1);
+    const int64_t nw = (int64_t)rp.a.n_work_tiles * L.range_count;
+    if (L.cell >= 0)
+      HIPCHK(x, launch_power_cell(L.np, (nw + 7) / 8 * 8, rp.block, rp.lds_cell, ls, rp.a, L.range_begin,
+                                  L.range_count, L.cell));
+    else
+      HIPCHK(x, launch_power(L.np, (nw + 7) / 8 * 8, rp.block, rp.lds, ls, rp.a, L.range_begin, L.range_count));
   }
-  for (int k = 0; k < gtp_ctx::N_SIDE; ++k) {
-    HIPCHK(x, hipEventRecord(x->join[k], x->side[k]));
-    HIPCHK(x, hipStreamWaitEvent(s, x->join[k], 0));
+  for (int j = 0; j < gtp_ctx::N_SIDE; ++j) {
+    HIPCHK(x, hipEventRecord(x->join[j], x->side[j]));
+    HIPCHK(x, hipStreamWaitEvent(s, x->join[j], 0));
   }
-  HIPCHK(x, hipEventRecord(ev2, s));
-  HIPCHK(x, launch_cycle_reduce(x->d_cyc_part.p, n_range, (int32_t)TC, x->C, x->T, x->t_max, q_e, q_l,
+  return GTP_OK;
+}
+
+struct EvSet {
+  hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+  ~EvSet() {
+    for (hipEvent_t v : e)
+      if (v) (void)hipEventDestroy(v);
+  }
+};
+
+}  // namespace
+
+GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
+  if (!x) return GTP_ERR_ARG;
+  if (!x->have_trace) return x->fail(GTP_ERR_STATE, "gtp_run: call gtp_set_trace first");
+  if (!res) return x->fail(GTP_ERR_ARG, "gtp_run: null result");
+  HIPCHK(x, hipSetDevice(x->device));
+  hipStream_t s = x->stream;
+  EvSet ev;
+  for (hipEvent_t& e : ev.e) HIPCHK(x, hipEventCreate(&e));
+  HIPCHK(x, hipEventRecord(ev.e[0], s));
+  RunPlan rp;
+  gtp_status st = prepare_run(x, 0, &rp);
+  if (st != GTP_OK) return st;
+  const PowerArgs& a = rp.a;
+  HIPCHK(x, hipEventRecord(ev.e[1], s));
+  HIPCHK(x, launch_glitch_annotate(x->d_net_off.p, x->d_ev.p, x->d_chunk_off.p, x->N, x->n_chunk,
+                                   (uint32_t)x->T, a.inv_T, x->d_aux.p, s));
+  x->aux_valid = true;
+  if ((st = launch_groups(x, rp)) != GTP_OK) return st;
+  HIPCHK(x, hipEventRecord(ev.e[2], s));
+  HIPCHK(x, launch_cycle_reduce(x->d_cyc_part.p, rp.n_range, (int32_t)rp.TC, x->C, x->T, x->t_max, rp.q_e, rp.q_l,
                                 1.0 / x->period, x->d_cycle.p, s));
-  HIPCHK(x, launch_gate_reduce(x->d_pg.p, x->d_pg_tog.p, x->d_pg_glc.p, x->d_order.p, n_sched,
-                               n_tile, x->G, q_e / ((double)x->t_max * x->scale), q_l / (double)x->t_max,
-                               x->d_gate.p, x->d_gtog.p, x->d_gglc.p, s));
+  HIPCHK(x, launch_gate_reduce(x->d_pg.p, x->d_pg_tog.p, x->d_pg_glc.p, x->d_order.p, rp.n_sched, rp.n_tile, x->G,
+                               rp.q_e / ((double)x->t_max * x->scale), rp.q_l / (double)x->t_max, x->d_gate.p,
+                               x->d_gtog.p, x->d_gglc.p, s));
   HIPCHK(x, launch_total(x->d_gate.p, x->d_gate_class.p, x->G, x->d_total_part.p, x->d_total.p, s));
   if (res->cycle_power)
     HIPCHK(x, hipMemcpyAsync(res->cycle_power, x->d_cycle.p, sizeof(double) * GTP_N_WAVE * x->C, hipMemcpyDeviceToHost, s));
@@ -865,30 +911,84 @@ GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
     HIPCHK(x, hipMemcpyAsync(res->gate_glitch, x->d_gglc.p, sizeof(uint64_t) * x->G, hipMemcpyDeviceToHost, s));
   double tot[30];
   HIPCHK(x, hipMemcpyAsync(tot, x->d_total.p, sizeof(tot), hipMemcpyDeviceToHost, s));
-  hipEvent_t ev3;
-  HIPCHK(x, hipEventCreate(&ev3));
-  HIPCHK(x, hipEventRecord(ev3, s));
+  HIPCHK(x, hipEventRecord(ev.e[3], s));
   HIPCHK(x, hipStreamSynchronize(s));
   float k_ms = 0, t_ms = 0;
-  HIPCHK(x, hipEventElapsedTime(&k_ms, ev1, ev2));
-  HIPCHK(x, hipEventElapsedTime(&t_ms, ev0, ev3));
-  (void)hipEventDestroy(ev3);
+  HIPCHK(x, hipEventElapsedTime(&k_ms, ev.e[1], ev.e[2]));
+  HIPCHK(x, hipEventElapsedTime(&t_ms, ev.e[0], ev.e[3]));
   if (res->total) std::memcpy(res->total, tot, sizeof(double) * 5);
   if (res->class_total) std::memcpy(res->class_total, tot + 5, sizeof(double) * 25);
   res->n_cycle = x->C;
   res->kernel_ms = k_ms;
   res->total_ms = t_ms;
   gtp_schedule_info& si = x->sched;
-  si.tile_cycles = (int32_t)TC;
-  si.block_lanes = block;
-  si.gates_per_wg = RG;
-  si.n_unit = (int64_t)n_tile * x->G;
-  si.n_workgroup = (int64_t)n_tile * n_range;
+  si.tile_cycles = (int32_t)rp.TC;
+  si.block_lanes = rp.block;
+  si.gates_per_wg = rp.RG;
+  si.n_unit = (int64_t)rp.n_tile * x->G;
+  si.n_workgroup = (int64_t)rp.n_tile * rp.n_range;
   si.split_target = (int32_t)x->launches.size();
-  si.n_tile = n_tile;
+  si.n_tile = rp.n_tile;
   si.n_event = x->E;
   si.n_pin_event = x->P_ev;
   si.autotune_ms = 0;
+  return GTP_OK;
+}
+
+GTP_API gtp_status gtp_run_tiles(gtp_ctx* x, int32_t tile_cycles, int32_t n_list, const int32_t* tiles,
+                                 double* cycle_power, double* kernel_ms) {
+  if (!x) return GTP_ERR_ARG;
+  if (!x->have_trace) return x->fail(GTP_ERR_STATE, "gtp_run_tiles: call gtp_set_trace first");
+  if (tile_cycles <= 0 || n_list <= 0 || !tiles || !cycle_power)
+    return x->fail(GTP_ERR_ARG, "gtp_run_tiles: bad arguments");
+  HIPCHK(x, hipSetDevice(x->device));
+  hipStream_t s = x->stream;
+  RunPlan rp;
+  gtp_status st = prepare_run(x, tile_cycles, &rp);
+  if (st != GTP_OK) return st;
+  for (int32_t l = 0; l < n_list; ++l)
+    if (tiles[l] < 0 || tiles[l] >= rp.n_tile)
+      return x->fail(GTP_ERR_ARG, "gtp_run_tiles: tile " + std::to_string(tiles[l]) + " outside [0, " +
+                                      std::to_string(rp.n_tile) + ")");
+  HIPCHK(x, x->d_tile_list.upload(tiles, n_list, s));
+  EvSet ev;
+  for (hipEvent_t& e : ev.e) HIPCHK(x, hipEventCreate(&e));
+  HIPCHK(x, hipEventRecord(ev.e[0], s));
+  if (!x->aux_valid) {
+    HIPCHK(x, launch_glitch_annotate(x->d_net_off.p, x->d_ev.p, x->d_chunk_off.p, x->N, x->n_chunk,
+                                     (uint32_t)x->T, rp.a.inv_T, x->d_aux.p, s));
+    x->aux_valid = true;
+  }
+  rp.a.tile_list = x->d_tile_list.p;
+  rp.a.n_work_tiles = n_list;
+  if ((st = launch_groups(x, rp)) != GTP_OK) return st;
+  const int64_t n_out = (int64_t)n_list * rp.TC;
+  HIPCHK(x, x->d_cycle.alloc((size_t)GTP_N_WAVE * std::max<int64_t>(n_out, x->C)));
+  HIPCHK(x, launch_cycle_reduce_list(x->d_cyc_part.p, rp.n_range, (int32_t)rp.TC, x->d_tile_list.p, n_list, x->C,
+                                     x->T, x->t_max, rp.q_e, rp.q_l, 1.0 / x->period, x->d_cycle.p, s));
+  HIPCHK(x, hipEventRecord(ev.e[1], s));
+  HIPCHK(x, hipMemcpyAsync(cycle_power, x->d_cycle.p, sizeof(double) * GTP_N_WAVE * n_out, hipMemcpyDeviceToHost, s));
+  HIPCHK(x, hipStreamSynchronize(s));
+  float ms = 0;
+  HIPCHK(x, hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
+  if (kernel_ms) *kernel_ms = ms;
+  return GTP_OK;
+}
+
+int32_t gtp::trace_dev(gtp_ctx* x, TraceDev* out) {
+  if (!x || !out) return GTP_ERR_ARG;
+  if (!x->have_trace) return x->fail(GTP_ERR_STATE, "call gtp_set_trace first");
+  out->device = x->device;
+  out->stream = x->stream;
+  out->n_net = x->N;
+  out->n_cycle = x->C;
+  out->T = x->T;
+  out->t_max = x->t_max;
+  out->net_off = x->d_net_off.p;
+  out->ev = x->d_ev.p;
+  out->chunk_off = x->d_chunk_off.p;
+  out->n_chunk = x->n_chunk;
+  out->host_net_off = x->h_net_off.data();
   return GTP_OK;
 }
 

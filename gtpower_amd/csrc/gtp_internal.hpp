@@ -96,6 +96,8 @@ struct PowerArgs {
   int32_t n_range;               // gate ranges per tile (all classes)
   int32_t range_gates;           // gates per range; order[] is padded with -1 to whole ranges
   int32_t xcd_remap;             // 1: remap blockIdx so each XCD takes a contiguous block of work
+  const int32_t* tile_list;      // tiles to evaluate (null: all n_tile tiles)
+  int32_t n_work_tiles;          // entries of tile_list, or n_tile
   // cell tables (staged into LDS)
   const CellDev* cells;
   int32_t n_cell;
@@ -142,6 +144,10 @@ hipError_t launch_power(int maxp, int64_t grid, int block, size_t lds, hipStream
 hipError_t launch_cycle_reduce(const long long* part, int32_t n_range, int32_t tile_cycles, int64_t n_cycle,
                                int64_t T, int64_t t_max, double q_energy, double q_leak, double inv_period,
                                double* out, hipStream_t s);
+hipError_t launch_cycle_reduce_list(const long long* part, int32_t n_range, int32_t tile_cycles,
+                                    const int32_t* tile_list, int32_t n_list, int64_t n_cycle, int64_t T,
+                                    int64_t t_max, double q_energy, double q_leak, double inv_period,
+                                    double* out, hipStream_t s);
 hipError_t launch_gate_reduce(const long long* pg, const uint32_t* tog, const uint32_t* glc,
                               const int32_t* order, int64_t n_sched, int32_t n_tile, int64_t n_gate,
                               double norm_energy, double norm_leak, double* gate_power,
@@ -156,4 +162,24 @@ hipError_t launch_total(const double* gate_power, const int32_t* gate_class, int
 hipError_t launch_power_cell(int np, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,
                              int32_t range_begin, int32_t range_count, int32_t cell);
 
+// Read-only view of a context's device event store, for the passes that run
+// over it besides the power kernels (toggle features, correlation).
+struct TraceDev {
+  int device;
+  hipStream_t stream;
+  int64_t n_net, n_cycle, T, t_max;
+  const int64_t* net_off;        // device [N+1]
+  const uint32_t* ev;            // device [E]
+  const int64_t* chunk_off;      // device [N+1] 256-event blocks per net (prefix)
+  int64_t n_chunk;
+  const int64_t* host_net_off;   // host [N+1]
+};
+
+}  // namespace gtp
+
+struct gtp_ctx;
+namespace gtp {
+// Fills *out from a context with a loaded trace; 0 on success, else a
+// negative status (the context's last error says why).
+int32_t trace_dev(gtp_ctx* ctx, TraceDev* out);
 }  // namespace gtp

@@ -216,15 +216,16 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
 
   // work item -> (tile, range); XCD-aware: each XCD takes a contiguous block of
   // tile-major work items, so the gates of one tile share that XCD's L2.
-  const int64_t n_work = (int64_t)a.n_tile * range_count;
+  const int64_t n_work = (int64_t)a.n_work_tiles * range_count;
   int64_t w = blockIdx.x;
   if (a.xcd_remap) {
     const int64_t per = (n_work + 7) / 8;
     w = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
   }
   if (w >= n_work) return;
-  const int tile = (int)(w / range_count);
-  const int range = range_begin + (int)(w - (int64_t)tile * range_count);
+  const int wt = (int)(w / range_count);
+  const int range = range_begin + (int)(w - (int64_t)wt * range_count);
+  const int tile = a.tile_list ? a.tile_list[wt] : wt;
 
   for (int i = threadIdx.x; i < N_BIN * TC; i += blockDim.x) bins[i] = 0;
   for (int i = threadIdx.x; i < a.n_leak; i += blockDim.x) {
@@ -635,15 +636,16 @@ __global__ __launch_bounds__(256) void power_cell_kernel(PowerArgs a, int32_t ra
   const int BW = N_BIN * TC;  // one copy of the bins
   long long* bins = reinterpret_cast<long long*>(smem_raw);
 
-  const int64_t n_work = (int64_t)a.n_tile * range_count;
+  const int64_t n_work = (int64_t)a.n_work_tiles * range_count;
   int64_t w = blockIdx.x;
   if (a.xcd_remap) {
     const int64_t per = (n_work + 7) / 8;
     w = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
   }
   if (w >= n_work) return;
-  const int tile = (int)(w / range_count);
-  const int range = range_begin + (int)(w - (int64_t)tile * range_count);
+  const int wt = (int)(w / range_count);
+  const int range = range_begin + (int)(w - (int64_t)wt * range_count);
+  const int tile = a.tile_list ? a.tile_list[wt] : wt;
 
   const CellDev cd = a.cells[cell];
   const uint32_t clk = cd.clock_mask;
@@ -944,6 +946,39 @@ __global__ void cycle_reduce_kernel(const long long* __restrict__ part, int32_t 
   out[4 * n_cycle + c] = (double)s[BIN_GSW] * q_energy * inv_period;
 }
 
+// cycle_reduce_kernel over the tiles of a list only: output column
+// l * tile_cycles + i is cycle tile_list[l] * tile_cycles + i (0 past the end)
+__global__ void cycle_reduce_list_kernel(const long long* __restrict__ part, int32_t n_range,
+                                         int32_t tile_cycles, const int32_t* __restrict__ tile_list,
+                                         int32_t n_list, int64_t n_cycle, int64_t T, int64_t t_max,
+                                         double q_energy, double q_leak, double inv_period,
+                                         double* __restrict__ out /* [5][n_list * tile_cycles] */) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n_out = (int64_t)n_list * tile_cycles;
+  if (j >= n_out) return;
+  const int64_t tile = tile_list[j / tile_cycles];
+  const int ci = (int)(j % tile_cycles);
+  const int64_t c = tile * tile_cycles + ci;
+  if (c >= n_cycle) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) out[k * n_out + j] = 0.0;
+    return;
+  }
+  long long s[N_BIN] = {0, 0, 0, 0, 0, 0};
+  for (int r = 0; r < n_range; ++r) {
+    const long long* p = part + ((tile * n_range + r) * N_BIN) * tile_cycles + ci;
+#pragma unroll
+    for (int k = 0; k < N_BIN; ++k) s[k] += p[(int64_t)k * tile_cycles];
+  }
+  const int64_t cend = min((c + 1) * T, t_max);
+  const double frac = (double)(cend - c * T) / (double)T;
+  out[0 * n_out + j] = ((double)s[BIN_LD] * frac + (double)s[BIN_LW]) * q_leak;
+  out[1 * n_out + j] = (double)(s[BIN_INT] + s[BIN_GINT]) * q_energy * inv_period;
+  out[2 * n_out + j] = (double)s[BIN_GINT] * q_energy * inv_period;
+  out[3 * n_out + j] = (double)(s[BIN_SW] + s[BIN_GSW]) * q_energy * inv_period;
+  out[4 * n_out + j] = (double)s[BIN_GSW] * q_energy * inv_period;
+}
+
 __global__ void gate_reduce_kernel(const long long* __restrict__ pg, const uint32_t* __restrict__ tog,
                                    const uint32_t* __restrict__ glc, const int32_t* __restrict__ order,
                                    int64_t n_sched, int32_t n_tile, int64_t n_gate, double norm_energy,
@@ -1049,6 +1084,17 @@ hipError_t launch_cycle_reduce(const long long* part, int32_t n_range, int32_t t
   if (n_cycle <= 0) return hipSuccess;
   cycle_reduce_kernel<<<dim3((unsigned)((n_cycle + 255) / 256)), dim3(256), 0, s>>>(
       part, n_range, tile_cycles, n_cycle, T, t_max, q_energy, q_leak, inv_period, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_cycle_reduce_list(const long long* part, int32_t n_range, int32_t tile_cycles,
+                                    const int32_t* tile_list, int32_t n_list, int64_t n_cycle, int64_t T,
+                                    int64_t t_max, double q_energy, double q_leak, double inv_period,
+                                    double* out, hipStream_t s) {
+  const int64_t n = (int64_t)n_list * tile_cycles;
+  if (n <= 0) return hipSuccess;
+  cycle_reduce_list_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(
+      part, n_range, tile_cycles, tile_list, n_list, n_cycle, T, t_max, q_energy, q_leak, inv_period, out);
   return hipGetLastError();
 }
 

@@ -1,0 +1,986 @@
+// Learned per-cycle power surrogate: a small MLP over per-cycle toggle bits,
+// trained and run by hand-written gfx950 BF16 MFMA kernels (FP32 accumulate).
+//
+// No counterpart in the reference (a new method on top of its path).  The
+// exact engine (gtp_run) produces the training targets and judges the
+// surrogate on sampled cycles.
+//
+//   features : bit k of cycle c = net sel[k] has an event (index >= 1) in cycle c
+//   model    : z1 = X W1^T + b1, h1 = GELU(z1), z2 = h1 W2^T + b2, h2 = GELU(z2),
+//              y = h2 . w3 + b3     (W1 [H][K], W2 [H][H], GELU(x) = x Phi(x))
+//   training : MSE on standardised targets, Adam, FP32 master weights, BF16
+//              GEMM operands (weights, activations, gated gradients)
+//
+// GEMM: C[M][N] = A[M][K] * B[K][N] on v_mfma_f32_16x16x32_bf16.  A workgroup
+// of 4 waves computes a 128 x 128 tile; each wave a 64 x 64 block (4 x 4 MFMA
+// tiles, 64 accumulators).  Operands are staged in LDS as A[m][k] and B[n][k]
+// (k contiguous, 16-byte fragment reads).  Operand modes cover row-major and
+// transposed BF16 matrices and bit-packed features, so no transpose is ever
+// materialised.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "gtp_internal.hpp"
+#include "gtpower.h"
+
+#define GTP_API extern "C" __attribute__((visibility("default")))
+
+namespace gtp {
+namespace mlp {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 128, BN = 128, BK = 32, LDSK = BK + 8;  // LDS row: 40 bf16 = 80 B
+
+enum { A_ROW = 0, A_COL = 1, A_BITS_ROW = 2 };
+enum { B_ROW = 0, B_TRANS = 1, B_BITS_ROW = 2 };
+enum { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_GELU_BACK = 2 };
+
+struct GemmArgs {
+  int M, N, K;
+  const void* A;
+  int lda;          // row stride of A as stored (elements; words for bits)
+  const void* B;
+  int ldb;
+  float* C;         // EPI_F32: result; EPI_BIAS_GELU: pre-activation (may be null)
+  int ldc;
+  bf16* Cb;         // bf16 result (activation or gated gradient)
+  int ldcb;
+  const float* bias;
+  const float* zin; // EPI_GELU_BACK: pre-activation
+  int ldz;
+};
+
+__device__ __forceinline__ bf16 f2bf(float x) { return (bf16)x; }
+__device__ __forceinline__ float bf2f(bf16 x) { return (float)x; }
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z) {
+  return 0.5f * (1.f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+// 8 bits -> 8 bf16 zeros and ones (1.0 = 0x3F80), as one 16-byte vector
+__device__ __forceinline__ uint4 expand8(uint32_t b) {
+  uint4 v;
+  v.x = ((b & 1u) ? 0x3F80u : 0u) | ((b & 2u) ? 0x3F800000u : 0u);
+  v.y = ((b & 4u) ? 0x3F80u : 0u) | ((b & 8u) ? 0x3F800000u : 0u);
+  v.z = ((b & 16u) ? 0x3F80u : 0u) | ((b & 32u) ? 0x3F800000u : 0u);
+  v.w = ((b & 64u) ? 0x3F80u : 0u) | ((b & 128u) ? 0x3F800000u : 0u);
+  return v;
+}
+
+template <int AM, int BMODE, int EPI>
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
+  __shared__ __align__(16) bf16 As[BM * LDSK];
+  __shared__ __align__(16) bf16 Bs[BN * LDSK];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  floatx4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  for (int k0 = 0; k0 < g.K; k0 += BK) {
+    // ---- A tile -> As[m][k]
+    if (AM == A_ROW) {
+      const bf16* A = (const bf16*)g.A;
+      const int r = tid >> 1, ks = (tid & 1) * 16;
+      uint4 v0 = {0, 0, 0, 0}, v1 = {0, 0, 0, 0};
+      if (m0 + r < g.M) {
+        const uint4* src = (const uint4*)(A + (int64_t)(m0 + r) * g.lda + k0 + ks);
+        v0 = src[0];
+        v1 = src[1];
+      }
+      uint4* dst = (uint4*)(As + r * LDSK + ks);
+      dst[0] = v0;
+      dst[1] = v1;
+    } else if (AM == A_COL) {
+      // A[m][k] = S[k][m], S row-major with stride lda
+      // (M is a multiple of 16 in this mode: whole 16-row groups or none)
+      const bf16* S = (const bf16*)g.A;
+      const int kk = tid >> 3, ms = (tid & 7) * 16;
+      union {
+        uint4 u[2];
+        bf16 h[16];
+      } v;
+      v.u[0] = v.u[1] = uint4{0, 0, 0, 0};
+      if (m0 + ms < g.M) {
+        const uint4* src = (const uint4*)(S + (int64_t)(k0 + kk) * g.lda + m0 + ms);
+        v.u[0] = src[0];
+        v.u[1] = src[1];
+      }
+#pragma unroll
+      for (int j = 0; j < 16; ++j) As[(ms + j) * LDSK + kk] = v.h[j];
+    } else {
+      // bit-packed rows: A[m][k] = bit k of row m
+      const uint32_t* X = (const uint32_t*)g.A;
+      const int r = tid >> 1, h = tid & 1;
+      const uint32_t w = (m0 + r < g.M) ? X[(int64_t)(m0 + r) * g.lda + (k0 >> 5)] : 0u;
+      uint4* dst = (uint4*)(As + r * LDSK + h * 16);
+      dst[0] = expand8((w >> (h * 16)) & 0xFFu);
+      dst[1] = expand8((w >> (h * 16 + 8)) & 0xFFu);
+    }
+    // ---- B tile -> Bs[n][k]
+    if (BMODE == B_TRANS) {
+      // B[k][n] = W[n][k], W row-major with stride ldb
+      const bf16* W = (const bf16*)g.B;
+      const int r = tid >> 1, ks = (tid & 1) * 16;
+      const uint4* src = (const uint4*)(W + (int64_t)(n0 + r) * g.ldb + k0 + ks);
+      uint4* dst = (uint4*)(Bs + r * LDSK + ks);
+      dst[0] = src[0];
+      dst[1] = src[1];
+    } else if (BMODE == B_ROW) {
+      const bf16* Bm = (const bf16*)g.B;
+      const int kk = tid >> 3, ns = (tid & 7) * 16;
+      union {
+        uint4 u[2];
+        bf16 h[16];
+      } v;
+      const uint4* src = (const uint4*)(Bm + (int64_t)(k0 + kk) * g.ldb + n0 + ns);
+      v.u[0] = src[0];
+      v.u[1] = src[1];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) Bs[(ns + j) * LDSK + kk] = v.h[j];
+    } else {
+      // B[k][n] = bit n of bit-packed row k
+      const uint32_t* X = (const uint32_t*)g.B;
+      if (tid < BN) {
+        const int kk = tid >> 2, wi = tid & 3;
+        const uint32_t w = X[(int64_t)(k0 + kk) * g.ldb + (n0 >> 5) + wi];
+#pragma unroll
+        for (int b = 0; b < 32; ++b) Bs[(wi * 32 + b) * LDSK + kk] = ((w >> b) & 1u) ? (bf16)1.0f : (bf16)0.0f;
+      }
+    }
+    __syncthreads();
+    // ---- 4 x 4 MFMA 16x16x32 per wave
+    bf16x8 af[4], bfr[4];
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) af[i] = *(const bf16x8*)(As + (wm * 64 + i * 16 + fr) * LDSK + fk);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bfr[j] = *(const bf16x8*)(Bs + (wn * 64 + j * 16 + fr) * LDSK + fk);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  // ---- epilogue: lane holds D[(lane/16)*4 + e][lane%16] of each 16x16 tile
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + e;
+        const int col = n0 + wn * 64 + j * 16 + (lane & 15);
+        if (row >= g.M) continue;
+        const float v = acc[i][j][e];
+        if (EPI == EPI_F32) {
+          g.C[(int64_t)row * g.ldc + col] = v;
+        } else if (EPI == EPI_BIAS_GELU) {
+          const float z = v + g.bias[col];
+          if (g.C) g.C[(int64_t)row * g.ldc + col] = z;
+          g.Cb[(int64_t)row * g.ldcb + col] = f2bf(gelu(z));
+        } else {
+          const float z = g.zin[(int64_t)row * g.ldz + col];
+          g.Cb[(int64_t)row * g.ldcb + col] = f2bf(v * gelu_grad(z));
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+// y[m] = h2[m] . w3 + b3 (one wave per row)
+__global__ void head_forward_kernel(const bf16* __restrict__ h2, const float* __restrict__ w3,
+                                    const float* __restrict__ b3, int M, int H, float* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (row >= M) return;
+  float s = 0.f;
+  for (int n = lane; n < H; n += 64) s += bf2f(h2[row * H + n]) * w3[n];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) out[row] = s + b3[0];
+}
+
+// d_out = 2 (out - y) / M and per-block loss partial sums
+__global__ void loss_kernel(const float* __restrict__ out, const float* __restrict__ y, int M,
+                            float* __restrict__ dout, float* __restrict__ loss_part) {
+  __shared__ float s[256];
+  const int m = blockIdx.x * 256 + threadIdx.x;
+  float l = 0.f;
+  if (m < M) {
+    const float d = out[m] - y[m];
+    dout[m] = 2.f * d / (float)M;
+    l = d * d;
+  }
+  s[threadIdx.x] = l;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss_part[blockIdx.x] = s[0];
+}
+
+// dW3[n] = sum_m h2[m][n] dout[m]; db3 = sum_m dout[m]   (thread per n)
+__global__ void head_grad_kernel(const bf16* __restrict__ h2, const float* __restrict__ dout, int M, int H,
+                                 float* __restrict__ dw3, float* __restrict__ db3) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < H) {
+    float s = 0.f;
+    for (int m = 0; m < M; ++m) s += bf2f(h2[(int64_t)m * H + n]) * dout[m];
+    dw3[n] = s;
+  }
+  if (n == 0) {
+    float s = 0.f;
+    for (int m = 0; m < M; ++m) s += dout[m];
+    db3[0] = s;
+  }
+}
+
+// dz2[m][n] = bf16(dout[m] w3[n] GELU'(z2[m][n]))
+__global__ void head_back_kernel(const float* __restrict__ dout, const float* __restrict__ w3,
+                                 const float* __restrict__ z2, int M, int H, bf16* __restrict__ dz2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)M * H) return;
+  const int m = (int)(i / H), n = (int)(i - (int64_t)m * H);
+  dz2[i] = f2bf(dout[m] * w3[n] * gelu_grad(z2[i]));
+}
+
+// db[n] = sum_m dz[m][n]   (thread per n)
+__global__ void colsum_kernel(const bf16* __restrict__ dz, int M, int H, float* __restrict__ db) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= H) return;
+  float s = 0.f;
+  for (int m = 0; m < M; ++m) s += bf2f(dz[(int64_t)m * H + n]);
+  db[n] = s;
+}
+
+__global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
+                            float* __restrict__ m2, int64_t n, float lr, float b1, float b2, float eps,
+                            float c1, float c2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float a = b1 * m1[i] + (1.f - b1) * gi;
+  const float v = b2 * m2[i] + (1.f - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = v;
+  p[i] -= lr * (a / c1) / (sqrtf(v / c2) + eps);
+}
+
+__global__ void cast_bf16_kernel(const float* __restrict__ src, bf16* __restrict__ dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = f2bf(src[i]);
+}
+
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void gather_rows_kernel(const uint32_t* __restrict__ X, const float* __restrict__ y,
+                                   const int32_t* __restrict__ idx, int rows, int words,
+                                   uint32_t* __restrict__ Xb, float* __restrict__ yb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)rows * words) return;
+  const int r = (int)(i / words), w = (int)(i - (int64_t)r * words);
+  const int src = idx[r];
+  Xb[i] = X[(int64_t)src * words + w];
+  if (w == 0) yb[r] = y[src];
+}
+
+__device__ __forceinline__ int64_t cycle_of_tick(uint32_t t, uint32_t T, float invT) {
+  uint32_t q = (uint32_t)((float)t * invT);
+  int32_t r = (int32_t)(t - q * T);
+  while (r < 0) { --q; r += (int32_t)T; }
+  while (r >= (int32_t)T) { ++q; r -= (int32_t)T; }
+  return (int64_t)q;
+}
+
+// Toggle bits of the selected features: feature k = (net, kind) with
+// sel_net[k] = kind * n_net + net; bit k of row c is set when the net has an
+// event of index >= 1 (a value change; index 0 is the initial value) in cycle
+// c0 + c, of any value (kind 0) or to value 1 (kind 1, a rise).  One block per
+// 256 events of a selected net: sel_chunk_off[k] is the first block of
+// feature k.
+__global__ __launch_bounds__(256) void toggle_bits_kernel(const int64_t* __restrict__ net_off,
+                                                          const uint32_t* __restrict__ ev,
+                                                          const int32_t* __restrict__ sel_net,
+                                                          const int64_t* __restrict__ sel_chunk_off, int32_t K,
+                                                          int64_t n_net, uint32_t T, float invT, int64_t c0,
+                                                          int64_t n_cyc, int words, uint32_t* __restrict__ X) {
+  __shared__ int32_t s_k;
+  const int64_t chunk = blockIdx.x;
+  if (threadIdx.x == 0) {
+    int lo = 0, hi = K;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (sel_chunk_off[mid] <= chunk) lo = mid; else hi = mid;
+    }
+    s_k = lo;
+  }
+  __syncthreads();
+  const int k = s_k;
+  const int64_t f = sel_net[k];
+  const bool rise_only = f >= n_net;
+  const int64_t n = rise_only ? f - n_net : f;
+  const int64_t b = net_off[n], cnt = net_off[n + 1] - b;
+  const int64_t i = (chunk - sel_chunk_off[k]) * 256 + threadIdx.x;
+  if (i < 1 || i >= cnt) return;
+  const uint32_t w = ev[b + i];
+  if (rise_only && (w & 3u) != 1u) return;
+  const int64_t c = cycle_of_tick(w >> 2, T, invT) - c0;
+  if (c < 0 || c >= n_cyc) return;
+  atomicOr(&X[c * words + (k >> 5)], 1u << (k & 31));
+}
+
+// Per net over the cycles with mask[c] != 0 (all when mask is null): the
+// number of cycles with a value change (kind 0) and with a change to 1 (kind
+// 1), and the sums of the quantised target over those cycles (int64: exact
+// and order-free); outputs are [2][n_net].  One block per 256 events
+// (chunk_off as for the glitch annotation).
+__global__ __launch_bounds__(256) void net_corr_kernel(const int64_t* __restrict__ net_off,
+                                                       const uint32_t* __restrict__ ev,
+                                                       const int64_t* __restrict__ chunk_off, int64_t n_net,
+                                                       uint32_t T, float invT, int64_t n_cyc,
+                                                       const uint8_t* __restrict__ mask,
+                                                       const long long* __restrict__ yq,
+                                                       unsigned long long* __restrict__ cnt_out,
+                                                       unsigned long long* __restrict__ sum_out) {
+  __shared__ int64_t s_net;
+  const int64_t chunk = blockIdx.x;
+  if (threadIdx.x == 0) {
+    int64_t lo = 0, hi = n_net;
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (chunk_off[mid] <= chunk) lo = mid; else hi = mid;
+    }
+    s_net = lo;
+  }
+  __syncthreads();
+  const int64_t n = s_net;
+  const int64_t b = net_off[n], cnt = net_off[n + 1] - b;
+  const int64_t i = (chunk - chunk_off[n]) * 256 + threadIdx.x;
+  bool take = false, take_rise = false;
+  int64_t c = 0;
+  if (i >= 1 && i < cnt) {
+    const uint32_t w = ev[b + i];
+    c = cycle_of_tick(w >> 2, T, invT);
+    if (c < n_cyc && (!mask || mask[c])) {
+      take = i == 1 || cycle_of_tick(ev[b + i - 1] >> 2, T, invT) != c;
+      if ((w & 3u) == 1u) {
+        // the first change to 1 of this net in cycle c
+        take_rise = true;
+        for (int64_t j = i - 1; j >= 1; --j) {
+          const uint32_t wj = ev[b + j];
+          if (cycle_of_tick(wj >> 2, T, invT) != c) break;
+          if ((wj & 3u) == 1u) {
+            take_rise = false;
+            break;
+          }
+        }
+      }
+    }
+  }
+  unsigned long long k = take ? 1ull : 0ull, kr = take_rise ? 1ull : 0ull;
+  long long s = take ? yq[c] : 0, sr = take_rise ? yq[c] : 0;
+  for (int off = 32; off > 0; off >>= 1) {
+    k += __shfl_xor(k, off);
+    s += __shfl_xor(s, off);
+    kr += __shfl_xor(kr, off);
+    sr += __shfl_xor(sr, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (k) {
+      atomicAdd(&cnt_out[n], k);
+      atomicAdd(&sum_out[n], (unsigned long long)s);
+    }
+    if (kr) {
+      atomicAdd(&cnt_out[n_net + n], kr);
+      atomicAdd(&sum_out[n_net + n], (unsigned long long)sr);
+    }
+  }
+}
+
+template <int AM, int BMODE, int EPI>
+hipError_t gemm(const GemmArgs& g, hipStream_t s) {
+  if (g.M <= 0) return hipSuccess;
+  if (g.N % BN || g.K % BK || (AM == A_COL && g.M % 16)) return hipErrorInvalidValue;
+  dim3 grid(g.N / BN, (g.M + BM - 1) / BM);
+  gemm_kernel<AM, BMODE, EPI><<<grid, dim3(256), 0, s>>>(g);
+  return hipGetLastError();
+}
+
+inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace mlp
+}  // namespace gtp
+
+using namespace gtp;
+using namespace gtp::mlp;
+
+struct gtp_mlp {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int K = 0, H = 0, words = 0;
+  std::string err;
+  // parameters (FP32 master) in one buffer: W1[H][K] b1[H] W2[H][H] b2[H] w3[H] b3[1]
+  int64_t n_param = 0;
+  float *P = nullptr, *G = nullptr, *M1 = nullptr, *M2 = nullptr;
+  bf16 *W1b = nullptr, *W2b = nullptr;
+  float y_mean = 0.f, y_std = 1.f;
+  int64_t adam_t = 0;
+  // batch workspace
+  int cap = 0;
+  uint32_t* Xb = nullptr;
+  float *yb = nullptr, *z1 = nullptr, *z2 = nullptr, *out = nullptr, *dout = nullptr, *lossp = nullptr;
+  bf16 *h1 = nullptr, *h2 = nullptr, *dz1 = nullptr, *dz2 = nullptr;
+  int32_t* idx = nullptr;
+
+  float* W1() { return P; }
+  float* b1() { return P + (int64_t)H * K; }
+  float* W2() { return b1() + H; }
+  float* b2() { return W2() + (int64_t)H * H; }
+  float* w3() { return b2() + H; }
+  float* b3() { return w3() + H; }
+  float* g(int which) {
+    float* base = G;
+    const int64_t off[6] = {0, (int64_t)H * K, (int64_t)H * K + H, (int64_t)H * K + H + (int64_t)H * H,
+                            (int64_t)H * K + 2 * H + (int64_t)H * H, (int64_t)H * K + 3 * H + (int64_t)H * H};
+    return base + off[which];
+  }
+  gtp_status fail(gtp_status s, const std::string& m) {
+    err = m;
+    return s;
+  }
+  void release_batch() {
+    for (void* p : {(void*)Xb, (void*)yb, (void*)z1, (void*)z2, (void*)out, (void*)dout, (void*)lossp,
+                    (void*)h1, (void*)h2, (void*)dz1, (void*)dz2, (void*)idx})
+      if (p) (void)hipFree(p);
+    Xb = nullptr; yb = z1 = z2 = out = dout = lossp = nullptr;
+    h1 = h2 = dz1 = dz2 = nullptr; idx = nullptr;
+    cap = 0;
+  }
+  hipError_t reserve(int n) {
+    if (n <= cap) return hipSuccess;
+    release_batch();
+    const int c = (n + 127) / 128 * 128;
+    hipError_t e;
+#define ALLOC(p, bytes) if ((e = hipMalloc((void**)&p, (bytes))) != hipSuccess) return e;
+    ALLOC(Xb, (size_t)c * words * 4);
+    ALLOC(yb, (size_t)c * 4);
+    ALLOC(z1, (size_t)c * H * 4);
+    ALLOC(z2, (size_t)c * H * 4);
+    ALLOC(out, (size_t)c * 4);
+    ALLOC(dout, (size_t)c * 4);
+    ALLOC(lossp, (size_t)(c / 256 + 2) * 4);
+    ALLOC(h1, (size_t)c * H * 2);
+    ALLOC(h2, (size_t)c * H * 2);
+    ALLOC(dz1, (size_t)c * H * 2);
+    ALLOC(dz2, (size_t)c * H * 2);
+    ALLOC(idx, (size_t)c * 4);
+#undef ALLOC
+    cap = c;
+    return hipSuccess;
+  }
+};
+
+#define MCHK(m, expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return (m)->fail(e_ == hipErrorOutOfMemory ? GTP_ERR_OOM : GTP_ERR_DEVICE,             \
+                       std::string(#expr) + ": " + hipGetErrorString(e_));                   \
+  } while (0)
+
+namespace {
+
+hipError_t refresh_bf16(gtp_mlp* m) {
+  const int64_t n1 = (int64_t)m->H * m->K, n2 = (int64_t)m->H * m->H;
+  cast_bf16_kernel<<<nblk(n1), 256, 0, m->stream>>>(m->W1(), m->W1b, n1);
+  cast_bf16_kernel<<<nblk(n2), 256, 0, m->stream>>>(m->W2(), m->W2b, n2);
+  return hipGetLastError();
+}
+
+// forward pass over n rows of bit features already on the device
+hipError_t forward(gtp_mlp* m, const uint32_t* X, int n, bool keep_z, float* out) {
+  const int H = m->H;
+  GemmArgs g{};
+  g.M = n; g.N = H; g.K = m->K;
+  g.A = X; g.lda = m->words;
+  g.B = m->W1b; g.ldb = m->K;
+  g.C = keep_z ? m->z1 : nullptr; g.ldc = H;
+  g.Cb = m->h1; g.ldcb = H;
+  g.bias = m->b1();
+  hipError_t e = gemm<A_BITS_ROW, B_TRANS, EPI_BIAS_GELU>(g, m->stream);
+  if (e != hipSuccess) return e;
+  GemmArgs g2{};
+  g2.M = n; g2.N = H; g2.K = H;
+  g2.A = m->h1; g2.lda = H;
+  g2.B = m->W2b; g2.ldb = H;
+  g2.C = keep_z ? m->z2 : nullptr; g2.ldc = H;
+  g2.Cb = m->h2; g2.ldcb = H;
+  g2.bias = m->b2();
+  if ((e = gemm<A_ROW, B_TRANS, EPI_BIAS_GELU>(g2, m->stream)) != hipSuccess) return e;
+  head_forward_kernel<<<nblk(n, 4), 256, 0, m->stream>>>(m->h2, m->w3(), m->b3(), n, H, out);
+  return hipGetLastError();
+}
+
+// forward + backward of one batch held in m->Xb / m->yb (standardised y);
+// gradients into m->G; loss partials into m->lossp
+hipError_t backward(gtp_mlp* m, int n) {
+  const int H = m->H;
+  hipError_t e = forward(m, m->Xb, n, true, m->out);
+  if (e != hipSuccess) return e;
+  loss_kernel<<<nblk(n), 256, 0, m->stream>>>(m->out, m->yb, n, m->dout, m->lossp);
+  head_grad_kernel<<<nblk(H), 256, 0, m->stream>>>(m->h2, m->dout, n, H, m->g(4), m->g(5));
+  head_back_kernel<<<nblk((int64_t)n * H), 256, 0, m->stream>>>(m->dout, m->w3(), m->z2, n, H, m->dz2);
+  // dW2 = dz2^T h1
+  GemmArgs a{};
+  a.M = H; a.N = H; a.K = n;
+  a.A = m->dz2; a.lda = H;
+  a.B = m->h1; a.ldb = H;
+  a.C = m->g(2); a.ldc = H;
+  if ((e = gemm<A_COL, B_ROW, EPI_F32>(a, m->stream)) != hipSuccess) return e;
+  colsum_kernel<<<nblk(H), 256, 0, m->stream>>>(m->dz2, n, H, m->g(3));
+  // dz1 = (dz2 W2) * GELU'(z1)
+  GemmArgs b{};
+  b.M = n; b.N = H; b.K = H;
+  b.A = m->dz2; b.lda = H;
+  b.B = m->W2b; b.ldb = H;
+  b.Cb = m->dz1; b.ldcb = H;
+  b.zin = m->z1; b.ldz = H;
+  if ((e = gemm<A_ROW, B_ROW, EPI_GELU_BACK>(b, m->stream)) != hipSuccess) return e;
+  colsum_kernel<<<nblk(H), 256, 0, m->stream>>>(m->dz1, n, H, m->g(1));
+  // dW1 = dz1^T X
+  GemmArgs c{};
+  c.M = H; c.N = m->K; c.K = n;
+  c.A = m->dz1; c.lda = H;
+  c.B = m->Xb; c.ldb = m->words;
+  c.C = m->g(0); c.ldc = m->K;
+  if ((e = gemm<A_COL, B_BITS_ROW, EPI_F32>(c, m->stream)) != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+}  // namespace
+
+GTP_API gtp_status gtp_mlp_create(const gtp_mlp_config* cfg, gtp_mlp** out) {
+  if (!cfg || !out || cfg->n_in <= 0 || cfg->n_in % 128 || cfg->n_hidden <= 0 || cfg->n_hidden % 128)
+    return GTP_ERR_ARG;
+  *out = nullptr;
+  auto* m = new (std::nothrow) gtp_mlp();
+  if (!m) return GTP_ERR_OOM;
+  m->K = cfg->n_in;
+  m->H = cfg->n_hidden;
+  m->words = m->K / 32;
+  int dev = cfg->device;
+  if (dev < 0 && hipGetDevice(&dev) != hipSuccess) dev = 0;
+  m->device = dev;
+  if (hipSetDevice(dev) != hipSuccess || hipStreamCreate(&m->stream) != hipSuccess) {
+    delete m;
+    return GTP_ERR_DEVICE;
+  }
+  const int K = m->K, H = m->H;
+  m->n_param = (int64_t)H * K + H + (int64_t)H * H + H + H + 1;
+  size_t pb = (size_t)m->n_param * 4;
+  if (hipMalloc((void**)&m->P, pb) != hipSuccess || hipMalloc((void**)&m->G, pb) != hipSuccess ||
+      hipMalloc((void**)&m->M1, pb) != hipSuccess || hipMalloc((void**)&m->M2, pb) != hipSuccess ||
+      hipMalloc((void**)&m->W1b, (size_t)H * K * 2) != hipSuccess ||
+      hipMalloc((void**)&m->W2b, (size_t)H * H * 2) != hipSuccess) {
+    gtp_mlp_destroy(m);
+    return GTP_ERR_OOM;
+  }
+  // He-style initialisation from a seeded generator
+  std::vector<float> p(m->n_param, 0.f);
+  std::mt19937_64 rng(cfg->seed);
+  std::normal_distribution<float> nd(0.f, 1.f);
+  const float s1 = std::sqrt(2.f / K), s2 = std::sqrt(2.f / H), s3 = std::sqrt(1.f / H);
+  for (int64_t i = 0; i < (int64_t)H * K; ++i) p[i] = nd(rng) * s1;
+  float* w2 = p.data() + (int64_t)H * K + H;
+  for (int64_t i = 0; i < (int64_t)H * H; ++i) w2[i] = nd(rng) * s2;
+  float* w3 = w2 + (int64_t)H * H + H;
+  for (int i = 0; i < H; ++i) w3[i] = nd(rng) * s3;
+  if (hipMemcpy(m->P, p.data(), pb, hipMemcpyHostToDevice) != hipSuccess) {
+    gtp_mlp_destroy(m);
+    return GTP_ERR_DEVICE;
+  }
+  std::vector<float> zero(m->n_param, 0.f);
+  (void)hipMemcpy(m->M1, zero.data(), pb, hipMemcpyHostToDevice);
+  (void)hipMemcpy(m->M2, zero.data(), pb, hipMemcpyHostToDevice);
+  (void)refresh_bf16(m);
+  (void)hipStreamSynchronize(m->stream);
+  *out = m;
+  return GTP_OK;
+}
+
+GTP_API void gtp_mlp_destroy(gtp_mlp* m) {
+  if (!m) return;
+  (void)hipSetDevice(m->device);
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  m->release_batch();
+  for (void* p : {(void*)m->P, (void*)m->G, (void*)m->M1, (void*)m->M2, (void*)m->W1b, (void*)m->W2b})
+    if (p) (void)hipFree(p);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+GTP_API const char* gtp_mlp_last_error(const gtp_mlp* m) { return m ? m->err.c_str() : "null model"; }
+
+GTP_API int64_t gtp_mlp_param_count(const gtp_mlp* m) { return m ? m->n_param : 0; }
+
+GTP_API gtp_status gtp_mlp_get_params(gtp_mlp* m, float* params, float* y_mean_std) {
+  if (!m || !params) return GTP_ERR_ARG;
+  MCHK(m, hipSetDevice(m->device));
+  MCHK(m, hipMemcpy(params, m->P, (size_t)m->n_param * 4, hipMemcpyDeviceToHost));
+  if (y_mean_std) {
+    y_mean_std[0] = m->y_mean;
+    y_mean_std[1] = m->y_std;
+  }
+  return GTP_OK;
+}
+
+GTP_API gtp_status gtp_mlp_set_params(gtp_mlp* m, const float* params, const float* y_mean_std) {
+  if (!m || !params) return GTP_ERR_ARG;
+  MCHK(m, hipSetDevice(m->device));
+  MCHK(m, hipMemcpy(m->P, params, (size_t)m->n_param * 4, hipMemcpyHostToDevice));
+  if (y_mean_std) {
+    m->y_mean = y_mean_std[0];
+    m->y_std = y_mean_std[1];
+  }
+  MCHK(m, refresh_bf16(m));
+  MCHK(m, hipStreamSynchronize(m->stream));
+  return GTP_OK;
+}
+
+// Gradients of the mean squared error of one batch (targets already
+// standardised by the caller), without updating the model.
+GTP_API gtp_status gtp_mlp_gradients(gtp_mlp* m, const uint32_t* bits, const float* y, int32_t n,
+                                     float* grads, float* loss) {
+  if (!m || !bits || !y || !grads || n <= 0 || n % 32) return m ? m->fail(GTP_ERR_ARG, "n must be a positive multiple of 32") : GTP_ERR_ARG;
+  MCHK(m, hipSetDevice(m->device));
+  MCHK(m, m->reserve(n));
+  MCHK(m, hipMemcpyAsync(m->Xb, bits, (size_t)n * m->words * 4, hipMemcpyHostToDevice, m->stream));
+  MCHK(m, hipMemcpyAsync(m->yb, y, (size_t)n * 4, hipMemcpyHostToDevice, m->stream));
+  MCHK(m, backward(m, n));
+  MCHK(m, hipMemcpyAsync(grads, m->G, (size_t)m->n_param * 4, hipMemcpyDeviceToHost, m->stream));
+  std::vector<float> lp(nblk(n));
+  MCHK(m, hipMemcpyAsync(lp.data(), m->lossp, lp.size() * 4, hipMemcpyDeviceToHost, m->stream));
+  MCHK(m, hipStreamSynchronize(m->stream));
+  if (loss) *loss = std::accumulate(lp.begin(), lp.end(), 0.f) / (float)n;
+  return GTP_OK;
+}
+
+GTP_API gtp_status gtp_mlp_train(gtp_mlp* m, const uint32_t* bits, const float* y, int64_t n,
+                                 const gtp_train_config* tc, float* epoch_loss) {
+  if (!m || !bits || !y || !tc || n < 32 || tc->batch <= 0 || tc->batch % 32 || tc->epochs <= 0)
+    return m ? m->fail(GTP_ERR_ARG, "gtp_mlp_train: bad arguments") : GTP_ERR_ARG;
+  MCHK(m, hipSetDevice(m->device));
+  // standardise the targets
+  double mu = 0, var = 0;
+  for (int64_t i = 0; i < n; ++i) mu += y[i];
+  mu /= (double)n;
+  for (int64_t i = 0; i < n; ++i) var += (y[i] - mu) * (y[i] - mu);
+  const double sd = std::sqrt(var / (double)n) > 0 ? std::sqrt(var / (double)n) : 1.0;
+  m->y_mean = (float)mu;
+  m->y_std = (float)sd;
+  std::vector<float> ys(n);
+  for (int64_t i = 0; i < n; ++i) ys[i] = (float)((y[i] - mu) / sd);
+  uint32_t* dX = nullptr;
+  float* dy = nullptr;
+  MCHK(m, hipMalloc((void**)&dX, (size_t)n * m->words * 4));
+  if (hipMalloc((void**)&dy, (size_t)n * 4) != hipSuccess) {
+    (void)hipFree(dX);
+    return m->fail(GTP_ERR_OOM, "gtp_mlp_train: target buffer");
+  }
+  struct Free2 {
+    void *a, *b;
+    ~Free2() {
+      (void)hipFree(a);
+      (void)hipFree(b);
+    }
+  } fr{dX, dy};
+  MCHK(m, hipMemcpyAsync(dX, bits, (size_t)n * m->words * 4, hipMemcpyHostToDevice, m->stream));
+  MCHK(m, hipMemcpyAsync(dy, ys.data(), (size_t)n * 4, hipMemcpyHostToDevice, m->stream));
+  const int B = tc->batch;
+  MCHK(m, m->reserve(B));
+  std::mt19937_64 rng(tc->seed);
+  std::vector<int32_t> perm(n);
+  std::iota(perm.begin(), perm.end(), 0);
+  const int64_t steps_per_epoch = n / B;
+  const int64_t total_steps = steps_per_epoch * tc->epochs;
+  int64_t step = 0;
+  std::vector<float> lp(nblk(B));
+  for (int ep = 0; ep < tc->epochs; ++ep) {
+    std::shuffle(perm.begin(), perm.end(), rng);
+    double eloss = 0;
+    for (int64_t s = 0; s < steps_per_epoch; ++s, ++step) {
+      MCHK(m, hipMemcpyAsync(m->idx, perm.data() + s * B, (size_t)B * 4, hipMemcpyHostToDevice, m->stream));
+      gather_rows_kernel<<<nblk((int64_t)B * m->words), 256, 0, m->stream>>>(dX, dy, m->idx, B, m->words, m->Xb, m->yb);
+      MCHK(m, backward(m, B));
+      // cosine learning-rate decay
+      const float lr = (float)(tc->lr * 0.5 * (1.0 + std::cos(M_PI * (double)step / (double)total_steps)));
+      ++m->adam_t;
+      const float c1 = 1.f - std::pow(0.9f, (float)m->adam_t), c2 = 1.f - std::pow(0.999f, (float)m->adam_t);
+      adam_kernel<<<nblk(m->n_param), 256, 0, m->stream>>>(m->P, m->G, m->M1, m->M2, m->n_param, lr, 0.9f,
+                                                            0.999f, 1e-8f, c1, c2);
+      MCHK(m, refresh_bf16(m));
+      if (epoch_loss && (s == steps_per_epoch - 1 || s % 16 == 0)) {
+        MCHK(m, hipMemcpyAsync(lp.data(), m->lossp, lp.size() * 4, hipMemcpyDeviceToHost, m->stream));
+        MCHK(m, hipStreamSynchronize(m->stream));
+        eloss += std::accumulate(lp.begin(), lp.end(), 0.0) / B;
+      }
+    }
+    if (epoch_loss) epoch_loss[ep] = (float)(eloss / ((steps_per_epoch + 15) / 16));
+  }
+  MCHK(m, hipStreamSynchronize(m->stream));
+  return GTP_OK;
+}
+
+GTP_API gtp_status gtp_mlp_predict(gtp_mlp* m, const uint32_t* bits, int64_t n, float* out) {
+  if (!m || !bits || !out || n <= 0) return GTP_ERR_ARG;
+  MCHK(m, hipSetDevice(m->device));
+  const int64_t chunk = 1 << 18;
+  MCHK(m, m->reserve((int)std::min<int64_t>(n, chunk)));
+  for (int64_t b = 0; b < n; b += chunk) {
+    const int c = (int)std::min<int64_t>(chunk, n - b);
+    MCHK(m, hipMemcpyAsync(m->Xb, bits + b * m->words, (size_t)c * m->words * 4, hipMemcpyHostToDevice, m->stream));
+    MCHK(m, forward(m, m->Xb, c, false, m->out));
+    MCHK(m, hipMemcpyAsync(out + b, m->out, (size_t)c * 4, hipMemcpyDeviceToHost, m->stream));
+    MCHK(m, hipStreamSynchronize(m->stream));
+  }
+  for (int64_t i = 0; i < n; ++i) out[i] = out[i] * m->y_std + m->y_mean;
+  return GTP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device-resident paths over the context's event store
+// ---------------------------------------------------------------------------
+namespace {
+
+// Feature selection upload: net of each feature and the first 256-event block
+// of each feature's events.
+struct FeatureSel {
+  int32_t* d_net = nullptr;
+  int64_t* d_chunk = nullptr;
+  int64_t n_chunk = 0;
+  ~FeatureSel() {
+    if (d_net) (void)hipFree(d_net);
+    if (d_chunk) (void)hipFree(d_chunk);
+  }
+  hipError_t build(const TraceDev& t, int K, const int32_t* nets, hipStream_t s) {
+    std::vector<int64_t> co(K + 1, 0);
+    for (int k = 0; k < K; ++k) {
+      const int64_t n = nets[k] % t.n_net;
+      const int64_t cnt = t.host_net_off[n + 1] - t.host_net_off[n];
+      co[k + 1] = co[k] + (cnt + 255) / 256;
+    }
+    n_chunk = co[K];
+    hipError_t e = hipMalloc((void**)&d_net, (size_t)K * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&d_chunk, (size_t)(K + 1) * 8);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_net, nets, (size_t)K * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_chunk, co.data(), (size_t)(K + 1) * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
+  }
+  void launch(const TraceDev& t, int K, int64_t c0, int64_t n_cyc, int words, uint32_t* X, hipStream_t s) const {
+    fill_u32_kernel<<<nblk(n_cyc * words), 256, 0, s>>>(X, n_cyc * words, 0u);
+    if (n_chunk > 0)
+      toggle_bits_kernel<<<(unsigned)n_chunk, 256, 0, s>>>(t.net_off, t.ev, d_net, d_chunk, K, t.n_net,
+                                                           (uint32_t)t.T, (float)(1.0 / (double)t.T), c0, n_cyc,
+                                                           words, X);
+  }
+};
+
+// feature ids are kind * n_net + net, kind 0 (any change) or 1 (rise)
+bool check_nets(const TraceDev& t, int K, const int32_t* nets) {
+  for (int k = 0; k < K; ++k)
+    if (nets[k] < 0 || nets[k] >= 2 * t.n_net) return false;
+  return true;
+}
+
+}  // namespace
+
+// Pearson correlation of each feature's per-cycle bit with y over the cycles
+// where mask[c] != 0 (all cycles when mask is null); corr is [2][n_net]:
+// kind 0 (any change), kind 1 (rise).
+GTP_API gtp_status gtp_feature_correlation(gtp_ctx* ctx, const double* y, const uint8_t* mask, float* corr) {
+  TraceDev t;
+  gtp_status st = gtp::trace_dev(ctx, &t);
+  if (st != GTP_OK) return st;
+  if (!y || !corr) return GTP_ERR_ARG;
+  if (hipSetDevice(t.device) != hipSuccess) return GTP_ERR_DEVICE;
+  const int64_t C = t.n_cycle, N = t.n_net;
+  double mu = 0, amax = 0;
+  int64_t nv = 0;
+  for (int64_t c = 0; c < C; ++c)
+    if (!mask || mask[c]) {
+      mu += y[c];
+      amax = std::max(amax, std::fabs(y[c]));
+      ++nv;
+    }
+  if (nv == 0) return GTP_ERR_ARG;
+  mu /= (double)nv;
+  double var = 0;
+  for (int64_t c = 0; c < C; ++c)
+    if (!mask || mask[c]) var += (y[c] - mu) * (y[c] - mu);
+  const double sy = std::sqrt(var / (double)nv);
+  if (!(amax > 0)) amax = 1.0;
+  const double qs = std::ldexp(1.0, 30) / amax;  // |yq| <= 2^30: sums over 2^33 cycles fit
+  std::vector<long long> yq(C);
+  for (int64_t c = 0; c < C; ++c) yq[c] = std::llrint(y[c] * qs);
+  long long* d_yq = nullptr;
+  unsigned long long *d_cnt = nullptr, *d_sum = nullptr;
+  uint8_t* d_mask = nullptr;
+  hipError_t e = hipMalloc((void**)&d_yq, (size_t)C * 8);
+  if (e == hipSuccess) e = hipMalloc((void**)&d_cnt, (size_t)2 * N * 8);
+  if (e == hipSuccess) e = hipMalloc((void**)&d_sum, (size_t)2 * N * 8);
+  if (e == hipSuccess && mask) e = hipMalloc((void**)&d_mask, (size_t)C);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_yq, yq.data(), (size_t)C * 8, hipMemcpyHostToDevice, t.stream);
+  if (e == hipSuccess && mask) e = hipMemcpyAsync(d_mask, mask, (size_t)C, hipMemcpyHostToDevice, t.stream);
+  std::vector<unsigned long long> cnt(2 * N), sum(2 * N);
+  if (e == hipSuccess) {
+    fill_u32_kernel<<<nblk(4 * N), 256, 0, t.stream>>>((uint32_t*)d_cnt, 4 * N, 0u);
+    fill_u32_kernel<<<nblk(4 * N), 256, 0, t.stream>>>((uint32_t*)d_sum, 4 * N, 0u);
+    if (t.n_chunk > 0)
+      net_corr_kernel<<<(unsigned)t.n_chunk, 256, 0, t.stream>>>(t.net_off, t.ev, t.chunk_off, N, (uint32_t)t.T,
+                                                                 (float)(1.0 / (double)t.T), C, d_mask, d_yq,
+                                                                 d_cnt, d_sum);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), d_cnt, (size_t)2 * N * 8, hipMemcpyDeviceToHost, t.stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(sum.data(), d_sum, (size_t)2 * N * 8, hipMemcpyDeviceToHost, t.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(t.stream);
+  for (void* p : {(void*)d_yq, (void*)d_cnt, (void*)d_sum, (void*)d_mask})
+    if (p) (void)hipFree(p);
+  if (e != hipSuccess) return e == hipErrorOutOfMemory ? GTP_ERR_OOM : GTP_ERR_DEVICE;
+  for (int64_t n = 0; n < 2 * N; ++n) {
+    const double p = (double)cnt[n] / (double)nv;
+    const double mean_on = cnt[n] ? (double)(long long)sum[n] / qs / (double)cnt[n] : 0.0;
+    const double cov = p * (mean_on - mu);
+    const double sb = std::sqrt(p * (1 - p));
+    corr[n] = (sb > 0 && sy > 0) ? (float)(cov / (sb * sy)) : 0.f;
+  }
+  return GTP_OK;
+}
+
+GTP_API gtp_status gtp_toggle_features(gtp_ctx* ctx, int32_t n_sel, const int32_t* nets, int64_t cycle_begin,
+                                       int64_t n_cycle, uint32_t* bits) {
+  TraceDev t;
+  gtp_status st = gtp::trace_dev(ctx, &t);
+  if (st != GTP_OK) return st;
+  if (n_sel <= 0 || n_sel % 32 || !nets || !bits || cycle_begin < 0 || n_cycle <= 0 || !check_nets(t, n_sel, nets))
+    return GTP_ERR_ARG;
+  if (hipSetDevice(t.device) != hipSuccess) return GTP_ERR_DEVICE;
+  const int words = n_sel / 32;
+  FeatureSel fs;
+  hipError_t e = fs.build(t, n_sel, nets, t.stream);
+  uint32_t* d_X = nullptr;
+  if (e == hipSuccess) e = hipMalloc((void**)&d_X, (size_t)n_cycle * words * 4);
+  if (e == hipSuccess) {
+    fs.launch(t, n_sel, cycle_begin, n_cycle, words, d_X, t.stream);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(bits, d_X, (size_t)n_cycle * words * 4, hipMemcpyDeviceToHost, t.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(t.stream);
+  if (d_X) (void)hipFree(d_X);
+  return e == hipSuccess ? GTP_OK : (e == hipErrorOutOfMemory ? GTP_ERR_OOM : GTP_ERR_DEVICE);
+}
+
+// Surrogate inference straight from the resident event store: toggle bits of
+// the selected nets (device), then the MLP.  `ms` receives the device time of
+// features + inference.
+GTP_API gtp_status gtp_mlp_predict_trace(gtp_mlp* m, gtp_ctx* ctx, const int32_t* nets, int64_t cycle_begin,
+                                         int64_t n_cycle, float* out, double* ms) {
+  if (!m) return GTP_ERR_ARG;
+  TraceDev t;
+  gtp_status st = gtp::trace_dev(ctx, &t);
+  if (st != GTP_OK) return m->fail(st, "gtp_mlp_predict_trace: the context has no trace");
+  if (!nets || !out || n_cycle <= 0 || cycle_begin < 0 || !check_nets(t, m->K, nets))
+    return m->fail(GTP_ERR_ARG, "gtp_mlp_predict_trace: bad arguments");
+  MCHK(m, hipSetDevice(m->device));
+  const int words = m->words;
+  FeatureSel fs;
+  MCHK(m, fs.build(t, m->K, nets, m->stream));
+  uint32_t* d_X = nullptr;
+  float* d_out = nullptr;
+  hipError_t e = hipMalloc((void**)&d_X, (size_t)n_cycle * words * 4);
+  if (e == hipSuccess) e = hipMalloc((void**)&d_out, (size_t)n_cycle * 4);
+  const int64_t chunk = 1 << 18;
+  if (e == hipSuccess) e = m->reserve((int)std::min<int64_t>(n_cycle, chunk));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipEventRecord(e0, m->stream);
+  if (e == hipSuccess) {
+    fs.launch(t, m->K, cycle_begin, n_cycle, words, d_X, m->stream);
+    e = hipGetLastError();
+    for (int64_t b = 0; b < n_cycle && e == hipSuccess; b += chunk) {
+      const int c = (int)std::min<int64_t>(chunk, n_cycle - b);
+      e = forward(m, d_X + b * words, c, false, d_out + b);
+    }
+  }
+  if (e == hipSuccess) e = hipEventRecord(e1, m->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, (size_t)n_cycle * 4, hipMemcpyDeviceToHost, m->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  float tms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&tms, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (d_X) (void)hipFree(d_X);
+  if (d_out) (void)hipFree(d_out);
+  if (e != hipSuccess) return m->fail(GTP_ERR_DEVICE, std::string("gtp_mlp_predict_trace: ") + hipGetErrorString(e));
+  for (int64_t i = 0; i < n_cycle; ++i) out[i] = out[i] * m->y_std + m->y_mean;
+  if (ms) *ms = tms;
+  return GTP_OK;
+}
+
+// Device GEMM entry for layout tests: C[M][N] (fp32) = A[M][K] * B[K][N] with
+// A given as row-major bf16 bits (a_mode 0), bf16 stored [K][M] (1) or packed
+// bits [M][K/32] (2), B as row-major [K][N] (0), stored [N][K] (1) or packed
+// bits [K][N/32] (2).  All pointers are host arrays.
+GTP_API gtp_status gtp_mlp_gemm_test(int32_t a_mode, int32_t b_mode, int32_t M, int32_t N, int32_t K,
+                                     const void* A, int64_t a_bytes, const void* B, int64_t b_bytes, float* C) {
+  if (M <= 0 || N % 128 || K % 32 || !A || !B || !C) return GTP_ERR_ARG;
+  void *dA = nullptr, *dB = nullptr;
+  float* dC = nullptr;
+  hipStream_t s = nullptr;
+  if (hipMalloc(&dA, a_bytes) != hipSuccess || hipMalloc(&dB, b_bytes) != hipSuccess ||
+      hipMalloc((void**)&dC, (size_t)M * N * 4) != hipSuccess)
+    return GTP_ERR_OOM;
+  hipError_t e = hipMemcpy(dA, A, a_bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dB, B, b_bytes, hipMemcpyHostToDevice);
+  GemmArgs g{};
+  g.M = M; g.N = N; g.K = K;
+  g.A = dA; g.B = dB; g.C = dC; g.ldc = N;
+  g.lda = a_mode == 0 ? K : (a_mode == 1 ? M : K / 32);
+  g.ldb = b_mode == 0 ? N : (b_mode == 1 ? K : N / 32);
+  if (e == hipSuccess) {
+    if (a_mode == 0 && b_mode == 0) e = gemm<A_ROW, B_ROW, EPI_F32>(g, s);
+    else if (a_mode == 0 && b_mode == 1) e = gemm<A_ROW, B_TRANS, EPI_F32>(g, s);
+    else if (a_mode == 1 && b_mode == 0) e = gemm<A_COL, B_ROW, EPI_F32>(g, s);
+    else if (a_mode == 1 && b_mode == 2) e = gemm<A_COL, B_BITS_ROW, EPI_F32>(g, s);
+    else if (a_mode == 2 && b_mode == 1) e = gemm<A_BITS_ROW, B_TRANS, EPI_F32>(g, s);
+    else e = hipErrorInvalidValue;
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(C, dC, (size_t)M * N * 4, hipMemcpyDeviceToHost);
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  (void)hipFree(dC);
+  return e == hipSuccess ? GTP_OK : (e == hipErrorInvalidValue ? GTP_ERR_UNSUPPORTED : GTP_ERR_DEVICE);
+}

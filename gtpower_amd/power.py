@@ -105,6 +105,46 @@ class HipPower:
                                 tg, gl, r.n_cycle, r.kernel_ms, r.total_ms, sched)
         return self.last
 
+    @property
+    def n_cycle(self):
+        T = self.trace.ticks_per_cycle
+        return -(-self.trace.t_max // T)
+
+    def power_tiles(self, tile_cycles, tiles):
+        """Exact per-cycle power of the cycles of `tiles` (tile j = cycles
+        [j*tile_cycles, (j+1)*tile_cycles)): returns (cycles [n], waves [5, n], ms),
+        cycles past the end of the trace dropped."""
+        tiles = np.ascontiguousarray(tiles, dtype=np.int32)
+        n_out = len(tiles) * tile_cycles
+        cp = np.zeros(5 * n_out)
+        ms = C.c_double()
+        self._check(self._lib.gtp_run_tiles(self._h, int(tile_cycles), len(tiles), abi.ptr(tiles, C.c_int32),
+                                            abi.ptr(cp, C.c_double), C.byref(ms)), "gtp_run_tiles")
+        cyc = (tiles.astype(np.int64)[:, None] * tile_cycles + np.arange(tile_cycles)[None, :]).reshape(-1)
+        keep = cyc < self.n_cycle
+        return cyc[keep], cp.reshape(5, n_out)[:, keep], ms.value
+
+    def feature_correlation(self, y, mask=None):
+        """Pearson correlation of each feature's per-cycle bit with y, indexed by
+        feature id = kind * n_net + net (kind 0: value change, 1: rise)."""
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        assert y.shape == (self.n_cycle,)
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        corr = np.zeros(2 * self.design.n_net, np.float32)
+        self._check(self._lib.gtp_feature_correlation(self._h, abi.ptr(y, C.c_double), abi.ptr(m, C.c_uint8),
+                                                      abi.ptr(corr, C.c_float)), "gtp_feature_correlation")
+        return corr
+
+    def toggle_features(self, nets, cycle_begin=0, n_cycle=None):
+        """Per-cycle bits of the features `nets` (ids kind * n_net + net):
+        uint32 [n_cycle][len(nets)/32]."""
+        nets = np.ascontiguousarray(nets, dtype=np.int32)
+        n = self.n_cycle - cycle_begin if n_cycle is None else n_cycle
+        bits = np.zeros((n, len(nets) // 32), np.uint32)
+        self._check(self._lib.gtp_toggle_features(self._h, len(nets), abi.ptr(nets, C.c_int32), int(cycle_begin),
+                                                  int(n), abi.ptr(bits, C.c_uint32)), "gtp_toggle_features")
+        return bits
+
     def print_cell_res(self, path, gate_names=None):
         """Write cuda_cell_power.txt (power/cuda/CudaPower.cu:3134-3165)."""
         res = self.last

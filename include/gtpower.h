@@ -209,6 +209,16 @@ gtp_status gtp_set_trace(gtp_ctx* ctx, const gtp_trace* trace);
  * (:2996-3132). */
 gtp_status gtp_run(gtp_ctx* ctx, gtp_result* res);
 
+/* Exact per-cycle power of a subset of the cycles: the tiles tiles[0..n_tiles)
+ * of tile_cycles cycles each (tile j covers cycles [j*tile_cycles,
+ * (j+1)*tile_cycles)).  cycle_power is [GTP_N_WAVE][n_tiles * tile_cycles],
+ * column l*tile_cycles + i holding cycle tiles[l]*tile_cycles + i (0 past the
+ * last cycle); the values equal those gtp_run returns for the same cycles.
+ * This is the judge of the learned surrogate.  No counterpart in the
+ * reference (new method). */
+gtp_status gtp_run_tiles(gtp_ctx* ctx, int32_t tile_cycles, int32_t n_tiles, const int32_t* tiles,
+                         double* cycle_power, double* kernel_ms);
+
 /* Evaluate several traces of one design in one persistent launch.  traces[k]
  * and results[k] pair up.  No counterpart in the reference (new method). */
 gtp_status gtp_run_batch(gtp_ctx* ctx, int32_t n_trace, const gtp_trace* traces,
@@ -228,6 +238,77 @@ gtp_status gtp_get_schedule(const gtp_ctx* ctx, gtp_schedule_info* info);
 gtp_status gtp_reference_cycles_per_thread(int64_t n, const int64_t* n_event, int64_t n_cycle,
                                            int64_t e_target, int64_t parallelism_floor,
                                            int64_t* out_ncpt);
+
+/* ---------------------------------------------------------------------------
+ * Learned per-cycle power surrogate.  No counterpart in the reference (a new
+ * method on top of its path): an MLP over per-cycle toggle bits of selected
+ * nets, n_in -> n_hidden -> n_hidden -> 1 with GELU (x * Phi(x)), trained and
+ * run on the device with BF16 matrix-core kernels accumulating in FP32.
+ * Training targets come from gtp_run; gtp_run_tiles judges it.
+ *
+ * Bit layout: row c of a feature matrix is n_in / 32 words; bit k of the row
+ * is bit (k % 32) of word k / 32.
+ * Parameter layout (FP32): W1[n_hidden][n_in], b1[n_hidden],
+ * W2[n_hidden][n_hidden], b2[n_hidden], w3[n_hidden], b3[1].
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int32_t device;    /* HIP device ordinal; -1 = the current device */
+  int32_t n_in;      /* feature bits, a multiple of 128 */
+  int32_t n_hidden;  /* hidden width, a multiple of 128 */
+  int32_t pad_;
+  uint64_t seed;     /* initialisation seed */
+} gtp_mlp_config;
+
+typedef struct {
+  int32_t batch;     /* rows per step, a multiple of 32 */
+  int32_t epochs;
+  float lr;          /* Adam peak learning rate (cosine decay to 0) */
+  int32_t pad_;
+  uint64_t seed;     /* shuffling seed */
+} gtp_train_config;
+
+typedef struct gtp_mlp gtp_mlp;
+
+gtp_status gtp_mlp_create(const gtp_mlp_config* cfg, gtp_mlp** out);
+void gtp_mlp_destroy(gtp_mlp* m);
+const char* gtp_mlp_last_error(const gtp_mlp* m);
+int64_t gtp_mlp_param_count(const gtp_mlp* m);
+/* y_mean_std (may be NULL): the target standardisation, mean and std. */
+gtp_status gtp_mlp_get_params(gtp_mlp* m, float* params, float* y_mean_std);
+gtp_status gtp_mlp_set_params(gtp_mlp* m, const float* params, const float* y_mean_std);
+/* Gradients of the mean squared error over n rows (n a multiple of 32), with
+ * targets y already standardised; the model is not changed. */
+gtp_status gtp_mlp_gradients(gtp_mlp* m, const uint32_t* bits, const float* y, int32_t n,
+                             float* grads, float* loss);
+/* Train on n rows with raw targets y (standardised inside); epoch_loss
+ * (may be NULL) receives the mean standardised loss of each epoch. */
+gtp_status gtp_mlp_train(gtp_mlp* m, const uint32_t* bits, const float* y, int64_t n,
+                         const gtp_train_config* tc, float* epoch_loss);
+/* Predictions in target units for n rows of host feature bits. */
+gtp_status gtp_mlp_predict(gtp_mlp* m, const uint32_t* bits, int64_t n, float* out);
+/* Features are per-cycle toggle bits of (net, kind) pairs, identified as
+ * kind * n_net + net: kind GTP_FEAT_CHANGE sets the bit of a cycle in which
+ * the net changes value, kind GTP_FEAT_RISE of one in which it changes to 1. */
+enum { GTP_FEAT_CHANGE = 0, GTP_FEAT_RISE = 1, GTP_N_FEAT_KIND = 2 };
+/* Predictions for cycles [cycle_begin, +n_cycle) of the trace loaded in ctx,
+ * features built on the device from its event store (feats[k] = feature id
+ * of input k); ms (may be NULL) receives the device time of both. */
+gtp_status gtp_mlp_predict_trace(gtp_mlp* m, gtp_ctx* ctx, const int32_t* feats, int64_t cycle_begin,
+                                 int64_t n_cycle, float* out, double* ms);
+/* Pearson correlation of every feature's per-cycle bit with y[n_cycle], over
+ * the cycles with mask[c] != 0 (NULL: all); corr is [GTP_N_FEAT_KIND * n_net],
+ * indexed by feature id. */
+gtp_status gtp_feature_correlation(gtp_ctx* ctx, const double* y, const uint8_t* mask, float* corr);
+/* Bits of the features feats[0..n_sel) (n_sel a multiple of 32) for cycles
+ * [cycle_begin, +n_cycle) into bits[n_cycle][n_sel / 32]. */
+gtp_status gtp_toggle_features(gtp_ctx* ctx, int32_t n_sel, const int32_t* feats, int64_t cycle_begin,
+                               int64_t n_cycle, uint32_t* bits);
+/* The surrogate's matrix-core GEMM on host arrays, C[M][N] = A[M][K] B[K][N]
+ * in FP32: a_mode 0 A row-major bf16, 1 A stored [K][M] bf16, 2 A bits
+ * [M][K/32]; b_mode 0 B row-major bf16, 1 B stored [N][K] bf16, 2 B bits
+ * [K][N/32].  N a multiple of 128, K of 32.  For layout tests. */
+gtp_status gtp_mlp_gemm_test(int32_t a_mode, int32_t b_mode, int32_t M, int32_t N, int32_t K,
+                             const void* A, int64_t a_bytes, const void* B, int64_t b_bytes, float* C);
 
 /* Writers for the reference's two result files.
  * gtp_write_cell_power: `cuda_cell_power.txt` (power/cuda/CudaPower.cu:3134-3165),
