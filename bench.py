@@ -6,10 +6,20 @@ primary-input toggle p = 0.15, events timed by the NLDM delays (glitches
 included).  One step = one full evaluation of per-gate and per-cycle power of
 the whole trace (gtp_run) with the event store resident in HBM.
 
+Surrogate leg (on by default): an MLP over 4096 per-cycle toggle bits
+(4096 -> 256 -> 256 -> 1, GELU, BF16 matrix-core kernels, FP32 accumulation)
+is trained on the exact per-cycle power of the first 50k cycles of that trace,
+then run on a fresh 100k-cycle trace of the same netlist (new seed), with the
+exact engine recomputing 1% of its cycles as the judge (fallback to the exact
+answer above 2% NRMSE).  Its throughput, error and speedup over the exact path
+are reported in the "surrogate" object.
+
 Prints one JSON line (stdout); progress goes to stderr.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload exact_200k]
                     [--gates G] [--cycles C] [--cpu-sample-cycles S] [--no-cpu]
+                    [--no-surrogate] [--surrogate-cycles C]
+    python bench.py --copy-calibration MiB   # plain device copies, for counter scaling
 """
 import argparse
 import json
@@ -46,6 +56,76 @@ def algorithmic_bytes(design, trace):
     return per_cycle * C, per_cycle
 
 
+def copy_calibration(mib, reps=5):
+    """Plain device-to-device copies of a known size (torch's elementwise copy
+    kernel): the reference reading for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {reps} copies of {mib} MiB (read {mib} MiB + write {mib} MiB each)")
+
+
+def surrogate_leg(eng, design, exact_cycle_power, w, n_fresh, steps, exact_cps):
+    """Train on the first 50k exact cycles, infer on a fresh trace, judge 1%."""
+    import numpy as np
+    from gtpower_amd import synthetic as S
+    from gtpower_amd.surrogate import MlpSurrogate, SurrogatePower, cycle_total, nrmse
+
+    n_train = min(50000, exact_cycle_power.shape[1])
+    mask = np.zeros(exact_cycle_power.shape[1], bool)
+    mask[:n_train] = True
+
+    class _Exact:
+        cycle_power = exact_cycle_power
+
+    sp = SurrogatePower(MlpSurrogate(4096, 256, seed=3), threshold=0.02, judge_fraction=0.01)
+    t0 = time.perf_counter()
+    sp.fit(eng, train_mask=mask, exact=_Exact)
+    train_s = time.perf_counter() - t0
+    log(f"surrogate trained on {n_train} exact cycles in {train_s:.1f} s "
+        f"(best validation loss {float(np.nanmin(sp.train_loss[:, 1])):.4f} at epoch "
+        f"{int(np.nanargmin(sp.train_loss[:, 1]))}, {int((sp.feats >= design.n_net).sum())} rise features)")
+    eng.trace = None
+    tk = dict(w["trace"])
+    tk["n_cycle"] = n_fresh
+    tseed = S.MASTER_SEED ^ (w["seed"] * 0x9E3779B1)
+    t0 = time.perf_counter()
+    fresh = design.trace(seed=tseed + 1001, **tk)
+    log(f"fresh trace: {fresh.n_cycle} cycles, {fresh.n_event} events in {time.perf_counter() - t0:.1f} s")
+    eng.read_power_activities(fresh)
+    fresh.close()
+    outs = [sp.evaluate(eng) for _ in range(max(1, steps))]
+    out = outs[-1]
+    surr_ms = sum(o.surrogate_ms for o in outs) / len(outs)
+    judge_ms = sum(o.judge_ms for o in outs) / len(outs)
+    exact = eng.power(want=("cycle",))
+    y = cycle_total(exact.cycle_power)
+    pred, _ = sp.mlp.predict_trace(eng, sp.feats)
+    full = nrmse(pred, y)
+    r2 = 1.0 - float(np.mean((pred - y) ** 2) / y.var())
+    cps = y.size / ((surr_ms + judge_ms) * 1e-3)
+    flop = 2.0 * (4096 * 256 + 256 * 256 + 256) * y.size
+    log(f"surrogate: {surr_ms:.3f} ms + judge {judge_ms:.3f} ms for {y.size} cycles -> {cps:.4g} cycles/s; "
+        f"judged nrmse {out.nrmse:.4f}, full nrmse {full:.4f}, r2 {r2:.3f}, source {out.source}")
+    sp.mlp.close()
+    return {
+        "cycles_per_s": cps, "speedup_vs_exact": cps / exact_cps, "source": out.source,
+        "nrmse_judged": out.nrmse, "nrmse_full": full, "r2": r2,
+        "constant_predictor_nrmse": float(y.std() / y.mean()),
+        "nrmse_range": float(np.sqrt(np.mean((pred - y) ** 2)) / (y.max() - y.min())),
+        "judged_cycles": out.judged_cycles, "train_cycles": n_train, "infer_cycles": int(y.size),
+        "surrogate_ms": surr_ms, "judge_ms": judge_ms, "exact_ms_fresh": exact.kernel_ms,
+        "train_s": round(train_s, 2), "features": 4096, "hidden": [256, 256], "dtype": "bf16 (fp32 accumulate)",
+        "mfma": {"achieved": flop / (surr_ms * 1e-3) / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
+                 "frac": flop / (surr_ms * 1e-3) / 1e12 / 2500.0,
+                 "note": "features + inference window; FLOP = 2 (4096*256 + 256*256 + 256) per cycle"},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,12 +134,19 @@ def main():
     ap.add_argument("--workload", default="exact_200k")
     ap.add_argument("--gates", type=int, default=None)
     ap.add_argument("--cycles", type=int, default=None)
-    ap.add_argument("--cpu-sample-cycles", type=int, default=2000)
+    ap.add_argument("--cpu-sample-cycles", type=int, default=20000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-cycles", type=int, default=0)
     ap.add_argument("--gates-per-wg", type=int, default=0)
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--surrogate-cycles", type=int, default=100000)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path shards by gate or by "
@@ -104,12 +191,25 @@ def main():
         f"{value:.4g} cycles/s, total power {r.total_power:.6e} W")
     achieved = algo_bytes / (avg_kernel_ms * 1e-3) / 1e9
     sched = r.schedule
+    # HBM bytes per evaluation from the committed FETCH_SIZE / WRITE_SIZE
+    # profile of this workload (copy-calibrated), when the run is that workload
+    traffic, traffic_src = None, None
+    tf = REPO / "profiles" / f"power_eval_{args.workload}_traffic.json"
+    if tf.exists() and args.gates is None and args.cycles is None:
+        traffic = json.loads(tf.read_text())["traffic_bytes"]
+        traffic_src = str(tf.relative_to(REPO))
+    trace.close()
+
+    surrogate = None
+    if not args.no_surrogate:
+        surrogate = surrogate_leg(eng, design, r.cycle_power, w,
+                                  min(args.surrogate_cycles, args.cycles or args.surrogate_cycles),
+                                  3, value)
     eng.close()
 
     cpu = None
     if not args.no_cpu:
         import oracle
-        trace.close()
         sample = min(args.cpu_sample_cycles, n_cycle)
         tseed = S.MASTER_SEED ^ (w["seed"] * 0x9E3779B1)
         tk = dict(w["trace"])
@@ -142,10 +242,13 @@ def main():
                    "gates_per_wg": sched.get("gates_per_wg"), "kernel_ms": avg_kernel_ms,
                    "generate_s": round(gen_s, 2)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "algorithmic_bytes": algo_bytes, "bytes_per_cycle": bytes_per_cycle,
-                     "kernel": "power_tile_kernel"},
+                     "kernel": "power evaluation: glitch_annotate_kernel + power_cell_kernel<2..4> + "
+                               "power_tile_kernel (one window of concurrent launches, HIP events on the "
+                               "launching stream)"},
         "cpu_baseline": cpu,
+        "surrogate": surrogate,
     }
     print(json.dumps(out), flush=True)
 

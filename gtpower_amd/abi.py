@@ -94,7 +94,8 @@ class GtpMlpConfig(C.Structure):
 
 class GtpTrainConfig(C.Structure):
     _fields_ = [("batch", C.c_int32), ("epochs", C.c_int32), ("lr", C.c_float),
-                ("pad_", C.c_int32), ("seed", C.c_uint64)]
+                ("weight_decay", C.c_float), ("seed", C.c_uint64), ("val_rows", C.c_int32),
+                ("input_dropout", C.c_int32)]
 
 
 class GtpError(RuntimeError):
@@ -153,7 +154,9 @@ def load_engine():
     lib.gtp_mlp_param_count.restype = C.c_int64
     lib.gtp_mlp_get_params.argtypes = [C.c_void_p, p_f32, p_f32]
     lib.gtp_mlp_set_params.argtypes = [C.c_void_p, p_f32, p_f32]
-    lib.gtp_mlp_gradients.argtypes = [C.c_void_p, p_u32, p_f32, C.c_int32, p_f32, p_f32]
+    lib.gtp_keep_mask.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+    lib.gtp_keep_mask.restype = C.c_uint32
+    lib.gtp_mlp_gradients.argtypes = [C.c_void_p, p_u32, p_f32, C.c_int32, C.c_uint32, p_f32, p_f32]
     lib.gtp_mlp_train.argtypes = [C.c_void_p, p_u32, p_f32, C.c_int64, C.POINTER(GtpTrainConfig), p_f32]
     lib.gtp_mlp_predict.argtypes = [C.c_void_p, p_u32, C.c_int64, p_f32]
     lib.gtp_mlp_predict_trace.argtypes = [C.c_void_p, C.c_void_p, p_i32, C.c_int64, C.c_int64, p_f32, p_f64]

@@ -146,6 +146,7 @@ struct gtp_ctx {
   std::vector<int64_t> h_net_off;  // host copy of the event offsets
   bool aux_valid = false;          // d_aux holds the annotation of this trace
   DArr<int32_t> d_tile_list;
+  DArr<int32_t> d_tile_idx_sub;    // tile index of a subset run's tiles
 
   // schedule: ranges of range_gates gates; a range holds gates of one
   // specialised cell type, or generic gates
@@ -154,17 +155,23 @@ struct gtp_ctx {
     int np;            // pins (specialised) or MAXP of the generic kernel
     int32_t range_begin, range_count;
   };
-  std::vector<int32_t> order;
-  std::vector<Launch> launches;
-  int32_t range_gates = 0, n_range = 0;
-  int64_t order_T = 0;
-  DArr<int32_t> d_order;
+  struct OrderSet {
+    std::vector<int32_t> order;
+    std::vector<Launch> launches;
+    int32_t range_gates = 0, n_range = 0;
+    int64_t order_T = 0;
+    DArr<int32_t> d_order;
+    // per-work-tile partials of this schedule's runs
+    DArr<long long> d_pg;
+    DArr<uint32_t> d_pg_tog, d_pg_glc;
+    DArr<long long> d_cyc_part;
+  };
+  // two cached schedules: the full run's and the tile-subset run's (whose
+  // ranges are sized for its few tiles)
+  OrderSet orders[2];
   gtp_schedule_info sched{};
 
   // scratch and results
-  DArr<long long> d_pg;
-  DArr<uint32_t> d_pg_tog, d_pg_glc;
-  DArr<long long> d_cyc_part;
   DArr<double> d_cycle, d_gate, d_total, d_total_part;
   DArr<unsigned long long> d_gtog, d_gglc;
   DArr<unsigned int> d_maxbits;
@@ -651,7 +658,7 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
     pev += w;
   }
   x->P_ev = pev;
-  x->order.clear();
+  for (auto& o : x->orders) o.order.clear();
   x->have_trace = true;
   return GTP_OK;
 }
@@ -661,7 +668,7 @@ namespace {
 // Schedule: gates grouped by kernel (one group per single-output cell type
 // without `when` conditions, one generic group), each group sorted by work (pin
 // events), heaviest first, and padded with -1 to whole ranges.
-void build_order(gtp_ctx* x, int32_t range_gates) {
+void build_order(gtp_ctx* x, gtp_ctx::OrderSet& os, int32_t range_gates) {
   const int nc = (int)x->cells.size();
   std::vector<std::vector<int32_t>> by_cell(nc);
   std::vector<int32_t> generic;
@@ -681,21 +688,21 @@ void build_order(gtp_ctx* x, int32_t range_gates) {
     if (x->gate_work[p] != x->gate_work[q]) return x->gate_work[p] > x->gate_work[q];
     return p < q;
   };
-  x->order.clear();
-  x->launches.clear();
-  x->range_gates = range_gates;
+  os.order.clear();
+  os.launches.clear();
+  os.range_gates = range_gates;
   auto add = [&](std::vector<int32_t>& v, int cell, int np) {
     if (v.empty()) return;
     std::sort(v.begin(), v.end(), by_work);
     const int32_t nr = (int32_t)((v.size() + range_gates - 1) / range_gates);
-    const int32_t rb = (int32_t)(x->order.size() / range_gates);
-    x->order.insert(x->order.end(), v.begin(), v.end());
-    x->order.resize((size_t)(rb + nr) * range_gates, -1);
-    x->launches.push_back(gtp_ctx::Launch{cell, np, rb, nr});
+    const int32_t rb = (int32_t)(os.order.size() / range_gates);
+    os.order.insert(os.order.end(), v.begin(), v.end());
+    os.order.resize((size_t)(rb + nr) * range_gates, -1);
+    os.launches.push_back(gtp_ctx::Launch{cell, np, rb, nr});
   };
   for (int c = 0; c < nc; ++c) add(by_cell[c], c, x->cells[c].n_pin);
   add(generic, -1, generic_maxp <= MAXP_SMALL ? MAXP_SMALL : MAXP_LARGE);
-  x->n_range = (int32_t)(x->order.size() / range_gates);
+  os.n_range = (int32_t)(os.order.size() / range_gates);
 }
 
 }  // namespace
@@ -704,6 +711,7 @@ namespace {
 
 // Everything a power evaluation needs besides the launches themselves.
 struct RunPlan {
+  gtp_ctx::OrderSet* os = nullptr;
   int64_t TC = 0;
   int32_t n_tile = 0, RG = 0, n_range = 0;
   int block = 256;
@@ -714,8 +722,10 @@ struct RunPlan {
 };
 
 // Schedule parameters, order, tile index, fixed-point quanta and scratch.
-// tc_req > 0 asks for exactly that tile length.
-gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, RunPlan* rp) {
+// tc_req > 0 asks for exactly that tile length; n_work_tiles > 0 is the
+// number of tiles the launches will cover (a subset run), which sizes the
+// gate ranges and selects the subset run's cached schedule.
+gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, int32_t n_work_tiles, RunPlan* rp) {
   hipStream_t s = x->stream;
   int64_t TC = tc_req > 0 ? tc_req : (x->opt.tile_cycles > 0 ? x->opt.tile_cycles : 128);
   if (x->T < (1 << 24)) TC = std::min<int64_t>(TC, std::max<int64_t>(1, ((1 << 24) - 1) / x->T));
@@ -726,24 +736,30 @@ gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, RunPlan* rp) {
                                     std::to_string(TC) + ")");
   const int32_t n_tile = (int32_t)((x->C + TC - 1) / TC);
   // gates per range: enough workgroups to fill the GPU several times over
-  int32_t RG = x->opt.gates_per_wg;
+  const bool subset = n_work_tiles > 0;
+  const int32_t work_tiles = subset ? n_work_tiles : n_tile;
+  int32_t RG = subset ? 0 : x->opt.gates_per_wg;
   if (RG <= 0) {
-    const double want = (double)x->G * n_tile / 8192.0;
-    RG = 256;
+    const double want = (double)x->G * work_tiles / 8192.0;
+    RG = subset ? 64 : 256;
     while (RG < 4096 && RG * 2 <= want) RG *= 2;
   }
-  // the power kernel is compiled for at most 256 lanes per workgroup
-  const int block = x->opt.block_lanes > 0 ? std::min(256, std::max(64, (x->opt.block_lanes + 63) / 64 * 64)) : 256;
-  if (x->order.empty() || x->range_gates != RG || x->order_T != x->T) {
-    x->order_T = x->T;
-    build_order(x, RG);
-    HIPCHK(x, x->d_order.upload(x->order.data(), x->order.size(), s));
+  // the power kernel is compiled for at most 256 lanes per workgroup; a range
+  // narrower than that takes a narrower workgroup
+  int block = x->opt.block_lanes > 0 ? std::min(256, std::max(64, (x->opt.block_lanes + 63) / 64 * 64)) : 256;
+  block = std::min(block, std::max(64, RG));
+  gtp_ctx::OrderSet& os = x->orders[subset ? 1 : 0];
+  if (os.order.empty() || os.range_gates != RG || os.order_T != x->T) {
+    os.order_T = x->T;
+    build_order(x, os, RG);
+    HIPCHK(x, os.d_order.upload(os.order.data(), os.order.size(), s));
   }
-  const int64_t n_sched = (int64_t)x->order.size();
-  const int32_t n_range = x->n_range;
-  if (x->tile_idx_tc != TC || !x->d_tile_idx.p) {
+  const int64_t n_sched = (int64_t)os.order.size();
+  const int32_t n_range = os.n_range;
+  rp->os = &os;
+  if (!subset && (x->tile_idx_tc != TC || !x->d_tile_idx.p)) {
     HIPCHK(x, x->d_tile_idx.alloc((size_t)x->N * (n_tile + 1) + 1));
-    HIPCHK(x, launch_tile_index(x->d_net_off.p, x->d_ev.p, x->N, n_tile, TC * x->T, x->d_tile_idx.p, s));
+    HIPCHK(x, launch_tile_index(x->d_net_off.p, x->d_ev.p, x->N, n_tile, TC * x->T, nullptr, 0, x->d_tile_idx.p, s));
     x->tile_idx_tc = (int32_t)TC;
   }
   // fixed-point quanta: powers of two with one term <= 2^30 quanta, so a term
@@ -767,10 +783,10 @@ gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, RunPlan* rp) {
     HIPCHK(x, hipStreamSynchronize(s));
   }
   // scratch
-  HIPCHK(x, x->d_pg.alloc((size_t)N_PG * n_tile * n_sched));
-  HIPCHK(x, x->d_pg_tog.alloc((size_t)n_tile * n_sched));
-  HIPCHK(x, x->d_pg_glc.alloc((size_t)n_tile * n_sched));
-  HIPCHK(x, x->d_cyc_part.alloc((size_t)n_tile * n_range * N_BIN * TC));
+  HIPCHK(x, os.d_pg.alloc((size_t)N_PG * work_tiles * n_sched));
+  HIPCHK(x, os.d_pg_tog.alloc((size_t)work_tiles * n_sched));
+  HIPCHK(x, os.d_pg_glc.alloc((size_t)work_tiles * n_sched));
+  HIPCHK(x, os.d_cyc_part.alloc((size_t)work_tiles * n_range * N_BIN * TC));
   HIPCHK(x, x->d_cycle.alloc((size_t)GTP_N_WAVE * x->C));
   HIPCHK(x, x->d_gate.alloc((size_t)GTP_N_COMPONENT * x->G));
   HIPCHK(x, x->d_total.alloc(5 + 25));
@@ -780,7 +796,7 @@ gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, RunPlan* rp) {
 
   PowerArgs& a = rp->a;
   a = PowerArgs{};
-  a.order = x->d_order.p;
+  a.order = os.d_order.p;
   a.n_sched = n_sched;
   a.gate_cell = x->d_gate_cell.p;
   a.gate_pin_off = x->d_gate_pin_off.p;
@@ -815,10 +831,10 @@ gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, RunPlan* rp) {
   a.n_leak = (int32_t)x->leak_f.size();
   a.inv_q_energy = inv_q_e;
   a.inv_q_leak = inv_q_l;
-  a.pg = x->d_pg.p;
-  a.pg_tog = x->d_pg_tog.p;
-  a.pg_glitch = x->d_pg_glc.p;
-  a.cyc_part = x->d_cyc_part.p;
+  a.pg = os.d_pg.p;
+  a.pg_tog = os.d_pg_tog.p;
+  a.pg_glitch = os.d_pg_glc.p;
+  a.cyc_part = os.d_cyc_part.p;
   rp->lds = (size_t)N_BIN * TC * 8 + (size_t)a.n_leak * 8 + (size_t)(a.n_leak + (a.n_leak & 1)) * 4 +
             (size_t)a.n_cell * sizeof(CellDev);
   if (rp->lds > 160 * 1024) return x->fail(GTP_ERR_LIMIT, "gtp_run: tile needs more LDS than a workgroup has");
@@ -842,14 +858,15 @@ gtp_status launch_groups(gtp_ctx* x, const RunPlan& rp) {
   hipStream_t s = x->stream;
   HIPCHK(x, hipEventRecord(x->fork, s));
   for (int k = 0; k < gtp_ctx::N_SIDE; ++k) HIPCHK(x, hipStreamWaitEvent(x->side[k], x->fork, 0));
-  std::vector<size_t> idx(x->launches.size());
+  const std::vector<gtp_ctx::Launch>& launches = rp.os->launches;
+  std::vector<size_t> idx(launches.size());
   std::iota(idx.begin(), idx.end(), 0);
   std::sort(idx.begin(), idx.end(), [&](size_t p, size_t q) {
-    return x->launches[p].range_count > x->launches[q].range_count;
+    return launches[p].range_count > launches[q].range_count;
   });
   int k = 0;
   for (size_t li : idx) {
-    const gtp_ctx::Launch& L = x->launches[li];
+    const gtp_ctx::Launch& L = launches[li];
     hipStream_t ls = (k == 0) ? s : x->side[k - 1];
     k = (k + 1) % (gtp_ctx::N_SIDE + 1);
     const int64_t nw = (int64_t)rp.a.n_work_tiles * L.range_count;
@@ -886,7 +903,7 @@ GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
   for (hipEvent_t& e : ev.e) HIPCHK(x, hipEventCreate(&e));
   HIPCHK(x, hipEventRecord(ev.e[0], s));
   RunPlan rp;
-  gtp_status st = prepare_run(x, 0, &rp);
+  gtp_status st = prepare_run(x, 0, 0, &rp);
   if (st != GTP_OK) return st;
   const PowerArgs& a = rp.a;
   HIPCHK(x, hipEventRecord(ev.e[1], s));
@@ -895,9 +912,9 @@ GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
   x->aux_valid = true;
   if ((st = launch_groups(x, rp)) != GTP_OK) return st;
   HIPCHK(x, hipEventRecord(ev.e[2], s));
-  HIPCHK(x, launch_cycle_reduce(x->d_cyc_part.p, rp.n_range, (int32_t)rp.TC, x->C, x->T, x->t_max, rp.q_e, rp.q_l,
+  HIPCHK(x, launch_cycle_reduce(rp.os->d_cyc_part.p, rp.n_range, (int32_t)rp.TC, x->C, x->T, x->t_max, rp.q_e, rp.q_l,
                                 1.0 / x->period, x->d_cycle.p, s));
-  HIPCHK(x, launch_gate_reduce(x->d_pg.p, x->d_pg_tog.p, x->d_pg_glc.p, x->d_order.p, rp.n_sched, rp.n_tile, x->G,
+  HIPCHK(x, launch_gate_reduce(rp.os->d_pg.p, rp.os->d_pg_tog.p, rp.os->d_pg_glc.p, rp.os->d_order.p, rp.n_sched, rp.n_tile, x->G,
                                rp.q_e / ((double)x->t_max * x->scale), rp.q_l / (double)x->t_max, x->d_gate.p,
                                x->d_gtog.p, x->d_gglc.p, s));
   HIPCHK(x, launch_total(x->d_gate.p, x->d_gate_class.p, x->G, x->d_total_part.p, x->d_total.p, s));
@@ -927,7 +944,7 @@ GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
   si.gates_per_wg = rp.RG;
   si.n_unit = (int64_t)rp.n_tile * x->G;
   si.n_workgroup = (int64_t)rp.n_tile * rp.n_range;
-  si.split_target = (int32_t)x->launches.size();
+  si.split_target = (int32_t)rp.os->launches.size();
   si.n_tile = rp.n_tile;
   si.n_event = x->E;
   si.n_pin_event = x->P_ev;
@@ -944,7 +961,7 @@ GTP_API gtp_status gtp_run_tiles(gtp_ctx* x, int32_t tile_cycles, int32_t n_list
   HIPCHK(x, hipSetDevice(x->device));
   hipStream_t s = x->stream;
   RunPlan rp;
-  gtp_status st = prepare_run(x, tile_cycles, &rp);
+  gtp_status st = prepare_run(x, tile_cycles, n_list, &rp);
   if (st != GTP_OK) return st;
   for (int32_t l = 0; l < n_list; ++l)
     if (tiles[l] < 0 || tiles[l] >= rp.n_tile)
@@ -954,17 +971,21 @@ GTP_API gtp_status gtp_run_tiles(gtp_ctx* x, int32_t tile_cycles, int32_t n_list
   EvSet ev;
   for (hipEvent_t& e : ev.e) HIPCHK(x, hipEventCreate(&e));
   HIPCHK(x, hipEventRecord(ev.e[0], s));
-  if (!x->aux_valid) {
-    HIPCHK(x, launch_glitch_annotate(x->d_net_off.p, x->d_ev.p, x->d_chunk_off.p, x->N, x->n_chunk,
-                                     (uint32_t)x->T, rp.a.inv_T, x->d_aux.p, s));
-    x->aux_valid = true;
-  }
+  // tile index of the listed tiles only
+  HIPCHK(x, x->d_tile_idx_sub.alloc((size_t)x->N * n_list + 1));
+  HIPCHK(x, launch_tile_index(x->d_net_off.p, x->d_ev.p, x->N, rp.n_tile, rp.TC * x->T, x->d_tile_list.p, n_list,
+                              x->d_tile_idx_sub.p, s));
+  // glitch annotation of the listed tiles' events, unless the whole trace is annotated
+  if (!x->aux_valid)
+    HIPCHK(x, launch_glitch_annotate_tiles(x->d_net_off.p, x->d_ev.p, x->d_tile_idx_sub.p, x->d_tile_list.p, n_list,
+                                           x->N, rp.TC * x->T, (uint32_t)x->T, rp.a.inv_T, x->d_aux.p, s));
+  rp.a.tile_idx = x->d_tile_idx_sub.p;
   rp.a.tile_list = x->d_tile_list.p;
   rp.a.n_work_tiles = n_list;
   if ((st = launch_groups(x, rp)) != GTP_OK) return st;
   const int64_t n_out = (int64_t)n_list * rp.TC;
   HIPCHK(x, x->d_cycle.alloc((size_t)GTP_N_WAVE * std::max<int64_t>(n_out, x->C)));
-  HIPCHK(x, launch_cycle_reduce_list(x->d_cyc_part.p, rp.n_range, (int32_t)rp.TC, x->d_tile_list.p, n_list, x->C,
+  HIPCHK(x, launch_cycle_reduce_list(rp.os->d_cyc_part.p, rp.n_range, (int32_t)rp.TC, x->d_tile_list.p, n_list, x->C,
                                      x->T, x->t_max, rp.q_e, rp.q_l, 1.0 / x->period, x->d_cycle.p, s));
   HIPCHK(x, hipEventRecord(ev.e[1], s));
   HIPCHK(x, hipMemcpyAsync(cycle_power, x->d_cycle.p, sizeof(double) * GTP_N_WAVE * n_out, hipMemcpyDeviceToHost, s));

@@ -85,7 +85,8 @@ struct PowerArgs {
   const int64_t* net_off;        // [N+1]
   const uint32_t* ev;            // [E]
   const uint32_t* aux;           // [E] glitch pulse width of each event (0 = not a glitch)
-  const int32_t* tile_idx;       // [N * (n_tile + 1)] first event index >= tile start (net-relative)
+  const int32_t* tile_idx;       // [N][n_tile + 1] first event index >= tile start (net-relative);
+                                 // [N][n_work_tiles] by list position when tile_list is set
   int32_t n_tile;
   int32_t tile_cycles;
   int64_t T;                     // ticks per cycle
@@ -108,10 +109,11 @@ struct PowerArgs {
   float inv_q_energy;
   float inv_q_leak;
   // outputs
-  long long* pg;                 // [N_PG][n_tile][n_sched] per-gate sums in quanta (leakage: quanta * ticks)
-  uint32_t* pg_tog;              // [n_tile][n_sched]
-  uint32_t* pg_glitch;           // [n_tile][n_sched]
-  long long* cyc_part;           // [n_tile][n_range][N_BIN][tile_cycles]
+  // (partials are indexed by work tile: the tile itself, or its position in tile_list)
+  long long* pg;                 // [N_PG][n_work_tiles][n_sched] per-gate sums in quanta (leakage: quanta * ticks)
+  uint32_t* pg_tog;              // [n_work_tiles][n_sched]
+  uint32_t* pg_glitch;           // [n_work_tiles][n_sched]
+  long long* cyc_part;           // [n_work_tiles][n_range][N_BIN][tile_cycles]
 };
 
 // Arguments of the energy (table interpolation) kernel.
@@ -137,7 +139,8 @@ struct EnergyArgs {
 // Launch wrappers, defined with the kernels.
 hipError_t launch_pack(const int64_t* t, const uint8_t* v, uint32_t* out, int64_t n, hipStream_t s);
 hipError_t launch_tile_index(const int64_t* net_off, const uint32_t* ev, int64_t n_net, int32_t n_tile,
-                             int64_t tile_ticks, int32_t* out, hipStream_t s);
+                             int64_t tile_ticks, const int32_t* tile_list, int32_t n_list, int32_t* out,
+                             hipStream_t s);
 hipError_t launch_energy(const EnergyArgs& a, hipStream_t s);
 hipError_t launch_power(int maxp, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,
                         int32_t range_begin, int32_t range_count);
@@ -157,6 +160,9 @@ constexpr int BIN_COPIES = 4;  // lane-interleaved copies of the per-cycle bins 
 hipError_t launch_glitch_annotate(const int64_t* net_off, const uint32_t* ev, const int64_t* chunk_off,
                                   int64_t n_net, int64_t n_chunk, uint32_t T, float invT, uint32_t* aux,
                                   hipStream_t s);
+hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, const uint32_t* ev, const int32_t* idx_begin,
+                                        const int32_t* tile_list, int32_t n_list, int64_t n_net, int64_t tile_ticks,
+                                        uint32_t T, float invT, uint32_t* aux, hipStream_t s);
 hipError_t launch_total(const double* gate_power, const int32_t* gate_class, int64_t n_gate, double* part,
                         double* out, hipStream_t s);
 hipError_t launch_power_cell(int np, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,

@@ -80,15 +80,19 @@ __global__ void pack_events_kernel(const int64_t* __restrict__ t, const uint8_t*
   for (; i < n; i += stride) out[i] = ((uint32_t)t[i] << 2) | (uint32_t)(v[i] & 3u);
 }
 
+// out[n][k] = first event of net n at or after the start of tile k, for
+// k < n_tile, and the net's event count for k == n_tile; with a tile list,
+// out[n][l] for the tile tile_list[l] (no end column).
 __global__ void tile_index_kernel(const int64_t* __restrict__ net_off, const uint32_t* __restrict__ ev,
                                   int64_t n_net, int32_t n_tile, int64_t tile_ticks,
+                                  const int32_t* __restrict__ tile_list, int32_t n_list,
                                   int32_t* __restrict__ out) {
-  const int64_t per = (int64_t)n_tile + 1;
+  const int64_t per = tile_list ? (int64_t)n_list : (int64_t)n_tile + 1;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (; i < n_net * per; i += stride) {
     const int64_t n = i / per;
-    const int32_t k = (int32_t)(i - n * per);
+    const int32_t k = tile_list ? tile_list[i - n * per] : (int32_t)(i - n * per);
     const int64_t b = net_off[n];
     const int32_t cnt = (int32_t)(net_off[n + 1] - b);
     int32_t lo = 0, hi = cnt;
@@ -245,7 +249,9 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
   const uint32_t tend = last_tile ? EV_TIME_INF : (uint32_t)((tc0 + TC) * (int64_t)T);
   const uint32_t lend = last_tile ? (uint32_t)a.t_max : tend;
   const uint32_t t_max = (uint32_t)a.t_max;
-  const int64_t tile_stride = (int64_t)a.n_tile + 1;
+  // tile index: [net][n_tile + 1] by tile, or [net][n_work_tiles] by list position
+  const int64_t tile_stride = a.tile_list ? (int64_t)a.n_work_tiles : (int64_t)a.n_tile + 1;
+  const int tile_col = a.tile_list ? wt : tile;
 
   long long base_acc = 0;
   const int64_t gi_begin = (int64_t)range * a.range_gates;
@@ -297,7 +303,7 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
           cv[p] = a.pin_default[pb + p];
         } else {
           evp[p] = a.ev + nb;
-          const int32_t i0 = a.tile_idx[(int64_t)net * tile_stride + tile];
+          const int32_t i0 = a.tile_idx[(int64_t)net * tile_stride + tile_col];
           const bool isclk = (cd.clock_mask >> p) & 1u;
           if (i0 == 0) {
             const uint32_t e0 = evp[p][0];
@@ -503,8 +509,8 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
     }
     flush_pending();
     if (lend > prev_t) ff_add_prod(acc_hi[PG_LEAK], acc_lo[PG_LEAK], s_leak[loff + sb], (float)(lend - prev_t));
-    const int64_t slot = (int64_t)tile * a.n_sched + gi;
-    const int64_t plane = (int64_t)a.n_tile * a.n_sched;
+    const int64_t slot = (int64_t)wt * a.n_sched + gi;
+    const int64_t plane = (int64_t)a.n_work_tiles * a.n_sched;
 #pragma unroll
     for (int k = 0; k < N_PG; ++k) {
       const float iq = k == PG_LEAK ? a.inv_q_leak : a.inv_q_energy;  // powers of two: exact scaling
@@ -528,7 +534,7 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
     }
   }
   __syncthreads();
-  long long* out = a.cyc_part + ((int64_t)tile * a.n_range + range) * N_BIN * TC;
+  long long* out = a.cyc_part + ((int64_t)wt * a.n_range + range) * N_BIN * TC;
   for (int i = threadIdx.x; i < N_BIN * TC; i += blockDim.x) out[i] = bins[i];
 }
 
@@ -546,6 +552,45 @@ __device__ __forceinline__ int group_pos(const uint32_t* ev, int64_t i, uint32_t
   int j = 0;
   for (int64_t k = i - 1; k >= 1 && ev_time(ev[k]) >= cstart; --k) ++j;
   return j;
+}
+
+// aux value of event i (of cnt) of one net's event list e
+__device__ __forceinline__ uint32_t annotate_one(const uint32_t* e, int64_t i, int64_t cnt, uint32_t T, float invT) {
+  if (i < 1) return 0;
+  const uint32_t t = ev_time(e[i]);
+  const uint32_t c = cycle_of(t, T, invT);
+  const uint32_t cs = c * T, ce = cs + T;
+  const int j = group_pos(e, i, cs);
+  const bool has1 = i + 1 < cnt;
+  const uint32_t t1 = has1 ? ev_time(e[i + 1]) : 0;
+  const bool last = !has1 || t1 >= ce;
+  if (!(!last || (j & 1))) return 0;
+  int64_t pw = -1;
+  if (i - 1 >= 1) {
+    bool fprev;
+    if (j > 0) {
+      fprev = true;  // same group and not its last event
+    } else {
+      const uint32_t tp = ev_time(e[i - 1]);
+      const uint32_t cps = cycle_of(tp, T, invT) * T;
+      fprev = group_pos(e, i - 1, cps) & 1;  // last of its group: flagged iff at an odd position
+    }
+    if (fprev) pw = t - ev_time(e[i - 1]);
+  }
+  if (has1) {
+    bool fnext;
+    const bool has2 = i + 2 < cnt;
+    const uint32_t t2 = has2 ? ev_time(e[i + 2]) : 0;
+    if (!last) {
+      const bool last1 = !has2 || t2 >= ce;
+      fnext = !last1 || ((j + 1) & 1);
+    } else {
+      const uint32_t c1e = (cycle_of(t1, T, invT) + 1) * T;
+      fnext = has2 && t2 < c1e;
+    }
+    if (fnext) pw = max(pw, (int64_t)(t1 - t));
+  }
+  return pw > 0 ? (uint32_t)pw : 0u;
 }
 
 __global__ __launch_bounds__(256) void glitch_annotate_kernel(const int64_t* __restrict__ net_off,
@@ -569,46 +614,27 @@ __global__ __launch_bounds__(256) void glitch_annotate_kernel(const int64_t* __r
   const int64_t cnt = net_off[n + 1] - b;
   const int64_t i = (chunk - chunk_off[n]) * 256 + threadIdx.x;
   if (i >= cnt) return;
+  aux[b + i] = annotate_one(ev + b, i, cnt, T, invT);
+}
+
+// The same for the events of listed tiles only: thread (n, l) annotates the
+// events of net n from idx_begin[n][l] up to (not including) the first event
+// at or after the end of tile tile_list[l].
+__global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_t* __restrict__ net_off,
+                                                                    const uint32_t* __restrict__ ev,
+                                                                    const int32_t* __restrict__ idx_begin,
+                                                                    const int32_t* __restrict__ tile_list,
+                                                                    int32_t n_list, int64_t n_net, int64_t tile_ticks,
+                                                                    uint32_t T, float invT,
+                                                                    uint32_t* __restrict__ aux) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_net * n_list) return;
+  const int64_t n = i / n_list;
+  const int64_t b = net_off[n];
+  const int64_t cnt = net_off[n + 1] - b;
+  const int64_t tend = ((int64_t)tile_list[i - n * n_list] + 1) * tile_ticks;
   const uint32_t* e = ev + b;
-  uint32_t out = 0;
-  if (i >= 1) {
-    const uint32_t t = ev_time(e[i]);
-    const uint32_t c = cycle_of(t, T, invT);
-    const uint32_t cs = c * T, ce = cs + T;
-    const int j = group_pos(e, i, cs);
-    const bool has1 = i + 1 < cnt;
-    const uint32_t t1 = has1 ? ev_time(e[i + 1]) : 0;
-    const bool last = !has1 || t1 >= ce;
-    if (!last || (j & 1)) {
-      int64_t pw = -1;
-      if (i - 1 >= 1) {
-        bool fprev;
-        if (j > 0) {
-          fprev = true;  // same group and not its last event
-        } else {
-          const uint32_t tp = ev_time(e[i - 1]);
-          const uint32_t cps = cycle_of(tp, T, invT) * T;
-          fprev = group_pos(e, i - 1, cps) & 1;  // last of its group: flagged iff at an odd position
-        }
-        if (fprev) pw = t - ev_time(e[i - 1]);
-      }
-      if (has1) {
-        bool fnext;
-        const bool has2 = i + 2 < cnt;
-        const uint32_t t2 = has2 ? ev_time(e[i + 2]) : 0;
-        if (!last) {
-          const bool last1 = !has2 || t2 >= ce;
-          fnext = !last1 || ((j + 1) & 1);
-        } else {
-          const uint32_t c1e = (cycle_of(t1, T, invT) + 1) * T;
-          fnext = has2 && t2 < c1e;
-        }
-        if (fnext) pw = max(pw, (int64_t)(t1 - t));
-      }
-      out = pw > 0 ? (uint32_t)pw : 0u;
-    }
-  }
-  aux[b + i] = out;
+  for (int64_t k = idx_begin[i]; k < cnt && (int64_t)ev_time(e[k]) < tend; ++k) aux[b + k] = annotate_one(e, k, cnt, T, invT);
 }
 
 // ---------------------------------------------------------------------------
@@ -667,7 +693,9 @@ __global__ __launch_bounds__(256) void power_cell_kernel(PowerArgs a, int32_t ra
   const uint32_t tend = last_tile ? EV_TIME_INF : (uint32_t)((tc0 + TC) * (int64_t)T);
   const uint32_t lend = last_tile ? (uint32_t)a.t_max : tend;
   const uint32_t t_max = (uint32_t)a.t_max;
-  const int64_t tile_stride = (int64_t)a.n_tile + 1;
+  // tile index: [net][n_tile + 1] by tile, or [net][n_work_tiles] by list position
+  const int64_t tile_stride = a.tile_list ? (int64_t)a.n_work_tiles : (int64_t)a.n_tile + 1;
+  const int tile_col = a.tile_list ? wt : tile;
   const float inv_qe = a.inv_q_energy, inv_ql = a.inv_q_leak;
   const int64_t aux_delta = a.aux - a.ev;
 
@@ -729,7 +757,7 @@ __global__ __launch_bounds__(256) void power_cell_kernel(PowerArgs a, int32_t ra
       } else {
         const uint32_t* evp = a.ev + nb;
         endp[p] = evp + cn;
-        const int32_t i0 = a.tile_idx[(int64_t)net * tile_stride + tile];
+        const int32_t i0 = a.tile_idx[(int64_t)net * tile_stride + tile_col];
         int32_t i = i0;
         if (i0 == 0) {
           const uint32_t e0 = evp[0];
@@ -887,8 +915,8 @@ __global__ __launch_bounds__(256) void power_cell_kernel(PowerArgs a, int32_t ra
     }
     if (lend > prev_t) acc[PG_LEAK] += (long long)(int)s_leakq[sb] * (long long)(lend - prev_t);
     flush();
-    const int64_t slot = (int64_t)tile * a.n_sched + gi;
-    const int64_t plane = (int64_t)a.n_tile * a.n_sched;
+    const int64_t slot = (int64_t)wt * a.n_sched + gi;
+    const int64_t plane = (int64_t)a.n_work_tiles * a.n_sched;
 #pragma unroll
     for (int k = 0; k < N_PG; ++k) a.pg[k * plane + slot] = acc[k];
     a.pg_tog[slot] = tog;
@@ -913,7 +941,7 @@ __global__ __launch_bounds__(256) void power_cell_kernel(PowerArgs a, int32_t ra
     }
   }
   __syncthreads();
-  long long* out = a.cyc_part + ((int64_t)tile * a.n_range + range) * N_BIN * TC;
+  long long* out = a.cyc_part + ((int64_t)wt * a.n_range + range) * N_BIN * TC;
   for (int i = threadIdx.x; i < BW; i += blockDim.x) out[i] = bins[i];
 }
 
@@ -946,8 +974,9 @@ __global__ void cycle_reduce_kernel(const long long* __restrict__ part, int32_t 
   out[4 * n_cycle + c] = (double)s[BIN_GSW] * q_energy * inv_period;
 }
 
-// cycle_reduce_kernel over the tiles of a list only: output column
-// l * tile_cycles + i is cycle tile_list[l] * tile_cycles + i (0 past the end)
+// cycle_reduce_kernel over the tiles of a list only (partials indexed by list
+// position): output column l * tile_cycles + i is cycle
+// tile_list[l] * tile_cycles + i (0 past the end)
 __global__ void cycle_reduce_list_kernel(const long long* __restrict__ part, int32_t n_range,
                                          int32_t tile_cycles, const int32_t* __restrict__ tile_list,
                                          int32_t n_list, int64_t n_cycle, int64_t T, int64_t t_max,
@@ -956,7 +985,8 @@ __global__ void cycle_reduce_list_kernel(const long long* __restrict__ part, int
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t n_out = (int64_t)n_list * tile_cycles;
   if (j >= n_out) return;
-  const int64_t tile = tile_list[j / tile_cycles];
+  const int64_t l = j / tile_cycles;
+  const int64_t tile = tile_list[l];
   const int ci = (int)(j % tile_cycles);
   const int64_t c = tile * tile_cycles + ci;
   if (c >= n_cycle) {
@@ -966,7 +996,7 @@ __global__ void cycle_reduce_list_kernel(const long long* __restrict__ part, int
   }
   long long s[N_BIN] = {0, 0, 0, 0, 0, 0};
   for (int r = 0; r < n_range; ++r) {
-    const long long* p = part + ((tile * n_range + r) * N_BIN) * tile_cycles + ci;
+    const long long* p = part + ((l * n_range + r) * N_BIN) * tile_cycles + ci;
 #pragma unroll
     for (int k = 0; k < N_BIN; ++k) s[k] += p[(int64_t)k * tile_cycles];
   }
@@ -1053,11 +1083,13 @@ hipError_t launch_pack(const int64_t* t, const uint8_t* v, uint32_t* out, int64_
 }
 
 hipError_t launch_tile_index(const int64_t* net_off, const uint32_t* ev, int64_t n_net, int32_t n_tile,
-                             int64_t tile_ticks, int32_t* out, hipStream_t s) {
-  const int64_t n = n_net * ((int64_t)n_tile + 1);
+                             int64_t tile_ticks, const int32_t* tile_list, int32_t n_list, int32_t* out,
+                             hipStream_t s) {
+  const int64_t n = n_net * (tile_list ? (int64_t)n_list : (int64_t)n_tile + 1);
   if (n <= 0) return hipSuccess;
   const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
-  tile_index_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(net_off, ev, n_net, n_tile, tile_ticks, out);
+  tile_index_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(net_off, ev, n_net, n_tile, tile_ticks, tile_list,
+                                                                n_list, out);
   return hipGetLastError();
 }
 
@@ -1121,6 +1153,16 @@ hipError_t launch_glitch_annotate(const int64_t* net_off, const uint32_t* ev, co
                                   hipStream_t s) {
   if (n_chunk <= 0) return hipSuccess;
   glitch_annotate_kernel<<<dim3((unsigned)n_chunk), dim3(256), 0, s>>>(net_off, ev, chunk_off, n_net, T, invT, aux);
+  return hipGetLastError();
+}
+
+hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, const uint32_t* ev, const int32_t* idx_begin,
+                                        const int32_t* tile_list, int32_t n_list, int64_t n_net, int64_t tile_ticks,
+                                        uint32_t T, float invT, uint32_t* aux, hipStream_t s) {
+  const int64_t n = n_net * n_list;
+  if (n <= 0) return hipSuccess;
+  glitch_annotate_tiles_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(
+      net_off, ev, idx_begin, tile_list, n_list, n_net, tile_ticks, T, invT, aux);
   return hipGetLastError();
 }
 

@@ -58,7 +58,22 @@ struct GemmArgs {
   const float* bias;
   const float* zin; // EPI_GELU_BACK: pre-activation
   int ldz;
+  // training-time input dropout on bit operands: a bit of row r, word w is
+  // kept when bit b of keep_mask(drop_seed, r, w) is set (probability 1/2) and
+  // then counts 2 (inverted dropout); drop_seed == 0: no dropout
+  uint32_t drop_seed;
 };
+
+// 32 pseudo-random keep bits for (seed, row, word): a murmur3-style finaliser
+__host__ __device__ __forceinline__ uint32_t keep_mask(uint32_t seed, uint32_t row, uint32_t word) {
+  uint32_t h = seed ^ (row * 0x9E3779B1u) ^ (word * 0x85EBCA77u);
+  h ^= h >> 16;
+  h *= 0x7FEB352Du;
+  h ^= h >> 15;
+  h *= 0x846CA68Bu;
+  h ^= h >> 16;
+  return h;
+}
 
 __device__ __forceinline__ bf16 f2bf(float x) { return (bf16)x; }
 __device__ __forceinline__ float bf2f(bf16 x) { return (float)x; }
@@ -68,13 +83,15 @@ __device__ __forceinline__ float gelu_grad(float z) {
   return 0.5f * (1.f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
-// 8 bits -> 8 bf16 zeros and ones (1.0 = 0x3F80), as one 16-byte vector
-__device__ __forceinline__ uint4 expand8(uint32_t b) {
+// 8 bits -> 8 bf16 zeros and `one`s (bf16 bits of 1.0 = 0x3F80, of 2.0 = 0x4000),
+// as one 16-byte vector
+__device__ __forceinline__ uint4 expand8(uint32_t b, uint32_t one = 0x3F80u) {
+  const uint32_t hi = one << 16;
   uint4 v;
-  v.x = ((b & 1u) ? 0x3F80u : 0u) | ((b & 2u) ? 0x3F800000u : 0u);
-  v.y = ((b & 4u) ? 0x3F80u : 0u) | ((b & 8u) ? 0x3F800000u : 0u);
-  v.z = ((b & 16u) ? 0x3F80u : 0u) | ((b & 32u) ? 0x3F800000u : 0u);
-  v.w = ((b & 64u) ? 0x3F80u : 0u) | ((b & 128u) ? 0x3F800000u : 0u);
+  v.x = ((b & 1u) ? one : 0u) | ((b & 2u) ? hi : 0u);
+  v.y = ((b & 4u) ? one : 0u) | ((b & 8u) ? hi : 0u);
+  v.z = ((b & 16u) ? one : 0u) | ((b & 32u) ? hi : 0u);
+  v.w = ((b & 64u) ? one : 0u) | ((b & 128u) ? hi : 0u);
   return v;
 }
 
@@ -127,10 +144,15 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
       // bit-packed rows: A[m][k] = bit k of row m
       const uint32_t* X = (const uint32_t*)g.A;
       const int r = tid >> 1, h = tid & 1;
-      const uint32_t w = (m0 + r < g.M) ? X[(int64_t)(m0 + r) * g.lda + (k0 >> 5)] : 0u;
+      uint32_t w = (m0 + r < g.M) ? X[(int64_t)(m0 + r) * g.lda + (k0 >> 5)] : 0u;
+      uint32_t one = 0x3F80u;
+      if (g.drop_seed) {
+        w &= keep_mask(g.drop_seed, (uint32_t)(m0 + r), (uint32_t)(k0 >> 5));
+        one = 0x4000u;
+      }
       uint4* dst = (uint4*)(As + r * LDSK + h * 16);
-      dst[0] = expand8((w >> (h * 16)) & 0xFFu);
-      dst[1] = expand8((w >> (h * 16 + 8)) & 0xFFu);
+      dst[0] = expand8((w >> (h * 16)) & 0xFFu, one);
+      dst[1] = expand8((w >> (h * 16 + 8)) & 0xFFu, one);
     }
     // ---- B tile -> Bs[n][k]
     if (BMODE == B_TRANS) {
@@ -158,9 +180,14 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
       const uint32_t* X = (const uint32_t*)g.B;
       if (tid < BN) {
         const int kk = tid >> 2, wi = tid & 3;
-        const uint32_t w = X[(int64_t)(k0 + kk) * g.ldb + (n0 >> 5) + wi];
+        uint32_t w = X[(int64_t)(k0 + kk) * g.ldb + (n0 >> 5) + wi];
+        bf16 one = (bf16)1.0f;
+        if (g.drop_seed) {
+          w &= keep_mask(g.drop_seed, (uint32_t)(k0 + kk), (uint32_t)((n0 >> 5) + wi));
+          one = (bf16)2.0f;
+        }
 #pragma unroll
-        for (int b = 0; b < 32; ++b) Bs[(wi * 32 + b) * LDSK + kk] = ((w >> b) & 1u) ? (bf16)1.0f : (bf16)0.0f;
+        for (int b = 0; b < 32; ++b) Bs[(wi * 32 + b) * LDSK + kk] = ((w >> b) & 1u) ? one : (bf16)0.0f;
       }
     }
     __syncthreads();
@@ -270,9 +297,10 @@ __global__ void colsum_kernel(const bf16* __restrict__ dz, int M, int H, float* 
   db[n] = s;
 }
 
+// Adam with decoupled weight decay (AdamW): p -= lr (m^ / (sqrt(v^) + eps) + wd p)
 __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
                             float* __restrict__ m2, int64_t n, float lr, float b1, float b2, float eps,
-                            float c1, float c2) {
+                            float c1, float c2, float wd) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float gi = g[i];
@@ -280,7 +308,8 @@ __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, 
   const float v = b2 * m2[i] + (1.f - b2) * gi * gi;
   m1[i] = a;
   m2[i] = v;
-  p[i] -= lr * (a / c1) / (sqrtf(v / c2) + eps);
+  const float pi = p[i];
+  p[i] = pi - lr * ((a / c1) / (sqrtf(v / c2) + eps) + wd * pi);
 }
 
 __global__ void cast_bf16_kernel(const float* __restrict__ src, bf16* __restrict__ dst, int64_t n) {
@@ -341,12 +370,19 @@ __global__ __launch_bounds__(256) void toggle_bits_kernel(const int64_t* __restr
   const int64_t n = rise_only ? f - n_net : f;
   const int64_t b = net_off[n], cnt = net_off[n + 1] - b;
   const int64_t i = (chunk - sel_chunk_off[k]) * 256 + threadIdx.x;
-  if (i < 1 || i >= cnt) return;
-  const uint32_t w = ev[b + i];
-  if (rise_only && (w & 3u) != 1u) return;
-  const int64_t c = cycle_of_tick(w >> 2, T, invT) - c0;
-  if (c < 0 || c >= n_cyc) return;
-  atomicOr(&X[c * words + (k >> 5)], 1u << (k & 31));
+  int64_t c = -1;
+  if (i >= 1 && i < cnt) {
+    const uint32_t w = ev[b + i];
+    if (!rise_only || (w & 3u) == 1u) {
+      c = cycle_of_tick(w >> 2, T, invT) - c0;
+      if (c >= n_cyc) c = -1;
+    }
+  }
+  // the block's events are one net's, in time order: of the lanes that set the
+  // same bit of the same cycle, only the first does the atomic
+  const int64_t cp = __shfl_up(c, 1);
+  const bool first = (threadIdx.x & 63) == 0 || cp != c;
+  if (c >= 0 && first) atomicOr(&X[c * words + (k >> 5)], 1u << (k & 31));
 }
 
 // Per net over the cycles with mask[c] != 0 (all when mask is null): the
@@ -518,11 +554,13 @@ hipError_t refresh_bf16(gtp_mlp* m) {
 }
 
 // forward pass over n rows of bit features already on the device
-hipError_t forward(gtp_mlp* m, const uint32_t* X, int n, bool keep_z, float* out) {
+// (drop_seed != 0: training-time input dropout)
+hipError_t forward(gtp_mlp* m, const uint32_t* X, int n, bool keep_z, float* out, uint32_t drop_seed = 0) {
   const int H = m->H;
   GemmArgs g{};
   g.M = n; g.N = H; g.K = m->K;
   g.A = X; g.lda = m->words;
+  g.drop_seed = drop_seed;
   g.B = m->W1b; g.ldb = m->K;
   g.C = keep_z ? m->z1 : nullptr; g.ldc = H;
   g.Cb = m->h1; g.ldcb = H;
@@ -543,9 +581,9 @@ hipError_t forward(gtp_mlp* m, const uint32_t* X, int n, bool keep_z, float* out
 
 // forward + backward of one batch held in m->Xb / m->yb (standardised y);
 // gradients into m->G; loss partials into m->lossp
-hipError_t backward(gtp_mlp* m, int n) {
+hipError_t backward(gtp_mlp* m, int n, uint32_t drop_seed = 0) {
   const int H = m->H;
-  hipError_t e = forward(m, m->Xb, n, true, m->out);
+  hipError_t e = forward(m, m->Xb, n, true, m->out, drop_seed);
   if (e != hipSuccess) return e;
   loss_kernel<<<nblk(n), 256, 0, m->stream>>>(m->out, m->yb, n, m->dout, m->lossp);
   head_grad_kernel<<<nblk(H), 256, 0, m->stream>>>(m->h2, m->dout, n, H, m->g(4), m->g(5));
@@ -573,6 +611,7 @@ hipError_t backward(gtp_mlp* m, int n) {
   c.A = m->dz1; c.lda = H;
   c.B = m->Xb; c.ldb = m->words;
   c.C = m->g(0); c.ldc = m->K;
+  c.drop_seed = drop_seed;
   if ((e = gemm<A_COL, B_BITS_ROW, EPI_F32>(c, m->stream)) != hipSuccess) return e;
   return hipGetLastError();
 }
@@ -639,6 +678,8 @@ GTP_API void gtp_mlp_destroy(gtp_mlp* m) {
   delete m;
 }
 
+GTP_API uint32_t gtp_keep_mask(uint32_t seed, uint32_t row, uint32_t word) { return keep_mask(seed, row, word); }
+
 GTP_API const char* gtp_mlp_last_error(const gtp_mlp* m) { return m ? m->err.c_str() : "null model"; }
 
 GTP_API int64_t gtp_mlp_param_count(const gtp_mlp* m) { return m ? m->n_param : 0; }
@@ -670,13 +711,13 @@ GTP_API gtp_status gtp_mlp_set_params(gtp_mlp* m, const float* params, const flo
 // Gradients of the mean squared error of one batch (targets already
 // standardised by the caller), without updating the model.
 GTP_API gtp_status gtp_mlp_gradients(gtp_mlp* m, const uint32_t* bits, const float* y, int32_t n,
-                                     float* grads, float* loss) {
+                                     uint32_t drop_seed, float* grads, float* loss) {
   if (!m || !bits || !y || !grads || n <= 0 || n % 32) return m ? m->fail(GTP_ERR_ARG, "n must be a positive multiple of 32") : GTP_ERR_ARG;
   MCHK(m, hipSetDevice(m->device));
   MCHK(m, m->reserve(n));
   MCHK(m, hipMemcpyAsync(m->Xb, bits, (size_t)n * m->words * 4, hipMemcpyHostToDevice, m->stream));
   MCHK(m, hipMemcpyAsync(m->yb, y, (size_t)n * 4, hipMemcpyHostToDevice, m->stream));
-  MCHK(m, backward(m, n));
+  MCHK(m, backward(m, n, drop_seed));
   MCHK(m, hipMemcpyAsync(grads, m->G, (size_t)m->n_param * 4, hipMemcpyDeviceToHost, m->stream));
   std::vector<float> lp(nblk(n));
   MCHK(m, hipMemcpyAsync(lp.data(), m->lossp, lp.size() * 4, hipMemcpyDeviceToHost, m->stream));
@@ -685,67 +726,108 @@ GTP_API gtp_status gtp_mlp_gradients(gtp_mlp* m, const uint32_t* bits, const flo
   return GTP_OK;
 }
 
+// Train on n rows; the last tc->val_rows of them are held out: after every
+// epoch their mean squared error is measured and the parameters of the best
+// epoch are kept.  epoch_loss[2 ep] = mean training loss of epoch ep,
+// epoch_loss[2 ep + 1] = its validation loss (NaN without validation rows),
+// both on standardised targets.
 GTP_API gtp_status gtp_mlp_train(gtp_mlp* m, const uint32_t* bits, const float* y, int64_t n,
                                  const gtp_train_config* tc, float* epoch_loss) {
-  if (!m || !bits || !y || !tc || n < 32 || tc->batch <= 0 || tc->batch % 32 || tc->epochs <= 0)
-    return m ? m->fail(GTP_ERR_ARG, "gtp_mlp_train: bad arguments") : GTP_ERR_ARG;
+  if (!m) return GTP_ERR_ARG;
+  if (!bits || !y || !tc || tc->batch <= 0 || tc->batch % 32 || tc->epochs <= 0 || tc->val_rows < 0 ||
+      n - tc->val_rows < tc->batch || !(tc->weight_decay >= 0.f) || (tc->input_dropout != 0 && tc->input_dropout != 1))
+    return m->fail(GTP_ERR_ARG, "gtp_mlp_train: bad arguments (batch a multiple of 32, at least one batch of "
+                                "training rows besides the validation rows, weight_decay >= 0)");
   MCHK(m, hipSetDevice(m->device));
-  // standardise the targets
+  const int64_t n_val = tc->val_rows, n_tr = n - n_val;
+  // standardise the targets with the training rows' statistics
   double mu = 0, var = 0;
-  for (int64_t i = 0; i < n; ++i) mu += y[i];
-  mu /= (double)n;
-  for (int64_t i = 0; i < n; ++i) var += (y[i] - mu) * (y[i] - mu);
-  const double sd = std::sqrt(var / (double)n) > 0 ? std::sqrt(var / (double)n) : 1.0;
+  for (int64_t i = 0; i < n_tr; ++i) mu += y[i];
+  mu /= (double)n_tr;
+  for (int64_t i = 0; i < n_tr; ++i) var += (y[i] - mu) * (y[i] - mu);
+  const double sd = std::sqrt(var / (double)n_tr) > 0 ? std::sqrt(var / (double)n_tr) : 1.0;
   m->y_mean = (float)mu;
   m->y_std = (float)sd;
   std::vector<float> ys(n);
   for (int64_t i = 0; i < n; ++i) ys[i] = (float)((y[i] - mu) / sd);
   uint32_t* dX = nullptr;
-  float* dy = nullptr;
-  MCHK(m, hipMalloc((void**)&dX, (size_t)n * m->words * 4));
-  if (hipMalloc((void**)&dy, (size_t)n * 4) != hipSuccess) {
-    (void)hipFree(dX);
-    return m->fail(GTP_ERR_OOM, "gtp_mlp_train: target buffer");
-  }
-  struct Free2 {
-    void *a, *b;
-    ~Free2() {
-      (void)hipFree(a);
-      (void)hipFree(b);
+  float *dy = nullptr, *best = nullptr;
+  struct Free3 {
+    void **a, **b, **c;
+    ~Free3() {
+      for (void** p : {a, b, c})
+        if (*p) (void)hipFree(*p);
     }
-  } fr{dX, dy};
+  } fr{(void**)&dX, (void**)&dy, (void**)&best};
+  MCHK(m, hipMalloc((void**)&dX, (size_t)n * m->words * 4));
+  MCHK(m, hipMalloc((void**)&dy, (size_t)n * 4));
+  if (n_val > 0) MCHK(m, hipMalloc((void**)&best, (size_t)m->n_param * 4));
   MCHK(m, hipMemcpyAsync(dX, bits, (size_t)n * m->words * 4, hipMemcpyHostToDevice, m->stream));
   MCHK(m, hipMemcpyAsync(dy, ys.data(), (size_t)n * 4, hipMemcpyHostToDevice, m->stream));
   const int B = tc->batch;
   MCHK(m, m->reserve(B));
   std::mt19937_64 rng(tc->seed);
-  std::vector<int32_t> perm(n);
+  std::vector<int32_t> perm(n_tr);
   std::iota(perm.begin(), perm.end(), 0);
-  const int64_t steps_per_epoch = n / B;
+  const int64_t steps_per_epoch = n_tr / B;
   const int64_t total_steps = steps_per_epoch * tc->epochs;
   int64_t step = 0;
-  std::vector<float> lp(nblk(B));
+  std::vector<float> lp(nblk(B)), vp;
+  double best_val = INFINITY;
   for (int ep = 0; ep < tc->epochs; ++ep) {
     std::shuffle(perm.begin(), perm.end(), rng);
     double eloss = 0;
+    int64_t n_loss = 0;
     for (int64_t s = 0; s < steps_per_epoch; ++s, ++step) {
       MCHK(m, hipMemcpyAsync(m->idx, perm.data() + s * B, (size_t)B * 4, hipMemcpyHostToDevice, m->stream));
       gather_rows_kernel<<<nblk((int64_t)B * m->words), 256, 0, m->stream>>>(dX, dy, m->idx, B, m->words, m->Xb, m->yb);
-      MCHK(m, backward(m, B));
+      // a fresh, never-zero dropout seed per step
+      const uint32_t ds = tc->input_dropout ? (uint32_t)(rng() | 1u) : 0u;
+      MCHK(m, backward(m, B, ds));
       // cosine learning-rate decay
       const float lr = (float)(tc->lr * 0.5 * (1.0 + std::cos(M_PI * (double)step / (double)total_steps)));
       ++m->adam_t;
       const float c1 = 1.f - std::pow(0.9f, (float)m->adam_t), c2 = 1.f - std::pow(0.999f, (float)m->adam_t);
       adam_kernel<<<nblk(m->n_param), 256, 0, m->stream>>>(m->P, m->G, m->M1, m->M2, m->n_param, lr, 0.9f,
-                                                            0.999f, 1e-8f, c1, c2);
+                                                            0.999f, 1e-8f, c1, c2, tc->weight_decay);
       MCHK(m, refresh_bf16(m));
       if (epoch_loss && (s == steps_per_epoch - 1 || s % 16 == 0)) {
         MCHK(m, hipMemcpyAsync(lp.data(), m->lossp, lp.size() * 4, hipMemcpyDeviceToHost, m->stream));
         MCHK(m, hipStreamSynchronize(m->stream));
         eloss += std::accumulate(lp.begin(), lp.end(), 0.0) / B;
+        ++n_loss;
       }
     }
-    if (epoch_loss) epoch_loss[ep] = (float)(eloss / ((steps_per_epoch + 15) / 16));
+    // the host permutation is rewritten next epoch: let the copies finish
+    MCHK(m, hipStreamSynchronize(m->stream));
+    double vloss = NAN;
+    if (n_val > 0) {
+      double se = 0;
+      for (int64_t b0 = 0; b0 < n_val; b0 += m->cap) {
+        const int c = (int)std::min<int64_t>(m->cap, n_val - b0);
+        MCHK(m, forward(m, dX + (n_tr + b0) * m->words, c, false, m->out));
+        vp.resize(c);
+        MCHK(m, hipMemcpyAsync(vp.data(), m->out, (size_t)c * 4, hipMemcpyDeviceToHost, m->stream));
+        MCHK(m, hipStreamSynchronize(m->stream));
+        for (int i = 0; i < c; ++i) {
+          const double d = (double)vp[i] - (double)ys[n_tr + b0 + i];
+          se += d * d;
+        }
+      }
+      vloss = se / (double)n_val;
+      if (vloss < best_val) {
+        best_val = vloss;
+        MCHK(m, hipMemcpyAsync(best, m->P, (size_t)m->n_param * 4, hipMemcpyDeviceToDevice, m->stream));
+      }
+    }
+    if (epoch_loss) {
+      epoch_loss[2 * ep] = (float)(n_loss ? eloss / (double)n_loss : NAN);
+      epoch_loss[2 * ep + 1] = (float)vloss;
+    }
+  }
+  if (n_val > 0) {
+    MCHK(m, hipMemcpyAsync(m->P, best, (size_t)m->n_param * 4, hipMemcpyDeviceToDevice, m->stream));
+    MCHK(m, refresh_bf16(m));
   }
   MCHK(m, hipStreamSynchronize(m->stream));
   return GTP_OK;

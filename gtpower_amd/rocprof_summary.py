@@ -1,0 +1,172 @@
+"""Summaries of rocprofv3 databases for the power path.
+
+    python -m gtpower_amd.rocprof_summary trace DB [--out FILE] [--cmd "..."]
+    python -m gtpower_amd.rocprof_summary pmc FETCH_DB WRITE_DB --copy FETCH_DB WRITE_DB --copy-mib N
+                                          [--out FILE] [--json FILE] [--cmd "..."]
+
+`trace` reads a `rocprofv3 --kernel-trace` database: per-kernel dispatch
+statistics, and the exact engine's evaluation windows -- from the start of a
+full glitch annotation (glitch_annotate_kernel) to the end of the last power
+kernel (power_cell_kernel / power_tile_kernel) before that run's
+cycle_reduce_kernel -- which is the interval the engine times with HIP events.
+
+`pmc` reads two `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` databases of
+the same program, and two of plain device copies of a known size, and reports
+HBM traffic per evaluation window: the counters (KiB per dispatch) summed over
+the window's kernels, each scaled by known bytes / counter reading of the
+copies.
+"""
+import argparse
+import json
+import sqlite3
+import statistics
+import sys
+
+WINDOW_START = "glitch_annotate_kernel("
+WINDOW_KERNELS = ("power_cell_kernel", "power_tile_kernel")
+WINDOW_END = "cycle_reduce_kernel("
+
+
+def _short(name):
+    n = name[5:] if name.startswith("void ") else name
+    return n.split("(")[0]
+
+
+def dispatches(db):
+    c = sqlite3.connect(db)
+    rows = c.execute("select dispatch_id, name, start, end from kernels order by start").fetchall()
+    return [dict(id=r[0], name=r[1], start=r[2], end=r[3]) for r in rows]
+
+
+def windows(disp):
+    """Evaluation windows: lists of dispatches, annotate first."""
+    out, cur = [], None
+    for d in disp:
+        if WINDOW_START in d["name"]:
+            cur = [d]
+        elif cur is not None and any(k in d["name"] for k in WINDOW_KERNELS):
+            cur.append(d)
+        elif cur is not None and WINDOW_END in d["name"]:
+            out.append(cur)
+            cur = None
+    return out
+
+
+def kernel_table(disp):
+    by = {}
+    for d in disp:
+        by.setdefault(_short(d["name"]), []).append((d["end"] - d["start"]) / 1e3)
+    rows = sorted(by.items(), key=lambda kv: -sum(kv[1]))
+    lines = [f"{'kernel':58s} {'calls':>6s} {'total_ms':>10s} {'avg_us':>10s} {'min_us':>10s} {'max_us':>10s}"]
+    for k, v in rows:
+        lines.append(f"{k[:58]:58s} {len(v):6d} {sum(v) / 1e3:10.3f} {statistics.mean(v):10.1f} "
+                     f"{min(v):10.1f} {max(v):10.1f}")
+    return lines
+
+
+def trace_summary(db):
+    disp = dispatches(db)
+    lines = ["per-kernel dispatch statistics (device time)"] + kernel_table(disp)
+    ws = windows(disp)
+    if ws:
+        dur = [(max(d["end"] for d in w) - w[0]["start"]) / 1e6 for w in ws]
+        lines += ["", f"exact evaluation windows (annotate start -> last power kernel end): {len(ws)}",
+                  "  per window ms: " + " ".join(f"{x:.3f}" for x in dur),
+                  f"  mean {statistics.mean(dur):.3f} ms, min {min(dur):.3f} ms, max {max(dur):.3f} ms",
+                  f"  kernels per window: {len(ws[0])}"]
+    return lines, ws
+
+
+def _counter_by_dispatch(db, counter):
+    c = sqlite3.connect(db)
+    rows = c.execute("select dispatch_id, kernel_name, sum(value) from counters_collection "
+                     "where counter_name = ? group by dispatch_id order by dispatch_id", (counter,)).fetchall()
+    return [(r[0], r[1], r[2]) for r in rows]
+
+
+def _windows_from_counters(rows):
+    out, cur = [], None
+    for did, name, val in rows:
+        if WINDOW_START in name:
+            cur = [(did, name, val)]
+        elif cur is not None and any(k in name for k in WINDOW_KERNELS):
+            cur.append((did, name, val))
+        elif cur is not None and WINDOW_END in name:
+            out.append(cur)
+            cur = None
+    return out
+
+
+def _copy_factor(db, counter, mib):
+    rows = [r for r in _counter_by_dispatch(db, counter) if "copy" in r[1].lower() or "elementwise" in r[1].lower()]
+    big = [v for _, _, v in rows if v > 0.5 * mib * 1024]
+    if not big:
+        raise SystemExit(f"{db}: no copy dispatch of about {mib} MiB found for {counter}")
+    return mib * 1024.0 / statistics.median(big), statistics.median(big)
+
+
+def pmc_summary(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_mib):
+    ff, f_read = _copy_factor(copy_fetch_db, "FETCH_SIZE", copy_mib)
+    fw, w_read = _copy_factor(copy_write_db, "WRITE_SIZE", copy_mib)
+    wf = _windows_from_counters(_counter_by_dispatch(fetch_db, "FETCH_SIZE"))
+    ww = _windows_from_counters(_counter_by_dispatch(write_db, "WRITE_SIZE"))
+    if not wf or not ww:
+        raise SystemExit("no evaluation window found in the counter databases")
+    fetch = [sum(v for _, _, v in w) * ff * 1024 for w in wf]
+    write = [sum(v for _, _, v in w) * fw * 1024 for w in ww]
+    per_kernel = {}
+    for w in wf:
+        for _, name, v in w:
+            per_kernel.setdefault(_short(name), [0.0, 0.0])[0] += v * ff * 1024 / len(wf)
+    for w in ww:
+        for _, name, v in w:
+            per_kernel.setdefault(_short(name), [0.0, 0.0])[1] += v * fw * 1024 / len(ww)
+    res = {"fetch_bytes": statistics.mean(fetch), "write_bytes": statistics.mean(write),
+           "traffic_bytes": statistics.mean(fetch) + statistics.mean(write),
+           "windows": len(wf), "copy_mib": copy_mib,
+           "fetch_scale": ff, "write_scale": fw,
+           "copy_fetch_kib_read": f_read, "copy_write_kib_read": w_read}
+    lines = [f"copy calibration: {copy_mib} MiB copy reads FETCH_SIZE {f_read:.0f} KiB, WRITE_SIZE {w_read:.0f} KiB "
+             f"-> scale fetch x{ff:.4f}, write x{fw:.4f}",
+             f"evaluation windows: {len(wf)} (fetch), {len(ww)} (write)",
+             f"HBM traffic per evaluation window: fetch {res['fetch_bytes'] / 1e9:.3f} GB, "
+             f"write {res['write_bytes'] / 1e9:.3f} GB, total {res['traffic_bytes'] / 1e9:.3f} GB",
+             "", f"{'kernel':58s} {'fetch_GB':>10s} {'write_GB':>10s}  (per window)"]
+    for k, (f, w) in sorted(per_kernel.items(), key=lambda kv: -kv[1][0]):
+        lines.append(f"{k[:58]:58s} {f / 1e9:10.3f} {w / 1e9:10.3f}")
+    return lines, res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="mode", required=True)
+    t = sub.add_parser("trace")
+    t.add_argument("db")
+    p = sub.add_parser("pmc")
+    p.add_argument("fetch_db")
+    p.add_argument("write_db")
+    p.add_argument("--copy", nargs=2, required=True, metavar=("FETCH_DB", "WRITE_DB"))
+    p.add_argument("--copy-mib", type=int, required=True)
+    p.add_argument("--json")
+    for q in (t, p):
+        q.add_argument("--out")
+        q.add_argument("--cmd", default="")
+    a = ap.parse_args(argv)
+    if a.mode == "trace":
+        lines, _ = trace_summary(a.db)
+    else:
+        lines, res = pmc_summary(a.fetch_db, a.write_db, a.copy[0], a.copy[1], a.copy_mib)
+        if a.json:
+            with open(a.json, "w") as f:
+                json.dump(res, f, indent=1)
+    if a.cmd:
+        lines = [f"command: {a.cmd}", ""] + lines
+    text = "\n".join(lines) + "\n"
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(text)
+    sys.stdout.write(text)
+
+
+if __name__ == "__main__":
+    main()

@@ -77,24 +77,30 @@ class MlpSurrogate:
         self._check(self._lib.gtp_mlp_set_params(self._h, abi.ptr(p, C.c_float), abi.ptr(ms, C.c_float)),
                     "gtp_mlp_set_params")
 
-    def gradients(self, bits, y_std):
-        """(loss, grads) of the mean squared error on standardised targets."""
+    def gradients(self, bits, y_std, drop_seed=0):
+        """(loss, grads) of the mean squared error on standardised targets
+        (drop_seed != 0: with the training-time input dropout of that seed)."""
         bits = np.ascontiguousarray(bits, dtype=np.uint32)
         y = np.ascontiguousarray(y_std, dtype=np.float32)
         g = np.zeros(self.n_param, np.float32)
         loss = C.c_float()
         self._check(self._lib.gtp_mlp_gradients(self._h, abi.ptr(bits, C.c_uint32), abi.ptr(y, C.c_float), len(y),
-                                                abi.ptr(g, C.c_float), C.byref(loss)), "gtp_mlp_gradients")
+                                                drop_seed, abi.ptr(g, C.c_float), C.byref(loss)),
+                    "gtp_mlp_gradients")
         return float(loss.value), g
 
-    def train(self, bits, y, epochs=30, batch=256, lr=5e-4, seed=7):
+    def train(self, bits, y, epochs=30, batch=256, lr=5e-4, weight_decay=0.0, val_rows=0, input_dropout=False,
+              seed=7):
+        """Train on the rows of (bits, y); the last val_rows rows are held out
+        and the best epoch on them is kept.  Returns [epochs, 2]: mean training
+        loss and validation loss per epoch (standardised targets)."""
         bits = np.ascontiguousarray(bits, dtype=np.uint32)
         y = np.ascontiguousarray(y, dtype=np.float32)
-        tc = abi.GtpTrainConfig(batch, epochs, lr, 0, seed)
-        el = np.zeros(epochs, np.float32)
+        tc = abi.GtpTrainConfig(batch, epochs, lr, weight_decay, seed, val_rows, 1 if input_dropout else 0)
+        el = np.zeros(2 * epochs, np.float32)
         self._check(self._lib.gtp_mlp_train(self._h, abi.ptr(bits, C.c_uint32), abi.ptr(y, C.c_float), len(y),
                                             C.byref(tc), abi.ptr(el, C.c_float)), "gtp_mlp_train")
-        return el
+        return el.reshape(epochs, 2)
 
     def predict(self, bits):
         bits = np.ascontiguousarray(bits, dtype=np.uint32)
@@ -160,19 +166,26 @@ class SurrogatePower:
         self.seed = seed
         self.feats = None
 
-    def fit(self, engine, train_mask=None, exact=None, epochs=30, batch=256, lr=5e-4, seed=7):
+    def fit(self, engine, train_mask=None, exact=None, epochs=30, batch=256, lr=5e-4, weight_decay=0.0,
+            val_fraction=0.1, input_dropout=True, seed=7):
         """Train on the exact per-cycle power of the cycles with train_mask set
-        (all cycles when None) of the trace loaded in `engine`.  `exact` may
-        pass a PowerResult of that trace already computed."""
+        (all cycles when None) of the trace loaded in `engine`; the last
+        val_fraction of those cycles are held out to pick the best epoch.
+        `exact` may pass a PowerResult of that trace already computed."""
         if exact is None:
             exact = engine.power(want=("cycle",))
         y = cycle_total(exact.cycle_power)
         mask = np.ones(y.size, bool) if train_mask is None else np.asarray(train_mask, bool)
-        corr = engine.feature_correlation(y, mask.astype(np.uint8))
+        rows = np.nonzero(mask)[0]
+        n_val = int(val_fraction * len(rows))
+        fit_mask = np.zeros(y.size, np.uint8)
+        fit_mask[rows[:len(rows) - n_val]] = 1  # features are ranked on the training rows only
+        corr = engine.feature_correlation(y, fit_mask)
         self.feats = select_features(corr, self.mlp.n_in)
         bits = engine.toggle_features(self.feats)
-        rows = np.nonzero(mask)[0]
-        self.train_loss = self.mlp.train(bits[rows], y[rows], epochs=epochs, batch=batch, lr=lr, seed=seed)
+        self.train_loss = self.mlp.train(bits[rows], y[rows], epochs=epochs, batch=batch, lr=lr,
+                                         weight_decay=weight_decay, val_rows=n_val, input_dropout=input_dropout,
+                                         seed=seed)
         return self
 
     def judge_tiles(self, n_cycle):

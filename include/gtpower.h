@@ -260,11 +260,14 @@ typedef struct {
 } gtp_mlp_config;
 
 typedef struct {
-  int32_t batch;     /* rows per step, a multiple of 32 */
+  int32_t batch;        /* rows per step, a multiple of 32 */
   int32_t epochs;
-  float lr;          /* Adam peak learning rate (cosine decay to 0) */
-  int32_t pad_;
-  uint64_t seed;     /* shuffling seed */
+  float lr;             /* Adam peak learning rate (cosine decay to 0) */
+  float weight_decay;   /* decoupled weight decay (AdamW); 0 = none */
+  uint64_t seed;        /* shuffling seed */
+  int32_t val_rows;     /* the last val_rows rows are held out; the best epoch's parameters are kept */
+  int32_t input_dropout; /* 1: each input bit of a training row is dropped with probability 1/2 and
+                            kept ones count 2 (inverted dropout); 0: none */
 } gtp_train_config;
 
 typedef struct gtp_mlp gtp_mlp;
@@ -277,11 +280,18 @@ int64_t gtp_mlp_param_count(const gtp_mlp* m);
 gtp_status gtp_mlp_get_params(gtp_mlp* m, float* params, float* y_mean_std);
 gtp_status gtp_mlp_set_params(gtp_mlp* m, const float* params, const float* y_mean_std);
 /* Gradients of the mean squared error over n rows (n a multiple of 32), with
- * targets y already standardised; the model is not changed. */
+ * targets y already standardised; the model is not changed.  drop_seed != 0
+ * applies the training-time input dropout: bit b of word w of row r is kept
+ * when bit b of gtp_keep_mask(drop_seed, r, w) is set, and counts 2. */
 gtp_status gtp_mlp_gradients(gtp_mlp* m, const uint32_t* bits, const float* y, int32_t n,
-                             float* grads, float* loss);
-/* Train on n rows with raw targets y (standardised inside); epoch_loss
- * (may be NULL) receives the mean standardised loss of each epoch. */
+                             uint32_t drop_seed, float* grads, float* loss);
+/* The dropout keep bits of (seed, row, word) (a murmur3-style finaliser of
+ * seed ^ row * 0x9E3779B1 ^ word * 0x85EBCA77). */
+uint32_t gtp_keep_mask(uint32_t seed, uint32_t row, uint32_t word);
+/* Train on n rows with raw targets y (standardised inside with the training
+ * rows' mean and std).  epoch_loss (may be NULL) is [2 * epochs]: the mean
+ * training loss and the validation loss (NaN without validation rows) of each
+ * epoch, on standardised targets. */
 gtp_status gtp_mlp_train(gtp_mlp* m, const uint32_t* bits, const float* y, int64_t n,
                          const gtp_train_config* tc, float* epoch_loss);
 /* Predictions in target units for n rows of host feature bits. */

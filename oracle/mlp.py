@@ -44,6 +44,31 @@ def gelu_grad(z):
     return 0.5 * (1.0 + _erf(z / math.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / math.sqrt(2.0 * math.pi)
 
 
+def keep_mask(seed, rows, words):
+    """The training-time input-dropout keep bits of (seed, row, word), uint32
+    arithmetic: h = seed ^ row*0x9E3779B1 ^ word*0x85EBCA77, then the
+    murmur3 finaliser steps (xor-shift 16, * 0x7FEB352D, xor-shift 15,
+    * 0x846CA68B, xor-shift 16)."""
+    r = np.asarray(rows, dtype=np.uint64)
+    w = np.asarray(words, dtype=np.uint64)
+    M = np.uint64(0xFFFFFFFF)
+    h = (np.uint64(seed) ^ ((r * np.uint64(0x9E3779B1)) & M) ^ ((w * np.uint64(0x85EBCA77)) & M)) & M
+    h ^= h >> np.uint64(16)
+    h = (h * np.uint64(0x7FEB352D)) & M
+    h ^= h >> np.uint64(15)
+    h = (h * np.uint64(0x846CA68B)) & M
+    h ^= h >> np.uint64(16)
+    return h.astype(np.uint32)
+
+
+def dropped_inputs(bits, n_in, seed):
+    """Inputs of a training step with input dropout: kept bits count 2."""
+    b = np.asarray(bits, dtype=np.uint32)
+    R, W = b.shape
+    km = keep_mask(seed, np.arange(R)[:, None], np.arange(W)[None, :])
+    return 2.0 * unpack_bits(b & km, n_in)
+
+
 def unpack_bits(bits, n_in):
     """[M][n_in/32] uint32 rows -> [M][n_in] 0/1 float64 (bit k = bit k%32 of word k/32)."""
     b = np.asarray(bits, dtype=np.uint32)

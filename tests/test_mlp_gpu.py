@@ -98,17 +98,20 @@ def test_gradients_match_oracle():
     Xb, bits = _data(n, n_in, 2)
     rng = np.random.default_rng(3)
     y = rng.standard_normal(n).astype(np.float32)
-    loss, g = m.gradients(bits, y)
-    lref, gref = OM.loss_and_grad(p.astype(np.float64), Xb, y.astype(np.float64), n_in, H, bf16_points=True)
-    assert abs(loss - lref) <= 1e-4 * lref
-    sizes = [H * n_in, H, H * H, H, H, 1]
-    names = ["W1", "b1", "W2", "b2", "w3", "b3"]
-    o = 0
-    for nm, sz in zip(names, sizes):
-        a, b = g[o:o + sz].astype(np.float64), gref[o:o + sz]
-        err = np.linalg.norm(a - b)
-        assert err <= 2e-3 * np.linalg.norm(b) + 1e-12, f"{nm}: rel err {err / np.linalg.norm(b):.3g}"
-        o += sz
+    # without dropout, then with the training-time input dropout of one seed
+    for seed in (0, 0x1234567):
+        loss, g = m.gradients(bits, y, drop_seed=seed)
+        X = Xb if seed == 0 else OM.dropped_inputs(bits, n_in, seed)
+        lref, gref = OM.loss_and_grad(p.astype(np.float64), X, y.astype(np.float64), n_in, H, bf16_points=True)
+        assert abs(loss - lref) <= 1e-4 * lref
+        sizes = [H * n_in, H, H * H, H, H, 1]
+        names = ["W1", "b1", "W2", "b2", "w3", "b3"]
+        o = 0
+        for nm, sz in zip(names, sizes):
+            a, b = g[o:o + sz].astype(np.float64), gref[o:o + sz]
+            err = np.linalg.norm(a - b)
+            assert err <= 2e-3 * np.linalg.norm(b) + 1e-12, f"seed {seed} {nm}: rel err {err / np.linalg.norm(b):.3g}"
+            o += sz
     m.close()
 
 
@@ -121,7 +124,8 @@ def test_training_fits_a_linear_target():
     y = (Xb @ w + 50.0).astype(np.float32)
     m = MlpSurrogate(n_in, H, seed=2)
     losses = m.train(bits, y, epochs=12, batch=256, lr=2e-3, seed=1)
-    assert losses[-1] < 0.1 * losses[0]
+    assert losses[-1, 0] < 0.1 * losses[0, 0]
+    assert np.isnan(losses[:, 1]).all()  # no validation rows
     Xt, bt = _data(2048, n_in, 6, p=0.2)
     pred = m.predict(bt)
     yt = Xt @ w + 50.0

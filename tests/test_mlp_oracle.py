@@ -115,3 +115,28 @@ def test_nrmse_definition():
     exact = np.array([1.0, 2.0, 3.0, 2.0])
     pred = exact + np.array([0.1, -0.1, 0.1, -0.1])
     assert abs(nrmse(pred, exact) - 0.1 / 2.0) < 1e-15
+
+
+def test_keep_mask_known_values_and_balance():
+    # the finaliser on hand-computed inputs: seed=1,row=0,word=0 -> h = 1
+    h = 1
+    h ^= h >> 16
+    h = (h * 0x7FEB352D) & 0xFFFFFFFF
+    h ^= h >> 15
+    h = (h * 0x846CA68B) & 0xFFFFFFFF
+    h ^= h >> 16
+    assert int(M.keep_mask(1, 0, 0)) == h
+    # about half of the bits are kept, independently of row and word
+    km = M.keep_mask(12345, np.arange(2000)[:, None], np.arange(64)[None, :])
+    frac = np.unpackbits(km.view(np.uint8)).mean()
+    assert abs(frac - 0.5) < 0.005
+
+
+def test_keep_mask_matches_engine_export():
+    from gtpower_amd import abi
+    try:
+        lib = abi.load_engine()
+    except (RuntimeError, OSError):
+        pytest.skip("engine library not built")
+    for seed, r, w in [(1, 0, 0), (0xDEADBEEF, 17, 3), (7, 123456, 127), (0xFFFFFFFF, 0xFFFFFFFF, 5)]:
+        assert lib.gtp_keep_mask(seed, r, w) == int(M.keep_mask(seed, r, w))
