@@ -39,7 +39,8 @@ p_u8 = C.POINTER(C.c_uint8)
 class GtpOptions(C.Structure):
     _fields_ = [("device", C.c_int32), ("tile_cycles", C.c_int32), ("block_lanes", C.c_int32),
                 ("gates_per_wg", C.c_int32), ("split_target", C.c_int32), ("autotune", C.c_int32),
-                ("max_event_num", C.c_int64), ("host_threads", C.c_int32), ("verbose", C.c_int32)]
+                ("max_event_num", C.c_int64), ("host_threads", C.c_int32), ("verbose", C.c_int32),
+                ("round_tiles", C.c_int32), ("pad_", C.c_int32)]
 
 
 class GtpTable(C.Structure):

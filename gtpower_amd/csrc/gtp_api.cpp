@@ -168,7 +168,9 @@ struct gtp_ctx {
   };
   // two cached schedules: the full run's and the tile-subset run's (whose
   // ranges are sized for its few tiles)
-  OrderSet orders[2];
+  OrderSet orders[3];              // [2]: batched runs
+  int n_cu = 256;                  // compute units of the device
+  DArr<unsigned int> d_wq;         // persistent work-queue counters, 8 per launch
   gtp_schedule_info sched{};
 
   // scratch and results
@@ -392,6 +394,11 @@ GTP_API gtp_status gtp_create(const gtp_options* opt, gtp_ctx** out) {
   }
   x->threads = x->opt.host_threads > 0 ? x->opt.host_threads
                                        : (int)std::max(1u, std::thread::hardware_concurrency());
+  {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+      x->n_cu = ncu;
+  }
   *out = x;
   return GTP_OK;
 }
@@ -548,11 +555,11 @@ GTP_API gtp_status gtp_set_design(gtp_ctx* x, const gtp_library* L, const gtp_de
 }
 
 // ---------------------------------------------------------------------------
-GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
-  if (!x) return GTP_ERR_ARG;
-  if (!x->have_design) return x->fail(GTP_ERR_STATE, "gtp_set_trace: call gtp_set_design first");
-  HIPCHK(x, hipSetDevice(x->device));
-  x->have_trace = false;
+namespace {
+
+// Checks one trace against the design; *T_out = ticks per cycle
+// (power/PowerUtils.cc:27-32).
+gtp_status validate_trace(gtp_ctx* x, const gtp_trace* tr, int64_t* T_out) {
   if (!tr || !tr->net_event_offset || tr->n_net != x->N)
     return x->fail(GTP_ERR_ARG, "gtp_trace: n_net must equal the design's n_net");
   if (!(tr->time_scale > 0) || !(tr->clk_period > 0) || tr->t_max <= 0)
@@ -566,9 +573,6 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
     if (off[n + 1] < off[n]) return x->fail(GTP_ERR_ARG, "gtp_trace: net_event_offset decreases at " + std::to_string(n));
   const int64_t E = off[N];
   if (E > 0 && (!tr->event_time || !tr->event_value)) return x->fail(GTP_ERR_ARG, "gtp_trace: event arrays missing");
-  if (x->opt.max_event_num > 0 && E > x->opt.max_event_num)
-    return x->fail(GTP_ERR_OOM, "gtp_trace: " + std::to_string(E) + " events exceed max_event_num");
-  // validation: strictly increasing times within [0, t_max], values 0..3
   std::atomic<int64_t> bad{-1};
   parallel_for(N, x->threads, [&](int64_t n) {
     int64_t prev = -1;
@@ -585,10 +589,34 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
   if (bad.load() >= 0)
     return x->fail(GTP_ERR_ARG, "gtp_trace: events of net " + std::to_string(bad.load()) +
                                     " are not strictly increasing in [0, t_max] or carry a value > 3");
-  double ratio = tr->clk_period / tr->time_scale;
-  int64_t T = (int64_t)std::ceil(ratio);  // power/PowerUtils.cc:27-32
+  const double ratio = tr->clk_period / tr->time_scale;
+  int64_t T = (int64_t)std::ceil(ratio);
   if (std::fabs(ratio - std::round(ratio)) < 1e-9 * std::max(1.0, ratio)) T = (int64_t)std::llround(ratio);
   if (T < 1 || T >= (int64_t)EV_TIME_INF) return x->fail(GTP_ERR_UNSUPPORTED, "gtp_trace: ticks per cycle out of range");
+  *T_out = T;
+  return GTP_OK;
+}
+
+// Arc delays on the 2^-16 tick grid of the related-pin comparison.
+gtp_status ensure_arc_q(gtp_ctx* x, double time_scale);
+
+}  // namespace
+
+GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
+  if (!x) return GTP_ERR_ARG;
+  if (!x->have_design) return x->fail(GTP_ERR_STATE, "gtp_set_trace: call gtp_set_design first");
+  HIPCHK(x, hipSetDevice(x->device));
+  x->have_trace = false;
+  int64_t T = 0;
+  {
+    const gtp_status st = validate_trace(x, tr, &T);
+    if (st != GTP_OK) return st;
+  }
+  const int64_t N = x->N;
+  const int64_t* off = tr->net_event_offset;
+  const int64_t E = off[N];
+  if (x->opt.max_event_num > 0 && E > x->opt.max_event_num)
+    return x->fail(GTP_ERR_OOM, "gtp_trace: " + std::to_string(E) + " events exceed max_event_num");
   x->T = T;
   x->t_max = tr->t_max;
   x->C = (tr->t_max + T - 1) / T;
@@ -626,24 +654,9 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
     HIPCHK(x, x->d_aux.alloc(std::max<int64_t>(E, 1)));
     HIPCHK(x, hipStreamSynchronize(s));
   }
-  // arc delays on the 2^-20 tick grid of the related-pin comparison
-  if (x->arc_q_scale != tr->time_scale) {
-    std::vector<int32_t> aq(x->arc_delay.size());
-    const double f = (double)(1 << ARC_FRAC_BITS) / tr->time_scale;
-    for (size_t i = 0; i < aq.size(); ++i) {
-      const double dl = x->arc_delay[i];
-      if (std::isnan(dl)) {
-        aq[i] = ARC_NONE;
-        continue;
-      }
-      const double q = std::round(dl * f);
-      if (!(std::fabs(q) < 2147483000.0))
-        return x->fail(GTP_ERR_LIMIT, "gtp_trace: an arc delay exceeds 2^15 ticks (the 2^-16-tick comparison grid)");
-      aq[i] = (int32_t)q;
-    }
-    HIPCHK(x, x->d_arc_q.upload(aq.data(), std::max<size_t>(aq.size(), 1), s));
-    HIPCHK(x, hipStreamSynchronize(s));
-    x->arc_q_scale = tr->time_scale;
+  {
+    gtp_status st = ensure_arc_q(x, tr->time_scale);
+    if (st != GTP_OK) return st;
   }
   // per-gate work (pin events) for the schedule
   x->gate_work.assign(x->G, 0);
@@ -665,10 +678,32 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
 
 namespace {
 
+gtp_status ensure_arc_q(gtp_ctx* x, double time_scale) {
+  if (x->arc_q_scale == time_scale) return GTP_OK;
+  std::vector<int32_t> aq(x->arc_delay.size());
+  const double f = (double)(1 << ARC_FRAC_BITS) / time_scale;
+  for (size_t i = 0; i < aq.size(); ++i) {
+    const double dl = x->arc_delay[i];
+    if (std::isnan(dl)) {
+      aq[i] = ARC_NONE;
+      continue;
+    }
+    const double q = std::round(dl * f);
+    if (!(std::fabs(q) < 2147483000.0))
+      return x->fail(GTP_ERR_LIMIT, "gtp_trace: an arc delay exceeds 2^15 ticks (the 2^-16-tick comparison grid)");
+    aq[i] = (int32_t)q;
+  }
+  HIPCHK(x, x->d_arc_q.upload(aq.data(), std::max<size_t>(aq.size(), 1), x->stream));
+  HIPCHK(x, hipStreamSynchronize(x->stream));
+  x->arc_q_scale = time_scale;
+  return GTP_OK;
+}
+
 // Schedule: gates grouped by kernel (one group per single-output cell type
 // without `when` conditions, one generic group), each group sorted by work (pin
 // events), heaviest first, and padded with -1 to whole ranges.
-void build_order(gtp_ctx* x, gtp_ctx::OrderSet& os, int32_t range_gates) {
+void build_order(gtp_ctx* x, gtp_ctx::OrderSet& os, int32_t range_gates, const std::vector<int64_t>& work,
+                 int64_t T) {
   const int nc = (int)x->cells.size();
   std::vector<std::vector<int32_t>> by_cell(nc);
   std::vector<int32_t> generic;
@@ -677,7 +712,7 @@ void build_order(gtp_ctx* x, gtp_ctx::OrderSet& os, int32_t range_gates) {
     const HostCell& hc = x->cells[x->gate_cell[g]];
     // the specialised kernel compares arrivals on 32 bits: cycles below 2^15 ticks
     const bool fast = hc.sbits == 0 && hc.n_pin - hc.n_in == 1 && hc.n_pin >= 2 && hc.n_pin <= 6 &&
-                      x->T < 32768;
+                      T < 32768;
     if (fast) by_cell[x->gate_cell[g]].push_back((int32_t)g);
     else {
       generic.push_back((int32_t)g);
@@ -685,7 +720,7 @@ void build_order(gtp_ctx* x, gtp_ctx::OrderSet& os, int32_t range_gates) {
     }
   }
   auto by_work = [&](int32_t p, int32_t q) {
-    if (x->gate_work[p] != x->gate_work[q]) return x->gate_work[p] > x->gate_work[q];
+    if (work[p] != work[q]) return work[p] > work[q];
     return p < q;
   };
   os.order.clear();
@@ -721,6 +756,78 @@ struct RunPlan {
   PowerArgs a{};
 };
 
+// Fixed-point quanta: powers of two with one term <= 2^30 quanta, so a term
+// converts through int32 and every sum is exact in int64 (a cycle would need
+// 2^33 terms, a gate's leakage 2^33 ticks of trace, to overflow).  Uploads
+// the leakage table in quanta.
+struct Quanta {
+  float inv_e = 0, inv_l = 0;
+  double q_e = 0, q_l = 0;
+};
+
+gtp_status make_quanta(gtp_ctx* x, int64_t t_max, Quanta* q) {
+  double emax = std::max((double)x->energy_max * 1.000001, x->csw_max);
+  if (!(emax > 0)) emax = 1e-30;
+  const int ke = (int)std::floor(30.0 - std::log2(emax));
+  const double lmax = x->leak_max > 0 ? x->leak_max * 1.000001 : 1e-30;
+  const int kl = (int)std::floor(30.0 - std::log2(lmax));
+  q->inv_e = std::ldexp(1.0f, ke);
+  q->inv_l = std::ldexp(1.0f, kl);
+  q->q_e = std::ldexp(1.0, -ke);
+  q->q_l = std::ldexp(1.0, -kl);
+  if (!std::isfinite(q->inv_e) || !std::isfinite(q->inv_l) || q->inv_e == 0.f || q->inv_l == 0.f)
+    return x->fail(GTP_ERR_UNSUPPORTED, "gtp_run: energy or leakage magnitudes outside the FP32 range");
+  if ((double)t_max * 2147483648.0 >= 9.2e18)
+    return x->fail(GTP_ERR_LIMIT, "gtp_run: trace too long for the int64 leakage sums");
+  std::vector<long long> lq(x->leak_f.size());
+  for (size_t i = 0; i < lq.size(); ++i) lq[i] = std::llrint((double)x->leak_f[i] * (double)q->inv_l);
+  HIPCHK(x, hipMemcpyAsync(x->d_leak_q.p, lq.data(), lq.size() * sizeof(long long), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(x, hipStreamSynchronize(x->stream));
+  return GTP_OK;
+}
+
+// The kernel arguments that do not depend on the trace.
+void fill_args(gtp_ctx* x, const gtp_ctx::OrderSet& os, const Quanta& qt, int64_t TC, int64_t T, double scale,
+               PowerArgs& a) {
+  a = PowerArgs{};
+  a.order = os.d_order.p;
+  a.n_sched = (int64_t)os.order.size();
+  a.gate_cell = x->d_gate_cell.p;
+  a.gate_pin_off = x->d_gate_pin_off.p;
+  a.gate_arc_off = x->d_gate_arc_off.p;
+  a.gate_energy_off = x->d_gate_energy_off.p;
+  a.pin_net = x->d_pin_net.p;
+  a.pin_gk = x->d_pin_gk.p;
+  a.time_scale = (float)scale;
+  a.pin_csw = x->d_pin_csw.p;
+  a.pin_default = x->d_pin_default.p;
+  a.energy = x->d_energy.p;
+  a.arc_q = x->d_arc_q.p;
+  a.tile_cycles = (int32_t)TC;
+  a.T = T;
+  a.inv_T = (float)(1.0 / (double)T);
+  a.n_range = os.n_range;
+  a.range_gates = os.range_gates;
+  a.xcd_remap = 1;
+  a.cells = x->d_cells.p;
+  a.n_cell = (int32_t)x->cell_dev.size();
+  a.leak = x->d_leak.p;
+  a.leak_q = x->d_leak_q.p;
+  a.n_leak = (int32_t)x->leak_f.size();
+  a.inv_q_energy = qt.inv_e;
+  a.inv_q_leak = qt.inv_l;
+}
+
+gtp_status lds_sizes(gtp_ctx* x, const PowerArgs& a, RunPlan* rp) {
+  const int64_t TC = a.tile_cycles;
+  rp->lds = (size_t)N_BIN * TC * 8 + (size_t)a.n_leak * 8 + (size_t)(a.n_leak + (a.n_leak & 1)) * 4 +
+            (size_t)a.n_cell * sizeof(CellDev);
+  rp->lds_cell = (size_t)N_BIN * TC * 8 * BIN_COPIES;
+  if (rp->lds > 160 * 1024 || rp->lds_cell > 160 * 1024)
+    return x->fail(GTP_ERR_LIMIT, "gtp_run: tile needs more LDS than a workgroup has");
+  return GTP_OK;
+}
+
 // Schedule parameters, order, tile index, fixed-point quanta and scratch.
 // tc_req > 0 asks for exactly that tile length; n_work_tiles > 0 is the
 // number of tiles the launches will cover (a subset run), which sizes the
@@ -751,7 +858,7 @@ gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, int32_t n_work_tiles, RunPlan
   gtp_ctx::OrderSet& os = x->orders[subset ? 1 : 0];
   if (os.order.empty() || os.range_gates != RG || os.order_T != x->T) {
     os.order_T = x->T;
-    build_order(x, os, RG);
+    build_order(x, os, RG, x->gate_work, x->T);
     HIPCHK(x, os.d_order.upload(os.order.data(), os.order.size(), s));
   }
   const int64_t n_sched = (int64_t)os.order.size();
@@ -762,25 +869,10 @@ gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, int32_t n_work_tiles, RunPlan
     HIPCHK(x, launch_tile_index(x->d_net_off.p, x->d_ev.p, x->N, n_tile, TC * x->T, nullptr, 0, x->d_tile_idx.p, s));
     x->tile_idx_tc = (int32_t)TC;
   }
-  // fixed-point quanta: powers of two with one term <= 2^30 quanta, so a term
-  // converts through int32 and every sum is exact in int64 (a cycle would
-  // need 2^33 terms, a gate's leakage 2^33 ticks of trace, to overflow)
-  double emax = std::max((double)x->energy_max * 1.000001, x->csw_max);
-  if (!(emax > 0)) emax = 1e-30;
-  const int ke = (int)std::floor(30.0 - std::log2(emax));
-  const float inv_q_e = std::ldexp(1.0f, ke);
-  const double lmax = x->leak_max > 0 ? x->leak_max * 1.000001 : 1e-30;
-  const int kl = (int)std::floor(30.0 - std::log2(lmax));
-  const float inv_q_l = std::ldexp(1.0f, kl);
-  if (!std::isfinite(inv_q_e) || !std::isfinite(inv_q_l) || inv_q_e == 0.f || inv_q_l == 0.f)
-    return x->fail(GTP_ERR_UNSUPPORTED, "gtp_run: energy or leakage magnitudes outside the FP32 range");
-  if ((double)x->t_max * 2147483648.0 >= 9.2e18)
-    return x->fail(GTP_ERR_LIMIT, "gtp_run: trace too long for the int64 leakage sums");
+  Quanta qt;
   {
-    std::vector<long long> lq(x->leak_f.size());
-    for (size_t i = 0; i < lq.size(); ++i) lq[i] = std::llrint((double)x->leak_f[i] * (double)inv_q_l);
-    HIPCHK(x, hipMemcpyAsync(x->d_leak_q.p, lq.data(), lq.size() * sizeof(long long), hipMemcpyHostToDevice, s));
-    HIPCHK(x, hipStreamSynchronize(s));
+    const gtp_status st = make_quanta(x, x->t_max, &qt);
+    if (st != GTP_OK) return st;
   }
   // scratch
   HIPCHK(x, os.d_pg.alloc((size_t)N_PG * work_tiles * n_sched));
@@ -795,70 +887,43 @@ gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, int32_t n_work_tiles, RunPlan
   HIPCHK(x, x->d_gglc.alloc(x->G));
 
   PowerArgs& a = rp->a;
-  a = PowerArgs{};
-  a.order = os.d_order.p;
-  a.n_sched = n_sched;
-  a.gate_cell = x->d_gate_cell.p;
-  a.gate_pin_off = x->d_gate_pin_off.p;
-  a.gate_arc_off = x->d_gate_arc_off.p;
-  a.gate_energy_off = x->d_gate_energy_off.p;
-  a.pin_net = x->d_pin_net.p;
-  a.pin_gk = x->d_pin_gk.p;
-  a.time_scale = (float)x->scale;
-  a.pin_csw = x->d_pin_csw.p;
-  a.pin_default = x->d_pin_default.p;
-  a.energy = x->d_energy.p;
-  a.arc_q = x->d_arc_q.p;
+  fill_args(x, os, qt, TC, x->T, x->scale, a);
   a.net_off = x->d_net_off.p;
   a.ev = x->d_ev.p;
   a.aux = x->d_aux.p;
   a.tile_idx = x->d_tile_idx.p;
   a.n_tile = n_tile;
-  a.tile_cycles = (int32_t)TC;
-  a.T = x->T;
   a.t_max = x->t_max;
   a.n_cycle = x->C;
-  a.inv_T = (float)(1.0 / (double)x->T);
-  a.n_range = n_range;
-  a.range_gates = RG;
-  a.xcd_remap = 1;
-  a.tile_list = nullptr;
   a.n_work_tiles = n_tile;
-  a.cells = x->d_cells.p;
-  a.n_cell = (int32_t)x->cell_dev.size();
-  a.leak = x->d_leak.p;
-  a.leak_q = x->d_leak_q.p;
-  a.n_leak = (int32_t)x->leak_f.size();
-  a.inv_q_energy = inv_q_e;
-  a.inv_q_leak = inv_q_l;
   a.pg = os.d_pg.p;
   a.pg_tog = os.d_pg_tog.p;
   a.pg_glitch = os.d_pg_glc.p;
   a.cyc_part = os.d_cyc_part.p;
-  rp->lds = (size_t)N_BIN * TC * 8 + (size_t)a.n_leak * 8 + (size_t)(a.n_leak + (a.n_leak & 1)) * 4 +
-            (size_t)a.n_cell * sizeof(CellDev);
-  if (rp->lds > 160 * 1024) return x->fail(GTP_ERR_LIMIT, "gtp_run: tile needs more LDS than a workgroup has");
-  rp->lds_cell = (size_t)N_BIN * TC * 8 * BIN_COPIES;
-  if (rp->lds_cell > 160 * 1024) return x->fail(GTP_ERR_LIMIT, "gtp_run: tile needs more LDS than a workgroup has");
   rp->TC = TC;
   rp->n_tile = n_tile;
   rp->RG = RG;
   rp->n_range = n_range;
   rp->block = block;
   rp->n_sched = n_sched;
-  rp->q_e = std::ldexp(1.0, -ke);
-  rp->q_l = std::ldexp(1.0, -kl);
-  return GTP_OK;
+  rp->q_e = qt.q_e;
+  rp->q_l = qt.q_l;
+  return lds_sizes(x, a, rp);
 }
 
 // The per-cell-group power launches over n_work_tiles tiles (rp.a), run
 // concurrently on the main and the side streams (all default priority): the
 // launches write disjoint outputs.
-gtp_status launch_groups(gtp_ctx* x, const RunPlan& rp) {
+gtp_status launch_groups(gtp_ctx* x, const RunPlan& rp, bool persistent = false) {
   hipStream_t s = x->stream;
+  const std::vector<gtp_ctx::Launch>& all = rp.os->launches;
+  if (persistent) {
+    HIPCHK(x, x->d_wq.alloc(8 * std::max<size_t>(all.size(), 1)));
+    HIPCHK(x, launch_zero_u32(x->d_wq.p, (int64_t)(8 * all.size()), s));
+  }
   HIPCHK(x, hipEventRecord(x->fork, s));
   for (int k = 0; k < gtp_ctx::N_SIDE; ++k) HIPCHK(x, hipStreamWaitEvent(x->side[k], x->fork, 0));
-  const std::vector<gtp_ctx::Launch>& launches = rp.os->launches;
+  const std::vector<gtp_ctx::Launch>& launches = all;
   std::vector<size_t> idx(launches.size());
   std::iota(idx.begin(), idx.end(), 0);
   std::sort(idx.begin(), idx.end(), [&](size_t p, size_t q) {
@@ -870,11 +935,18 @@ gtp_status launch_groups(gtp_ctx* x, const RunPlan& rp) {
     hipStream_t ls = (k == 0) ? s : x->side[k - 1];
     k = (k + 1) % (gtp_ctx::N_SIDE + 1);
     const int64_t nw = (int64_t)rp.a.n_work_tiles * L.range_count;
+    // persistent: a few workgroups per compute unit drain the launch's queue
+    int64_t grid = (nw + 7) / 8 * 8;
+    unsigned int* wq = nullptr;
+    if (persistent) {
+      grid = std::min<int64_t>(grid, (int64_t)x->n_cu * 4 / 8 * 8 + 8);
+      wq = x->d_wq.p + 8 * li;
+    }
     if (L.cell >= 0)
-      HIPCHK(x, launch_power_cell(L.np, (nw + 7) / 8 * 8, rp.block, rp.lds_cell, ls, rp.a, L.range_begin,
-                                  L.range_count, L.cell));
+      HIPCHK(x, launch_power_cell(L.np, grid, rp.block, rp.lds_cell, ls, rp.a, L.range_begin, L.range_count, L.cell,
+                                  wq));
     else
-      HIPCHK(x, launch_power(L.np, (nw + 7) / 8 * 8, rp.block, rp.lds, ls, rp.a, L.range_begin, L.range_count));
+      HIPCHK(x, launch_power(L.np, grid, rp.block, rp.lds, ls, rp.a, L.range_begin, L.range_count, wq));
   }
   for (int j = 0; j < gtp_ctx::N_SIDE; ++j) {
     HIPCHK(x, hipEventRecord(x->join[j], x->side[j]));
@@ -1013,12 +1085,300 @@ int32_t gtp::trace_dev(gtp_ctx* x, TraceDev* out) {
   return GTP_OK;
 }
 
+// Several traces of the design in one set of persistent launches.  Traces are
+// grouped into device loads under the event budget; a load's traces become
+// one event store of n_trace_load * N virtual nets (trace k's net n is
+// k * N + n), annotated in one pass; its work tiles (trace, tile) are
+// processed in rounds bounded by the partials' memory, each round one
+// persistent launch per cell group over all the round's traces.  Per-trace
+// results are exact integer sums, identical to gtp_run of each trace.
 GTP_API gtp_status gtp_run_batch(gtp_ctx* x, int32_t n_trace, const gtp_trace* traces, gtp_result* results) {
   if (!x) return GTP_ERR_ARG;
-  (void)n_trace;
-  (void)traces;
-  (void)results;
-  return x->fail(GTP_ERR_UNSUPPORTED, "gtp_run_batch: not built yet");
+  if (!x->have_design) return x->fail(GTP_ERR_STATE, "gtp_run_batch: call gtp_set_design first");
+  if (n_trace <= 0 || !traces || !results) return x->fail(GTP_ERR_ARG, "gtp_run_batch: no traces");
+  HIPCHK(x, hipSetDevice(x->device));
+  hipStream_t s = x->stream;
+  const int64_t N = x->N, G = x->G;
+  // validation: one tick length and time scale for the batch
+  int64_t T = 0;
+  std::vector<int64_t> C(n_trace), E(n_trace);
+  int64_t tmax_max = 0;
+  for (int32_t k = 0; k < n_trace; ++k) {
+    int64_t Tk = 0;
+    const gtp_status st = validate_trace(x, &traces[k], &Tk);
+    if (st != GTP_OK) return x->fail(st, "trace " + std::to_string(k) + ": " + x->err);
+    if (k == 0) T = Tk;
+    if (Tk != T || traces[k].time_scale != traces[0].time_scale || traces[k].clk_period != traces[0].clk_period)
+      return x->fail(GTP_ERR_UNSUPPORTED, "gtp_run_batch: traces must share time_scale and clk_period");
+    C[k] = (traces[k].t_max + T - 1) / T;
+    E[k] = traces[k].net_event_offset[N];
+    tmax_max = std::max<int64_t>(tmax_max, traces[k].t_max);
+  }
+  {
+    const gtp_status st = ensure_arc_q(x, traces[0].time_scale);
+    if (st != GTP_OK) return st;
+  }
+  Quanta qt;
+  {
+    const gtp_status st = make_quanta(x, tmax_max, &qt);
+    if (st != GTP_OK) return st;
+  }
+  // tile length (common to the batch) and schedule: gates ordered by their
+  // pin events summed over the batch
+  int64_t TC = x->opt.tile_cycles > 0 ? x->opt.tile_cycles : 128;
+  if (T < (1 << 24)) TC = std::min<int64_t>(TC, std::max<int64_t>(1, ((1 << 24) - 1) / T));
+  TC = std::max<int64_t>(1, std::min<int64_t>(TC, 4096));
+  std::vector<int32_t> n_tile(n_trace);
+  int64_t total_tiles = 0;
+  for (int32_t k = 0; k < n_trace; ++k) {
+    n_tile[k] = (int32_t)((C[k] + TC - 1) / TC);
+    total_tiles += n_tile[k];
+  }
+  std::vector<int64_t> work(G, 0);
+  parallel_for(G, x->threads, [&](int64_t g) {
+    int64_t w = 0;
+    for (int64_t q = x->gate_pin_off[g]; q < x->gate_pin_off[g + 1]; ++q) {
+      const int32_t net = x->pin_net[q];
+      if (net < 0) continue;
+      for (int32_t k = 0; k < n_trace; ++k) w += traces[k].net_event_offset[net + 1] - traces[k].net_event_offset[net];
+    }
+    work[g] = w;
+  });
+  // partials per work tile: N_PG int64 + 2 uint32 per gate, bins per range
+  const int64_t n_sched_est = G + 4096LL * 16;
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(x, hipMemGetInfo(&free_b, &total_b));
+  const double per_tile_est = (double)n_sched_est * (N_PG * 8 + 8) + 64.0 * N_BIN * TC * 8;
+  const int64_t budget_scratch = std::min<int64_t>((int64_t)8 << 30, (int64_t)(free_b / 4));
+  int32_t W_round = (int32_t)std::max<int64_t>(1, std::min<int64_t>(total_tiles, (int64_t)(budget_scratch / per_tile_est)));
+  if (x->opt.round_tiles > 0) W_round = (int32_t)std::min<int64_t>(x->opt.round_tiles, total_tiles);
+  int32_t RG = x->opt.gates_per_wg;
+  if (RG <= 0) {
+    const double want = (double)G * std::min<int64_t>(W_round, total_tiles) / 8192.0;
+    RG = 256;
+    while (RG < 4096 && RG * 2 <= want) RG *= 2;
+  }
+  int block = x->opt.block_lanes > 0 ? std::min(256, std::max(64, (x->opt.block_lanes + 63) / 64 * 64)) : 256;
+  block = std::min(block, std::max(64, RG));
+  gtp_ctx::OrderSet& os = x->orders[2];
+  build_order(x, os, RG, work, T);
+  os.order_T = -1;  // rebuilt for every batch
+  HIPCHK(x, os.d_order.upload(os.order.data(), os.order.size(), s));
+  const int64_t n_sched = (int64_t)os.order.size();
+  const int32_t n_range = os.n_range;
+  HIPCHK(x, os.d_pg.alloc((size_t)N_PG * W_round * n_sched));
+  HIPCHK(x, os.d_pg_tog.alloc((size_t)W_round * n_sched));
+  HIPCHK(x, os.d_pg_glc.alloc((size_t)W_round * n_sched));
+  HIPCHK(x, os.d_cyc_part.alloc((size_t)W_round * n_range * N_BIN * TC));
+  RunPlan rp;
+  rp.os = &os;
+  fill_args(x, os, qt, TC, T, traces[0].time_scale, rp.a);
+  rp.a.pg = os.d_pg.p;
+  rp.a.pg_tog = os.d_pg_tog.p;
+  rp.a.pg_glitch = os.d_pg_glc.p;
+  rp.a.cyc_part = os.d_cyc_part.p;
+  rp.TC = TC;
+  rp.RG = RG;
+  rp.n_range = n_range;
+  rp.block = block;
+  rp.n_sched = n_sched;
+  rp.q_e = qt.q_e;
+  rp.q_l = qt.q_l;
+  {
+    const gtp_status st = lds_sizes(x, rp.a, &rp);
+    if (st != GTP_OK) return st;
+  }
+  // device loads under the event budget (event word + glitch width per event)
+  int64_t ev_budget = x->opt.max_event_num > 0 ? x->opt.max_event_num
+                                               : (int64_t)((double)free_b * 0.45 / 8.0);
+  std::vector<std::pair<int32_t, int32_t>> loads;
+  for (int32_t k = 0; k < n_trace;) {
+    if (E[k] > ev_budget)
+      return x->fail(GTP_ERR_OOM, "gtp_run_batch: trace " + std::to_string(k) + " alone exceeds the event budget");
+    int64_t e = 0;
+    int32_t j = k;
+    while (j < n_trace && e + E[j] <= ev_budget) e += E[j++];
+    loads.emplace_back(k, j);
+    k = j;
+  }
+  float kernel_ms_total = 0.f;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(x, hipEventCreate(&e0));
+  HIPCHK(x, hipEventCreate(&e1));
+  struct EvG {
+    hipEvent_t a, b;
+    ~EvG() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } evg{e0, e1};
+  const float invT = (float)(1.0 / (double)T);
+  for (const auto& ld : loads) {
+    const int32_t k0 = ld.first, nt = ld.second - ld.first;
+    // virtual-net CSR of the load
+    std::vector<int64_t> voff((size_t)nt * N + 1, 0);
+    int64_t E_load = 0;
+    for (int32_t k = 0; k < nt; ++k) {
+      const int64_t* off = traces[k0 + k].net_event_offset;
+      for (int64_t n = 0; n < N; ++n) voff[(size_t)k * N + n] = E_load + off[n];
+      E_load += off[N];
+    }
+    voff[(size_t)nt * N] = E_load;
+    DArr<int64_t> d_voff, d_chunk;
+    DArr<uint32_t> d_ev, d_aux;
+    HIPCHK(x, d_voff.upload(voff.data(), voff.size(), s));
+    HIPCHK(x, d_ev.alloc(std::max<int64_t>(E_load, 1)));
+    HIPCHK(x, d_aux.alloc(std::max<int64_t>(E_load, 1)));
+    {
+      const int64_t CH = 1 << 26;
+      DArr<int64_t> st;
+      DArr<uint8_t> sv;
+      HIPCHK(x, st.alloc(CH));
+      HIPCHK(x, sv.alloc(CH));
+      int64_t base = 0;
+      for (int32_t k = 0; k < nt; ++k) {
+        const gtp_trace& tr = traces[k0 + k];
+        for (int64_t b0 = 0; b0 < E[k0 + k]; b0 += CH) {
+          const int64_t n = std::min(CH, E[k0 + k] - b0);
+          HIPCHK(x, hipMemcpyAsync(st.p, tr.event_time + b0, n * sizeof(int64_t), hipMemcpyHostToDevice, s));
+          HIPCHK(x, hipMemcpyAsync(sv.p, tr.event_value + b0, n, hipMemcpyHostToDevice, s));
+          HIPCHK(x, launch_pack(st.p, sv.p, d_ev.p + base + b0, n, s));
+        }
+        base += E[k0 + k];
+      }
+      HIPCHK(x, hipStreamSynchronize(s));
+    }
+    std::vector<int64_t> co((size_t)nt * N + 1, 0);
+    for (size_t v = 0; v < (size_t)nt * N; ++v) co[v + 1] = co[v] + (voff[v + 1] - voff[v] + 255) / 256;
+    const int64_t n_chunk = co[(size_t)nt * N];
+    HIPCHK(x, d_chunk.upload(co.data(), co.size(), s));
+    // per-trace tile indexes and descriptors; work tiles trace-major
+    std::vector<TraceDesc> td(nt);
+    int64_t tidx_total = 0;
+    for (int32_t k = 0; k < nt; ++k) {
+      td[k].net_base = (int64_t)k * N;
+      td[k].tile_idx_base = tidx_total;
+      td[k].t_max = traces[k0 + k].t_max;
+      td[k].n_cycle = C[k0 + k];
+      td[k].n_tile = n_tile[k0 + k];
+      td[k].pad_ = 0;
+      tidx_total += N * ((int64_t)n_tile[k0 + k] + 1);
+    }
+    DArr<int32_t> d_tidx, d_wt_tile, d_wt_trace, d_wt_begin, d_rtrace;
+    DArr<TraceDesc> d_td;
+    HIPCHK(x, d_tidx.alloc(std::max<int64_t>(tidx_total, 1)));
+    HIPCHK(x, d_td.upload(td.data(), td.size(), s));
+    std::vector<int32_t> wt_tile, wt_trace;
+    std::vector<int64_t> out_base(nt + 1, 0);
+    for (int32_t k = 0; k < nt; ++k) {
+      for (int32_t t = 0; t < n_tile[k0 + k]; ++t) {
+        wt_tile.push_back(t);
+        wt_trace.push_back(k);
+      }
+      out_base[k + 1] = out_base[k] + (int64_t)GTP_N_WAVE * C[k0 + k];
+    }
+    HIPCHK(x, d_wt_tile.upload(wt_tile.data(), wt_tile.size(), s));
+    HIPCHK(x, d_wt_trace.upload(wt_trace.data(), wt_trace.size(), s));
+    DArr<int64_t> d_out_base;
+    HIPCHK(x, d_out_base.upload(out_base.data(), out_base.size(), s));
+    DArr<double> d_cyc;
+    HIPCHK(x, d_cyc.alloc(std::max<int64_t>(out_base[nt], 1)));
+    DArr<unsigned long long> d_acc, d_cnt;
+    HIPCHK(x, d_acc.alloc((size_t)nt * N_PG * G + 1));
+    HIPCHK(x, d_cnt.alloc((size_t)nt * 2 * G + 1));
+    HIPCHK(x, launch_zero_u64(d_acc.p, (int64_t)nt * N_PG * G, s));
+    HIPCHK(x, launch_zero_u64(d_cnt.p, (int64_t)nt * 2 * G, s));
+    HIPCHK(x, hipEventRecord(e0, s));
+    for (int32_t k = 0; k < nt; ++k)
+      HIPCHK(x, launch_tile_index(d_voff.p + (int64_t)k * N, d_ev.p, N, n_tile[k0 + k], TC * T, nullptr, 0,
+                                  d_tidx.p + td[k].tile_idx_base, s));
+    HIPCHK(x, launch_glitch_annotate(d_voff.p, d_ev.p, d_chunk.p, (int64_t)nt * N, n_chunk, (uint32_t)T, invT,
+                                     d_aux.p, s));
+    rp.a.net_off = d_voff.p;
+    rp.a.ev = d_ev.p;
+    rp.a.aux = d_aux.p;
+    rp.a.tile_idx = d_tidx.p;
+    rp.a.traces = d_td.p;
+    const int32_t n_wt = (int32_t)wt_tile.size();
+    for (int32_t r0 = 0; r0 < n_wt; r0 += W_round) {
+      const int32_t W = std::min(W_round, n_wt - r0);
+      rp.a.tile_list = d_wt_tile.p + r0;
+      rp.a.wt_trace = d_wt_trace.p + r0;
+      rp.a.n_work_tiles = W;
+      {
+        const gtp_status st = launch_groups(x, rp, true);
+        if (st != GTP_OK) return st;
+      }
+      HIPCHK(x, launch_cycle_reduce_batch(os.d_cyc_part.p, n_range, (int32_t)TC, rp.a.tile_list, rp.a.wt_trace, W,
+                                          d_td.p, d_out_base.p, T, qt.q_e, qt.q_l, 1.0 / traces[0].clk_period,
+                                          d_cyc.p, s));
+      // the round's traces and their contiguous work-tile ranges
+      std::vector<int32_t> rb, rt;
+      for (int32_t i = 0; i < W; ++i)
+        if (i == 0 || wt_trace[r0 + i] != wt_trace[r0 + i - 1]) {
+          rb.push_back(i);
+          rt.push_back(wt_trace[r0 + i]);
+        }
+      rb.push_back(W);
+      HIPCHK(x, d_wt_begin.upload(rb.data(), rb.size(), s));
+      HIPCHK(x, d_rtrace.upload(rt.data(), rt.size(), s));
+      HIPCHK(x, launch_gate_accumulate(os.d_pg.p, os.d_pg_tog.p, os.d_pg_glc.p, os.d_order.p, n_sched, W,
+                                       d_wt_begin.p, d_rtrace.p, (int32_t)rt.size(), G, (long long*)d_acc.p, d_cnt.p,
+                                       s));
+      // the host vectors of the uploads above are reused next round
+      HIPCHK(x, hipStreamSynchronize(s));
+    }
+    HIPCHK(x, hipEventRecord(e1, s));
+    HIPCHK(x, hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIPCHK(x, hipEventElapsedTime(&ms, e0, e1));
+    kernel_ms_total += ms;
+    // per-trace results
+    HIPCHK(x, x->d_gate.alloc((size_t)GTP_N_COMPONENT * G));
+    HIPCHK(x, x->d_total.alloc(5 + 25));
+    HIPCHK(x, x->d_total_part.alloc((size_t)TOTAL_BLOCKS * 30));
+    for (int32_t k = 0; k < nt; ++k) {
+      gtp_result& res = results[k0 + k];
+      const double tm = (double)traces[k0 + k].t_max;
+      HIPCHK(x, launch_gate_finalize((const long long*)d_acc.p + (int64_t)k * N_PG * G, G,
+                                     qt.q_e / (tm * traces[0].time_scale), qt.q_l / tm, x->d_gate.p, s));
+      HIPCHK(x, launch_total(x->d_gate.p, x->d_gate_class.p, G, x->d_total_part.p, x->d_total.p, s));
+      if (res.gate_power)
+        HIPCHK(x, hipMemcpyAsync(res.gate_power, x->d_gate.p, sizeof(double) * GTP_N_COMPONENT * G,
+                                 hipMemcpyDeviceToHost, s));
+      if (res.cycle_power)
+        HIPCHK(x, hipMemcpyAsync(res.cycle_power, d_cyc.p + out_base[k], sizeof(double) * GTP_N_WAVE * C[k0 + k],
+                                 hipMemcpyDeviceToHost, s));
+      if (res.gate_toggle_x2)
+        HIPCHK(x, hipMemcpyAsync(res.gate_toggle_x2, d_cnt.p + (int64_t)k * 2 * G, sizeof(uint64_t) * G,
+                                 hipMemcpyDeviceToHost, s));
+      if (res.gate_glitch)
+        HIPCHK(x, hipMemcpyAsync(res.gate_glitch, d_cnt.p + ((int64_t)k * 2 + 1) * G, sizeof(uint64_t) * G,
+                                 hipMemcpyDeviceToHost, s));
+      double tot[30];
+      HIPCHK(x, hipMemcpyAsync(tot, x->d_total.p, sizeof(tot), hipMemcpyDeviceToHost, s));
+      HIPCHK(x, hipStreamSynchronize(s));
+      if (res.total) std::memcpy(res.total, tot, sizeof(double) * 5);
+      if (res.class_total) std::memcpy(res.class_total, tot + 5, sizeof(double) * 25);
+      res.n_cycle = C[k0 + k];
+    }
+  }
+  for (int32_t k = 0; k < n_trace; ++k) {
+    results[k].kernel_ms = kernel_ms_total;
+    results[k].total_ms = kernel_ms_total;
+  }
+  gtp_schedule_info& si = x->sched;
+  si.tile_cycles = (int32_t)TC;
+  si.block_lanes = block;
+  si.gates_per_wg = RG;
+  si.n_unit = total_tiles * G;
+  si.n_workgroup = total_tiles * n_range;
+  si.split_target = (int32_t)os.launches.size();
+  si.n_tile = total_tiles;
+  si.n_event = std::accumulate(E.begin(), E.end(), (int64_t)0);
+  si.n_pin_event = std::accumulate(work.begin(), work.end(), (int64_t)0);
+  si.autotune_ms = 0;
+  return GTP_OK;
 }
 
 GTP_API gtp_status gtp_get_schedule(const gtp_ctx* x, gtp_schedule_info* info) {

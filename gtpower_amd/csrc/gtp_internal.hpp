@@ -63,6 +63,14 @@ struct TableDev {             // offsets into the staged float table data
   int32_t axis1, axis2, values;
 };
 
+// One trace of a batched run.
+struct TraceDesc {
+  int64_t net_base;        // first (virtual) net of the trace in the event store
+  int64_t tile_idx_base;   // offset of the trace's [N][n_tile + 1] tile index
+  int64_t t_max, n_cycle;
+  int32_t n_tile, pad_;
+};
+
 // Arguments of the power kernel.
 struct PowerArgs {
   // gates in schedule order
@@ -99,6 +107,8 @@ struct PowerArgs {
   int32_t xcd_remap;             // 1: remap blockIdx so each XCD takes a contiguous block of work
   const int32_t* tile_list;      // tiles to evaluate (null: all n_tile tiles)
   int32_t n_work_tiles;          // entries of tile_list, or n_tile
+  const int32_t* wt_trace;       // batched runs: trace of each work tile (tile_list: its tile); null otherwise
+  const TraceDesc* traces;       // batched runs: [n_trace]
   // cell tables (staged into LDS)
   const CellDev* cells;
   int32_t n_cell;
@@ -143,7 +153,7 @@ hipError_t launch_tile_index(const int64_t* net_off, const uint32_t* ev, int64_t
                              hipStream_t s);
 hipError_t launch_energy(const EnergyArgs& a, hipStream_t s);
 hipError_t launch_power(int maxp, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,
-                        int32_t range_begin, int32_t range_count);
+                        int32_t range_begin, int32_t range_count, unsigned int* wq);
 hipError_t launch_cycle_reduce(const long long* part, int32_t n_range, int32_t tile_cycles, int64_t n_cycle,
                                int64_t T, int64_t t_max, double q_energy, double q_leak, double inv_period,
                                double* out, hipStream_t s);
@@ -151,6 +161,18 @@ hipError_t launch_cycle_reduce_list(const long long* part, int32_t n_range, int3
                                     const int32_t* tile_list, int32_t n_list, int64_t n_cycle, int64_t T,
                                     int64_t t_max, double q_energy, double q_leak, double inv_period,
                                     double* out, hipStream_t s);
+hipError_t launch_cycle_reduce_batch(const long long* part, int32_t n_range, int32_t tile_cycles,
+                                     const int32_t* tile_list, const int32_t* wt_trace, int32_t n_wt,
+                                     const TraceDesc* traces, const int64_t* out_base, int64_t T, double q_energy,
+                                     double q_leak, double inv_period, double* out, hipStream_t s);
+hipError_t launch_gate_accumulate(const long long* pg, const uint32_t* tog, const uint32_t* glc,
+                                  const int32_t* order, int64_t n_sched, int32_t n_wt, const int32_t* wt_begin,
+                                  const int32_t* round_trace, int32_t n_round_trace, int64_t n_gate, long long* acc,
+                                  unsigned long long* acc_cnt, hipStream_t s);
+hipError_t launch_gate_finalize(const long long* acc, int64_t n_gate, double norm_energy, double norm_leak,
+                                double* gate_power, hipStream_t s);
+hipError_t launch_zero_u64(unsigned long long* p, int64_t n, hipStream_t s);
+hipError_t launch_zero_u32(unsigned int* p, int64_t n, hipStream_t s);
 hipError_t launch_gate_reduce(const long long* pg, const uint32_t* tog, const uint32_t* glc,
                               const int32_t* order, int64_t n_sched, int32_t n_tile, int64_t n_gate,
                               double norm_energy, double norm_leak, double* gate_power,
@@ -166,7 +188,7 @@ hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, const uint32_t* 
 hipError_t launch_total(const double* gate_power, const int32_t* gate_class, int64_t n_gate, double* part,
                         double* out, hipStream_t s);
 hipError_t launch_power_cell(int np, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,
-                             int32_t range_begin, int32_t range_count, int32_t cell);
+                             int32_t range_begin, int32_t range_count, int32_t cell, unsigned int* wq);
 
 // Read-only view of a context's device event store, for the passes that run
 // over it besides the power kernels (toggle features, correlation).

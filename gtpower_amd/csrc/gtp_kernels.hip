@@ -205,11 +205,75 @@ __global__ void energy_kernel(EnergyArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Work items of the power kernels: (work tile, gate range).  A work tile is a
+// tile of the loaded trace, a listed tile (subset runs), or a tile of one of
+// several traces (batched runs).
+// ---------------------------------------------------------------------------
+struct WorkTile {
+  int tile;                // tile within its trace
+  int tile_col;            // column of the tile index
+  int32_t n_tile;          // tiles of its trace
+  int64_t tile_stride;     // tile index row length
+  int64_t t_max, n_cycle;  // of its trace
+  int64_t net_base;        // first net of its trace in the event store
+  const int32_t* tidx;     // tile index of its trace
+};
+
+__device__ __forceinline__ WorkTile resolve_work_tile(const PowerArgs& a, int wt) {
+  WorkTile k;
+  if (a.wt_trace) {
+    const TraceDesc td = a.traces[a.wt_trace[wt]];
+    k.tile = a.tile_list[wt];
+    k.tile_col = k.tile;
+    k.n_tile = td.n_tile;
+    k.tile_stride = (int64_t)td.n_tile + 1;
+    k.t_max = td.t_max;
+    k.n_cycle = td.n_cycle;
+    k.net_base = td.net_base;
+    k.tidx = a.tile_idx + td.tile_idx_base;
+  } else {
+    k.tile = a.tile_list ? a.tile_list[wt] : wt;
+    k.tile_col = a.tile_list ? wt : k.tile;
+    k.n_tile = a.n_tile;
+    k.tile_stride = a.tile_list ? (int64_t)a.n_work_tiles : (int64_t)a.n_tile + 1;
+    k.t_max = a.t_max;
+    k.n_cycle = a.n_cycle;
+    k.net_base = 0;
+    k.tidx = a.tile_idx;
+  }
+  return k;
+}
+
+// The next work item of this workgroup; false when there is none.  Called by
+// every thread of the workgroup (the result is uniform).  Without a work
+// queue a workgroup takes one item: blockIdx remapped so that each XCD
+// (blockIdx % 8) takes a contiguous block of tile-major items and the gates
+// of one tile share that XCD's L2.  With a queue (wq[8], zeroed before the
+// launch) the workgroups are persistent: each takes items of its XCD's block
+// from that XCD's counter until the block is exhausted.
+__device__ __forceinline__ bool next_work_item(const PowerArgs& a, unsigned int* wq, int64_t n_work, int64_t it,
+                                               int64_t& w, long long* s_item) {
+  const int64_t per = (n_work + 7) / 8;
+  const int xcd = (int)(blockIdx.x % 8);
+  if (!wq) {
+    if (it > 0) return false;
+    w = a.xcd_remap ? (int64_t)xcd * per + blockIdx.x / 8 : (int64_t)blockIdx.x;
+    return w < n_work;
+  }
+  __syncthreads();  // every thread has read the previous item
+  if (threadIdx.x == 0) *s_item = (long long)atomicAdd(&wq[xcd], 1u);
+  __syncthreads();
+  const int64_t k = *s_item;
+  w = (int64_t)xcd * per + k;
+  return k < per && w < n_work;
+}
+
+// ---------------------------------------------------------------------------
 // The fused power kernel
 // ---------------------------------------------------------------------------
 template <int MAXP>
 __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t range_begin,
-                                                          int32_t range_count) {
+                                                          int32_t range_count, unsigned int* wq) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int TC = a.tile_cycles;
   long long* bins = reinterpret_cast<long long*>(smem_raw);
@@ -217,41 +281,35 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
   float* s_leak = reinterpret_cast<float*>(s_leakq + a.n_leak);
   CellDev* s_cells = reinterpret_cast<CellDev*>(s_leak + a.n_leak + (a.n_leak & 1));
   __shared__ long long s_base;
+  __shared__ long long s_item;
 
-  // work item -> (tile, range); XCD-aware: each XCD takes a contiguous block of
-  // tile-major work items, so the gates of one tile share that XCD's L2.
-  const int64_t n_work = (int64_t)a.n_work_tiles * range_count;
-  int64_t w = blockIdx.x;
-  if (a.xcd_remap) {
-    const int64_t per = (n_work + 7) / 8;
-    w = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
-  }
-  if (w >= n_work) return;
-  const int wt = (int)(w / range_count);
-  const int range = range_begin + (int)(w - (int64_t)wt * range_count);
-  const int tile = a.tile_list ? a.tile_list[wt] : wt;
-
-  for (int i = threadIdx.x; i < N_BIN * TC; i += blockDim.x) bins[i] = 0;
   for (int i = threadIdx.x; i < a.n_leak; i += blockDim.x) {
     s_leak[i] = a.leak[i];
     s_leakq[i] = a.leak_q[i];
   }
   for (int i = threadIdx.x; i < a.n_cell; i += blockDim.x) s_cells[i] = a.cells[i];
+
+  const int64_t n_work = (int64_t)a.n_work_tiles * range_count;
+  int64_t w = 0;
+  for (int64_t it = 0; next_work_item(a, wq, n_work, it, w, &s_item); ++it) {
+  const int wt = (int)(w / range_count);
+  const int range = range_begin + (int)(w - (int64_t)wt * range_count);
+  const WorkTile wk = resolve_work_tile(a, wt);
+  const int tile = wk.tile;
+
+  for (int i = threadIdx.x; i < N_BIN * TC; i += blockDim.x) bins[i] = 0;
   if (threadIdx.x == 0) s_base = 0;
   __syncthreads();
 
   const uint32_t T = (uint32_t)a.T;
   const float invT = a.inv_T;
   const int64_t tc0 = (int64_t)tile * TC;
-  const int tcn = (int)min((int64_t)TC, a.n_cycle - tc0);
-  const bool last_tile = tile == a.n_tile - 1;
+  const int tcn = (int)min((int64_t)TC, wk.n_cycle - tc0);
+  const bool last_tile = tile == wk.n_tile - 1;
   const uint32_t tstart = (uint32_t)(tc0 * (int64_t)T);
   const uint32_t tend = last_tile ? EV_TIME_INF : (uint32_t)((tc0 + TC) * (int64_t)T);
-  const uint32_t lend = last_tile ? (uint32_t)a.t_max : tend;
-  const uint32_t t_max = (uint32_t)a.t_max;
-  // tile index: [net][n_tile + 1] by tile, or [net][n_work_tiles] by list position
-  const int64_t tile_stride = a.tile_list ? (int64_t)a.n_work_tiles : (int64_t)a.n_tile + 1;
-  const int tile_col = a.tile_list ? wt : tile;
+  const uint32_t lend = last_tile ? (uint32_t)wk.t_max : tend;
+  const uint32_t t_max = (uint32_t)wk.t_max;
 
   long long base_acc = 0;
   const int64_t gi_begin = (int64_t)range * a.range_gates;
@@ -296,14 +354,14 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
         csw[p] = a.pin_csw[pb + p];
         int64_t nb = 0;
         if (net >= 0) {
-          nb = a.net_off[net];
-          cnt[p] = (int32_t)(a.net_off[net + 1] - nb);
+          nb = a.net_off[net + wk.net_base];
+          cnt[p] = (int32_t)(a.net_off[net + wk.net_base + 1] - nb);
         }
         if (cnt[p] == 0) {
           cv[p] = a.pin_default[pb + p];
         } else {
           evp[p] = a.ev + nb;
-          const int32_t i0 = a.tile_idx[(int64_t)net * tile_stride + tile_col];
+          const int32_t i0 = wk.tidx[(int64_t)net * wk.tile_stride + wk.tile_col];
           const bool isclk = (cd.clock_mask >> p) & 1u;
           if (i0 == 0) {
             const uint32_t e0 = evp[p][0];
@@ -451,7 +509,7 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
           // closest to t, within the same cycle (TimeBasedPower.cc:412-473)
           const int o = p - nin;
           int rel = -1, relrf = 0;
-          int64_t best = (int64_t)a.t_max << ARC_FRAC_BITS;
+          int64_t best = (int64_t)wk.t_max << ARC_FRAC_BITS;
 #pragma unroll
           for (int q = 0; q < MAXP; ++q) {
             if (q >= nin || cnt[q] == 0) continue;
@@ -536,6 +594,8 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
   __syncthreads();
   long long* out = a.cyc_part + ((int64_t)wt * a.n_range + range) * N_BIN * TC;
   for (int i = threadIdx.x; i < N_BIN * TC; i += blockDim.x) out[i] = bins[i];
+  __syncthreads();  // the bins are cleared for the next item
+  }
 }
 
 
@@ -649,7 +709,7 @@ __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_
 // ---------------------------------------------------------------------------
 template <int NP>
 __global__ __launch_bounds__(256) void power_cell_kernel(PowerArgs a, int32_t range_begin,
-                                                          int32_t range_count, int32_t cell) {
+                                                          int32_t range_count, int32_t cell, unsigned int* wq) {
   constexpr int NIN = NP - 1;
   constexpr int OUT = NP - 1;
   constexpr int NS = 6 * NIN + 2;
@@ -662,40 +722,35 @@ __global__ __launch_bounds__(256) void power_cell_kernel(PowerArgs a, int32_t ra
   const int BW = N_BIN * TC;  // one copy of the bins
   long long* bins = reinterpret_cast<long long*>(smem_raw);
 
-  const int64_t n_work = (int64_t)a.n_work_tiles * range_count;
-  int64_t w = blockIdx.x;
-  if (a.xcd_remap) {
-    const int64_t per = (n_work + 7) / 8;
-    w = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
-  }
-  if (w >= n_work) return;
-  const int wt = (int)(w / range_count);
-  const int range = range_begin + (int)(w - (int64_t)wt * range_count);
-  const int tile = a.tile_list ? a.tile_list[wt] : wt;
-
+  __shared__ long long s_item;
   const CellDev cd = a.cells[cell];
   const uint32_t clk = cd.clock_mask;
-  for (int i = threadIdx.x; i < BW * BIN_COPIES; i += blockDim.x) bins[i] = 0;
   for (int i = threadIdx.x; i < NSTATE; i += blockDim.x) {
     s_leak[i] = a.leak[cd.leak_off + i];
     s_leakq[i] = a.leak_q[cd.leak_off + i];
   }
+  long long* mybins = bins + (threadIdx.x % BIN_COPIES) * BW;
+
+  const int64_t n_work = (int64_t)a.n_work_tiles * range_count;
+  int64_t w = 0;
+  for (int64_t it = 0; next_work_item(a, wq, n_work, it, w, &s_item); ++it) {
+  const int wt = (int)(w / range_count);
+  const int range = range_begin + (int)(w - (int64_t)wt * range_count);
+  const WorkTile wk = resolve_work_tile(a, wt);
+  const int tile = wk.tile;
+  for (int i = threadIdx.x; i < BW * BIN_COPIES; i += blockDim.x) bins[i] = 0;
   if (threadIdx.x == 0) s_base = 0;
   __syncthreads();
-  long long* mybins = bins + (threadIdx.x % BIN_COPIES) * BW;
 
   const uint32_t T = (uint32_t)a.T;
   const float invT = a.inv_T;
   const int64_t tc0 = (int64_t)tile * TC;
-  const int tcn = (int)min((int64_t)TC, a.n_cycle - tc0);
-  const bool last_tile = tile == a.n_tile - 1;
+  const int tcn = (int)min((int64_t)TC, wk.n_cycle - tc0);
+  const bool last_tile = tile == wk.n_tile - 1;
   const uint32_t tstart = (uint32_t)(tc0 * (int64_t)T);
   const uint32_t tend = last_tile ? EV_TIME_INF : (uint32_t)((tc0 + TC) * (int64_t)T);
-  const uint32_t lend = last_tile ? (uint32_t)a.t_max : tend;
-  const uint32_t t_max = (uint32_t)a.t_max;
-  // tile index: [net][n_tile + 1] by tile, or [net][n_work_tiles] by list position
-  const int64_t tile_stride = a.tile_list ? (int64_t)a.n_work_tiles : (int64_t)a.n_tile + 1;
-  const int tile_col = a.tile_list ? wt : tile;
+  const uint32_t lend = last_tile ? (uint32_t)wk.t_max : tend;
+  const uint32_t t_max = (uint32_t)wk.t_max;
   const float inv_qe = a.inv_q_energy, inv_ql = a.inv_q_leak;
   const int64_t aux_delta = a.aux - a.ev;
 
@@ -749,15 +804,15 @@ __global__ __launch_bounds__(256) void power_cell_kernel(PowerArgs a, int32_t ra
       if (p == OUT) csw = a.pin_csw[pb + p];
       int64_t nb = 0, cn = 0;
       if (net >= 0) {
-        nb = a.net_off[net];
-        cn = a.net_off[net + 1] - nb;
+        nb = a.net_off[net + wk.net_base];
+        cn = a.net_off[net + wk.net_base + 1] - nb;
       }
       if (cn == 0) {
         cv[p] = a.pin_default[pb + p];
       } else {
         const uint32_t* evp = a.ev + nb;
         endp[p] = evp + cn;
-        const int32_t i0 = a.tile_idx[(int64_t)net * tile_stride + tile_col];
+        const int32_t i0 = wk.tidx[(int64_t)net * wk.tile_stride + wk.tile_col];
         int32_t i = i0;
         if (i0 == 0) {
           const uint32_t e0 = evp[0];
@@ -943,10 +998,12 @@ __global__ __launch_bounds__(256) void power_cell_kernel(PowerArgs a, int32_t ra
   __syncthreads();
   long long* out = a.cyc_part + ((int64_t)wt * a.n_range + range) * N_BIN * TC;
   for (int i = threadIdx.x; i < BW; i += blockDim.x) out[i] = bins[i];
+  __syncthreads();  // the bins are cleared for the next item
+  }
 }
 
-template __global__ void power_tile_kernel<MAXP_SMALL>(PowerArgs, int32_t, int32_t);
-template __global__ void power_tile_kernel<MAXP_LARGE>(PowerArgs, int32_t, int32_t);
+template __global__ void power_tile_kernel<MAXP_SMALL>(PowerArgs, int32_t, int32_t, unsigned int*);
+template __global__ void power_tile_kernel<MAXP_LARGE>(PowerArgs, int32_t, int32_t, unsigned int*);
 
 // ---------------------------------------------------------------------------
 // Reductions
@@ -1007,6 +1064,89 @@ __global__ void cycle_reduce_list_kernel(const long long* __restrict__ part, int
   out[2 * n_out + j] = (double)s[BIN_GINT] * q_energy * inv_period;
   out[3 * n_out + j] = (double)(s[BIN_SW] + s[BIN_GSW]) * q_energy * inv_period;
   out[4 * n_out + j] = (double)s[BIN_GSW] * q_energy * inv_period;
+}
+
+// Batched runs: per-cycle outputs of the work tiles of one round.  Column
+// j = wt * tile_cycles + i is cycle tile_list[wt] * tile_cycles + i of trace
+// wt_trace[wt], written to out + out_base[trace] as [5][n_cycle of the trace].
+__global__ void cycle_reduce_batch_kernel(const long long* __restrict__ part, int32_t n_range,
+                                          int32_t tile_cycles, const int32_t* __restrict__ tile_list,
+                                          const int32_t* __restrict__ wt_trace, int32_t n_wt,
+                                          const TraceDesc* __restrict__ traces,
+                                          const int64_t* __restrict__ out_base, int64_t T, double q_energy,
+                                          double q_leak, double inv_period, double* __restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= (int64_t)n_wt * tile_cycles) return;
+  const int64_t l = j / tile_cycles;
+  const int ci = (int)(j % tile_cycles);
+  const int tr = wt_trace[l];
+  const TraceDesc td = traces[tr];
+  const int64_t c = (int64_t)tile_list[l] * tile_cycles + ci;
+  if (c >= td.n_cycle) return;
+  long long s[N_BIN] = {0, 0, 0, 0, 0, 0};
+  for (int r = 0; r < n_range; ++r) {
+    const long long* p = part + ((l * n_range + r) * N_BIN) * tile_cycles + ci;
+#pragma unroll
+    for (int k = 0; k < N_BIN; ++k) s[k] += p[(int64_t)k * tile_cycles];
+  }
+  const int64_t cend = min((c + 1) * T, td.t_max);
+  const double frac = (double)(cend - c * T) / (double)T;
+  double* o = out + out_base[tr];
+  const int64_t C = td.n_cycle;
+  o[0 * C + c] = ((double)s[BIN_LD] * frac + (double)s[BIN_LW]) * q_leak;
+  o[1 * C + c] = (double)(s[BIN_INT] + s[BIN_GINT]) * q_energy * inv_period;
+  o[2 * C + c] = (double)s[BIN_GINT] * q_energy * inv_period;
+  o[3 * C + c] = (double)(s[BIN_SW] + s[BIN_GSW]) * q_energy * inv_period;
+  o[4 * C + c] = (double)s[BIN_GSW] * q_energy * inv_period;
+}
+
+// Batched runs: add one round's per-gate partials into exact per-trace sums.
+// Work tiles of a trace are contiguous in the round: [wt_begin[k], wt_begin[k+1])
+// for the round's k-th trace round_trace[k]; grid.y = traces in the round.
+__global__ void gate_accumulate_kernel(const long long* __restrict__ pg, const uint32_t* __restrict__ tog,
+                                       const uint32_t* __restrict__ glc, const int32_t* __restrict__ order,
+                                       int64_t n_sched, int32_t n_wt, const int32_t* __restrict__ wt_begin,
+                                       const int32_t* __restrict__ round_trace, int64_t n_gate,
+                                       long long* __restrict__ acc /* [trace][N_PG][G] */,
+                                       unsigned long long* __restrict__ acc_cnt /* [trace][2][G] */) {
+  const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= n_sched) return;
+  const int g = order[gi];
+  if (g < 0) return;
+  const int k = blockIdx.y;
+  const int tr = round_trace[k];
+  const int64_t plane = (int64_t)n_wt * n_sched;
+  long long v[N_PG] = {0, 0, 0, 0, 0};
+  unsigned long long tg = 0, gg = 0;
+  for (int t = wt_begin[k]; t < wt_begin[k + 1]; ++t) {
+#pragma unroll
+    for (int c = 0; c < N_PG; ++c) v[c] += pg[c * plane + (int64_t)t * n_sched + gi];
+    tg += tog[(int64_t)t * n_sched + gi];
+    gg += glc[(int64_t)t * n_sched + gi];
+  }
+  long long* A = acc + (int64_t)tr * N_PG * n_gate;
+#pragma unroll
+  for (int c = 0; c < N_PG; ++c) A[(int64_t)c * n_gate + g] += v[c];
+  acc_cnt[((int64_t)tr * 2 + 0) * n_gate + g] += tg;
+  acc_cnt[((int64_t)tr * 2 + 1) * n_gate + g] += gg;
+}
+
+// Batched runs: per-gate power of one trace from its exact sums.
+__global__ void gate_finalize_kernel(const long long* __restrict__ acc, int64_t n_gate, double norm_energy,
+                                     double norm_leak, double* __restrict__ gate_power) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)N_PG * n_gate) return;
+  gate_power[i] = (double)acc[i] * (i / n_gate == PG_LEAK ? norm_leak : norm_energy);
+}
+
+__global__ void fill_u64_kernel(unsigned long long* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+__global__ void fill_u32_zero_kernel(unsigned int* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
 }
 
 __global__ void gate_reduce_kernel(const long long* __restrict__ pg, const uint32_t* __restrict__ tog,
@@ -1101,12 +1241,12 @@ hipError_t launch_energy(const EnergyArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_power(int maxp, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,
-                        int32_t range_begin, int32_t range_count) {
+                        int32_t range_begin, int32_t range_count, unsigned int* wq) {
   if (grid <= 0) return hipSuccess;
   if (maxp <= MAXP_SMALL)
-    power_tile_kernel<MAXP_SMALL><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count);
+    power_tile_kernel<MAXP_SMALL><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count, wq);
   else
-    power_tile_kernel<MAXP_LARGE><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count);
+    power_tile_kernel<MAXP_LARGE><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count, wq);
   return hipGetLastError();
 }
 
@@ -1127,6 +1267,48 @@ hipError_t launch_cycle_reduce_list(const long long* part, int32_t n_range, int3
   if (n <= 0) return hipSuccess;
   cycle_reduce_list_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(
       part, n_range, tile_cycles, tile_list, n_list, n_cycle, T, t_max, q_energy, q_leak, inv_period, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_cycle_reduce_batch(const long long* part, int32_t n_range, int32_t tile_cycles,
+                                     const int32_t* tile_list, const int32_t* wt_trace, int32_t n_wt,
+                                     const TraceDesc* traces, const int64_t* out_base, int64_t T, double q_energy,
+                                     double q_leak, double inv_period, double* out, hipStream_t s) {
+  const int64_t n = (int64_t)n_wt * tile_cycles;
+  if (n <= 0) return hipSuccess;
+  cycle_reduce_batch_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(
+      part, n_range, tile_cycles, tile_list, wt_trace, n_wt, traces, out_base, T, q_energy, q_leak, inv_period, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gate_accumulate(const long long* pg, const uint32_t* tog, const uint32_t* glc,
+                                  const int32_t* order, int64_t n_sched, int32_t n_wt, const int32_t* wt_begin,
+                                  const int32_t* round_trace, int32_t n_round_trace, int64_t n_gate, long long* acc,
+                                  unsigned long long* acc_cnt, hipStream_t s) {
+  if (n_sched <= 0 || n_round_trace <= 0) return hipSuccess;
+  gate_accumulate_kernel<<<dim3((unsigned)((n_sched + 255) / 256), (unsigned)n_round_trace), dim3(256), 0, s>>>(
+      pg, tog, glc, order, n_sched, n_wt, wt_begin, round_trace, n_gate, acc, acc_cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_gate_finalize(const long long* acc, int64_t n_gate, double norm_energy, double norm_leak,
+                                double* gate_power, hipStream_t s) {
+  const int64_t n = (int64_t)N_PG * n_gate;
+  if (n <= 0) return hipSuccess;
+  gate_finalize_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(acc, n_gate, norm_energy, norm_leak,
+                                                                              gate_power);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_u64(unsigned long long* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  fill_u64_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(unsigned int* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  fill_u32_zero_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(p, n);
   return hipGetLastError();
 }
 
@@ -1167,14 +1349,15 @@ hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, const uint32_t* 
 }
 
 hipError_t launch_power_cell(int np, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,
-                             int32_t range_begin, int32_t range_count, int32_t cell) {
+                             int32_t range_begin, int32_t range_count, int32_t cell, unsigned int* wq) {
   if (grid <= 0) return hipSuccess;
+  const dim3 g((unsigned)grid), b(block);
   switch (np) {
-    case 2: power_cell_kernel<2><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count, cell); break;
-    case 3: power_cell_kernel<3><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count, cell); break;
-    case 4: power_cell_kernel<4><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count, cell); break;
-    case 5: power_cell_kernel<5><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count, cell); break;
-    case 6: power_cell_kernel<6><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count, cell); break;
+    case 2: power_cell_kernel<2><<<g, b, lds, s>>>(a, range_begin, range_count, cell, wq); break;
+    case 3: power_cell_kernel<3><<<g, b, lds, s>>>(a, range_begin, range_count, cell, wq); break;
+    case 4: power_cell_kernel<4><<<g, b, lds, s>>>(a, range_begin, range_count, cell, wq); break;
+    case 5: power_cell_kernel<5><<<g, b, lds, s>>>(a, range_begin, range_count, cell, wq); break;
+    case 6: power_cell_kernel<6><<<g, b, lds, s>>>(a, range_begin, range_count, cell, wq); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

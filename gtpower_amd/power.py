@@ -105,6 +105,38 @@ class HipPower:
                                 tg, gl, r.n_cycle, r.kernel_ms, r.total_ms, sched)
         return self.last
 
+    def power_batch(self, traces, want=("gate", "cycle", "counts")):
+        """Evaluate several traces of the loaded design in one batched run
+        (gtp_run_batch); returns one PowerResult per trace."""
+        G = self.n_gate
+        views, outs, rs = [], [], []
+        for tr in traces:
+            v = tr.as_abi()
+            Cn = -(-tr.t_max // tr.ticks_per_cycle)
+            o = dict(gp=np.zeros(5 * G) if "gate" in want else None,
+                     cp=np.zeros(5 * Cn) if "cycle" in want else None,
+                     tot=np.zeros(5), cls=np.zeros(25),
+                     tg=np.zeros(G, np.uint64) if "counts" in want else None,
+                     gl=np.zeros(G, np.uint64) if "counts" in want else None, C=Cn)
+            views.append(v)
+            outs.append(o)
+            rs.append(abi.GtpResult(abi.ptr(o["gp"], C.c_double), abi.ptr(o["cp"], C.c_double),
+                                    abi.ptr(o["tot"], C.c_double), abi.ptr(o["cls"], C.c_double),
+                                    abi.ptr(o["tg"], C.c_uint64), abi.ptr(o["gl"], C.c_uint64), 0, 0.0, 0.0))
+        tv = (abi.GtpTrace * len(views))(*views)
+        rv = (abi.GtpResult * len(rs))(*rs)
+        self._check(self._lib.gtp_run_batch(self._h, len(views), tv, rv), "gtp_run_batch")
+        si = abi.GtpScheduleInfo()
+        self._check(self._lib.gtp_get_schedule(self._h, C.byref(si)), "gtp_get_schedule")
+        sched = {k: getattr(si, k) for k, _ in abi.GtpScheduleInfo._fields_}
+        res = []
+        for o, r in zip(outs, rv):
+            res.append(PowerResult(o["gp"].reshape(5, G) if o["gp"] is not None else None,
+                                   o["cp"].reshape(5, o["C"]) if o["cp"] is not None else None,
+                                   o["tot"], o["cls"].reshape(5, 5), o["tg"], o["gl"], r.n_cycle, r.kernel_ms,
+                                   r.total_ms, sched))
+        return res
+
     @property
     def n_cycle(self):
         T = self.trace.ticks_per_cycle

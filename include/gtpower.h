@@ -76,6 +76,8 @@ typedef struct {
   int64_t max_event_num;     /* device event budget (-max_event_num); 0 = no limit */
   int32_t host_threads;      /* host worker threads (-multi_thread_number); 0 = all cores */
   int32_t verbose;           /* 0 quiet, 1 timings to stderr */
+  int32_t round_tiles;       /* batched runs: work tiles per round; 0 = sized by device memory */
+  int32_t pad_;
 } gtp_options;
 
 /* A Liberty NLDM table, 1-D (n2 == 0) or 2-D, row-major values[i1 * n2 + i2].
@@ -219,8 +221,13 @@ gtp_status gtp_run(gtp_ctx* ctx, gtp_result* res);
 gtp_status gtp_run_tiles(gtp_ctx* ctx, int32_t tile_cycles, int32_t n_tiles, const int32_t* tiles,
                          double* cycle_power, double* kernel_ms);
 
-/* Evaluate several traces of one design in one persistent launch.  traces[k]
- * and results[k] pair up.  No counterpart in the reference (new method). */
+/* Evaluate several traces of one design (same time scale and clock) in
+ * persistent launches over all their tiles.  traces[k] and results[k] pair
+ * up; each result equals gtp_run's for that trace, and kernel_ms / total_ms
+ * hold the device time of the whole batch.  Traces are loaded in groups under
+ * the event budget (max_event_num, else device memory), each group processed
+ * in rounds of round_tiles work tiles.  The resident trace of gtp_set_trace is
+ * not touched.  No counterpart in the reference (new method). */
 gtp_status gtp_run_batch(gtp_ctx* ctx, int32_t n_trace, const gtp_trace* traces,
                          gtp_result* results);
 

@@ -180,3 +180,57 @@ def test_when_conditioned_library_uses_the_generic_kernel(library):
     tr = d.trace(300, toggle_p=0.3, seed=42)
     res, ref = run_both(d, tr)
     compare(res, ref, "when")
+
+
+def _batch_traces(d, lengths, seed0):
+    return [d.trace(n, toggle_p=p, seed=seed0 + i) for i, (n, p) in enumerate(lengths)]
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(round_tiles=3), dict(max_event_num=400000, round_tiles=2)])
+def test_batch_equals_individual_runs(opts):
+    """gtp_run_batch over traces of uneven length (some shorter than a tile)
+    gives each trace exactly what gtp_run gives it: counts and every power
+    value bit-identical (exact integer sums), across device loads and rounds."""
+    print(f"[batch] 1500 gates, 6 traces, opts={opts}", flush=True)
+    d = S.make_design(1500, dff_fraction=0.15, clock_ns=2.0, seed=41)
+    lengths = [(700, 0.15), (37, 0.3), (1200, 0.05), (129, 0.2), (1, 0.2), (400, 0.25)]
+    trs = _batch_traces(d, lengths, 500)
+    singles = []
+    with HipPower(tile_cycles=64) as eng:
+        eng.set_design(d.library, d)
+        for tr in trs:
+            eng.read_power_activities(tr)
+            singles.append(eng.power())
+    with HipPower(tile_cycles=64, **opts) as eng:
+        eng.set_design(d.library, d)
+        batch = eng.power_batch(trs)
+    for k, (a, b) in enumerate(zip(singles, batch)):
+        assert b.n_cycle == a.n_cycle
+        assert np.array_equal(a.gate_toggle_x2, b.gate_toggle_x2), k
+        assert np.array_equal(a.gate_glitch, b.gate_glitch), k
+        assert np.array_equal(a.cycle_power, b.cycle_power), k
+        assert np.array_equal(a.gate_power, b.gate_power), k
+        assert np.array_equal(a.total, b.total), k
+
+
+def test_batch_matches_oracle():
+    print("[batch] oracle parity of a 3-trace batch", flush=True)
+    d = S.make_design(800, dff_fraction=0.1, clock_ns=10.0, seed=43)
+    trs = _batch_traces(d, [(300, 0.2), (90, 0.3), (500, 0.1)], 700)
+    with HipPower() as eng:
+        eng.set_design(d.library, d)
+        batch = eng.power_batch(trs)
+    for k, tr in enumerate(trs):
+        compare(batch[k], oracle.run(d.library, d, tr, threads=0), f"batch trace {k}")
+
+
+def test_batch_rejects_mixed_clocks():
+    d = S.make_design(200, dff_fraction=0.1, clock_ns=10.0, seed=44)
+    t1 = d.trace(50, toggle_p=0.2, seed=1)
+    d2 = S.make_design(200, dff_fraction=0.1, clock_ns=5.0, seed=44)
+    t2 = d2.trace(50, toggle_p=0.2, seed=2)
+    from gtpower_amd import abi
+    with HipPower() as eng:
+        eng.set_design(d.library, d)
+        with pytest.raises(abi.GtpError):
+            eng.power_batch([t1, t2])
