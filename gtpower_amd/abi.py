@@ -144,6 +144,7 @@ def load_engine():
                                                     C.c_int64, p_i64]
     lib.gtp_write_cell_power.argtypes = [C.c_char_p, C.c_int64, p_f64, C.c_char_p, p_i32, C.c_char_p]
     lib.gtp_write_per_cycle.argtypes = [C.c_char_p, C.c_char_p, C.c_int64, p_f64]
+    lib.gtp_autotune.argtypes = [C.c_void_p, C.POINTER(GtpScheduleInfo)]
     lib.gtp_run_tiles.argtypes = [C.c_void_p, C.c_int32, C.c_int32, p_i32, p_f64, p_f64]
     p_f32 = C.POINTER(C.c_float)
     lib.gtp_mlp_create.argtypes = [C.POINTER(GtpMlpConfig), C.POINTER(C.c_void_p)]
@@ -167,7 +168,7 @@ def load_engine():
                                       C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, p_f32]
     for name in ("gtp_create", "gtp_set_design", "gtp_set_trace", "gtp_run", "gtp_run_batch",
                  "gtp_get_schedule", "gtp_reference_cycles_per_thread", "gtp_write_cell_power",
-                 "gtp_write_per_cycle", "gtp_run_tiles", "gtp_mlp_create", "gtp_mlp_get_params",
+                 "gtp_write_per_cycle", "gtp_run_tiles", "gtp_autotune", "gtp_mlp_create", "gtp_mlp_get_params",
                  "gtp_mlp_set_params", "gtp_mlp_gradients", "gtp_mlp_train", "gtp_mlp_predict",
                  "gtp_mlp_predict_trace", "gtp_feature_correlation", "gtp_toggle_features",
                  "gtp_mlp_gemm_test"):

@@ -170,6 +170,9 @@ struct gtp_ctx {
   // ranges are sized for its few tiles)
   OrderSet orders[3];              // [2]: batched runs
   int n_cu = 256;                  // compute units of the device
+  // autotuner's choice for the loaded trace (0 = none)
+  int32_t tuned_tc = 0, tuned_rg = 0;
+  double tune_ms = 0;
   DArr<unsigned int> d_wq;         // persistent work-queue counters, 8 per launch
   gtp_schedule_info sched{};
 
@@ -672,6 +675,8 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
   }
   x->P_ev = pev;
   for (auto& o : x->orders) o.order.clear();
+  x->tuned_tc = x->tuned_rg = 0;
+  x->tune_ms = 0;
   x->have_trace = true;
   return GTP_OK;
 }
@@ -832,9 +837,10 @@ gtp_status lds_sizes(gtp_ctx* x, const PowerArgs& a, RunPlan* rp) {
 // tc_req > 0 asks for exactly that tile length; n_work_tiles > 0 is the
 // number of tiles the launches will cover (a subset run), which sizes the
 // gate ranges and selects the subset run's cached schedule.
-gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, int32_t n_work_tiles, RunPlan* rp) {
+gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, int32_t n_work_tiles, RunPlan* rp, int32_t rg_req = 0) {
   hipStream_t s = x->stream;
-  int64_t TC = tc_req > 0 ? tc_req : (x->opt.tile_cycles > 0 ? x->opt.tile_cycles : 128);
+  int64_t TC = tc_req > 0 ? tc_req
+                          : (x->opt.tile_cycles > 0 ? x->opt.tile_cycles : (x->tuned_tc > 0 ? x->tuned_tc : 128));
   if (x->T < (1 << 24)) TC = std::min<int64_t>(TC, std::max<int64_t>(1, ((1 << 24) - 1) / x->T));
   TC = std::max<int64_t>(1, std::min<int64_t>(TC, x->C));
   TC = std::min<int64_t>(TC, 4096);
@@ -845,7 +851,7 @@ gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, int32_t n_work_tiles, RunPlan
   // gates per range: enough workgroups to fill the GPU several times over
   const bool subset = n_work_tiles > 0;
   const int32_t work_tiles = subset ? n_work_tiles : n_tile;
-  int32_t RG = subset ? 0 : x->opt.gates_per_wg;
+  int32_t RG = subset ? 0 : (rg_req > 0 ? rg_req : (x->opt.gates_per_wg > 0 ? x->opt.gates_per_wg : x->tuned_rg));
   if (RG <= 0) {
     const double want = (double)x->G * work_tiles / 8192.0;
     RG = subset ? 64 : 256;
@@ -965,17 +971,80 @@ struct EvSet {
 
 }  // namespace
 
+namespace {
+gtp_status run_once(gtp_ctx* x, gtp_result* res, int64_t tc_req, int32_t rg_req);
+}
+
+// Autotuner: times whole evaluations of the loaded trace over candidate tile
+// lengths (at the default gate ranges), then candidate gate-range sizes at the
+// best tile length, and keeps the fastest pair for this trace.
+GTP_API gtp_status gtp_autotune(gtp_ctx* x, gtp_schedule_info* best) {
+  if (!x) return GTP_ERR_ARG;
+  if (!x->have_trace) return x->fail(GTP_ERR_STATE, "gtp_autotune: call gtp_set_trace first");
+  HIPCHK(x, hipSetDevice(x->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  gtp_result r{};
+  auto time_of = [&](int64_t tc, int32_t rg, double* ms) -> gtp_status {
+    const gtp_status st = run_once(x, &r, tc, rg);
+    *ms = r.kernel_ms;
+    return st;
+  };
+  const int64_t tc_cap = std::min<int64_t>(std::min<int64_t>(4096, x->C),
+                                           x->T < (1 << 24) ? std::max<int64_t>(1, ((1 << 24) - 1) / x->T) : 1);
+  // a first evaluation warms the caches and builds the shared state
+  double ms = 0, best_ms = 0;
+  gtp_status st = time_of(std::min<int64_t>(128, tc_cap), 0, &ms);
+  if (st != GTP_OK) return st;
+  int64_t best_tc = 0;
+  int32_t best_rg = 0;
+  for (int64_t tc : {32, 64, 128, 256, 512}) {
+    if (tc > tc_cap) continue;
+    if ((st = time_of(tc, 0, &ms)) != GTP_OK) return st;
+    if (best_tc == 0 || ms < best_ms) {
+      best_ms = ms;
+      best_tc = tc;
+      best_rg = x->sched.gates_per_wg;
+    }
+  }
+  if (best_tc == 0) best_tc = tc_cap;
+  for (int32_t rg : {512, 1024, 2048, 4096, 8192}) {
+    if (rg == best_rg) continue;
+    if ((st = time_of(best_tc, rg, &ms)) != GTP_OK) return st;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best_rg = rg;
+    }
+  }
+  x->tuned_tc = (int32_t)best_tc;
+  x->tuned_rg = best_rg;
+  x->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  x->sched.tile_cycles = (int32_t)best_tc;
+  x->sched.gates_per_wg = best_rg;
+  x->sched.autotune_ms = x->tune_ms;
+  if (best) *best = x->sched;
+  return GTP_OK;
+}
+
 GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
   if (!x) return GTP_ERR_ARG;
   if (!x->have_trace) return x->fail(GTP_ERR_STATE, "gtp_run: call gtp_set_trace first");
   if (!res) return x->fail(GTP_ERR_ARG, "gtp_run: null result");
+  if (x->opt.autotune && x->tuned_tc == 0 && x->opt.tile_cycles <= 0) {
+    const gtp_status st = gtp_autotune(x, nullptr);
+    if (st != GTP_OK) return st;
+  }
+  return run_once(x, res, 0, 0);
+}
+
+namespace {
+gtp_status run_once(gtp_ctx* x, gtp_result* res, int64_t tc_req, int32_t rg_req) {
   HIPCHK(x, hipSetDevice(x->device));
   hipStream_t s = x->stream;
   EvSet ev;
   for (hipEvent_t& e : ev.e) HIPCHK(x, hipEventCreate(&e));
   HIPCHK(x, hipEventRecord(ev.e[0], s));
   RunPlan rp;
-  gtp_status st = prepare_run(x, 0, 0, &rp);
+  gtp_status st = prepare_run(x, tc_req, 0, &rp, rg_req);
   if (st != GTP_OK) return st;
   const PowerArgs& a = rp.a;
   HIPCHK(x, hipEventRecord(ev.e[1], s));
@@ -1020,9 +1089,10 @@ GTP_API gtp_status gtp_run(gtp_ctx* x, gtp_result* res) {
   si.n_tile = rp.n_tile;
   si.n_event = x->E;
   si.n_pin_event = x->P_ev;
-  si.autotune_ms = 0;
+  si.autotune_ms = x->tune_ms;
   return GTP_OK;
 }
+}  // namespace
 
 GTP_API gtp_status gtp_run_tiles(gtp_ctx* x, int32_t tile_cycles, int32_t n_list, const int32_t* tiles,
                                  double* cycle_power, double* kernel_ms) {
@@ -1125,7 +1195,9 @@ GTP_API gtp_status gtp_run_batch(gtp_ctx* x, int32_t n_trace, const gtp_trace* t
   }
   // tile length (common to the batch) and schedule: gates ordered by their
   // pin events summed over the batch
-  int64_t TC = x->opt.tile_cycles > 0 ? x->opt.tile_cycles : 128;
+  // tile length: the option, else the autotuner's choice on the resident
+  // trace (gtp_autotune), else 128
+  int64_t TC = x->opt.tile_cycles > 0 ? x->opt.tile_cycles : (x->tuned_tc > 0 ? x->tuned_tc : 128);
   if (T < (1 << 24)) TC = std::min<int64_t>(TC, std::max<int64_t>(1, ((1 << 24) - 1) / T));
   TC = std::max<int64_t>(1, std::min<int64_t>(TC, 4096));
   std::vector<int32_t> n_tile(n_trace);
@@ -1152,7 +1224,7 @@ GTP_API gtp_status gtp_run_batch(gtp_ctx* x, int32_t n_trace, const gtp_trace* t
   const int64_t budget_scratch = std::min<int64_t>((int64_t)8 << 30, (int64_t)(free_b / 4));
   int32_t W_round = (int32_t)std::max<int64_t>(1, std::min<int64_t>(total_tiles, (int64_t)(budget_scratch / per_tile_est)));
   if (x->opt.round_tiles > 0) W_round = (int32_t)std::min<int64_t>(x->opt.round_tiles, total_tiles);
-  int32_t RG = x->opt.gates_per_wg;
+  int32_t RG = x->opt.gates_per_wg > 0 ? x->opt.gates_per_wg : x->tuned_rg;
   if (RG <= 0) {
     const double want = (double)G * std::min<int64_t>(W_round, total_tiles) / 8192.0;
     RG = 256;

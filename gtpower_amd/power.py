@@ -105,6 +105,13 @@ class HipPower:
                                 tg, gl, r.n_cycle, r.kernel_ms, r.total_ms, sched)
         return self.last
 
+    def autotune(self):
+        """Pick tile length and gate-range size by timing the loaded trace
+        (gtp_autotune); returns the chosen schedule."""
+        si = abi.GtpScheduleInfo()
+        self._check(self._lib.gtp_autotune(self._h, C.byref(si)), "gtp_autotune")
+        return {k: getattr(si, k) for k, _ in abi.GtpScheduleInfo._fields_}
+
     def power_batch(self, traces, want=("gate", "cycle", "counts")):
         """Evaluate several traces of the loaded design in one batched run
         (gtp_run_batch); returns one PowerResult per trace."""

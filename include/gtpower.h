@@ -239,6 +239,15 @@ typedef struct {
 } gtp_schedule_info;
 gtp_status gtp_get_schedule(const gtp_ctx* ctx, gtp_schedule_info* info);
 
+/* Autotuner: times evaluations of the loaded trace over candidate tile lengths
+ * (32..512 cycles) and gate-range sizes (512..8192 gates per workgroup item),
+ * and keeps the fastest pair for later gtp_run and gtp_run_batch calls until
+ * the next gtp_set_trace (options that fix tile_cycles / gates_per_wg win).
+ * gtp_run calls it by itself when options.autotune is set.  Replaces the
+ * reference's fixed cycles-per-thread heuristic (power/cuda/Gate.cuh:354-376)
+ * as the schedule choice. */
+gtp_status gtp_autotune(gtp_ctx* ctx, gtp_schedule_info* best);
+
 /* The reference's event-density heuristic for cycles per thread
  * (Gate::setNCyclePerThreadByEventCount, power/cuda/Gate.cuh:354-376), for
  * n gates with n_event[g] events each over n_cycle cycles. */

@@ -234,3 +234,30 @@ def test_batch_rejects_mixed_clocks():
         eng.set_design(d.library, d)
         with pytest.raises(abi.GtpError):
             eng.power_batch([t1, t2])
+
+
+def test_autotune_keeps_results_and_feeds_batches():
+    """The autotuner only changes the schedule: results are bit-identical to
+    the default schedule's, its choice is one of its candidates, and batched
+    runs of the same context use it."""
+    print("[autotune] 3000 gates x 1500 cycles", flush=True)
+    d = S.make_design(3000, dff_fraction=0.15, clock_ns=2.0, seed=45)
+    tr = d.trace(1500, toggle_p=0.15, seed=46)
+    with HipPower() as eng:
+        eng.set_design(d.library, d)
+        eng.read_power_activities(tr)
+        ref = eng.power()
+    with HipPower(autotune=1) as eng:
+        eng.set_design(d.library, d)
+        eng.read_power_activities(tr)
+        res = eng.power()
+        sched = res.schedule
+        assert sched["autotune_ms"] > 0
+        assert sched["tile_cycles"] in (32, 64, 128, 256, 512)
+        assert sched["gates_per_wg"] in (256, 512, 1024, 2048, 4096, 8192)
+        batch = eng.power_batch([tr])
+        assert batch[0].schedule["tile_cycles"] == sched["tile_cycles"]
+    for r in (res, batch[0]):
+        assert np.array_equal(r.cycle_power, ref.cycle_power)
+        assert np.array_equal(r.gate_power, ref.gate_power)
+        assert np.array_equal(r.gate_toggle_x2, ref.gate_toggle_x2)
