@@ -69,6 +69,46 @@ def copy_calibration(mib, reps=5):
     log(f"copy calibration: {reps} copies of {mib} MiB (read {mib} MiB + write {mib} MiB each)")
 
 
+def batch_leg(eng, design, w, n_trace, max_cycles, seed):
+    """Batching: n_trace traces of the netlist with log-uniform lengths in
+    [max_cycles / 20, max_cycles] and toggle p in [0.05, 0.3], each with its own
+    seed, evaluated by one batched run (gtp_run_batch) and, for comparison, one
+    run per trace (gtp_set_trace + gtp_run)."""
+    import numpy as np
+    from gtpower_amd import synthetic as S
+    rng = np.random.default_rng(seed)
+    lo = max(1, max_cycles // 20)
+    lengths = np.exp(rng.uniform(np.log(lo), np.log(max_cycles), n_trace)).astype(int)
+    ps = rng.uniform(0.05, 0.3, n_trace)
+    tk = dict(w["trace"])
+    t0 = time.perf_counter()
+    trs = []
+    for i in range(n_trace):
+        tk["n_cycle"] = int(lengths[i])
+        tk["toggle_p"] = float(ps[i])
+        trs.append(design.trace(seed=seed + 7919 * (i + 1), **tk))
+    gen_s = time.perf_counter() - t0
+    n_cyc = int(sum(t.n_cycle for t in trs))
+    n_ev = int(sum(t.n_event for t in trs))
+    log(f"batch: {n_trace} traces, {n_cyc} cycles ({lengths.min()}..{lengths.max()}), {n_ev} events, "
+        f"generated in {gen_s:.1f} s")
+    eng.power_batch(trs, want=("cycle",))  # warm-up
+    b = eng.power_batch(trs, want=("cycle",))
+    batch_ms = b[0].kernel_ms
+    single_ms = 0.0
+    for tr in trs:
+        eng.read_power_activities(tr)
+        single_ms += eng.power(want=("cycle",)).kernel_ms
+    for tr in trs:
+        tr.close()
+    eng.trace = None
+    log(f"batch: one batched run {batch_ms:.2f} ms vs one run per trace {single_ms:.2f} ms")
+    return {"n_trace": n_trace, "cycles": n_cyc, "events": n_ev, "min_cycles": int(lengths.min()),
+            "max_cycles": int(lengths.max()), "batch_ms": batch_ms, "per_trace_runs_ms": single_ms,
+            "cycles_per_s": n_cyc / (batch_ms * 1e-3), "speedup_vs_per_trace_runs": single_ms / batch_ms,
+            "tile_cycles": b[0].schedule.get("tile_cycles"), "gates_per_wg": b[0].schedule.get("gates_per_wg")}
+
+
 def surrogate_leg(eng, design, exact_cycle_power, w, n_fresh, steps, exact_cps):
     """Train on the first 50k exact cycles, infer on a fresh trace, judge 1%."""
     import numpy as np
@@ -140,6 +180,9 @@ def main():
     ap.add_argument("--tile-cycles", type=int, default=0)
     ap.add_argument("--gates-per-wg", type=int, default=0)
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--batch-traces", type=int, default=64)
+    ap.add_argument("--batch-max-cycles", type=int, default=2000)
     ap.add_argument("--surrogate-cycles", type=int, default=100000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
     args = ap.parse_args()
@@ -175,6 +218,10 @@ def main():
     t2 = time.perf_counter()
     log(f"set_design {t1 - t0:.2f} s, event store upload+build {t2 - t1:.2f} s")
 
+    if not args.no_autotune and not args.tile_cycles and not args.gates_per_wg:
+        sched = eng.autotune()
+        log(f"autotune: tile_cycles {sched['tile_cycles']}, gates_per_wg {sched['gates_per_wg']} "
+            f"({sched['autotune_ms'] / 1e3:.1f} s)")
     for i in range(args.warmup):
         r = eng.power()
         log(f"warmup {i}: {r.total_ms:.2f} ms (kernel {r.kernel_ms:.2f} ms)")
@@ -205,6 +252,10 @@ def main():
         surrogate = surrogate_leg(eng, design, r.cycle_power, w,
                                   min(args.surrogate_cycles, args.cycles or args.surrogate_cycles),
                                   3, value)
+    # after the surrogate leg: the batch replaces the resident trace
+    batch = None
+    if args.batch_traces > 0:
+        batch = batch_leg(eng, design, w, args.batch_traces, args.batch_max_cycles, S.MASTER_SEED ^ 0xBA7C4)
     eng.close()
 
     cpu = None
@@ -239,7 +290,8 @@ def main():
         "config": {"workload": args.workload, "path": "exact", "n_gate": design.n_gate, "n_pin": design.n_pin,
                    "n_cycle": n_cycle, "n_event": n_event, "clock_ns": trace.clk_period * 1e9,
                    "events_per_cycle": n_event / n_cycle, "tile_cycles": sched.get("tile_cycles"),
-                   "gates_per_wg": sched.get("gates_per_wg"), "kernel_ms": avg_kernel_ms,
+                   "gates_per_wg": sched.get("gates_per_wg"), "autotune_ms": sched.get("autotune_ms"),
+                   "kernel_ms": avg_kernel_ms,
                    "generate_s": round(gen_s, 2)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
@@ -249,6 +301,7 @@ def main():
                                "launching stream)"},
         "cpu_baseline": cpu,
         "surrogate": surrogate,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
 
