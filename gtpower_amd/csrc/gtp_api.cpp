@@ -568,7 +568,7 @@ gtp_status validate_trace(gtp_ctx* x, const gtp_trace* tr, int64_t* T_out) {
   if (!(tr->time_scale > 0) || !(tr->clk_period > 0) || tr->t_max <= 0)
     return x->fail(GTP_ERR_ARG, "gtp_trace: time_scale, clk_period and t_max must be positive");
   if (tr->t_max >= (int64_t)EV_TIME_INF)
-    return x->fail(GTP_ERR_UNSUPPORTED, "gtp_trace: t_max must be below 2^30 - 1 ticks");
+    return x->fail(GTP_ERR_UNSUPPORTED, "gtp_trace: t_max must be below 2^29 - 1 ticks");
   const int64_t N = x->N;
   const int64_t* off = tr->net_event_offset;
   if (off[0] != 0) return x->fail(GTP_ERR_ARG, "gtp_trace: net_event_offset[0] != 0");
@@ -630,7 +630,7 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
   HIPCHK(x, x->d_net_off.upload(off, N + 1, s));
   x->h_net_off.assign(off, off + N + 1);
   x->aux_valid = false;
-  HIPCHK(x, x->d_ev.alloc(std::max<int64_t>(E, 1)));
+  HIPCHK(x, x->d_ev.alloc(E + EV_PAD));
   // chunked upload + pack into the 32-bit event store
   {
     const int64_t CH = 1 << 26;
@@ -1299,7 +1299,7 @@ GTP_API gtp_status gtp_run_batch(gtp_ctx* x, int32_t n_trace, const gtp_trace* t
     DArr<int64_t> d_voff, d_chunk;
     DArr<uint32_t> d_ev, d_aux;
     HIPCHK(x, d_voff.upload(voff.data(), voff.size(), s));
-    HIPCHK(x, d_ev.alloc(std::max<int64_t>(E_load, 1)));
+    HIPCHK(x, d_ev.alloc(E_load + EV_PAD));
     HIPCHK(x, d_aux.alloc(std::max<int64_t>(E_load, 1)));
     {
       const int64_t CH = 1 << 26;

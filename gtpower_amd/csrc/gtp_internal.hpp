@@ -2,9 +2,13 @@
 // kernels (gtp_kernels.hip).
 //
 // Device event store: one 32-bit word per event, per-net CSR order,
-//   word = (time << 2) | value,  time < 2^30 ticks, value in {0,1,2,3}.
-// Times compare as words (the value bits are below the time bits), so a
+//   word = (time << 3) | (glitch << 2) | value,
+//   time < 2^29 - 1 ticks, value in {0,1,2,3}, glitch set by the glitch
+//   annotation when the event's glitch pulse width (aux) is nonzero.
+// Times compare as words (the low bits are below the time bits), so a
 // lower_bound on time is a lower_bound on words.  EV_INF is past every event.
+// The array is padded by 4 words so that 16-byte loads of the last quad stay
+// inside it.
 //
 // Results are accumulated as signed 64-bit fixed point: every FP32-evaluated
 // term x is added as rint(x / quantum), with a power-of-two quantum such that
@@ -18,9 +22,11 @@
 
 namespace gtp {
 
-constexpr uint32_t EV_TIME_BITS = 30;
-constexpr uint32_t EV_INF = 0xFFFFFFFFu;       // time field 2^30 - 1
-constexpr uint32_t EV_TIME_INF = 0x3FFFFFFFu;
+constexpr uint32_t EV_SHIFT = 3;                // time bits start here
+constexpr uint32_t EV_GLITCH = 4u;              // the event carries a glitch pulse width
+constexpr uint32_t EV_INF = 0xFFFFFFFFu;       // time field 2^29 - 1
+constexpr uint32_t EV_TIME_INF = 0x1FFFFFFFu;
+constexpr int64_t EV_PAD = 4;                   // words of padding after the event array
 constexpr int MAXP_SMALL = 4;                   // kernel variant for cells with <= 4 pins
 constexpr int MAXP_LARGE = 6;                   // GTP_MAX_CELL_PINS
 constexpr int MAX_CELLS_LDS = 64;               // cell types staged in LDS
@@ -91,8 +97,9 @@ struct PowerArgs {
   const int32_t* arc_q;          // [sum arcs] delay in 2^-16 ticks, ARC_NONE = no arc
   // event store
   const int64_t* net_off;        // [N+1]
-  const uint32_t* ev;            // [E]
-  const uint32_t* aux;           // [E] glitch pulse width of each event (0 = not a glitch)
+  const uint32_t* ev;            // [E + EV_PAD]
+  const uint32_t* aux;           // [E] glitch pulse width of each event (0 = not a glitch); read
+                                 // only for events whose word has EV_GLITCH set
   const int32_t* tile_idx;       // [N][n_tile + 1] first event index >= tile start (net-relative);
                                  // [N][n_work_tiles] by list position when tile_list is set
   int32_t n_tile;
@@ -179,10 +186,10 @@ hipError_t launch_gate_reduce(const long long* pg, const uint32_t* tog, const ui
                               unsigned long long* gate_tog, unsigned long long* gate_glitch, hipStream_t s);
 constexpr int TOTAL_BLOCKS = 256;
 constexpr int BIN_COPIES = 4;  // lane-interleaved copies of the per-cycle bins (specialised kernel)
-hipError_t launch_glitch_annotate(const int64_t* net_off, const uint32_t* ev, const int64_t* chunk_off,
+hipError_t launch_glitch_annotate(const int64_t* net_off, uint32_t* ev, const int64_t* chunk_off,
                                   int64_t n_net, int64_t n_chunk, uint32_t T, float invT, uint32_t* aux,
                                   hipStream_t s);
-hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, const uint32_t* ev, const int32_t* idx_begin,
+hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, uint32_t* ev, const int32_t* idx_begin,
                                         const int32_t* tile_list, int32_t n_list, int64_t n_net, int64_t tile_ticks,
                                         uint32_t T, float invT, uint32_t* aux, hipStream_t s);
 hipError_t launch_total(const double* gate_power, const int32_t* gate_class, int64_t n_gate, double* part,

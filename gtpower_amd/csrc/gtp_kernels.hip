@@ -24,7 +24,7 @@
 
 namespace gtp {
 
-__device__ __forceinline__ uint32_t ev_time(uint32_t w) { return w >> 2; }
+__device__ __forceinline__ uint32_t ev_time(uint32_t w) { return w >> EV_SHIFT; }
 __device__ __forceinline__ int ev_val(uint32_t w) { return (int)(w & 3u); }
 
 // power/PowerUtils.cc:246-256 getNToggle and :213-236 getNRise, times two.
@@ -36,7 +36,7 @@ __device__ __forceinline__ int rise_x2(int p, int c) {
   return (c == 2 && p == 0) ? 1 : 0;
 }
 
-// floor(t / T) for t < 2^30: a float estimate, then integer correction.
+// floor(t / T) for t < 2^29: a float estimate, then integer correction.
 __device__ __forceinline__ uint32_t cycle_of(uint32_t t, uint32_t T, float invT) {
   uint32_t q = (uint32_t)((float)t * invT);
   int32_t r = (int32_t)(t - q * T);
@@ -77,7 +77,7 @@ __global__ void pack_events_kernel(const int64_t* __restrict__ t, const uint8_t*
                                    uint32_t* __restrict__ out, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) out[i] = ((uint32_t)t[i] << 2) | (uint32_t)(v[i] & 3u);
+  for (; i < n; i += stride) out[i] = ((uint32_t)t[i] << EV_SHIFT) | (uint32_t)(v[i] & 3u);
 }
 
 // out[n][k] = first event of net n at or after the start of tile k, for
@@ -100,7 +100,7 @@ __global__ void tile_index_kernel(const int64_t* __restrict__ net_off, const uin
       lo = cnt;
     } else {
       const int64_t tk = (int64_t)k * tile_ticks;
-      const uint32_t key = tk >= (int64_t)EV_TIME_INF ? EV_INF : (uint32_t)(tk << 2);
+      const uint32_t key = tk >= (int64_t)EV_TIME_INF ? EV_INF : (uint32_t)(tk << EV_SHIFT);
       while (lo < hi) {
         int32_t mid = (lo + hi) >> 1;
         if (ev[b + mid] < key) lo = mid + 1; else hi = mid;
@@ -263,7 +263,9 @@ __device__ __forceinline__ bool next_work_item(const PowerArgs& a, unsigned int*
   __syncthreads();  // every thread has read the previous item
   if (threadIdx.x == 0) *s_item = (long long)atomicAdd(&wq[xcd], 1u);
   __syncthreads();
-  const int64_t k = *s_item;
+  // the item is the same for the whole workgroup: keep it (and what is derived
+  // from it) in scalar registers
+  const int64_t k = (int64_t)__builtin_amdgcn_readfirstlane((int)*s_item);
   w = (int64_t)xcd * per + k;
   return k < per && w < n_work;
 }
@@ -654,7 +656,7 @@ __device__ __forceinline__ uint32_t annotate_one(const uint32_t* e, int64_t i, i
 }
 
 __global__ __launch_bounds__(256) void glitch_annotate_kernel(const int64_t* __restrict__ net_off,
-                                                              const uint32_t* __restrict__ ev,
+                                                              uint32_t* __restrict__ ev,
                                                               const int64_t* __restrict__ chunk_off,
                                                               int64_t n_net, uint32_t T, float invT,
                                                               uint32_t* __restrict__ aux) {
@@ -674,14 +676,19 @@ __global__ __launch_bounds__(256) void glitch_annotate_kernel(const int64_t* __r
   const int64_t cnt = net_off[n + 1] - b;
   const int64_t i = (chunk - chunk_off[n]) * 256 + threadIdx.x;
   if (i >= cnt) return;
-  aux[b + i] = annotate_one(ev + b, i, cnt, T, invT);
+  // other lanes read this word's time while the flag bit is rewritten: the
+  // time bits do not change
+  const uint32_t pw = annotate_one(ev + b, i, cnt, T, invT);
+  aux[b + i] = pw;
+  const uint32_t w = ev[b + i];
+  ev[b + i] = pw ? (w | EV_GLITCH) : (w & ~EV_GLITCH);
 }
 
 // The same for the events of listed tiles only: thread (n, l) annotates the
 // events of net n from idx_begin[n][l] up to (not including) the first event
 // at or after the end of tile tile_list[l].
 __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_t* __restrict__ net_off,
-                                                                    const uint32_t* __restrict__ ev,
+                                                                    uint32_t* __restrict__ ev,
                                                                     const int32_t* __restrict__ idx_begin,
                                                                     const int32_t* __restrict__ tile_list,
                                                                     int32_t n_list, int64_t n_net, int64_t tile_ticks,
@@ -693,8 +700,12 @@ __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_
   const int64_t b = net_off[n];
   const int64_t cnt = net_off[n + 1] - b;
   const int64_t tend = ((int64_t)tile_list[i - n * n_list] + 1) * tile_ticks;
-  const uint32_t* e = ev + b;
-  for (int64_t k = idx_begin[i]; k < cnt && (int64_t)ev_time(e[k]) < tend; ++k) aux[b + k] = annotate_one(e, k, cnt, T, invT);
+  uint32_t* e = ev + b;
+  for (int64_t k = idx_begin[i]; k < cnt && (int64_t)ev_time(e[k]) < tend; ++k) {
+    const uint32_t pw = annotate_one(e, k, cnt, T, invT);
+    aux[b + k] = pw;
+    e[k] = pw ? (e[k] | EV_GLITCH) : (e[k] & ~EV_GLITCH);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -707,9 +718,11 @@ __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_
 // bins so that lanes working on the same cycle do not serialise on one address.
 // Semantics are those of power_tile_kernel.
 // ---------------------------------------------------------------------------
+// Occupancy targets by pin count (waves per SIMD): 4 for up to 3 pins, 3 for
+// 4 pins, 2 above; they fit the register budget without spilling.
 template <int NP>
-__global__ __launch_bounds__(256) void power_cell_kernel(PowerArgs a, int32_t range_begin,
-                                                          int32_t range_count, int32_t cell, unsigned int* wq) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NP <= 3 ? 4 : (NP == 4 ? 3 : 2))))
+void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, int32_t cell, unsigned int* wq) {
   constexpr int NIN = NP - 1;
   constexpr int OUT = NP - 1;
   constexpr int NS = 6 * NIN + 2;
@@ -782,7 +795,6 @@ __global__ __launch_bounds__(256) void power_cell_kernel(PowerArgs a, int32_t ra
     const uint32_t* cur[NP];   // next unconsumed event
     const uint32_t* endp[NP];
     uint32_t w0[NP], w1[NP];   // words of the next two events
-    uint32_t x0[NP], x1[NP];   // their glitch pulse widths
     uint32_t lt[NP];           // time of the last consumed event (for the related-pin search)
     int cv[NP], rfl[NP];       // current value; edge of the last consumed event (0 rise, 1 fall)
     bool first[NP];            // no event consumed yet and event 0 is next
@@ -794,7 +806,6 @@ __global__ __launch_bounds__(256) void power_cell_kernel(PowerArgs a, int32_t ra
       cur[p] = a.ev;
       endp[p] = a.ev;
       w0[p] = w1[p] = EV_INF;
-      x0[p] = x1[p] = 0;
       lt[p] = 0;
       cv[p] = 2;
       rfl[p] = 1;
@@ -831,14 +842,8 @@ __global__ __launch_bounds__(256) void power_cell_kernel(PowerArgs a, int32_t ra
           if (i0 - 1 >= 1) rfl[p] = rise_x2(ev_val(evp[i0 - 2]), cv[p]) > 0 ? 0 : 1;
         }
         cur[p] = evp + i;
-        if (cur[p] < endp[p]) {
-          w0[p] = cur[p][0];
-          x0[p] = cur[p][aux_delta];
-        }
-        if (cur[p] + 1 < endp[p]) {
-          w1[p] = cur[p][1];
-          x1[p] = cur[p][aux_delta + 1];
-        }
+        if (cur[p] < endp[p]) w0[p] = cur[p][0];
+        if (cur[p] + 1 < endp[p]) w1[p] = cur[p][1];
       }
       if (cv[p] == 1) sb |= 1u << p;
     }
@@ -904,7 +909,8 @@ __global__ __launch_bounds__(256) void power_cell_kernel(PowerArgs a, int32_t ra
         tog += (uint32_t)tx2;
         float ratio = -1.f;
         if (!((clk >> p) & 1u)) {
-          const uint32_t pw = x0[p];
+          // the glitch width is read only for events the annotation flagged
+          const uint32_t pw = (w0[p] & EV_GLITCH) ? cur[p][aux_delta] : 0u;
           if (pw) {
             const float r = (float)pw * gk[p];
             ratio = fminf(1.f, r * r);
@@ -957,13 +963,7 @@ __global__ __launch_bounds__(256) void power_cell_kernel(PowerArgs a, int32_t ra
         cv[p] = nv;
         ++cur[p];
         w0[p] = w1[p];
-        x0[p] = x1[p];
-        if (cur[p] + 1 < endp[p]) {
-          w1[p] = cur[p][1];
-          x1[p] = cur[p][aux_delta + 1];
-        } else {
-          w1[p] = EV_INF;
-        }
+        w1[p] = cur[p] + 1 < endp[p] ? cur[p][1] : EV_INF;
       }
       sb = sa;
       prev_t = t;
@@ -1330,7 +1330,7 @@ hipError_t launch_total(const double* gate_power, const int32_t* gate_class, int
   return hipGetLastError();
 }
 
-hipError_t launch_glitch_annotate(const int64_t* net_off, const uint32_t* ev, const int64_t* chunk_off,
+hipError_t launch_glitch_annotate(const int64_t* net_off, uint32_t* ev, const int64_t* chunk_off,
                                   int64_t n_net, int64_t n_chunk, uint32_t T, float invT, uint32_t* aux,
                                   hipStream_t s) {
   if (n_chunk <= 0) return hipSuccess;
@@ -1338,7 +1338,7 @@ hipError_t launch_glitch_annotate(const int64_t* net_off, const uint32_t* ev, co
   return hipGetLastError();
 }
 
-hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, const uint32_t* ev, const int32_t* idx_begin,
+hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, uint32_t* ev, const int32_t* idx_begin,
                                         const int32_t* tile_list, int32_t n_list, int64_t n_net, int64_t tile_ticks,
                                         uint32_t T, float invT, uint32_t* aux, hipStream_t s) {
   const int64_t n = n_net * n_list;

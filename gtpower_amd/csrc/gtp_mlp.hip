@@ -374,7 +374,7 @@ __global__ __launch_bounds__(256) void toggle_bits_kernel(const int64_t* __restr
   if (i >= 1 && i < cnt) {
     const uint32_t w = ev[b + i];
     if (!rise_only || (w & 3u) == 1u) {
-      c = cycle_of_tick(w >> 2, T, invT) - c0;
+      c = cycle_of_tick(w >> EV_SHIFT, T, invT) - c0;
       if (c >= n_cyc) c = -1;
     }
   }
@@ -416,15 +416,15 @@ __global__ __launch_bounds__(256) void net_corr_kernel(const int64_t* __restrict
   int64_t c = 0;
   if (i >= 1 && i < cnt) {
     const uint32_t w = ev[b + i];
-    c = cycle_of_tick(w >> 2, T, invT);
+    c = cycle_of_tick(w >> EV_SHIFT, T, invT);
     if (c < n_cyc && (!mask || mask[c])) {
-      take = i == 1 || cycle_of_tick(ev[b + i - 1] >> 2, T, invT) != c;
+      take = i == 1 || cycle_of_tick(ev[b + i - 1] >> EV_SHIFT, T, invT) != c;
       if ((w & 3u) == 1u) {
         // the first change to 1 of this net in cycle c
         take_rise = true;
         for (int64_t j = i - 1; j >= 1; --j) {
           const uint32_t wj = ev[b + j];
-          if (cycle_of_tick(wj >> 2, T, invT) != c) break;
+          if (cycle_of_tick(wj >> EV_SHIFT, T, invT) != c) break;
           if ((wj & 3u) == 1u) {
             take_rise = false;
             break;
