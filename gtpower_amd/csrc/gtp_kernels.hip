@@ -853,123 +853,140 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
 #pragma unroll
     for (int k = 0; k < N_PG; ++k) acc[k] = 0;
     uint32_t tog = 0, glc = 0;
-    int pc = -1;
-    long long pq[N_BIN];     // this gate's sums for the pending cycle pc
-#pragma unroll
-    for (int k = 0; k < N_BIN; ++k) pq[k] = 0;
     uint32_t prev_t = tstart;
+    long long lq_cur = s_leakq[sb];  // leakage of the current state, in quanta and in W
+    float lf_cur = s_leak[sb];
 
-    auto flush = [&]() {
-      if (pc >= 0 && pc < tcn) {
-#pragma unroll
-        for (int k = 0; k < N_BIN; ++k)
-          if (pq[k] != 0) atomicAdd((unsigned long long*)&mybins[k * TC + pc], (unsigned long long)pq[k]);
-      }
-      acc[PG_INT] += pq[BIN_INT];
-      acc[PG_GINT] += pq[BIN_GINT];
-      acc[PG_SW] += pq[BIN_SW];
-      acc[PG_GSW] += pq[BIN_GSW];
-#pragma unroll
-      for (int k = 0; k < N_BIN; ++k) pq[k] = 0;
-    };
-
+    // One pin event per step: the earliest pending event, the output pin's
+    // first when several pins change at the same time (so that the related
+    // input search sees only input events strictly before it).  Pins changing
+    // together thus take successive steps of zero duration, which leaves the
+    // leakage integral unchanged.  Everything per pin is picked by selects, so
+    // the step has no branch per pin.
     for (;;) {
-      uint32_t wm = w0[0];
+      int ps = OUT;
+      uint32_t ws = w0[OUT];
+      uint32_t t = ev_time(ws);
 #pragma unroll
-      for (int p = 1; p < NP; ++p) wm = min(wm, w0[p]);
-      const uint32_t t = ev_time(wm);
+      for (int p = 0; p < NIN; ++p) {
+        const uint32_t tp = ev_time(w0[p]);
+        if (tp < t) {
+          t = tp;
+          ws = w0[p];
+          ps = p;
+        }
+      }
       if (t >= tend) break;
+      int pv = cv[0];
+      float gks = gk[0];
+      const uint32_t* cps = cur[0];
+      const uint32_t* eps = endp[0];
+      bool fs = first[0];
+#pragma unroll
+      for (int p = 1; p < NP; ++p)
+        if (p == ps) {
+          pv = cv[p];
+          gks = gk[p];
+          cps = cur[p];
+          eps = endp[p];
+          fs = first[p];
+        }
+      const bool is_out = ps == OUT;
+      const int nv = ev_val(ws);
       const uint32_t c = cycle_of(t, T, invT);
       const uint32_t cstart = c * T, cend = cstart + T;
       const int cr = (int)((int64_t)c - tc0);
-      if (cr != pc) {
-        flush();
-        pc = cr;
-      }
-      bool trig[NP];
-      uint32_t sa = sb;
-#pragma unroll
-      for (int p = 0; p < NP; ++p) {
-        trig[p] = ev_time(w0[p]) == t;
-        if (trig[p]) sa = (sa & ~(1u << p)) | ((ev_val(w0[p]) == 1 ? 1u : 0u) << p);
-      }
-      // leakage of the previous state over [prev_t, t), exact in quanta * ticks
-      acc[PG_LEAK] += (long long)(int)s_leakq[sb] * (long long)(t - prev_t);
-      if (sa != sb) {
-        pq[BIN_LD] += s_leakq[sa] - s_leakq[sb];
-        pq[BIN_LW] += qterm((s_leak[sa] - s_leak[sb]) * ((float)(min(cend, t_max) - t) * invT), inv_ql);
-      }
-#pragma unroll
-      for (int p = 0; p < NP; ++p) {
-        if (!trig[p]) continue;
-        const int pv = cv[p], nv = ev_val(w0[p]);
-        const int tx2 = toggle_x2(pv, nv);
-        const int r2 = rise_x2(pv, nv);
-        const int rf = r2 > 0 ? 0 : 1;
-        tog += (uint32_t)tx2;
-        float ratio = -1.f;
-        if (!((clk >> p) & 1u)) {
-          // the glitch width is read only for events the annotation flagged
-          const uint32_t pw = (w0[p] & EV_GLITCH) ? cur[p][aux_delta] : 0u;
-          if (pw) {
-            const float r = (float)pw * gk[p];
-            ratio = fminf(1.f, r * r);
-          }
-        }
-        if (ratio > 0.f) ++glc;
-        float En;
-        if (p < NIN) {
-          En = rf ? E[2 * p + 1] : E[2 * p];
-        } else {
-          // related input: latest event before t in the same cycle whose
-          // arc-delayed arrival is closest to t (TimeBasedPower.cc:412-473)
-          int32_t best = INT32_MAX;  // every same-cycle candidate is closer than t_max
-          En = rf ? E[6 * NIN + 1] : E[6 * NIN];  // default when none qualifies
-#pragma unroll
-          for (int q = 0; q < NIN; ++q) {
-            const bool hi_rf = rfl[q] != 0, lo_rf = rf != 0;
-            const int32_t d01 = lo_rf ? D[q * 4 + 1] : D[q * 4 + 0];
-            const int32_t d23 = lo_rf ? D[q * 4 + 3] : D[q * 4 + 2];
-            const int32_t d = hi_rf ? d23 : d01;
-            const float e01 = lo_rf ? E[2 * NIN + q * 4 + 1] : E[2 * NIN + q * 4 + 0];
-            const float e23 = lo_rf ? E[2 * NIN + q * 4 + 3] : E[2 * NIN + q * 4 + 2];
-            const float eq = hi_rf ? e23 : e01;
-            const bool ok = endp[q] != a.ev && d != ARC_NONE && lt[q] >= cstart && lt[q] < cend;
-            // within one cycle t - lt < T < 2^15, so the grid difference fits 32 bits
-            int32_t diff = (int32_t)((t - lt[q]) << ARC_FRAC_BITS) - d;
-            diff = diff < 0 ? -diff : diff;
-            if (ok && diff < best) {
-              best = diff;
-              En = eq;
-            }
-          }
-          if (r2 > 0) {
-            const float es = 0.5f * (float)r2 * csw;
-            if (ratio > 0.f) pq[BIN_GSW] += qterm(ratio * es, inv_qe);
-            else pq[BIN_SW] += qterm(es, inv_qe);
-          }
-        }
-        const float e = En * (0.5f * (float)tx2);
-        if (ratio > 0.f) pq[BIN_GINT] += qterm(ratio * e, inv_qe);
-        else pq[BIN_INT] += qterm(e, inv_qe);
-      }
-#pragma unroll
-      for (int p = 0; p < NP; ++p) {
-        if (!trig[p]) continue;
-        const int nv = ev_val(w0[p]);
-        rfl[p] = first[p] ? 1 : (rise_x2(cv[p], nv) > 0 ? 0 : 1);
-        first[p] = false;
-        lt[p] = t;
-        cv[p] = nv;
-        ++cur[p];
-        w0[p] = w1[p];
-        w1[p] = cur[p] + 1 < endp[p] ? cur[p][1] : EV_INF;
+      const bool in_tile = cr < tcn;
+      long long* cb = mybins + cr;
+
+      // leakage: the previous state over [prev_t, t), exact in quanta * ticks;
+      // the step to the new state enters this cycle's bins
+      acc[PG_LEAK] += (long long)(int)lq_cur * (long long)(t - prev_t);
+      const uint32_t sa = (sb & ~(1u << ps)) | ((nv == 1 ? 1u : 0u) << ps);
+      const long long lq_new = s_leakq[sa];
+      const float lf_new = s_leak[sa];
+      if (sa != sb && in_tile) {
+        atomicAdd((unsigned long long*)&cb[BIN_LD * TC], (unsigned long long)(lq_new - lq_cur));
+        atomicAdd((unsigned long long*)&cb[BIN_LW * TC],
+                  (unsigned long long)qterm((lf_new - lf_cur) * ((float)(min(cend, t_max) - t) * invT), inv_ql));
       }
       sb = sa;
+      lq_cur = lq_new;
+      lf_cur = lf_new;
+
+      const int tx2 = toggle_x2(pv, nv);
+      const int r2 = rise_x2(pv, nv);
+      const int rf = r2 > 0 ? 0 : 1;
+      tog += (uint32_t)tx2;
+      // glitch scaling; the width is read only for events the annotation flagged
+      float ratio = -1.f;
+      if (!((clk >> ps) & 1u) && (ws & EV_GLITCH)) {
+        const uint32_t pw = cps[aux_delta];
+        const float r = (float)pw * gks;
+        ratio = fminf(1.f, r * r);
+      }
+      const bool gl = ratio > 0.f;
+      glc += gl ? 1u : 0u;
+
+      // input pin: its table at its own edge; output pin: the related input's
+      float En = rf ? E[1] : E[0];
+#pragma unroll
+      for (int p = 1; p < NIN; ++p)
+        if (p == ps) En = rf ? E[2 * p + 1] : E[2 * p];
+      if (is_out) {
+        // related input: latest event before t in the same cycle whose
+        // arc-delayed arrival is closest to t (TimeBasedPower.cc:412-473)
+        int32_t best = INT32_MAX;  // every same-cycle candidate is closer than t_max
+        En = rf ? E[6 * NIN + 1] : E[6 * NIN];  // default when none qualifies
+#pragma unroll
+        for (int q = 0; q < NIN; ++q) {
+          const bool hi_rf = rfl[q] != 0, lo_rf = rf != 0;
+          const int32_t d01 = lo_rf ? D[q * 4 + 1] : D[q * 4 + 0];
+          const int32_t d23 = lo_rf ? D[q * 4 + 3] : D[q * 4 + 2];
+          const int32_t d = hi_rf ? d23 : d01;
+          const float e01 = lo_rf ? E[2 * NIN + q * 4 + 1] : E[2 * NIN + q * 4 + 0];
+          const float e23 = lo_rf ? E[2 * NIN + q * 4 + 3] : E[2 * NIN + q * 4 + 2];
+          const float eq = hi_rf ? e23 : e01;
+          const bool ok = endp[q] != a.ev && d != ARC_NONE && lt[q] >= cstart && lt[q] < cend;
+          // within one cycle t - lt < T < 2^15, so the grid difference fits 32 bits
+          int32_t diff = (int32_t)((t - lt[q]) << ARC_FRAC_BITS) - d;
+          diff = diff < 0 ? -diff : diff;
+          if (ok && diff < best) {
+            best = diff;
+            En = eq;
+          }
+        }
+        if (r2 > 0) {
+          const float es = 0.5f * (float)r2 * csw;
+          const long long qs = qterm(gl ? ratio * es : es, inv_qe);
+          if (gl) acc[PG_GSW] += qs;
+          else acc[PG_SW] += qs;
+          if (in_tile) atomicAdd((unsigned long long*)&cb[(gl ? BIN_GSW : BIN_SW) * TC], (unsigned long long)qs);
+        }
+      }
+      const float e = En * (0.5f * (float)tx2);
+      const long long qi = qterm(gl ? ratio * e : e, inv_qe);
+      if (gl) acc[PG_GINT] += qi;
+      else acc[PG_INT] += qi;
+      if (in_tile && qi != 0)
+        atomicAdd((unsigned long long*)&cb[(gl ? BIN_GINT : BIN_INT) * TC], (unsigned long long)qi);
+
+      // consume the event of pin ps
+      const uint32_t wn = cps + 2 < eps ? cps[2] : EV_INF;
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+        if (p == ps) {
+          rfl[p] = fs ? 1 : rf;
+          first[p] = false;
+          lt[p] = t;
+          cv[p] = nv;
+          cur[p] = cps + 1;
+          w0[p] = w1[p];
+          w1[p] = wn;
+        }
       prev_t = t;
     }
-    if (lend > prev_t) acc[PG_LEAK] += (long long)(int)s_leakq[sb] * (long long)(lend - prev_t);
-    flush();
+    if (lend > prev_t) acc[PG_LEAK] += (long long)(int)lq_cur * (long long)(lend - prev_t);
     const int64_t slot = (int64_t)wt * a.n_sched + gi;
     const int64_t plane = (int64_t)a.n_work_tiles * a.n_sched;
 #pragma unroll
