@@ -811,6 +811,7 @@ void fill_args(gtp_ctx* x, const gtp_ctx::OrderSet& os, const Quanta& qt, int64_
   a.tile_cycles = (int32_t)TC;
   a.T = T;
   a.inv_T = (float)(1.0 / (double)T);
+  cycle_magic(T, &a.cyc_m, &a.cyc_sh);
   a.n_range = os.n_range;
   a.range_gates = os.range_gates;
   a.xcd_remap = 1;

@@ -108,6 +108,11 @@ struct PowerArgs {
   int64_t t_max;
   int64_t n_cycle;
   float inv_T;
+  // floor(t / T) = (t * cyc_m) >> cyc_sh exactly for t < 2^29: cyc_sh = 29 +
+  // ceil(log2 T), cyc_m = ceil(2^cyc_sh / T) (Granlund and Montgomery, 1994,
+  // "Division by invariant integers using multiplication", theorem 4.2)
+  uint32_t cyc_m;
+  int32_t cyc_sh;
   // work decomposition
   int32_t n_range;               // gate ranges per tile (all classes)
   int32_t range_gates;           // gates per range; order[] is padded with -1 to whole ranges
@@ -196,6 +201,16 @@ hipError_t launch_total(const double* gate_power, const int32_t* gate_class, int
                         double* out, hipStream_t s);
 hipError_t launch_power_cell(int np, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,
                              int32_t range_begin, int32_t range_count, int32_t cell, unsigned int* wq);
+
+// The multiply-shift pair of PowerArgs.cyc_m / cyc_sh for T ticks per cycle.
+inline void cycle_magic(int64_t T, uint32_t* m, int32_t* sh) {
+  int l = 0;
+  while ((int64_t(1) << l) < T) ++l;
+  const int s = 29 + l;
+  *sh = s;
+  *m = (uint32_t)(((unsigned __int128)1 << s) / (unsigned __int128)T +
+                  ((((unsigned __int128)1 << s) % (unsigned __int128)T) ? 1 : 0));
+}
 
 // Read-only view of a context's device event store, for the passes that run
 // over it besides the power kernels (toggle features, correlation).

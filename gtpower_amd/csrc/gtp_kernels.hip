@@ -45,6 +45,11 @@ __device__ __forceinline__ uint32_t cycle_of(uint32_t t, uint32_t T, float invT)
   return q;
 }
 
+// floor(t / T) by the exact multiply-shift of PowerArgs (t < 2^29).
+__device__ __forceinline__ uint32_t cycle_mul(uint32_t t, uint32_t m, int sh) {
+  return (uint32_t)(((uint64_t)m * (uint64_t)t) >> sh);
+}
+
 // Compensated accumulation (TwoSum): hi + lo carries ~48 significant bits.
 __device__ __forceinline__ void ff_add(float& hi, float& lo, float x) {
   float s = hi + x;
@@ -792,9 +797,13 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
       asm volatile("" : "+v"(D[k]));
     }
 
+    // per pin: the next four event words in flight (w0 is the next event),
+    // and the glitch widths of the next two, read one step ahead when the
+    // annotation flagged the event (0 otherwise)
     const uint32_t* cur[NP];   // next unconsumed event
     const uint32_t* endp[NP];
-    uint32_t w0[NP], w1[NP];   // words of the next two events
+    uint32_t w0[NP], w1[NP], w2[NP], w3[NP];
+    uint32_t x0[NP], x1[NP];
     uint32_t lt[NP];           // time of the last consumed event (for the related-pin search)
     int cv[NP], rfl[NP];       // current value; edge of the last consumed event (0 rise, 1 fall)
     bool first[NP];            // no event consumed yet and event 0 is next
@@ -805,7 +814,8 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
     for (int p = 0; p < NP; ++p) {
       cur[p] = a.ev;
       endp[p] = a.ev;
-      w0[p] = w1[p] = EV_INF;
+      w0[p] = w1[p] = w2[p] = w3[p] = EV_INF;
+      x0[p] = x1[p] = 0;
       lt[p] = 0;
       cv[p] = 2;
       rfl[p] = 1;
@@ -842,8 +852,13 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
           if (i0 - 1 >= 1) rfl[p] = rise_x2(ev_val(evp[i0 - 2]), cv[p]) > 0 ? 0 : 1;
         }
         cur[p] = evp + i;
-        if (cur[p] < endp[p]) w0[p] = cur[p][0];
-        if (cur[p] + 1 < endp[p]) w1[p] = cur[p][1];
+        const int64_t left = cn - i;
+        if (left > 0) w0[p] = cur[p][0];
+        if (left > 1) w1[p] = cur[p][1];
+        if (left > 2) w2[p] = cur[p][2];
+        if (left > 3) w3[p] = cur[p][3];
+        if (left > 0 && (w0[p] & EV_GLITCH)) x0[p] = cur[p][aux_delta];
+        if (left > 1 && (w1[p] & EV_GLITCH)) x1[p] = cur[p][aux_delta + 1];
       }
       if (cv[p] == 1) sb |= 1u << p;
     }
@@ -882,6 +897,7 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
       const uint32_t* cps = cur[0];
       const uint32_t* eps = endp[0];
       bool fs = first[0];
+      uint32_t pws = x0[0];
 #pragma unroll
       for (int p = 1; p < NP; ++p)
         if (p == ps) {
@@ -890,10 +906,11 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
           cps = cur[p];
           eps = endp[p];
           fs = first[p];
+          pws = x0[p];
         }
       const bool is_out = ps == OUT;
       const int nv = ev_val(ws);
-      const uint32_t c = cycle_of(t, T, invT);
+      const uint32_t c = cycle_mul(t, a.cyc_m, a.cyc_sh);
       const uint32_t cstart = c * T, cend = cstart + T;
       const int cr = (int)((int64_t)c - tc0);
       const bool in_tile = cr < tcn;
@@ -920,59 +937,66 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
       tog += (uint32_t)tx2;
       // glitch scaling; the width is read only for events the annotation flagged
       float ratio = -1.f;
-      if (!((clk >> ps) & 1u) && (ws & EV_GLITCH)) {
-        const uint32_t pw = cps[aux_delta];
-        const float r = (float)pw * gks;
+      if (!((clk >> ps) & 1u) && pws != 0u) {
+        const float r = (float)pws * gks;
         ratio = fminf(1.f, r * r);
       }
       const bool gl = ratio > 0.f;
       glc += gl ? 1u : 0u;
 
-      // input pin: its table at its own edge; output pin: the related input's
-      float En = rf ? E[1] : E[0];
+      // input pin: its table at its own edge
+      float En_in = rf ? E[1] : E[0];
 #pragma unroll
       for (int p = 1; p < NIN; ++p)
-        if (p == ps) En = rf ? E[2 * p + 1] : E[2 * p];
-      if (is_out) {
-        // related input: latest event before t in the same cycle whose
-        // arc-delayed arrival is closest to t (TimeBasedPower.cc:412-473)
-        int32_t best = INT32_MAX;  // every same-cycle candidate is closer than t_max
-        En = rf ? E[6 * NIN + 1] : E[6 * NIN];  // default when none qualifies
+        if (p == ps) En_in = rf ? E[2 * p + 1] : E[2 * p];
+      // output pin: the related input's table, the related input being the
+      // latest event before t in the same cycle whose arc-delayed arrival is
+      // closest to t (TimeBasedPower.cc:412-473).  Evaluated on every step
+      // (no branch) and selected.
+      int32_t best = INT32_MAX;  // every same-cycle candidate is closer than t_max
+      float En_out = rf ? E[6 * NIN + 1] : E[6 * NIN];  // default when none qualifies
 #pragma unroll
-        for (int q = 0; q < NIN; ++q) {
-          const bool hi_rf = rfl[q] != 0, lo_rf = rf != 0;
-          const int32_t d01 = lo_rf ? D[q * 4 + 1] : D[q * 4 + 0];
-          const int32_t d23 = lo_rf ? D[q * 4 + 3] : D[q * 4 + 2];
-          const int32_t d = hi_rf ? d23 : d01;
-          const float e01 = lo_rf ? E[2 * NIN + q * 4 + 1] : E[2 * NIN + q * 4 + 0];
-          const float e23 = lo_rf ? E[2 * NIN + q * 4 + 3] : E[2 * NIN + q * 4 + 2];
-          const float eq = hi_rf ? e23 : e01;
-          const bool ok = endp[q] != a.ev && d != ARC_NONE && lt[q] >= cstart && lt[q] < cend;
-          // within one cycle t - lt < T < 2^15, so the grid difference fits 32 bits
-          int32_t diff = (int32_t)((t - lt[q]) << ARC_FRAC_BITS) - d;
-          diff = diff < 0 ? -diff : diff;
-          if (ok && diff < best) {
-            best = diff;
-            En = eq;
-          }
-        }
-        if (r2 > 0) {
-          const float es = 0.5f * (float)r2 * csw;
-          const long long qs = qterm(gl ? ratio * es : es, inv_qe);
-          if (gl) acc[PG_GSW] += qs;
-          else acc[PG_SW] += qs;
-          if (in_tile) atomicAdd((unsigned long long*)&cb[(gl ? BIN_GSW : BIN_SW) * TC], (unsigned long long)qs);
-        }
+      for (int q = 0; q < NIN; ++q) {
+        const bool hi_rf = rfl[q] != 0, lo_rf = rf != 0;
+        const int32_t d01 = lo_rf ? D[q * 4 + 1] : D[q * 4 + 0];
+        const int32_t d23 = lo_rf ? D[q * 4 + 3] : D[q * 4 + 2];
+        const int32_t d = hi_rf ? d23 : d01;
+        const float e01 = lo_rf ? E[2 * NIN + q * 4 + 1] : E[2 * NIN + q * 4 + 0];
+        const float e23 = lo_rf ? E[2 * NIN + q * 4 + 3] : E[2 * NIN + q * 4 + 2];
+        const float eq = hi_rf ? e23 : e01;
+        const bool ok = endp[q] != a.ev && d != ARC_NONE && lt[q] >= cstart && lt[q] < cend;
+        // within one cycle t - lt < T < 2^15, so the grid difference fits 32 bits
+        int32_t diff = (int32_t)((t - lt[q]) << ARC_FRAC_BITS) - d;
+        diff = diff < 0 ? -diff : diff;
+        const bool take = ok && diff < best;
+        best = take ? diff : best;
+        En_out = take ? eq : En_out;
+      }
+      const float En = is_out ? En_out : En_in;
+      // switching: rising output edges
+      {
+        const float es = is_out ? 0.5f * (float)r2 * csw : 0.f;
+        const long long qs = qterm(gl ? ratio * es : es, inv_qe);
+        acc[PG_GSW] += gl ? qs : 0;
+        acc[PG_SW] += gl ? 0 : qs;
+        if (in_tile && qs != 0)
+          atomicAdd((unsigned long long*)&cb[(gl ? BIN_GSW : BIN_SW) * TC], (unsigned long long)qs);
       }
       const float e = En * (0.5f * (float)tx2);
       const long long qi = qterm(gl ? ratio * e : e, inv_qe);
-      if (gl) acc[PG_GINT] += qi;
-      else acc[PG_INT] += qi;
+      acc[PG_GINT] += gl ? qi : 0;
+      acc[PG_INT] += gl ? 0 : qi;
       if (in_tile && qi != 0)
         atomicAdd((unsigned long long*)&cb[(gl ? BIN_GINT : BIN_INT) * TC], (unsigned long long)qi);
 
-      // consume the event of pin ps
-      const uint32_t wn = cps + 2 < eps ? cps[2] : EV_INF;
+      // consume the event of pin ps: the word four ahead, and the glitch width
+      // of the event that becomes second
+      const uint32_t wn = cps + 4 < eps ? cps[4] : EV_INF;
+      uint32_t w2s = w2[0];
+#pragma unroll
+      for (int p = 1; p < NP; ++p)
+        if (p == ps) w2s = w2[p];
+      const uint32_t xn = (w2s & EV_GLITCH) ? cps[aux_delta + 2] : 0u;
 #pragma unroll
       for (int p = 0; p < NP; ++p)
         if (p == ps) {
@@ -982,7 +1006,11 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
           cv[p] = nv;
           cur[p] = cps + 1;
           w0[p] = w1[p];
-          w1[p] = wn;
+          w1[p] = w2[p];
+          w2[p] = w3[p];
+          w3[p] = wn;
+          x0[p] = x1[p];
+          x1[p] = xn;
         }
       prev_t = t;
     }
