@@ -140,6 +140,7 @@ struct gtp_ctx {
   DArr<uint32_t> d_ev;
   DArr<uint32_t> d_aux;          // glitch pulse width per event
   DArr<int64_t> d_chunk_off;     // annotation blocks per net (prefix)
+  DArr<int32_t> d_chunk_net;     // net of each annotation block
   int64_t n_chunk = 0;
   DArr<int32_t> d_tile_idx;
   int32_t tile_idx_tc = 0;
@@ -654,6 +655,8 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
     for (int64_t n = 0; n < N; ++n) co[n + 1] = co[n] + (off[n + 1] - off[n] + 255) / 256;
     x->n_chunk = co[N];
     HIPCHK(x, x->d_chunk_off.upload(co.data(), co.size(), s));
+    HIPCHK(x, x->d_chunk_net.alloc(std::max<int64_t>(x->n_chunk, 1)));
+    HIPCHK(x, launch_chunk_net(x->d_chunk_off.p, N, x->d_chunk_net.p, s));
     HIPCHK(x, x->d_aux.alloc(std::max<int64_t>(E, 1)));
     HIPCHK(x, hipStreamSynchronize(s));
   }
@@ -1049,7 +1052,7 @@ gtp_status run_once(gtp_ctx* x, gtp_result* res, int64_t tc_req, int32_t rg_req)
   if (st != GTP_OK) return st;
   const PowerArgs& a = rp.a;
   HIPCHK(x, hipEventRecord(ev.e[1], s));
-  HIPCHK(x, launch_glitch_annotate(x->d_net_off.p, x->d_ev.p, x->d_chunk_off.p, x->N, x->n_chunk,
+  HIPCHK(x, launch_glitch_annotate(x->d_net_off.p, x->d_ev.p, x->d_chunk_off.p, x->d_chunk_net.p, x->n_chunk,
                                    (uint32_t)x->T, a.inv_T, x->d_aux.p, s));
   x->aux_valid = true;
   if ((st = launch_groups(x, rp)) != GTP_OK) return st;
@@ -1151,6 +1154,7 @@ int32_t gtp::trace_dev(gtp_ctx* x, TraceDev* out) {
   out->net_off = x->d_net_off.p;
   out->ev = x->d_ev.p;
   out->chunk_off = x->d_chunk_off.p;
+  out->chunk_net = x->d_chunk_net.p;
   out->n_chunk = x->n_chunk;
   out->host_net_off = x->h_net_off.data();
   return GTP_OK;
@@ -1325,6 +1329,9 @@ GTP_API gtp_status gtp_run_batch(gtp_ctx* x, int32_t n_trace, const gtp_trace* t
     for (size_t v = 0; v < (size_t)nt * N; ++v) co[v + 1] = co[v] + (voff[v + 1] - voff[v] + 255) / 256;
     const int64_t n_chunk = co[(size_t)nt * N];
     HIPCHK(x, d_chunk.upload(co.data(), co.size(), s));
+    DArr<int32_t> d_chunk_net;
+    HIPCHK(x, d_chunk_net.alloc(std::max<int64_t>(n_chunk, 1)));
+    HIPCHK(x, launch_chunk_net(d_chunk.p, (int64_t)nt * N, d_chunk_net.p, s));
     // per-trace tile indexes and descriptors; work tiles trace-major
     std::vector<TraceDesc> td(nt);
     int64_t tidx_total = 0;
@@ -1365,7 +1372,7 @@ GTP_API gtp_status gtp_run_batch(gtp_ctx* x, int32_t n_trace, const gtp_trace* t
     for (int32_t k = 0; k < nt; ++k)
       HIPCHK(x, launch_tile_index(d_voff.p + (int64_t)k * N, d_ev.p, N, n_tile[k0 + k], TC * T, nullptr, 0,
                                   d_tidx.p + td[k].tile_idx_base, s));
-    HIPCHK(x, launch_glitch_annotate(d_voff.p, d_ev.p, d_chunk.p, (int64_t)nt * N, n_chunk, (uint32_t)T, invT,
+    HIPCHK(x, launch_glitch_annotate(d_voff.p, d_ev.p, d_chunk.p, d_chunk_net.p, n_chunk, (uint32_t)T, invT,
                                      d_aux.p, s));
     rp.a.net_off = d_voff.p;
     rp.a.ev = d_ev.p;

@@ -191,8 +191,9 @@ hipError_t launch_gate_reduce(const long long* pg, const uint32_t* tog, const ui
                               unsigned long long* gate_tog, unsigned long long* gate_glitch, hipStream_t s);
 constexpr int TOTAL_BLOCKS = 256;
 constexpr int BIN_COPIES = 4;  // lane-interleaved copies of the per-cycle bins (specialised kernel)
+hipError_t launch_chunk_net(const int64_t* chunk_off, int64_t n_net, int32_t* chunk_net, hipStream_t s);
 hipError_t launch_glitch_annotate(const int64_t* net_off, uint32_t* ev, const int64_t* chunk_off,
-                                  int64_t n_net, int64_t n_chunk, uint32_t T, float invT, uint32_t* aux,
+                                  const int32_t* chunk_net, int64_t n_chunk, uint32_t T, float invT, uint32_t* aux,
                                   hipStream_t s);
 hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, uint32_t* ev, const int32_t* idx_begin,
                                         const int32_t* tile_list, int32_t n_list, int64_t n_net, int64_t tile_ticks,
@@ -221,6 +222,7 @@ struct TraceDev {
   const int64_t* net_off;        // device [N+1]
   const uint32_t* ev;            // device [E]
   const int64_t* chunk_off;      // device [N+1] 256-event blocks per net (prefix)
+  const int32_t* chunk_net;      // device [n_chunk] net of each block
   int64_t n_chunk;
   const int64_t* host_net_off;   // host [N+1]
 };

@@ -660,38 +660,39 @@ __device__ __forceinline__ uint32_t annotate_one(const uint32_t* e, int64_t i, i
   return pw > 0 ? (uint32_t)pw : 0u;
 }
 
+// chunk_net[c] = the net whose events block c covers (blocks of 256 events,
+// chunk_off[n] = first block of net n); one thread per net.
+__global__ void chunk_net_kernel(const int64_t* __restrict__ chunk_off, int64_t n_net, int32_t* __restrict__ chunk_net) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_net) return;
+  for (int64_t c = chunk_off[n]; c < chunk_off[n + 1]; ++c) chunk_net[c] = (int32_t)n;
+}
+
 __global__ __launch_bounds__(256) void glitch_annotate_kernel(const int64_t* __restrict__ net_off,
                                                               uint32_t* __restrict__ ev,
                                                               const int64_t* __restrict__ chunk_off,
-                                                              int64_t n_net, uint32_t T, float invT,
+                                                              const int32_t* __restrict__ chunk_net,
+                                                              uint32_t T, float invT,
                                                               uint32_t* __restrict__ aux) {
-  __shared__ int64_t s_net;
   const int64_t chunk = blockIdx.x;
-  if (threadIdx.x == 0) {
-    int64_t lo = 0, hi = n_net;  // last n with chunk_off[n] <= chunk
-    while (hi - lo > 1) {
-      const int64_t mid = (lo + hi) >> 1;
-      if (chunk_off[mid] <= chunk) lo = mid; else hi = mid;
-    }
-    s_net = lo;
-  }
-  __syncthreads();
-  const int64_t n = s_net;
+  const int64_t n = chunk_net[chunk];
   const int64_t b = net_off[n];
   const int64_t cnt = net_off[n + 1] - b;
   const int64_t i = (chunk - chunk_off[n]) * 256 + threadIdx.x;
   if (i >= cnt) return;
-  // other lanes read this word's time while the flag bit is rewritten: the
-  // time bits do not change
+  // The width is read only for flagged events, so only nonzero widths are
+  // written; the word is rewritten only when its flag changes (other lanes
+  // read its time meanwhile: the time bits do not change).
   const uint32_t pw = annotate_one(ev + b, i, cnt, T, invT);
-  aux[b + i] = pw;
+  if (pw) aux[b + i] = pw;
   const uint32_t w = ev[b + i];
-  ev[b + i] = pw ? (w | EV_GLITCH) : (w & ~EV_GLITCH);
+  const uint32_t nw = pw ? (w | EV_GLITCH) : (w & ~EV_GLITCH);
+  if (nw != w) ev[b + i] = nw;
 }
 
-// The same for the events of listed tiles only: thread (n, l) annotates the
-// events of net n from idx_begin[n][l] up to (not including) the first event
-// at or after the end of tile tile_list[l].
+// The same for the events of listed tiles only: the 8 threads of group
+// (n, l) annotate, interleaved, the events of net n from idx_begin[n][l] up to
+// (not including) the first event at or after the end of tile tile_list[l].
 __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_t* __restrict__ net_off,
                                                                     uint32_t* __restrict__ ev,
                                                                     const int32_t* __restrict__ idx_begin,
@@ -699,17 +700,21 @@ __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_
                                                                     int32_t n_list, int64_t n_net, int64_t tile_ticks,
                                                                     uint32_t T, float invT,
                                                                     uint32_t* __restrict__ aux) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = tid >> 3;
+  const int j = (int)(tid & 7);
   if (i >= n_net * n_list) return;
   const int64_t n = i / n_list;
   const int64_t b = net_off[n];
   const int64_t cnt = net_off[n + 1] - b;
   const int64_t tend = ((int64_t)tile_list[i - n * n_list] + 1) * tile_ticks;
   uint32_t* e = ev + b;
-  for (int64_t k = idx_begin[i]; k < cnt && (int64_t)ev_time(e[k]) < tend; ++k) {
+  for (int64_t k = idx_begin[i] + j; k < cnt && (int64_t)ev_time(e[k]) < tend; k += 8) {
     const uint32_t pw = annotate_one(e, k, cnt, T, invT);
-    aux[b + k] = pw;
-    e[k] = pw ? (e[k] | EV_GLITCH) : (e[k] & ~EV_GLITCH);
+    if (pw) aux[b + k] = pw;
+    const uint32_t w = e[k];
+    const uint32_t nw = pw ? (w | EV_GLITCH) : (w & ~EV_GLITCH);
+    if (nw != w) e[k] = nw;
   }
 }
 
@@ -1375,18 +1380,25 @@ hipError_t launch_total(const double* gate_power, const int32_t* gate_class, int
   return hipGetLastError();
 }
 
+hipError_t launch_chunk_net(const int64_t* chunk_off, int64_t n_net, int32_t* chunk_net, hipStream_t s) {
+  if (n_net <= 0) return hipSuccess;
+  chunk_net_kernel<<<dim3((unsigned)((n_net + 255) / 256)), dim3(256), 0, s>>>(chunk_off, n_net, chunk_net);
+  return hipGetLastError();
+}
+
 hipError_t launch_glitch_annotate(const int64_t* net_off, uint32_t* ev, const int64_t* chunk_off,
-                                  int64_t n_net, int64_t n_chunk, uint32_t T, float invT, uint32_t* aux,
+                                  const int32_t* chunk_net, int64_t n_chunk, uint32_t T, float invT, uint32_t* aux,
                                   hipStream_t s) {
   if (n_chunk <= 0) return hipSuccess;
-  glitch_annotate_kernel<<<dim3((unsigned)n_chunk), dim3(256), 0, s>>>(net_off, ev, chunk_off, n_net, T, invT, aux);
+  glitch_annotate_kernel<<<dim3((unsigned)n_chunk), dim3(256), 0, s>>>(net_off, ev, chunk_off, chunk_net, T, invT,
+                                                                      aux);
   return hipGetLastError();
 }
 
 hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, uint32_t* ev, const int32_t* idx_begin,
                                         const int32_t* tile_list, int32_t n_list, int64_t n_net, int64_t tile_ticks,
                                         uint32_t T, float invT, uint32_t* aux, hipStream_t s) {
-  const int64_t n = n_net * n_list;
+  const int64_t n = n_net * n_list * 8;
   if (n <= 0) return hipSuccess;
   glitch_annotate_tiles_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(
       net_off, ev, idx_begin, tile_list, n_list, n_net, tile_ticks, T, invT, aux);

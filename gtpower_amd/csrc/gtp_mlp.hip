@@ -346,25 +346,16 @@ __device__ __forceinline__ int64_t cycle_of_tick(uint32_t t, uint32_t T, float i
 // event of index >= 1 (a value change; index 0 is the initial value) in cycle
 // c0 + c, of any value (kind 0) or to value 1 (kind 1, a rise).  One block per
 // 256 events of a selected net: sel_chunk_off[k] is the first block of
-// feature k.
+// feature k, chunk_feat[b] the feature of block b.
 __global__ __launch_bounds__(256) void toggle_bits_kernel(const int64_t* __restrict__ net_off,
                                                           const uint32_t* __restrict__ ev,
                                                           const int32_t* __restrict__ sel_net,
-                                                          const int64_t* __restrict__ sel_chunk_off, int32_t K,
+                                                          const int64_t* __restrict__ sel_chunk_off,
+                                                          const int32_t* __restrict__ chunk_feat,
                                                           int64_t n_net, uint32_t T, float invT, int64_t c0,
                                                           int64_t n_cyc, int words, uint32_t* __restrict__ X) {
-  __shared__ int32_t s_k;
   const int64_t chunk = blockIdx.x;
-  if (threadIdx.x == 0) {
-    int lo = 0, hi = K;
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (sel_chunk_off[mid] <= chunk) lo = mid; else hi = mid;
-    }
-    s_k = lo;
-  }
-  __syncthreads();
-  const int k = s_k;
+  const int k = chunk_feat[chunk];
   const int64_t f = sel_net[k];
   const bool rise_only = f >= n_net;
   const int64_t n = rise_only ? f - n_net : f;
@@ -392,24 +383,15 @@ __global__ __launch_bounds__(256) void toggle_bits_kernel(const int64_t* __restr
 // (chunk_off as for the glitch annotation).
 __global__ __launch_bounds__(256) void net_corr_kernel(const int64_t* __restrict__ net_off,
                                                        const uint32_t* __restrict__ ev,
-                                                       const int64_t* __restrict__ chunk_off, int64_t n_net,
+                                                       const int64_t* __restrict__ chunk_off,
+                                                       const int32_t* __restrict__ chunk_net, int64_t n_net,
                                                        uint32_t T, float invT, int64_t n_cyc,
                                                        const uint8_t* __restrict__ mask,
                                                        const long long* __restrict__ yq,
                                                        unsigned long long* __restrict__ cnt_out,
                                                        unsigned long long* __restrict__ sum_out) {
-  __shared__ int64_t s_net;
   const int64_t chunk = blockIdx.x;
-  if (threadIdx.x == 0) {
-    int64_t lo = 0, hi = n_net;
-    while (hi - lo > 1) {
-      const int64_t mid = (lo + hi) >> 1;
-      if (chunk_off[mid] <= chunk) lo = mid; else hi = mid;
-    }
-    s_net = lo;
-  }
-  __syncthreads();
-  const int64_t n = s_net;
+  const int64_t n = chunk_net[chunk];
   const int64_t b = net_off[n], cnt = net_off[n + 1] - b;
   const int64_t i = (chunk - chunk_off[n]) * 256 + threadIdx.x;
   bool take = false, take_rise = false;
@@ -859,10 +841,12 @@ namespace {
 struct FeatureSel {
   int32_t* d_net = nullptr;
   int64_t* d_chunk = nullptr;
+  int32_t* d_chunk_feat = nullptr;
   int64_t n_chunk = 0;
   ~FeatureSel() {
     if (d_net) (void)hipFree(d_net);
     if (d_chunk) (void)hipFree(d_chunk);
+    if (d_chunk_feat) (void)hipFree(d_chunk_feat);
   }
   hipError_t build(const TraceDev& t, int K, const int32_t* nets, hipStream_t s) {
     std::vector<int64_t> co(K + 1, 0);
@@ -872,17 +856,23 @@ struct FeatureSel {
       co[k + 1] = co[k] + (cnt + 255) / 256;
     }
     n_chunk = co[K];
+    std::vector<int32_t> cf(std::max<int64_t>(n_chunk, 1));
+    for (int k = 0; k < K; ++k)
+      for (int64_t c = co[k]; c < co[k + 1]; ++c) cf[c] = k;
     hipError_t e = hipMalloc((void**)&d_net, (size_t)K * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&d_chunk, (size_t)(K + 1) * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&d_chunk_feat, cf.size() * 4);
     if (e == hipSuccess) e = hipMemcpyAsync(d_net, nets, (size_t)K * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(d_chunk, co.data(), (size_t)(K + 1) * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_chunk_feat, cf.data(), cf.size() * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     return e;
   }
   void launch(const TraceDev& t, int K, int64_t c0, int64_t n_cyc, int words, uint32_t* X, hipStream_t s) const {
+    (void)K;
     fill_u32_kernel<<<nblk(n_cyc * words), 256, 0, s>>>(X, n_cyc * words, 0u);
     if (n_chunk > 0)
-      toggle_bits_kernel<<<(unsigned)n_chunk, 256, 0, s>>>(t.net_off, t.ev, d_net, d_chunk, K, t.n_net,
+      toggle_bits_kernel<<<(unsigned)n_chunk, 256, 0, s>>>(t.net_off, t.ev, d_net, d_chunk, d_chunk_feat, t.n_net,
                                                            (uint32_t)t.T, (float)(1.0 / (double)t.T), c0, n_cyc,
                                                            words, X);
   }
@@ -939,7 +929,8 @@ GTP_API gtp_status gtp_feature_correlation(gtp_ctx* ctx, const double* y, const 
     fill_u32_kernel<<<nblk(4 * N), 256, 0, t.stream>>>((uint32_t*)d_cnt, 4 * N, 0u);
     fill_u32_kernel<<<nblk(4 * N), 256, 0, t.stream>>>((uint32_t*)d_sum, 4 * N, 0u);
     if (t.n_chunk > 0)
-      net_corr_kernel<<<(unsigned)t.n_chunk, 256, 0, t.stream>>>(t.net_off, t.ev, t.chunk_off, N, (uint32_t)t.T,
+      net_corr_kernel<<<(unsigned)t.n_chunk, 256, 0, t.stream>>>(t.net_off, t.ev, t.chunk_off, t.chunk_net, N,
+                                                                 (uint32_t)t.T,
                                                                  (float)(1.0 / (double)t.T), C, d_mask, d_yq,
                                                                  d_cnt, d_sum);
     e = hipGetLastError();
