@@ -103,11 +103,37 @@ __global__ void tile_index_kernel(const int64_t* __restrict__ net_off, const uin
     int32_t lo = 0, hi = cnt;
     if (k == n_tile) {
       lo = cnt;
-    } else {
+    } else if (cnt > 0) {
       const int64_t tk = (int64_t)k * tile_ticks;
       const uint32_t key = tk >= (int64_t)EV_TIME_INF ? EV_INF : (uint32_t)(tk << EV_SHIFT);
+      // lower bound of key: bracket it from an interpolated guess (events are
+      // spread over [first, last] time) with doubling steps, then bisect
+      const uint32_t t0 = ev_time(ev[b]), t1 = ev_time(ev[b + cnt - 1]);
+      int32_t g = 0;
+      if ((uint32_t)tk > t0 && t1 > t0)
+        g = (int32_t)min((double)(cnt - 1), (double)(cnt - 1) * ((double)((uint32_t)tk - t0) / (double)(t1 - t0)));
+      if (ev[b + g] < key) {
+        int32_t step = 1;
+        lo = g + 1;
+        hi = g + 1;
+        while (hi < cnt && ev[b + hi] < key) {
+          lo = hi + 1;
+          hi = min(cnt, hi + step);
+          step <<= 1;
+        }
+      } else {
+        int32_t step = 1;
+        hi = g;
+        lo = g;
+        while (lo > 0 && ev[b + lo - 1] >= key) {
+          hi = lo - 1;
+          lo = max(0, lo - step);
+          step <<= 1;
+        }
+      }
+      // now every index < lo is < key and every index >= hi is >= key
       while (lo < hi) {
-        int32_t mid = (lo + hi) >> 1;
+        const int32_t mid = (lo + hi) >> 1;
         if (ev[b + mid] < key) lo = mid + 1; else hi = mid;
       }
     }

@@ -341,39 +341,98 @@ __device__ __forceinline__ int64_t cycle_of_tick(uint32_t t, uint32_t T, float i
   return (int64_t)q;
 }
 
-// Toggle bits of the selected features: feature k = (net, kind) with
-// sel_net[k] = kind * n_net + net; bit k of row c is set when the net has an
-// event of index >= 1 (a value change; index 0 is the initial value) in cycle
-// c0 + c, of any value (kind 0) or to value 1 (kind 1, a rise).  One block per
-// 256 events of a selected net: sel_chunk_off[k] is the first block of
-// feature k, chunk_feat[b] the feature of block b.
-__global__ __launch_bounds__(256) void toggle_bits_kernel(const int64_t* __restrict__ net_off,
-                                                          const uint32_t* __restrict__ ev,
-                                                          const int32_t* __restrict__ sel_net,
-                                                          const int64_t* __restrict__ sel_chunk_off,
-                                                          const int32_t* __restrict__ chunk_feat,
-                                                          int64_t n_net, uint32_t T, float invT, int64_t c0,
-                                                          int64_t n_cyc, int words, uint32_t* __restrict__ X) {
-  const int64_t chunk = blockIdx.x;
-  const int k = chunk_feat[chunk];
-  const int64_t f = sel_net[k];
-  const bool rise_only = f >= n_net;
-  const int64_t n = rise_only ? f - n_net : f;
-  const int64_t b = net_off[n], cnt = net_off[n + 1] - b;
-  const int64_t i = (chunk - sel_chunk_off[k]) * 256 + threadIdx.x;
-  int64_t c = -1;
-  if (i >= 1 && i < cnt) {
-    const uint32_t w = ev[b + i];
-    if (!rise_only || (w & 3u) == 1u) {
-      c = cycle_of_tick(w >> EV_SHIFT, T, invT) - c0;
-      if (c >= n_cyc) c = -1;
+// First index i in [0, cnt) with e[i] >= key (cnt if none): bracket from an
+// interpolated guess with doubling steps, then bisect.
+__device__ __forceinline__ int64_t lower_bound_ev(const uint32_t* e, int64_t cnt, uint32_t key) {
+  if (cnt <= 0) return 0;
+  const uint32_t t0 = e[0] >> EV_SHIFT, t1 = e[cnt - 1] >> EV_SHIFT, tk = key >> EV_SHIFT;
+  int64_t g = 0;
+  if (tk > t0 && t1 > t0) g = (int64_t)min((double)(cnt - 1), (double)(cnt - 1) * ((double)(tk - t0) / (double)(t1 - t0)));
+  int64_t lo, hi, step = 1;
+  if (e[g] < key) {
+    lo = hi = g + 1;
+    while (hi < cnt && e[hi] < key) {
+      lo = hi + 1;
+      hi = min(cnt, hi + step);
+      step <<= 1;
+    }
+  } else {
+    lo = hi = g;
+    while (lo > 0 && e[lo - 1] >= key) {
+      hi = lo - 1;
+      lo = max((int64_t)0, lo - step);
+      step <<= 1;
     }
   }
-  // the block's events are one net's, in time order: of the lanes that set the
-  // same bit of the same cycle, only the first does the atomic
-  const int64_t cp = __shfl_up(c, 1);
-  const bool first = (threadIdx.x & 63) == 0 || cp != c;
-  if (c >= 0 && first) atomicOr(&X[c * words + (k >> 5)], 1u << (k & 31));
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (e[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+constexpr int FB_CYCLES = 2048;  // cycles per feature-bit block
+
+// Bits of the selected features: feature k = (net, kind) with
+// feat[k] = kind * n_net + net; bit k of row c is set when the net has an
+// event of index >= 1 (a value change; index 0 is the initial value) in cycle
+// c0 + c, of any value (kind 0) or to value 1 (kind 1, a rise).  Block (w, cb)
+// builds word w of rows [cb * FB_CYCLES, +FB_CYCLES) in LDS: its 32 features'
+// events in those cycles are walked together (coalesced), the bits set by LDS
+// atomics, and the words written out; no global atomics.
+__global__ __launch_bounds__(256) void feature_bits_kernel(const int64_t* __restrict__ net_off,
+                                                           const uint32_t* __restrict__ ev,
+                                                           const int32_t* __restrict__ feat, int64_t n_net,
+                                                           int64_t T, uint32_t cyc_m, int cyc_sh, int64_t c0,
+                                                           int64_t n_cyc, int words, uint32_t* __restrict__ X) {
+  __shared__ uint32_t bits[FB_CYCLES];
+  __shared__ int64_t s_beg[32];
+  __shared__ int32_t s_off[33];
+  __shared__ uint32_t s_rise;
+  const int w = blockIdx.x;
+  const int64_t cs = (int64_t)blockIdx.y * FB_CYCLES;
+  const int nr = (int)min((int64_t)FB_CYCLES, n_cyc - cs);
+  for (int i = threadIdx.x; i < nr; i += blockDim.x) bits[i] = 0;
+  if (threadIdx.x == 0) s_rise = 0;
+  __syncthreads();
+  if (threadIdx.x < 32) {
+    const int j = threadIdx.x;
+    const int64_t f = feat[(int64_t)w * 32 + j];
+    const bool rise = f >= n_net;
+    const int64_t n = rise ? f - n_net : f;
+    const int64_t b = net_off[n], cnt = net_off[n + 1] - b;
+    const int64_t tlo = (c0 + cs) * T, thi = (c0 + cs + nr) * T;
+    const uint32_t klo = tlo >= (int64_t)EV_TIME_INF ? EV_INF : (uint32_t)(tlo << EV_SHIFT);
+    const uint32_t khi = thi >= (int64_t)EV_TIME_INF ? EV_INF : (uint32_t)(thi << EV_SHIFT);
+    const int64_t i0 = max((int64_t)1, lower_bound_ev(ev + b, cnt, klo));
+    const int64_t i1 = lower_bound_ev(ev + b, cnt, khi);
+    s_beg[j] = b + i0;
+    s_off[j + 1] = (int32_t)max((int64_t)0, i1 - i0);
+    if (rise) atomicOr(&s_rise, 1u << j);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s_off[0] = 0;
+    for (int j = 0; j < 32; ++j) s_off[j + 1] += s_off[j];
+  }
+  __syncthreads();
+  const int total = s_off[32];
+  const uint32_t rise_mask = s_rise;
+  const int64_t cbase = c0 + cs;
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    int lo = 0, hi = 32;  // last j with s_off[j] <= e
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (s_off[mid] <= e) lo = mid; else hi = mid;
+    }
+    const uint32_t wd = ev[s_beg[lo] + (e - s_off[lo])];
+    if (((rise_mask >> lo) & 1u) && (wd & 3u) != 1u) continue;
+    const uint32_t t = wd >> EV_SHIFT;
+    const int64_t c = (int64_t)(((uint64_t)cyc_m * (uint64_t)t) >> cyc_sh) - cbase;
+    atomicOr(&bits[c], 1u << lo);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nr; i += blockDim.x) X[(cs + i) * words + w] = bits[i];
 }
 
 // Per net over the cycles with mask[c] != 0 (all when mask is null): the
@@ -840,41 +899,23 @@ namespace {
 // of each feature's events.
 struct FeatureSel {
   int32_t* d_net = nullptr;
-  int64_t* d_chunk = nullptr;
-  int32_t* d_chunk_feat = nullptr;
-  int64_t n_chunk = 0;
   ~FeatureSel() {
     if (d_net) (void)hipFree(d_net);
-    if (d_chunk) (void)hipFree(d_chunk);
-    if (d_chunk_feat) (void)hipFree(d_chunk_feat);
   }
   hipError_t build(const TraceDev& t, int K, const int32_t* nets, hipStream_t s) {
-    std::vector<int64_t> co(K + 1, 0);
-    for (int k = 0; k < K; ++k) {
-      const int64_t n = nets[k] % t.n_net;
-      const int64_t cnt = t.host_net_off[n + 1] - t.host_net_off[n];
-      co[k + 1] = co[k] + (cnt + 255) / 256;
-    }
-    n_chunk = co[K];
-    std::vector<int32_t> cf(std::max<int64_t>(n_chunk, 1));
-    for (int k = 0; k < K; ++k)
-      for (int64_t c = co[k]; c < co[k + 1]; ++c) cf[c] = k;
+    (void)t;
     hipError_t e = hipMalloc((void**)&d_net, (size_t)K * 4);
-    if (e == hipSuccess) e = hipMalloc((void**)&d_chunk, (size_t)(K + 1) * 8);
-    if (e == hipSuccess) e = hipMalloc((void**)&d_chunk_feat, cf.size() * 4);
     if (e == hipSuccess) e = hipMemcpyAsync(d_net, nets, (size_t)K * 4, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_chunk, co.data(), (size_t)(K + 1) * 8, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_chunk_feat, cf.data(), cf.size() * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     return e;
   }
   void launch(const TraceDev& t, int K, int64_t c0, int64_t n_cyc, int words, uint32_t* X, hipStream_t s) const {
     (void)K;
-    fill_u32_kernel<<<nblk(n_cyc * words), 256, 0, s>>>(X, n_cyc * words, 0u);
-    if (n_chunk > 0)
-      toggle_bits_kernel<<<(unsigned)n_chunk, 256, 0, s>>>(t.net_off, t.ev, d_net, d_chunk, d_chunk_feat, t.n_net,
-                                                           (uint32_t)t.T, (float)(1.0 / (double)t.T), c0, n_cyc,
-                                                           words, X);
+    uint32_t m = 0;
+    int32_t sh = 0;
+    cycle_magic(t.T, &m, &sh);
+    const dim3 grid((unsigned)words, (unsigned)((n_cyc + FB_CYCLES - 1) / FB_CYCLES));
+    feature_bits_kernel<<<grid, 256, 0, s>>>(t.net_off, t.ev, d_net, t.n_net, t.T, m, sh, c0, n_cyc, words, X);
   }
 };
 
