@@ -657,7 +657,7 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
     HIPCHK(x, x->d_chunk_off.upload(co.data(), co.size(), s));
     HIPCHK(x, x->d_chunk_net.alloc(std::max<int64_t>(x->n_chunk, 1)));
     HIPCHK(x, launch_chunk_net(x->d_chunk_off.p, N, x->d_chunk_net.p, s));
-    HIPCHK(x, x->d_aux.alloc(std::max<int64_t>(E, 1)));
+    HIPCHK(x, x->d_aux.alloc(E + EV_PAD));
     HIPCHK(x, hipStreamSynchronize(s));
   }
   {
@@ -1305,7 +1305,7 @@ GTP_API gtp_status gtp_run_batch(gtp_ctx* x, int32_t n_trace, const gtp_trace* t
     DArr<uint32_t> d_ev, d_aux;
     HIPCHK(x, d_voff.upload(voff.data(), voff.size(), s));
     HIPCHK(x, d_ev.alloc(E_load + EV_PAD));
-    HIPCHK(x, d_aux.alloc(std::max<int64_t>(E_load, 1)));
+    HIPCHK(x, d_aux.alloc(E_load + EV_PAD));
     {
       const int64_t CH = 1 << 26;
       DArr<int64_t> st;

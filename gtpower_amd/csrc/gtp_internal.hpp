@@ -24,7 +24,7 @@ namespace gtp {
 
 constexpr uint32_t EV_SHIFT = 3;                // time bits start here
 constexpr uint32_t EV_GLITCH = 4u;              // the event carries a glitch pulse width
-constexpr uint32_t EV_INF = 0xFFFFFFFFu;       // time field 2^29 - 1
+constexpr uint32_t EV_INF = 0xFFFFFFF8u;       // time field 2^29 - 1, glitch bit and value clear
 constexpr uint32_t EV_TIME_INF = 0x1FFFFFFFu;
 constexpr int64_t EV_PAD = 4;                   // words of padding after the event array
 constexpr int MAXP_SMALL = 4;                   // kernel variant for cells with <= 4 pins
@@ -98,7 +98,7 @@ struct PowerArgs {
   // event store
   const int64_t* net_off;        // [N+1]
   const uint32_t* ev;            // [E + EV_PAD]
-  const uint32_t* aux;           // [E] glitch pulse width of each event (0 = not a glitch); read
+  const uint32_t* aux;           // [E + EV_PAD] glitch pulse width of each event (0 = not a glitch); read
                                  // only for events whose word has EV_GLITCH set
   const int32_t* tile_idx;       // [N][n_tile + 1] first event index >= tile start (net-relative);
                                  // [N][n_work_tiles] by list position when tile_list is set

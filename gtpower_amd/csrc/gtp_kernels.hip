@@ -828,10 +828,11 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
       asm volatile("" : "+v"(D[k]));
     }
 
-    // per pin: the next four event words in flight (w0 is the next event),
-    // and the glitch widths of the next two, read one step ahead when the
-    // annotation flagged the event (0 otherwise)
-    const uint32_t* cur[NP];   // next unconsumed event
+    // per pin: a buffer of the next event words (w0 is the next event),
+    // refilled four at a time by one aligned 16-byte load when its quad is
+    // used up (EV_INF past the net's end), and the glitch widths of the next
+    // two events, read one step ahead when the annotation flagged the event
+    const uint32_t* cur[NP];   // the event of w0
     const uint32_t* endp[NP];
     uint32_t w0[NP], w1[NP], w2[NP], w3[NP];
     uint32_t x0[NP], x1[NP];
@@ -883,13 +884,27 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
           if (i0 - 1 >= 1) rfl[p] = rise_x2(ev_val(evp[i0 - 2]), cv[p]) > 0 ? 0 : 1;
         }
         cur[p] = evp + i;
-        const int64_t left = cn - i;
-        if (left > 0) w0[p] = cur[p][0];
-        if (left > 1) w1[p] = cur[p][1];
-        if (left > 2) w2[p] = cur[p][2];
-        if (left > 3) w3[p] = cur[p][3];
-        if (left > 0 && (w0[p] & EV_GLITCH)) x0[p] = cur[p][aux_delta];
-        if (left > 1 && (w1[p] & EV_GLITCH)) x1[p] = cur[p][aux_delta + 1];
+        // the aligned quad holding event i (the array is padded), shifted so
+        // that w0 is event i
+        const int64_t qa = (nb + i) & ~(int64_t)3;
+        const uint32_t* qp = a.ev + qa;
+        const uint4 v = *reinterpret_cast<const uint4*>(qp);
+        const int64_t e = nb + cn;  // end (global index)
+        w0[p] = qa < e ? v.x : EV_INF;
+        w1[p] = qa + 1 < e ? v.y : EV_INF;
+        w2[p] = qa + 2 < e ? v.z : EV_INF;
+        w3[p] = qa + 3 < e ? v.w : EV_INF;
+        const int sh = (int)(nb + i - qa);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          if (k < sh) {
+            w0[p] = w1[p];
+            w1[p] = w2[p];
+            w2[p] = w3[p];
+            w3[p] = EV_INF;
+          }
+        if (w0[p] & EV_GLITCH) x0[p] = cur[p][aux_delta];
+        if (w1[p] & EV_GLITCH) x1[p] = cur[p][aux_delta + 1];
       }
       if (cv[p] == 1) sb |= 1u << p;
     }
@@ -1020,14 +1035,39 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
       if (in_tile && qi != 0)
         atomicAdd((unsigned long long*)&cb[(gl ? BIN_GINT : BIN_INT) * TC], (unsigned long long)qi);
 
-      // consume the event of pin ps: the word four ahead, and the glitch width
-      // of the event that becomes second
-      const uint32_t wn = cps + 4 < eps ? cps[4] : EV_INF;
-      uint32_t w2s = w2[0];
+      // consume the event of pin ps: shift its buffer, or refill it from the
+      // next aligned quad when the current one is used up; then the glitch
+      // width of the event that becomes second
+      uint32_t b1 = w1[0], b2 = w2[0], b3 = w3[0];
 #pragma unroll
       for (int p = 1; p < NP; ++p)
-        if (p == ps) w2s = w2[p];
-      const uint32_t xn = (w2s & EV_GLITCH) ? cps[aux_delta + 2] : 0u;
+        if (p == ps) {
+          b1 = w1[p];
+          b2 = w2[p];
+          b3 = w3[p];
+        }
+      const uint32_t* cn1 = cps + 1;
+      uint32_t n0 = b1, n1 = b2, n2 = b3, n3 = EV_INF;
+      uint32_t x1s = x1[0];
+#pragma unroll
+      for (int p = 1; p < NP; ++p)
+        if (p == ps) x1s = x1[p];
+      uint32_t xn0 = x1s;  // the width of the event that becomes next
+      // the event after the consumed one opens a new quad (the array is
+      // 16-byte aligned): its quad is loaded whole, and its first width
+      if ((((uintptr_t)cn1) & 15u) == 0) {
+        n0 = n1 = n2 = EV_INF;
+        if (cn1 < eps) {
+          const uint4 v = *reinterpret_cast<const uint4*>(cn1);
+          n0 = v.x;
+          n1 = cn1 + 1 < eps ? v.y : EV_INF;
+          n2 = cn1 + 2 < eps ? v.z : EV_INF;
+          n3 = cn1 + 3 < eps ? v.w : EV_INF;
+        }
+        xn0 = (n0 & EV_GLITCH) ? cn1[aux_delta] : 0u;
+      }
+      // EV_INF has the glitch bit clear: no width is read past the events
+      const uint32_t xn = (n1 & EV_GLITCH) ? cn1[aux_delta + 1] : 0u;
 #pragma unroll
       for (int p = 0; p < NP; ++p)
         if (p == ps) {
@@ -1035,12 +1075,12 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
           first[p] = false;
           lt[p] = t;
           cv[p] = nv;
-          cur[p] = cps + 1;
-          w0[p] = w1[p];
-          w1[p] = w2[p];
-          w2[p] = w3[p];
-          w3[p] = wn;
-          x0[p] = x1[p];
+          cur[p] = cn1;
+          w0[p] = n0;
+          w1[p] = n1;
+          w2[p] = n2;
+          w3[p] = n3;
+          x0[p] = xn0;
           x1[p] = xn;
         }
       prev_t = t;
