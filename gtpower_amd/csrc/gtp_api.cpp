@@ -831,7 +831,7 @@ gtp_status lds_sizes(gtp_ctx* x, const PowerArgs& a, RunPlan* rp) {
   const int64_t TC = a.tile_cycles;
   rp->lds = (size_t)N_BIN * TC * 8 + (size_t)a.n_leak * 8 + (size_t)(a.n_leak + (a.n_leak & 1)) * 4 +
             (size_t)a.n_cell * sizeof(CellDev);
-  rp->lds_cell = (size_t)N_BIN * TC * 8 * BIN_COPIES;
+  rp->lds_cell = (size_t)N_BIN * (TC + 1) * 8 * BIN_COPIES;  // one spare column per bin row
   if (rp->lds > 160 * 1024 || rp->lds_cell > 160 * 1024)
     return x->fail(GTP_ERR_LIMIT, "gtp_run: tile needs more LDS than a workgroup has");
   return GTP_OK;

@@ -717,8 +717,10 @@ __global__ __launch_bounds__(256) void glitch_annotate_kernel(const int64_t* __r
 }
 
 // The same for the events of listed tiles only: the 8 threads of group
-// (n, l) annotate, interleaved, the events of net n from idx_begin[n][l] up to
-// (not including) the first event at or after the end of tile tile_list[l].
+// (n, l) annotate, interleaved, the events of net n from the one before
+// idx_begin[n][l] up to the first event at or after the end of tile
+// tile_list[l], both included: the power kernels read the glitch flags of
+// the events next to a tile's own.
 __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_t* __restrict__ net_off,
                                                                     uint32_t* __restrict__ ev,
                                                                     const int32_t* __restrict__ idx_begin,
@@ -735,7 +737,8 @@ __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_
   const int64_t cnt = net_off[n + 1] - b;
   const int64_t tend = ((int64_t)tile_list[i - n * n_list] + 1) * tile_ticks;
   uint32_t* e = ev + b;
-  for (int64_t k = idx_begin[i] + j; k < cnt && (int64_t)ev_time(e[k]) < tend; k += 8) {
+  for (int64_t k = max<int64_t>(idx_begin[i] - 1, 0) + j; k < cnt; k += 8) {
+    if ((int64_t)ev_time(e[k]) >= tend && k > 0 && (int64_t)ev_time(e[k - 1]) >= tend) break;
     const uint32_t pw = annotate_one(e, k, cnt, T, invT);
     if (pw) aux[b + k] = pw;
     const uint32_t w = e[k];
@@ -768,7 +771,11 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
   __shared__ long long s_leakq[NSTATE];
   __shared__ long long s_base;
   const int TC = a.tile_cycles;
-  const int BW = N_BIN * TC;  // one copy of the bins
+  // one copy of the bins: N_BIN rows of TC + 1 columns; the last column takes
+  // the steps of events in cycles past the tile (the last tile's end), so the
+  // bin updates need no test
+  const int TCP = TC + 1;
+  const int BW = N_BIN * TCP;
   long long* bins = reinterpret_cast<long long*>(smem_raw);
 
   __shared__ long long s_item;
@@ -794,14 +801,12 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
   const uint32_t T = (uint32_t)a.T;
   const float invT = a.inv_T;
   const int64_t tc0 = (int64_t)tile * TC;
-  const int tcn = (int)min((int64_t)TC, wk.n_cycle - tc0);
   const bool last_tile = tile == wk.n_tile - 1;
   const uint32_t tstart = (uint32_t)(tc0 * (int64_t)T);
   const uint32_t tend = last_tile ? EV_TIME_INF : (uint32_t)((tc0 + TC) * (int64_t)T);
   const uint32_t lend = last_tile ? (uint32_t)wk.t_max : tend;
   const uint32_t t_max = (uint32_t)wk.t_max;
   const float inv_qe = a.inv_q_energy, inv_ql = a.inv_q_leak;
-  const int64_t aux_delta = a.aux - a.ev;
 
   long long base_acc = 0;
   const int64_t gi_begin = (int64_t)range * a.range_gates;
@@ -812,102 +817,118 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
     const int64_t pb = a.gate_pin_off[g];
     const int64_t eb = a.gate_energy_off[g];
     const int64_t ab = a.gate_arc_off[g];
-    // per-gate energies and arc delays, kept in registers: the empty asm
-    // statements stop the compiler from turning the selections below into a
-    // dynamically indexed (scratch) array
+    // per-gate energies and arc delays, kept in registers (the empty asm
+    // statements before the step loop stop the compiler from turning the
+    // selections below into a dynamically indexed scratch array)
     float E[NS];
 #pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      E[k] = a.energy[eb + k];
-      asm volatile("" : "+v"(E[k]));
-    }
+    for (int k = 0; k < NS; ++k) E[k] = a.energy[eb + k];
     int32_t D[4 * NIN];
 #pragma unroll
-    for (int k = 0; k < 4 * NIN; ++k) {
-      D[k] = a.arc_q[ab + k];
-      asm volatile("" : "+v"(D[k]));
-    }
+    for (int k = 0; k < 4 * NIN; ++k) D[k] = a.arc_q[ab + k];
 
-    // per pin: a buffer of the next event words (w0 is the next event),
-    // refilled four at a time by one aligned 16-byte load when its quad is
-    // used up (EV_INF past the net's end), and the glitch widths of the next
-    // two events, read one step ahead when the annotation flagged the event
-    const uint32_t* cur[NP];   // the event of w0
-    const uint32_t* endp[NP];
-    uint32_t w0[NP], w1[NP], w2[NP], w3[NP];
-    uint32_t x0[NP], x1[NP];
-    uint32_t lt[NP];           // time of the last consumed event (for the related-pin search)
-    int cv[NP], rfl[NP];       // current value; edge of the last consumed event (0 rise, 1 fall)
-    bool first[NP];            // no event consumed yet and event 0 is next
+    // Per pin: the current and next event words cw/nw (EV_INF past the net's
+    // events), the last consumed event lw (its time, value and glitch flag;
+    // before the first event of the tile the event before it, or a word
+    // holding the initial state), the number of events from the current one
+    // on, and two buffers fed by aligned 16-byte loads: words 2 and 3 of the
+    // quad holding the current event (q2, q3) and the whole quad after it
+    // (n0..n3).  The quad after is loaded at the end of the step that moves a
+    // pin into a new quad and first read in the next step, so its load
+    // overlaps a whole step.  Glitch widths come from the flags of the
+    // neighbouring events (lw and nw), so no width array is read.
+    // The set-up loads go out level by level for all pins at once (pin ->
+    // net -> net range -> tile start -> event words): every address is kept
+    // inside the arrays and unused results are discarded.
+    const uint32_t* cp[NP];    // the current event
+    int32_t rem[NP];           // events from the current one to the net's end
+    uint32_t q2[NP], q3[NP], n0[NP], n1[NP], n2[NP], n3[NP];
+    uint32_t cw[NP], nw[NP], lw[NP];
     float gk[NP];
-    float csw = 0.f;
+    uint32_t rflm = (1u << NP) - 1;  // bit p: pin p's last consumed event was not a rise (TimeBasedPower.cc:420-431)
+    uint32_t firstm = 0;             // bit p: pin p's next event is the net's event 0, after time 0
+    uint32_t hasm = 0;               // bit p: pin p's net has events
     uint32_t sb = 0;
+    const float csw = a.pin_csw[pb + OUT];
+    int32_t net[NP];
+    uint32_t pdef[NP];
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
-      cur[p] = a.ev;
-      endp[p] = a.ev;
-      w0[p] = w1[p] = w2[p] = w3[p] = EV_INF;
-      x0[p] = x1[p] = 0;
-      lt[p] = 0;
-      cv[p] = 2;
-      rfl[p] = 1;
-      first[p] = false;
-      const int32_t net = a.pin_net[pb + p];
+      net[p] = a.pin_net[pb + p];
       gk[p] = a.pin_gk[pb + p] * a.time_scale;
-      if (p == OUT) csw = a.pin_csw[pb + p];
-      int64_t nb = 0, cn = 0;
-      if (net >= 0) {
-        nb = a.net_off[net + wk.net_base];
-        cn = a.net_off[net + wk.net_base + 1] - nb;
-      }
-      if (cn == 0) {
-        cv[p] = a.pin_default[pb + p];
-      } else {
-        const uint32_t* evp = a.ev + nb;
-        endp[p] = evp + cn;
-        const int32_t i0 = wk.tidx[(int64_t)net * wk.tile_stride + wk.tile_col];
-        int32_t i = i0;
-        if (i0 == 0) {
-          const uint32_t e0 = evp[0];
-          if (tile == 0 && ev_time(e0) == 0) {
-            cv[p] = ev_val(e0);  // time-0 event is the initial state (TimeBasedPower.cc:782-787)
+      pdef[p] = (uint32_t)a.pin_default[pb + p];
+    }
+    int64_t nb[NP], cn[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int64_t ni = (net[p] >= 0 ? net[p] : 0) + wk.net_base;
+      nb[p] = a.net_off[ni];
+      const int64_t ne = a.net_off[ni + 1];
+      cn[p] = net[p] >= 0 ? ne - nb[p] : 0;
+    }
+    int32_t i0[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int32_t v = wk.tidx[(int64_t)(net[p] >= 0 ? net[p] : 0) * wk.tile_stride + wk.tile_col];
+      i0[p] = cn[p] > 0 ? v : 0;
+    }
+    // the event before the tile (event 0 when there is none) and the one before that
+    uint32_t wp1[NP], wp2[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      wp1[p] = a.ev[nb[p] + max(i0[p] - 1, 0)];
+      wp2[p] = a.ev[nb[p] + max(i0[p] - 2, 0)];
+    }
+    int64_t g0[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      uint32_t l = pdef[p];  // time 0, no flag, the default state
+      int32_t i = i0[p];
+      if (cn[p] > 0) {
+        hasm |= 1u << p;
+        if (i0[p] == 0) {
+          if (tile == 0 && ev_time(wp1[p]) == 0) {
+            l = wp1[p];  // the time-0 event is the initial state (TimeBasedPower.cc:782-787)
             i = 1;
           } else {
-            cv[p] = 2;
-            lt[p] = ev_time(e0);
-            first[p] = true;
+            l = (ev_time(wp1[p]) << EV_SHIFT) | 2u;  // unknown state before the net's first event
+            firstm |= 1u << p;
           }
         } else {
-          const uint32_t el = evp[i0 - 1];
-          cv[p] = ev_val(el);
-          lt[p] = ev_time(el);
-          if (i0 - 1 >= 1) rfl[p] = rise_x2(ev_val(evp[i0 - 2]), cv[p]) > 0 ? 0 : 1;
+          l = wp1[p];
+          if (i0[p] - 1 >= 1 && rise_x2(ev_val(wp2[p]), ev_val(l)) > 0) rflm &= ~(1u << p);
         }
-        cur[p] = evp + i;
-        // the aligned quad holding event i (the array is padded), shifted so
-        // that w0 is event i
-        const int64_t qa = (nb + i) & ~(int64_t)3;
-        const uint32_t* qp = a.ev + qa;
-        const uint4 v = *reinterpret_cast<const uint4*>(qp);
-        const int64_t e = nb + cn;  // end (global index)
-        w0[p] = qa < e ? v.x : EV_INF;
-        w1[p] = qa + 1 < e ? v.y : EV_INF;
-        w2[p] = qa + 2 < e ? v.z : EV_INF;
-        w3[p] = qa + 3 < e ? v.w : EV_INF;
-        const int sh = (int)(nb + i - qa);
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-          if (k < sh) {
-            w0[p] = w1[p];
-            w1[p] = w2[p];
-            w2[p] = w3[p];
-            w3[p] = EV_INF;
-          }
-        if (w0[p] & EV_GLITCH) x0[p] = cur[p][aux_delta];
-        if (w1[p] & EV_GLITCH) x1[p] = cur[p][aux_delta + 1];
       }
-      if (cv[p] == 1) sb |= 1u << p;
+      lw[p] = l;
+      if (ev_val(l) == 1) sb |= 1u << p;
+      rem[p] = (int32_t)(cn[p] - i);
+      g0[p] = nb[p] + i;
+      cp[p] = a.ev + g0[p];
     }
+    // the quad holding the current event and, when the net has events past
+    // it, the quad after (the store is padded by a quad)
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int64_t qa = g0[p] & ~(int64_t)3;
+      const int kk = (int)(g0[p] - qa);
+      const bool more = rem[p] > 4 - kk;
+      const uint4 v = *reinterpret_cast<const uint4*>(a.ev + qa);
+      const uint4 u = *reinterpret_cast<const uint4*>(a.ev + (more ? qa + 4 : qa));
+      q2[p] = v.z;
+      q3[p] = v.w;
+      n0[p] = more ? u.x : EV_INF;
+      n1[p] = more ? u.y : EV_INF;
+      n2[p] = more ? u.z : EV_INF;
+      n3[p] = more ? u.w : EV_INF;
+      const uint32_t c0 = kk == 0 ? v.x : kk == 1 ? v.y : kk == 2 ? v.z : v.w;
+      const uint32_t nx = kk == 0 ? v.y : kk == 1 ? v.z : kk == 2 ? v.w : u.x;
+      cw[p] = rem[p] >= 1 ? c0 : EV_INF;
+      nw[p] = rem[p] >= 2 ? nx : EV_INF;
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) asm volatile("" : "+v"(E[k]));
+#pragma unroll
+    for (int k = 0; k < 4 * NIN; ++k) asm volatile("" : "+v"(D[k]));
     base_acc += s_leakq[sb];
 
     long long acc[N_PG];     // per-gate sums in quanta (leakage: quanta * ticks)
@@ -926,40 +947,34 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
     // the step has no branch per pin.
     for (;;) {
       int ps = OUT;
-      uint32_t ws = w0[OUT];
+      uint32_t ws = cw[OUT];
       uint32_t t = ev_time(ws);
 #pragma unroll
       for (int p = 0; p < NIN; ++p) {
-        const uint32_t tp = ev_time(w0[p]);
+        const uint32_t tp = ev_time(cw[p]);
         if (tp < t) {
           t = tp;
-          ws = w0[p];
+          ws = cw[p];
           ps = p;
         }
       }
       if (t >= tend) break;
-      int pv = cv[0];
+      uint32_t lws = lw[0], nws = nw[0];
       float gks = gk[0];
-      const uint32_t* cps = cur[0];
-      const uint32_t* eps = endp[0];
-      bool fs = first[0];
-      uint32_t pws = x0[0];
 #pragma unroll
       for (int p = 1; p < NP; ++p)
         if (p == ps) {
-          pv = cv[p];
+          lws = lw[p];
+          nws = nw[p];
           gks = gk[p];
-          cps = cur[p];
-          eps = endp[p];
-          fs = first[p];
-          pws = x0[p];
         }
+      const bool fs = (firstm >> ps) & 1u;
+      const int pv = ev_val(lws);
       const bool is_out = ps == OUT;
       const int nv = ev_val(ws);
       const uint32_t c = cycle_mul(t, a.cyc_m, a.cyc_sh);
       const uint32_t cstart = c * T, cend = cstart + T;
-      const int cr = (int)((int64_t)c - tc0);
-      const bool in_tile = cr < tcn;
+      const int cr = min((int)((int64_t)c - tc0), TC);
       long long* cb = mybins + cr;
 
       // leakage: the previous state over [prev_t, t), exact in quanta * ticks;
@@ -968,11 +983,10 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
       const uint32_t sa = (sb & ~(1u << ps)) | ((nv == 1 ? 1u : 0u) << ps);
       const long long lq_new = s_leakq[sa];
       const float lf_new = s_leak[sa];
-      if (sa != sb && in_tile) {
-        atomicAdd((unsigned long long*)&cb[BIN_LD * TC], (unsigned long long)(lq_new - lq_cur));
-        atomicAdd((unsigned long long*)&cb[BIN_LW * TC],
-                  (unsigned long long)qterm((lf_new - lf_cur) * ((float)(min(cend, t_max) - t) * invT), inv_ql));
-      }
+      // (zero when the state does not change)
+      atomicAdd((unsigned long long*)&cb[BIN_LD * TCP], (unsigned long long)(lq_new - lq_cur));
+      atomicAdd((unsigned long long*)&cb[BIN_LW * TCP],
+                (unsigned long long)qterm((lf_new - lf_cur) * ((float)(min(cend, t_max) - t) * invT), inv_ql));
       sb = sa;
       lq_cur = lq_new;
       lf_cur = lf_new;
@@ -981,13 +995,13 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
       const int r2 = rise_x2(pv, nv);
       const int rf = r2 > 0 ? 0 : 1;
       tog += (uint32_t)tx2;
-      // glitch scaling; the width is read only for events the annotation flagged
-      float ratio = -1.f;
-      if (!((clk >> ps) & 1u) && pws != 0u) {
-        const float r = (float)pws * gks;
-        ratio = fminf(1.f, r * r);
-      }
-      const bool gl = ratio > 0.f;
+      // glitch scaling (TimeBasedPower.cc:303-339): the width to the flagged
+      // neighbours of a flagged event (event 0 is never flagged)
+      const uint32_t pwp = (lws & EV_GLITCH) ? t - ev_time(lws) : 0u;
+      const uint32_t pwn = (nws & EV_GLITCH) ? ev_time(nws) - t : 0u;
+      const float r = (float)max(pwp, pwn) * gks;
+      const float ratio = fminf(1.f, r * r);
+      const bool gl = (ws & EV_GLITCH) && !((clk >> ps) & 1u) && ratio > 0.f;
       glc += gl ? 1u : 0u;
 
       // input pin: its table at its own edge
@@ -1003,16 +1017,17 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
       float En_out = rf ? E[6 * NIN + 1] : E[6 * NIN];  // default when none qualifies
 #pragma unroll
       for (int q = 0; q < NIN; ++q) {
-        const bool hi_rf = rfl[q] != 0, lo_rf = rf != 0;
+        const bool hi_rf = (rflm >> q) & 1u, lo_rf = rf != 0;
         const int32_t d01 = lo_rf ? D[q * 4 + 1] : D[q * 4 + 0];
         const int32_t d23 = lo_rf ? D[q * 4 + 3] : D[q * 4 + 2];
         const int32_t d = hi_rf ? d23 : d01;
         const float e01 = lo_rf ? E[2 * NIN + q * 4 + 1] : E[2 * NIN + q * 4 + 0];
         const float e23 = lo_rf ? E[2 * NIN + q * 4 + 3] : E[2 * NIN + q * 4 + 2];
         const float eq = hi_rf ? e23 : e01;
-        const bool ok = endp[q] != a.ev && d != ARC_NONE && lt[q] >= cstart && lt[q] < cend;
+        const uint32_t ltq = ev_time(lw[q]);
+        const bool ok = ((hasm >> q) & 1u) && d != ARC_NONE && ltq >= cstart && ltq < cend;
         // within one cycle t - lt < T < 2^15, so the grid difference fits 32 bits
-        int32_t diff = (int32_t)((t - lt[q]) << ARC_FRAC_BITS) - d;
+        int32_t diff = (int32_t)((t - ltq) << ARC_FRAC_BITS) - d;
         diff = diff < 0 ? -diff : diff;
         const bool take = ok && diff < best;
         best = take ? diff : best;
@@ -1025,65 +1040,66 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
         const long long qs = qterm(gl ? ratio * es : es, inv_qe);
         acc[PG_GSW] += gl ? qs : 0;
         acc[PG_SW] += gl ? 0 : qs;
-        if (in_tile && qs != 0)
-          atomicAdd((unsigned long long*)&cb[(gl ? BIN_GSW : BIN_SW) * TC], (unsigned long long)qs);
+        atomicAdd((unsigned long long*)&cb[(gl ? BIN_GSW : BIN_SW) * TCP], (unsigned long long)qs);
       }
       const float e = En * (0.5f * (float)tx2);
       const long long qi = qterm(gl ? ratio * e : e, inv_qe);
       acc[PG_GINT] += gl ? qi : 0;
       acc[PG_INT] += gl ? 0 : qi;
-      if (in_tile && qi != 0)
-        atomicAdd((unsigned long long*)&cb[(gl ? BIN_GINT : BIN_INT) * TC], (unsigned long long)qi);
+      atomicAdd((unsigned long long*)&cb[(gl ? BIN_GINT : BIN_INT) * TCP], (unsigned long long)qi);
 
-      // consume the event of pin ps: shift its buffer, or refill it from the
-      // next aligned quad when the current one is used up; then the glitch
-      // width of the event that becomes second
-      uint32_t b1 = w1[0], b2 = w2[0], b3 = w3[0];
+      // consume the event of pin ps: the next word becomes current, and the
+      // word after it comes from q2/q3 or the quad after; entering a new quad
+      // moves that quad's words 2 and 3 into q2/q3
+      const uint32_t* cps = cp[0];
+      int32_t rs = rem[0];
+      uint32_t sq2 = q2[0], sq3 = q3[0], sn0 = n0[0], sn1 = n1[0];
 #pragma unroll
       for (int p = 1; p < NP; ++p)
         if (p == ps) {
-          b1 = w1[p];
-          b2 = w2[p];
-          b3 = w3[p];
+          cps = cp[p];
+          rs = rem[p];
+          sq2 = q2[p];
+          sq3 = q3[p];
+          sn0 = n0[p];
+          sn1 = n1[p];
         }
       const uint32_t* cn1 = cps + 1;
-      uint32_t n0 = b1, n1 = b2, n2 = b3, n3 = EV_INF;
-      uint32_t x1s = x1[0];
-#pragma unroll
-      for (int p = 1; p < NP; ++p)
-        if (p == ps) x1s = x1[p];
-      uint32_t xn0 = x1s;  // the width of the event that becomes next
-      // the event after the consumed one opens a new quad (the array is
-      // 16-byte aligned): its quad is loaded whole, and its first width
-      if ((((uintptr_t)cn1) & 15u) == 0) {
-        n0 = n1 = n2 = EV_INF;
-        if (cn1 < eps) {
-          const uint4 v = *reinterpret_cast<const uint4*>(cn1);
-          n0 = v.x;
-          n1 = cn1 + 1 < eps ? v.y : EV_INF;
-          n2 = cn1 + 2 < eps ? v.z : EV_INF;
-          n3 = cn1 + 3 < eps ? v.w : EV_INF;
-        }
-        xn0 = (n0 & EV_GLITCH) ? cn1[aux_delta] : 0u;
-      }
-      // EV_INF has the glitch bit clear: no width is read past the events
-      const uint32_t xn = (n1 & EV_GLITCH) ? cn1[aux_delta + 1] : 0u;
+      const int32_t rn = rs - 1;
+      const int kn = (int)(((uintptr_t)cn1 >> 2) & 3u);  // position of the new current event in its quad
+      const bool enter = kn == 0;
+      const uint32_t nx = kn == 1 ? sq2 : kn == 2 ? sq3 : kn == 3 ? sn0 : sn1;
+      const uint32_t nwn = rn >= 2 ? nx : EV_INF;
 #pragma unroll
       for (int p = 0; p < NP; ++p)
         if (p == ps) {
-          rfl[p] = fs ? 1 : rf;
-          first[p] = false;
-          lt[p] = t;
-          cv[p] = nv;
-          cur[p] = cn1;
-          w0[p] = n0;
-          w1[p] = n1;
-          w2[p] = n2;
-          w3[p] = n3;
-          x0[p] = xn0;
-          x1[p] = xn;
+          cp[p] = cn1;
+          rem[p] = rn;
+          cw[p] = nws;
+          nw[p] = nwn;
+          lw[p] = ws;
+          if (enter) {
+            q2[p] = n2[p];
+            q3[p] = n3[p];
+          }
         }
+      rflm = (rflm & ~(1u << ps)) | ((fs ? 1u : (uint32_t)rf) << ps);
+      firstm &= ~(1u << ps);
       prev_t = t;
+      // the quad after a newly entered quad, when the net has events there;
+      // every read of n0..n3 above precedes this load
+      asm volatile("" ::: "memory");
+      if (enter && rn > 4) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+          if (p == ps) {
+            const uint4 v = *reinterpret_cast<const uint4*>(cp[p] + 4);
+            n0[p] = v.x;
+            n1[p] = v.y;
+            n2[p] = v.z;
+            n3[p] = v.w;
+          }
+      }
     }
     if (lend > prev_t) acc[PG_LEAK] += (long long)(int)lq_cur * (long long)(lend - prev_t);
     const int64_t slot = (int64_t)wt * a.n_sched + gi;
@@ -1106,14 +1122,17 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
   if (threadIdx.x == 0) {
     long long run = s_base;
     for (int ci = 0; ci < TC; ++ci) {
-      const long long d = bins[BIN_LD * TC + ci];
-      bins[BIN_LD * TC + ci] = run;
+      const long long d = bins[BIN_LD * TCP + ci];
+      bins[BIN_LD * TCP + ci] = run;
       run += d;
     }
   }
   __syncthreads();
   long long* out = a.cyc_part + ((int64_t)wt * a.n_range + range) * N_BIN * TC;
-  for (int i = threadIdx.x; i < BW; i += blockDim.x) out[i] = bins[i];
+  for (int i = threadIdx.x; i < N_BIN * TC; i += blockDim.x) {
+    const int b = i / TC;
+    out[i] = bins[b * TCP + (i - b * TC)];
+  }
   __syncthreads();  // the bins are cleared for the next item
   }
 }

@@ -204,6 +204,27 @@ def test_judge_tiles_equal_full_run(engine_small):
         eng.power_tiles(tc, np.array([n_tile], np.int32))
 
 
+def test_judge_tiles_before_any_full_run():
+    """Tiles evaluated on a fresh trace (only their own events and the ones
+    next to them annotated), adjacent and edge tiles included, then the same
+    engine after a different trace: bit-identical to the full run each time."""
+    d = S.make_design(2000, dff_fraction=0.15, clock_ns=2.0, seed=31)
+    with HipPower() as eng:
+        eng.set_design(d.library, d)
+        for seed in (32, 33):
+            print(f"[mlp] fresh-trace run_tiles seed {seed}", flush=True)
+            tr = d.trace(2500, toggle_p=0.2, seed=seed)
+            eng.read_power_activities(tr)
+            tc = 64
+            n_tile = -(-tr.n_cycle // tc)
+            tiles = np.array(sorted({0, 3, 4, n_tile // 2, n_tile - 1}), np.int32)
+            cyc, waves, _ = eng.power_tiles(tc, tiles)
+            full = eng.power(want=("cycle",))
+            assert np.allclose(waves, full.cycle_power[:, cyc], rtol=1e-9, atol=1e-30)
+            if full.schedule["tile_cycles"] == tc:
+                assert np.array_equal(waves, full.cycle_power[:, cyc])
+
+
 def test_surrogate_gain_and_fallback():
     """What the trained network gains: per-cycle power of a fresh trace at a
     fraction of the exact engine's device time, within 2% NRMSE (RMSE / mean
