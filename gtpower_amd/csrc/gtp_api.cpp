@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <numeric>
@@ -831,7 +832,14 @@ gtp_status lds_sizes(gtp_ctx* x, const PowerArgs& a, RunPlan* rp) {
   const int64_t TC = a.tile_cycles;
   rp->lds = (size_t)N_BIN * TC * 8 + (size_t)a.n_leak * 8 + (size_t)(a.n_leak + (a.n_leak & 1)) * 4 +
             (size_t)a.n_cell * sizeof(CellDev);
-  rp->lds_cell = (size_t)N_BIN * (TC + 1) * 8 * BIN_COPIES;  // one spare column per bin row
+  // the specialised kernel's bins (one spare column per row): up to 4
+  // lane-interleaved copies, as many as fit 32 KB (measured: more copies of
+  // longer tiles cost occupancy and gain nothing), at least one
+  const size_t one = (size_t)N_BIN * (TC + 1) * 8;
+  int copies = (int)std::max<size_t>(1, std::min<size_t>(4, (32 * 1024) / one));
+  if (const char* e = std::getenv("GTP_BIN_COPIES")) copies = std::max(1, std::atoi(e));
+  rp->a.bin_copies = copies;
+  rp->lds_cell = one * copies;
   if (rp->lds > 160 * 1024 || rp->lds_cell > 160 * 1024)
     return x->fail(GTP_ERR_LIMIT, "gtp_run: tile needs more LDS than a workgroup has");
   return GTP_OK;

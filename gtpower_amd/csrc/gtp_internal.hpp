@@ -127,6 +127,7 @@ struct PowerArgs {
   const float* leak;             // [n_leak] leakage per gate state (W)
   const long long* leak_q;       // [n_leak] the same in leakage quanta
   int32_t n_leak;
+  int32_t bin_copies;            // lane-interleaved copies of the per-cycle bins (specialised kernel)
   // fixed-point scales (powers of two): term / quantum
   float inv_q_energy;
   float inv_q_leak;
@@ -190,7 +191,6 @@ hipError_t launch_gate_reduce(const long long* pg, const uint32_t* tog, const ui
                               double norm_energy, double norm_leak, double* gate_power,
                               unsigned long long* gate_tog, unsigned long long* gate_glitch, hipStream_t s);
 constexpr int TOTAL_BLOCKS = 256;
-constexpr int BIN_COPIES = 4;  // lane-interleaved copies of the per-cycle bins (specialised kernel)
 hipError_t launch_chunk_net(const int64_t* chunk_off, int64_t n_net, int32_t* chunk_net, hipStream_t s);
 hipError_t launch_glitch_annotate(const int64_t* net_off, uint32_t* ev, const int64_t* chunk_off,
                                   const int32_t* chunk_net, int64_t n_chunk, uint32_t T, float invT, uint32_t* aux,

@@ -753,7 +753,7 @@ __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_
 // condition.  Every workgroup takes gates of one cell type, so the pin loops
 // are compile-time, the per-gate energies and arc delays live in registers and
 // the cell's leakage table in LDS.  Glitch flags come from the annotation
-// pass.  Per-cycle sums go to BIN_COPIES lane-interleaved copies of the LDS
+// pass.  Per-cycle sums go to a.bin_copies lane-interleaved copies of the LDS
 // bins so that lanes working on the same cycle do not serialise on one address.
 // Semantics are those of power_tile_kernel.
 // ---------------------------------------------------------------------------
@@ -785,7 +785,8 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
     s_leak[i] = a.leak[cd.leak_off + i];
     s_leakq[i] = a.leak_q[cd.leak_off + i];
   }
-  long long* mybins = bins + (threadIdx.x % BIN_COPIES) * BW;
+  const int n_copy = a.bin_copies;
+  long long* mybins = bins + (threadIdx.x % n_copy) * BW;
 
   const int64_t n_work = (int64_t)a.n_work_tiles * range_count;
   int64_t w = 0;
@@ -794,7 +795,7 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
   const int range = range_begin + (int)(w - (int64_t)wt * range_count);
   const WorkTile wk = resolve_work_tile(a, wt);
   const int tile = wk.tile;
-  for (int i = threadIdx.x; i < BW * BIN_COPIES; i += blockDim.x) bins[i] = 0;
+  for (int i = threadIdx.x; i < BW * n_copy; i += blockDim.x) bins[i] = 0;
   if (threadIdx.x == 0) s_base = 0;
   __syncthreads();
 
@@ -1115,7 +1116,7 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
   // fold the bin copies into copy 0, then S_c = base + earlier leakage steps
   for (int i = threadIdx.x; i < BW; i += blockDim.x) {
     long long v = bins[i];
-    for (int k = 1; k < BIN_COPIES; ++k) v += bins[k * BW + i];
+    for (int k = 1; k < n_copy; ++k) v += bins[k * BW + i];
     bins[i] = v;
   }
   __syncthreads();
