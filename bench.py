@@ -109,7 +109,7 @@ def batch_leg(eng, design, w, n_trace, max_cycles, seed):
             "tile_cycles": b[0].schedule.get("tile_cycles"), "gates_per_wg": b[0].schedule.get("gates_per_wg")}
 
 
-def surrogate_leg(eng, design, exact_cycle_power, w, n_fresh, steps, exact_cps):
+def surrogate_leg(eng, design, exact_cycle_power, w, n_fresh, steps, exact_cps, judge_tile=64):
     """Train on the first 50k exact cycles, infer on a fresh trace, judge 1%."""
     import numpy as np
     from gtpower_amd import synthetic as S
@@ -122,7 +122,8 @@ def surrogate_leg(eng, design, exact_cycle_power, w, n_fresh, steps, exact_cps):
     class _Exact:
         cycle_power = exact_cycle_power
 
-    sp = SurrogatePower(MlpSurrogate(4096, 256, seed=3), threshold=0.02, judge_fraction=0.01)
+    sp = SurrogatePower(MlpSurrogate(4096, 256, seed=3), threshold=0.02, judge_fraction=0.01,
+                        judge_tile=judge_tile)
     t0 = time.perf_counter()
     sp.fit(eng, train_mask=mask, exact=_Exact)
     train_s = time.perf_counter() - t0
@@ -157,7 +158,7 @@ def surrogate_leg(eng, design, exact_cycle_power, w, n_fresh, steps, exact_cps):
         "nrmse_judged": out.nrmse, "nrmse_full": full, "r2": r2,
         "constant_predictor_nrmse": float(y.std() / y.mean()),
         "nrmse_range": float(np.sqrt(np.mean((pred - y) ** 2)) / (y.max() - y.min())),
-        "judged_cycles": out.judged_cycles, "train_cycles": n_train, "infer_cycles": int(y.size),
+        "judged_cycles": out.judged_cycles, "judge_tile": sp.judge_tile, "train_cycles": n_train, "infer_cycles": int(y.size),
         "surrogate_ms": surr_ms, "judge_ms": judge_ms, "exact_ms_fresh": exact.kernel_ms,
         "train_s": round(train_s, 2), "features": 4096, "hidden": [256, 256], "dtype": "bf16 (fp32 accumulate)",
         "mfma": {"achieved": flop / (surr_ms * 1e-3) / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
@@ -184,6 +185,7 @@ def main():
     ap.add_argument("--batch-traces", type=int, default=64)
     ap.add_argument("--batch-max-cycles", type=int, default=2000)
     ap.add_argument("--surrogate-cycles", type=int, default=100000)
+    ap.add_argument("--judge-tile", type=int, default=64, help="cycles per tile the exact judge recomputes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
     args = ap.parse_args()
 
@@ -251,7 +253,7 @@ def main():
     if not args.no_surrogate:
         surrogate = surrogate_leg(eng, design, r.cycle_power, w,
                                   min(args.surrogate_cycles, args.cycles or args.surrogate_cycles),
-                                  3, value)
+                                  3, value, judge_tile=args.judge_tile)
     # after the surrogate leg: the batch replaces the resident trace
     batch = None
     if args.batch_traces > 0:

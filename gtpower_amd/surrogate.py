@@ -156,7 +156,7 @@ class SurrogateOutput:
 class SurrogatePower:
     """Surrogate with an exact judge and an exact fallback."""
 
-    def __init__(self, mlp, threshold=0.02, judge_fraction=0.01, judge_tile=32, seed=0x47545057):
+    def __init__(self, mlp, threshold=0.02, judge_fraction=0.01, judge_tile=64, seed=0x47545057):
         # short judge tiles: a tile's gates walk its cycles serially, so many
         # short tiles finish sooner than few long ones
         self.mlp = mlp
