@@ -99,6 +99,18 @@ class GtpTrainConfig(C.Structure):
                 ("input_dropout", C.c_int32)]
 
 
+class GtpVcdInfo(C.Structure):
+    _fields_ = [("n_var", C.c_int64), ("n_id", C.c_int64), ("n_stream", C.c_int64), ("n_event", C.c_int64),
+                ("t_max", C.c_int64), ("time_scale", C.c_double), ("n_token", C.c_int64),
+                ("n_change", C.c_int64), ("body_bytes", C.c_int64), ("device_ms", C.c_double),
+                ("total_ms", C.c_double)]
+
+
+class GtpVcdVarInfo(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("id", C.c_char_p), ("width", C.c_int32), ("id_index", C.c_int32),
+                ("first_stream", C.c_int64)]
+
+
 class GtpError(RuntimeError):
     def __init__(self, status, where, text=""):
         self.status = status
@@ -166,12 +178,22 @@ def load_engine():
     lib.gtp_toggle_features.argtypes = [C.c_void_p, C.c_int32, p_i32, C.c_int64, C.c_int64, p_u32]
     lib.gtp_mlp_gemm_test.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                       C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, p_f32]
+    lib.gtp_vcd_create.argtypes = [C.c_int32, C.POINTER(C.c_void_p)]
+    lib.gtp_vcd_destroy.argtypes = [C.c_void_p]
+    lib.gtp_vcd_destroy.restype = None
+    lib.gtp_vcd_last_error.argtypes = [C.c_void_p]
+    lib.gtp_vcd_last_error.restype = C.c_char_p
+    lib.gtp_vcd_parse.argtypes = [C.c_void_p, C.c_char_p, C.c_int64, C.POINTER(GtpVcdInfo)]
+    lib.gtp_vcd_var.argtypes = [C.c_void_p, C.c_int64, C.POINTER(GtpVcdVarInfo)]
+    lib.gtp_vcd_events.argtypes = [C.c_void_p, p_i64, p_i64, p_u8]
+    lib.gtp_set_trace_vcd.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
     for name in ("gtp_create", "gtp_set_design", "gtp_set_trace", "gtp_run", "gtp_run_batch",
                  "gtp_get_schedule", "gtp_reference_cycles_per_thread", "gtp_write_cell_power",
                  "gtp_write_per_cycle", "gtp_run_tiles", "gtp_autotune", "gtp_mlp_create", "gtp_mlp_get_params",
                  "gtp_mlp_set_params", "gtp_mlp_gradients", "gtp_mlp_train", "gtp_mlp_predict",
                  "gtp_mlp_predict_trace", "gtp_feature_correlation", "gtp_toggle_features",
-                 "gtp_mlp_gemm_test"):
+                 "gtp_mlp_gemm_test", "gtp_vcd_create", "gtp_vcd_parse", "gtp_vcd_var", "gtp_vcd_events",
+                 "gtp_set_trace_vcd"):
         getattr(lib, name).restype = C.c_int32
     _ENGINE = lib
     return lib

@@ -19,7 +19,7 @@ CSRC = PKG / "csrc"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GTPOWER_ARCH", "gfx950")
 
-ENGINE_SOURCES = ["gtp_api.cpp", "gtp_kernels.hip", "gtp_mlp.hip"]
+ENGINE_SOURCES = ["gtp_api.cpp", "gtp_kernels.hip", "gtp_mlp.hip", "gtp_vcd.hip"]
 
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",

@@ -30,38 +30,6 @@ namespace {
 
 thread_local std::string g_create_error;
 
-template <class T>
-struct DArr {
-  T* p = nullptr;
-  size_t n = 0;
-  DArr() = default;
-  DArr(const DArr&) = delete;
-  DArr& operator=(const DArr&) = delete;
-  ~DArr() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  hipError_t alloc(size_t count) {
-    if (count == n && p) return hipSuccess;
-    release();
-    if (count == 0) return hipSuccess;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
-    if (e != hipSuccess) {
-      p = nullptr;
-      return e;
-    }
-    n = count;
-    return hipSuccess;
-  }
-  hipError_t upload(const T* src, size_t count, hipStream_t s) {
-    hipError_t e = alloc(count);
-    if (e != hipSuccess || count == 0) return e;
-    return hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s);
-  }
-};
-
 template <class F>
 void parallel_for(int64_t n, int threads, F f) {
   if (n <= 0) return;
@@ -607,6 +575,10 @@ gtp_status ensure_arc_q(gtp_ctx* x, double time_scale);
 
 }  // namespace
 
+namespace {
+gtp_status finish_trace(gtp_ctx* x);
+}  // namespace
+
 GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
   if (!x) return GTP_ERR_ARG;
   if (!x->have_design) return x->fail(GTP_ERR_STATE, "gtp_set_trace: call gtp_set_design first");
@@ -648,6 +620,17 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
     }
     HIPCHK(x, hipStreamSynchronize(s));
   }
+  return finish_trace(x);
+}
+
+namespace {
+// The per-trace structures after the event store (x->d_ev, x->d_net_off,
+// x->h_net_off, T, C, t_max, E) is in place.
+gtp_status finish_trace(gtp_ctx* x) {
+  const int64_t N = x->N;
+  const int64_t* off = x->h_net_off.data();
+  const int64_t E = x->E;
+  hipStream_t s = x->stream;
   x->d_tile_idx.release();
   x->tile_idx_tc = 0;
   {
@@ -662,7 +645,7 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
     HIPCHK(x, hipStreamSynchronize(s));
   }
   {
-    gtp_status st = ensure_arc_q(x, tr->time_scale);
+    gtp_status st = ensure_arc_q(x, x->scale);
     if (st != GTP_OK) return st;
   }
   // per-gate work (pin events) for the schedule
@@ -683,6 +666,46 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
   x->tune_ms = 0;
   x->have_trace = true;
   return GTP_OK;
+}
+}  // namespace
+
+GTP_API gtp_status gtp_set_trace_vcd(gtp_ctx* x, const gtp_vcd* v, double clk_period) {
+  if (!x) return GTP_ERR_ARG;
+  if (!x->have_design) return x->fail(GTP_ERR_STATE, "gtp_set_trace_vcd: call gtp_set_design first");
+  VcdView vw{};
+  if (!vcd_view(v, &vw)) return x->fail(GTP_ERR_STATE, "gtp_set_trace_vcd: no parsed VCD");
+  if (vw.device != x->device) return x->fail(GTP_ERR_ARG, "gtp_set_trace_vcd: the VCD was parsed on another device");
+  if (vw.n_stream != x->N)
+    return x->fail(GTP_ERR_ARG, "gtp_set_trace_vcd: the VCD has " + std::to_string(vw.n_stream) +
+                                    " streams, the design " + std::to_string(x->N) + " nets");
+  if (!(vw.time_scale > 0) || !(clk_period > 0) || vw.t_max <= 0)
+    return x->fail(GTP_ERR_ARG, "gtp_set_trace_vcd: time scale, clock period and the last time must be positive");
+  if (vw.t_max >= (int64_t)EV_TIME_INF)
+    return x->fail(GTP_ERR_UNSUPPORTED, "gtp_set_trace_vcd: the last time must be below 2^29 - 1 ticks");
+  if (x->opt.max_event_num > 0 && vw.n_event > x->opt.max_event_num)
+    return x->fail(GTP_ERR_OOM, "gtp_set_trace_vcd: " + std::to_string(vw.n_event) + " events exceed max_event_num");
+  const double ratio = clk_period / vw.time_scale;
+  int64_t T = (int64_t)std::ceil(ratio);
+  if (std::fabs(ratio - std::round(ratio)) < 1e-9 * std::max(1.0, ratio)) T = (int64_t)std::llround(ratio);
+  if (T < 1 || T >= (int64_t)EV_TIME_INF) return x->fail(GTP_ERR_UNSUPPORTED, "gtp_set_trace_vcd: ticks per cycle out of range");
+  HIPCHK(x, hipSetDevice(x->device));
+  x->have_trace = false;
+  hipStream_t s = x->stream;
+  const int64_t N = x->N, E = vw.n_event;
+  x->T = T;
+  x->t_max = vw.t_max;
+  x->C = (vw.t_max + T - 1) / T;
+  x->scale = vw.time_scale;
+  x->period = clk_period;
+  x->E = E;
+  x->aux_valid = false;
+  HIPCHK(x, x->d_net_off.alloc(N + 1));
+  HIPCHK(x, hipMemcpyAsync(x->d_net_off.p, vw.d_off, sizeof(int64_t) * (N + 1), hipMemcpyDeviceToDevice, s));
+  x->h_net_off.assign(vw.h_off, vw.h_off + N + 1);
+  HIPCHK(x, x->d_ev.alloc(E + EV_PAD));
+  HIPCHK(x, hipMemcpyAsync(x->d_ev.p, vw.ev, sizeof(uint32_t) * (E + EV_PAD), hipMemcpyDeviceToDevice, s));
+  HIPCHK(x, hipStreamSynchronize(s));
+  return finish_trace(x);
 }
 
 namespace {

@@ -1,5 +1,5 @@
-// Internal layouts shared by the host runtime (gtp_api.cpp) and the gfx950
-// kernels (gtp_kernels.hip).
+// Internal layouts shared by the host runtime (gtp_api.cpp), the gfx950
+// kernels (gtp_kernels.hip) and the device VCD ingest (gtp_vcd.hip).
 //
 // Device event store: one 32-bit word per event, per-net CSR order,
 //   word = (time << 3) | (glitch << 2) | value,
@@ -21,6 +21,39 @@
 #include <hip/hip_runtime.h>
 
 namespace gtp {
+
+// Device array owned by the host runtime.
+template <class T>
+struct DArr {
+  T* p = nullptr;
+  size_t n = 0;
+  DArr() = default;
+  DArr(const DArr&) = delete;
+  DArr& operator=(const DArr&) = delete;
+  ~DArr() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    if (count == n && p) return hipSuccess;
+    release();
+    if (count == 0) return hipSuccess;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
+    n = count;
+    return hipSuccess;
+  }
+  hipError_t upload(const T* src, size_t count, hipStream_t s) {
+    hipError_t e = alloc(count);
+    if (e != hipSuccess || count == 0) return e;
+    return hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s);
+  }
+};
 
 constexpr uint32_t EV_SHIFT = 3;                // time bits start here
 constexpr uint32_t EV_GLITCH = 4u;              // the event carries a glitch pulse width
@@ -234,4 +267,19 @@ namespace gtp {
 // Fills *out from a context with a loaded trace; 0 on success, else a
 // negative status (the context's last error says why).
 int32_t trace_dev(gtp_ctx* ctx, TraceDev* out);
+// A parsed VCD's per-(id, bit) event store on its device (gtp_vcd.hip), for
+// gtp_set_trace_vcd: words as above, padded by EV_PAD words.
+struct VcdView {
+  int device;
+  const uint32_t* ev;       // [n_event + EV_PAD]
+  const int64_t* d_off;     // [n_stream + 1] on the device
+  const int64_t* h_off;     // the same on the host
+  int64_t n_stream, n_event, t_max;
+  double time_scale;
+};
+}  // namespace gtp
+struct gtp_vcd;
+namespace gtp {
+bool vcd_view(const gtp_vcd* v, VcdView* out);
+
 }  // namespace gtp

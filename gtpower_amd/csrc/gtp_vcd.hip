@@ -1,0 +1,976 @@
+// Device VCD ingest: VCD text -> the per-(id, bit) event store of the power
+// engine, built on the GPU.
+//
+// Replaces the reference's single-threaded reader VcdReader::read /
+// parseVarValues (power/VcdReader.cc:80-131, :238-285) with
+// Vcd::varAppendValue / varAppendBusValue (power/Vcd.cc:203-232), and the host
+// phases that cut per-bit event ranges out of it and copy them into the
+// device event store (power/cuda/CudaPower.cu:1874-2086, value table
+// :1129-1145).  The header (declarations, a few KB to MB) is read on the host
+// with the reference's token rules; the value-change section, which is nearly
+// all of the text, is parsed on the device:
+//
+//   token starts     whitespace -> non-whitespace transitions, counted per
+//                    4 KB block, block offsets scanned, starts written
+//   token parse      kind (time / scalar / bus / other), time and bus values
+//                    (stoll / strtol base 2 semantics, saturating)
+//   bus ids          a bus token takes the next token as its id, so which
+//                    tokens are ids follows from the parity of each run of
+//                    bus-looking tokens (a max-scan gives run starts)
+//   times            each change takes the latest time token before it that
+//                    is not an id (max-scan over token indices)
+//   changes          compacted in text order; ids looked up in a device hash
+//                    table of the header's ids (exact byte compare)
+//   grouping         stable radix sort of changes by id (text order kept)
+//   dedupe           a change equal to the previous change of its id is
+//                    dropped (ids mixing scalar and bus values replay the
+//                    reference's last-kept rule sequentially); an id whose
+//                    first change is after 0 gets X (scalar) or 0 (bus) at 0
+//   validation       kept times of an id strictly increasing and below
+//                    2^29 - 1 ticks (max-scan of (id, time) keys)
+//   store            per-stream counts scanned into offsets, every kept change
+//                    scattered to the streams of its id's bits as 32-bit event
+//                    words (time << 3 | value)
+//
+// Streams are numbered per unique id in declaration order, bits 0..w-1 (w =
+// the widest declaration of the id); bit b of a bus value is (value >> b) & 1.
+// Value codes: 0 -> 0, 1 -> 1, X -> 2, Z -> 3 and U -> 2 (the reference's
+// table leaves U unmapped).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "gtp_internal.hpp"
+#include "gtpower.h"
+
+#define GTP_API extern "C" __attribute__((visibility("default")))
+
+namespace gtp {
+namespace vcd {
+
+enum : uint8_t { TK_OTHER = 0, TK_TIME = 1, TK_SCALAR = 2, TK_BUS = 3, TK_TIME_BAD = 4 };
+enum : int { GTP_VCD_ERR_TIME = 1, GTP_VCD_ERR_ID = 2, GTP_VCD_ERR_ORDER = 3, GTP_VCD_ERR_RANGE = 4 };
+
+// the first error in text order: (item << 8) | code, ~0 when none
+struct DevErr {
+  unsigned long long packed;
+};
+
+__device__ __forceinline__ bool is_ws(uint8_t c) { return c == ' ' || (c >= 9 && c <= 13); }
+__device__ __forceinline__ uint8_t upper(uint8_t c) { return (c >= 'a' && c <= 'z') ? (uint8_t)(c - 32) : c; }
+
+__device__ void set_err(DevErr* e, int code, int64_t where) {
+  atomicMin(&e->packed, ((unsigned long long)where << 8) | (unsigned long long)code);
+}
+
+constexpr int TB = 256;             // threads per block
+constexpr int BPT = 16;             // bytes per thread
+constexpr int BPB = TB * BPT;       // bytes per block
+
+// token starts among the 16 bytes of one thread: bit j set when byte
+// base + j starts a token (it is not whitespace, the byte before it is, and
+// it lies in [b0, L)); the byte before b0 is whitespace by construction
+__device__ __forceinline__ uint32_t start_bits(const uint8_t* t, int64_t base, int64_t b0, int64_t L) {
+  const uint4 q = *reinterpret_cast<const uint4*>(t + base);
+  const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+  uint8_t prev = base > 0 ? t[base - 1] : (uint8_t)' ';
+  uint32_t bits = 0;
+#pragma unroll
+  for (int j = 0; j < BPT; ++j) {
+    const uint8_t c = (uint8_t)(w[j >> 2] >> (8 * (j & 3)));
+    const int64_t i = base + j;
+    if (i >= b0 && i < L && !is_ws(c) && is_ws(prev)) bits |= 1u << j;
+    prev = c;
+  }
+  return bits;
+}
+
+__device__ __forceinline__ int block_excl_sum(int v, int* lds, int* total) {
+  // 256 threads, 4 waves of 64
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) lds[wv] = x;
+  __syncthreads();
+  int wo = 0;
+  for (int k = 0; k < wv; ++k) wo += lds[k];
+  *total = lds[0] + lds[1] + lds[2] + lds[3];
+  __syncthreads();
+  return wo + x - v;
+}
+
+__global__ __launch_bounds__(TB) void count_starts_kernel(const uint8_t* __restrict__ t, int64_t base0, int64_t b0,
+                                                          int64_t L, uint64_t* __restrict__ cnt) {
+  __shared__ int lds[4];
+  const int64_t base = base0 + (int64_t)blockIdx.x * BPB + threadIdx.x * BPT;
+  const int c = __popc(start_bits(t, base, b0, L));
+  int total;
+  (void)block_excl_sum(c, lds, &total);
+  if (threadIdx.x == 0) cnt[blockIdx.x] = (uint64_t)total;
+}
+
+__global__ __launch_bounds__(TB) void write_starts_kernel(const uint8_t* __restrict__ t, int64_t base0, int64_t b0,
+                                                          int64_t L, const uint64_t* __restrict__ blk_off,
+                                                          uint32_t* __restrict__ tok, int64_t K) {
+  __shared__ int lds[4];
+  const int64_t base = base0 + (int64_t)blockIdx.x * BPB + threadIdx.x * BPT;
+  uint32_t bits = start_bits(t, base, b0, L);
+  int total;
+  int64_t o = (int64_t)blk_off[blockIdx.x] + block_excl_sum(__popc(bits), lds, &total);
+  while (bits) {
+    const int j = __ffs(bits) - 1;
+    bits &= bits - 1;
+    if (o < K) tok[o] = (uint32_t)(base + j);  // the last token is dropped when the text ends inside it
+    ++o;
+  }
+}
+
+// Kind, length and value of every token (power/VcdReader.cc:245-278).
+__global__ void parse_tokens_kernel(const uint8_t* __restrict__ t, int64_t L, const uint32_t* __restrict__ tok,
+                                    int64_t K, uint8_t* __restrict__ kind, uint32_t* __restrict__ len,
+                                    int64_t* __restrict__ val) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  const int64_t s = tok[k];
+  int64_t e = s + 1;
+  while (e < L && !is_ws(t[e])) ++e;
+  const int64_t n = e - s;
+  const uint8_t c0 = upper(t[s]);
+  uint8_t kd = TK_OTHER;
+  int64_t v = 0;
+  if (c0 == '#' && n > 1) {
+    // std::stoll: optional sign, decimal digits; no digit or overflow = error
+    int64_t p = s + 1;
+    bool neg = false;
+    if (t[p] == '+' || t[p] == '-') {
+      neg = t[p] == '-';
+      ++p;
+    }
+    uint64_t acc = 0;
+    bool any = false, ovf = false;
+    const uint64_t lim = neg ? (1ull << 63) : (1ull << 63) - 1;
+    for (; p < e && t[p] >= '0' && t[p] <= '9'; ++p) {
+      any = true;
+      const uint64_t d = t[p] - '0';
+      if (acc > (lim - d) / 10) ovf = true;
+      else acc = acc * 10 + d;
+    }
+    if (!any || ovf) {
+      kd = TK_TIME_BAD;
+    } else {
+      kd = TK_TIME;
+      v = neg ? (int64_t)(0 - acc) : (int64_t)acc;
+    }
+  } else if (c0 == '0' || c0 == '1' || c0 == 'X' || c0 == 'U' || c0 == 'Z') {
+    kd = TK_SCALAR;
+    v = c0;
+  } else if (c0 == 'B') {
+    kd = TK_BUS;
+    const uint8_t c1 = n > 1 ? upper(t[s + 1]) : 0;
+    if (!(c1 == 'X' || c1 == 'U' || c1 == 'Z')) {
+      // strtol(..., 2): optional sign, binary digits, saturating
+      int64_t p = s + 1;
+      bool neg = false;
+      if (p < e && (t[p] == '+' || t[p] == '-')) {
+        neg = t[p] == '-';
+        ++p;
+      }
+      uint64_t acc = 0;
+      bool ovf = false;
+      const uint64_t lim = neg ? (1ull << 63) : (1ull << 63) - 1;
+      for (; p < e && (t[p] == '0' || t[p] == '1'); ++p) {
+        const uint64_t d = t[p] - '0';
+        if (ovf || acc > (lim - d) / 2) ovf = true;
+        else acc = acc * 2 + d;
+      }
+      if (ovf) v = neg ? INT64_MIN : INT64_MAX;
+      else v = neg ? (int64_t)(0 - acc) : (int64_t)acc;
+    }
+  }
+  kind[k] = kd;
+  len[k] = (uint32_t)n;
+  val[k] = v;
+}
+
+// start of the run of bus-looking tokens holding token k (k itself when the
+// token before is not one), as max-scan input
+__global__ void run_start_in_kernel(const uint8_t* __restrict__ kind, int64_t K, int32_t* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  out[k] = (k == 0 || kind[k - 1] != TK_BUS) ? (int32_t)k : -1;
+}
+
+// token k is the id of the bus token before it
+__device__ __forceinline__ bool consumed_at(const uint8_t* kind, const int32_t* rs, int64_t k) {
+  return k > 0 && kind[k - 1] == TK_BUS && (((k - 1 - rs[k - 1]) & 1) == 0);
+}
+
+// time tokens (not ids) as max-scan input; change flags
+__global__ void time_in_kernel(const uint8_t* __restrict__ kind, const int32_t* __restrict__ rs, int64_t K,
+                               int32_t* __restrict__ ti_in, uint32_t* __restrict__ is_change, DevErr* err) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  const bool cons = consumed_at(kind, rs, k);
+  const uint8_t kd = kind[k];
+  ti_in[k] = (kd == TK_TIME && !cons) ? (int32_t)k : -1;
+  if (kd == TK_TIME_BAD && !cons) set_err(err, GTP_VCD_ERR_TIME, k);
+  bool ch = false;
+  if (!cons && kd == TK_SCALAR) ch = true;
+  if (!cons && kd == TK_BUS) {
+    if (k + 1 < K) ch = true;
+    else set_err(err, GTP_VCD_ERR_ID, k);  // the bus value has no id before the end
+  }
+  is_change[k] = ch ? 1u : 0u;
+}
+
+__device__ __forceinline__ uint64_t fnv1a(const uint8_t* s, int64_t n) {
+  uint64_t h = 1469598103934665603ull;
+  for (int64_t i = 0; i < n; ++i) h = (h ^ s[i]) * 1099511628211ull;
+  return h;
+}
+
+__device__ int32_t lookup_id(const uint8_t* s, int64_t n, const int32_t* tab, uint64_t mask, const uint8_t* idc,
+                             const int64_t* ido) {
+  uint64_t i = fnv1a(s, n) & mask;
+  for (;;) {
+    const int32_t u = tab[i];
+    if (u < 0) return -1;
+    const int64_t b = ido[u], m = ido[u + 1] - b;
+    if (m == n) {
+      bool eq = true;
+      for (int64_t j = 0; j < n && eq; ++j) eq = idc[b + j] == s[j];
+      if (eq) return u;
+    }
+    i = (i + 1) & mask;
+  }
+}
+
+struct ChangeArgs {
+  const uint8_t* t;
+  const uint32_t* tok;
+  const uint8_t* kind;
+  const uint32_t* len;
+  const int64_t* val;
+  const int32_t* ti;        // inclusive max-scan of time tokens
+  const uint32_t* chpos;    // exclusive scan of is_change
+  const uint32_t* is_change;
+  int64_t K;
+  const int32_t* tab;
+  uint64_t mask;
+  const uint8_t* idc;
+  const int64_t* ido;
+  uint32_t* ch_uid;
+  uint32_t* ch_idx;
+  uint32_t* ch_time;
+  uint64_t* ch_val;
+  uint8_t* ch_bus;
+  DevErr* err;
+};
+
+// one change per scalar token or (bus token, id token) pair, in text order
+__global__ void emit_changes_kernel(ChangeArgs a) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.K || !a.is_change[k]) return;
+  const uint32_t c = a.chpos[k];
+  const bool bus = a.kind[k] == TK_BUS;
+  const uint8_t* id;
+  int64_t n;
+  if (bus) {
+    id = a.t + a.tok[k + 1];
+    n = a.len[k + 1];
+  } else {
+    id = a.t + a.tok[k] + 1;
+    n = (int64_t)a.len[k] - 1;
+  }
+  const int32_t u = lookup_id(id, n, a.tab, a.mask, a.idc, a.ido);
+  if (u < 0) set_err(a.err, GTP_VCD_ERR_ID, k);
+  const int32_t tk = a.ti[k];
+  const int64_t time = tk >= 0 ? a.val[tk] : 0;
+  if (time < 0 || time >= (int64_t)EV_TIME_INF) set_err(a.err, GTP_VCD_ERR_RANGE, k);
+  a.ch_uid[c] = u < 0 ? 0u : (uint32_t)u;
+  a.ch_idx[c] = c;
+  a.ch_time[c] = (uint32_t)(time < 0 ? 0 : min(time, (int64_t)EV_TIME_INF));
+  a.ch_val[c] = (uint64_t)a.val[k];
+  a.ch_bus[c] = bus ? 1 : 0;
+}
+
+// segment bounds per id and the parallel keep rule: a change equal to the
+// previous change of its id is dropped (exact when the id's changes are all
+// scalar or all bus); ids mixing both are flagged for the sequential pass
+__global__ void keep_kernel(const uint32_t* __restrict__ key, const uint32_t* __restrict__ perm, int64_t n,
+                            const uint64_t* __restrict__ ch_val, const uint8_t* __restrict__ ch_bus,
+                            int64_t* __restrict__ seg_start, int64_t* __restrict__ seg_end,
+                            uint32_t* __restrict__ mixed, uint32_t* __restrict__ keep) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t u = key[p], c = perm[p];
+  const bool first = p == 0 || key[p - 1] != u;
+  const bool last = p == n - 1 || key[p + 1] != u;
+  if (first) seg_start[u] = p;
+  if (last) seg_end[u] = p + 1;
+  bool k = true;
+  if (!first) {
+    const uint32_t cp = perm[p - 1];
+    if (ch_bus[c] != ch_bus[cp]) mixed[u] = 1;
+    k = ch_val[c] != ch_val[cp];
+  }
+  keep[p] = k ? 1u : 0u;
+}
+
+// ids with both scalar and bus changes: the reference's rule with the last
+// kept value (a scalar compares its character with the last value's, '\0'
+// for a bus value; a bus compares its value with the last one's, 0 for a
+// scalar) (power/Vcd.cc:203-232), one thread per id
+__global__ void keep_mixed_kernel(const uint32_t* __restrict__ mixed, int64_t n_id,
+                                  const int64_t* __restrict__ seg_start, const int64_t* __restrict__ seg_end,
+                                  const uint32_t* __restrict__ perm, const uint64_t* __restrict__ ch_val,
+                                  const uint8_t* __restrict__ ch_bus, uint32_t* __restrict__ keep) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n_id || !mixed[u]) return;
+  uint64_t last_char = 0, last_bus = 0;
+  for (int64_t p = seg_start[u]; p < seg_end[u]; ++p) {
+    const uint32_t c = perm[p];
+    const bool bus = ch_bus[c] != 0;
+    const uint64_t v = ch_val[c];
+    const bool k = p == seg_start[u] || (bus ? v != last_bus : v != last_char);
+    keep[p] = k ? 1u : 0u;
+    if (k) {
+      last_char = bus ? 0 : v;
+      last_bus = bus ? v : 0;
+    }
+  }
+}
+
+// (id, time + 1) keys of kept changes for the order check
+__global__ void order_key_kernel(const uint32_t* __restrict__ key, const uint32_t* __restrict__ perm, int64_t n,
+                                 const uint32_t* __restrict__ keep, const uint32_t* __restrict__ ch_time,
+                                 uint64_t* __restrict__ out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  out[p] = keep[p] ? (((uint64_t)key[p] << 32) | ((uint64_t)ch_time[perm[p]] + 1)) : 0ull;
+}
+
+// a kept change no later than an earlier kept change of its id
+__global__ void order_check_kernel(const uint32_t* __restrict__ key, int64_t n, const uint32_t* __restrict__ keep,
+                                   const uint64_t* __restrict__ okey, const uint64_t* __restrict__ omax,
+                                   DevErr* err) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n || !keep[p]) return;
+  const uint64_t prev = omax[p];  // exclusive max of earlier keys
+  if ((prev >> 32) == key[p] && (prev & 0xffffffffull) >= (okey[p] & 0xffffffffull))
+    set_err(err, GTP_VCD_ERR_ORDER, p);
+}
+
+// events per stream: the id's kept changes, plus the initial value when its
+// first change is after 0
+__global__ void stream_count_kernel(const int32_t* __restrict__ stream_uid, int64_t S,
+                                    const int64_t* __restrict__ seg_start, const int64_t* __restrict__ seg_end,
+                                    const uint32_t* __restrict__ ks, const uint32_t* __restrict__ perm,
+                                    const uint32_t* __restrict__ ch_time, int64_t* __restrict__ cnt) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const int32_t u = stream_uid[s];
+  const int64_t a = seg_start[u];
+  int64_t c = 0;
+  if (a >= 0) c = (int64_t)(ks[seg_end[u]] - ks[a]) + (ch_time[perm[a]] != 0 ? 1 : 0);
+  cnt[s] = c;
+}
+
+struct ScatterArgs {
+  const uint32_t* key;
+  const uint32_t* perm;
+  int64_t n;
+  const uint32_t* keep;
+  const uint32_t* ks;
+  const int64_t* seg_start;
+  const uint32_t* ch_time;
+  const uint64_t* ch_val;
+  const uint8_t* ch_bus;
+  const int32_t* uid_width;
+  const int64_t* stream_base;
+  const int64_t* stream_off;
+  uint32_t* ev;
+};
+
+__device__ __forceinline__ uint32_t code_of(uint64_t ch) {
+  return ch == '0' ? 0u : ch == '1' ? 1u : ch == 'Z' ? 3u : 2u;
+}
+
+// every kept change to the streams of its id's bits; the initial value of an
+// id whose first change is after 0 at rank 0
+__global__ void scatter_kernel(ScatterArgs a) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n || !a.keep[p]) return;
+  const uint32_t u = a.key[p], c = a.perm[p];
+  const int64_t s0 = a.seg_start[u];
+  const uint32_t cf = a.perm[s0];
+  const int init = a.ch_time[cf] != 0 ? 1 : 0;
+  const int64_t rank = (int64_t)(a.ks[p] - a.ks[s0]) + init;
+  const uint32_t tw = a.ch_time[c] << EV_SHIFT;
+  const bool bus = a.ch_bus[c] != 0;
+  const uint64_t v = a.ch_val[c];
+  const int w = a.uid_width[u];
+  const int64_t sb = a.stream_base[u];
+  for (int b = 0; b < w; ++b) {
+    const uint32_t code = bus ? (uint32_t)((v >> b) & 1u) : code_of(v);
+    a.ev[a.stream_off[sb + b] + rank] = tw | code;
+  }
+  if (p == s0 && init) {
+    const uint32_t code0 = a.ch_bus[cf] ? 0u : 2u;  // bus 0 or X at time 0
+    for (int b = 0; b < w; ++b) a.ev[a.stream_off[sb + b]] = code0;
+  }
+}
+
+__global__ void fill_words_kernel(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void fill_i64_kernel(int64_t* __restrict__ p, int64_t n, int64_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+struct MaxI32 {
+  __device__ __host__ int32_t operator()(int32_t a, int32_t b) const { return a > b ? a : b; }
+};
+struct MaxU64 {
+  __device__ __host__ uint64_t operator()(uint64_t a, uint64_t b) const { return a > b ? a : b; }
+};
+
+inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace vcd
+}  // namespace gtp
+
+using namespace gtp;
+using namespace gtp::vcd;
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct gtp_vcd {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string error;
+  // header
+  struct Var {
+    std::string name, id;
+    int32_t width;
+    int32_t uid;
+  };
+  std::vector<Var> vars;
+  std::vector<std::string> ids;     // unique ids in declaration order
+  std::vector<int32_t> uid_width;   // widest declaration
+  std::vector<int64_t> stream_base; // [n_id + 1]
+  double time_scale = 1.0;
+  // result
+  DArr<uint32_t> d_ev;              // [n_event + EV_PAD] event words
+  DArr<int64_t> d_off;              // [n_stream + 1]
+  std::vector<int64_t> h_off;
+  gtp_vcd_info info{};
+  bool parsed = false;
+
+  gtp_status fail(gtp_status st, const std::string& m) {
+    error = m;
+    return st;
+  }
+};
+
+namespace {
+
+bool host_ws(int c) { return c == ' ' || (c >= 9 && c <= 13); }
+
+// The reference's token reader over the header (power/VcdReader.cc:287-310):
+// the character after a token is consumed with it; a token that the text's
+// end cuts off is dropped.
+struct HeaderReader {
+  const char* t;
+  int64_t L, p = 0;
+  bool tok(std::string& out) {
+    out.clear();
+    while (p < L && host_ws((unsigned char)t[p])) ++p;
+    while (p < L && !host_ws((unsigned char)t[p])) out.push_back(t[p++]);
+    if (p >= L) {
+      out.clear();
+      return false;
+    }
+    ++p;  // the terminating whitespace
+    return true;
+  }
+  std::vector<std::string> stmt() {
+    std::vector<std::string> v;
+    std::string s;
+    while (tok(s) && s != "$end") v.push_back(s);
+    return v;
+  }
+};
+
+// power/VcdReader.cc:95-122, :145-236: declarations up to the token that
+// starts the value changes; returns the offset right after that token
+gtp_status read_header(gtp_vcd* v, const char* text, int64_t L, int64_t* body) {
+  HeaderReader r{text, L};
+  std::vector<std::string> scope;
+  std::unordered_map<std::string, int32_t> uid_of;
+  v->vars.clear();
+  v->ids.clear();
+  v->uid_width.clear();
+  v->time_scale = 1.0;
+  double unit = 1.0;
+  static const char* types[] = {"wire", "reg", "parameter", "integer", "real", "supply0", "supply1", "time",
+                                "tri", "triand", "trior", "trireg", "tri0", "tri1", "wand", "wor"};
+  auto set_unit = [&](const std::string& u) {
+    if (u == "fs") unit = 1e-15;
+    else if (u == "ps") unit = 1e-12;
+    else if (u == "ns") unit = 1e-9;
+    else return false;
+    return true;
+  };
+  std::string t;
+  *body = L;
+  while (r.tok(t)) {
+    if (t == "$date" || t == "$comment" || t == "$version" || t == "$enddefinitions") {
+      r.stmt();
+    } else if (t == "$timescale") {
+      std::vector<std::string> ts = r.stmt();
+      try {
+        if (ts.size() == 1) {
+          size_t last = 0;
+          const double s = std::stod(ts[0], &last);
+          if (!set_unit(ts[0].substr(last))) return v->fail(GTP_ERR_ARG, "vcd: unknown timescale unit");
+          v->time_scale = s * unit;
+        } else if (ts.size() == 2) {
+          if (!set_unit(ts[1])) return v->fail(GTP_ERR_ARG, "vcd: unknown timescale unit");
+          v->time_scale = std::stod(ts[0]) * unit;
+        } else {
+          return v->fail(GTP_ERR_ARG, "vcd: timescale syntax error");
+        }
+      } catch (...) {
+        return v->fail(GTP_ERR_ARG, "vcd: timescale syntax error");
+      }
+    } else if (t == "$var") {
+      std::vector<std::string> ts = r.stmt();
+      if (ts.size() != 4 && ts.size() != 5) return v->fail(GTP_ERR_ARG, "vcd: variable syntax error");
+      bool known = false;
+      for (const char* ty : types) known = known || ts[0] == ty;
+      if (!known) continue;  // the reference warns and skips the variable
+      int width = 0;
+      try {
+        width = std::stoi(ts[1]);
+      } catch (...) {
+        return v->fail(GTP_ERR_ARG, "vcd: variable width syntax error");
+      }
+      if (width < 1 || width > 64) return v->fail(GTP_ERR_UNSUPPORTED, "vcd: variable width outside [1, 64]");
+      std::string name;
+      for (const std::string& c : scope) name += c + "/";
+      for (char c : ts[3]) {
+        if (c == '/') name += "\\/";
+        else name += c;
+      }
+      if (ts.size() == 5) {
+        if (name[0] == '\\') name += ' ';
+        name += ts[4];
+      }
+      auto it = uid_of.find(ts[2]);
+      int32_t uid;
+      if (it == uid_of.end()) {
+        uid = (int32_t)v->ids.size();
+        uid_of[ts[2]] = uid;
+        v->ids.push_back(ts[2]);
+        v->uid_width.push_back(width);
+      } else {
+        uid = it->second;
+        v->uid_width[uid] = std::max(v->uid_width[uid], (int32_t)width);
+      }
+      v->vars.push_back(gtp_vcd::Var{name, ts[2], (int32_t)width, uid});
+    } else if (t == "$scope") {
+      std::vector<std::string> ts = r.stmt();
+      if (ts.size() < 2) return v->fail(GTP_ERR_ARG, "vcd: scope syntax error");
+      scope.push_back(ts[1]);
+    } else if (t == "$upscope") {
+      r.stmt();
+      if (scope.empty()) return v->fail(GTP_ERR_ARG, "vcd: upscope without scope");
+      scope.pop_back();
+    } else if (t == "$dumpall" || t == "$dumpvars" || t[0] != '$') {
+      *body = r.p;  // the value changes start after this token
+      return GTP_OK;
+    } else {
+      return v->fail(GTP_ERR_ARG, "vcd: unhandled vcd command " + t);
+    }
+  }
+  return GTP_OK;
+}
+
+uint64_t host_fnv1a(const std::string& s) {
+  uint64_t h = 1469598103934665603ull;
+  for (unsigned char c : s) h = (h ^ c) * 1099511628211ull;
+  return h;
+}
+
+const char* err_text(int code) {
+  switch (code) {
+    case GTP_VCD_ERR_TIME: return "time syntax error";
+    case GTP_VCD_ERR_ID: return "unknown variable id";
+    case GTP_VCD_ERR_ORDER: return "value changes of an id not increasing in time";
+    case GTP_VCD_ERR_RANGE: return "value change time outside [0, 2^29 - 1) ticks";
+    default: return "error";
+  }
+}
+
+}  // namespace
+
+#define VCHK(v, expr)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return (v)->fail(GTP_ERR_DEVICE, std::string("vcd: ") + #expr + ": " + \
+                                               hipGetErrorString(e_));                      \
+  } while (0)
+
+GTP_API gtp_status gtp_vcd_create(int32_t device, gtp_vcd** out) {
+  if (!out) return GTP_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return GTP_ERR_DEVICE;
+  gtp_vcd* v = new gtp_vcd();
+  v->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&v->stream) != hipSuccess) {
+    delete v;
+    return GTP_ERR_DEVICE;
+  }
+  *out = v;
+  return GTP_OK;
+}
+
+GTP_API void gtp_vcd_destroy(gtp_vcd* v) {
+  if (!v) return;
+  (void)hipSetDevice(v->device);
+  v->d_ev.release();
+  v->d_off.release();
+  if (v->stream) (void)hipStreamDestroy(v->stream);
+  delete v;
+}
+
+GTP_API const char* gtp_vcd_last_error(const gtp_vcd* v) { return v ? v->error.c_str() : "null handle"; }
+
+GTP_API gtp_status gtp_vcd_parse(gtp_vcd* v, const char* text, int64_t len, gtp_vcd_info* info) {
+  if (!v) return GTP_ERR_ARG;
+  if (!text || len < 0) return v->fail(GTP_ERR_ARG, "gtp_vcd_parse: bad text");
+  if (len >= (int64_t)UINT32_MAX - 64)
+    return v->fail(GTP_ERR_UNSUPPORTED, "gtp_vcd_parse: texts of 4 GiB or more are read in several parts");
+  v->parsed = false;
+  v->error.clear();
+  VCHK(v, hipSetDevice(v->device));
+  hipStream_t s = v->stream;
+  int64_t b0 = 0;
+  {
+    const gtp_status st = read_header(v, text, len, &b0);
+    if (st != GTP_OK) return st;
+  }
+  const int64_t U = (int64_t)v->ids.size();
+  v->stream_base.assign(U + 1, 0);
+  for (int64_t u = 0; u < U; ++u) v->stream_base[u + 1] = v->stream_base[u] + v->uid_width[u];
+  const int64_t S = v->stream_base[U];
+  std::vector<int32_t> stream_uid(std::max<int64_t>(S, 1), 0);
+  for (int64_t u = 0; u < U; ++u)
+    for (int64_t q = v->stream_base[u]; q < v->stream_base[u + 1]; ++q) stream_uid[q] = (int32_t)u;
+  // id table: open addressing over the unique ids, exact compare on the device
+  uint64_t tsz = 16;
+  while (tsz < 2 * (uint64_t)std::max<int64_t>(U, 1)) tsz <<= 1;
+  std::vector<int32_t> tab(tsz, -1);
+  std::vector<int64_t> ido(U + 1, 0);
+  std::string idc;
+  for (int64_t u = 0; u < U; ++u) {
+    uint64_t i = host_fnv1a(v->ids[u]) & (tsz - 1);
+    while (tab[i] >= 0) i = (i + 1) & (tsz - 1);
+    tab[i] = (int32_t)u;
+    idc += v->ids[u];
+    ido[u + 1] = (int64_t)idc.size();
+  }
+  // the body's last token is dropped when the text ends inside it
+  const bool cut = len > b0 && !host_ws((unsigned char)text[len - 1]);
+
+  const auto t0 = std::chrono::steady_clock::now();
+  const int64_t b0al = b0 & ~(int64_t)15;
+  const int64_t n_blk = std::max<int64_t>(1, (len - b0al + BPB - 1) / BPB);
+  DArr<uint8_t> d_text;
+  VCHK(v, d_text.alloc((size_t)(b0al + n_blk * BPB + 16)));
+  VCHK(v, hipMemsetAsync(d_text.p, ' ', (size_t)(b0al + n_blk * BPB + 16), s));
+  VCHK(v, hipMemcpyAsync(d_text.p, text, (size_t)len, hipMemcpyHostToDevice, s));
+  DArr<int32_t> d_tab, d_stream_uid, d_uid_width;
+  DArr<int64_t> d_ido, d_stream_base;
+  DArr<uint8_t> d_idc;
+  VCHK(v, d_tab.upload(tab.data(), tab.size(), s));
+  VCHK(v, d_ido.upload(ido.data(), ido.size(), s));
+  VCHK(v, d_idc.upload(reinterpret_cast<const uint8_t*>(idc.data()), std::max<size_t>(idc.size(), 1), s));
+  VCHK(v, d_stream_uid.upload(stream_uid.data(), stream_uid.size(), s));
+  VCHK(v, d_uid_width.upload(v->uid_width.empty() ? stream_uid.data() : v->uid_width.data(),
+                             std::max<size_t>(v->uid_width.size(), 1), s));
+  VCHK(v, d_stream_base.upload(v->stream_base.data(), v->stream_base.size(), s));
+  DArr<DevErr> d_err;
+  VCHK(v, d_err.alloc(1));
+  {
+    DevErr e0{~0ull};
+    VCHK(v, hipMemcpyAsync(d_err.p, &e0, sizeof(e0), hipMemcpyHostToDevice, s));
+  }
+  hipEvent_t ev0, ev1;
+  VCHK(v, hipEventCreate(&ev0));
+  VCHK(v, hipEventCreate(&ev1));
+  VCHK(v, hipEventRecord(ev0, s));
+
+  // token starts
+  DArr<uint64_t> d_bcnt, d_boff;
+  VCHK(v, d_bcnt.alloc(n_blk));
+  VCHK(v, d_boff.alloc(n_blk));
+  count_starts_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(d_text.p, b0al, b0, len, d_bcnt.p);
+  VCHK(v, hipGetLastError());
+  DArr<uint8_t> d_tmp;
+  size_t tmp_bytes = 0;
+  auto tmp = [&](size_t need) -> hipError_t {
+    if (need > tmp_bytes) {
+      hipError_t e = d_tmp.alloc(need);
+      if (e != hipSuccess) return e;
+      tmp_bytes = need;
+    }
+    return hipSuccess;
+  };
+  {
+    size_t need = 0;
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, d_bcnt.p, d_boff.p, n_blk, s));
+    VCHK(v, tmp(need));
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(d_tmp.p, need, d_bcnt.p, d_boff.p, n_blk, s));
+  }
+  uint64_t last_off = 0, last_cnt = 0;
+  VCHK(v, hipMemcpyAsync(&last_off, d_boff.p + n_blk - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  VCHK(v, hipMemcpyAsync(&last_cnt, d_bcnt.p + n_blk - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  VCHK(v, hipStreamSynchronize(s));
+  int64_t K = (int64_t)(last_off + last_cnt);
+  if (cut && K > 0) --K;
+  DArr<uint32_t> d_tok;
+  VCHK(v, d_tok.alloc(std::max<int64_t>(K, 1)));
+  write_starts_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(d_text.p, b0al, b0, len, d_boff.p, d_tok.p, K);
+  VCHK(v, hipGetLastError());
+
+  // tokens
+  DArr<uint8_t> d_kind;
+  DArr<uint32_t> d_len, d_is_change, d_chpos;
+  DArr<int64_t> d_val;
+  DArr<int32_t> d_rs_in, d_rs, d_ti_in, d_ti;
+  const int64_t K1 = std::max<int64_t>(K, 1);
+  VCHK(v, d_kind.alloc(K1));
+  VCHK(v, d_len.alloc(K1));
+  VCHK(v, d_val.alloc(K1));
+  VCHK(v, d_rs_in.alloc(K1));
+  VCHK(v, d_rs.alloc(K1));
+  VCHK(v, d_ti_in.alloc(K1));
+  VCHK(v, d_ti.alloc(K1));
+  VCHK(v, d_is_change.alloc(K1 + 1));
+  VCHK(v, d_chpos.alloc(K1 + 1));
+  int64_t n_change = 0;
+  int64_t t_max = 0;
+  if (K > 0) {
+    parse_tokens_kernel<<<nblk(K), 256, 0, s>>>(d_text.p, len, d_tok.p, K, d_kind.p, d_len.p, d_val.p);
+    VCHK(v, hipGetLastError());
+    run_start_in_kernel<<<nblk(K), 256, 0, s>>>(d_kind.p, K, d_rs_in.p);
+    VCHK(v, hipGetLastError());
+    size_t need = 0;
+    VCHK(v, hipcub::DeviceScan::InclusiveScan(nullptr, need, d_rs_in.p, d_rs.p, MaxI32(), K, s));
+    VCHK(v, tmp(need));
+    VCHK(v, hipcub::DeviceScan::InclusiveScan(d_tmp.p, need, d_rs_in.p, d_rs.p, MaxI32(), K, s));
+    time_in_kernel<<<nblk(K), 256, 0, s>>>(d_kind.p, d_rs.p, K, d_ti_in.p, d_is_change.p, d_err.p);
+    VCHK(v, hipGetLastError());
+    VCHK(v, hipcub::DeviceScan::InclusiveScan(nullptr, need, d_ti_in.p, d_ti.p, MaxI32(), K, s));
+    VCHK(v, tmp(need));
+    VCHK(v, hipcub::DeviceScan::InclusiveScan(d_tmp.p, need, d_ti_in.p, d_ti.p, MaxI32(), K, s));
+    fill_u32_kernel<<<1, 1, 0, s>>>(d_is_change.p + K, 1, 0u);
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, d_is_change.p, d_chpos.p, K + 1, s));
+    VCHK(v, tmp(need));
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(d_tmp.p, need, d_is_change.p, d_chpos.p, K + 1, s));
+    uint32_t nc = 0;
+    int32_t tlast = -1;
+    VCHK(v, hipMemcpyAsync(&nc, d_chpos.p + K, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    VCHK(v, hipMemcpyAsync(&tlast, d_ti.p + K - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    VCHK(v, hipStreamSynchronize(s));
+    n_change = nc;
+    if (tlast >= 0) VCHK(v, hipMemcpy(&t_max, d_val.p + tlast, sizeof(int64_t), hipMemcpyDeviceToHost));
+  }
+
+  // changes in text order, grouped by id (stable)
+  const int64_t NC1 = std::max<int64_t>(n_change, 1);
+  DArr<uint32_t> d_ch_uid, d_ch_idx, d_key, d_perm, d_ch_time, d_keep, d_ks, d_mixed;
+  DArr<uint64_t> d_ch_val, d_okey, d_omax;
+  DArr<uint8_t> d_ch_bus;
+  DArr<int64_t> d_seg_start, d_seg_end;
+  VCHK(v, d_ch_uid.alloc(NC1));
+  VCHK(v, d_ch_idx.alloc(NC1));
+  VCHK(v, d_key.alloc(NC1));
+  VCHK(v, d_perm.alloc(NC1));
+  VCHK(v, d_ch_time.alloc(NC1));
+  VCHK(v, d_ch_val.alloc(NC1));
+  VCHK(v, d_ch_bus.alloc(NC1));
+  VCHK(v, d_keep.alloc(NC1 + 1));
+  VCHK(v, d_ks.alloc(NC1 + 1));
+  VCHK(v, d_seg_start.alloc(std::max<int64_t>(U, 1)));
+  VCHK(v, d_seg_end.alloc(std::max<int64_t>(U, 1)));
+  VCHK(v, d_mixed.alloc(std::max<int64_t>(U, 1)));
+  fill_i64_kernel<<<nblk(std::max<int64_t>(U, 1)), 256, 0, s>>>(d_seg_start.p, std::max<int64_t>(U, 1), -1);
+  fill_i64_kernel<<<nblk(std::max<int64_t>(U, 1)), 256, 0, s>>>(d_seg_end.p, std::max<int64_t>(U, 1), 0);
+  fill_u32_kernel<<<nblk(std::max<int64_t>(U, 1)), 256, 0, s>>>(d_mixed.p, std::max<int64_t>(U, 1), 0u);
+  VCHK(v, hipGetLastError());
+  if (n_change > 0) {
+    ChangeArgs ca{d_text.p, d_tok.p, d_kind.p, d_len.p, d_val.p, d_ti.p, d_chpos.p, d_is_change.p, K,
+                  d_tab.p, tsz - 1, d_idc.p, d_ido.p, d_ch_uid.p, d_ch_idx.p, d_ch_time.p, d_ch_val.p,
+                  d_ch_bus.p, d_err.p};
+    emit_changes_kernel<<<nblk(K), 256, 0, s>>>(ca);
+    VCHK(v, hipGetLastError());
+    int bits = 1;
+    while (bits < 32 && (1ll << bits) < U) ++bits;
+    size_t need = 0;
+    VCHK(v, hipcub::DeviceRadixSort::SortPairs(nullptr, need, d_ch_uid.p, d_key.p, d_ch_idx.p, d_perm.p,
+                                               n_change, 0, bits, s));
+    VCHK(v, tmp(need));
+    VCHK(v, hipcub::DeviceRadixSort::SortPairs(d_tmp.p, need, d_ch_uid.p, d_key.p, d_ch_idx.p, d_perm.p,
+                                               n_change, 0, bits, s));
+    keep_kernel<<<nblk(n_change), 256, 0, s>>>(d_key.p, d_perm.p, n_change, d_ch_val.p, d_ch_bus.p,
+                                               d_seg_start.p, d_seg_end.p, d_mixed.p, d_keep.p);
+    VCHK(v, hipGetLastError());
+    if (U > 0) {
+      keep_mixed_kernel<<<nblk(U), 256, 0, s>>>(d_mixed.p, U, d_seg_start.p, d_seg_end.p, d_perm.p, d_ch_val.p,
+                                                d_ch_bus.p, d_keep.p);
+      VCHK(v, hipGetLastError());
+    }
+    VCHK(v, d_okey.alloc(NC1));
+    VCHK(v, d_omax.alloc(NC1));
+    order_key_kernel<<<nblk(n_change), 256, 0, s>>>(d_key.p, d_perm.p, n_change, d_keep.p, d_ch_time.p, d_okey.p);
+    VCHK(v, hipGetLastError());
+    VCHK(v, hipcub::DeviceScan::ExclusiveScan(nullptr, need, d_okey.p, d_omax.p, MaxU64(), (uint64_t)0,
+                                              n_change, s));
+    VCHK(v, tmp(need));
+    VCHK(v, hipcub::DeviceScan::ExclusiveScan(d_tmp.p, need, d_okey.p, d_omax.p, MaxU64(), (uint64_t)0,
+                                              n_change, s));
+    order_check_kernel<<<nblk(n_change), 256, 0, s>>>(d_key.p, n_change, d_keep.p, d_okey.p, d_omax.p, d_err.p);
+    VCHK(v, hipGetLastError());
+  }
+  fill_u32_kernel<<<1, 1, 0, s>>>(d_keep.p + n_change, 1, 0u);
+  {
+    size_t need = 0;
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, d_keep.p, d_ks.p, n_change + 1, s));
+    VCHK(v, tmp(need));
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(d_tmp.p, need, d_keep.p, d_ks.p, n_change + 1, s));
+  }
+  // streams
+  DArr<int64_t> d_scnt;
+  VCHK(v, d_scnt.alloc(S + 1));
+  VCHK(v, v->d_off.alloc(S + 1));
+  if (S > 0) {
+    stream_count_kernel<<<nblk(S), 256, 0, s>>>(d_stream_uid.p, S, d_seg_start.p, d_seg_end.p, d_ks.p, d_perm.p,
+                                                d_ch_time.p, d_scnt.p);
+    VCHK(v, hipGetLastError());
+  }
+  fill_i64_kernel<<<1, 1, 0, s>>>(d_scnt.p + S, 1, 0);
+  {
+    size_t need = 0;
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, d_scnt.p, v->d_off.p, S + 1, s));
+    VCHK(v, tmp(need));
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(d_tmp.p, need, d_scnt.p, v->d_off.p, S + 1, s));
+  }
+  DevErr he{};
+  int64_t E = 0;
+  VCHK(v, hipMemcpyAsync(&he, d_err.p, sizeof(he), hipMemcpyDeviceToHost, s));
+  VCHK(v, hipMemcpyAsync(&E, v->d_off.p + S, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  VCHK(v, hipStreamSynchronize(s));
+  if (he.packed != ~0ull) {
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    return v->fail(GTP_ERR_ARG, std::string("vcd: ") + err_text((int)(he.packed & 0xff)) + " (item " +
+                                    std::to_string(he.packed >> 8) + " of the value-change section)");
+  }
+  VCHK(v, v->d_ev.alloc(E + EV_PAD));
+  fill_words_kernel<<<1, 64, 0, s>>>(v->d_ev.p + E, EV_PAD, EV_INF);
+  if (n_change > 0) {
+    ScatterArgs sa{d_key.p, d_perm.p, n_change, d_keep.p, d_ks.p, d_seg_start.p, d_ch_time.p, d_ch_val.p,
+                   d_ch_bus.p, d_uid_width.p, d_stream_base.p, v->d_off.p, v->d_ev.p};
+    scatter_kernel<<<nblk(n_change), 256, 0, s>>>(sa);
+    VCHK(v, hipGetLastError());
+  }
+  VCHK(v, hipEventRecord(ev1, s));
+  v->h_off.resize(S + 1);
+  VCHK(v, hipMemcpyAsync(v->h_off.data(), v->d_off.p, sizeof(int64_t) * (S + 1), hipMemcpyDeviceToHost, s));
+  VCHK(v, hipStreamSynchronize(s));
+  float ms = 0;
+  VCHK(v, hipEventElapsedTime(&ms, ev0, ev1));
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  const auto t1 = std::chrono::steady_clock::now();
+  v->info = gtp_vcd_info{};
+  v->info.n_var = (int64_t)v->vars.size();
+  v->info.n_id = U;
+  v->info.n_stream = S;
+  v->info.n_event = E;
+  v->info.t_max = t_max;
+  v->info.time_scale = v->time_scale;
+  v->info.n_token = K;
+  v->info.n_change = n_change;
+  v->info.body_bytes = len - b0;
+  v->info.device_ms = ms;
+  v->info.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  v->parsed = true;
+  if (info) *info = v->info;
+  return GTP_OK;
+}
+
+GTP_API gtp_status gtp_vcd_var(const gtp_vcd* v, int64_t i, gtp_vcd_var_info* out) {
+  if (!v || !out || i < 0 || i >= (int64_t)v->vars.size()) return GTP_ERR_ARG;
+  const gtp_vcd::Var& x = v->vars[i];
+  out->name = x.name.c_str();
+  out->id = x.id.c_str();
+  out->width = x.width;
+  out->id_index = x.uid;
+  out->first_stream = v->stream_base[x.uid];
+  return GTP_OK;
+}
+
+GTP_API gtp_status gtp_vcd_events(gtp_vcd* v, int64_t* offsets, int64_t* times, uint8_t* values) {
+  if (!v || !v->parsed) return GTP_ERR_STATE;
+  if (!offsets) return GTP_ERR_ARG;
+  const int64_t S = v->info.n_stream, E = v->info.n_event;
+  std::memcpy(offsets, v->h_off.data(), sizeof(int64_t) * (S + 1));
+  if (E > 0 && (!times || !values)) return GTP_ERR_ARG;
+  std::vector<uint32_t> w(E);
+  if (E > 0) {
+    VCHK(v, hipSetDevice(v->device));
+    VCHK(v, hipMemcpy(w.data(), v->d_ev.p, sizeof(uint32_t) * E, hipMemcpyDeviceToHost));
+  }
+  for (int64_t e = 0; e < E; ++e) {
+    times[e] = w[e] >> EV_SHIFT;
+    values[e] = (uint8_t)(w[e] & 3u);
+  }
+  return GTP_OK;
+}
+
+namespace gtp {
+bool vcd_view(const gtp_vcd* v, VcdView* out) {
+  if (!v || !v->parsed) return false;
+  out->device = v->device;
+  out->ev = v->d_ev.p;
+  out->d_off = v->d_off.p;
+  out->h_off = v->h_off.data();
+  out->n_stream = v->info.n_stream;
+  out->n_event = v->info.n_event;
+  out->t_max = v->info.t_max;
+  out->time_scale = v->info.time_scale;
+  return true;
+}
+}  // namespace gtp

@@ -51,6 +51,22 @@ def options(**kw):
     return o
 
 
+class VcdClock:
+    """The clock grid of a trace loaded from a VCD (what power() needs of it)."""
+
+    def __init__(self, info, clk_period):
+        self.t_max = int(info["t_max"])
+        self.time_scale = float(info["time_scale"])
+        self.clk_period = float(clk_period)
+        self.n_event = int(info["n_event"])
+        ratio = self.clk_period / self.time_scale
+        T = int(np.ceil(ratio))
+        if abs(ratio - round(ratio)) < 1e-9 * max(1.0, ratio):
+            T = int(round(ratio))
+        self.ticks_per_cycle = T
+        self.n_cycle = -(-self.t_max // T)
+
+
 class HipPower:
     """One engine context bound to one HIP device."""
 
@@ -82,6 +98,12 @@ class HipPower:
         tv = trace.as_abi()
         self._check(self._lib.gtp_set_trace(self._h, C.byref(tv)), "gtp_set_trace")
         self.trace = trace
+
+    def read_vcd(self, vcd, clk_period):
+        """Load a VCD parsed on the device (gtpower_amd.vcd.VcdText) as the
+        trace: the design's nets are the VCD's (id, bit) streams."""
+        self._check(self._lib.gtp_set_trace_vcd(self._h, vcd.handle, float(clk_period)), "gtp_set_trace_vcd")
+        self.trace = VcdClock(vcd.info, clk_period)
 
     def power(self, want=("gate", "cycle", "counts")):
         G = self.n_gate

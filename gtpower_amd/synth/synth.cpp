@@ -18,6 +18,8 @@
 #include "synth.h"
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstdio>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -689,5 +691,128 @@ void gtps_trace_view(const gtps_trace* t, gtp_trace* v) {
 }
 
 int64_t gtps_trace_events(const gtps_trace* t) { return t->off.empty() ? 0 : t->off.back(); }
+
+// VCD text of a per-net trace (CSR off/time/value over n_net nets): one
+// variable per net, except the n_bus ranges [bus_first[k], +bus_width[k])
+// (ascending, disjoint), each one bus whose bit b is net bus_first[k] + b.
+// Variables are declared in net order with ids of printable characters, so
+// that the reader's stream of (id, bit) is the net of the same index.  Times
+// are written at each distinct event time; a bus is written whole when any
+// of its bits changes.  The text is malloc'ed; free it with gtps_free.
+int32_t gtps_vcd_write(const int64_t* off, const int64_t* time, const uint8_t* value, int64_t n_net,
+                       int64_t t_max, const char* timescale, int64_t n_bus, const int64_t* bus_first,
+                       const int32_t* bus_width, char** out, int64_t* out_len) {
+  try {
+    if (!off || !out || !out_len || n_net <= 0 || !timescale) {
+      g_err = "gtps_vcd_write: bad arguments";
+      return GTP_ERR_ARG;
+    }
+    // variables: (first net, width)
+    std::vector<std::pair<int64_t, int32_t>> vars;
+    std::vector<int64_t> var_of(n_net, -1);
+    int64_t b = 0;
+    for (int64_t n = 0; n < n_net;) {
+      if (b < n_bus && bus_first[b] == n) {
+        if (bus_width[b] < 1 || bus_width[b] > 64 || n + bus_width[b] > n_net) {
+          g_err = "gtps_vcd_write: bad bus";
+          return GTP_ERR_ARG;
+        }
+        for (int32_t k = 0; k < bus_width[b]; ++k) var_of[n + k] = (int64_t)vars.size();
+        vars.emplace_back(n, bus_width[b]);
+        n += bus_width[b];
+        ++b;
+      } else {
+        var_of[n] = (int64_t)vars.size();
+        vars.emplace_back(n, 1);
+        ++n;
+      }
+    }
+    auto id_of = [](int64_t i) {
+      std::string s;
+      do {
+        s.push_back((char)(33 + i % 94));
+        i /= 94;
+      } while (i > 0);
+      return s;
+    };
+    static const char vch[4] = {'0', '1', 'x', 'z'};
+    std::string txt;
+    txt.reserve((size_t)(off[n_net] * 9 + vars.size() * 40 + 4096));
+    txt += "$date\n\tsynthetic\n$end\n$version\n\tgtpower_amd synthetic trace\n$end\n$timescale\n\t";
+    txt += timescale;
+    txt += "\n$end\n$scope module top $end\n";
+    for (size_t i = 0; i < vars.size(); ++i) {
+      const int32_t w = vars[i].second;
+      if (w == 1) {
+        txt += "$var wire 1 " + id_of((int64_t)i) + " n" + std::to_string(vars[i].first) + " $end\n";
+      } else {
+        txt += "$var wire " + std::to_string(w) + " " + id_of((int64_t)i) + " bus" +
+               std::to_string(vars[i].first) + " [" + std::to_string(w - 1) + ":0] $end\n";
+      }
+    }
+    txt += "$upscope $end\n$enddefinitions $end\n";
+    // all events by (time, net)
+    const int64_t E = off[n_net];
+    std::vector<uint64_t> key((size_t)E);
+    for (int64_t n = 0; n < n_net; ++n)
+      for (int64_t e = off[n]; e < off[n + 1]; ++e) key[e] = ((uint64_t)time[e] << 32) | (uint64_t)n;
+    std::sort(key.begin(), key.end());
+    std::vector<uint8_t> cur(n_net, 2);
+    std::vector<int64_t> pos(n_net);
+    for (int64_t n = 0; n < n_net; ++n) pos[n] = off[n];
+    std::vector<uint8_t> touched(vars.size(), 0);
+    std::vector<int64_t> tl;
+    char buf[80];
+    int64_t i = 0;
+    bool first_time = true;
+    while (i < E) {
+      const int64_t t = (int64_t)(key[i] >> 32);
+      std::snprintf(buf, sizeof(buf), "#%lld\n", (long long)t);
+      txt += buf;
+      if (first_time) txt += "$dumpvars\n";
+      tl.clear();
+      for (; i < E && (int64_t)(key[i] >> 32) == t; ++i) {
+        const int64_t n = (int64_t)(key[i] & 0xffffffffull);
+        cur[n] = value[pos[n]++];
+        const int64_t v = var_of[n];
+        if (vars[v].second == 1) {
+          txt.push_back(vch[cur[n] & 3]);
+          txt += id_of(v);
+          txt.push_back('\n');
+        } else if (!touched[v]) {
+          touched[v] = 1;
+          tl.push_back(v);
+        }
+      }
+      for (int64_t v : tl) {
+        touched[v] = 0;
+        txt.push_back('b');
+        for (int32_t k = vars[v].second - 1; k >= 0; --k) txt.push_back(vch[cur[vars[v].first + k] & 3]);
+        txt.push_back(' ');
+        txt += id_of(v);
+        txt.push_back('\n');
+      }
+      if (first_time) txt += "$end\n";
+      first_time = false;
+    }
+    std::snprintf(buf, sizeof(buf), "#%lld\n", (long long)t_max);
+    txt += buf;
+    char* m = (char*)std::malloc(txt.size() + 1);
+    if (!m) {
+      g_err = "gtps_vcd_write: out of memory";
+      return GTP_ERR_OOM;
+    }
+    std::memcpy(m, txt.data(), txt.size());
+    m[txt.size()] = 0;
+    *out = m;
+    *out_len = (int64_t)txt.size();
+    return GTP_OK;
+  } catch (const std::exception& e) {
+    g_err = std::string("gtps_vcd_write: ") + e.what();
+    return GTP_ERR_OOM;
+  }
+}
+
+void gtps_free(void* p) { std::free(p); }
 
 }  // extern "C"

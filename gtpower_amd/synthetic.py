@@ -153,6 +153,44 @@ def slice_trace(trace, n_cycle):
                       trace.time_scale, trace.clk_period, T)
 
 
+def _timescale_text(time_scale):
+    for unit, sec in (("ns", 1e-9), ("ps", 1e-12), ("fs", 1e-15)):
+        k = time_scale / sec
+        if abs(k - round(k)) < 1e-9 * max(1.0, k) and 1 <= round(k) <= 1000:
+            return f"{int(round(k))}{unit}"
+    raise ValueError(f"time scale {time_scale} s is not a whole number of fs, ps or ns")
+
+
+def vcd_text(trace, buses=()):
+    """The trace as VCD text (bytes): one scalar variable per net, except the
+    (first net, width) ranges of `buses`, each written as one bus whose bit b
+    is net first + b.  Variables are declared in net order with distinct ids,
+    so stream k of the VCD reader is net k."""
+    lib = abi.load_synth()
+    lib.gtps_vcd_write.argtypes = [abi.p_i64, abi.p_i64, abi.p_u8, C.c_int64, C.c_int64, C.c_char_p,
+                                   C.c_int64, abi.p_i64, abi.p_i32, C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]
+    lib.gtps_vcd_write.restype = C.c_int32
+    lib.gtps_free.argtypes = [C.c_void_p]
+    lib.gtps_free.restype = None
+    off = np.ascontiguousarray(trace.net_event_offset, np.int64)
+    t = np.ascontiguousarray(trace.event_time, np.int64)
+    v = np.ascontiguousarray(trace.event_value, np.uint8)
+    bf = np.ascontiguousarray([b[0] for b in buses], np.int64)
+    bw = np.ascontiguousarray([b[1] for b in buses], np.int32)
+    out = C.c_void_p()
+    n = C.c_int64()
+    st = lib.gtps_vcd_write(abi.ptr(off, C.c_int64), abi.ptr(t, C.c_int64), abi.ptr(v, C.c_uint8), len(off) - 1,
+                            int(trace.t_max), _timescale_text(trace.time_scale).encode(), len(bf),
+                            abi.ptr(bf, C.c_int64) if len(bf) else None,
+                            abi.ptr(bw, C.c_int32) if len(bw) else None, C.byref(out), C.byref(n))
+    if st != 0:
+        raise RuntimeError(lib.gtps_last_error().decode() if hasattr(lib, "gtps_last_error") else "gtps_vcd_write")
+    try:
+        return C.string_at(out.value, n.value)
+    finally:
+        lib.gtps_free(out)
+
+
 class Design:
     """A generated design: netlist + timing + the gtp_design view."""
 

@@ -336,6 +336,52 @@ gtp_status gtp_toggle_features(gtp_ctx* ctx, int32_t n_sel, const int32_t* feats
 gtp_status gtp_mlp_gemm_test(int32_t a_mode, int32_t b_mode, int32_t M, int32_t N, int32_t K,
                              const void* A, int64_t a_bytes, const void* B, int64_t b_bytes, float* C);
 
+/* ---- Device VCD ingest --------------------------------------------------
+ * VCD text -> the per-(id, bit) event store, parsed on the device.  Replaces
+ * the reference's single-threaded reader (VcdReader::read / parseVarValues,
+ * power/VcdReader.cc:80-131, :238-285, with Vcd::varAppendValue /
+ * varAppendBusValue, power/Vcd.cc:203-232) and the host event-store build
+ * that cuts per-bit ranges out of it (power/cuda/CudaPower.cu:1874-2086, value
+ * table :1129-1145).  Same token rules, same dropping of a change equal to
+ * its id's last value, same initial X (scalar) or 0 (bus) at time 0.  One
+ * event stream per (unique id, bit), ids numbered in declaration order, bits
+ * 0..w-1 (w = the widest declaration of the id, at most 64); values 0, 1,
+ * X = 2, Z = 3, U = 2.  Errors where the reference reports a file error, and
+ * when an id's kept times do not increase strictly or reach 2^29 - 1 ticks. */
+typedef struct gtp_vcd gtp_vcd;
+typedef struct {
+  int64_t n_var;        /* declared variables ($var) */
+  int64_t n_id;         /* unique ids */
+  int64_t n_stream;     /* event streams (sum of id widths): the trace's n_net */
+  int64_t n_event;      /* events over all streams */
+  int64_t t_max;        /* the last time of the text (the trace's t_max) */
+  double time_scale;    /* seconds per tick ($timescale) */
+  int64_t n_token;      /* tokens of the value-change section */
+  int64_t n_change;     /* value changes before dropping repeats */
+  int64_t body_bytes;   /* bytes of the value-change section */
+  double device_ms;     /* device time of the parse (text resident on the device) */
+  double total_ms;      /* wall time including the text upload */
+} gtp_vcd_info;
+typedef struct {
+  const char* name;     /* scope path joined by '/', as the reference names it */
+  const char* id;
+  int32_t width;        /* declared width */
+  int32_t id_index;     /* unique id */
+  int64_t first_stream; /* stream of bit 0 of its id */
+} gtp_vcd_var_info;
+gtp_status gtp_vcd_create(int32_t device, gtp_vcd** out);
+void gtp_vcd_destroy(gtp_vcd* v);
+const char* gtp_vcd_last_error(const gtp_vcd* v);
+/* Parse text[0, len) (under 4 GiB); the result stays on the device. */
+gtp_status gtp_vcd_parse(gtp_vcd* v, const char* text, int64_t len, gtp_vcd_info* info);
+/* Declared variable i; the strings live until the next parse. */
+gtp_status gtp_vcd_var(const gtp_vcd* v, int64_t i, gtp_vcd_var_info* out);
+/* Copy the streams out: offsets [n_stream + 1], times and values [n_event]. */
+gtp_status gtp_vcd_events(gtp_vcd* v, int64_t* offsets, int64_t* times, uint8_t* values);
+/* Load a parsed VCD as ctx's trace (device to device; the design's n_net
+ * must equal n_stream): gtp_set_trace for text input. */
+gtp_status gtp_set_trace_vcd(gtp_ctx* ctx, const gtp_vcd* v, double clk_period);
+
 /* Writers for the reference's two result files.
  * gtp_write_cell_power: `cuda_cell_power.txt` (power/cuda/CudaPower.cu:3134-3165),
  *   one row per gate: name, internal, glitch internal, switching, glitch

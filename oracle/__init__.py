@@ -46,6 +46,18 @@ def lib():
         L.oracle_toggle_x2.restype = C.c_int32
         L.oracle_rise_x2.argtypes = [C.c_int32, C.c_int32]
         L.oracle_rise_x2.restype = C.c_int32
+        L.oracle_vcd_parse.argtypes = [C.c_char_p, C.c_int64, C.c_char_p, C.c_int64]
+        L.oracle_vcd_parse.restype = C.c_void_p
+        L.oracle_vcd_info.argtypes = [C.c_void_p, abi.p_i64, C.POINTER(C.c_double)]
+        L.oracle_vcd_info.restype = None
+        for f, r in (("name", C.c_char_p), ("id", C.c_char_p), ("width", C.c_int32), ("stream", C.c_int64)):
+            fn = getattr(L, "oracle_vcd_var_" + f)
+            fn.argtypes = [C.c_void_p, C.c_int64]
+            fn.restype = r
+        L.oracle_vcd_events.argtypes = [C.c_void_p, abi.p_i64, abi.p_i64, abi.p_u8]
+        L.oracle_vcd_events.restype = None
+        L.oracle_vcd_free.argtypes = [C.c_void_p]
+        L.oracle_vcd_free.restype = None
         _LIB = L
     return _LIB
 
@@ -97,3 +109,35 @@ def table_value(library, table, slew, cap):
 
 def cycles_per_thread(n_event, n_cycle, e_target=8, parallelism_floor=512):
     return lib().oracle_cycles_per_thread(n_event, n_cycle, e_target, parallelism_floor)
+
+
+class VcdError(Exception):
+    pass
+
+
+def vcd_parse(text):
+    """Read a VCD text (bytes) as the reference does (oracle/vcd.cpp).
+
+    Returns dict(vars=[(name, id, width, first stream)], offsets [S+1],
+    times [E], values [E], t_max, time_scale, n_id); raises VcdError."""
+    L = lib()
+    buf = bytes(text)
+    err = C.create_string_buffer(512)
+    h = L.oracle_vcd_parse(buf, len(buf), err, 512)
+    if not h:
+        raise VcdError(err.value.decode())
+    try:
+        info = np.zeros(5, np.int64)
+        ts = C.c_double()
+        L.oracle_vcd_info(h, abi.ptr(info, C.c_int64), C.byref(ts))
+        n_var, n_id, S, E, t_max = (int(x) for x in info)
+        vars_ = [(L.oracle_vcd_var_name(h, i).decode(), L.oracle_vcd_var_id(h, i).decode(),
+                  int(L.oracle_vcd_var_width(h, i)), int(L.oracle_vcd_var_stream(h, i))) for i in range(n_var)]
+        off = np.zeros(S + 1, np.int64)
+        times = np.zeros(max(E, 1), np.int64)
+        vals = np.zeros(max(E, 1), np.uint8)
+        L.oracle_vcd_events(h, abi.ptr(off, C.c_int64), abi.ptr(times, C.c_int64), abi.ptr(vals, C.c_uint8))
+    finally:
+        L.oracle_vcd_free(h)
+    return dict(vars=vars_, offsets=off, times=times[:E], values=vals[:E], t_max=t_max,
+                time_scale=ts.value, n_id=n_id)

@@ -1,5 +1,5 @@
 """Build recipe of the oracle (test infrastructure): oracle/liboracle.so from
-oracle/oracle.cpp with g++.  Nothing of the reference is compiled: the
+oracle/oracle.cpp (power path) and oracle/vcd.cpp (VCD reading) with g++.  Nothing of the reference is compiled: the
 reference needs nvcc, Tcl, SWIG, flex/bison, CUDD and Eigen, none of which this
 image has, so oracle/_ref stays empty."""
 import subprocess
@@ -11,7 +11,7 @@ HERE = Path(__file__).resolve().parent
 def build():
     out = HERE / "liboracle.so"
     cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-Wall",
-           str(HERE / "oracle.cpp"), "-o", str(out)]
+           str(HERE / "oracle.cpp"), str(HERE / "vcd.cpp"), "-o", str(out)]
     print("+", " ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     return out
