@@ -109,6 +109,71 @@ def batch_leg(eng, design, w, n_trace, max_cycles, seed):
             "tile_cycles": b[0].schedule.get("tile_cycles"), "gates_per_wg": b[0].schedule.get("gates_per_wg")}
 
 
+def vcd_leg(design, w, n_cycle, steps, cpu):
+    """Device VCD ingest: the netlist's trace over n_cycle cycles written as
+    VCD text (nets that never carry X or Z grouped into 16-bit buses), parsed
+    on the device with the text resident in HBM; the CPU restatement of the
+    reference reader (single-threaded, as the reference reads its stream) on
+    the same text as the baseline."""
+    import numpy as np
+    from gtpower_amd import synthetic as S
+    from gtpower_amd.vcd import VcdText
+
+    tk = dict(w["trace"])
+    tk["n_cycle"] = n_cycle
+    tseed = S.MASTER_SEED ^ (w["seed"] * 0x9E3779B1)
+    t0 = time.perf_counter()
+    tr = design.trace(seed=tseed + 2003, **tk)
+    off = tr.net_event_offset
+    two = np.ones(design.n_net, bool)
+    bad = np.nonzero(tr.event_value >= 2)[0]
+    if len(bad):
+        two[np.unique(np.searchsorted(off, bad, side="right") - 1)] = False
+    buses, n = [], 0
+    while n < design.n_net:
+        m = n
+        while m < design.n_net and two[m] and m - n < 16:
+            m += 1
+        if m - n >= 2:
+            buses.append((n, m - n))
+            n = m
+        else:
+            n += 1
+    text = S.vcd_text(tr, buses=buses)
+    gen_s = time.perf_counter() - t0
+    tr.close()
+    log(f"vcd: {len(text) / 1e6:.1f} MB of VCD text ({n_cycle} cycles, {len(buses)} buses) in {gen_s:.1f} s")
+    with VcdText(device=0) as v:
+        v.parse(text)  # warmup: the reader's scratch memory is allocated once
+        infos = [v.parse(text) for _ in range(max(1, steps))]
+    info = infos[-1]
+    dev_ms = sum(i["device_ms"] for i in infos) / len(infos)
+    tot_ms = sum(i["total_ms"] for i in infos) / len(infos)
+    body = info["body_bytes"]
+    algo = body + 4 * info["n_event"] + 8 * (info["n_stream"] + 1)
+    gbs = algo / (dev_ms * 1e-3) / 1e9
+    log(f"vcd: device parse {dev_ms:.2f} ms ({body / (dev_ms * 1e-3) / 1e9:.1f} GB/s of text, "
+        f"{info['n_event'] / (dev_ms * 1e-3):.3g} events/s), {tot_ms:.1f} ms with the upload")
+    res = {"text_bytes": len(text), "body_bytes": body, "n_token": info["n_token"], "n_change": info["n_change"],
+           "n_event": info["n_event"], "n_stream": info["n_stream"], "cycles": n_cycle,
+           "device_ms": dev_ms, "total_ms_with_upload": tot_ms,
+           "text_gb_per_s": body / (dev_ms * 1e-3) / 1e9, "events_per_s": info["n_event"] / (dev_ms * 1e-3),
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": gbs / HBM_PEAK_GBS,
+                        "note": "algorithmic bytes = text read once + 4 B per event word + 8 B per stream offset"}}
+    if cpu:
+        import oracle
+        t0 = time.perf_counter()
+        ref = oracle.vcd_parse(text)
+        cpu_s = time.perf_counter() - t0
+        assert len(ref["times"]) == info["n_event"]
+        res["cpu_baseline"] = {"value": body / cpu_s / 1e9, "unit": "GB/s of text", "cores": 1, "kind": "port",
+                               "sample": f"the same {len(text) / 1e6:.0f} MB text, CPU restatement of the "
+                                         "reference reader and per-bit store build (single thread)"}
+        log(f"vcd: cpu reader {cpu_s:.2f} s ({body / cpu_s / 1e6:.1f} MB/s)")
+    return res
+
+
 def surrogate_leg(eng, design, exact_cycle_power, w, n_fresh, steps, exact_cps, judge_tile=64):
     """Train on the first 50k exact cycles, infer on a fresh trace, judge 1%."""
     import numpy as np
@@ -186,6 +251,7 @@ def main():
     ap.add_argument("--batch-max-cycles", type=int, default=2000)
     ap.add_argument("--surrogate-cycles", type=int, default=100000)
     ap.add_argument("--judge-tile", type=int, default=64, help="cycles per tile the exact judge recomputes")
+    ap.add_argument("--vcd-cycles", type=int, default=2000, help="cycles of the VCD ingest leg (0: off)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
     args = ap.parse_args()
 
@@ -260,6 +326,10 @@ def main():
         batch = batch_leg(eng, design, w, args.batch_traces, args.batch_max_cycles, S.MASTER_SEED ^ 0xBA7C4)
     eng.close()
 
+    vcd = None
+    if args.vcd_cycles > 0:
+        vcd = vcd_leg(design, w, min(args.vcd_cycles, n_cycle), 3, not args.no_cpu)
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -304,6 +374,7 @@ def main():
         "cpu_baseline": cpu,
         "surrogate": surrogate,
         "batch": batch,
+        "vcd_ingest": vcd,
     }
     print(json.dumps(out), flush=True)
 

@@ -21,7 +21,8 @@
 //                    is not an id (max-scan over token indices)
 //   changes          compacted in text order; ids looked up in a device hash
 //                    table of the header's ids (exact byte compare)
-//   grouping         stable radix sort of changes by id (text order kept)
+//   grouping         stable radix sort of changes by id (text order kept),
+//                    carrying each change's packed time, kind and value
 //   dedupe           a change equal to the previous change of its id is
 //                    dropped (ids mixing scalar and bus values replay the
 //                    reference's last-kept rule sequentially); an id whose
@@ -136,9 +137,24 @@ __global__ __launch_bounds__(TB) void write_starts_kernel(const uint8_t* __restr
 }
 
 // Kind, length and value of every token (power/VcdReader.cc:245-278).
+// Ids of 1 to 3 characters in [33, 126] index a direct table: key =
+// sum of (c_i - 32) * 95^i (digits >= 1, so lengths do not collide).
+constexpr int32_t SHORT_ID_KEYS = 95 * 95 * 95;
+__device__ __forceinline__ int32_t short_id_key(const uint8_t* s, int64_t n) {
+  if (n < 1 || n > 3) return -1;
+  int32_t k = 0, m = 1;
+  for (int64_t i = 0; i < n; ++i) {
+    const uint8_t c = s[i];
+    if (c < 33 || c > 126) return -1;
+    k += (c - 32) * m;
+    m *= 95;
+  }
+  return k;
+}
+
 __global__ void parse_tokens_kernel(const uint8_t* __restrict__ t, int64_t L, const uint32_t* __restrict__ tok,
                                     int64_t K, uint8_t* __restrict__ kind, uint32_t* __restrict__ len,
-                                    int64_t* __restrict__ val) {
+                                    int64_t* __restrict__ val, int32_t* __restrict__ skey) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= K) return;
   const int64_t s = tok[k];
@@ -200,6 +216,7 @@ __global__ void parse_tokens_kernel(const uint8_t* __restrict__ t, int64_t L, co
   kind[k] = kd;
   len[k] = (uint32_t)n;
   val[k] = v;
+  skey[k] = kd == TK_SCALAR ? short_id_key(t + s + 1, n - 1) : -1;  // a scalar's id
 }
 
 // start of the run of bus-looking tokens holding token k (k itself when the
@@ -239,8 +256,9 @@ __device__ __forceinline__ uint64_t fnv1a(const uint8_t* s, int64_t n) {
   return h;
 }
 
-__device__ int32_t lookup_id(const uint8_t* s, int64_t n, const int32_t* tab, uint64_t mask, const uint8_t* idc,
-                             const int64_t* ido) {
+__device__ int32_t lookup_id(const uint8_t* s, int64_t n, int32_t key, const int32_t* direct, const int32_t* tab,
+                             uint64_t mask, const uint8_t* idc, const int64_t* ido) {
+  if (key >= 0) return direct[key];
   uint64_t i = fnv1a(s, n) & mask;
   for (;;) {
     const int32_t u = tab[i];
@@ -261,19 +279,19 @@ struct ChangeArgs {
   const uint8_t* kind;
   const uint32_t* len;
   const int64_t* val;
+  const int32_t* skey;      // short id key of scalar tokens
   const int32_t* ti;        // inclusive max-scan of time tokens
   const uint32_t* chpos;    // exclusive scan of is_change
   const uint32_t* is_change;
   int64_t K;
+  const int32_t* direct;    // [SHORT_ID_KEYS] id of a short key, -1 = none
   const int32_t* tab;
   uint64_t mask;
   const uint8_t* idc;
   const int64_t* ido;
   uint32_t* ch_uid;
-  uint32_t* ch_idx;
-  uint32_t* ch_time;
-  uint64_t* ch_val;
-  uint8_t* ch_bus;
+  uint64_t* ch_pack;   // bus << 63 | time << 32 | (scalar: value letter, bus: change index)
+  uint64_t* ch_val;    // bus value of bus changes
   DevErr* err;
 };
 
@@ -285,44 +303,58 @@ __global__ void emit_changes_kernel(ChangeArgs a) {
   const bool bus = a.kind[k] == TK_BUS;
   const uint8_t* id;
   int64_t n;
+  int32_t key;
   if (bus) {
     id = a.t + a.tok[k + 1];
     n = a.len[k + 1];
+    key = short_id_key(id, n);
   } else {
     id = a.t + a.tok[k] + 1;
     n = (int64_t)a.len[k] - 1;
+    key = a.skey[k];
   }
-  const int32_t u = lookup_id(id, n, a.tab, a.mask, a.idc, a.ido);
+  const int32_t u = lookup_id(id, n, key, a.direct, a.tab, a.mask, a.idc, a.ido);
   if (u < 0) set_err(a.err, GTP_VCD_ERR_ID, k);
   const int32_t tk = a.ti[k];
   const int64_t time = tk >= 0 ? a.val[tk] : 0;
   if (time < 0 || time >= (int64_t)EV_TIME_INF) set_err(a.err, GTP_VCD_ERR_RANGE, k);
   a.ch_uid[c] = u < 0 ? 0u : (uint32_t)u;
-  a.ch_idx[c] = c;
-  a.ch_time[c] = (uint32_t)(time < 0 ? 0 : min(time, (int64_t)EV_TIME_INF));
-  a.ch_val[c] = (uint64_t)a.val[k];
-  a.ch_bus[c] = bus ? 1 : 0;
+  const uint64_t tt = (uint64_t)(time < 0 ? 0 : min(time, (int64_t)EV_TIME_INF));
+  a.ch_pack[c] = ((uint64_t)(bus ? 1 : 0) << 63) | (tt << 32) | (bus ? (uint64_t)c : (uint64_t)a.val[k]);
+  if (bus) a.ch_val[c] = (uint64_t)a.val[k];
+}
+
+// the sorted payload unpacked; bus values are the only gather
+__global__ void unpack_kernel(const uint64_t* __restrict__ s_pack, int64_t n, const uint64_t* __restrict__ ch_val,
+                              uint32_t* __restrict__ s_time, uint64_t* __restrict__ s_val,
+                              uint8_t* __restrict__ s_bus) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint64_t w = s_pack[p];
+  const bool bus = (w >> 63) != 0;
+  s_time[p] = (uint32_t)((w >> 32) & 0x7fffffffu);
+  s_bus[p] = bus ? 1 : 0;
+  s_val[p] = bus ? ch_val[(uint32_t)w] : (w & 0xffffffffu);
 }
 
 // segment bounds per id and the parallel keep rule: a change equal to the
 // previous change of its id is dropped (exact when the id's changes are all
 // scalar or all bus); ids mixing both are flagged for the sequential pass
-__global__ void keep_kernel(const uint32_t* __restrict__ key, const uint32_t* __restrict__ perm, int64_t n,
-                            const uint64_t* __restrict__ ch_val, const uint8_t* __restrict__ ch_bus,
-                            int64_t* __restrict__ seg_start, int64_t* __restrict__ seg_end,
-                            uint32_t* __restrict__ mixed, uint32_t* __restrict__ keep) {
+__global__ void keep_kernel(const uint32_t* __restrict__ key, int64_t n, const uint64_t* __restrict__ s_val,
+                            const uint8_t* __restrict__ s_bus, int64_t* __restrict__ seg_start,
+                            int64_t* __restrict__ seg_end, uint32_t* __restrict__ mixed,
+                            uint32_t* __restrict__ keep) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const uint32_t u = key[p], c = perm[p];
+  const uint32_t u = key[p];
   const bool first = p == 0 || key[p - 1] != u;
   const bool last = p == n - 1 || key[p + 1] != u;
   if (first) seg_start[u] = p;
   if (last) seg_end[u] = p + 1;
   bool k = true;
   if (!first) {
-    const uint32_t cp = perm[p - 1];
-    if (ch_bus[c] != ch_bus[cp]) mixed[u] = 1;
-    k = ch_val[c] != ch_val[cp];
+    if (s_bus[p] != s_bus[p - 1]) mixed[u] = 1;
+    k = s_val[p] != s_val[p - 1];
   }
   keep[p] = k ? 1u : 0u;
 }
@@ -333,15 +365,14 @@ __global__ void keep_kernel(const uint32_t* __restrict__ key, const uint32_t* __
 // scalar) (power/Vcd.cc:203-232), one thread per id
 __global__ void keep_mixed_kernel(const uint32_t* __restrict__ mixed, int64_t n_id,
                                   const int64_t* __restrict__ seg_start, const int64_t* __restrict__ seg_end,
-                                  const uint32_t* __restrict__ perm, const uint64_t* __restrict__ ch_val,
-                                  const uint8_t* __restrict__ ch_bus, uint32_t* __restrict__ keep) {
+                                  const uint64_t* __restrict__ s_val, const uint8_t* __restrict__ s_bus,
+                                  uint32_t* __restrict__ keep) {
   const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= n_id || !mixed[u]) return;
   uint64_t last_char = 0, last_bus = 0;
   for (int64_t p = seg_start[u]; p < seg_end[u]; ++p) {
-    const uint32_t c = perm[p];
-    const bool bus = ch_bus[c] != 0;
-    const uint64_t v = ch_val[c];
+    const bool bus = s_bus[p] != 0;
+    const uint64_t v = s_val[p];
     const bool k = p == seg_start[u] || (bus ? v != last_bus : v != last_char);
     keep[p] = k ? 1u : 0u;
     if (k) {
@@ -352,12 +383,11 @@ __global__ void keep_mixed_kernel(const uint32_t* __restrict__ mixed, int64_t n_
 }
 
 // (id, time + 1) keys of kept changes for the order check
-__global__ void order_key_kernel(const uint32_t* __restrict__ key, const uint32_t* __restrict__ perm, int64_t n,
-                                 const uint32_t* __restrict__ keep, const uint32_t* __restrict__ ch_time,
-                                 uint64_t* __restrict__ out) {
+__global__ void order_key_kernel(const uint32_t* __restrict__ key, int64_t n, const uint32_t* __restrict__ keep,
+                                 const uint32_t* __restrict__ s_time, uint64_t* __restrict__ out) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  out[p] = keep[p] ? (((uint64_t)key[p] << 32) | ((uint64_t)ch_time[perm[p]] + 1)) : 0ull;
+  out[p] = keep[p] ? (((uint64_t)key[p] << 32) | ((uint64_t)s_time[p] + 1)) : 0ull;
 }
 
 // a kept change no later than an earlier kept change of its id
@@ -375,27 +405,26 @@ __global__ void order_check_kernel(const uint32_t* __restrict__ key, int64_t n, 
 // first change is after 0
 __global__ void stream_count_kernel(const int32_t* __restrict__ stream_uid, int64_t S,
                                     const int64_t* __restrict__ seg_start, const int64_t* __restrict__ seg_end,
-                                    const uint32_t* __restrict__ ks, const uint32_t* __restrict__ perm,
-                                    const uint32_t* __restrict__ ch_time, int64_t* __restrict__ cnt) {
+                                    const uint32_t* __restrict__ ks, const uint32_t* __restrict__ s_time,
+                                    int64_t* __restrict__ cnt) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
   const int32_t u = stream_uid[s];
   const int64_t a = seg_start[u];
   int64_t c = 0;
-  if (a >= 0) c = (int64_t)(ks[seg_end[u]] - ks[a]) + (ch_time[perm[a]] != 0 ? 1 : 0);
+  if (a >= 0) c = (int64_t)(ks[seg_end[u]] - ks[a]) + (s_time[a] != 0 ? 1 : 0);
   cnt[s] = c;
 }
 
 struct ScatterArgs {
   const uint32_t* key;
-  const uint32_t* perm;
   int64_t n;
   const uint32_t* keep;
   const uint32_t* ks;
   const int64_t* seg_start;
-  const uint32_t* ch_time;
-  const uint64_t* ch_val;
-  const uint8_t* ch_bus;
+  const uint32_t* s_time;
+  const uint64_t* s_val;
+  const uint8_t* s_bus;
   const int32_t* uid_width;
   const int64_t* stream_base;
   const int64_t* stream_off;
@@ -411,14 +440,13 @@ __device__ __forceinline__ uint32_t code_of(uint64_t ch) {
 __global__ void scatter_kernel(ScatterArgs a) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.n || !a.keep[p]) return;
-  const uint32_t u = a.key[p], c = a.perm[p];
+  const uint32_t u = a.key[p];
   const int64_t s0 = a.seg_start[u];
-  const uint32_t cf = a.perm[s0];
-  const int init = a.ch_time[cf] != 0 ? 1 : 0;
+  const int init = a.s_time[s0] != 0 ? 1 : 0;
   const int64_t rank = (int64_t)(a.ks[p] - a.ks[s0]) + init;
-  const uint32_t tw = a.ch_time[c] << EV_SHIFT;
-  const bool bus = a.ch_bus[c] != 0;
-  const uint64_t v = a.ch_val[c];
+  const uint32_t tw = a.s_time[p] << EV_SHIFT;
+  const bool bus = a.s_bus[p] != 0;
+  const uint64_t v = a.s_val[p];
   const int w = a.uid_width[u];
   const int64_t sb = a.stream_base[u];
   for (int b = 0; b < w; ++b) {
@@ -426,7 +454,7 @@ __global__ void scatter_kernel(ScatterArgs a) {
     a.ev[a.stream_off[sb + b] + rank] = tw | code;
   }
   if (p == s0 && init) {
-    const uint32_t code0 = a.ch_bus[cf] ? 0u : 2u;  // bus 0 or X at time 0
+    const uint32_t code0 = bus ? 0u : 2u;  // bus 0 or X at time 0
     for (int b = 0; b < w; ++b) a.ev[a.stream_off[sb + b]] = code0;
   }
 }
@@ -480,6 +508,15 @@ struct gtp_vcd {
   // result
   DArr<uint32_t> d_ev;              // [n_event + EV_PAD] event words
   DArr<int64_t> d_off;              // [n_stream + 1]
+  // scratch, kept between parses (grown, never shrunk)
+  struct Scratch {
+    DArr<uint8_t> text, idc, kind, s_bus, tmp;
+    DArr<int32_t> tab, direct, stream_uid, uid_width, rs_in, rs, ti_in, ti, skey;
+    DArr<int64_t> ido, stream_base, val, seg_start, seg_end, scnt;
+    DArr<uint64_t> bcnt, boff, ch_val, ch_pack, s_pack, s_val, okey, omax;
+    DArr<uint32_t> tok, len, is_change, chpos, ch_uid, key, s_time, keep, ks, mixed;
+    DArr<unsigned long long> err;
+  } sc;
   std::vector<int64_t> h_off;
   gtp_vcd_info info{};
   bool parsed = false;
@@ -665,6 +702,19 @@ GTP_API void gtp_vcd_destroy(gtp_vcd* v) {
 
 GTP_API const char* gtp_vcd_last_error(const gtp_vcd* v) { return v ? v->error.c_str() : "null handle"; }
 
+namespace {
+template <class T>
+hipError_t grow(DArr<T>& a, size_t n) {
+  return a.n >= n && a.p ? hipSuccess : a.alloc(n);
+}
+template <class T>
+hipError_t put(DArr<T>& a, const T* src, size_t n, hipStream_t s) {
+  hipError_t e = grow(a, std::max<size_t>(n, 1));
+  if (e != hipSuccess || n == 0) return e;
+  return hipMemcpyAsync(a.p, src, n * sizeof(T), hipMemcpyHostToDevice, s);
+}
+}  // namespace
+
 GTP_API gtp_status gtp_vcd_parse(gtp_vcd* v, const char* text, int64_t len, gtp_vcd_info* info) {
   if (!v) return GTP_ERR_ARG;
   if (!text || len < 0) return v->fail(GTP_ERR_ARG, "gtp_vcd_parse: bad text");
@@ -674,29 +724,48 @@ GTP_API gtp_status gtp_vcd_parse(gtp_vcd* v, const char* text, int64_t len, gtp_
   v->error.clear();
   VCHK(v, hipSetDevice(v->device));
   hipStream_t s = v->stream;
+  gtp_vcd::Scratch& S_ = v->sc;
   int64_t b0 = 0;
   {
     const gtp_status st = read_header(v, text, len, &b0);
     if (st != GTP_OK) return st;
   }
   const int64_t U = (int64_t)v->ids.size();
+  if (U >= (int64_t)INT32_MAX) return v->fail(GTP_ERR_LIMIT, "gtp_vcd_parse: too many ids");
   v->stream_base.assign(U + 1, 0);
   for (int64_t u = 0; u < U; ++u) v->stream_base[u + 1] = v->stream_base[u] + v->uid_width[u];
   const int64_t S = v->stream_base[U];
   std::vector<int32_t> stream_uid(std::max<int64_t>(S, 1), 0);
   for (int64_t u = 0; u < U; ++u)
     for (int64_t q = v->stream_base[u]; q < v->stream_base[u + 1]; ++q) stream_uid[q] = (int32_t)u;
-  // id table: open addressing over the unique ids, exact compare on the device
+  // ids of 1-3 printable characters: a direct table; longer ones: open
+  // addressing, exact compare on the device
+  std::vector<int32_t> direct(SHORT_ID_KEYS, -1);
   uint64_t tsz = 16;
   while (tsz < 2 * (uint64_t)std::max<int64_t>(U, 1)) tsz <<= 1;
   std::vector<int32_t> tab(tsz, -1);
   std::vector<int64_t> ido(U + 1, 0);
   std::string idc;
   for (int64_t u = 0; u < U; ++u) {
-    uint64_t i = host_fnv1a(v->ids[u]) & (tsz - 1);
+    const std::string& id = v->ids[u];
+    int32_t key = -1;
+    if (!id.empty() && id.size() <= 3) {
+      key = 0;
+      int32_t m = 1;
+      for (unsigned char c : id) {
+        if (c < 33 || c > 126) {
+          key = -1;
+          break;
+        }
+        key += (c - 32) * m;
+        m *= 95;
+      }
+    }
+    if (key >= 0) direct[key] = (int32_t)u;
+    uint64_t i = host_fnv1a(id) & (tsz - 1);
     while (tab[i] >= 0) i = (i + 1) & (tsz - 1);
     tab[i] = (int32_t)u;
-    idc += v->ids[u];
+    idc += id;
     ido[u + 1] = (int64_t)idc.size();
   }
   // the body's last token is dropped when the text ends inside it
@@ -705,203 +774,193 @@ GTP_API gtp_status gtp_vcd_parse(gtp_vcd* v, const char* text, int64_t len, gtp_
   const auto t0 = std::chrono::steady_clock::now();
   const int64_t b0al = b0 & ~(int64_t)15;
   const int64_t n_blk = std::max<int64_t>(1, (len - b0al + BPB - 1) / BPB);
-  DArr<uint8_t> d_text;
-  VCHK(v, d_text.alloc((size_t)(b0al + n_blk * BPB + 16)));
-  VCHK(v, hipMemsetAsync(d_text.p, ' ', (size_t)(b0al + n_blk * BPB + 16), s));
-  VCHK(v, hipMemcpyAsync(d_text.p, text, (size_t)len, hipMemcpyHostToDevice, s));
-  DArr<int32_t> d_tab, d_stream_uid, d_uid_width;
-  DArr<int64_t> d_ido, d_stream_base;
-  DArr<uint8_t> d_idc;
-  VCHK(v, d_tab.upload(tab.data(), tab.size(), s));
-  VCHK(v, d_ido.upload(ido.data(), ido.size(), s));
-  VCHK(v, d_idc.upload(reinterpret_cast<const uint8_t*>(idc.data()), std::max<size_t>(idc.size(), 1), s));
-  VCHK(v, d_stream_uid.upload(stream_uid.data(), stream_uid.size(), s));
-  VCHK(v, d_uid_width.upload(v->uid_width.empty() ? stream_uid.data() : v->uid_width.data(),
-                             std::max<size_t>(v->uid_width.size(), 1), s));
-  VCHK(v, d_stream_base.upload(v->stream_base.data(), v->stream_base.size(), s));
-  DArr<DevErr> d_err;
-  VCHK(v, d_err.alloc(1));
-  {
-    DevErr e0{~0ull};
-    VCHK(v, hipMemcpyAsync(d_err.p, &e0, sizeof(e0), hipMemcpyHostToDevice, s));
-  }
+  const size_t text_bytes = (size_t)(b0al + n_blk * BPB + 16);
+  VCHK(v, grow(S_.text, text_bytes));
+  VCHK(v, hipMemsetAsync(S_.text.p + len, ' ', text_bytes - (size_t)len, s));
+  VCHK(v, hipMemcpyAsync(S_.text.p, text, (size_t)len, hipMemcpyHostToDevice, s));
+  VCHK(v, put(S_.tab, tab.data(), tab.size(), s));
+  VCHK(v, put(S_.direct, direct.data(), direct.size(), s));
+  VCHK(v, put(S_.ido, ido.data(), ido.size(), s));
+  VCHK(v, put(S_.idc, reinterpret_cast<const uint8_t*>(idc.data()), idc.size(), s));
+  VCHK(v, put(S_.stream_uid, stream_uid.data(), stream_uid.size(), s));
+  VCHK(v, put(S_.uid_width, v->uid_width.data(), v->uid_width.size(), s));
+  VCHK(v, put(S_.stream_base, v->stream_base.data(), v->stream_base.size(), s));
+  const unsigned long long no_err = ~0ull;  // alive until the stream synchronizes below
+  VCHK(v, put(S_.err, &no_err, 1, s));
+  DevErr* d_err = reinterpret_cast<DevErr*>(S_.err.p);
+  // upper bounds of tokens and changes: a token and its whitespace take 2 bytes
+  const int64_t K_max = (len - b0) / 2 + 2;
+  VCHK(v, grow(S_.bcnt, n_blk));
+  VCHK(v, grow(S_.boff, n_blk));
+  VCHK(v, grow(S_.tok, K_max));
+  VCHK(v, grow(S_.kind, K_max));
+  VCHK(v, grow(S_.len, K_max));
+  VCHK(v, grow(S_.val, K_max));
+  VCHK(v, grow(S_.skey, K_max));
+  VCHK(v, grow(S_.rs_in, K_max));
+  VCHK(v, grow(S_.rs, K_max));
+  VCHK(v, grow(S_.ti_in, K_max));
+  VCHK(v, grow(S_.ti, K_max));
+  VCHK(v, grow(S_.is_change, K_max + 1));
+  VCHK(v, grow(S_.chpos, K_max + 1));
+  VCHK(v, grow(S_.seg_start, std::max<int64_t>(U, 1)));
+  VCHK(v, grow(S_.seg_end, std::max<int64_t>(U, 1)));
+  VCHK(v, grow(S_.mixed, std::max<int64_t>(U, 1)));
+  VCHK(v, grow(S_.scnt, S + 1));
+  VCHK(v, v->d_off.alloc(S + 1));
+  auto tmp = [&](size_t need) -> hipError_t { return grow(S_.tmp, std::max<size_t>(need, 1)); };
   hipEvent_t ev0, ev1;
   VCHK(v, hipEventCreate(&ev0));
   VCHK(v, hipEventCreate(&ev1));
+  struct EvGuard {
+    hipEvent_t a, b;
+    ~EvGuard() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } evg{ev0, ev1};
   VCHK(v, hipEventRecord(ev0, s));
 
   // token starts
-  DArr<uint64_t> d_bcnt, d_boff;
-  VCHK(v, d_bcnt.alloc(n_blk));
-  VCHK(v, d_boff.alloc(n_blk));
-  count_starts_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(d_text.p, b0al, b0, len, d_bcnt.p);
+  count_starts_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(S_.text.p, b0al, b0, len, S_.bcnt.p);
   VCHK(v, hipGetLastError());
-  DArr<uint8_t> d_tmp;
-  size_t tmp_bytes = 0;
-  auto tmp = [&](size_t need) -> hipError_t {
-    if (need > tmp_bytes) {
-      hipError_t e = d_tmp.alloc(need);
-      if (e != hipSuccess) return e;
-      tmp_bytes = need;
-    }
-    return hipSuccess;
-  };
   {
     size_t need = 0;
-    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, d_bcnt.p, d_boff.p, n_blk, s));
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, S_.bcnt.p, S_.boff.p, n_blk, s));
     VCHK(v, tmp(need));
-    VCHK(v, hipcub::DeviceScan::ExclusiveSum(d_tmp.p, need, d_bcnt.p, d_boff.p, n_blk, s));
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(S_.tmp.p, need, S_.bcnt.p, S_.boff.p, n_blk, s));
   }
   uint64_t last_off = 0, last_cnt = 0;
-  VCHK(v, hipMemcpyAsync(&last_off, d_boff.p + n_blk - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  VCHK(v, hipMemcpyAsync(&last_cnt, d_bcnt.p + n_blk - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  VCHK(v, hipMemcpyAsync(&last_off, S_.boff.p + n_blk - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  VCHK(v, hipMemcpyAsync(&last_cnt, S_.bcnt.p + n_blk - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   VCHK(v, hipStreamSynchronize(s));
   int64_t K = (int64_t)(last_off + last_cnt);
   if (cut && K > 0) --K;
-  DArr<uint32_t> d_tok;
-  VCHK(v, d_tok.alloc(std::max<int64_t>(K, 1)));
-  write_starts_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(d_text.p, b0al, b0, len, d_boff.p, d_tok.p, K);
+  if (K > K_max) return v->fail(GTP_ERR_DEVICE, "gtp_vcd_parse: token count above its bound");
+  write_starts_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(S_.text.p, b0al, b0, len, S_.boff.p, S_.tok.p, K);
   VCHK(v, hipGetLastError());
 
   // tokens
-  DArr<uint8_t> d_kind;
-  DArr<uint32_t> d_len, d_is_change, d_chpos;
-  DArr<int64_t> d_val;
-  DArr<int32_t> d_rs_in, d_rs, d_ti_in, d_ti;
-  const int64_t K1 = std::max<int64_t>(K, 1);
-  VCHK(v, d_kind.alloc(K1));
-  VCHK(v, d_len.alloc(K1));
-  VCHK(v, d_val.alloc(K1));
-  VCHK(v, d_rs_in.alloc(K1));
-  VCHK(v, d_rs.alloc(K1));
-  VCHK(v, d_ti_in.alloc(K1));
-  VCHK(v, d_ti.alloc(K1));
-  VCHK(v, d_is_change.alloc(K1 + 1));
-  VCHK(v, d_chpos.alloc(K1 + 1));
   int64_t n_change = 0;
   int64_t t_max = 0;
   if (K > 0) {
-    parse_tokens_kernel<<<nblk(K), 256, 0, s>>>(d_text.p, len, d_tok.p, K, d_kind.p, d_len.p, d_val.p);
+    parse_tokens_kernel<<<nblk(K), 256, 0, s>>>(S_.text.p, len, S_.tok.p, K, S_.kind.p, S_.len.p, S_.val.p,
+                                                S_.skey.p);
     VCHK(v, hipGetLastError());
-    run_start_in_kernel<<<nblk(K), 256, 0, s>>>(d_kind.p, K, d_rs_in.p);
+    run_start_in_kernel<<<nblk(K), 256, 0, s>>>(S_.kind.p, K, S_.rs_in.p);
     VCHK(v, hipGetLastError());
     size_t need = 0;
-    VCHK(v, hipcub::DeviceScan::InclusiveScan(nullptr, need, d_rs_in.p, d_rs.p, MaxI32(), K, s));
+    VCHK(v, hipcub::DeviceScan::InclusiveScan(nullptr, need, S_.rs_in.p, S_.rs.p, MaxI32(), K, s));
     VCHK(v, tmp(need));
-    VCHK(v, hipcub::DeviceScan::InclusiveScan(d_tmp.p, need, d_rs_in.p, d_rs.p, MaxI32(), K, s));
-    time_in_kernel<<<nblk(K), 256, 0, s>>>(d_kind.p, d_rs.p, K, d_ti_in.p, d_is_change.p, d_err.p);
+    VCHK(v, hipcub::DeviceScan::InclusiveScan(S_.tmp.p, need, S_.rs_in.p, S_.rs.p, MaxI32(), K, s));
+    time_in_kernel<<<nblk(K), 256, 0, s>>>(S_.kind.p, S_.rs.p, K, S_.ti_in.p, S_.is_change.p, d_err);
     VCHK(v, hipGetLastError());
-    VCHK(v, hipcub::DeviceScan::InclusiveScan(nullptr, need, d_ti_in.p, d_ti.p, MaxI32(), K, s));
+    VCHK(v, hipcub::DeviceScan::InclusiveScan(nullptr, need, S_.ti_in.p, S_.ti.p, MaxI32(), K, s));
     VCHK(v, tmp(need));
-    VCHK(v, hipcub::DeviceScan::InclusiveScan(d_tmp.p, need, d_ti_in.p, d_ti.p, MaxI32(), K, s));
-    fill_u32_kernel<<<1, 1, 0, s>>>(d_is_change.p + K, 1, 0u);
-    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, d_is_change.p, d_chpos.p, K + 1, s));
+    VCHK(v, hipcub::DeviceScan::InclusiveScan(S_.tmp.p, need, S_.ti_in.p, S_.ti.p, MaxI32(), K, s));
+    fill_u32_kernel<<<1, 1, 0, s>>>(S_.is_change.p + K, 1, 0u);
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, S_.is_change.p, S_.chpos.p, K + 1, s));
     VCHK(v, tmp(need));
-    VCHK(v, hipcub::DeviceScan::ExclusiveSum(d_tmp.p, need, d_is_change.p, d_chpos.p, K + 1, s));
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(S_.tmp.p, need, S_.is_change.p, S_.chpos.p, K + 1, s));
     uint32_t nc = 0;
     int32_t tlast = -1;
-    VCHK(v, hipMemcpyAsync(&nc, d_chpos.p + K, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    VCHK(v, hipMemcpyAsync(&tlast, d_ti.p + K - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    VCHK(v, hipMemcpyAsync(&nc, S_.chpos.p + K, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    VCHK(v, hipMemcpyAsync(&tlast, S_.ti.p + K - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     VCHK(v, hipStreamSynchronize(s));
     n_change = nc;
-    if (tlast >= 0) VCHK(v, hipMemcpy(&t_max, d_val.p + tlast, sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (tlast >= 0) {
+      VCHK(v, hipMemcpyAsync(&t_max, S_.val.p + tlast, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+      VCHK(v, hipStreamSynchronize(s));
+    }
   }
 
-  // changes in text order, grouped by id (stable)
+  // changes in text order, grouped by id (stable), payload gathered once
   const int64_t NC1 = std::max<int64_t>(n_change, 1);
-  DArr<uint32_t> d_ch_uid, d_ch_idx, d_key, d_perm, d_ch_time, d_keep, d_ks, d_mixed;
-  DArr<uint64_t> d_ch_val, d_okey, d_omax;
-  DArr<uint8_t> d_ch_bus;
-  DArr<int64_t> d_seg_start, d_seg_end;
-  VCHK(v, d_ch_uid.alloc(NC1));
-  VCHK(v, d_ch_idx.alloc(NC1));
-  VCHK(v, d_key.alloc(NC1));
-  VCHK(v, d_perm.alloc(NC1));
-  VCHK(v, d_ch_time.alloc(NC1));
-  VCHK(v, d_ch_val.alloc(NC1));
-  VCHK(v, d_ch_bus.alloc(NC1));
-  VCHK(v, d_keep.alloc(NC1 + 1));
-  VCHK(v, d_ks.alloc(NC1 + 1));
-  VCHK(v, d_seg_start.alloc(std::max<int64_t>(U, 1)));
-  VCHK(v, d_seg_end.alloc(std::max<int64_t>(U, 1)));
-  VCHK(v, d_mixed.alloc(std::max<int64_t>(U, 1)));
-  fill_i64_kernel<<<nblk(std::max<int64_t>(U, 1)), 256, 0, s>>>(d_seg_start.p, std::max<int64_t>(U, 1), -1);
-  fill_i64_kernel<<<nblk(std::max<int64_t>(U, 1)), 256, 0, s>>>(d_seg_end.p, std::max<int64_t>(U, 1), 0);
-  fill_u32_kernel<<<nblk(std::max<int64_t>(U, 1)), 256, 0, s>>>(d_mixed.p, std::max<int64_t>(U, 1), 0u);
+  VCHK(v, grow(S_.ch_uid, NC1));
+  VCHK(v, grow(S_.key, NC1));
+  VCHK(v, grow(S_.ch_pack, NC1));
+  VCHK(v, grow(S_.s_pack, NC1));
+  VCHK(v, grow(S_.ch_val, NC1));
+  VCHK(v, grow(S_.s_time, NC1));
+  VCHK(v, grow(S_.s_val, NC1));
+  VCHK(v, grow(S_.s_bus, NC1));
+  VCHK(v, grow(S_.keep, NC1 + 1));
+  VCHK(v, grow(S_.ks, NC1 + 1));
+  VCHK(v, grow(S_.okey, NC1));
+  VCHK(v, grow(S_.omax, NC1));
+  const int64_t U1 = std::max<int64_t>(U, 1);
+  fill_i64_kernel<<<nblk(U1), 256, 0, s>>>(S_.seg_start.p, U1, -1);
+  fill_i64_kernel<<<nblk(U1), 256, 0, s>>>(S_.seg_end.p, U1, 0);
+  fill_u32_kernel<<<nblk(U1), 256, 0, s>>>(S_.mixed.p, U1, 0u);
   VCHK(v, hipGetLastError());
   if (n_change > 0) {
-    ChangeArgs ca{d_text.p, d_tok.p, d_kind.p, d_len.p, d_val.p, d_ti.p, d_chpos.p, d_is_change.p, K,
-                  d_tab.p, tsz - 1, d_idc.p, d_ido.p, d_ch_uid.p, d_ch_idx.p, d_ch_time.p, d_ch_val.p,
-                  d_ch_bus.p, d_err.p};
+    ChangeArgs ca{S_.text.p, S_.tok.p, S_.kind.p, S_.len.p, S_.val.p, S_.skey.p, S_.ti.p, S_.chpos.p,
+                  S_.is_change.p, K, S_.direct.p, S_.tab.p, tsz - 1, S_.idc.p, S_.ido.p, S_.ch_uid.p,
+                  S_.ch_pack.p, S_.ch_val.p, d_err};
     emit_changes_kernel<<<nblk(K), 256, 0, s>>>(ca);
     VCHK(v, hipGetLastError());
     int bits = 1;
     while (bits < 32 && (1ll << bits) < U) ++bits;
     size_t need = 0;
-    VCHK(v, hipcub::DeviceRadixSort::SortPairs(nullptr, need, d_ch_uid.p, d_key.p, d_ch_idx.p, d_perm.p,
+    VCHK(v, hipcub::DeviceRadixSort::SortPairs(nullptr, need, S_.ch_uid.p, S_.key.p, S_.ch_pack.p, S_.s_pack.p,
                                                n_change, 0, bits, s));
     VCHK(v, tmp(need));
-    VCHK(v, hipcub::DeviceRadixSort::SortPairs(d_tmp.p, need, d_ch_uid.p, d_key.p, d_ch_idx.p, d_perm.p,
+    VCHK(v, hipcub::DeviceRadixSort::SortPairs(S_.tmp.p, need, S_.ch_uid.p, S_.key.p, S_.ch_pack.p, S_.s_pack.p,
                                                n_change, 0, bits, s));
-    keep_kernel<<<nblk(n_change), 256, 0, s>>>(d_key.p, d_perm.p, n_change, d_ch_val.p, d_ch_bus.p,
-                                               d_seg_start.p, d_seg_end.p, d_mixed.p, d_keep.p);
+    unpack_kernel<<<nblk(n_change), 256, 0, s>>>(S_.s_pack.p, n_change, S_.ch_val.p, S_.s_time.p, S_.s_val.p,
+                                                 S_.s_bus.p);
+    VCHK(v, hipGetLastError());
+    keep_kernel<<<nblk(n_change), 256, 0, s>>>(S_.key.p, n_change, S_.s_val.p, S_.s_bus.p, S_.seg_start.p,
+                                               S_.seg_end.p, S_.mixed.p, S_.keep.p);
     VCHK(v, hipGetLastError());
     if (U > 0) {
-      keep_mixed_kernel<<<nblk(U), 256, 0, s>>>(d_mixed.p, U, d_seg_start.p, d_seg_end.p, d_perm.p, d_ch_val.p,
-                                                d_ch_bus.p, d_keep.p);
+      keep_mixed_kernel<<<nblk(U), 256, 0, s>>>(S_.mixed.p, U, S_.seg_start.p, S_.seg_end.p, S_.s_val.p,
+                                                S_.s_bus.p, S_.keep.p);
       VCHK(v, hipGetLastError());
     }
-    VCHK(v, d_okey.alloc(NC1));
-    VCHK(v, d_omax.alloc(NC1));
-    order_key_kernel<<<nblk(n_change), 256, 0, s>>>(d_key.p, d_perm.p, n_change, d_keep.p, d_ch_time.p, d_okey.p);
+    order_key_kernel<<<nblk(n_change), 256, 0, s>>>(S_.key.p, n_change, S_.keep.p, S_.s_time.p, S_.okey.p);
     VCHK(v, hipGetLastError());
-    VCHK(v, hipcub::DeviceScan::ExclusiveScan(nullptr, need, d_okey.p, d_omax.p, MaxU64(), (uint64_t)0,
+    VCHK(v, hipcub::DeviceScan::ExclusiveScan(nullptr, need, S_.okey.p, S_.omax.p, MaxU64(), (uint64_t)0,
                                               n_change, s));
     VCHK(v, tmp(need));
-    VCHK(v, hipcub::DeviceScan::ExclusiveScan(d_tmp.p, need, d_okey.p, d_omax.p, MaxU64(), (uint64_t)0,
+    VCHK(v, hipcub::DeviceScan::ExclusiveScan(S_.tmp.p, need, S_.okey.p, S_.omax.p, MaxU64(), (uint64_t)0,
                                               n_change, s));
-    order_check_kernel<<<nblk(n_change), 256, 0, s>>>(d_key.p, n_change, d_keep.p, d_okey.p, d_omax.p, d_err.p);
+    order_check_kernel<<<nblk(n_change), 256, 0, s>>>(S_.key.p, n_change, S_.keep.p, S_.okey.p, S_.omax.p, d_err);
     VCHK(v, hipGetLastError());
   }
-  fill_u32_kernel<<<1, 1, 0, s>>>(d_keep.p + n_change, 1, 0u);
+  fill_u32_kernel<<<1, 1, 0, s>>>(S_.keep.p + n_change, 1, 0u);
   {
     size_t need = 0;
-    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, d_keep.p, d_ks.p, n_change + 1, s));
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, S_.keep.p, S_.ks.p, n_change + 1, s));
     VCHK(v, tmp(need));
-    VCHK(v, hipcub::DeviceScan::ExclusiveSum(d_tmp.p, need, d_keep.p, d_ks.p, n_change + 1, s));
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(S_.tmp.p, need, S_.keep.p, S_.ks.p, n_change + 1, s));
   }
   // streams
-  DArr<int64_t> d_scnt;
-  VCHK(v, d_scnt.alloc(S + 1));
-  VCHK(v, v->d_off.alloc(S + 1));
   if (S > 0) {
-    stream_count_kernel<<<nblk(S), 256, 0, s>>>(d_stream_uid.p, S, d_seg_start.p, d_seg_end.p, d_ks.p, d_perm.p,
-                                                d_ch_time.p, d_scnt.p);
+    stream_count_kernel<<<nblk(S), 256, 0, s>>>(S_.stream_uid.p, S, S_.seg_start.p, S_.seg_end.p, S_.ks.p,
+                                                S_.s_time.p, S_.scnt.p);
     VCHK(v, hipGetLastError());
   }
-  fill_i64_kernel<<<1, 1, 0, s>>>(d_scnt.p + S, 1, 0);
+  fill_i64_kernel<<<1, 1, 0, s>>>(S_.scnt.p + S, 1, 0);
   {
     size_t need = 0;
-    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, d_scnt.p, v->d_off.p, S + 1, s));
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, S_.scnt.p, v->d_off.p, S + 1, s));
     VCHK(v, tmp(need));
-    VCHK(v, hipcub::DeviceScan::ExclusiveSum(d_tmp.p, need, d_scnt.p, v->d_off.p, S + 1, s));
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(S_.tmp.p, need, S_.scnt.p, v->d_off.p, S + 1, s));
   }
-  DevErr he{};
+  unsigned long long he = 0;
   int64_t E = 0;
-  VCHK(v, hipMemcpyAsync(&he, d_err.p, sizeof(he), hipMemcpyDeviceToHost, s));
+  VCHK(v, hipMemcpyAsync(&he, S_.err.p, sizeof(he), hipMemcpyDeviceToHost, s));
   VCHK(v, hipMemcpyAsync(&E, v->d_off.p + S, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   VCHK(v, hipStreamSynchronize(s));
-  if (he.packed != ~0ull) {
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
-    return v->fail(GTP_ERR_ARG, std::string("vcd: ") + err_text((int)(he.packed & 0xff)) + " (item " +
-                                    std::to_string(he.packed >> 8) + " of the value-change section)");
-  }
-  VCHK(v, v->d_ev.alloc(E + EV_PAD));
+  if (he != ~0ull)
+    return v->fail(GTP_ERR_ARG, std::string("vcd: ") + err_text((int)(he & 0xff)) + " (item " +
+                                    std::to_string(he >> 8) + " of the value-change section)");
+  VCHK(v, grow(v->d_ev, E + EV_PAD));
   fill_words_kernel<<<1, 64, 0, s>>>(v->d_ev.p + E, EV_PAD, EV_INF);
   if (n_change > 0) {
-    ScatterArgs sa{d_key.p, d_perm.p, n_change, d_keep.p, d_ks.p, d_seg_start.p, d_ch_time.p, d_ch_val.p,
-                   d_ch_bus.p, d_uid_width.p, d_stream_base.p, v->d_off.p, v->d_ev.p};
+    ScatterArgs sa{S_.key.p, n_change, S_.keep.p, S_.ks.p, S_.seg_start.p, S_.s_time.p, S_.s_val.p, S_.s_bus.p,
+                   S_.uid_width.p, S_.stream_base.p, v->d_off.p, v->d_ev.p};
     scatter_kernel<<<nblk(n_change), 256, 0, s>>>(sa);
     VCHK(v, hipGetLastError());
   }
@@ -911,8 +970,6 @@ GTP_API gtp_status gtp_vcd_parse(gtp_vcd* v, const char* text, int64_t len, gtp_
   VCHK(v, hipStreamSynchronize(s));
   float ms = 0;
   VCHK(v, hipEventElapsedTime(&ms, ev0, ev1));
-  (void)hipEventDestroy(ev0);
-  (void)hipEventDestroy(ev1);
   const auto t1 = std::chrono::steady_clock::now();
   v->info = gtp_vcd_info{};
   v->info.n_var = (int64_t)v->vars.size();

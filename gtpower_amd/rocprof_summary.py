@@ -5,7 +5,8 @@
                                           [--out FILE] [--json FILE] [--cmd "..."]
 
 `trace` reads a `rocprofv3 --kernel-trace` database: per-kernel dispatch
-statistics, and the exact engine's evaluation windows -- from the start of a
+statistics, the device VCD parse windows (token count to event scatter), and
+the exact engine's evaluation windows -- from the start of a
 full glitch annotation (glitch_annotate_kernel) to the end of the last power
 kernel (power_cell_kernel / power_tile_kernel) before that run's
 cycle_reduce_kernel -- which is the interval the engine times with HIP events.
@@ -64,9 +65,41 @@ def kernel_table(disp):
     return lines
 
 
+VCD_START = "count_starts_kernel("
+VCD_END = "scatter_kernel("
+
+
+def vcd_windows(disp):
+    """Device VCD parses: from the token count to the event scatter."""
+    out, cur = [], None
+    for d in disp:
+        if VCD_START in d["name"]:
+            cur = [d]
+        elif cur is not None:
+            cur.append(d)
+            if VCD_END in d["name"]:
+                out.append(cur)
+                cur = None
+    return out
+
+
 def trace_summary(db):
     disp = dispatches(db)
     lines = ["per-kernel dispatch statistics (device time)"] + kernel_table(disp)
+    vw = vcd_windows(disp)
+    if vw:
+        dur = [(w[-1]["end"] - w[0]["start"]) / 1e6 for w in vw]
+        busy = [sum(d["end"] - d["start"] for d in w) / 1e6 for w in vw]
+        lines += ["", f"device VCD parse windows (token count start -> event scatter end): {len(vw)}",
+                  "  per window ms: " + " ".join(f"{x:.3f}" for x in dur),
+                  "  kernel busy ms: " + " ".join(f"{x:.3f}" for x in busy),
+                  f"  kernels per window: {len(vw[-1])}"]
+        by = {}
+        for d in vw[-1]:
+            by.setdefault(_short(d["name"]), 0.0)
+            by[_short(d["name"])] += (d["end"] - d["start"]) / 1e3
+        lines.append("  last window by kernel (us): " +
+                     ", ".join(f"{k.split('::')[-1]} {v:.0f}" for k, v in sorted(by.items(), key=lambda kv: -kv[1])))
     ws = windows(disp)
     if ws:
         dur = [(max(d["end"] for d in w) - w[0]["start"]) / 1e6 for w in ws]
