@@ -201,11 +201,17 @@ def leakage_states(when, pin_names):
 
 
 class CellLibrary:
-    """The eight sky130 cells, flattened for the engine and the generator."""
+    """Cells flattened for the engine and the generator: by default the eight
+    sky130 cells of the bundled JSON (sky130 units: ns, pF, nW, pJ); or any
+    library read by `gtpower_amd.liberty.read_liberty` (doc=...), with its own
+    units, table axis order and internal-power scale factors."""
 
-    def __init__(self, json_path=DATA_JSON, cells=None):
-        doc = json.loads(Path(json_path).read_text())
+    def __init__(self, json_path=DATA_JSON, cells=None, doc=None):
+        if doc is None:
+            doc = json.loads(Path(json_path).read_text())
         self.doc = doc
+        u = doc.get("units_si", {"time": NS, "capacitance": PF, "leakage_power": NW, "energy": PJ})
+        self.unit = u
         names = cells or list(doc["cells"].keys())
         self.cell_names = names
         self.full_names = [doc["cells"][n]["name"] for n in names]
@@ -224,13 +230,16 @@ class CellLibrary:
             if n > abi.MAX_CELL_PINS:
                 raise ValueError(f"{short}: {n} pins > {abi.MAX_CELL_PINS}")
             pidx = {p: k for k, p in enumerate(pin_names)}
-            leak_default = c["cell_leakage_power"] * NW if np.isfinite(c["cell_leakage_power"]) else 0.0
+            lu = u["leakage_power"]
+            leak_default = c["cell_leakage_power"] * lu if np.isfinite(c["cell_leakage_power"]) else 0.0
             leak = None
             if c["leakage_power"]:
                 leak = np.full(1 << n, leak_default)
                 for lp in c["leakage_power"]:
                     for s in leakage_states(lp["when"], pin_names):
-                        leak[s] = lp["value"] * NW
+                        leak[s] = lp["value"] * lu
+            # internal power at the operating conditions (liberty/Liberty.cc:280-316)
+            pscale = float(c.get("internal_power_scale", 1.0))
             clock_mask = 0
             for k, p in enumerate(pins):
                 if p["clock"]:
@@ -239,8 +248,8 @@ class CellLibrary:
             for k, p in enumerate(pins):
                 for ip in p["internal_power"]:
                     rel = pidx[ip["related_pin"]] if ip["related_pin"] else -1
-                    rise = self._add_power_table(ip["rise_power"])
-                    fall = self._add_power_table(ip["fall_power"])
+                    rise = self._add_power_table(ip["rise_power"], pscale)
+                    fall = self._add_power_table(ip["fall_power"], pscale)
                     ipowers.append((ci, k, rel, rise, fall, when_truth(ip["when"], pin_names)))
             # timing arcs for the generator: (out_k, in_k, from_rf, to_rf) -> (delay table, slew table)
             arcs = {}
@@ -275,7 +284,7 @@ class CellLibrary:
                         func |= 1 << s
             self.cells.append(dict(
                 short=short, name=c["name"], pins=pin_names, n_pin=n, n_in=len(ins),
-                n_out=len(outs), pin_cap=[p["capacitance"] * PF for p in pins],
+                n_out=len(outs), pin_cap=[p["capacitance"] * u["capacitance"] for p in pins],
                 leak=leak, leak_default=leak_default, clock_mask=clock_mask,
                 cls=abi.CLASS_SEQ if is_ff else abi.CLASS_COMB, arcs=arcs, func=func,
                 is_ff=is_ff,
@@ -288,6 +297,12 @@ class CellLibrary:
     def _add_table(self, t, var1, var2, yscale, xscale1, xscale2):
         if t is None:
             return -1
+        if "vars" in t:
+            # each axis named by its template variable, in its unit
+            var1 = t["vars"][0]
+            var2 = t["vars"][1] if len(t["vars"]) > 1 else abi.AXIS_CAP
+            ax_unit = {abi.AXIS_SLEW: self.unit["time"], abi.AXIS_CAP: self.unit["capacitance"]}
+            xscale1, xscale2 = ax_unit[var1], ax_unit[var2]
         ax1 = np.asarray(t["index_1"], dtype=np.float64) * xscale1
         vals = np.asarray(t["values"], dtype=np.float64) * yscale
         off1 = sum(len(d) for d in self._data)
@@ -304,13 +319,17 @@ class CellLibrary:
         self._tables.append((len(ax1), n2, var1, var2, off1, off2, offv))
         return len(self._tables) - 1
 
-    def _add_power_table(self, t):
-        # power_inputs_1: input_transition_time; power_outputs_1: (transition, total_output_net_capacitance)
-        return self._add_table(t, abi.AXIS_SLEW, abi.AXIS_CAP, PJ, NS, PF)
+    def _add_power_table(self, t, scale=1.0):
+        # sky130 power_inputs_1: input_transition_time; power_outputs_1:
+        # (input_transition_time, total_output_net_capacitance); energies in
+        # voltage_unit^2 * capacitive_load_unit
+        u = self.unit
+        return self._add_table(t, abi.AXIS_SLEW, abi.AXIS_CAP, u["energy"] * scale, u["time"], u["capacitance"])
 
     def _add_timing_table(self, t):
-        # del_1_7_7: input_net_transition x total_output_net_capacitance, values in ns
-        return self._add_table(t, abi.AXIS_SLEW, abi.AXIS_CAP, NS, NS, PF)
+        # sky130 del_1_7_7: input_net_transition x total_output_net_capacitance, values in time_unit
+        u = self.unit
+        return self._add_table(t, abi.AXIS_SLEW, abi.AXIS_CAP, u["time"], u["time"], u["capacitance"])
 
     def _finish(self):
         self.table_data = np.concatenate(self._data) if self._data else np.zeros(1)
