@@ -151,6 +151,7 @@ def load_engine():
     lib.gtp_set_trace.argtypes = [C.c_void_p, C.POINTER(GtpTrace)]
     lib.gtp_run.argtypes = [C.c_void_p, C.POINTER(GtpResult)]
     lib.gtp_run_batch.argtypes = [C.c_void_p, C.c_int32, C.POINTER(GtpTrace), C.POINTER(GtpResult)]
+    lib.gtp_run_sliced.argtypes = [C.c_void_p, C.POINTER(GtpTrace), C.c_int64, C.POINTER(GtpResult)]
     lib.gtp_get_schedule.argtypes = [C.c_void_p, C.POINTER(GtpScheduleInfo)]
     lib.gtp_reference_cycles_per_thread.argtypes = [C.c_int64, p_i64, C.c_int64, C.c_int64,
                                                     C.c_int64, p_i64]
@@ -187,7 +188,7 @@ def load_engine():
     lib.gtp_vcd_var.argtypes = [C.c_void_p, C.c_int64, C.POINTER(GtpVcdVarInfo)]
     lib.gtp_vcd_events.argtypes = [C.c_void_p, p_i64, p_i64, p_u8]
     lib.gtp_set_trace_vcd.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
-    for name in ("gtp_create", "gtp_set_design", "gtp_set_trace", "gtp_run", "gtp_run_batch",
+    for name in ("gtp_create", "gtp_set_design", "gtp_set_trace", "gtp_run", "gtp_run_batch", "gtp_run_sliced",
                  "gtp_get_schedule", "gtp_reference_cycles_per_thread", "gtp_write_cell_power",
                  "gtp_write_per_cycle", "gtp_run_tiles", "gtp_autotune", "gtp_mlp_create", "gtp_mlp_get_params",
                  "gtp_mlp_set_params", "gtp_mlp_gradients", "gtp_mlp_train", "gtp_mlp_predict",

@@ -1198,6 +1198,310 @@ int32_t gtp::trace_dev(gtp_ctx* x, TraceDev* out) {
 // processed in rounds bounded by the partials' memory, each round one
 // persistent launch per cell group over all the round's traces.  Per-trace
 // results are exact integer sums, identical to gtp_run of each trace.
+namespace {
+
+// One unit of a multi-trace run: a host event CSR (offsets relative to its own
+// time/value arrays), the trace it belongs to (t_max, cycles, tiles: the whole
+// trace's, so that the first and last tile keep their meaning), the tiles of
+// it to evaluate and the result slot they accumulate into.
+struct RunItem {
+  const int64_t* off;        // [N + 1], off[0] == 0
+  const int64_t* time;
+  const uint8_t* value;
+  int64_t t_max, n_cycle;
+  int32_t n_tile, tile_begin, tile_end, slot;
+};
+
+// Schedule, fixed-point scales and scratch of a multi-trace run.
+struct MultiPlan {
+  RunPlan rp;
+  Quanta qt;
+  int64_t T = 0, TC = 0;
+  int32_t W_round = 1, RG = 0, block = 256, n_range = 0;
+  int64_t n_sched = 0, total_tiles = 0, ev_budget = 0;
+  float invT = 0;
+  double time_scale = 0, clk_period = 0;
+  std::vector<int64_t> work;
+};
+
+// work[g] = pin events of gate g over the run (orders the gates)
+// cycles[k]: cycles of trace k (sizes the rounds)
+gtp_status multi_setup(gtp_ctx* x, int64_t T, double time_scale, double clk_period, int64_t tmax_max,
+                       const std::vector<int64_t>& cycles, std::vector<int64_t>&& work, MultiPlan* mp) {
+  hipStream_t s = x->stream;
+  const int64_t G = x->G;
+  mp->T = T;
+  mp->time_scale = time_scale;
+  mp->clk_period = clk_period;
+  mp->work = std::move(work);
+  mp->invT = (float)(1.0 / (double)T);
+  {
+    const gtp_status st = ensure_arc_q(x, time_scale);
+    if (st != GTP_OK) return st;
+  }
+  {
+    const gtp_status st = make_quanta(x, tmax_max, &mp->qt);
+    if (st != GTP_OK) return st;
+  }
+  // tile length: the option, else the autotuner's choice on the resident
+  // trace (gtp_autotune), else 128
+  int64_t TC = x->opt.tile_cycles > 0 ? x->opt.tile_cycles : (x->tuned_tc > 0 ? x->tuned_tc : 128);
+  if (T < (1 << 24)) TC = std::min<int64_t>(TC, std::max<int64_t>(1, ((1 << 24) - 1) / T));
+  TC = std::max<int64_t>(1, std::min<int64_t>(TC, 4096));
+  mp->TC = TC;
+  int64_t total_tiles = 0;
+  for (int64_t c : cycles) total_tiles += (c + TC - 1) / TC;
+  mp->total_tiles = total_tiles;
+  // partials per work tile: N_PG int64 + 2 uint32 per gate, bins per range
+  const int64_t n_sched_est = G + 4096LL * 16;
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(x, hipMemGetInfo(&free_b, &total_b));
+  const double per_tile_est = (double)n_sched_est * (N_PG * 8 + 8) + 64.0 * N_BIN * TC * 8;
+  const int64_t budget_scratch = std::min<int64_t>((int64_t)8 << 30, (int64_t)(free_b / 4));
+  int32_t W_round = (int32_t)std::max<int64_t>(1, std::min<int64_t>(std::max<int64_t>(total_tiles, 1),
+                                                                     (int64_t)(budget_scratch / per_tile_est)));
+  if (x->opt.round_tiles > 0) W_round = (int32_t)std::min<int64_t>(x->opt.round_tiles, std::max<int64_t>(total_tiles, 1));
+  int32_t RG = x->opt.gates_per_wg > 0 ? x->opt.gates_per_wg : x->tuned_rg;
+  if (RG <= 0) {
+    const double want = (double)G * std::min<int64_t>(W_round, total_tiles) / 8192.0;
+    RG = 256;
+    while (RG < 4096 && RG * 2 <= want) RG *= 2;
+  }
+  int block = x->opt.block_lanes > 0 ? std::min(256, std::max(64, (x->opt.block_lanes + 63) / 64 * 64)) : 256;
+  block = std::min(block, std::max(64, RG));
+  gtp_ctx::OrderSet& os = x->orders[2];
+  build_order(x, os, RG, mp->work, T);
+  os.order_T = -1;  // rebuilt for every multi-trace run
+  HIPCHK(x, os.d_order.upload(os.order.data(), os.order.size(), s));
+  const int64_t n_sched = (int64_t)os.order.size();
+  const int32_t n_range = os.n_range;
+  HIPCHK(x, os.d_pg.alloc((size_t)N_PG * W_round * n_sched));
+  HIPCHK(x, os.d_pg_tog.alloc((size_t)W_round * n_sched));
+  HIPCHK(x, os.d_pg_glc.alloc((size_t)W_round * n_sched));
+  HIPCHK(x, os.d_cyc_part.alloc((size_t)W_round * n_range * N_BIN * TC));
+  RunPlan& rp = mp->rp;
+  rp.os = &os;
+  fill_args(x, os, mp->qt, TC, T, time_scale, rp.a);
+  rp.a.pg = os.d_pg.p;
+  rp.a.pg_tog = os.d_pg_tog.p;
+  rp.a.pg_glitch = os.d_pg_glc.p;
+  rp.a.cyc_part = os.d_cyc_part.p;
+  rp.TC = TC;
+  rp.RG = RG;
+  rp.n_range = n_range;
+  rp.block = block;
+  rp.n_sched = n_sched;
+  rp.q_e = mp->qt.q_e;
+  rp.q_l = mp->qt.q_l;
+  {
+    const gtp_status st = lds_sizes(x, rp.a, &rp);
+    if (st != GTP_OK) return st;
+  }
+  mp->W_round = W_round;
+  mp->RG = RG;
+  mp->block = block;
+  mp->n_range = n_range;
+  mp->n_sched = n_sched;
+  // device loads under the event budget (event word + glitch width per event)
+  mp->ev_budget = x->opt.max_event_num > 0 ? x->opt.max_event_num : (int64_t)((double)free_b * 0.45 / 8.0);
+  return GTP_OK;
+}
+
+// One device load: the items' events as one CSR of virtual nets (item k's
+// net n is k N + n), annotated, indexed, and their tiles evaluated in rounds
+// of W_round work tiles; per-gate sums add into acc/cnt of each item's slot,
+// per-cycle values go to cyc at slot_base[slot].
+gtp_status run_load(gtp_ctx* x, MultiPlan& mp, const std::vector<RunItem>& items, DArr<unsigned long long>& d_acc,
+                    DArr<unsigned long long>& d_cnt, DArr<double>& d_cyc, const std::vector<int64_t>& slot_base,
+                    float* ms_out) {
+  hipStream_t s = x->stream;
+  const int64_t N = x->N, G = x->G, T = mp.T, TC = mp.TC;
+  const int32_t nt = (int32_t)items.size();
+  RunPlan& rp = mp.rp;
+  gtp_ctx::OrderSet& os = *rp.os;
+  std::vector<int64_t> voff((size_t)nt * N + 1, 0);
+  std::vector<int64_t> E(nt);
+  int64_t E_load = 0;
+  for (int32_t k = 0; k < nt; ++k) {
+    const int64_t* off = items[k].off;
+    for (int64_t n = 0; n < N; ++n) voff[(size_t)k * N + n] = E_load + off[n];
+    E[k] = off[N];
+    E_load += off[N];
+  }
+  voff[(size_t)nt * N] = E_load;
+  DArr<int64_t> d_voff, d_chunk;
+  DArr<uint32_t> d_ev, d_aux;
+  HIPCHK(x, d_voff.upload(voff.data(), voff.size(), s));
+  HIPCHK(x, d_ev.alloc(E_load + EV_PAD));
+  HIPCHK(x, d_aux.alloc(E_load + EV_PAD));
+  {
+    const int64_t CH = 1 << 26;
+    DArr<int64_t> st;
+    DArr<uint8_t> sv;
+    HIPCHK(x, st.alloc(CH));
+    HIPCHK(x, sv.alloc(CH));
+    int64_t base = 0;
+    for (int32_t k = 0; k < nt; ++k) {
+      for (int64_t b0 = 0; b0 < E[k]; b0 += CH) {
+        const int64_t n = std::min(CH, E[k] - b0);
+        HIPCHK(x, hipMemcpyAsync(st.p, items[k].time + b0, n * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        HIPCHK(x, hipMemcpyAsync(sv.p, items[k].value + b0, n, hipMemcpyHostToDevice, s));
+        HIPCHK(x, launch_pack(st.p, sv.p, d_ev.p + base + b0, n, s));
+      }
+      base += E[k];
+    }
+    HIPCHK(x, hipStreamSynchronize(s));
+  }
+  std::vector<int64_t> co((size_t)nt * N + 1, 0);
+  for (size_t v = 0; v < (size_t)nt * N; ++v) co[v + 1] = co[v] + (voff[v + 1] - voff[v] + 255) / 256;
+  const int64_t n_chunk = co[(size_t)nt * N];
+  HIPCHK(x, d_chunk.upload(co.data(), co.size(), s));
+  DArr<int32_t> d_chunk_net;
+  HIPCHK(x, d_chunk_net.alloc(std::max<int64_t>(n_chunk, 1)));
+  HIPCHK(x, launch_chunk_net(d_chunk.p, (int64_t)nt * N, d_chunk_net.p, s));
+  // per-item tile indexes and descriptors; work tiles item-major
+  std::vector<TraceDesc> td(nt);
+  int64_t tidx_total = 0;
+  for (int32_t k = 0; k < nt; ++k) {
+    td[k].net_base = (int64_t)k * N;
+    td[k].tile_idx_base = tidx_total;
+    td[k].t_max = items[k].t_max;
+    td[k].n_cycle = items[k].n_cycle;
+    td[k].n_tile = items[k].n_tile;
+    td[k].pad_ = 0;
+    tidx_total += N * ((int64_t)items[k].n_tile + 1);
+  }
+  DArr<int32_t> d_tidx, d_wt_tile, d_wt_trace, d_wt_begin, d_rslot;
+  DArr<TraceDesc> d_td;
+  HIPCHK(x, d_tidx.alloc(std::max<int64_t>(tidx_total, 1)));
+  HIPCHK(x, d_td.upload(td.data(), td.size(), s));
+  std::vector<int32_t> wt_tile, wt_trace;
+  std::vector<int64_t> out_base(nt);
+  for (int32_t k = 0; k < nt; ++k) {
+    for (int32_t t = items[k].tile_begin; t < items[k].tile_end; ++t) {
+      wt_tile.push_back(t);
+      wt_trace.push_back(k);
+    }
+    out_base[k] = slot_base[items[k].slot];
+  }
+  const int32_t n_wt = (int32_t)wt_tile.size();
+  if (n_wt == 0) {
+    *ms_out = 0.f;
+    return GTP_OK;
+  }
+  HIPCHK(x, d_wt_tile.upload(wt_tile.data(), wt_tile.size(), s));
+  HIPCHK(x, d_wt_trace.upload(wt_trace.data(), wt_trace.size(), s));
+  DArr<int64_t> d_out_base;
+  HIPCHK(x, d_out_base.upload(out_base.data(), out_base.size(), s));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(x, hipEventCreate(&e0));
+  HIPCHK(x, hipEventCreate(&e1));
+  struct EvG {
+    hipEvent_t a, b;
+    ~EvG() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } evg{e0, e1};
+  HIPCHK(x, hipEventRecord(e0, s));
+  for (int32_t k = 0; k < nt; ++k)
+    HIPCHK(x, launch_tile_index(d_voff.p + (int64_t)k * N, d_ev.p, N, items[k].n_tile, TC * T, nullptr, 0,
+                                d_tidx.p + td[k].tile_idx_base, s));
+  HIPCHK(x, launch_glitch_annotate(d_voff.p, d_ev.p, d_chunk.p, d_chunk_net.p, n_chunk, (uint32_t)T, mp.invT,
+                                   d_aux.p, s));
+  rp.a.net_off = d_voff.p;
+  rp.a.ev = d_ev.p;
+  rp.a.aux = d_aux.p;
+  rp.a.tile_idx = d_tidx.p;
+  rp.a.traces = d_td.p;
+  const int32_t n_range = mp.n_range;
+  for (int32_t r0 = 0; r0 < n_wt; r0 += mp.W_round) {
+    const int32_t W = std::min(mp.W_round, n_wt - r0);
+    rp.a.tile_list = d_wt_tile.p + r0;
+    rp.a.wt_trace = d_wt_trace.p + r0;
+    rp.a.n_work_tiles = W;
+    {
+      const gtp_status st = launch_groups(x, rp, true);
+      if (st != GTP_OK) return st;
+    }
+    HIPCHK(x, launch_cycle_reduce_batch(os.d_cyc_part.p, n_range, (int32_t)TC, rp.a.tile_list, rp.a.wt_trace, W,
+                                        d_td.p, d_out_base.p, T, mp.qt.q_e, mp.qt.q_l, 1.0 / mp.clk_period,
+                                        d_cyc.p, s));
+    // the round's items, their contiguous work-tile ranges and result slots
+    std::vector<int32_t> rb, rs;
+    for (int32_t i = 0; i < W; ++i)
+      if (i == 0 || wt_trace[r0 + i] != wt_trace[r0 + i - 1]) {
+        rb.push_back(i);
+        rs.push_back(items[wt_trace[r0 + i]].slot);
+      }
+    rb.push_back(W);
+    HIPCHK(x, d_wt_begin.upload(rb.data(), rb.size(), s));
+    HIPCHK(x, d_rslot.upload(rs.data(), rs.size(), s));
+    HIPCHK(x, launch_gate_accumulate(os.d_pg.p, os.d_pg_tog.p, os.d_pg_glc.p, os.d_order.p, mp.n_sched, W,
+                                     d_wt_begin.p, d_rslot.p, (int32_t)rs.size(), G, (long long*)d_acc.p, d_cnt.p,
+                                     s));
+    // the host vectors of the uploads above are reused next round
+    HIPCHK(x, hipStreamSynchronize(s));
+  }
+  HIPCHK(x, hipEventRecord(e1, s));
+  HIPCHK(x, hipStreamSynchronize(s));
+  float ms = 0.f;
+  HIPCHK(x, hipEventElapsedTime(&ms, e0, e1));
+  *ms_out = ms;
+  return GTP_OK;
+}
+
+// Results of one slot: per-gate power from its exact sums, totals, per-cycle
+// values and counts.
+gtp_status finish_slot(gtp_ctx* x, const MultiPlan& mp, int32_t slot, const DArr<unsigned long long>& d_acc,
+                       const DArr<unsigned long long>& d_cnt, const DArr<double>& d_cyc, int64_t cyc_base,
+                       int64_t t_max, int64_t n_cycle, gtp_result& res) {
+  hipStream_t s = x->stream;
+  const int64_t G = x->G;
+  HIPCHK(x, x->d_gate.alloc((size_t)GTP_N_COMPONENT * G));
+  HIPCHK(x, x->d_total.alloc(5 + 25));
+  HIPCHK(x, x->d_total_part.alloc((size_t)TOTAL_BLOCKS * 30));
+  const double tm = (double)t_max;
+  HIPCHK(x, launch_gate_finalize((const long long*)d_acc.p + (int64_t)slot * N_PG * G, G,
+                                 mp.qt.q_e / (tm * mp.time_scale), mp.qt.q_l / tm, x->d_gate.p, s));
+  HIPCHK(x, launch_total(x->d_gate.p, x->d_gate_class.p, G, x->d_total_part.p, x->d_total.p, s));
+  if (res.gate_power)
+    HIPCHK(x, hipMemcpyAsync(res.gate_power, x->d_gate.p, sizeof(double) * GTP_N_COMPONENT * G,
+                             hipMemcpyDeviceToHost, s));
+  if (res.cycle_power)
+    HIPCHK(x, hipMemcpyAsync(res.cycle_power, d_cyc.p + cyc_base, sizeof(double) * GTP_N_WAVE * n_cycle,
+                             hipMemcpyDeviceToHost, s));
+  if (res.gate_toggle_x2)
+    HIPCHK(x, hipMemcpyAsync(res.gate_toggle_x2, d_cnt.p + (int64_t)slot * 2 * G, sizeof(uint64_t) * G,
+                             hipMemcpyDeviceToHost, s));
+  if (res.gate_glitch)
+    HIPCHK(x, hipMemcpyAsync(res.gate_glitch, d_cnt.p + ((int64_t)slot * 2 + 1) * G, sizeof(uint64_t) * G,
+                             hipMemcpyDeviceToHost, s));
+  double tot[30];
+  HIPCHK(x, hipMemcpyAsync(tot, x->d_total.p, sizeof(tot), hipMemcpyDeviceToHost, s));
+  HIPCHK(x, hipStreamSynchronize(s));
+  if (res.total) std::memcpy(res.total, tot, sizeof(double) * 5);
+  if (res.class_total) std::memcpy(res.class_total, tot + 5, sizeof(double) * 25);
+  res.n_cycle = n_cycle;
+  return GTP_OK;
+}
+
+void record_multi_schedule(gtp_ctx* x, const MultiPlan& mp, int64_t n_event) {
+  gtp_schedule_info& si = x->sched;
+  si.tile_cycles = (int32_t)mp.TC;
+  si.block_lanes = mp.block;
+  si.gates_per_wg = mp.RG;
+  si.n_unit = mp.total_tiles * x->G;
+  si.n_workgroup = mp.total_tiles * mp.n_range;
+  si.split_target = (int32_t)mp.rp.os->launches.size();
+  si.n_tile = mp.total_tiles;
+  si.n_event = n_event;
+  si.n_pin_event = std::accumulate(mp.work.begin(), mp.work.end(), (int64_t)0);
+  si.autotune_ms = 0;
+}
+
+}  // namespace
+
 GTP_API gtp_status gtp_run_batch(gtp_ctx* x, int32_t n_trace, const gtp_trace* traces, gtp_result* results) {
   if (!x) return GTP_ERR_ARG;
   if (!x->have_design) return x->fail(GTP_ERR_STATE, "gtp_run_batch: call gtp_set_design first");
@@ -1220,28 +1524,7 @@ GTP_API gtp_status gtp_run_batch(gtp_ctx* x, int32_t n_trace, const gtp_trace* t
     E[k] = traces[k].net_event_offset[N];
     tmax_max = std::max<int64_t>(tmax_max, traces[k].t_max);
   }
-  {
-    const gtp_status st = ensure_arc_q(x, traces[0].time_scale);
-    if (st != GTP_OK) return st;
-  }
-  Quanta qt;
-  {
-    const gtp_status st = make_quanta(x, tmax_max, &qt);
-    if (st != GTP_OK) return st;
-  }
-  // tile length (common to the batch) and schedule: gates ordered by their
-  // pin events summed over the batch
-  // tile length: the option, else the autotuner's choice on the resident
-  // trace (gtp_autotune), else 128
-  int64_t TC = x->opt.tile_cycles > 0 ? x->opt.tile_cycles : (x->tuned_tc > 0 ? x->tuned_tc : 128);
-  if (T < (1 << 24)) TC = std::min<int64_t>(TC, std::max<int64_t>(1, ((1 << 24) - 1) / T));
-  TC = std::max<int64_t>(1, std::min<int64_t>(TC, 4096));
-  std::vector<int32_t> n_tile(n_trace);
-  int64_t total_tiles = 0;
-  for (int32_t k = 0; k < n_trace; ++k) {
-    n_tile[k] = (int32_t)((C[k] + TC - 1) / TC);
-    total_tiles += n_tile[k];
-  }
+  // gates ordered by their pin events summed over the batch
   std::vector<int64_t> work(G, 0);
   parallel_for(G, x->threads, [&](int64_t g) {
     int64_t w = 0;
@@ -1252,243 +1535,179 @@ GTP_API gtp_status gtp_run_batch(gtp_ctx* x, int32_t n_trace, const gtp_trace* t
     }
     work[g] = w;
   });
-  // partials per work tile: N_PG int64 + 2 uint32 per gate, bins per range
-  const int64_t n_sched_est = G + 4096LL * 16;
-  size_t free_b = 0, total_b = 0;
-  HIPCHK(x, hipMemGetInfo(&free_b, &total_b));
-  const double per_tile_est = (double)n_sched_est * (N_PG * 8 + 8) + 64.0 * N_BIN * TC * 8;
-  const int64_t budget_scratch = std::min<int64_t>((int64_t)8 << 30, (int64_t)(free_b / 4));
-  int32_t W_round = (int32_t)std::max<int64_t>(1, std::min<int64_t>(total_tiles, (int64_t)(budget_scratch / per_tile_est)));
-  if (x->opt.round_tiles > 0) W_round = (int32_t)std::min<int64_t>(x->opt.round_tiles, total_tiles);
-  int32_t RG = x->opt.gates_per_wg > 0 ? x->opt.gates_per_wg : x->tuned_rg;
-  if (RG <= 0) {
-    const double want = (double)G * std::min<int64_t>(W_round, total_tiles) / 8192.0;
-    RG = 256;
-    while (RG < 4096 && RG * 2 <= want) RG *= 2;
-  }
-  int block = x->opt.block_lanes > 0 ? std::min(256, std::max(64, (x->opt.block_lanes + 63) / 64 * 64)) : 256;
-  block = std::min(block, std::max(64, RG));
-  gtp_ctx::OrderSet& os = x->orders[2];
-  build_order(x, os, RG, work, T);
-  os.order_T = -1;  // rebuilt for every batch
-  HIPCHK(x, os.d_order.upload(os.order.data(), os.order.size(), s));
-  const int64_t n_sched = (int64_t)os.order.size();
-  const int32_t n_range = os.n_range;
-  HIPCHK(x, os.d_pg.alloc((size_t)N_PG * W_round * n_sched));
-  HIPCHK(x, os.d_pg_tog.alloc((size_t)W_round * n_sched));
-  HIPCHK(x, os.d_pg_glc.alloc((size_t)W_round * n_sched));
-  HIPCHK(x, os.d_cyc_part.alloc((size_t)W_round * n_range * N_BIN * TC));
-  RunPlan rp;
-  rp.os = &os;
-  fill_args(x, os, qt, TC, T, traces[0].time_scale, rp.a);
-  rp.a.pg = os.d_pg.p;
-  rp.a.pg_tog = os.d_pg_tog.p;
-  rp.a.pg_glitch = os.d_pg_glc.p;
-  rp.a.cyc_part = os.d_cyc_part.p;
-  rp.TC = TC;
-  rp.RG = RG;
-  rp.n_range = n_range;
-  rp.block = block;
-  rp.n_sched = n_sched;
-  rp.q_e = qt.q_e;
-  rp.q_l = qt.q_l;
+  // tile count with the schedule's tile length (fixed by multi_setup)
+  MultiPlan mp;
   {
-    const gtp_status st = lds_sizes(x, rp.a, &rp);
+    const gtp_status st = multi_setup(x, T, traces[0].time_scale, traces[0].clk_period, tmax_max, C,
+                                      std::move(work), &mp);
     if (st != GTP_OK) return st;
   }
-  // device loads under the event budget (event word + glitch width per event)
-  int64_t ev_budget = x->opt.max_event_num > 0 ? x->opt.max_event_num
-                                               : (int64_t)((double)free_b * 0.45 / 8.0);
+  const int64_t TC = mp.TC;
+  std::vector<int32_t> n_tile(n_trace);
+  for (int32_t k = 0; k < n_trace; ++k) n_tile[k] = (int32_t)((C[k] + TC - 1) / TC);
   std::vector<std::pair<int32_t, int32_t>> loads;
   for (int32_t k = 0; k < n_trace;) {
-    if (E[k] > ev_budget)
+    if (E[k] > mp.ev_budget)
       return x->fail(GTP_ERR_OOM, "gtp_run_batch: trace " + std::to_string(k) + " alone exceeds the event budget");
     int64_t e = 0;
     int32_t j = k;
-    while (j < n_trace && e + E[j] <= ev_budget) e += E[j++];
+    while (j < n_trace && e + E[j] <= mp.ev_budget) e += E[j++];
     loads.emplace_back(k, j);
     k = j;
   }
+  // exact per-trace sums and per-cycle values of every trace
+  std::vector<int64_t> slot_base(n_trace + 1, 0);
+  for (int32_t k = 0; k < n_trace; ++k) slot_base[k + 1] = slot_base[k] + (int64_t)GTP_N_WAVE * C[k];
+  DArr<unsigned long long> d_acc, d_cnt;
+  DArr<double> d_cyc;
+  HIPCHK(x, d_acc.alloc((size_t)n_trace * N_PG * G + 1));
+  HIPCHK(x, d_cnt.alloc((size_t)n_trace * 2 * G + 1));
+  HIPCHK(x, d_cyc.alloc(std::max<int64_t>(slot_base[n_trace], 1)));
+  HIPCHK(x, launch_zero_u64(d_acc.p, (int64_t)n_trace * N_PG * G, s));
+  HIPCHK(x, launch_zero_u64(d_cnt.p, (int64_t)n_trace * 2 * G, s));
   float kernel_ms_total = 0.f;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  HIPCHK(x, hipEventCreate(&e0));
-  HIPCHK(x, hipEventCreate(&e1));
-  struct EvG {
-    hipEvent_t a, b;
-    ~EvG() {
-      (void)hipEventDestroy(a);
-      (void)hipEventDestroy(b);
-    }
-  } evg{e0, e1};
-  const float invT = (float)(1.0 / (double)T);
   for (const auto& ld : loads) {
-    const int32_t k0 = ld.first, nt = ld.second - ld.first;
-    // virtual-net CSR of the load
-    std::vector<int64_t> voff((size_t)nt * N + 1, 0);
-    int64_t E_load = 0;
-    for (int32_t k = 0; k < nt; ++k) {
-      const int64_t* off = traces[k0 + k].net_event_offset;
-      for (int64_t n = 0; n < N; ++n) voff[(size_t)k * N + n] = E_load + off[n];
-      E_load += off[N];
-    }
-    voff[(size_t)nt * N] = E_load;
-    DArr<int64_t> d_voff, d_chunk;
-    DArr<uint32_t> d_ev, d_aux;
-    HIPCHK(x, d_voff.upload(voff.data(), voff.size(), s));
-    HIPCHK(x, d_ev.alloc(E_load + EV_PAD));
-    HIPCHK(x, d_aux.alloc(E_load + EV_PAD));
-    {
-      const int64_t CH = 1 << 26;
-      DArr<int64_t> st;
-      DArr<uint8_t> sv;
-      HIPCHK(x, st.alloc(CH));
-      HIPCHK(x, sv.alloc(CH));
-      int64_t base = 0;
-      for (int32_t k = 0; k < nt; ++k) {
-        const gtp_trace& tr = traces[k0 + k];
-        for (int64_t b0 = 0; b0 < E[k0 + k]; b0 += CH) {
-          const int64_t n = std::min(CH, E[k0 + k] - b0);
-          HIPCHK(x, hipMemcpyAsync(st.p, tr.event_time + b0, n * sizeof(int64_t), hipMemcpyHostToDevice, s));
-          HIPCHK(x, hipMemcpyAsync(sv.p, tr.event_value + b0, n, hipMemcpyHostToDevice, s));
-          HIPCHK(x, launch_pack(st.p, sv.p, d_ev.p + base + b0, n, s));
-        }
-        base += E[k0 + k];
-      }
-      HIPCHK(x, hipStreamSynchronize(s));
-    }
-    std::vector<int64_t> co((size_t)nt * N + 1, 0);
-    for (size_t v = 0; v < (size_t)nt * N; ++v) co[v + 1] = co[v] + (voff[v + 1] - voff[v] + 255) / 256;
-    const int64_t n_chunk = co[(size_t)nt * N];
-    HIPCHK(x, d_chunk.upload(co.data(), co.size(), s));
-    DArr<int32_t> d_chunk_net;
-    HIPCHK(x, d_chunk_net.alloc(std::max<int64_t>(n_chunk, 1)));
-    HIPCHK(x, launch_chunk_net(d_chunk.p, (int64_t)nt * N, d_chunk_net.p, s));
-    // per-trace tile indexes and descriptors; work tiles trace-major
-    std::vector<TraceDesc> td(nt);
-    int64_t tidx_total = 0;
-    for (int32_t k = 0; k < nt; ++k) {
-      td[k].net_base = (int64_t)k * N;
-      td[k].tile_idx_base = tidx_total;
-      td[k].t_max = traces[k0 + k].t_max;
-      td[k].n_cycle = C[k0 + k];
-      td[k].n_tile = n_tile[k0 + k];
-      td[k].pad_ = 0;
-      tidx_total += N * ((int64_t)n_tile[k0 + k] + 1);
-    }
-    DArr<int32_t> d_tidx, d_wt_tile, d_wt_trace, d_wt_begin, d_rtrace;
-    DArr<TraceDesc> d_td;
-    HIPCHK(x, d_tidx.alloc(std::max<int64_t>(tidx_total, 1)));
-    HIPCHK(x, d_td.upload(td.data(), td.size(), s));
-    std::vector<int32_t> wt_tile, wt_trace;
-    std::vector<int64_t> out_base(nt + 1, 0);
-    for (int32_t k = 0; k < nt; ++k) {
-      for (int32_t t = 0; t < n_tile[k0 + k]; ++t) {
-        wt_tile.push_back(t);
-        wt_trace.push_back(k);
-      }
-      out_base[k + 1] = out_base[k] + (int64_t)GTP_N_WAVE * C[k0 + k];
-    }
-    HIPCHK(x, d_wt_tile.upload(wt_tile.data(), wt_tile.size(), s));
-    HIPCHK(x, d_wt_trace.upload(wt_trace.data(), wt_trace.size(), s));
-    DArr<int64_t> d_out_base;
-    HIPCHK(x, d_out_base.upload(out_base.data(), out_base.size(), s));
-    DArr<double> d_cyc;
-    HIPCHK(x, d_cyc.alloc(std::max<int64_t>(out_base[nt], 1)));
-    DArr<unsigned long long> d_acc, d_cnt;
-    HIPCHK(x, d_acc.alloc((size_t)nt * N_PG * G + 1));
-    HIPCHK(x, d_cnt.alloc((size_t)nt * 2 * G + 1));
-    HIPCHK(x, launch_zero_u64(d_acc.p, (int64_t)nt * N_PG * G, s));
-    HIPCHK(x, launch_zero_u64(d_cnt.p, (int64_t)nt * 2 * G, s));
-    HIPCHK(x, hipEventRecord(e0, s));
-    for (int32_t k = 0; k < nt; ++k)
-      HIPCHK(x, launch_tile_index(d_voff.p + (int64_t)k * N, d_ev.p, N, n_tile[k0 + k], TC * T, nullptr, 0,
-                                  d_tidx.p + td[k].tile_idx_base, s));
-    HIPCHK(x, launch_glitch_annotate(d_voff.p, d_ev.p, d_chunk.p, d_chunk_net.p, n_chunk, (uint32_t)T, invT,
-                                     d_aux.p, s));
-    rp.a.net_off = d_voff.p;
-    rp.a.ev = d_ev.p;
-    rp.a.aux = d_aux.p;
-    rp.a.tile_idx = d_tidx.p;
-    rp.a.traces = d_td.p;
-    const int32_t n_wt = (int32_t)wt_tile.size();
-    for (int32_t r0 = 0; r0 < n_wt; r0 += W_round) {
-      const int32_t W = std::min(W_round, n_wt - r0);
-      rp.a.tile_list = d_wt_tile.p + r0;
-      rp.a.wt_trace = d_wt_trace.p + r0;
-      rp.a.n_work_tiles = W;
-      {
-        const gtp_status st = launch_groups(x, rp, true);
-        if (st != GTP_OK) return st;
-      }
-      HIPCHK(x, launch_cycle_reduce_batch(os.d_cyc_part.p, n_range, (int32_t)TC, rp.a.tile_list, rp.a.wt_trace, W,
-                                          d_td.p, d_out_base.p, T, qt.q_e, qt.q_l, 1.0 / traces[0].clk_period,
-                                          d_cyc.p, s));
-      // the round's traces and their contiguous work-tile ranges
-      std::vector<int32_t> rb, rt;
-      for (int32_t i = 0; i < W; ++i)
-        if (i == 0 || wt_trace[r0 + i] != wt_trace[r0 + i - 1]) {
-          rb.push_back(i);
-          rt.push_back(wt_trace[r0 + i]);
-        }
-      rb.push_back(W);
-      HIPCHK(x, d_wt_begin.upload(rb.data(), rb.size(), s));
-      HIPCHK(x, d_rtrace.upload(rt.data(), rt.size(), s));
-      HIPCHK(x, launch_gate_accumulate(os.d_pg.p, os.d_pg_tog.p, os.d_pg_glc.p, os.d_order.p, n_sched, W,
-                                       d_wt_begin.p, d_rtrace.p, (int32_t)rt.size(), G, (long long*)d_acc.p, d_cnt.p,
-                                       s));
-      // the host vectors of the uploads above are reused next round
-      HIPCHK(x, hipStreamSynchronize(s));
-    }
-    HIPCHK(x, hipEventRecord(e1, s));
-    HIPCHK(x, hipStreamSynchronize(s));
+    std::vector<RunItem> items;
+    for (int32_t k = ld.first; k < ld.second; ++k)
+      items.push_back(RunItem{traces[k].net_event_offset, traces[k].event_time, traces[k].event_value,
+                              traces[k].t_max, C[k], n_tile[k], 0, n_tile[k], k});
     float ms = 0.f;
-    HIPCHK(x, hipEventElapsedTime(&ms, e0, e1));
+    const gtp_status st = run_load(x, mp, items, d_acc, d_cnt, d_cyc, slot_base, &ms);
+    if (st != GTP_OK) return st;
     kernel_ms_total += ms;
-    // per-trace results
-    HIPCHK(x, x->d_gate.alloc((size_t)GTP_N_COMPONENT * G));
-    HIPCHK(x, x->d_total.alloc(5 + 25));
-    HIPCHK(x, x->d_total_part.alloc((size_t)TOTAL_BLOCKS * 30));
-    for (int32_t k = 0; k < nt; ++k) {
-      gtp_result& res = results[k0 + k];
-      const double tm = (double)traces[k0 + k].t_max;
-      HIPCHK(x, launch_gate_finalize((const long long*)d_acc.p + (int64_t)k * N_PG * G, G,
-                                     qt.q_e / (tm * traces[0].time_scale), qt.q_l / tm, x->d_gate.p, s));
-      HIPCHK(x, launch_total(x->d_gate.p, x->d_gate_class.p, G, x->d_total_part.p, x->d_total.p, s));
-      if (res.gate_power)
-        HIPCHK(x, hipMemcpyAsync(res.gate_power, x->d_gate.p, sizeof(double) * GTP_N_COMPONENT * G,
-                                 hipMemcpyDeviceToHost, s));
-      if (res.cycle_power)
-        HIPCHK(x, hipMemcpyAsync(res.cycle_power, d_cyc.p + out_base[k], sizeof(double) * GTP_N_WAVE * C[k0 + k],
-                                 hipMemcpyDeviceToHost, s));
-      if (res.gate_toggle_x2)
-        HIPCHK(x, hipMemcpyAsync(res.gate_toggle_x2, d_cnt.p + (int64_t)k * 2 * G, sizeof(uint64_t) * G,
-                                 hipMemcpyDeviceToHost, s));
-      if (res.gate_glitch)
-        HIPCHK(x, hipMemcpyAsync(res.gate_glitch, d_cnt.p + ((int64_t)k * 2 + 1) * G, sizeof(uint64_t) * G,
-                                 hipMemcpyDeviceToHost, s));
-      double tot[30];
-      HIPCHK(x, hipMemcpyAsync(tot, x->d_total.p, sizeof(tot), hipMemcpyDeviceToHost, s));
-      HIPCHK(x, hipStreamSynchronize(s));
-      if (res.total) std::memcpy(res.total, tot, sizeof(double) * 5);
-      if (res.class_total) std::memcpy(res.class_total, tot + 5, sizeof(double) * 25);
-      res.n_cycle = C[k0 + k];
-    }
+  }
+  for (int32_t k = 0; k < n_trace; ++k) {
+    const gtp_status st = finish_slot(x, mp, k, d_acc, d_cnt, d_cyc, slot_base[k], traces[k].t_max, C[k], results[k]);
+    if (st != GTP_OK) return st;
   }
   for (int32_t k = 0; k < n_trace; ++k) {
     results[k].kernel_ms = kernel_ms_total;
     results[k].total_ms = kernel_ms_total;
   }
-  gtp_schedule_info& si = x->sched;
-  si.tile_cycles = (int32_t)TC;
-  si.block_lanes = block;
-  si.gates_per_wg = RG;
-  si.n_unit = total_tiles * G;
-  si.n_workgroup = total_tiles * n_range;
-  si.split_target = (int32_t)os.launches.size();
-  si.n_tile = total_tiles;
-  si.n_event = std::accumulate(E.begin(), E.end(), (int64_t)0);
-  si.n_pin_event = std::accumulate(work.begin(), work.end(), (int64_t)0);
-  si.autotune_ms = 0;
+  record_multi_schedule(x, mp, std::accumulate(E.begin(), E.end(), (int64_t)0));
+  return GTP_OK;
+}
+
+GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_events, gtp_result* res) {
+  if (!x) return GTP_ERR_ARG;
+  if (!x->have_design) return x->fail(GTP_ERR_STATE, "gtp_run_sliced: call gtp_set_design first");
+  if (!tr || !res) return x->fail(GTP_ERR_ARG, "gtp_run_sliced: no trace or result");
+  HIPCHK(x, hipSetDevice(x->device));
+  hipStream_t s = x->stream;
+  const int64_t N = x->N, G = x->G;
+  int64_t T = 0;
+  {
+    const gtp_status st = validate_trace(x, tr, &T);
+    if (st != GTP_OK) return st;
+  }
+  const int64_t* off = tr->net_event_offset;
+  const int64_t* tm = tr->event_time;
+  const int64_t C = (tr->t_max + T - 1) / T;
+  const int64_t E = off[N];
+  std::vector<int64_t> work(G, 0);
+  parallel_for(G, x->threads, [&](int64_t g) {
+    int64_t w = 0;
+    for (int64_t q = x->gate_pin_off[g]; q < x->gate_pin_off[g + 1]; ++q) {
+      const int32_t net = x->pin_net[q];
+      if (net >= 0) w += off[net + 1] - off[net];
+    }
+    work[g] = w;
+  });
+  MultiPlan mp;
+  {
+    const gtp_status st = multi_setup(x, T, tr->time_scale, tr->clk_period, tr->t_max, std::vector<int64_t>{C},
+                                      std::move(work), &mp);
+    if (st != GTP_OK) return st;
+  }
+  const int64_t TC = mp.TC;
+  const int32_t n_tile = (int32_t)((C + TC - 1) / TC);
+  const int64_t budget = max_events > 0 ? std::min(max_events, mp.ev_budget) : mp.ev_budget;
+  // The slice of net n for tiles [k0, k1): its events from the one before the
+  // cycle group of the last event before the interval (so that group's
+  // positions and the state before the interval come out as in a whole-trace
+  // run; event 0 keeps index 0 when the slice starts there) to the second
+  // event at or after the interval's end (the flag of the first one).
+  std::vector<int64_t> sb(N), se(N);
+  auto slice_bounds = [&](int32_t k0, int32_t k1) {
+    const int64_t t0 = (int64_t)k0 * TC * T;
+    const int64_t t1 = k1 >= n_tile ? INT64_MAX : (int64_t)k1 * TC * T;
+    int64_t total = 0;
+    std::atomic<int64_t> tot{0};
+    parallel_for(N, x->threads, [&](int64_t n) {
+      const int64_t a = off[n], b = off[n + 1];
+      const int64_t i0 = std::lower_bound(tm + a, tm + b, t0) - tm;
+      const int64_t i1 = t1 == INT64_MAX ? b : std::lower_bound(tm + i0, tm + b, t1) - tm;
+      int64_t st = a;
+      if (i0 - a >= 2) {
+        const int64_t cs = (tm[i0 - 1] / T) * T;
+        int64_t f = i0 - 1;
+        while (f - 1 >= a + 1 && tm[f - 1] >= cs) --f;
+        st = std::min(i0 - 2, f - 1);
+        if (st < a) st = a;
+      }
+      sb[n] = st;
+      se[n] = std::min(b, i1 + 2);
+      tot += se[n] - sb[n];
+    });
+    total = tot.load();
+    return total;
+  };
+  // intervals: whole tiles, halved until the slice fits the budget
+  std::vector<std::pair<int32_t, int32_t>> ivals;
+  {
+    const int32_t guess = (int32_t)std::max<int64_t>(1, std::min<int64_t>(n_tile, (E + budget - 1) / std::max<int64_t>(budget, 1) * 2));
+    const int32_t step0 = (n_tile + guess - 1) / guess;
+    for (int32_t k0 = 0; k0 < n_tile;) {
+      int32_t step = std::max(1, std::min(step0, n_tile - k0));
+      while (true) {
+        const int64_t need = slice_bounds(k0, k0 + step);
+        if (need <= budget) break;
+        if (step == 1)
+          return x->fail(GTP_ERR_OOM, "gtp_run_sliced: one tile of " + std::to_string(TC) + " cycles needs " +
+                                          std::to_string(need) + " events, over the budget");
+        step = (step + 1) / 2;
+      }
+      ivals.emplace_back(k0, k0 + step);
+      k0 += step;
+    }
+  }
+  std::vector<int64_t> slot_base{0, (int64_t)GTP_N_WAVE * C};
+  DArr<unsigned long long> d_acc, d_cnt;
+  DArr<double> d_cyc;
+  HIPCHK(x, d_acc.alloc((size_t)N_PG * G + 1));
+  HIPCHK(x, d_cnt.alloc((size_t)2 * G + 1));
+  HIPCHK(x, d_cyc.alloc(std::max<int64_t>(slot_base[1], 1)));
+  HIPCHK(x, launch_zero_u64(d_acc.p, (int64_t)N_PG * G, s));
+  HIPCHK(x, launch_zero_u64(d_cnt.p, (int64_t)2 * G, s));
+  float kernel_ms_total = 0.f;
+  std::vector<int64_t> so(N + 1);
+  std::vector<int64_t> st_time;
+  std::vector<uint8_t> st_val;
+  for (const auto& iv : ivals) {
+    slice_bounds(iv.first, iv.second);
+    so[0] = 0;
+    for (int64_t n = 0; n < N; ++n) so[n + 1] = so[n] + (se[n] - sb[n]);
+    st_time.resize(std::max<int64_t>(so[N], 1));
+    st_val.resize(std::max<int64_t>(so[N], 1));
+    parallel_for(N, x->threads, [&](int64_t n) {
+      std::memcpy(st_time.data() + so[n], tm + sb[n], sizeof(int64_t) * (se[n] - sb[n]));
+      std::memcpy(st_val.data() + so[n], tr->event_value + sb[n], se[n] - sb[n]);
+    });
+    std::vector<RunItem> items{RunItem{so.data(), st_time.data(), st_val.data(), tr->t_max, C, n_tile, iv.first,
+                                       iv.second, 0}};
+    float ms = 0.f;
+    const gtp_status st = run_load(x, mp, items, d_acc, d_cnt, d_cyc, slot_base, &ms);
+    if (st != GTP_OK) return st;
+    kernel_ms_total += ms;
+  }
+  {
+    const gtp_status st = finish_slot(x, mp, 0, d_acc, d_cnt, d_cyc, 0, tr->t_max, C, *res);
+    if (st != GTP_OK) return st;
+  }
+  res->kernel_ms = kernel_ms_total;
+  res->total_ms = kernel_ms_total;
+  record_multi_schedule(x, mp, E);
+  x->sched.split_target = (int32_t)ivals.size();
   return GTP_OK;
 }
 

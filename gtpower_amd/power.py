@@ -166,6 +166,27 @@ class HipPower:
                                    r.total_ms, sched))
         return res
 
+    def power_sliced(self, trace, max_events=0, want=("gate", "cycle", "counts")):
+        """Evaluate a host trace in cycle intervals whose device event store
+        stays within max_events (gtp_run_sliced): for traces larger than
+        device memory; bit-identical to power() over the whole trace."""
+        G = self.n_gate
+        v = trace.as_abi()
+        Cn = -(-trace.t_max // trace.ticks_per_cycle)
+        gp = np.zeros(5 * G) if "gate" in want else None
+        cp = np.zeros(5 * Cn) if "cycle" in want else None
+        tot, cls = np.zeros(5), np.zeros(25)
+        tg = np.zeros(G, np.uint64) if "counts" in want else None
+        gl = np.zeros(G, np.uint64) if "counts" in want else None
+        r = abi.GtpResult(abi.ptr(gp, C.c_double), abi.ptr(cp, C.c_double), abi.ptr(tot, C.c_double),
+                          abi.ptr(cls, C.c_double), abi.ptr(tg, C.c_uint64), abi.ptr(gl, C.c_uint64), 0, 0.0, 0.0)
+        self._check(self._lib.gtp_run_sliced(self._h, C.byref(v), int(max_events), C.byref(r)), "gtp_run_sliced")
+        si = abi.GtpScheduleInfo()
+        self._check(self._lib.gtp_get_schedule(self._h, C.byref(si)), "gtp_get_schedule")
+        sched = {k: getattr(si, k) for k, _ in abi.GtpScheduleInfo._fields_}
+        return PowerResult(gp.reshape(5, G) if gp is not None else None, cp.reshape(5, Cn) if cp is not None else None,
+                           tot, cls.reshape(5, 5), tg, gl, r.n_cycle, r.kernel_ms, r.total_ms, sched)
+
     @property
     def n_cycle(self):
         T = self.trace.ticks_per_cycle

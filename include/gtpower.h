@@ -231,6 +231,18 @@ gtp_status gtp_run_tiles(gtp_ctx* ctx, int32_t tile_cycles, int32_t n_tiles, con
 gtp_status gtp_run_batch(gtp_ctx* ctx, int32_t n_trace, const gtp_trace* traces,
                          gtp_result* results);
 
+/* Evaluate a trace held in host memory in cycle intervals whose device event
+ * store stays within max_events (<= 0: the device memory budget), for traces
+ * larger than device memory.  Each interval's per-net slice carries the
+ * events before it that the state, the glitch flags and the related-pin
+ * choice at its start depend on, and the two after its end; per-gate sums
+ * are exact integers over intervals, so the result is bit-identical to one
+ * gtp_run over the whole trace.  The resident trace of gtp_set_trace is not
+ * touched.  Replaces the reference's multi-round interval loop
+ * (power/cuda/CudaPower.cu:1261-1314, getTimeIntervals / renewGateWaveform
+ * :1421-1599); the interval count is reported in gtp_schedule_info.split_target. */
+gtp_status gtp_run_sliced(gtp_ctx* ctx, const gtp_trace* trace, int64_t max_events, gtp_result* result);
+
 /* Schedule of the last run: tile length, lanes, workgroups, units. */
 typedef struct {
   int32_t tile_cycles, block_lanes, gates_per_wg, split_target;

@@ -13,6 +13,7 @@ import pytest
 
 import oracle
 from gtpower_amd import synthetic as S
+from gtpower_amd.abi import GtpError
 from gtpower_amd.power import HipPower
 
 pytestmark = pytest.mark.gpu
@@ -211,6 +212,64 @@ def test_batch_equals_individual_runs(opts):
         assert np.array_equal(a.cycle_power, b.cycle_power), k
         assert np.array_equal(a.gate_power, b.gate_power), k
         assert np.array_equal(a.total, b.total), k
+
+
+@pytest.mark.parametrize("parts", [1, 3, 11])
+def test_sliced_equals_full_run(parts):
+    """gtp_run_sliced with an event budget that forces about `parts`
+    intervals gives exactly what one gtp_run over the whole trace gives:
+    counts and every power value bit-identical (the slices carry the events
+    that the state, glitch flags and related-pin choice at their edges need;
+    per-gate sums are exact integers over intervals)."""
+    print(f"[sliced] 2000 gates x 1500 cycles, ~{parts} intervals", flush=True)
+    d = S.make_design(2000, dff_fraction=0.15, clock_ns=2.0, seed=45)
+    tr = d.trace(1500, toggle_p=0.2, seed=46)
+    with HipPower(tile_cycles=64) as eng:
+        eng.set_design(d.library, d)
+        eng.read_power_activities(tr)
+        a = eng.power()
+        b = eng.power_sliced(tr, max_events=(tr.n_event + parts - 1) // parts + 4 * d.n_net if parts > 1 else 0)
+    if parts > 1:
+        assert b.schedule["split_target"] >= min(parts, 2)
+    assert b.n_cycle == a.n_cycle
+    assert np.array_equal(a.gate_toggle_x2, b.gate_toggle_x2)
+    assert np.array_equal(a.gate_glitch, b.gate_glitch)
+    assert np.array_equal(a.cycle_power, b.cycle_power)
+    assert np.array_equal(a.gate_power, b.gate_power)
+    assert np.array_equal(a.total, b.total)
+
+
+def test_sliced_generic_kernel_equals_full_run():
+    """The same with nand2 evaluated by the generic kernel (a `when`
+    condition on its input power), which reads the glitch widths the slice's
+    annotation writes."""
+    from gtpower_amd.library import CellLibrary
+    lib = CellLibrary()
+    c = lib.index("nand2_1")
+    ipa = lib.ipower_array
+    k = [i for i in range(len(ipa)) if ipa[i]["cell"] == c and ipa[i]["pin"] == 0][0]
+    ipa[k]["when_truth"] = sum(1 << s for s in (2, 3, 6, 7))
+    lib.ipowers = [tuple(r) for r in ipa[["cell", "pin", "related_pin", "rise_table", "fall_table", "when_truth"]]]
+    print("[sliced] generic kernel, ~7 intervals", flush=True)
+    d = S.make_design(1200, dff_fraction=0.1, clock_ns=2.0, seed=49, library=lib)
+    tr = d.trace(900, toggle_p=0.3, seed=50)
+    with HipPower(tile_cycles=32) as eng:
+        eng.set_design(lib, d)
+        eng.read_power_activities(tr)
+        a = eng.power()
+        b = eng.power_sliced(tr, max_events=tr.n_event // 7 + 4 * d.n_net)
+    assert b.schedule["split_target"] >= 2
+    for key in ("gate_toggle_x2", "gate_glitch", "cycle_power", "gate_power", "total"):
+        assert np.array_equal(getattr(a, key), getattr(b, key)), key
+
+
+def test_sliced_rejects_a_budget_below_one_tile():
+    d = S.make_design(300, dff_fraction=0.1, clock_ns=2.0, seed=47)
+    tr = d.trace(300, toggle_p=0.2, seed=48)
+    with HipPower(tile_cycles=64) as eng:
+        eng.set_design(d.library, d)
+        with pytest.raises(GtpError):
+            eng.power_sliced(tr, max_events=10)
 
 
 def test_batch_matches_oracle():
