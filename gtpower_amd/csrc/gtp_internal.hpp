@@ -17,6 +17,7 @@
 // sum of (state leakage in quanta) * (ticks in that state).
 #pragma once
 #include <stdint.h>
+#include <stdlib.h>
 
 #include <hip/hip_runtime.h>
 
@@ -65,6 +66,17 @@ constexpr int MAXP_LARGE = 6;                   // GTP_MAX_CELL_PINS
 constexpr int MAX_CELLS_LDS = 64;               // cell types staged in LDS
 constexpr int ARC_FRAC_BITS = 16;               // related-pin comparison grid: 2^-16 tick
 constexpr int32_t ARC_NONE = INT32_MIN;         // arc delays are int32 on that grid (< 2^15 ticks)
+// Grid cap of kernels that stride over per-event work: a dispatch counts its
+// work-items in 32 bits, so blocks x threads must stay below 2^32.
+constexpr int64_t MAX_STRIDE_BLOCKS = 1 << 22;
+// The cap in force: GTP_STRIDE_BLOCKS (1 .. MAX_STRIDE_BLOCKS) lowers it, so
+// that small inputs run the stride loops too; read at every launch.
+inline int64_t stride_blocks(int64_t n_block) {
+  const char* e = getenv("GTP_STRIDE_BLOCKS");
+  const long long v = e ? atoll(e) : 0;
+  const int64_t cap = v > 0 && v < MAX_STRIDE_BLOCKS ? (int64_t)v : MAX_STRIDE_BLOCKS;
+  return n_block < cap ? n_block : cap;
+}
 
 // Per-cycle accumulator components (LDS bins and partials).
 enum { BIN_INT = 0, BIN_GINT = 1, BIN_SW = 2, BIN_GSW = 3, BIN_LD = 4, BIN_LW = 5, N_BIN = 6 };

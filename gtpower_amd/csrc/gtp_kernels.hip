@@ -636,9 +636,9 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
 // Glitch filtering, once per net event (power/TimeBasedPower.cc:589-637 and
 // :303-339): aux[e] = the pulse width (ticks) that scales event e when its
 // clock-cycle glitch flag is set, else 0.  A flagged event always has a flagged
-// neighbour in its own cycle group, so a flagged event has aux > 0.  One block
-// takes 256 consecutive events of one net; chunk_off[n] is the first block of
-// net n.
+// neighbour in its own cycle group, so a flagged event has aux > 0.  A chunk
+// is 256 consecutive events of one net (chunk_off[n] is the first chunk of net
+// n); blocks take chunks in a grid stride.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int group_pos(const uint32_t* ev, int64_t i, uint32_t cstart) {
   // number of events before i in the same cycle group (event 0 is never grouped)
@@ -698,22 +698,25 @@ __global__ __launch_bounds__(256) void glitch_annotate_kernel(const int64_t* __r
                                                               uint32_t* __restrict__ ev,
                                                               const int64_t* __restrict__ chunk_off,
                                                               const int32_t* __restrict__ chunk_net,
-                                                              uint32_t T, float invT,
+                                                              int64_t n_chunk, uint32_t T, float invT,
                                                               uint32_t* __restrict__ aux) {
-  const int64_t chunk = blockIdx.x;
-  const int64_t n = chunk_net[chunk];
-  const int64_t b = net_off[n];
-  const int64_t cnt = net_off[n + 1] - b;
-  const int64_t i = (chunk - chunk_off[n]) * 256 + threadIdx.x;
-  if (i >= cnt) return;
-  // The width is read only for flagged events, so only nonzero widths are
-  // written; the word is rewritten only when its flag changes (other lanes
-  // read its time meanwhile: the time bits do not change).
-  const uint32_t pw = annotate_one(ev + b, i, cnt, T, invT);
-  if (pw) aux[b + i] = pw;
-  const uint32_t w = ev[b + i];
-  const uint32_t nw = pw ? (w | EV_GLITCH) : (w & ~EV_GLITCH);
-  if (nw != w) ev[b + i] = nw;
+  // blocks stride over the chunks: one block per chunk would exceed the 2^32
+  // work-items of a dispatch above about 4.3e9 events
+  for (int64_t chunk = blockIdx.x; chunk < n_chunk; chunk += gridDim.x) {
+    const int64_t n = chunk_net[chunk];
+    const int64_t b = net_off[n];
+    const int64_t cnt = net_off[n + 1] - b;
+    const int64_t i = (chunk - chunk_off[n]) * 256 + threadIdx.x;
+    if (i >= cnt) continue;
+    // The width is read only for flagged events, so only nonzero widths are
+    // written; the word is rewritten only when its flag changes (other lanes
+    // read its time meanwhile: the time bits do not change).
+    const uint32_t pw = annotate_one(ev + b, i, cnt, T, invT);
+    if (pw) aux[b + i] = pw;
+    const uint32_t w = ev[b + i];
+    const uint32_t nw = pw ? (w | EV_GLITCH) : (w & ~EV_GLITCH);
+    if (nw != w) ev[b + i] = nw;
+  }
 }
 
 // The same for the events of listed tiles only: the 8 threads of group
@@ -1476,8 +1479,9 @@ hipError_t launch_glitch_annotate(const int64_t* net_off, uint32_t* ev, const in
                                   const int32_t* chunk_net, int64_t n_chunk, uint32_t T, float invT, uint32_t* aux,
                                   hipStream_t s) {
   if (n_chunk <= 0) return hipSuccess;
-  glitch_annotate_kernel<<<dim3((unsigned)n_chunk), dim3(256), 0, s>>>(net_off, ev, chunk_off, chunk_net, T, invT,
-                                                                      aux);
+  const int64_t blocks = stride_blocks(n_chunk);
+  glitch_annotate_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(net_off, ev, chunk_off, chunk_net, n_chunk, T,
+                                                                     invT, aux);
   return hipGetLastError();
 }
 

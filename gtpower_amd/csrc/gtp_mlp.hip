@@ -438,8 +438,8 @@ __global__ __launch_bounds__(256) void feature_bits_kernel(const int64_t* __rest
 // Per net over the cycles with mask[c] != 0 (all when mask is null): the
 // number of cycles with a value change (kind 0) and with a change to 1 (kind
 // 1), and the sums of the quantised target over those cycles (int64: exact
-// and order-free); outputs are [2][n_net].  One block per 256 events
-// (chunk_off as for the glitch annotation).
+// and order-free); outputs are [2][n_net].  One block-iteration per 256
+// events (chunk_off as for the glitch annotation).
 __global__ __launch_bounds__(256) void net_corr_kernel(const int64_t* __restrict__ net_off,
                                                        const uint32_t* __restrict__ ev,
                                                        const int64_t* __restrict__ chunk_off,
@@ -448,48 +448,52 @@ __global__ __launch_bounds__(256) void net_corr_kernel(const int64_t* __restrict
                                                        const uint8_t* __restrict__ mask,
                                                        const long long* __restrict__ yq,
                                                        unsigned long long* __restrict__ cnt_out,
-                                                       unsigned long long* __restrict__ sum_out) {
-  const int64_t chunk = blockIdx.x;
-  const int64_t n = chunk_net[chunk];
-  const int64_t b = net_off[n], cnt = net_off[n + 1] - b;
-  const int64_t i = (chunk - chunk_off[n]) * 256 + threadIdx.x;
-  bool take = false, take_rise = false;
-  int64_t c = 0;
-  if (i >= 1 && i < cnt) {
-    const uint32_t w = ev[b + i];
-    c = cycle_of_tick(w >> EV_SHIFT, T, invT);
-    if (c < n_cyc && (!mask || mask[c])) {
-      take = i == 1 || cycle_of_tick(ev[b + i - 1] >> EV_SHIFT, T, invT) != c;
-      if ((w & 3u) == 1u) {
-        // the first change to 1 of this net in cycle c
-        take_rise = true;
-        for (int64_t j = i - 1; j >= 1; --j) {
-          const uint32_t wj = ev[b + j];
-          if (cycle_of_tick(wj >> EV_SHIFT, T, invT) != c) break;
-          if ((wj & 3u) == 1u) {
-            take_rise = false;
-            break;
+                                                       unsigned long long* __restrict__ sum_out, int64_t n_chunk) {
+  // blocks stride over the chunks (a dispatch counts work-items in 32 bits);
+  // every lane of a block runs the same iterations, so the wave sums below
+  // see all 64 lanes
+  for (int64_t chunk = blockIdx.x; chunk < n_chunk; chunk += gridDim.x) {
+    const int64_t n = chunk_net[chunk];
+    const int64_t b = net_off[n], cnt = net_off[n + 1] - b;
+    const int64_t i = (chunk - chunk_off[n]) * 256 + threadIdx.x;
+    bool take = false, take_rise = false;
+    int64_t c = 0;
+    if (i >= 1 && i < cnt) {
+      const uint32_t w = ev[b + i];
+      c = cycle_of_tick(w >> EV_SHIFT, T, invT);
+      if (c < n_cyc && (!mask || mask[c])) {
+        take = i == 1 || cycle_of_tick(ev[b + i - 1] >> EV_SHIFT, T, invT) != c;
+        if ((w & 3u) == 1u) {
+          // the first change to 1 of this net in cycle c
+          take_rise = true;
+          for (int64_t j = i - 1; j >= 1; --j) {
+            const uint32_t wj = ev[b + j];
+            if (cycle_of_tick(wj >> EV_SHIFT, T, invT) != c) break;
+            if ((wj & 3u) == 1u) {
+              take_rise = false;
+              break;
+            }
           }
         }
       }
     }
-  }
-  unsigned long long k = take ? 1ull : 0ull, kr = take_rise ? 1ull : 0ull;
-  long long s = take ? yq[c] : 0, sr = take_rise ? yq[c] : 0;
-  for (int off = 32; off > 0; off >>= 1) {
-    k += __shfl_xor(k, off);
-    s += __shfl_xor(s, off);
-    kr += __shfl_xor(kr, off);
-    sr += __shfl_xor(sr, off);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    if (k) {
-      atomicAdd(&cnt_out[n], k);
-      atomicAdd(&sum_out[n], (unsigned long long)s);
+    unsigned long long k = take ? 1ull : 0ull, kr = take_rise ? 1ull : 0ull;
+    long long s = take ? yq[c] : 0, sr = take_rise ? yq[c] : 0;
+    for (int off = 32; off > 0; off >>= 1) {
+      k += __shfl_xor(k, off);
+      s += __shfl_xor(s, off);
+      kr += __shfl_xor(kr, off);
+      sr += __shfl_xor(sr, off);
     }
-    if (kr) {
-      atomicAdd(&cnt_out[n_net + n], kr);
-      atomicAdd(&sum_out[n_net + n], (unsigned long long)sr);
+    if ((threadIdx.x & 63) == 0) {
+      if (k) {
+        atomicAdd(&cnt_out[n], k);
+        atomicAdd(&sum_out[n], (unsigned long long)s);
+      }
+      if (kr) {
+        atomicAdd(&cnt_out[n_net + n], kr);
+        atomicAdd(&sum_out[n_net + n], (unsigned long long)sr);
+      }
     }
   }
 }
@@ -970,10 +974,9 @@ GTP_API gtp_status gtp_feature_correlation(gtp_ctx* ctx, const double* y, const 
     fill_u32_kernel<<<nblk(4 * N), 256, 0, t.stream>>>((uint32_t*)d_cnt, 4 * N, 0u);
     fill_u32_kernel<<<nblk(4 * N), 256, 0, t.stream>>>((uint32_t*)d_sum, 4 * N, 0u);
     if (t.n_chunk > 0)
-      net_corr_kernel<<<(unsigned)t.n_chunk, 256, 0, t.stream>>>(t.net_off, t.ev, t.chunk_off, t.chunk_net, N,
-                                                                 (uint32_t)t.T,
-                                                                 (float)(1.0 / (double)t.T), C, d_mask, d_yq,
-                                                                 d_cnt, d_sum);
+      net_corr_kernel<<<(unsigned)stride_blocks(t.n_chunk), 256, 0, t.stream>>>(
+          t.net_off, t.ev, t.chunk_off, t.chunk_net, N, (uint32_t)t.T, (float)(1.0 / (double)t.T), C, d_mask, d_yq,
+          d_cnt, d_sum, t.n_chunk);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), d_cnt, (size_t)2 * N * 8, hipMemcpyDeviceToHost, t.stream);
