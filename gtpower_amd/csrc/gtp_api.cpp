@@ -107,14 +107,13 @@ struct gtp_ctx {
   std::vector<int64_t> gate_work;
   DArr<int64_t> d_net_off;
   DArr<uint32_t> d_ev;
-  DArr<uint32_t> d_aux;          // glitch pulse width per event
   DArr<int64_t> d_chunk_off;     // annotation blocks per net (prefix)
   DArr<int32_t> d_chunk_net;     // net of each annotation block
   int64_t n_chunk = 0;
   DArr<int32_t> d_tile_idx;
   int32_t tile_idx_tc = 0;
   std::vector<int64_t> h_net_off;  // host copy of the event offsets
-  bool aux_valid = false;          // d_aux holds the annotation of this trace
+  bool glitch_valid = false;       // the EV_GLITCH bits hold the annotation of this trace
   DArr<int32_t> d_tile_list;
   DArr<int32_t> d_tile_idx_sub;    // tile index of a subset run's tiles
 
@@ -603,7 +602,7 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
   hipStream_t s = x->stream;
   HIPCHK(x, x->d_net_off.upload(off, N + 1, s));
   x->h_net_off.assign(off, off + N + 1);
-  x->aux_valid = false;
+  x->glitch_valid = false;
   HIPCHK(x, x->d_ev.alloc(E + EV_PAD));
   // chunked upload + pack into the 32-bit event store
   {
@@ -641,7 +640,6 @@ gtp_status finish_trace(gtp_ctx* x) {
     HIPCHK(x, x->d_chunk_off.upload(co.data(), co.size(), s));
     HIPCHK(x, x->d_chunk_net.alloc(std::max<int64_t>(x->n_chunk, 1)));
     HIPCHK(x, launch_chunk_net(x->d_chunk_off.p, N, x->d_chunk_net.p, s));
-    HIPCHK(x, x->d_aux.alloc(E + EV_PAD));
     HIPCHK(x, hipStreamSynchronize(s));
   }
   {
@@ -698,7 +696,7 @@ GTP_API gtp_status gtp_set_trace_vcd(gtp_ctx* x, const gtp_vcd* v, double clk_pe
   x->scale = vw.time_scale;
   x->period = clk_period;
   x->E = E;
-  x->aux_valid = false;
+  x->glitch_valid = false;
   HIPCHK(x, x->d_net_off.alloc(N + 1));
   HIPCHK(x, hipMemcpyAsync(x->d_net_off.p, vw.d_off, sizeof(int64_t) * (N + 1), hipMemcpyDeviceToDevice, s));
   x->h_net_off.assign(vw.h_off, vw.h_off + N + 1);
@@ -931,7 +929,6 @@ gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, int32_t n_work_tiles, RunPlan
   fill_args(x, os, qt, TC, x->T, x->scale, a);
   a.net_off = x->d_net_off.p;
   a.ev = x->d_ev.p;
-  a.aux = x->d_aux.p;
   a.tile_idx = x->d_tile_idx.p;
   a.n_tile = n_tile;
   a.t_max = x->t_max;
@@ -1084,8 +1081,8 @@ gtp_status run_once(gtp_ctx* x, gtp_result* res, int64_t tc_req, int32_t rg_req)
   const PowerArgs& a = rp.a;
   HIPCHK(x, hipEventRecord(ev.e[1], s));
   HIPCHK(x, launch_glitch_annotate(x->d_net_off.p, x->d_ev.p, x->d_chunk_off.p, x->d_chunk_net.p, x->n_chunk,
-                                   (uint32_t)x->T, a.inv_T, x->d_aux.p, s));
-  x->aux_valid = true;
+                                   (uint32_t)x->T, a.inv_T, s));
+  x->glitch_valid = true;
   if ((st = launch_groups(x, rp)) != GTP_OK) return st;
   HIPCHK(x, hipEventRecord(ev.e[2], s));
   HIPCHK(x, launch_cycle_reduce(rp.os->d_cyc_part.p, rp.n_range, (int32_t)rp.TC, x->C, x->T, x->t_max, rp.q_e, rp.q_l,
@@ -1153,9 +1150,9 @@ GTP_API gtp_status gtp_run_tiles(gtp_ctx* x, int32_t tile_cycles, int32_t n_list
   HIPCHK(x, launch_tile_index(x->d_net_off.p, x->d_ev.p, x->N, rp.n_tile, rp.TC * x->T, x->d_tile_list.p, n_list,
                               x->d_tile_idx_sub.p, s));
   // glitch annotation of the listed tiles' events, unless the whole trace is annotated
-  if (!x->aux_valid)
+  if (!x->glitch_valid)
     HIPCHK(x, launch_glitch_annotate_tiles(x->d_net_off.p, x->d_ev.p, x->d_tile_idx_sub.p, x->d_tile_list.p, n_list,
-                                           x->N, rp.TC * x->T, (uint32_t)x->T, rp.a.inv_T, x->d_aux.p, s));
+                                           x->N, rp.TC * x->T, (uint32_t)x->T, rp.a.inv_T, s));
   rp.a.tile_idx = x->d_tile_idx_sub.p;
   rp.a.tile_list = x->d_tile_list.p;
   rp.a.n_work_tiles = n_list;
@@ -1302,8 +1299,8 @@ gtp_status multi_setup(gtp_ctx* x, int64_t T, double time_scale, double clk_peri
   mp->block = block;
   mp->n_range = n_range;
   mp->n_sched = n_sched;
-  // device loads under the event budget (event word + glitch width per event)
-  mp->ev_budget = x->opt.max_event_num > 0 ? x->opt.max_event_num : (int64_t)((double)free_b * 0.45 / 8.0);
+  // device loads under the event budget (one 32-bit word per event)
+  mp->ev_budget = x->opt.max_event_num > 0 ? x->opt.max_event_num : (int64_t)((double)free_b * 0.45 / 4.0);
   return GTP_OK;
 }
 
@@ -1330,10 +1327,9 @@ gtp_status run_load(gtp_ctx* x, MultiPlan& mp, const std::vector<RunItem>& items
   }
   voff[(size_t)nt * N] = E_load;
   DArr<int64_t> d_voff, d_chunk;
-  DArr<uint32_t> d_ev, d_aux;
+  DArr<uint32_t> d_ev;
   HIPCHK(x, d_voff.upload(voff.data(), voff.size(), s));
   HIPCHK(x, d_ev.alloc(E_load + EV_PAD));
-  HIPCHK(x, d_aux.alloc(E_load + EV_PAD));
   {
     const int64_t CH = 1 << 26;
     DArr<int64_t> st;
@@ -1407,11 +1403,9 @@ gtp_status run_load(gtp_ctx* x, MultiPlan& mp, const std::vector<RunItem>& items
   for (int32_t k = 0; k < nt; ++k)
     HIPCHK(x, launch_tile_index(d_voff.p + (int64_t)k * N, d_ev.p, N, items[k].n_tile, TC * T, nullptr, 0,
                                 d_tidx.p + td[k].tile_idx_base, s));
-  HIPCHK(x, launch_glitch_annotate(d_voff.p, d_ev.p, d_chunk.p, d_chunk_net.p, n_chunk, (uint32_t)T, mp.invT,
-                                   d_aux.p, s));
+  HIPCHK(x, launch_glitch_annotate(d_voff.p, d_ev.p, d_chunk.p, d_chunk_net.p, n_chunk, (uint32_t)T, mp.invT, s));
   rp.a.net_off = d_voff.p;
   rp.a.ev = d_ev.p;
-  rp.a.aux = d_aux.p;
   rp.a.tile_idx = d_tidx.p;
   rp.a.traces = d_td.p;
   const int32_t n_range = mp.n_range;

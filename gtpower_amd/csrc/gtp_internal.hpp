@@ -4,7 +4,7 @@
 // Device event store: one 32-bit word per event, per-net CSR order,
 //   word = (time << 3) | (glitch << 2) | value,
 //   time < 2^29 - 1 ticks, value in {0,1,2,3}, glitch set by the glitch
-//   annotation when the event's glitch pulse width (aux) is nonzero.
+//   annotation when the event's glitch pulse width is nonzero.
 // Times compare as words (the low bits are below the time bits), so a
 // lower_bound on time is a lower_bound on words.  EV_INF is past every event.
 // The array is padded by 4 words so that 16-byte loads of the last quad stay
@@ -143,8 +143,6 @@ struct PowerArgs {
   // event store
   const int64_t* net_off;        // [N+1]
   const uint32_t* ev;            // [E + EV_PAD]
-  const uint32_t* aux;           // [E + EV_PAD] glitch pulse width of each event (0 = not a glitch); read
-                                 // only for events whose word has EV_GLITCH set
   const int32_t* tile_idx;       // [N][n_tile + 1] first event index >= tile start (net-relative);
                                  // [N][n_work_tiles] by list position when tile_list is set
   int32_t n_tile;
@@ -238,11 +236,10 @@ hipError_t launch_gate_reduce(const long long* pg, const uint32_t* tog, const ui
 constexpr int TOTAL_BLOCKS = 256;
 hipError_t launch_chunk_net(const int64_t* chunk_off, int64_t n_net, int32_t* chunk_net, hipStream_t s);
 hipError_t launch_glitch_annotate(const int64_t* net_off, uint32_t* ev, const int64_t* chunk_off,
-                                  const int32_t* chunk_net, int64_t n_chunk, uint32_t T, float invT, uint32_t* aux,
-                                  hipStream_t s);
+                                  const int32_t* chunk_net, int64_t n_chunk, uint32_t T, float invT, hipStream_t s);
 hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, uint32_t* ev, const int32_t* idx_begin,
                                         const int32_t* tile_list, int32_t n_list, int64_t n_net, int64_t tile_ticks,
-                                        uint32_t T, float invT, uint32_t* aux, hipStream_t s);
+                                        uint32_t T, float invT, hipStream_t s);
 hipError_t launch_total(const double* gate_power, const int32_t* gate_class, int64_t n_gate, double* part,
                         double* out, hipStream_t s);
 hipError_t launch_power_cell(int np, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,

@@ -634,11 +634,10 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
 
 // ---------------------------------------------------------------------------
 // Glitch filtering, once per net event (power/TimeBasedPower.cc:589-637 and
-// :303-339): aux[e] = the pulse width (ticks) that scales event e when its
-// clock-cycle glitch flag is set, else 0.  A flagged event always has a flagged
-// neighbour in its own cycle group, so a flagged event has aux > 0.  A chunk
-// is 256 consecutive events of one net (chunk_off[n] is the first chunk of net
-// n); blocks take chunks in a grid stride.
+// :303-339): the EV_GLITCH bit of event e is set when its clock-cycle glitch
+// pulse width (ticks) is nonzero.  The power kernels recompute that width from
+// the flags and times of e's neighbours.  A chunk is 256 consecutive events of
+// one net (chunk_off[n] is the first chunk of net n).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int group_pos(const uint32_t* ev, int64_t i, uint32_t cstart) {
   // number of events before i in the same cycle group (event 0 is never grouped)
@@ -647,7 +646,7 @@ __device__ __forceinline__ int group_pos(const uint32_t* ev, int64_t i, uint32_t
   return j;
 }
 
-// aux value of event i (of cnt) of one net's event list e
+// glitch pulse width of event i (of cnt) of one net's event list e (0: none)
 __device__ __forceinline__ uint32_t annotate_one(const uint32_t* e, int64_t i, int64_t cnt, uint32_t T, float invT) {
   if (i < 1) return 0;
   const uint32_t t = ev_time(e[i]);
@@ -694,28 +693,167 @@ __global__ void chunk_net_kernel(const int64_t* __restrict__ chunk_off, int64_t 
   for (int64_t c = chunk_off[n]; c < chunk_off[n + 1]; ++c) chunk_net[c] = (int32_t)n;
 }
 
+// Whole-store annotation, the rule of annotate_one restated for one wave per
+// chunk range.  With j(m) the position of event m in its cycle group (events
+// from index 1 on) and last(m) whether it ends the group, event m is a
+// candidate when m >= 1 and (!last(m) || j(m) odd); event k then gets
+// pw = max(t_k - t_{k-1} if k-1 is a candidate, t_{k+1} - t_k if k+1 is one)
+// when k itself is a candidate, else 0.
+//
+// Every wave takes a contiguous range of chunks and walks it in order; lane l
+// holds events i0 + 4l .. i0 + 4l + 3 of the chunk in registers, the events
+// next to the chunk (i0 - 1, i0 + 256, i0 + 257) are loaded by every lane, and
+// neighbours across lanes come from shuffles.  Group positions come from the
+// start of each event's group: a 64-lane max-scan of the last group start in
+// each lane, seeded with the group start of event i0 - 1, which is carried
+// from the previous chunk of the same net (found in memory only for a wave's
+// first chunk).  The next chunk's words are loaded while the current one is
+// evaluated.
+constexpr uint32_t AN_NONE = 0xFFFFFFFEu;  // cycle of index <= 0 (never grouped)
+constexpr uint32_t AN_END = 0xFFFFFFFFu;   // cycle past the net's last event
+
+__device__ __forceinline__ uint32_t an_cycle(uint32_t w, int64_t k, int64_t cnt, uint32_t T, float invT) {
+  return k >= cnt ? AN_END : (k <= 0 ? AN_NONE : cycle_of(ev_time(w), T, invT));
+}
+
+struct AnChunk {
+  int64_t b, cnt, i0, next_first;  // net start and count, chunk start, first chunk of the next net
+};
+
 __global__ __launch_bounds__(256) void glitch_annotate_kernel(const int64_t* __restrict__ net_off,
                                                               uint32_t* __restrict__ ev,
                                                               const int64_t* __restrict__ chunk_off,
                                                               const int32_t* __restrict__ chunk_net,
-                                                              int64_t n_chunk, uint32_t T, float invT,
-                                                              uint32_t* __restrict__ aux) {
-  // blocks stride over the chunks: one block per chunk would exceed the 2^32
-  // work-items of a dispatch above about 4.3e9 events
-  for (int64_t chunk = blockIdx.x; chunk < n_chunk; chunk += gridDim.x) {
-    const int64_t n = chunk_net[chunk];
-    const int64_t b = net_off[n];
-    const int64_t cnt = net_off[n + 1] - b;
-    const int64_t i = (chunk - chunk_off[n]) * 256 + threadIdx.x;
-    if (i >= cnt) continue;
-    // The width is read only for flagged events, so only nonzero widths are
-    // written; the word is rewritten only when its flag changes (other lanes
-    // read its time meanwhile: the time bits do not change).
-    const uint32_t pw = annotate_one(ev + b, i, cnt, T, invT);
-    if (pw) aux[b + i] = pw;
-    const uint32_t w = ev[b + i];
-    const uint32_t nw = pw ? (w | EV_GLITCH) : (w & ~EV_GLITCH);
-    if (nw != w) ev[b + i] = nw;
+                                                              int64_t n_chunk, uint32_t T, float invT) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t n_wave = (int64_t)gridDim.x * 4;
+  const int64_t per = (n_chunk + n_wave - 1) / n_wave;
+  const int64_t c_begin = wave * per;
+  const int64_t c_end = min(n_chunk, c_begin + per);
+  if (c_begin >= c_end) return;  // the whole wave: no barriers follow
+  auto chunk_of_net = [&](int64_t c) {
+    const int64_t n = chunk_net[c];
+    AnChunk a;
+    a.b = net_off[n];
+    a.cnt = net_off[n + 1] - a.b;
+    const int64_t first = chunk_off[n];
+    a.i0 = (c - first) * 256;
+    a.next_first = chunk_off[n + 1];
+    return a;
+  };
+  // Words of neighbouring chunks are read for their time bits only, which
+  // other waves never change (they may rewrite the glitch bit meanwhile).
+  uint32_t w[4], hp, hn0, hn1;
+  auto fetch = [&](const AnChunk& a, uint32_t (&dst)[4], uint32_t& p, uint32_t& n0, uint32_t& n1) {
+    const int64_t k0 = a.i0 + 4 * lane;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) dst[r] = k0 + r < a.cnt ? ev[a.b + k0 + r] : EV_INF;
+    p = a.i0 >= 1 ? ev[a.b + a.i0 - 1] : EV_INF;
+    n0 = a.i0 + 256 < a.cnt ? ev[a.b + a.i0 + 256] : EV_INF;
+    n1 = a.i0 + 257 < a.cnt ? ev[a.b + a.i0 + 257] : EV_INF;
+  };
+  AnChunk cur = chunk_of_net(c_begin);
+  fetch(cur, w, hp, hn0, hn1);
+  // group start of event i0 - 1 (absolute index; unused when i0 - 1 < 1)
+  int64_t s_in = cur.i0 - 1;
+  if (s_in >= 1) {
+    const uint32_t cc = cycle_of(ev_time(hp), T, invT);
+    while (s_in - 1 >= 1 && cycle_of(ev_time(ev[cur.b + s_in - 1]), T, invT) == cc) --s_in;
+  }
+  for (int64_t c = c_begin; c < c_end; ++c) {
+    // the next chunk: same net, or the first chunk of a later one
+    const bool has_next = c + 1 < c_end;
+    const bool same = c + 1 < cur.next_first;
+    AnChunk nxt = cur;
+    uint32_t wn[4] = {EV_INF, EV_INF, EV_INF, EV_INF}, hpn = EV_INF, hn0n = EV_INF, hn1n = EV_INF;
+    if (has_next) {
+      if (same)
+        nxt.i0 = cur.i0 + 256;
+      else
+        nxt = chunk_of_net(c + 1);
+      fetch(nxt, wn, hpn, hn0n, hn1n);
+    }
+    // ---- the chunk's events: k = i0 + 4 lane + r
+    const int64_t i0 = cur.i0, cnt = cur.cnt;
+    const int64_t k0 = i0 + 4 * lane;
+    uint32_t cy[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) cy[r] = an_cycle(w[r], k0 + r, cnt, T, invT);
+    const uint32_t c_up = __shfl_up(cy[3], 1);
+    const uint32_t c_dn0 = __shfl_down(cy[0], 1), c_dn1 = __shfl_down(cy[1], 1);
+    const uint32_t cp0 = lane == 0 ? an_cycle(hp, i0 - 1, cnt, T, invT) : c_up;
+    const uint32_t cn3 = lane == 63 ? an_cycle(hn0, i0 + 256, cnt, T, invT) : c_dn0;
+    const uint32_t cnn3 = lane == 63 ? an_cycle(hn1, i0 + 257, cnt, T, invT) : c_dn1;
+    // last group start in this lane (chunk-relative position), then the scan
+    bool st[4];
+    st[0] = cy[0] != cp0;
+    st[1] = cy[1] != cy[0];
+    st[2] = cy[2] != cy[1];
+    st[3] = cy[3] != cy[2];
+    const int p0 = 4 * lane;
+    int ls = INT32_MIN;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (st[r]) ls = p0 + r;
+    int v = ls;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int o = __shfl_up(v, d);
+      if (lane >= d) v = max(v, o);
+    }
+    int run = __shfl_up(v, 1);
+    if (lane == 0) run = INT32_MIN;
+    run = max(run, (int)(s_in - i0));  // the group of event i0 - 1 (negative position)
+    int j[4];
+    bool cand[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (st[r]) run = p0 + r;
+      j[r] = p0 + r - run;
+      const uint32_t cn = r < 3 ? cy[r + 1] : cn3;
+      const int64_t k = k0 + r;
+      cand[r] = k >= 1 && k < cnt && (cn != cy[r] ? (j[r] & 1) != 0 : true);
+    }
+    // candidates next to the lane's events
+    const bool cand_up = __shfl_up((int)cand[3], 1) != 0;
+    const bool cand_dn = __shfl_down((int)cand[0], 1) != 0;
+    bool cprev0 = cand_up, cnext3 = cand_dn;
+    if (lane == 0) {  // event i0 - 1: its group ends at i0 when cycles differ
+      const int64_t k = i0 - 1;
+      const uint32_t ck = cp0;
+      cprev0 = k >= 1 && (cy[0] != ck ? ((k - s_in) & 1) != 0 : true);
+    }
+    if (lane == 63) {  // event i0 + 256
+      const int64_t k = i0 + 256;
+      const int jn = cn3 != cy[3] ? 0 : j[3] + 1;
+      cnext3 = k < cnt && (cnn3 != cn3 ? (jn & 1) != 0 : true);
+    }
+    const uint32_t t_up = __shfl_up(ev_time(w[3]), 1), t_dn = __shfl_down(ev_time(w[0]), 1);
+    const int32_t tp0 = (int32_t)(lane == 0 ? ev_time(hp) : t_up);
+    const int32_t tn3 = (int32_t)(lane == 63 ? ev_time(hn0) : t_dn);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int32_t t = (int32_t)ev_time(w[r]);
+      const bool cp = r == 0 ? cprev0 : cand[r - 1];
+      const bool cn = r == 3 ? cnext3 : cand[r + 1];
+      const int32_t tp = r == 0 ? tp0 : (int32_t)ev_time(w[r - 1]);
+      const int32_t tn = r == 3 ? tn3 : (int32_t)ev_time(w[r + 1]);
+      int32_t pw = 0;
+      if (cand[r]) pw = max(cp ? t - tp : 0, cn ? tn - t : 0);
+      // the word is rewritten only when its flag changes
+      const uint32_t nw = pw > 0 ? (w[r] | EV_GLITCH) : (w[r] & ~EV_GLITCH);
+      if (k0 + r < cnt && nw != w[r]) ev[cur.b + k0 + r] = nw;
+    }
+    // group start of event i0 + 255 for the next chunk of this net
+    const int run63 = __shfl(run, 63);
+    s_in = same ? i0 + run63 : 0;
+    cur = nxt;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) w[r] = wn[r];
+    hp = hpn;
+    hn0 = hn0n;
+    hn1 = hn1n;
   }
 }
 
@@ -729,24 +867,25 @@ __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_
                                                                     const int32_t* __restrict__ idx_begin,
                                                                     const int32_t* __restrict__ tile_list,
                                                                     int32_t n_list, int64_t n_net, int64_t tile_ticks,
-                                                                    uint32_t T, float invT,
-                                                                    uint32_t* __restrict__ aux) {
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t i = tid >> 3;
-  const int j = (int)(tid & 7);
-  if (i >= n_net * n_list) return;
-  const int64_t n = i / n_list;
-  const int64_t b = net_off[n];
-  const int64_t cnt = net_off[n + 1] - b;
-  const int64_t tend = ((int64_t)tile_list[i - n * n_list] + 1) * tile_ticks;
-  uint32_t* e = ev + b;
-  for (int64_t k = max<int64_t>(idx_begin[i] - 1, 0) + j; k < cnt; k += 8) {
-    if ((int64_t)ev_time(e[k]) >= tend && k > 0 && (int64_t)ev_time(e[k - 1]) >= tend) break;
-    const uint32_t pw = annotate_one(e, k, cnt, T, invT);
-    if (pw) aux[b + k] = pw;
-    const uint32_t w = e[k];
-    const uint32_t nw = pw ? (w | EV_GLITCH) : (w & ~EV_GLITCH);
-    if (nw != w) e[k] = nw;
+                                                                    uint32_t T, float invT) {
+  // threads stride over the (net, tile, lane) triples under a capped grid
+  const int64_t n_thr = n_net * n_list * 8;
+  for (int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; tid < n_thr;
+       tid += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = tid >> 3;
+    const int j = (int)(tid & 7);
+    const int64_t n = i / n_list;
+    const int64_t b = net_off[n];
+    const int64_t cnt = net_off[n + 1] - b;
+    const int64_t tend = ((int64_t)tile_list[i - n * n_list] + 1) * tile_ticks;
+    uint32_t* e = ev + b;
+    for (int64_t k = max<int64_t>(idx_begin[i] - 1, 0) + j; k < cnt; k += 8) {
+      if ((int64_t)ev_time(e[k]) >= tend && k > 0 && (int64_t)ev_time(e[k - 1]) >= tend) break;
+      const uint32_t pw = annotate_one(e, k, cnt, T, invT);
+      const uint32_t w = e[k];
+      const uint32_t nw = pw ? (w | EV_GLITCH) : (w & ~EV_GLITCH);
+      if (nw != w) e[k] = nw;
+    }
   }
 }
 
@@ -1476,22 +1615,29 @@ hipError_t launch_chunk_net(const int64_t* chunk_off, int64_t n_net, int32_t* ch
 }
 
 hipError_t launch_glitch_annotate(const int64_t* net_off, uint32_t* ev, const int64_t* chunk_off,
-                                  const int32_t* chunk_net, int64_t n_chunk, uint32_t T, float invT, uint32_t* aux,
-                                  hipStream_t s) {
+                                  const int32_t* chunk_net, int64_t n_chunk, uint32_t T, float invT, hipStream_t s) {
   if (n_chunk <= 0) return hipSuccess;
-  const int64_t blocks = stride_blocks(n_chunk);
+  // the persistent grid: as many blocks as the device holds at once, each
+  // wave taking a contiguous range of chunks
+  int dev = 0, n_cu = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, glitch_annotate_kernel, 256, 0);
+  if (e != hipSuccess) return e;
+  const int64_t resident = std::max<int64_t>(1, (int64_t)n_cu * std::max(per_cu, 1));
+  const int64_t blocks = stride_blocks(std::min<int64_t>((n_chunk + 3) / 4, resident));
   glitch_annotate_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(net_off, ev, chunk_off, chunk_net, n_chunk, T,
-                                                                     invT, aux);
+                                                                     invT);
   return hipGetLastError();
 }
 
 hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, uint32_t* ev, const int32_t* idx_begin,
                                         const int32_t* tile_list, int32_t n_list, int64_t n_net, int64_t tile_ticks,
-                                        uint32_t T, float invT, uint32_t* aux, hipStream_t s) {
+                                        uint32_t T, float invT, hipStream_t s) {
   const int64_t n = n_net * n_list * 8;
   if (n <= 0) return hipSuccess;
-  glitch_annotate_tiles_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(
-      net_off, ev, idx_begin, tile_list, n_list, n_net, tile_ticks, T, invT, aux);
+  glitch_annotate_tiles_kernel<<<dim3((unsigned)stride_blocks((n + 255) / 256)), dim3(256), 0, s>>>(
+      net_off, ev, idx_begin, tile_list, n_list, n_net, tile_ticks, T, invT);
   return hipGetLastError();
 }
 
