@@ -1081,7 +1081,7 @@ gtp_status run_once(gtp_ctx* x, gtp_result* res, int64_t tc_req, int32_t rg_req)
   const PowerArgs& a = rp.a;
   HIPCHK(x, hipEventRecord(ev.e[1], s));
   HIPCHK(x, launch_glitch_annotate(x->d_net_off.p, x->d_ev.p, x->d_chunk_off.p, x->d_chunk_net.p, x->n_chunk,
-                                   (uint32_t)x->T, a.inv_T, s));
+                                   (uint32_t)x->T, s));
   x->glitch_valid = true;
   if ((st = launch_groups(x, rp)) != GTP_OK) return st;
   HIPCHK(x, hipEventRecord(ev.e[2], s));
@@ -1403,7 +1403,7 @@ gtp_status run_load(gtp_ctx* x, MultiPlan& mp, const std::vector<RunItem>& items
   for (int32_t k = 0; k < nt; ++k)
     HIPCHK(x, launch_tile_index(d_voff.p + (int64_t)k * N, d_ev.p, N, items[k].n_tile, TC * T, nullptr, 0,
                                 d_tidx.p + td[k].tile_idx_base, s));
-  HIPCHK(x, launch_glitch_annotate(d_voff.p, d_ev.p, d_chunk.p, d_chunk_net.p, n_chunk, (uint32_t)T, mp.invT, s));
+  HIPCHK(x, launch_glitch_annotate(d_voff.p, d_ev.p, d_chunk.p, d_chunk_net.p, n_chunk, (uint32_t)T, s));
   rp.a.net_off = d_voff.p;
   rp.a.ev = d_ev.p;
   rp.a.tile_idx = d_tidx.p;

@@ -236,7 +236,7 @@ hipError_t launch_gate_reduce(const long long* pg, const uint32_t* tog, const ui
 constexpr int TOTAL_BLOCKS = 256;
 hipError_t launch_chunk_net(const int64_t* chunk_off, int64_t n_net, int32_t* chunk_net, hipStream_t s);
 hipError_t launch_glitch_annotate(const int64_t* net_off, uint32_t* ev, const int64_t* chunk_off,
-                                  const int32_t* chunk_net, int64_t n_chunk, uint32_t T, float invT, hipStream_t s);
+                                  const int32_t* chunk_net, int64_t n_chunk, uint32_t T, hipStream_t s);
 hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, uint32_t* ev, const int32_t* idx_begin,
                                         const int32_t* tile_list, int32_t n_list, int64_t n_net, int64_t tile_ticks,
                                         uint32_t T, float invT, hipStream_t s);

@@ -701,102 +701,140 @@ __global__ void chunk_net_kernel(const int64_t* __restrict__ chunk_off, int64_t 
 // when k itself is a candidate, else 0.
 //
 // Every wave takes a contiguous range of chunks and walks it in order; lane l
-// holds events i0 + 4l .. i0 + 4l + 3 of the chunk in registers, the events
-// next to the chunk (i0 - 1, i0 + 256, i0 + 257) are loaded by every lane, and
-// neighbours across lanes come from shuffles.  Group positions come from the
+// holds the events at chunk positions 4l .. 4l + 3 in registers, the events
+// next to the chunk (positions -1, 256, 257) are loaded by lanes 0 and 63,
+// and neighbours across lanes come from shuffles.  Group positions come from the
 // start of each event's group: a 64-lane max-scan of the last group start in
-// each lane, seeded with the group start of event i0 - 1, which is carried
-// from the previous chunk of the same net (found in memory only for a wave's
-// first chunk).  The next chunk's words are loaded while the current one is
+// each lane, seeded with the group start of the event at position -1, which
+// is carried from the previous chunk of the same net (found in memory only
+// for a wave's first chunk).  Chunks are loaded two ahead of the one being
 // evaluated.
 constexpr uint32_t AN_NONE = 0xFFFFFFFEu;  // cycle of index <= 0 (never grouped)
 constexpr uint32_t AN_END = 0xFFFFFFFFu;   // cycle past the net's last event
 
-__device__ __forceinline__ uint32_t an_cycle(uint32_t w, int64_t k, int64_t cnt, uint32_t T, float invT) {
-  return k >= cnt ? AN_END : (k <= 0 ? AN_NONE : cycle_of(ev_time(w), T, invT));
+// A wave-uniform value moved to scalar registers, so that branches on it are
+// scalar branches.
+__device__ __forceinline__ int64_t an_uniform(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-struct AnChunk {
-  int64_t b, cnt, i0, next_first;  // net start and count, chunk start, first chunk of the next net
-};
+// Chunk c of a range, after the chunk at e/left/next_first (same net: the
+// next 256 events; else the first chunk of a later net).
+__device__ __forceinline__ void an_next_chunk(const int64_t* __restrict__ net_off, uint32_t* ev,
+                                              const int64_t* __restrict__ chunk_off,
+                                              const int32_t* __restrict__ chunk_net, int64_t c, uint32_t*& e,
+                                              int64_t& left, int64_t& next_first, bool& first) {
+  if (c < next_first) {
+    e += 256;
+    left -= 256;
+    first = false;
+    return;
+  }
+  const int64_t n = an_uniform(chunk_net[c]);
+  const int64_t b = an_uniform(net_off[n]), i0 = (c - an_uniform(chunk_off[n])) * 256;
+  e = ev + b + i0;
+  left = an_uniform(net_off[n + 1]) - b - i0;
+  next_first = an_uniform(chunk_off[n + 1]);
+  first = i0 == 0;
+}
+
+// Raw words of one chunk for lane `lane`: positions 4 lane + r; lane 0 also
+// -1 and lane 63 also 256, 257 (the other lanes load position 0 in their
+// place: per-lane addresses keep these loads from being treated as uniform
+// values, which would wait for them at once).  Every position is clamped into
+// the net (a chunk holds at least one event), so every load is issued
+// unconditionally; absent positions are masked to EV_INF only when the chunk
+// is evaluated, which leaves the loads in flight until then.  Words of
+// neighbouring chunks are read for their time bits only, which other waves
+// never change (they may rewrite the glitch bit meanwhile).
+__device__ __forceinline__ void an_fetch(const uint32_t* e, int64_t left, bool first, int lane, uint32_t (&x)[7]) {
+  const int p0 = 4 * lane;
+  const int lim = left >= 256 ? 256 : (int)left;  // valid positions, >= 1
+#pragma unroll
+  for (int r = 0; r < 4; ++r) x[r] = e[min(p0 + r, lim - 1)];
+  x[4] = e[lane == 0 && !first ? -1 : 0];
+  x[5] = e[lane == 63 && left > 256 ? 256 : 0];
+  x[6] = e[lane == 63 && left > 257 ? 257 : 0];
+}
+
+// One chunk's state in a wave's pipeline.
+#define AN_SLOT(X) uint32_t x##X[7]; uint32_t* e##X = nullptr; int64_t left##X = 0, nf##X = -1; bool first##X = false
 
 __global__ __launch_bounds__(256) void glitch_annotate_kernel(const int64_t* __restrict__ net_off,
                                                               uint32_t* __restrict__ ev,
                                                               const int64_t* __restrict__ chunk_off,
                                                               const int32_t* __restrict__ chunk_net,
-                                                              int64_t n_chunk, uint32_t T, float invT) {
+                                                              int64_t n_chunk, uint32_t cyc_m, int cyc_sh) {
   const int lane = threadIdx.x & 63;
-  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  // the wave index in scalar registers: the chunk loop is then a scalar loop
+  const int64_t wave = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t n_wave = (int64_t)gridDim.x * 4;
   const int64_t per = (n_chunk + n_wave - 1) / n_wave;
   const int64_t c_begin = wave * per;
   const int64_t c_end = min(n_chunk, c_begin + per);
   if (c_begin >= c_end) return;  // the whole wave: no barriers follow
-  auto chunk_of_net = [&](int64_t c) {
-    const int64_t n = chunk_net[c];
-    AnChunk a;
-    a.b = net_off[n];
-    a.cnt = net_off[n + 1] - a.b;
-    const int64_t first = chunk_off[n];
-    a.i0 = (c - first) * 256;
-    a.next_first = chunk_off[n + 1];
-    return a;
-  };
-  // Words of neighbouring chunks are read for their time bits only, which
-  // other waves never change (they may rewrite the glitch bit meanwhile).
-  uint32_t w[4], hp, hn0, hn1;
-  auto fetch = [&](const AnChunk& a, uint32_t (&dst)[4], uint32_t& p, uint32_t& n0, uint32_t& n1) {
-    const int64_t k0 = a.i0 + 4 * lane;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) dst[r] = k0 + r < a.cnt ? ev[a.b + k0 + r] : EV_INF;
-    p = a.i0 >= 1 ? ev[a.b + a.i0 - 1] : EV_INF;
-    n0 = a.i0 + 256 < a.cnt ? ev[a.b + a.i0 + 256] : EV_INF;
-    n1 = a.i0 + 257 < a.cnt ? ev[a.b + a.i0 + 257] : EV_INF;
-  };
-  AnChunk cur = chunk_of_net(c_begin);
-  fetch(cur, w, hp, hn0, hn1);
-  // group start of event i0 - 1 (absolute index; unused when i0 - 1 < 1)
-  int64_t s_in = cur.i0 - 1;
-  if (s_in >= 1) {
-    const uint32_t cc = cycle_of(ev_time(hp), T, invT);
-    while (s_in - 1 >= 1 && cycle_of(ev_time(ev[cur.b + s_in - 1]), T, invT) == cc) --s_in;
+  // Three chunk slots take turns: the chunk being evaluated, the next one
+  // (loaded) and the one after (being loaded); the loop is unrolled three
+  // times so that no slot's registers are copied while its loads are in
+  // flight.  Past the range's end the last chunk is loaded again (never
+  // evaluated).
+  AN_SLOT(A);
+  AN_SLOT(B);
+  AN_SLOT(C);
+  an_next_chunk(net_off, ev, chunk_off, chunk_net, c_begin, eA, leftA, nfA, firstA);
+  an_fetch(eA, leftA, firstA, lane, xA);
+  eB = eA, leftB = leftA, nfB = nfA, firstB = firstA;
+  if (c_begin + 1 < c_end) an_next_chunk(net_off, ev, chunk_off, chunk_net, c_begin + 1, eB, leftB, nfB, firstB);
+  an_fetch(eB, leftB, firstB, lane, xB);
+  // group start of the event at position -1, relative to the chunk: found in
+  // memory for the wave's first chunk, carried afterwards (a net's first chunk
+  // has no event at -1 and ignores it: its position 1 always starts a group)
+  int s_rel = -1;
+  if (!firstA) {
+    const int64_t i0 = (c_begin - an_uniform(chunk_off[an_uniform(chunk_net[c_begin])])) * 256;  // >= 256
+    const uint32_t cc = __builtin_amdgcn_readlane(cycle_mul(ev_time(xA[4]), cyc_m, cyc_sh), 0);
+    int64_t q = 0;  // events of the group before position -1 (index i0 - 2 - q >= 1)
+    while (i0 - 2 - q >= 1 &&
+           __builtin_amdgcn_readfirstlane(cycle_mul(ev_time(eA[-2 - q]), cyc_m, cyc_sh)) == cc)
+      ++q;
+    s_rel = (int)(-1 - q);
   }
-  for (int64_t c = c_begin; c < c_end; ++c) {
-    // the next chunk: same net, or the first chunk of a later one
-    const bool has_next = c + 1 < c_end;
-    const bool same = c + 1 < cur.next_first;
-    AnChunk nxt = cur;
-    uint32_t wn[4] = {EV_INF, EV_INF, EV_INF, EV_INF}, hpn = EV_INF, hn0n = EV_INF, hn1n = EV_INF;
-    if (has_next) {
-      if (same)
-        nxt.i0 = cur.i0 + 256;
-      else
-        nxt = chunk_of_net(c + 1);
-      fetch(nxt, wn, hpn, hn0n, hn1n);
-    }
-    // ---- the chunk's events: k = i0 + 4 lane + r
-    const int64_t i0 = cur.i0, cnt = cur.cnt;
-    const int64_t k0 = i0 + 4 * lane;
+  const int p0 = 4 * lane;
+  // Evaluate chunk c held in slot (x0, e0, left0, first0) while loading chunk
+  // c + 2 into slot z from slot y's state (chunk c + 1).
+  auto step = [&](int64_t c, const uint32_t (&x0)[7], uint32_t* e0, int64_t left0, bool first0, uint32_t* ey,
+                  int64_t lefty, int64_t nfy, bool firsty, uint32_t (&xz)[7], uint32_t*& ez, int64_t& leftz,
+                  int64_t& nfz, bool& firstz) {
+    ez = ey, leftz = lefty, nfz = nfy, firstz = firsty;
+    if (c + 2 < c_end) an_next_chunk(net_off, ev, chunk_off, chunk_net, c + 2, ez, leftz, nfz, firstz);
+    an_fetch(ez, leftz, firstz, lane, xz);
+    // ---- chunk c: positions p = 4 lane + r, rem of them valid
+    const int rem = left0 >= 256 ? 256 : (int)left0;
+    uint32_t w[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) w[r] = p0 + r < rem ? x0[r] : EV_INF;
+    const uint32_t hp = first0 ? EV_INF : x0[4];         // meaningful in lane 0
+    const uint32_t hn0 = left0 > 256 ? x0[5] : EV_INF;  // meaningful in lane 63
+    const uint32_t hn1 = left0 > 257 ? x0[6] : EV_INF;
     uint32_t cy[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) cy[r] = an_cycle(w[r], k0 + r, cnt, T, invT);
-    const uint32_t c_up = __shfl_up(cy[3], 1);
-    const uint32_t c_dn0 = __shfl_down(cy[0], 1), c_dn1 = __shfl_down(cy[1], 1);
-    const uint32_t cp0 = lane == 0 ? an_cycle(hp, i0 - 1, cnt, T, invT) : c_up;
-    const uint32_t cn3 = lane == 63 ? an_cycle(hn0, i0 + 256, cnt, T, invT) : c_dn0;
-    const uint32_t cnn3 = lane == 63 ? an_cycle(hn1, i0 + 257, cnt, T, invT) : c_dn1;
-    // last group start in this lane (chunk-relative position), then the scan
-    bool st[4];
-    st[0] = cy[0] != cp0;
-    st[1] = cy[1] != cy[0];
-    st[2] = cy[2] != cy[1];
-    st[3] = cy[3] != cy[2];
-    const int p0 = 4 * lane;
-    int ls = INT32_MIN;
-#pragma unroll
     for (int r = 0; r < 4; ++r)
-      if (st[r]) ls = p0 + r;
-    int v = ls;
+      cy[r] = p0 + r >= rem ? AN_END
+                            : ((first0 && p0 + r == 0) ? AN_NONE : cycle_mul(ev_time(w[r]), cyc_m, cyc_sh));
+    // position -1 is index i0 - 1 >= 255, absent when the chunk starts the net
+    const uint32_t c_m1 = first0 ? AN_NONE : cycle_mul(ev_time(hp), cyc_m, cyc_sh);
+    const uint32_t c_256 = left0 > 256 ? cycle_mul(ev_time(hn0), cyc_m, cyc_sh) : AN_END;
+    const uint32_t c_257 = left0 > 257 ? cycle_mul(ev_time(hn1), cyc_m, cyc_sh) : AN_END;
+    const uint32_t c_up = __shfl_up(cy[3], 1);
+    const uint32_t c_dn = __shfl_down(cy[0], 1);
+    const uint32_t cp0 = lane == 0 ? c_m1 : c_up;
+    const uint32_t cn3 = lane == 63 ? c_256 : c_dn;
+    // group starts (bit r), the last one in this lane, then the scan
+    const uint32_t st = (cy[0] != cp0 ? 1u : 0u) | (cy[1] != cy[0] ? 2u : 0u) | (cy[2] != cy[1] ? 4u : 0u) |
+                        (cy[3] != cy[2] ? 8u : 0u);
+    int v = st ? p0 + 31 - __builtin_clz(st) : INT32_MIN;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
       const int o = __shfl_up(v, d);
@@ -804,30 +842,25 @@ __global__ __launch_bounds__(256) void glitch_annotate_kernel(const int64_t* __r
     }
     int run = __shfl_up(v, 1);
     if (lane == 0) run = INT32_MIN;
-    run = max(run, (int)(s_in - i0));  // the group of event i0 - 1 (negative position)
+    run = max(run, s_rel);
     int j[4];
-    bool cand[4];
+    uint32_t cand = 0;  // bit r: candidate
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      if (st[r]) run = p0 + r;
+      if (st & (1u << r)) run = p0 + r;
       j[r] = p0 + r - run;
       const uint32_t cn = r < 3 ? cy[r + 1] : cn3;
-      const int64_t k = k0 + r;
-      cand[r] = k >= 1 && k < cnt && (cn != cy[r] ? (j[r] & 1) != 0 : true);
+      // a real cycle: index >= 1 and inside the net
+      if (cy[r] < AN_NONE && (cn != cy[r] ? (j[r] & 1) != 0 : true)) cand |= 1u << r;
     }
     // candidates next to the lane's events
-    const bool cand_up = __shfl_up((int)cand[3], 1) != 0;
-    const bool cand_dn = __shfl_down((int)cand[0], 1) != 0;
-    bool cprev0 = cand_up, cnext3 = cand_dn;
-    if (lane == 0) {  // event i0 - 1: its group ends at i0 when cycles differ
-      const int64_t k = i0 - 1;
-      const uint32_t ck = cp0;
-      cprev0 = k >= 1 && (cy[0] != ck ? ((k - s_in) & 1) != 0 : true);
-    }
-    if (lane == 63) {  // event i0 + 256
-      const int64_t k = i0 + 256;
-      const int jn = cn3 != cy[3] ? 0 : j[3] + 1;
-      cnext3 = k < cnt && (cnn3 != cn3 ? (jn & 1) != 0 : true);
+    bool cprev0 = (__shfl_up(cand, 1) & 8u) != 0;
+    bool cnext3 = (__shfl_down(cand, 1) & 1u) != 0;
+    if (lane == 0)  // position -1: its group ends at position 0 when cycles differ
+      cprev0 = c_m1 < AN_NONE && (cy[0] != c_m1 ? ((-1 - s_rel) & 1) != 0 : true);
+    if (lane == 63) {  // position 256
+      const int jn = c_256 != cy[3] ? 0 : j[3] + 1;
+      cnext3 = c_256 < AN_NONE && (c_257 != c_256 ? (jn & 1) != 0 : true);
     }
     const uint32_t t_up = __shfl_up(ev_time(w[3]), 1), t_dn = __shfl_down(ev_time(w[0]), 1);
     const int32_t tp0 = (int32_t)(lane == 0 ? ev_time(hp) : t_up);
@@ -835,27 +868,29 @@ __global__ __launch_bounds__(256) void glitch_annotate_kernel(const int64_t* __r
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int32_t t = (int32_t)ev_time(w[r]);
-      const bool cp = r == 0 ? cprev0 : cand[r - 1];
-      const bool cn = r == 3 ? cnext3 : cand[r + 1];
+      const bool cp = r == 0 ? cprev0 : ((cand >> (r - 1)) & 1u) != 0;
+      const bool cn = r == 3 ? cnext3 : ((cand >> (r + 1)) & 1u) != 0;
       const int32_t tp = r == 0 ? tp0 : (int32_t)ev_time(w[r - 1]);
       const int32_t tn = r == 3 ? tn3 : (int32_t)ev_time(w[r + 1]);
       int32_t pw = 0;
-      if (cand[r]) pw = max(cp ? t - tp : 0, cn ? tn - t : 0);
+      if ((cand >> r) & 1u) pw = max(cp ? t - tp : 0, cn ? tn - t : 0);
       // the word is rewritten only when its flag changes
       const uint32_t nw = pw > 0 ? (w[r] | EV_GLITCH) : (w[r] & ~EV_GLITCH);
-      if (k0 + r < cnt && nw != w[r]) ev[cur.b + k0 + r] = nw;
+      if (p0 + r < rem && nw != w[r]) e0[p0 + r] = nw;
     }
-    // group start of event i0 + 255 for the next chunk of this net
-    const int run63 = __shfl(run, 63);
-    s_in = same ? i0 + run63 : 0;
-    cur = nxt;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) w[r] = wn[r];
-    hp = hpn;
-    hn0 = hn0n;
-    hn1 = hn1n;
+    // group start of position 255, relative to the next chunk (of this net)
+    s_rel = (int)__builtin_amdgcn_readlane(run, 63) - 256;
+  };
+  for (int64_t c = c_begin;;) {
+    step(c, xA, eA, leftA, firstA, eB, leftB, nfB, firstB, xC, eC, leftC, nfC, firstC);
+    if (++c >= c_end) break;
+    step(c, xB, eB, leftB, firstB, eC, leftC, nfC, firstC, xA, eA, leftA, nfA, firstA);
+    if (++c >= c_end) break;
+    step(c, xC, eC, leftC, firstC, eA, leftA, nfA, firstA, xB, eB, leftB, nfB, firstB);
+    if (++c >= c_end) break;
   }
 }
+#undef AN_SLOT
 
 // The same for the events of listed tiles only: the 8 threads of group
 // (n, l) annotate, interleaved, the events of net n from the one before
@@ -879,7 +914,7 @@ __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_
     const int64_t cnt = net_off[n + 1] - b;
     const int64_t tend = ((int64_t)tile_list[i - n * n_list] + 1) * tile_ticks;
     uint32_t* e = ev + b;
-    for (int64_t k = max<int64_t>(idx_begin[i] - 1, 0) + j; k < cnt; k += 8) {
+    for (int64_t k = (int64_t)max(idx_begin[i] - 1, 0) + j; k < cnt; k += 8) {
       if ((int64_t)ev_time(e[k]) >= tend && k > 0 && (int64_t)ev_time(e[k - 1]) >= tend) break;
       const uint32_t pw = annotate_one(e, k, cnt, T, invT);
       const uint32_t w = e[k];
@@ -1615,10 +1650,9 @@ hipError_t launch_chunk_net(const int64_t* chunk_off, int64_t n_net, int32_t* ch
 }
 
 hipError_t launch_glitch_annotate(const int64_t* net_off, uint32_t* ev, const int64_t* chunk_off,
-                                  const int32_t* chunk_net, int64_t n_chunk, uint32_t T, float invT, hipStream_t s) {
+                                  const int32_t* chunk_net, int64_t n_chunk, uint32_t T, hipStream_t s) {
   if (n_chunk <= 0) return hipSuccess;
-  // the persistent grid: as many blocks as the device holds at once, each
-  // wave taking a contiguous range of chunks
+  // the persistent grid: as many blocks as the device holds at once
   int dev = 0, n_cu = 0, per_cu = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -1626,8 +1660,11 @@ hipError_t launch_glitch_annotate(const int64_t* net_off, uint32_t* ev, const in
   if (e != hipSuccess) return e;
   const int64_t resident = std::max<int64_t>(1, (int64_t)n_cu * std::max(per_cu, 1));
   const int64_t blocks = stride_blocks(std::min<int64_t>((n_chunk + 3) / 4, resident));
-  glitch_annotate_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(net_off, ev, chunk_off, chunk_net, n_chunk, T,
-                                                                     invT);
+  uint32_t cyc_m = 0;
+  int32_t cyc_sh = 0;
+  cycle_magic(T, &cyc_m, &cyc_sh);
+  glitch_annotate_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(net_off, ev, chunk_off, chunk_net, n_chunk, cyc_m,
+                                                                     cyc_sh);
   return hipGetLastError();
 }
 
