@@ -78,6 +78,62 @@ def test_capped_grid_correlation(monkeypatch):
     assert np.array_equal(capped, full)
 
 
+def _burst_waves(seed):
+    """Nets whose cycle groups are longer than a 256-event chunk and cross
+    chunk boundaries (T = 1000 ticks, 400 cycles)."""
+    rng = np.random.default_rng(seed)
+
+    def alt(times, v0=1):
+        return [(int(t), (v0 + k) % 2) for k, t in enumerate(times)]
+
+    w0 = [(0, 0)] + alt(2000 + np.arange(700)) + alt(5100 + np.arange(257), 0)[:257] + \
+        alt([9100, 9200, 9300]) + alt(10000 + np.arange(256) * 3)
+    w0 = [w0[0]] + [(t, k % 2) for k, (t, _) in enumerate(w0[1:], start=1)]
+    w1 = alt(np.arange(1, 251) * 1000 + 500) + alt(260000 + np.arange(300)) + alt(np.arange(270, 280) * 1000 + 7)
+    t2 = np.sort(rng.choice(np.arange(1, 400000), size=1500, replace=False))
+    w2 = alt(t2)
+    w3 = [(0, 1), (123456, 0), (223456, 1)]
+    w4 = alt(np.arange(1, 800) * 500)
+    return [w0, w1, w2, w3, w4]
+
+
+@pytest.mark.parametrize("cap", ["1", "2", "3"])
+def test_long_cycle_groups_under_capped_grids(monkeypatch, library, cap):
+    # groups of 700, 257, 256 and 300 events, some across chunk boundaries;
+    # small grids start waves in the middle of nets and of groups
+    print(f"[stride] long cycle groups, GTP_STRIDE_BLOCKS={cap}", flush=True)
+    lib = library
+    cm, ca = lib.index("mux2_1"), lib.index("a21oi_1")
+    nets = [0, 1, 2, 3, 1, 2, 0, 4]
+    slew = np.linspace(30e-12, 90e-12, 16)
+    cap_f = np.array([0, 0, 0, 6e-15, 0, 0, 0, 3e-15])
+    arcs = np.full(24, 40e-12)
+    d = S.ArrayDesign(lib, [cm, ca], nets, slew, cap_f, arcs, 5)
+    tr = S.trace_from_lists(_burst_waves(7), 400000, 1e-12, 1e-9)
+    with HipPower() as eng:
+        eng.set_design(d.library, d)
+        eng.read_power_activities(tr)
+        full = eng.power()
+    monkeypatch.setenv("GTP_STRIDE_BLOCKS", cap)
+    with HipPower() as eng:
+        eng.set_design(d.library, d)
+        eng.read_power_activities(tr)
+        capped = eng.power()
+        batched = eng.power_batch([tr, tr])
+    monkeypatch.delenv("GTP_STRIDE_BLOCKS")
+    assert capped.gate_glitch.sum() > 500
+    same(capped, full, "resident run")
+    for k, b in enumerate(batched):
+        same(b, full, f"batched copy {k}")
+    ref = oracle.run(d.library, d, tr, threads=0)
+    assert np.array_equal(full.gate_glitch, ref["gate_glitch"])
+    assert np.array_equal(full.gate_toggle_x2, ref["gate_toggle_x2"])
+    err = np.abs(full.cycle_power - ref["cycle_power"])
+    assert np.all(err <= 1e-6 * ref["cycle_abs"] + 1e-30)
+    gerr = np.abs(full.gate_power - ref["gate_power"])
+    assert np.all(gerr <= 1e-6 * ref["gate_abs"] + 1e-30)
+
+
 def test_above_2e32_events_resident_equals_sliced():
     # 200k gates x 100k cycles: 4.56e9 events, one event store of 18 GB
     print("[stride] 200000 gates x 100000 cycles, resident vs sliced", flush=True)
