@@ -1207,6 +1207,7 @@ struct RunItem {
   const uint8_t* value;
   int64_t t_max, n_cycle;
   int32_t n_tile, tile_begin, tile_end, slot;
+  const uint32_t* words = nullptr;  // the events already packed as event words (time / value unused)
 };
 
 // Schedule, fixed-point scales and scratch of a multi-trace run.
@@ -1338,6 +1339,11 @@ gtp_status run_load(gtp_ctx* x, MultiPlan& mp, const std::vector<RunItem>& items
     HIPCHK(x, sv.alloc(CH));
     int64_t base = 0;
     for (int32_t k = 0; k < nt; ++k) {
+      if (items[k].words) {
+        HIPCHK(x, hipMemcpyAsync(d_ev.p + base, items[k].words, sizeof(uint32_t) * E[k], hipMemcpyHostToDevice, s));
+        base += E[k];
+        continue;
+      }
       for (int64_t b0 = 0; b0 < E[k]; b0 += CH) {
         const int64_t n = std::min(CH, E[k] - b0);
         HIPCHK(x, hipMemcpyAsync(st.p, items[k].time + b0, n * sizeof(int64_t), hipMemcpyHostToDevice, s));
@@ -1675,20 +1681,21 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
   HIPCHK(x, launch_zero_u64(d_cnt.p, (int64_t)2 * G, s));
   float kernel_ms_total = 0.f;
   std::vector<int64_t> so(N + 1);
-  std::vector<int64_t> st_time;
-  std::vector<uint8_t> st_val;
+  std::vector<uint32_t> st_words;  // the slice packed as event words on the host (4 B per event uploaded)
+  const uint8_t* val = tr->event_value;
   for (const auto& iv : ivals) {
     slice_bounds(iv.first, iv.second);
     so[0] = 0;
     for (int64_t n = 0; n < N; ++n) so[n + 1] = so[n] + (se[n] - sb[n]);
-    st_time.resize(std::max<int64_t>(so[N], 1));
-    st_val.resize(std::max<int64_t>(so[N], 1));
+    st_words.resize(std::max<int64_t>(so[N], 1));
     parallel_for(N, x->threads, [&](int64_t n) {
-      std::memcpy(st_time.data() + so[n], tm + sb[n], sizeof(int64_t) * (se[n] - sb[n]));
-      std::memcpy(st_val.data() + so[n], tr->event_value + sb[n], se[n] - sb[n]);
+      uint32_t* d = st_words.data() + so[n];
+      for (int64_t i = sb[n]; i < se[n]; ++i)
+        *d++ = ((uint32_t)tm[i] << EV_SHIFT) | (uint32_t)(val[i] & 3u);
     });
-    std::vector<RunItem> items{RunItem{so.data(), st_time.data(), st_val.data(), tr->t_max, C, n_tile, iv.first,
-                                       iv.second, 0}};
+    RunItem item{so.data(), nullptr, nullptr, tr->t_max, C, n_tile, iv.first, iv.second, 0};
+    item.words = st_words.data();
+    std::vector<RunItem> items{item};
     float ms = 0.f;
     const gtp_status st = run_load(x, mp, items, d_acc, d_cnt, d_cyc, slot_base, &ms);
     if (st != GTP_OK) return st;

@@ -8,7 +8,7 @@
 //                          binning in LDS and per-gate sums, all int64 fixed point
 //   cycle_reduce_kernel    per-cycle sum over gate ranges -> FP64 waveforms
 //   gate_reduce_kernel     per-gate sum over tiles (int64) -> FP64 per-gate power
-//   glitch_annotate_kernel clock-cycle glitch flags and pulse widths, once per net event
+//   glitch_annotate_kernel clock-cycle glitch flags, once per net event
 //   power_cell_kernel<NP>  the fused kernel specialised per single-output cell type without `when`
 //   total_*_kernel         FP64 design and class totals
 //
