@@ -1628,7 +1628,7 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
   // run; event 0 keeps index 0 when the slice starts there) to the second
   // event at or after the interval's end (the flag of the first one).
   std::vector<int64_t> sb(N), se(N);
-  auto slice_bounds = [&](int32_t k0, int32_t k1) {
+  auto slice_bounds = [&](int32_t k0, int32_t k1, std::vector<int64_t>& sb, std::vector<int64_t>& se) {
     const int64_t t0 = (int64_t)k0 * TC * T;
     const int64_t t1 = k1 >= n_tile ? INT64_MAX : (int64_t)k1 * TC * T;
     int64_t total = 0;
@@ -1660,7 +1660,7 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
     for (int32_t k0 = 0; k0 < n_tile;) {
       int32_t step = std::max(1, std::min(step0, n_tile - k0));
       while (true) {
-        const int64_t need = slice_bounds(k0, k0 + step);
+        const int64_t need = slice_bounds(k0, k0 + step, sb, se);
         if (need <= budget) break;
         if (step == 1)
           return x->fail(GTP_ERR_OOM, "gtp_run_sliced: one tile of " + std::to_string(TC) + " cycles needs " +
@@ -1680,25 +1680,45 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
   HIPCHK(x, launch_zero_u64(d_acc.p, (int64_t)N_PG * G, s));
   HIPCHK(x, launch_zero_u64(d_cnt.p, (int64_t)2 * G, s));
   float kernel_ms_total = 0.f;
-  std::vector<int64_t> so(N + 1);
-  std::vector<uint32_t> st_words;  // the slice packed as event words on the host (4 B per event uploaded)
+  // One interval's slices packed on the host as event words (4 B per event
+  // uploaded); two buffers alternate, so that the next interval is sliced and
+  // packed on a helper thread while the current one is uploaded and evaluated.
+  struct Packed {
+    std::vector<int64_t> sb, se, so;
+    std::vector<uint32_t> words;
+  };
+  Packed pk[2];
   const uint8_t* val = tr->event_value;
-  for (const auto& iv : ivals) {
-    slice_bounds(iv.first, iv.second);
-    so[0] = 0;
-    for (int64_t n = 0; n < N; ++n) so[n + 1] = so[n] + (se[n] - sb[n]);
-    st_words.resize(std::max<int64_t>(so[N], 1));
+  auto prepare = [&](Packed& P, const std::pair<int32_t, int32_t>& iv) {
+    P.sb.resize(N);
+    P.se.resize(N);
+    P.so.resize(N + 1);
+    slice_bounds(iv.first, iv.second, P.sb, P.se);
+    P.so[0] = 0;
+    for (int64_t n = 0; n < N; ++n) P.so[n + 1] = P.so[n] + (P.se[n] - P.sb[n]);
+    P.words.resize(std::max<int64_t>(P.so[N], 1));
     parallel_for(N, x->threads, [&](int64_t n) {
-      uint32_t* d = st_words.data() + so[n];
-      for (int64_t i = sb[n]; i < se[n]; ++i)
-        *d++ = ((uint32_t)tm[i] << EV_SHIFT) | (uint32_t)(val[i] & 3u);
+      uint32_t* d = P.words.data() + P.so[n];
+      for (int64_t i = P.sb[n]; i < P.se[n]; ++i) *d++ = ((uint32_t)tm[i] << EV_SHIFT) | (uint32_t)(val[i] & 3u);
     });
-    RunItem item{so.data(), nullptr, nullptr, tr->t_max, C, n_tile, iv.first, iv.second, 0};
-    item.words = st_words.data();
+  };
+  prepare(pk[0], ivals[0]);
+  for (size_t i = 0; i < ivals.size(); ++i) {
+    Packed& P = pk[i & 1];
+    std::thread next;
+    if (i + 1 < ivals.size()) next = std::thread([&, i] { prepare(pk[(i + 1) & 1], ivals[i + 1]); });
+    struct Join {
+      std::thread& t;
+      ~Join() {
+        if (t.joinable()) t.join();
+      }
+    } join{next};
+    RunItem item{P.so.data(), nullptr, nullptr, tr->t_max, C, n_tile, ivals[i].first, ivals[i].second, 0};
+    item.words = P.words.data();
     std::vector<RunItem> items{item};
     float ms = 0.f;
     const gtp_status st = run_load(x, mp, items, d_acc, d_cnt, d_cyc, slot_base, &ms);
-    if (st != GTP_OK) return st;
+    if (st != GTP_OK) return st;  // (the helper thread is joined first)
     kernel_ms_total += ms;
   }
   {
