@@ -1628,7 +1628,7 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
   // run; event 0 keeps index 0 when the slice starts there) to the second
   // event at or after the interval's end (the flag of the first one).
   std::vector<int64_t> sb(N), se(N);
-  auto slice_bounds = [&](int32_t k0, int32_t k1, std::vector<int64_t>& sb, std::vector<int64_t>& se) {
+  auto slice_bounds = [&](int32_t k0, int32_t k1, std::vector<int64_t>& lo, std::vector<int64_t>& hi) {
     const int64_t t0 = (int64_t)k0 * TC * T;
     const int64_t t1 = k1 >= n_tile ? INT64_MAX : (int64_t)k1 * TC * T;
     int64_t total = 0;
@@ -1645,9 +1645,9 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
         st = std::min(i0 - 2, f - 1);
         if (st < a) st = a;
       }
-      sb[n] = st;
-      se[n] = std::min(b, i1 + 2);
-      tot += se[n] - sb[n];
+      lo[n] = st;
+      hi[n] = std::min(b, i1 + 2);
+      tot += hi[n] - lo[n];
     });
     total = tot.load();
     return total;

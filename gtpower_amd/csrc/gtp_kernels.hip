@@ -892,11 +892,13 @@ __global__ __launch_bounds__(256) void glitch_annotate_kernel(const int64_t* __r
 }
 #undef AN_SLOT
 
-// The same for the events of listed tiles only: the 8 threads of group
+// The same for the events of listed tiles only: the AT_LANES threads of group
 // (n, l) annotate, interleaved, the events of net n from the one before
 // idx_begin[n][l] up to the first event at or after the end of tile
 // tile_list[l], both included: the power kernels read the glitch flags of
 // the events next to a tile's own.
+constexpr int AT_LANES = 8;  // threads per (net, listed tile): 2 and 16 measured slower
+
 __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_t* __restrict__ net_off,
                                                                     uint32_t* __restrict__ ev,
                                                                     const int32_t* __restrict__ idx_begin,
@@ -904,17 +906,17 @@ __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_
                                                                     int32_t n_list, int64_t n_net, int64_t tile_ticks,
                                                                     uint32_t T, float invT) {
   // threads stride over the (net, tile, lane) triples under a capped grid
-  const int64_t n_thr = n_net * n_list * 8;
+  const int64_t n_thr = n_net * n_list * AT_LANES;
   for (int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; tid < n_thr;
        tid += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = tid >> 3;
-    const int j = (int)(tid & 7);
+    const int64_t i = tid / AT_LANES;
+    const int j = (int)(tid % AT_LANES);
     const int64_t n = i / n_list;
     const int64_t b = net_off[n];
     const int64_t cnt = net_off[n + 1] - b;
     const int64_t tend = ((int64_t)tile_list[i - n * n_list] + 1) * tile_ticks;
     uint32_t* e = ev + b;
-    for (int64_t k = (int64_t)max(idx_begin[i] - 1, 0) + j; k < cnt; k += 8) {
+    for (int64_t k = (int64_t)max(idx_begin[i] - 1, 0) + j; k < cnt; k += AT_LANES) {
       if ((int64_t)ev_time(e[k]) >= tend && k > 0 && (int64_t)ev_time(e[k - 1]) >= tend) break;
       const uint32_t pw = annotate_one(e, k, cnt, T, invT);
       const uint32_t w = e[k];
@@ -1347,36 +1349,54 @@ __global__ void cycle_reduce_kernel(const long long* __restrict__ part, int32_t 
 // cycle_reduce_kernel over the tiles of a list only (partials indexed by list
 // position): output column l * tile_cycles + i is cycle
 // tile_list[l] * tile_cycles + i (0 past the end)
-__global__ void cycle_reduce_list_kernel(const long long* __restrict__ part, int32_t n_range,
-                                         int32_t tile_cycles, const int32_t* __restrict__ tile_list,
-                                         int32_t n_list, int64_t n_cycle, int64_t T, int64_t t_max,
-                                         double q_energy, double q_leak, double inv_period,
-                                         double* __restrict__ out /* [5][n_list * tile_cycles] */) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// The tile lists of a judge are short (about a thousand output cycles), so
+// each output cycle's sum over the gate ranges is split over RL_GROUPS
+// threads (exact int64 partial sums, combined in LDS).
+constexpr int RL_CYCLES = 32, RL_GROUPS = 8;  // a block: 32 output cycles x 8 range groups
+
+__global__ __launch_bounds__(RL_CYCLES * RL_GROUPS) void cycle_reduce_list_kernel(
+    const long long* __restrict__ part, int32_t n_range, int32_t tile_cycles, const int32_t* __restrict__ tile_list,
+    int32_t n_list, int64_t n_cycle, int64_t T, int64_t t_max, double q_energy, double q_leak, double inv_period,
+    double* __restrict__ out /* [5][n_list * tile_cycles] */) {
+  __shared__ long long s_part[RL_GROUPS][N_BIN][RL_CYCLES];
+  const int cl = threadIdx.x % RL_CYCLES, grp = threadIdx.x / RL_CYCLES;
+  const int64_t j = (int64_t)blockIdx.x * RL_CYCLES + cl;
   const int64_t n_out = (int64_t)n_list * tile_cycles;
-  if (j >= n_out) return;
-  const int64_t l = j / tile_cycles;
-  const int64_t tile = tile_list[l];
-  const int ci = (int)(j % tile_cycles);
-  const int64_t c = tile * tile_cycles + ci;
+  const bool live = j < n_out;
+  const int64_t l = live ? j / tile_cycles : 0;
+  const int ci = live ? (int)(j % tile_cycles) : 0;
+  const int64_t c = (int64_t)tile_list[l] * tile_cycles + ci;
+  long long sk[N_BIN] = {0, 0, 0, 0, 0, 0};
+  if (live && c < n_cycle) {
+    for (int r = grp; r < n_range; r += RL_GROUPS) {
+      const long long* p = part + ((l * n_range + r) * N_BIN) * tile_cycles + ci;
+#pragma unroll
+      for (int k = 0; k < N_BIN; ++k) sk[k] += p[(int64_t)k * tile_cycles];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < N_BIN; ++k) s_part[grp][k][cl] = sk[k];
+  __syncthreads();
+  if (grp != 0 || !live) return;
   if (c >= n_cycle) {
 #pragma unroll
     for (int k = 0; k < 5; ++k) out[k * n_out + j] = 0.0;
     return;
   }
-  long long s[N_BIN] = {0, 0, 0, 0, 0, 0};
-  for (int r = 0; r < n_range; ++r) {
-    const long long* p = part + ((l * n_range + r) * N_BIN) * tile_cycles + ci;
+  long long sum[N_BIN];
 #pragma unroll
-    for (int k = 0; k < N_BIN; ++k) s[k] += p[(int64_t)k * tile_cycles];
+  for (int k = 0; k < N_BIN; ++k) {
+    sum[k] = 0;
+#pragma unroll
+    for (int g = 0; g < RL_GROUPS; ++g) sum[k] += s_part[g][k][cl];
   }
   const int64_t cend = min((c + 1) * T, t_max);
   const double frac = (double)(cend - c * T) / (double)T;
-  out[0 * n_out + j] = ((double)s[BIN_LD] * frac + (double)s[BIN_LW]) * q_leak;
-  out[1 * n_out + j] = (double)(s[BIN_INT] + s[BIN_GINT]) * q_energy * inv_period;
-  out[2 * n_out + j] = (double)s[BIN_GINT] * q_energy * inv_period;
-  out[3 * n_out + j] = (double)(s[BIN_SW] + s[BIN_GSW]) * q_energy * inv_period;
-  out[4 * n_out + j] = (double)s[BIN_GSW] * q_energy * inv_period;
+  out[0 * n_out + j] = ((double)sum[BIN_LD] * frac + (double)sum[BIN_LW]) * q_leak;
+  out[1 * n_out + j] = (double)(sum[BIN_INT] + sum[BIN_GINT]) * q_energy * inv_period;
+  out[2 * n_out + j] = (double)sum[BIN_GINT] * q_energy * inv_period;
+  out[3 * n_out + j] = (double)(sum[BIN_SW] + sum[BIN_GSW]) * q_energy * inv_period;
+  out[4 * n_out + j] = (double)sum[BIN_GSW] * q_energy * inv_period;
 }
 
 // Batched runs: per-cycle outputs of the work tiles of one round.  Column
@@ -1578,7 +1598,7 @@ hipError_t launch_cycle_reduce_list(const long long* part, int32_t n_range, int3
                                     double* out, hipStream_t s) {
   const int64_t n = (int64_t)n_list * tile_cycles;
   if (n <= 0) return hipSuccess;
-  cycle_reduce_list_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(
+  cycle_reduce_list_kernel<<<dim3((unsigned)((n + RL_CYCLES - 1) / RL_CYCLES)), dim3(RL_CYCLES * RL_GROUPS), 0, s>>>(
       part, n_range, tile_cycles, tile_list, n_list, n_cycle, T, t_max, q_energy, q_leak, inv_period, out);
   return hipGetLastError();
 }
@@ -1671,7 +1691,7 @@ hipError_t launch_glitch_annotate(const int64_t* net_off, uint32_t* ev, const in
 hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, uint32_t* ev, const int32_t* idx_begin,
                                         const int32_t* tile_list, int32_t n_list, int64_t n_net, int64_t tile_ticks,
                                         uint32_t T, float invT, hipStream_t s) {
-  const int64_t n = n_net * n_list * 8;
+  const int64_t n = n_net * n_list * AT_LANES;
   if (n <= 0) return hipSuccess;
   glitch_annotate_tiles_kernel<<<dim3((unsigned)stride_blocks((n + 255) / 256)), dim3(256), 0, s>>>(
       net_off, ev, idx_begin, tile_list, n_list, n_net, tile_ticks, T, invT);
