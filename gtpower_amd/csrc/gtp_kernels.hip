@@ -1218,14 +1218,14 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
       {
         const float es = is_out ? 0.5f * (float)r2 * csw : 0.f;
         const long long qs = qterm(gl ? ratio * es : es, inv_qe);
+        acc[PG_SW] += qs;  // all switching; the glitch part is taken out after the loop
         acc[PG_GSW] += gl ? qs : 0;
-        acc[PG_SW] += gl ? 0 : qs;
         atomicAdd((unsigned long long*)&cb[(gl ? BIN_GSW : BIN_SW) * TCP], (unsigned long long)qs);
       }
       const float e = En * (0.5f * (float)tx2);
       const long long qi = qterm(gl ? ratio * e : e, inv_qe);
+      acc[PG_INT] += qi;  // all internal; the glitch part is taken out after the loop
       acc[PG_GINT] += gl ? qi : 0;
-      acc[PG_INT] += gl ? 0 : qi;
       atomicAdd((unsigned long long*)&cb[(gl ? BIN_GINT : BIN_INT) * TCP], (unsigned long long)qi);
 
       // consume the event of pin ps: the next word becomes current, and the
@@ -1282,6 +1282,8 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
       }
     }
     if (lend > prev_t) acc[PG_LEAK] += (long long)(int)lq_cur * (long long)(lend - prev_t);
+    acc[PG_SW] -= acc[PG_GSW];
+    acc[PG_INT] -= acc[PG_GINT];
     const int64_t slot = (int64_t)wt * a.n_sched + gi;
     const int64_t plane = (int64_t)a.n_work_tiles * a.n_sched;
 #pragma unroll
