@@ -54,6 +54,14 @@ void parallel_for(int64_t n, int threads, F f) {
   for (auto& th : pool) th.join();
 }
 
+// Joins a helper thread when the scope ends, on every return path.
+struct JoinGuard {
+  std::thread& t;
+  ~JoinGuard() {
+    if (t.joinable()) t.join();
+  }
+};
+
 struct HostCell {
   int n_pin = 0, n_in = 0, cls = 0, sbits = 0;
   uint32_t clock_mask = 0;
@@ -604,20 +612,35 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
   x->h_net_off.assign(off, off + N + 1);
   x->glitch_valid = false;
   HIPCHK(x, x->d_ev.alloc(E + EV_PAD));
-  // chunked upload + pack into the 32-bit event store
+  // The event words are packed on the host a chunk at a time (4 B per event
+  // uploaded instead of 9), the next chunk on a helper thread while the
+  // current one is copied; two buffers alternate.
   {
     const int64_t CH = 1 << 26;
-    DArr<int64_t> st;
-    DArr<uint8_t> sv;
-    HIPCHK(x, st.alloc(std::min<int64_t>(CH, std::max<int64_t>(E, 1))));
-    HIPCHK(x, sv.alloc(std::min<int64_t>(CH, std::max<int64_t>(E, 1))));
-    for (int64_t b = 0; b < E; b += CH) {
+    std::vector<uint32_t> buf[2];
+    const int64_t* tm = tr->event_time;
+    const uint8_t* val = tr->event_value;
+    auto pack = [&](int which, int64_t b) {
       const int64_t n = std::min(CH, E - b);
-      HIPCHK(x, hipMemcpyAsync(st.p, tr->event_time + b, n * sizeof(int64_t), hipMemcpyHostToDevice, s));
-      HIPCHK(x, hipMemcpyAsync(sv.p, tr->event_value + b, n, hipMemcpyHostToDevice, s));
-      HIPCHK(x, launch_pack(st.p, sv.p, x->d_ev.p + b, n, s));
+      buf[which].resize(n);
+      uint32_t* d = buf[which].data();
+      parallel_for((n + 65535) / 65536, x->threads, [&](int64_t blk) {
+        const int64_t i1 = std::min(n, (blk + 1) * 65536);
+        for (int64_t i = blk * 65536; i < i1; ++i)
+          d[i] = ((uint32_t)tm[b + i] << EV_SHIFT) | (uint32_t)(val[b + i] & 3u);
+      });
+    };
+    if (E > 0) pack(0, 0);
+    int k = 0;
+    for (int64_t b = 0; b < E; b += CH, ++k) {
+      const int64_t n = std::min(CH, E - b);
+      std::thread next;
+      if (b + CH < E) next = std::thread([&, b, k] { pack((k + 1) & 1, b + CH); });
+      JoinGuard join{next};
+      HIPCHK(x, hipMemcpyAsync(x->d_ev.p + b, buf[k & 1].data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+      // the buffer is packed again two chunks later
+      HIPCHK(x, hipStreamSynchronize(s));
     }
-    HIPCHK(x, hipStreamSynchronize(s));
   }
   return finish_trace(x);
 }
@@ -1707,12 +1730,7 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
     Packed& P = pk[i & 1];
     std::thread next;
     if (i + 1 < ivals.size()) next = std::thread([&, i] { prepare(pk[(i + 1) & 1], ivals[i + 1]); });
-    struct Join {
-      std::thread& t;
-      ~Join() {
-        if (t.joinable()) t.join();
-      }
-    } join{next};
+    JoinGuard join{next};
     RunItem item{P.so.data(), nullptr, nullptr, tr->t_max, C, n_tile, ivals[i].first, ivals[i].second, 0};
     item.words = P.words.data();
     std::vector<RunItem> items{item};
