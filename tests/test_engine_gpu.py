@@ -320,3 +320,41 @@ def test_autotune_keeps_results_and_feeds_batches():
         assert np.array_equal(r.cycle_power, ref.cycle_power)
         assert np.array_equal(r.gate_power, ref.gate_power)
         assert np.array_equal(r.gate_toggle_x2, ref.gate_toggle_x2)
+
+
+def test_silent_trace_and_a_single_partial_cycle():
+    """A trace with no events at all (leakage of the default states only, an
+    empty event store, no annotation chunks) and one shorter than a cycle,
+    resident, sliced and batched, against the oracle."""
+    print("[parity] silent trace and single partial cycle", flush=True)
+    d = S.make_design(300, dff_fraction=0.1, clock_ns=1.0, seed=41)
+    tr = d.trace(20, toggle_p=0.3, seed=42)
+    T = tr.ticks_per_cycle
+    N = len(tr.net_event_offset) - 1
+    silent = S.ArrayTrace(np.zeros(N + 1, np.int64), np.zeros(0, np.int64), np.zeros(0, np.uint8),
+                          20 * T, tr.time_scale, tr.clk_period, T)
+    short = S.slice_trace(tr, 1)
+    short = S.ArrayTrace(short.net_event_offset, short.event_time, short.event_value, T // 2 + 7,
+                         tr.time_scale, tr.clk_period, T)
+    # keep only the events before the shortened end (inclusive, as at t_max)
+    off = short.net_event_offset
+    keep = short.event_time <= short.t_max
+    counts = np.array([keep[off[n]:off[n + 1]].sum() for n in range(N)])
+    noff = np.zeros(N + 1, np.int64)
+    noff[1:] = np.cumsum(counts)
+    short = S.ArrayTrace(noff, short.event_time[keep], short.event_value[keep], short.t_max,
+                         tr.time_scale, tr.clk_period, T)
+    for name, t in (("silent", silent), ("single partial cycle", short)):
+        res, ref = run_both(d, t, tile_cycles=4)
+        compare(res, ref, name)
+        with HipPower(tile_cycles=4) as eng:
+            eng.set_design(d.library, d)
+            eng.read_power_activities(t)
+            full = eng.power()
+            sl = eng.power_sliced(t, max_events=max(t.n_event, 1) + 4 * d.n_net)
+            both = eng.power_batch([t, tr])
+        for other, what in ((sl, "sliced"), (both[0], "batched")):
+            assert np.array_equal(other.gate_glitch, full.gate_glitch), f"{name} {what}"
+            assert np.array_equal(other.gate_toggle_x2, full.gate_toggle_x2), f"{name} {what}"
+            assert np.array_equal(other.cycle_power, full.cycle_power), f"{name} {what}"
+    assert res.n_cycle == 1
