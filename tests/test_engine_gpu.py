@@ -358,3 +358,44 @@ def test_silent_trace_and_a_single_partial_cycle():
             assert np.array_equal(other.gate_toggle_x2, full.gate_toggle_x2), f"{name} {what}"
             assert np.array_equal(other.cycle_power, full.cycle_power), f"{name} {what}"
     assert res.n_cycle == 1
+
+
+def test_invalid_traces_are_rejected_and_the_context_stays_usable():
+    """Each malformed trace is refused with an error status (never a crash),
+    by the resident and the sliced entry points; a valid trace works after."""
+    print("[errors] malformed traces", flush=True)
+    d = S.make_design(60, dff_fraction=0.1, clock_ns=1.0, seed=51)
+    tr = d.trace(10, toggle_p=0.3, seed=52)
+    T = tr.ticks_per_cycle
+    off, tm, val = tr.net_event_offset, tr.event_time, tr.event_value
+    n_busy = int(np.argmax(np.diff(off) >= 2))  # a net with at least two events
+
+    def variant(off=off, tm=tm, val=val, t_max=tr.t_max, clk=tr.clk_period):
+        return S.ArrayTrace(off.copy(), tm.copy(), val.copy(), t_max, tr.time_scale, clk, T)
+
+    cases = {}
+    o = off.copy(); o[0] = 1
+    cases["offset 0 not zero"] = variant(off=o)
+    o = off.copy(); o[n_busy + 1] = o[n_busy] - 1 if o[n_busy] > 0 else -1
+    cases["decreasing offsets"] = variant(off=o)
+    t = tm.copy(); a = off[n_busy]; t[a], t[a + 1] = t[a + 1], t[a]
+    cases["unsorted times"] = variant(tm=t)
+    t = tm.copy(); t[off[n_busy]] = -5
+    cases["negative time"] = variant(tm=t)
+    t = tm.copy(); t[off[n_busy + 1] - 1] = tr.t_max + 1
+    cases["time past t_max"] = variant(tm=t)
+    v = val.copy(); v[off[n_busy]] = 7
+    cases["value above 3"] = variant(val=v)
+    cases["t_max beyond 2^29 - 1 ticks"] = variant(t_max=(1 << 29) + 5)
+    cases["t_max zero"] = variant(t_max=0)
+    with HipPower() as eng:
+        eng.set_design(d.library, d)
+        for name, bad in cases.items():
+            with pytest.raises(GtpError):
+                eng.read_power_activities(bad)
+            with pytest.raises(GtpError):
+                eng.power_sliced(bad, max_events=10 ** 6)
+        eng.read_power_activities(tr)
+        res = eng.power()
+    ref = oracle.run(d.library, d, tr, threads=0)
+    compare(res, ref, "after rejected traces")
