@@ -762,7 +762,8 @@ __device__ __forceinline__ void an_fetch(const uint32_t* e, int64_t left, bool f
 // One chunk's state in a wave's pipeline.
 #define AN_SLOT(X) uint32_t x##X[7]; uint32_t* e##X = nullptr; int64_t left##X = 0, nf##X = -1; bool first##X = false
 
-__global__ __launch_bounds__(256) void glitch_annotate_kernel(const int64_t* __restrict__ net_off,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void glitch_annotate_kernel(
+    const int64_t* __restrict__ net_off,
                                                               uint32_t* __restrict__ ev,
                                                               const int64_t* __restrict__ chunk_off,
                                                               const int32_t* __restrict__ chunk_net,
