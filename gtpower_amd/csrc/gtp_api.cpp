@@ -1175,7 +1175,7 @@ GTP_API gtp_status gtp_run_tiles(gtp_ctx* x, int32_t tile_cycles, int32_t n_list
   // glitch annotation of the listed tiles' events, unless the whole trace is annotated
   if (!x->glitch_valid)
     HIPCHK(x, launch_glitch_annotate_tiles(x->d_net_off.p, x->d_ev.p, x->d_tile_idx_sub.p, x->d_tile_list.p, n_list,
-                                           x->N, rp.TC * x->T, (uint32_t)x->T, rp.a.inv_T, s));
+                                           x->N, rp.TC * x->T, (uint32_t)x->T, s));
   rp.a.tile_idx = x->d_tile_idx_sub.p;
   rp.a.tile_list = x->d_tile_list.p;
   rp.a.n_work_tiles = n_list;

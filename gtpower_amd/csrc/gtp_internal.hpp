@@ -239,7 +239,7 @@ hipError_t launch_glitch_annotate(const int64_t* net_off, uint32_t* ev, const in
                                   const int32_t* chunk_net, int64_t n_chunk, uint32_t T, hipStream_t s);
 hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, uint32_t* ev, const int32_t* idx_begin,
                                         const int32_t* tile_list, int32_t n_list, int64_t n_net, int64_t tile_ticks,
-                                        uint32_t T, float invT, hipStream_t s);
+                                        uint32_t T, hipStream_t s);
 hipError_t launch_total(const double* gate_power, const int32_t* gate_class, int64_t n_gate, double* part,
                         double* out, hipStream_t s);
 hipError_t launch_power_cell(int np, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,

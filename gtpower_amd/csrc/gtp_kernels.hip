@@ -647,10 +647,11 @@ __device__ __forceinline__ int group_pos(const uint32_t* ev, int64_t i, uint32_t
 }
 
 // glitch pulse width of event i (of cnt) of one net's event list e (0: none)
-__device__ __forceinline__ uint32_t annotate_one(const uint32_t* e, int64_t i, int64_t cnt, uint32_t T, float invT) {
+__device__ __forceinline__ uint32_t annotate_one(const uint32_t* e, int64_t i, int64_t cnt, uint32_t T, uint32_t cyc_m,
+                                                 int cyc_sh) {
   if (i < 1) return 0;
   const uint32_t t = ev_time(e[i]);
-  const uint32_t c = cycle_of(t, T, invT);
+  const uint32_t c = cycle_mul(t, cyc_m, cyc_sh);
   const uint32_t cs = c * T, ce = cs + T;
   const int j = group_pos(e, i, cs);
   const bool has1 = i + 1 < cnt;
@@ -664,7 +665,7 @@ __device__ __forceinline__ uint32_t annotate_one(const uint32_t* e, int64_t i, i
       fprev = true;  // same group and not its last event
     } else {
       const uint32_t tp = ev_time(e[i - 1]);
-      const uint32_t cps = cycle_of(tp, T, invT) * T;
+      const uint32_t cps = cycle_mul(tp, cyc_m, cyc_sh) * T;
       fprev = group_pos(e, i - 1, cps) & 1;  // last of its group: flagged iff at an odd position
     }
     if (fprev) pw = t - ev_time(e[i - 1]);
@@ -677,7 +678,7 @@ __device__ __forceinline__ uint32_t annotate_one(const uint32_t* e, int64_t i, i
       const bool last1 = !has2 || t2 >= ce;
       fnext = !last1 || ((j + 1) & 1);
     } else {
-      const uint32_t c1e = (cycle_of(t1, T, invT) + 1) * T;
+      const uint32_t c1e = (cycle_mul(t1, cyc_m, cyc_sh) + 1) * T;
       fnext = has2 && t2 < c1e;
     }
     if (fnext) pw = max(pw, (int64_t)(t1 - t));
@@ -905,7 +906,7 @@ __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_
                                                                     const int32_t* __restrict__ idx_begin,
                                                                     const int32_t* __restrict__ tile_list,
                                                                     int32_t n_list, int64_t n_net, int64_t tile_ticks,
-                                                                    uint32_t T, float invT) {
+                                                                    uint32_t T, uint32_t cyc_m, int cyc_sh) {
   // threads stride over the (net, tile, lane) triples under a capped grid
   const int64_t n_thr = n_net * n_list * AT_LANES;
   for (int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; tid < n_thr;
@@ -919,7 +920,7 @@ __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_
     uint32_t* e = ev + b;
     for (int64_t k = (int64_t)max(idx_begin[i] - 1, 0) + j; k < cnt; k += AT_LANES) {
       if ((int64_t)ev_time(e[k]) >= tend && k > 0 && (int64_t)ev_time(e[k - 1]) >= tend) break;
-      const uint32_t pw = annotate_one(e, k, cnt, T, invT);
+      const uint32_t pw = annotate_one(e, k, cnt, T, cyc_m, cyc_sh);
       const uint32_t w = e[k];
       const uint32_t nw = pw ? (w | EV_GLITCH) : (w & ~EV_GLITCH);
       if (nw != w) e[k] = nw;
@@ -1693,11 +1694,14 @@ hipError_t launch_glitch_annotate(const int64_t* net_off, uint32_t* ev, const in
 
 hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, uint32_t* ev, const int32_t* idx_begin,
                                         const int32_t* tile_list, int32_t n_list, int64_t n_net, int64_t tile_ticks,
-                                        uint32_t T, float invT, hipStream_t s) {
+                                        uint32_t T, hipStream_t s) {
   const int64_t n = n_net * n_list * AT_LANES;
   if (n <= 0) return hipSuccess;
+  uint32_t cyc_m = 0;
+  int32_t cyc_sh = 0;
+  cycle_magic(T, &cyc_m, &cyc_sh);
   glitch_annotate_tiles_kernel<<<dim3((unsigned)stride_blocks((n + 255) / 256)), dim3(256), 0, s>>>(
-      net_off, ev, idx_begin, tile_list, n_list, n_net, tile_ticks, T, invT);
+      net_off, ev, idx_begin, tile_list, n_list, n_net, tile_ticks, T, cyc_m, cyc_sh);
   return hipGetLastError();
 }
 
