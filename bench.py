@@ -243,6 +243,8 @@ def main():
     ap.add_argument("--gates", type=int, default=None)
     ap.add_argument("--cycles", type=int, default=None)
     ap.add_argument("--cpu-sample-cycles", type=int, default=20000)
+    ap.add_argument("--cpu-sample-events", type=float, default=1e9,
+                    help="bound on the CPU sample's events (about 15 s of oracle work on 16 threads)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-cycles", type=int, default=0)
@@ -352,7 +354,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         import oracle
-        sample = min(args.cpu_sample_cycles, n_cycle)
+        # the first cycles of the same workload, bounded in cycles and in events
+        by_events = max(100, int(args.cpu_sample_events / max(n_event / n_cycle, 1.0)))
+        sample = min(args.cpu_sample_cycles, n_cycle, by_events)
         tseed = S.MASTER_SEED ^ (w["seed"] * 0x9E3779B1)
         tk = dict(w["trace"])
         tk["n_cycle"] = sample
