@@ -939,8 +939,8 @@ __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_
 // Semantics are those of power_tile_kernel.
 // ---------------------------------------------------------------------------
 // Occupancy targets by pin count (waves per SIMD): 5 for 2 pins (a few spills
-// outside the step loop; 12% faster than 4), 4 for 3 pins (5 spills inside the
-// loop and is 1.8x slower), 3 for 4 pins, 2 above.
+// outside the step loop; 12% faster than 4), 4 for 3 pins (at 5 it spills in
+// the step loop and runs 1.8x slower), 3 for 4 pins, 2 above.
 template <int NP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NP <= 2 ? 5 : (NP == 3 ? 4 : (NP == 4 ? 3 : 2)))))
 void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, int32_t cell, unsigned int* wq) {
