@@ -266,7 +266,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path shards by gate or by "
-              "cycle range across GPUs, which this build does not run.", file=sys.stderr)
+              "cycle range across GPUs, which this engine does not implement.", file=sys.stderr)
         sys.exit(2)
 
     from gtpower_amd import synthetic as S

@@ -42,8 +42,8 @@ enum {
   GTP_ERR_DEVICE = -2,      /* a HIP runtime call failed */
   GTP_ERR_OOM = -3,         /* device or host allocation failed, or the event budget is exceeded */
   GTP_ERR_STATE = -4,       /* call order violated (e.g. run before set_trace) */
-  GTP_ERR_UNSUPPORTED = -5, /* valid input outside what this build implements */
-  GTP_ERR_LIMIT = -6        /* a size limit of this build is exceeded */
+  GTP_ERR_UNSUPPORTED = -5, /* valid input outside what this library implements */
+  GTP_ERR_LIMIT = -6        /* a size limit of this library is exceeded */
 };
 
 /* Power components, in the column order of the reference's cell report
