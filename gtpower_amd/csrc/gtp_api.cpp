@@ -479,6 +479,17 @@ GTP_API gtp_status gtp_set_design(gtp_ctx* x, const gtp_library* L, const gtp_de
   x->arc_delay.assign(d->arc_delay, d->arc_delay + A);
   x->csw_max = csw_max;
   x->arc_q_scale = 0;
+  // inputs with an arc to the output in some gate of each single-output cell
+  // (the related-input search skips the others for the whole cell type)
+  for (CellDev& cd : x->cell_dev) cd.arc_mask = cd.n_pin - cd.n_in == 1 ? 0u : ~0u;
+  for (int64_t g = 0; g < G; ++g) {
+    CellDev& cd = x->cell_dev[d->gate_cell[g]];
+    if (cd.n_pin - cd.n_in != 1) continue;
+    const double* ad = d->arc_delay + d->gate_arc_offset[g];
+    for (int i = 0; i < cd.n_in; ++i)
+      for (int k = 0; k < 4; ++k)
+        if (!std::isnan(ad[i * 4 + k])) cd.arc_mask |= 1u << i;
+  }
   hipStream_t s = x->stream;
   HIPCHK(x, x->d_cells.upload(x->cell_dev.data(), x->cell_dev.size(), s));
   HIPCHK(x, x->d_leak.upload(x->leak_f.data(), x->leak_f.size(), s));

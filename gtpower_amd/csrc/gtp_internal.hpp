@@ -91,6 +91,7 @@ struct CellDev {
   int32_t leak_off;          // into the LDS leakage table (2^n_pin floats)
   int32_t n_slot;            // energy slots per gate of this cell
   int32_t slot_off[6];       // energy slot base of each pin
+  uint32_t arc_mask;         // bit q: some gate of this cell has an arc from input q (single output)
 };
 
 // One energy slot of a cell: which table to interpolate at which pin's slew.

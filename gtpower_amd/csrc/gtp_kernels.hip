@@ -963,6 +963,7 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
   __shared__ long long s_item;
   const CellDev cd = a.cells[cell];
   const uint32_t clk = cd.clock_mask;
+  const uint32_t arcm = cd.arc_mask;  // inputs that can be the related pin of an output event
   for (int i = threadIdx.x; i < NSTATE; i += blockDim.x) {
     s_leak[i] = a.leak[cd.leak_off + i];
     s_leakq[i] = a.leak_q[cd.leak_off + i];
@@ -1200,6 +1201,7 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
       float En_out = rf ? E[6 * NIN + 1] : E[6 * NIN];  // default when none qualifies
 #pragma unroll
       for (int q = 0; q < NIN; ++q) {
+        if (!((arcm >> q) & 1u)) continue;  // no gate of this cell has an arc from q
         const bool hi_rf = (rflm >> q) & 1u, lo_rf = rf != 0;
         const int32_t d01 = lo_rf ? D[q * 4 + 1] : D[q * 4 + 0];
         const int32_t d23 = lo_rf ? D[q * 4 + 3] : D[q * 4 + 2];
