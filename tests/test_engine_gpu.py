@@ -399,3 +399,27 @@ def test_invalid_traces_are_rejected_and_the_context_stays_usable():
         res = eng.power()
     ref = oracle.run(d.library, d, tr, threads=0)
     compare(res, ref, "after rejected traces")
+
+
+def test_long_clock_period_uses_the_generic_kernel():
+    """100 000 ticks per cycle (100 ns at 1 ps): every cell goes to the generic
+    kernel (the specialised one compares arrivals on 32 bits, below 2^15 ticks
+    per cycle) and the cycle multiply-shift uses its large-period constants;
+    resident, sliced and batched runs against the oracle."""
+    print("[parity] long clock period", flush=True)
+    d = S.make_design(300, dff_fraction=0.1, clock_ns=100.0, seed=61)
+    tr = d.trace(300, toggle_p=0.3, seed=62)
+    assert tr.ticks_per_cycle >= 32768
+    res, ref = run_both(d, tr, tile_cycles=32)
+    compare(res, ref, "long clock period")
+    assert res.gate_glitch.sum() > 0
+    with HipPower(tile_cycles=32) as eng:
+        eng.set_design(d.library, d)
+        eng.read_power_activities(tr)
+        full = eng.power()
+        sl = eng.power_sliced(tr, max_events=tr.n_event // 3 + 4 * d.n_net)
+        both = eng.power_batch([tr, tr])
+    assert sl.schedule["split_target"] > 1
+    for other, what in ((sl, "sliced"), (both[1], "batched")):
+        assert np.array_equal(other.gate_glitch, full.gate_glitch), what
+        assert np.array_equal(other.cycle_power, full.cycle_power), what
