@@ -1627,6 +1627,7 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
   if (!x->have_design) return x->fail(GTP_ERR_STATE, "gtp_run_sliced: call gtp_set_design first");
   if (!tr || !res) return x->fail(GTP_ERR_ARG, "gtp_run_sliced: no trace or result");
   HIPCHK(x, hipSetDevice(x->device));
+  const auto wall0 = std::chrono::steady_clock::now();
   hipStream_t s = x->stream;
   const int64_t N = x->N, G = x->G;
   int64_t T = 0;
@@ -1755,7 +1756,7 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
     if (st != GTP_OK) return st;
   }
   res->kernel_ms = kernel_ms_total;
-  res->total_ms = kernel_ms_total;
+  res->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
   record_multi_schedule(x, mp, E);
   x->sched.split_target = (int32_t)ivals.size();
   return GTP_OK;

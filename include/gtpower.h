@@ -240,7 +240,9 @@ gtp_status gtp_run_batch(gtp_ctx* ctx, int32_t n_trace, const gtp_trace* traces,
  * gtp_run over the whole trace.  The resident trace of gtp_set_trace is not
  * touched.  Replaces the reference's multi-round interval loop
  * (power/cuda/CudaPower.cu:1261-1314, getTimeIntervals / renewGateWaveform
- * :1421-1599); the interval count is reported in gtp_schedule_info.split_target. */
+ * :1421-1599); the interval count is reported in gtp_schedule_info.split_target.
+ * kernel_ms sums the intervals' device time; total_ms is the call's wall time
+ * (host slicing and uploads included). */
 gtp_status gtp_run_sliced(gtp_ctx* ctx, const gtp_trace* trace, int64_t max_events, gtp_result* result);
 
 /* Schedule of the last run: tile length, lanes, workgroups, units. */
