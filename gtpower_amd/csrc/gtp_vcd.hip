@@ -158,8 +158,17 @@ __global__ void parse_tokens_kernel(const uint8_t* __restrict__ t, int64_t L, co
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= K) return;
   const int64_t s = tok[k];
-  int64_t e = s + 1;
-  while (e < L && !is_ws(t[e])) ++e;
+  // the token ends where the whitespace before the next token starts (only
+  // whitespace separates tokens): step back from the next start, usually
+  // over one separator; the last token scans forward
+  int64_t e;
+  if (k + 1 < K) {
+    e = tok[k + 1];
+    while (e > s + 1 && is_ws(t[e - 1])) --e;
+  } else {
+    e = s + 1;
+    while (e < L && !is_ws(t[e])) ++e;
+  }
   const int64_t n = e - s;
   const uint8_t c0 = upper(t[s]);
   uint8_t kd = TK_OTHER;
