@@ -986,6 +986,19 @@ gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, int32_t n_work_tiles, RunPlan
 // The per-cell-group power launches over n_work_tiles tiles (rp.a), run
 // concurrently on the main and the side streams (all default priority): the
 // launches write disjoint outputs.
+// The clock pin whose edges the workgroups take once per work item (-1: the
+// gates step through their clock events): a 3-pin cell whose one clock pin is
+// pin 0, tiles of at most CLOCK_EDGE_TILE_MAX cycles, ranges of fewer than
+// 2^16 gates (16-bit state counts); GTP_CLOCK_EDGES=0 turns it off.
+int clock_edge_pin(const gtp_ctx* x, const gtp_ctx::Launch& L, const RunPlan& rp) {
+  if (const char* e = std::getenv("GTP_CLOCK_EDGES"))
+    if (std::atoi(e) == 0) return -1;
+  const HostCell& hc = x->cells[L.cell];
+  if (hc.n_pin != 3 || hc.clock_mask != 1u) return -1;
+  if (rp.a.tile_cycles > CLOCK_EDGE_TILE_MAX || rp.a.range_gates >= 65536) return -1;
+  return 0;
+}
+
 gtp_status launch_groups(gtp_ctx* x, const RunPlan& rp, bool persistent = false) {
   hipStream_t s = x->stream;
   const std::vector<gtp_ctx::Launch>& all = rp.os->launches;
@@ -1015,8 +1028,8 @@ gtp_status launch_groups(gtp_ctx* x, const RunPlan& rp, bool persistent = false)
       wq = x->d_wq.p + 8 * li;
     }
     if (L.cell >= 0)
-      HIPCHK(x, launch_power_cell(L.np, grid, rp.block, rp.lds_cell, ls, rp.a, L.range_begin, L.range_count, L.cell,
-                                  wq));
+      HIPCHK(x, launch_power_cell(L.np, clock_edge_pin(x, L, rp), grid, rp.block, rp.lds_cell, ls, rp.a, L.range_begin,
+                                  L.range_count, L.cell, wq));
     else
       HIPCHK(x, launch_power(L.np, grid, rp.block, rp.lds, ls, rp.a, L.range_begin, L.range_count, wq));
   }

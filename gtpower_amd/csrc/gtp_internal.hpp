@@ -243,8 +243,12 @@ hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, uint32_t* ev, co
                                         uint32_t T, hipStream_t s);
 hipError_t launch_total(const double* gate_power, const int32_t* gate_class, int64_t n_gate, double* part,
                         double* out, hipStream_t s);
-hipError_t launch_power_cell(int np, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,
+// kc >= 0: the cell's only clock pin (np 3, kc 0), whose edges the workgroup
+// takes once per work item for the gates on the item's clock net.
+hipError_t launch_power_cell(int np, int kc, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,
                              int32_t range_begin, int32_t range_count, int32_t cell, unsigned int* wq);
+// Longest tile (cycles) whose clock edges the workgroup takes.
+constexpr int CLOCK_EDGE_TILE_MAX = 512;
 
 // The multiply-shift pair of PowerArgs.cyc_m / cyc_sh for T ticks per cycle.
 inline void cycle_magic(int64_t T, uint32_t* m, int32_t* sh) {

@@ -941,9 +941,69 @@ __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_
 // Occupancy targets by pin count (waves per SIMD): 5 for 2 pins (a few spills
 // outside the step loop; 12% faster than 4), 4 for 3 pins (at 5 it spills in
 // the step loop and runs 1.8x slower), 3 for 4 pins, 2 above.
-template <int NP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NP <= 2 ? 5 : (NP == 3 ? 4 : (NP == 4 ? 3 : 2)))))
+// Clock edges taken per workgroup (power_cell_kernel<3, KC>, KC >= 0): the
+// edges of one tile of the clock net sit in LDS slots 1..n (slot 0: the state
+// before the tile at the tile start, slots n + 1 and n + 2: EV_INF), so a
+// tile holds at most CK_SLOTS - 3 edges and CK_TILE_MAX cycles.
+constexpr int CK_SLOTS = 1040;
+constexpr int CK_TILE_MAX = 512;
+
+// Inclusive prefix sums of x[0..n] in place (n < blockDim.x * 64), by the
+// whole workgroup; tmp holds one value per wave.
+template <typename V>
+__device__ void block_scan_inplace(V* x, int n, V* tmp) {
+  const int nt = (int)blockDim.x, j = (int)threadIdx.x;
+  const int seg = (n + nt) / nt;
+  const int b = j * seg;
+  V loc = 0;
+  for (int k = 0; k < seg; ++k)
+    if (b + k <= n) loc += x[b + k];
+  V v = loc;
+  const int lane = j & 63, wv = j >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const V y = __shfl_up(v, o, 64);
+    if (lane >= o) v += y;
+  }
+  if (lane == 63) tmp[wv] = v;
+  __syncthreads();
+  V run = v - loc;
+  for (int k = 0; k < wv; ++k) run += tmp[k];
+  for (int k = 0; k < seg; ++k)
+    if (b + k <= n) {
+      run += x[b + k];
+      x[b + k] = run;
+    }
+  __syncthreads();
+}
+
+// State without the clock bit KC (the histogram index) and back.
+template <int KC>
+__device__ __forceinline__ uint32_t ck_pack(uint32_t st) {
+  return ((st >> (KC + 1)) << KC) | (st & ((1u << KC) - 1u));
+}
+template <int KC>
+__device__ __forceinline__ uint32_t ck_unpack(uint32_t j, uint32_t b) {
+  return ((j >> KC) << (KC + 1)) | (b << KC) | (j & ((1u << KC) - 1u));
+}
+
+// (KC >= 0: the clock-edge tables hold the workgroups to 3 per compute unit,
+// so the registers of 3 waves per SIMD)
+template <int NP, int KC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    KC >= 0 ? 3 : (NP <= 2 ? 5 : (NP == 3 ? 4 : (NP == 4 ? 3 : 2))))))
 void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, int32_t cell, unsigned int* wq) {
+  // KC >= 0: pin KC is the cell's only clock pin; the gates whose clock pin is
+  // on the work item's clock net leave the clock events out of their merge and
+  // the workgroup takes them once per item.  The per-gate effects of a clock
+  // edge (internal energy, toggles, the leakage state) and the per-cycle bins
+  // are the same integers as when each gate steps through the edges: a gate's
+  // leakage splits its intervals between D/Q events by the clock-high ticks
+  // (prefix over the edges), and the bins get, per edge, the leakage steps of
+  // the gates in each non-clock state (counted through a difference histogram
+  // over the edges) and the summed edge energies.
+  constexpr bool CLKB = KC >= 0;
+  static_assert(!CLKB || NP == 3, "the histogram packs 4 non-clock states into 16-bit fields");
   constexpr int NIN = NP - 1;
   constexpr int OUT = NP - 1;
   constexpr int NS = 6 * NIN + 2;
@@ -961,6 +1021,17 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
   long long* bins = reinterpret_cast<long long*>(smem_raw);
 
   __shared__ long long s_item;
+  constexpr int CKE = CLKB ? CK_SLOTS : 1;
+  constexpr int CKC = CLKB ? CK_TILE_MAX + 2 : 1;
+  __shared__ uint32_t ck_w[CKE];              // time << 3 | not-a-rise << 2 | value
+  __shared__ uint32_t ck_h[CKE];              // clock-high ticks from the tile start to the edge
+  __shared__ unsigned long long ck_d[CKE];    // gates per non-clock state (16 bits each): differences, then prefix
+  __shared__ uint16_t ck_cf[CKC];             // first slot in each tile cycle (the clamped column)
+  __shared__ long long ck_q[4];               // summed edge energies of the item's gates, per edge kind
+  __shared__ int ck_n[4];                     // edges per kind: 2 * not-a-rise + (toggle_x2 == 2)
+  __shared__ long long ck_l[2];               // clock net: first event, events
+  __shared__ int ck_s[8];                     // clock net, first edge index, first-event flag, edges, overflow
+  __shared__ unsigned long long ck_t[4];      // scan scratch
   const CellDev cd = a.cells[cell];
   const uint32_t clk = cd.clock_mask;
   const uint32_t arcm = cd.arc_mask;  // inputs that can be the related pin of an output event
@@ -995,6 +1066,123 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
   long long base_acc = 0;
   const int64_t gi_begin = (int64_t)range * a.range_gates;
   const int64_t gi_end = min(a.n_sched, gi_begin + a.range_gates);
+
+  // The item's clock net (that of its first gate) and its edges in the tile:
+  // the events a gate's clock pin would consume, from the same start as the
+  // per-pin set-up below.  -1 edges: the gates step through their clock
+  // events (no clock net, more edges than slots, a longer tile, or more than
+  // two edges in a cycle).
+  int ck_ne = -1, ck_net = -1;
+  if constexpr (CLKB) {
+    if (threadIdx.x == 0) {
+      int cnet = -1;
+      int64_t cnb = 0, ccn = 0;
+      if (TC <= CK_TILE_MAX) {
+        for (int64_t gi = gi_begin; gi < gi_end; ++gi) {
+          const int g = a.order[gi];
+          if (g >= 0) {
+            cnet = a.pin_net[a.gate_pin_off[g] + KC];
+            break;
+          }
+        }
+      }
+      if (cnet >= 0) {
+        const int64_t ni = cnet + wk.net_base;
+        cnb = a.net_off[ni];
+        ccn = a.net_off[ni + 1] - cnb;
+      }
+      int32_t fi = 0, fx = 0;
+      if (ccn > 0) {
+        const int32_t i0 = wk.tidx[(int64_t)cnet * wk.tile_stride + wk.tile_col];
+        const uint32_t w1 = a.ev[cnb + max(i0 - 1, 0)];
+        uint32_t l = w1;
+        fi = i0;
+        if (i0 == 0) {
+          if (tile == 0 && ev_time(w1) == 0) {
+            fi = 1;  // the time-0 event is the initial state
+          } else {
+            l = 2u;  // unknown state before the net's first event
+            fx = 1;
+          }
+        }
+        ck_w[0] = (tstart << EV_SHIFT) | (l & 3u);
+        ck_h[0] = 0;
+      } else {
+        cnet = -1;
+      }
+      ck_l[0] = cnb;
+      ck_l[1] = ccn;
+      ck_s[0] = cnet;
+      ck_s[1] = fi;
+      ck_s[2] = fx;
+      ck_s[3] = 0;
+      ck_s[4] = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        ck_q[k] = 0;
+        ck_n[k] = 0;
+      }
+    }
+    __syncthreads();
+    const int cnet = ck_s[0];
+    if (cnet >= 0) {
+      const int64_t cnb = ck_l[0], ccn = ck_l[1];
+      const int32_t fi = ck_s[1];
+      const bool fx = ck_s[2] != 0;
+      const uint32_t w0 = ck_w[0];
+      constexpr int NE_MAX = CK_SLOTS - 3;
+      // edge k in slot k + 1; k = NE_MAX probes for one edge too many
+      for (int k = threadIdx.x; k <= NE_MAX; k += blockDim.x) {
+        const int64_t idx = (int64_t)fi + k;
+        if (idx >= ccn) break;
+        const uint32_t w = a.ev[cnb + idx];
+        const uint32_t t = ev_time(w);
+        if (t >= tend) break;
+        if (k == NE_MAX) {
+          ck_s[4] = 1;
+          break;
+        }
+        const uint32_t pw = k == 0 ? w0 : a.ev[cnb + idx - 1];
+        const uint32_t pt = k == 0 ? tstart : ev_time(pw);
+        const int pv = ev_val(pw), nv = ev_val(w);
+        const int r2 = rise_x2(pv, nv), tx2 = toggle_x2(pv, nv);
+        // TimeBasedPower.cc:420-431: after the net's first event (not at
+        // time 0) the edge counts as not a rise
+        const uint32_t nr = (k == 0 && fx) ? 1u : (r2 > 0 ? 0u : 1u);
+        ck_w[k + 1] = (t << EV_SHIFT) | (nr << 2) | (uint32_t)nv;
+        ck_h[k + 1] = pv == 1 ? t - pt : 0u;
+        if (tx2 > 0) atomicAdd(&ck_n[(r2 > 0 ? 0 : 2) + (tx2 == 2 ? 1 : 0)], 1);
+        atomicAdd(&ck_s[3], 1);
+      }
+    }
+    __syncthreads();
+    int ne = (cnet >= 0 && ck_s[4] == 0) ? ck_s[3] : -1;
+    if (ne >= 0) {
+      block_scan_inplace(ck_h, ne, reinterpret_cast<uint32_t*>(ck_t));
+      // first slot of each tile cycle; more than two edges in a cycle fall back
+      for (int sl = 1 + (int)threadIdx.x; sl <= ne; sl += blockDim.x) {
+        const auto col = [&](int q) {
+          return min((int)((int64_t)cycle_mul(ev_time(ck_w[q]), a.cyc_m, a.cyc_sh) - tc0), TC);
+        };
+        const int ce = col(sl);
+        const int cp = sl == 1 ? -1 : col(sl - 1);
+        for (int ci = cp + 1; ci <= ce; ++ci) ck_cf[ci] = (uint16_t)sl;
+        if (sl + 2 <= ne && col(sl + 2) == ce) ck_s[4] = 1;
+      }
+      const int last = ne >= 1 ? min((int)((int64_t)cycle_mul(ev_time(ck_w[ne]), a.cyc_m, a.cyc_sh) - tc0), TC) : -1;
+      for (int ci = last + 1 + (int)threadIdx.x; ci <= TC + 1; ci += blockDim.x) ck_cf[ci] = (uint16_t)(ne + 1);
+      for (int sl = threadIdx.x; sl <= ne + 1; sl += blockDim.x) ck_d[sl] = 0;
+      if (threadIdx.x == 0) {
+        ck_w[ne + 1] = EV_INF;
+        ck_w[ne + 2] = EV_INF;
+      }
+      __syncthreads();
+      if (ck_s[4] != 0) ne = -1;
+    }
+    ck_ne = ne;
+    ck_net = cnet;
+  }
+
   for (int64_t gi = gi_begin + threadIdx.x; gi < gi_end; gi += blockDim.x) {
     const int g = a.order[gi];
     if (g < 0) continue;
@@ -1122,6 +1310,26 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
     uint32_t prev_t = tstart;
     long long lq_cur = s_leakq[sb];  // leakage of the current state, in quanta and in W
     float lf_cur = s_leak[sb];
+    // CLKB: the leakage of the current non-clock state with the clock low and
+    // high, the clock-high ticks from the tile start to prev_t, and whether
+    // the clock events come from the item's edges (bulk)
+    int32_t lq_lo = 0, lq_hi = 0;
+    float lf_lo = 0.f, lf_hi = 0.f;
+    uint32_t h_prev = 0;
+    bool bulk = false;
+    if constexpr (CLKB) {
+      bulk = ck_ne >= 0 && net[KC] == ck_net;
+      if (bulk) {
+        cw[KC] = EV_INF;
+        nw[KC] = EV_INF;
+        atomicAdd(&ck_d[0], 1ull << (16 * ck_pack<KC>(sb)));
+      }
+      const uint32_t s0 = sb & ~(1u << KC), s1 = sb | (1u << KC);
+      lq_lo = (int32_t)s_leakq[s0];
+      lq_hi = (int32_t)s_leakq[s1];
+      lf_lo = s_leak[s0];
+      lf_hi = s_leak[s1];
+    }
 
     // One pin event per step: the earliest pending event, the output pin's
     // first when several pins change at the same time (so that the related
@@ -1163,17 +1371,66 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
 
       // leakage: the previous state over [prev_t, t), exact in quanta * ticks;
       // the step to the new state enters this cycle's bins
-      acc[PG_LEAK] += (long long)(int)lq_cur * (long long)(t - prev_t);
-      const uint32_t sa = (sb & ~(1u << ps)) | ((nv == 1 ? 1u : 0u) << ps);
-      const long long lq_new = s_leakq[sa];
-      const float lf_new = s_leak[sa];
-      // (zero when the state does not change)
-      atomicAdd((unsigned long long*)&cb[BIN_LD * TCP], (unsigned long long)(lq_new - lq_cur));
-      atomicAdd((unsigned long long*)&cb[BIN_LW * TCP],
-                (unsigned long long)qterm((lf_new - lf_cur) * ((float)(min(cend, t_max) - t) * invT), inv_ql));
-      sb = sa;
-      lq_cur = lq_new;
-      lf_cur = lf_new;
+      if constexpr (!CLKB) {
+        acc[PG_LEAK] += (long long)(int)lq_cur * (long long)(t - prev_t);
+        const uint32_t sa = (sb & ~(1u << ps)) | ((nv == 1 ? 1u : 0u) << ps);
+        const long long lq_new = s_leakq[sa];
+        const float lf_new = s_leak[sa];
+        // (zero when the state does not change)
+        atomicAdd((unsigned long long*)&cb[BIN_LD * TCP], (unsigned long long)(lq_new - lq_cur));
+        atomicAdd((unsigned long long*)&cb[BIN_LW * TCP],
+                  (unsigned long long)qterm((lf_new - lf_cur) * ((float)(min(cend, t_max) - t) * invT), inv_ql));
+        sb = sa;
+        lq_cur = lq_new;
+        lf_cur = lf_new;
+      } else {
+        // the clock: its own events (not bulk), or the item's edges consumed
+        // before this event: those before t, and those at t when the clock
+        // pin precedes pin ps (the output goes first at equal times, then the
+        // inputs in pin order); at most two edges share a cycle
+        uint32_t bit, H;
+        int el = 0;
+        if (bulk) {
+          const bool tie = ps != OUT && KC < ps;
+          const int e = ck_cf[cr];
+          const uint32_t we0 = ck_w[e], we1 = ck_w[e + 1];
+          const bool c0 = ev_time(we0) < t || (ev_time(we0) == t && tie);
+          const bool c1 = ev_time(we1) < t || (ev_time(we1) == t && tie);
+          el = e - 1 + (c0 ? 1 : 0) + (c1 ? 1 : 0);
+          const uint32_t wl = ck_w[el];
+          bit = (wl & 3u) == 1u ? 1u : 0u;
+          H = ck_h[el] + (bit ? t - ev_time(wl) : 0u);
+          if (el > 0) {
+            lw[KC] = wl;  // the latest consumed clock event, for the related-input search
+            rflm = (rflm & ~(1u << KC)) | (((wl >> 2) & 1u) << KC);
+          }
+        } else {
+          bit = (sb >> KC) & 1u;
+          H = h_prev + (bit ? t - prev_t : 0u);
+        }
+        const uint32_t dH = H - h_prev, dt = t - prev_t;
+        acc[PG_LEAK] += (long long)lq_hi * (long long)dH + (long long)lq_lo * (long long)(dt - dH);
+        h_prev = H;
+        const uint32_t sbe = (sb & ~(1u << KC)) | (bit << KC);
+        const uint32_t sa = (sbe & ~(1u << ps)) | ((nv == 1 ? 1u : 0u) << ps);
+        const uint32_t sa0 = sa & ~(1u << KC), sa1 = sa | (1u << KC);
+        const int32_t nlo = (int32_t)s_leakq[sa0], nhi = (int32_t)s_leakq[sa1];
+        const float flo = s_leak[sa0], fhi = s_leak[sa1];
+        const bool abit = (sa >> KC) & 1u;
+        const long long lq_c = bit ? lq_hi : lq_lo, lq_new = abit ? nhi : nlo;
+        const float lf_c = bit ? lf_hi : lf_lo, lf_new = abit ? fhi : flo;
+        atomicAdd((unsigned long long*)&cb[BIN_LD * TCP], (unsigned long long)(lq_new - lq_c));
+        atomicAdd((unsigned long long*)&cb[BIN_LW * TCP],
+                  (unsigned long long)qterm((lf_new - lf_c) * ((float)(min(cend, t_max) - t) * invT), inv_ql));
+        // a new non-clock state counts from the next unconsumed edge on
+        if (bulk && sa0 != (sbe & ~(1u << KC)))
+          atomicAdd(&ck_d[el + 1], (1ull << (16 * ck_pack<KC>(sa))) - (1ull << (16 * ck_pack<KC>(sbe))));
+        sb = sa;
+        lq_lo = nlo;
+        lq_hi = nhi;
+        lf_lo = flo;
+        lf_hi = fhi;
+      }
 
       const int tx2 = toggle_x2(pv, nv);
       const int r2 = rise_x2(pv, nv);
@@ -1286,7 +1543,34 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
           }
       }
     }
-    if (lend > prev_t) acc[PG_LEAK] += (long long)(int)lq_cur * (long long)(lend - prev_t);
+    if constexpr (!CLKB) {
+      if (lend > prev_t) acc[PG_LEAK] += (long long)(int)lq_cur * (long long)(lend - prev_t);
+    } else {
+      // to the tile end, or to the latest event consumed (an edge included)
+      uint32_t endp = max(lend, prev_t), H;
+      if (bulk) {
+        const uint32_t wn = ck_w[ck_ne];
+        endp = max(endp, ev_time(wn));
+        H = ck_h[ck_ne] + ((wn & 3u) == 1u ? endp - ev_time(wn) : 0u);
+        // the edges' internal energy and toggles, per edge kind
+        // (the clock pin's own table at its edge, as for any input event)
+        const float Er = E[2 * KC], Ef = E[2 * KC + 1];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int tx = (k & 1) ? 2 : 1;
+          const float En = (k >> 1) ? Ef : Er;
+          const long long q = qterm(En * (0.5f * (float)tx), inv_qe);
+          const int n = ck_n[k];
+          acc[PG_INT] += (long long)n * q;
+          tog += (uint32_t)(n * tx);
+          atomicAdd((unsigned long long*)&ck_q[k], (unsigned long long)q);
+        }
+      } else {
+        H = h_prev + (((sb >> KC) & 1u) ? endp - prev_t : 0u);
+      }
+      const uint32_t dH = H - h_prev, dt = endp - prev_t;
+      acc[PG_LEAK] += (long long)lq_hi * (long long)dH + (long long)lq_lo * (long long)(dt - dH);
+    }
     acc[PG_SW] -= acc[PG_GSW];
     acc[PG_INT] -= acc[PG_GINT];
     const int64_t slot = (int64_t)wt * a.n_sched + gi;
@@ -1299,6 +1583,42 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
   for (int off = 32; off > 0; off >>= 1) base_acc += __shfl_xor(base_acc, off);
   if ((threadIdx.x & 63) == 0) atomicAdd((unsigned long long*)&s_base, (unsigned long long)base_acc);
   __syncthreads();
+  if constexpr (CLKB) {
+    // the item's clock edges into copy 0 of the bins: per edge, the leakage
+    // steps of the bulk gates in each non-clock state and the summed edge
+    // energies (the terms each gate would add at the edge)
+    if (ck_ne >= 0) {
+      block_scan_inplace(ck_d, ck_ne, ck_t);
+      for (int sl = 1 + (int)threadIdx.x; sl <= ck_ne; sl += blockDim.x) {
+        const uint32_t w = ck_w[sl];
+        const int pv = ev_val(ck_w[sl - 1]), nv = ev_val(w);
+        const uint32_t t = ev_time(w);
+        const uint32_t c = cycle_mul(t, a.cyc_m, a.cyc_sh);
+        const uint32_t cend = c * T + T;
+        const int cr = min((int)((int64_t)c - tc0), TC);
+        const int tx2 = toggle_x2(pv, nv), r2 = rise_x2(pv, nv);
+        if (tx2 > 0)
+          atomicAdd((unsigned long long*)&bins[BIN_INT * TCP + cr],
+                    (unsigned long long)ck_q[(r2 > 0 ? 0 : 2) + (tx2 == 2 ? 1 : 0)]);
+        const uint32_t bo = pv == 1 ? 1u : 0u, bn = nv == 1 ? 1u : 0u;
+        if (bo != bn) {
+          const unsigned long long cnt = ck_d[sl];
+          const float fr = (float)(min(cend, t_max) - t) * invT;
+          long long ld = 0, lwq = 0;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const long long n = (long long)((cnt >> (16 * j)) & 0xFFFFull);
+            const uint32_t so = ck_unpack<KC>((uint32_t)j, bo), sn = ck_unpack<KC>((uint32_t)j, bn);
+            ld += n * (s_leakq[sn] - s_leakq[so]);
+            lwq += n * qterm((s_leak[sn] - s_leak[so]) * fr, inv_ql);
+          }
+          atomicAdd((unsigned long long*)&bins[BIN_LD * TCP + cr], (unsigned long long)ld);
+          atomicAdd((unsigned long long*)&bins[BIN_LW * TCP + cr], (unsigned long long)lwq);
+        }
+      }
+      __syncthreads();
+    }
+  }
   // fold the bin copies into copy 0, then S_c = base + earlier leakage steps
   for (int i = threadIdx.x; i < BW; i += blockDim.x) {
     long long v = bins[i];
@@ -1708,16 +2028,21 @@ hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, uint32_t* ev, co
   return hipGetLastError();
 }
 
-hipError_t launch_power_cell(int np, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,
+hipError_t launch_power_cell(int np, int kc, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,
                              int32_t range_begin, int32_t range_count, int32_t cell, unsigned int* wq) {
   if (grid <= 0) return hipSuccess;
   const dim3 g((unsigned)grid), b(block);
+  if (kc >= 0) {
+    if (np != 3 || kc != 0) return hipErrorInvalidValue;
+    power_cell_kernel<3, 0><<<g, b, lds, s>>>(a, range_begin, range_count, cell, wq);
+    return hipGetLastError();
+  }
   switch (np) {
-    case 2: power_cell_kernel<2><<<g, b, lds, s>>>(a, range_begin, range_count, cell, wq); break;
-    case 3: power_cell_kernel<3><<<g, b, lds, s>>>(a, range_begin, range_count, cell, wq); break;
-    case 4: power_cell_kernel<4><<<g, b, lds, s>>>(a, range_begin, range_count, cell, wq); break;
-    case 5: power_cell_kernel<5><<<g, b, lds, s>>>(a, range_begin, range_count, cell, wq); break;
-    case 6: power_cell_kernel<6><<<g, b, lds, s>>>(a, range_begin, range_count, cell, wq); break;
+    case 2: power_cell_kernel<2, -1><<<g, b, lds, s>>>(a, range_begin, range_count, cell, wq); break;
+    case 3: power_cell_kernel<3, -1><<<g, b, lds, s>>>(a, range_begin, range_count, cell, wq); break;
+    case 4: power_cell_kernel<4, -1><<<g, b, lds, s>>>(a, range_begin, range_count, cell, wq); break;
+    case 5: power_cell_kernel<5, -1><<<g, b, lds, s>>>(a, range_begin, range_count, cell, wq); break;
+    case 6: power_cell_kernel<6, -1><<<g, b, lds, s>>>(a, range_begin, range_count, cell, wq); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
