@@ -1,6 +1,6 @@
 """FP64 CPU restatement of the reference's time-based power path.
 
-TEST INFRASTRUCTURE ONLY: the checker of the HIP engine (tests/,
+TEST INFRASTRUCTURE ONLY: the oracle for the HIP engine (tests/,
 __graft_entry__.smoke()) and the CPU baseline of bench.py.  Product code never
 imports this package.
 """

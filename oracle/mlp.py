@@ -1,6 +1,6 @@
 """FP64 CPU restatement of the per-cycle power surrogate (an MLP over toggle bits).
 
-TEST INFRASTRUCTURE ONLY: the checker of the device surrogate's forward and
+TEST INFRASTRUCTURE ONLY: the oracle for the device surrogate's forward and
 backward kernels.  Product code never imports it.
 
 The surrogate has no counterpart in the reference; this module restates the

@@ -1,6 +1,6 @@
 // FP64 CPU restatement of GTPower's time-based power algorithm.
 //
-// TEST INFRASTRUCTURE ONLY: the checker of the HIP engine and the CPU baseline
+// TEST INFRASTRUCTURE ONLY: the oracle for the HIP engine and the CPU baseline
 // of the benchmark.  Every function cites the reference code it follows.  Where
 // the reference computes in float (util/Types.hh:3-18) this computes in double;
 // where its CPU and GPU paths differ, the CPU path (power/TimeBasedPower.cc) is

@@ -1,7 +1,7 @@
 /*
  * oracle.h -- FP64 CPU restatement of GTPower's time-based power algorithm.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker of the HIP engine and
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle for the HIP engine and
  * the CPU baseline of the benchmark; no product code path calls it.  Only
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
  *

@@ -1,6 +1,6 @@
 // CPU restatement of GTPower's VCD reading and per-bit event store build.
 //
-// TEST INFRASTRUCTURE ONLY: the checker of the device VCD ingest and the CPU
+// TEST INFRASTRUCTURE ONLY: the oracle for the device VCD ingest and the CPU
 // baseline of its benchmark.  It follows, token by token:
 //   * VcdReader::read / getToken / parseTimescale / parseVar / parseScope /
 //     parseUpscope / parseVarValues (power/VcdReader.cc:80-131, :145-310,
