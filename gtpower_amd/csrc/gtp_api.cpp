@@ -120,6 +120,10 @@ struct gtp_ctx {
   int64_t n_chunk = 0;
   DArr<int32_t> d_tile_idx;
   int32_t tile_idx_tc = 0;
+  // per-tile lane order of the full run's schedule (built with the tile index)
+  DArr<int32_t> d_perm;
+  int64_t perm_tc = 0;
+  int32_t perm_rg = 0;
   std::vector<int64_t> h_net_off;  // host copy of the event offsets
   bool glitch_valid = false;       // the EV_GLITCH bits hold the annotation of this trace
   DArr<int32_t> d_tile_list;
@@ -666,6 +670,8 @@ gtp_status finish_trace(gtp_ctx* x) {
   hipStream_t s = x->stream;
   x->d_tile_idx.release();
   x->tile_idx_tc = 0;
+  x->d_perm.release();
+  x->perm_tc = 0;
   {
     // 256-event annotation blocks per net
     std::vector<int64_t> co(N + 1, 0);
@@ -929,10 +935,12 @@ gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, int32_t n_work_tiles, RunPlan
   int block = x->opt.block_lanes > 0 ? std::min(256, std::max(64, (x->opt.block_lanes + 63) / 64 * 64)) : 256;
   block = std::min(block, std::max(64, RG));
   gtp_ctx::OrderSet& os = x->orders[subset ? 1 : 0];
+  bool rebuilt = false;
   if (os.order.empty() || os.range_gates != RG || os.order_T != x->T) {
     os.order_T = x->T;
     build_order(x, os, RG, x->gate_work, x->T);
     HIPCHK(x, os.d_order.upload(os.order.data(), os.order.size(), s));
+    rebuilt = true;
   }
   const int64_t n_sched = (int64_t)os.order.size();
   const int32_t n_range = os.n_range;
@@ -941,6 +949,19 @@ gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, int32_t n_work_tiles, RunPlan
     HIPCHK(x, x->d_tile_idx.alloc((size_t)x->N * (n_tile + 1) + 1));
     HIPCHK(x, launch_tile_index(x->d_net_off.p, x->d_ev.p, x->N, n_tile, TC * x->T, nullptr, 0, x->d_tile_idx.p, s));
     x->tile_idx_tc = (int32_t)TC;
+    rebuilt = true;
+  }
+  // per-tile lane order, like the tile index once per trace and schedule
+  // (GTP_TILE_ORDER=0: schedule order)
+  bool use_perm = !subset && RG <= TILE_ORDER_MAX_RANGE;
+  if (const char* e = std::getenv("GTP_TILE_ORDER"))
+    if (std::atoi(e) == 0) use_perm = false;
+  if (use_perm && (rebuilt || x->perm_tc != TC || x->perm_rg != RG || !x->d_perm.p)) {
+    HIPCHK(x, x->d_perm.alloc((size_t)n_tile * n_sched));
+    HIPCHK(x, launch_tile_order(os.d_order.p, n_sched, RG, x->d_gate_pin_off.p, x->d_pin_net.p, x->d_tile_idx.p,
+                                n_tile, x->d_perm.p, s));
+    x->perm_tc = TC;
+    x->perm_rg = RG;
   }
   Quanta qt;
   {
@@ -964,6 +985,7 @@ gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, int32_t n_work_tiles, RunPlan
   a.net_off = x->d_net_off.p;
   a.ev = x->d_ev.p;
   a.tile_idx = x->d_tile_idx.p;
+  a.perm = use_perm ? x->d_perm.p : nullptr;
   a.n_tile = n_tile;
   a.t_max = x->t_max;
   a.n_cycle = x->C;

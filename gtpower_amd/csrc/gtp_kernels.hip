@@ -142,6 +142,72 @@ __global__ void tile_index_kernel(const int64_t* __restrict__ net_off, const uin
 }
 
 // ---------------------------------------------------------------------------
+// Per-tile lane order.  A wave of the merge runs until its busiest lane's gate
+// is done, so the positions of each range are ordered per tile by the gate's
+// pin events in that tile (the tile index difference summed over its pins):
+// one workgroup per (range, tile) item sorts the range in LDS (bitonic, on
+// key << 32 | ~position, descending).  Only the order of the lanes changes:
+// every partial still goes to its schedule position.
+// ---------------------------------------------------------------------------
+constexpr int TO_MAX = TILE_ORDER_MAX_RANGE;  // longest range sorted
+
+__global__ __launch_bounds__(256) void tile_order_kernel(const int32_t* __restrict__ order, int64_t n_sched,
+                                                         int32_t range_gates, const int64_t* __restrict__ gate_pin_off,
+                                                         const int32_t* __restrict__ pin_net,
+                                                         const int32_t* __restrict__ tile_idx, int32_t n_tile,
+                                                         int32_t* __restrict__ perm) {
+  __shared__ unsigned long long key[TO_MAX];
+  const int64_t n_range = n_sched / range_gates;
+  const int64_t n_item = n_range * n_tile;
+  int n2 = 1;
+  while (n2 < range_gates) n2 <<= 1;
+  const int64_t stride = (int64_t)n_tile + 1;
+  for (int64_t item = blockIdx.x; item < n_item; item += gridDim.x) {
+    const int64_t range = item / n_tile;
+    const int tile = (int)(item - range * n_tile);
+    const int64_t p0 = range * range_gates;
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+      long long ev = -1;  // padding: after every gate
+      if (i < range_gates) {
+        const int g = order[p0 + i];
+        if (g >= 0) {
+          ev = 0;
+          for (int64_t q = gate_pin_off[g]; q < gate_pin_off[g + 1]; ++q) {
+            const int32_t n = pin_net[q];
+            if (n >= 0) ev += tile_idx[(int64_t)n * stride + tile + 1] - tile_idx[(int64_t)n * stride + tile];
+          }
+        }
+      } else {
+        ev = -2;  // beyond the range
+      }
+      // key then the position reversed, so that equal counts keep position order
+      key[i] = ((unsigned long long)(ev + 2) << 32) | (unsigned long long)(0xFFFFFFFFu - (uint32_t)i);
+    }
+    __syncthreads();
+    for (int k = 2; k <= n2; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+          const int l = i ^ j;
+          if (l > i) {
+            const unsigned long long a = key[i], b = key[l];
+            // descending overall: blocks with (i & k) == 0 descend
+            const bool desc = (i & k) == 0;
+            if (desc ? (a < b) : (a > b)) {
+              key[i] = b;
+              key[l] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int i = threadIdx.x; i < range_gates; i += blockDim.x)
+      perm[(int64_t)tile * n_sched + p0 + i] = (int32_t)(p0 + (int64_t)(0xFFFFFFFFu - (uint32_t)(key[i] & 0xFFFFFFFFull)));
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Liberty table interpolation (FP32), tables staged in LDS
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int find_index_f(const float* ax, int n, float x) {
@@ -1183,7 +1249,9 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
     ck_net = cnet;
   }
 
-  for (int64_t gi = gi_begin + threadIdx.x; gi < gi_end; gi += blockDim.x) {
+  for (int64_t gs = gi_begin + threadIdx.x; gs < gi_end; gs += blockDim.x) {
+    // the lane's schedule position: by this tile's order when there is one
+    const int64_t gi = a.perm ? (int64_t)a.perm[(int64_t)tile * a.n_sched + gs] : gs;
     const int g = a.order[gi];
     if (g < 0) continue;
     const int64_t pb = a.gate_pin_off[g];
@@ -1890,6 +1958,18 @@ hipError_t launch_tile_index(const int64_t* net_off, const uint32_t* ev, int64_t
   const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
   tile_index_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(net_off, ev, n_net, n_tile, tile_ticks, tile_list,
                                                                 n_list, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_order(const int32_t* order, int64_t n_sched, int32_t range_gates, const int64_t* gate_pin_off,
+                             const int32_t* pin_net, const int32_t* tile_idx, int32_t n_tile, int32_t* perm,
+                             hipStream_t s) {
+  if (n_sched <= 0 || n_tile <= 0) return hipSuccess;
+  if (range_gates <= 0 || range_gates > TO_MAX || n_sched % range_gates != 0) return hipErrorInvalidValue;
+  const int64_t n_range = n_sched / range_gates;
+  const int64_t n_item = n_range * n_tile;
+  tile_order_kernel<<<dim3((unsigned)std::min<int64_t>(n_item, 1 << 20)), dim3(256), 0, s>>>(
+      order, n_sched, range_gates, gate_pin_off, pin_net, tile_idx, n_tile, perm);
   return hipGetLastError();
 }
 
