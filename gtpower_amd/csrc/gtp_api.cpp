@@ -123,7 +123,7 @@ struct gtp_ctx {
   // per-tile lane order of the full run's schedule (built with the tile index)
   DArr<int32_t> d_perm;
   int64_t perm_tc = 0;
-  int32_t perm_rg = 0;
+  int32_t perm_rg = 0, perm_lanes = 0;
   std::vector<int64_t> h_net_off;  // host copy of the event offsets
   bool glitch_valid = false;       // the EV_GLITCH bits hold the annotation of this trace
   DArr<int32_t> d_tile_list;
@@ -956,12 +956,14 @@ gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, int32_t n_work_tiles, RunPlan
   bool use_perm = !subset && RG <= TILE_ORDER_MAX_RANGE;
   if (const char* e = std::getenv("GTP_TILE_ORDER"))
     if (std::atoi(e) == 0) use_perm = false;
-  if (use_perm && (rebuilt || x->perm_tc != TC || x->perm_rg != RG || !x->d_perm.p)) {
+  if (use_perm &&
+      (rebuilt || x->perm_tc != TC || x->perm_rg != RG || x->perm_lanes != block || !x->d_perm.p)) {
     HIPCHK(x, x->d_perm.alloc((size_t)n_tile * n_sched));
-    HIPCHK(x, launch_tile_order(os.d_order.p, n_sched, RG, x->d_gate_pin_off.p, x->d_pin_net.p, x->d_tile_idx.p,
-                                n_tile, x->d_perm.p, s));
+    HIPCHK(x, launch_tile_order(os.d_order.p, n_sched, RG, block, x->d_gate_pin_off.p, x->d_pin_net.p,
+                                x->d_tile_idx.p, n_tile, x->d_perm.p, s));
     x->perm_tc = TC;
     x->perm_rg = RG;
+    x->perm_lanes = block;
   }
   Quanta qt;
   {

@@ -213,11 +213,12 @@ hipError_t launch_tile_index(const int64_t* net_off, const uint32_t* ev, int64_t
                              hipStream_t s);
 // perm[tile][p] for a full run: within each range of range_gates schedule
 // positions, the positions ordered by their gate's pin events in the tile
-// (from the tile index), most first, ties by position; padding last.
+// (from the tile index), most first, ties by position, padding last, and
+// dealt to the merge's waves of `lanes` lanes (a snake over 64-rank groups).
 constexpr int32_t TILE_ORDER_MAX_RANGE = 4096;
-hipError_t launch_tile_order(const int32_t* order, int64_t n_sched, int32_t range_gates, const int64_t* gate_pin_off,
-                             const int32_t* pin_net, const int32_t* tile_idx, int32_t n_tile, int32_t* perm,
-                             hipStream_t s);
+hipError_t launch_tile_order(const int32_t* order, int64_t n_sched, int32_t range_gates, int32_t lanes,
+                             const int64_t* gate_pin_off, const int32_t* pin_net, const int32_t* tile_idx,
+                             int32_t n_tile, int32_t* perm, hipStream_t s);
 hipError_t launch_energy(const EnergyArgs& a, hipStream_t s);
 hipError_t launch_power(int maxp, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,
                         int32_t range_begin, int32_t range_count, unsigned int* wq);

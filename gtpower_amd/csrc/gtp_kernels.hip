@@ -146,13 +146,17 @@ __global__ void tile_index_kernel(const int64_t* __restrict__ net_off, const uin
 // is done, so the positions of each range are ordered per tile by the gate's
 // pin events in that tile (the tile index difference summed over its pins):
 // one workgroup per (range, tile) item sorts the range in LDS (bitonic, on
-// key << 32 | ~position, descending).  Only the order of the lanes changes:
-// every partial still goes to its schedule position.
+// key << 32 | ~position, descending).  Groups of 64 consecutive ranks form
+// the wave passes; they are dealt to the merge's waves in a snake (pass 0
+// waves 0..n-1, pass 1 waves n-1..0, ...) so that the waves of a workgroup,
+// which meet at its barriers, carry similar totals.  Only the order of the
+// lanes changes: every partial still goes to its schedule position.
 // ---------------------------------------------------------------------------
 constexpr int TO_MAX = TILE_ORDER_MAX_RANGE;  // longest range sorted
 
 __global__ __launch_bounds__(256) void tile_order_kernel(const int32_t* __restrict__ order, int64_t n_sched,
-                                                         int32_t range_gates, const int64_t* __restrict__ gate_pin_off,
+                                                         int32_t range_gates, int32_t lanes,
+                                                         const int64_t* __restrict__ gate_pin_off,
                                                          const int32_t* __restrict__ pin_net,
                                                          const int32_t* __restrict__ tile_idx, int32_t n_tile,
                                                          int32_t* __restrict__ perm) {
@@ -201,8 +205,20 @@ __global__ __launch_bounds__(256) void tile_order_kernel(const int32_t* __restri
         __syncthreads();
       }
     }
-    for (int i = threadIdx.x; i < range_gates; i += blockDim.x)
-      perm[(int64_t)tile * n_sched + p0 + i] = (int32_t)(p0 + (int64_t)(0xFFFFFFFFu - (uint32_t)(key[i] & 0xFFFFFFFFull)));
+    // rank r -> the position a lane of the merge takes as its r-th gate
+    const bool snake = lanes >= 64 && lanes % 64 == 0 && range_gates % lanes == 0;
+    const int nw = lanes / 64;
+    for (int r = threadIdx.x; r < range_gates; r += blockDim.x) {
+      int pos = r;
+      if (snake) {
+        const int k = r >> 6, pass = k / nw;
+        int wv = k - pass * nw;
+        if (pass & 1) wv = nw - 1 - wv;
+        pos = pass * lanes + wv * 64 + (r & 63);
+      }
+      perm[(int64_t)tile * n_sched + p0 + pos] =
+          (int32_t)(p0 + (int64_t)(0xFFFFFFFFu - (uint32_t)(key[r] & 0xFFFFFFFFull)));
+    }
     __syncthreads();
   }
 }
@@ -1961,15 +1977,15 @@ hipError_t launch_tile_index(const int64_t* net_off, const uint32_t* ev, int64_t
   return hipGetLastError();
 }
 
-hipError_t launch_tile_order(const int32_t* order, int64_t n_sched, int32_t range_gates, const int64_t* gate_pin_off,
-                             const int32_t* pin_net, const int32_t* tile_idx, int32_t n_tile, int32_t* perm,
-                             hipStream_t s) {
+hipError_t launch_tile_order(const int32_t* order, int64_t n_sched, int32_t range_gates, int32_t lanes,
+                             const int64_t* gate_pin_off, const int32_t* pin_net, const int32_t* tile_idx,
+                             int32_t n_tile, int32_t* perm, hipStream_t s) {
   if (n_sched <= 0 || n_tile <= 0) return hipSuccess;
   if (range_gates <= 0 || range_gates > TO_MAX || n_sched % range_gates != 0) return hipErrorInvalidValue;
   const int64_t n_range = n_sched / range_gates;
   const int64_t n_item = n_range * n_tile;
   tile_order_kernel<<<dim3((unsigned)std::min<int64_t>(n_item, 1 << 20)), dim3(256), 0, s>>>(
-      order, n_sched, range_gates, gate_pin_off, pin_net, tile_idx, n_tile, perm);
+      order, n_sched, range_gates, lanes, gate_pin_off, pin_net, tile_idx, n_tile, perm);
   return hipGetLastError();
 }
 

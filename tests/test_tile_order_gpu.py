@@ -24,15 +24,18 @@ def same(a, b, what):
     assert np.array_equal(a.cycle_power, b.cycle_power), f"{what}: cycle power differs"
 
 
-@pytest.mark.parametrize("tile_cycles,gates_per_wg", [(64, 128), (256, 512), (512, 1024)])
-def test_tile_order_equals_schedule_order(monkeypatch, tile_cycles, gates_per_wg):
-    print(f"[order] 6000 gates, tile {tile_cycles}, {gates_per_wg} gates per range", flush=True)
+@pytest.mark.parametrize("tile_cycles,gates_per_wg,block_lanes", [
+    (64, 128, 0), (256, 512, 0), (512, 1024, 0), (128, 192, 0), (128, 512, 64), (128, 100, 0)])
+def test_tile_order_equals_schedule_order(monkeypatch, tile_cycles, gates_per_wg, block_lanes):
+    # 192 gates: three waves, one pass; 64 lanes: eight passes of one wave;
+    # 100 gates: lanes not whole waves (ranks stay in sorted order)
+    print(f"[order] 6000 gates, tile {tile_cycles}, {gates_per_wg} gates per range, lanes {block_lanes}", flush=True)
     d = S.make_design(6000, dff_fraction=0.15, clock_ns=2.0, seed=81)
     tr = d.trace(1500, toggle_p=0.2, seed=181)
     res = {}
     for flag in ("1", "0"):
         monkeypatch.setenv("GTP_TILE_ORDER", flag)
-        with HipPower(tile_cycles=tile_cycles, gates_per_wg=gates_per_wg) as eng:
+        with HipPower(tile_cycles=tile_cycles, gates_per_wg=gates_per_wg, block_lanes=block_lanes) as eng:
             eng.set_design(d.library, d)
             eng.read_power_activities(tr)
             res[flag] = eng.power()
