@@ -976,12 +976,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void g
 }
 #undef AN_SLOT
 
-// The same for the events of listed tiles only: the AT_LANES threads of group
-// (n, l) annotate, interleaved, the events of net n from the one before
-// idx_begin[n][l] up to the first event at or after the end of tile
-// tile_list[l], both included: the power kernels read the glitch flags of
-// the events next to a tile's own.
-constexpr int AT_LANES = 8;  // threads per (net, listed tile): 2 and 16 measured slower
+// The same for the events of listed tiles only: one thread per (net n,
+// listed tile l) walks the events of n from the one before idx_begin[n][l] up
+// to the first event at or after the end of tile tile_list[l], both included
+// (the power kernels read the glitch flags of the events next to a tile's
+// own).  annotate_one's rule, with cand(m) = m >= 1 && (!last(m) || j(m) odd):
+// event m is flagged when cand(m) and the gap to a candidate neighbour is
+// positive.  The thread carries the group position j along its events and
+// reads each word once; only its first event and the one before it take
+// group_pos's walk back.
+constexpr int AN_PASS = 8;  // events per pass of a (net, tile) thread
+
+__device__ __forceinline__ bool cand_at(const uint32_t* e, int64_t i, int64_t cnt, uint32_t T, uint32_t cyc_m,
+                                        int cyc_sh) {
+  if (i < 1) return false;
+  const uint32_t cs = cycle_mul(ev_time(e[i]), cyc_m, cyc_sh) * T;
+  const bool last = i + 1 >= cnt || ev_time(e[i + 1]) >= cs + T;
+  return !last || (group_pos(e, i, cs) & 1);
+}
 
 __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_t* __restrict__ net_off,
                                                                     uint32_t* __restrict__ ev,
@@ -989,23 +1001,62 @@ __global__ __launch_bounds__(256) void glitch_annotate_tiles_kernel(const int64_
                                                                     const int32_t* __restrict__ tile_list,
                                                                     int32_t n_list, int64_t n_net, int64_t tile_ticks,
                                                                     uint32_t T, uint32_t cyc_m, int cyc_sh) {
-  // threads stride over the (net, tile, lane) triples under a capped grid
-  const int64_t n_thr = n_net * n_list * AT_LANES;
-  for (int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; tid < n_thr;
-       tid += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = tid / AT_LANES;
-    const int j = (int)(tid % AT_LANES);
+  // threads stride over the (net, tile) pairs under a capped grid
+  const int64_t n_thr = n_net * n_list;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_thr; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t n = i / n_list;
     const int64_t b = net_off[n];
     const int64_t cnt = net_off[n + 1] - b;
     const int64_t tend = ((int64_t)tile_list[i - n * n_list] + 1) * tile_ticks;
+    int64_t m = max(idx_begin[i] - 1, 0);
+    if (m >= cnt) continue;
     uint32_t* e = ev + b;
-    for (int64_t k = (int64_t)max(idx_begin[i] - 1, 0) + j; k < cnt; k += AT_LANES) {
-      if ((int64_t)ev_time(e[k]) >= tend && k > 0 && (int64_t)ev_time(e[k - 1]) >= tend) break;
-      const uint32_t pw = annotate_one(e, k, cnt, T, cyc_m, cyc_sh);
-      const uint32_t w = e[k];
-      const uint32_t nw = pw ? (w | EV_GLITCH) : (w & ~EV_GLITCH);
-      if (nw != w) e[k] = nw;
+    // words of m and m + 1; tp = time of m - 1; j = j(m)
+    uint32_t w0 = e[m];
+    uint32_t w1 = m + 1 < cnt ? e[m + 1] : 0u;
+    int j = group_pos(e, m, cycle_mul(ev_time(w0), cyc_m, cyc_sh) * T);
+    bool cprev = cand_at(e, m - 1, cnt, T, cyc_m, cyc_sh);
+    uint32_t tp = m >= 1 ? ev_time(e[m - 1]) : 0u;
+    // AN_PASS events per pass, their next words loaded together
+    for (bool done = false; !done; m += AN_PASS) {
+      uint32_t win[AN_PASS + 2];
+      win[0] = w0;
+      win[1] = w1;
+#pragma unroll
+      for (int q = 0; q < AN_PASS; ++q) win[2 + q] = m + 2 + q < cnt ? e[m + 2 + q] : 0u;
+#pragma unroll
+      for (int q = 0; q < AN_PASS; ++q) {
+        const int64_t mq = m + q;
+        const uint32_t t0 = ev_time(win[q]), t1 = ev_time(win[q + 1]), t2 = ev_time(win[q + 2]);
+        const bool has1 = mq + 1 < cnt, has2 = mq + 2 < cnt;
+        const uint32_t ce = cycle_mul(t0, cyc_m, cyc_sh) * T + T;
+        const bool last = !has1 || t1 >= ce;
+        const bool cand = mq >= 1 && (!last || (j & 1));
+        // the next event's group position and candidacy
+        const int j1 = (mq >= 1 && !last) ? j + 1 : 0;
+        bool cnext = false;
+        if (has1) {
+          const uint32_t ce1 = cycle_mul(t1, cyc_m, cyc_sh) * T + T;
+          const bool last1 = !has2 || t2 >= ce1;
+          cnext = !last1 || (j1 & 1);
+        }
+        int64_t pw = -1;
+        if (cand) {
+          if (mq >= 2 && cprev) pw = (int64_t)(t0 - tp);
+          if (has1 && cnext) pw = max(pw, (int64_t)(t1 - t0));
+        }
+        const uint32_t nw = pw > 0 ? (win[q] | EV_GLITCH) : (win[q] & ~EV_GLITCH);
+        if (nw != win[q]) e[mq] = nw;
+        if ((int64_t)t0 >= tend || !has1) {
+          done = true;
+          break;
+        }
+        cprev = cand;
+        tp = t0;
+        j = j1;
+      }
+      w0 = win[AN_PASS];
+      w1 = win[AN_PASS + 1];
     }
   }
 }
@@ -2114,7 +2165,7 @@ hipError_t launch_glitch_annotate(const int64_t* net_off, uint32_t* ev, const in
 hipError_t launch_glitch_annotate_tiles(const int64_t* net_off, uint32_t* ev, const int32_t* idx_begin,
                                         const int32_t* tile_list, int32_t n_list, int64_t n_net, int64_t tile_ticks,
                                         uint32_t T, hipStream_t s) {
-  const int64_t n = n_net * n_list * AT_LANES;
+  const int64_t n = n_net * n_list;
   if (n <= 0) return hipSuccess;
   uint32_t cyc_m = 0;
   int32_t cyc_sh = 0;
