@@ -958,12 +958,19 @@ gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, int32_t n_work_tiles, RunPlan
     if (std::atoi(e) == 0) use_perm = false;
   if (use_perm &&
       (rebuilt || x->perm_tc != TC || x->perm_rg != RG || x->perm_lanes != block || !x->d_perm.p)) {
-    HIPCHK(x, x->d_perm.alloc((size_t)n_tile * n_sched));
-    HIPCHK(x, launch_tile_order(os.d_order.p, n_sched, RG, block, x->d_gate_pin_off.p, x->d_pin_net.p,
-                                x->d_tile_idx.p, n_tile, x->d_perm.p, s));
-    x->perm_tc = TC;
-    x->perm_rg = RG;
-    x->perm_lanes = block;
+    // optional: without the memory, the lanes take the schedule order
+    if (x->d_perm.alloc((size_t)n_tile * n_sched) != hipSuccess) {
+      (void)hipGetLastError();
+      x->d_perm.release();
+      x->perm_tc = 0;
+      use_perm = false;
+    } else {
+      HIPCHK(x, launch_tile_order(os.d_order.p, n_sched, RG, block, x->d_gate_pin_off.p, x->d_pin_net.p,
+                                  x->d_tile_idx.p, n_tile, x->d_perm.p, s));
+      x->perm_tc = TC;
+      x->perm_rg = RG;
+      x->perm_lanes = block;
+    }
   }
   Quanta qt;
   {
