@@ -140,11 +140,12 @@ __global__ __launch_bounds__(TB) void write_starts_kernel(const uint8_t* __restr
 // Ids of 1 to 3 characters in [33, 126] index a direct table: key =
 // sum of (c_i - 32) * 95^i (digits >= 1, so lengths do not collide).
 constexpr int32_t SHORT_ID_KEYS = 95 * 95 * 95;
-__device__ __forceinline__ int32_t short_id_key(const uint8_t* s, int64_t n) {
+template <typename At>
+__device__ __forceinline__ int32_t short_id_key(At at, int64_t s, int64_t n) {
   if (n < 1 || n > 3) return -1;
   int32_t k = 0, m = 1;
   for (int64_t i = 0; i < n; ++i) {
-    const uint8_t c = s[i];
+    const uint8_t c = at(s + i);
     if (c < 33 || c > 126) return -1;
     k += (c - 32) * m;
     m *= 95;
@@ -152,41 +153,32 @@ __device__ __forceinline__ int32_t short_id_key(const uint8_t* s, int64_t n) {
   return k;
 }
 
-__global__ void parse_tokens_kernel(const uint8_t* __restrict__ t, int64_t L, const uint32_t* __restrict__ tok,
-                                    int64_t K, uint8_t* __restrict__ kind, uint32_t* __restrict__ len,
-                                    int64_t* __restrict__ val, int32_t* __restrict__ skey) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= K) return;
-  const int64_t s = tok[k];
-  // the token ends where the whitespace before the next token starts (only
-  // whitespace separates tokens): step back from the next start, usually
-  // over one separator; the last token scans forward
-  int64_t e;
-  if (k + 1 < K) {
-    e = tok[k + 1];
-    while (e > s + 1 && is_ws(t[e - 1])) --e;
-  } else {
-    e = s + 1;
-    while (e < L && !is_ws(t[e])) ++e;
-  }
+// One token [s, e), bytes read through at(p).
+template <typename At>
+__device__ __forceinline__ void parse_one(At at, int64_t k, int64_t s, int64_t e, uint8_t* __restrict__ kind,
+                                          uint32_t* __restrict__ len, int64_t* __restrict__ val,
+                                          int32_t* __restrict__ skey) {
   const int64_t n = e - s;
-  const uint8_t c0 = upper(t[s]);
+  const uint8_t c0 = upper(at(s));
   uint8_t kd = TK_OTHER;
   int64_t v = 0;
   if (c0 == '#' && n > 1) {
     // std::stoll: optional sign, decimal digits; no digit or overflow = error
     int64_t p = s + 1;
     bool neg = false;
-    if (t[p] == '+' || t[p] == '-') {
-      neg = t[p] == '-';
+    const uint8_t c1 = at(p);
+    if (c1 == '+' || c1 == '-') {
+      neg = c1 == '-';
       ++p;
     }
     uint64_t acc = 0;
     bool any = false, ovf = false;
     const uint64_t lim = neg ? (1ull << 63) : (1ull << 63) - 1;
-    for (; p < e && t[p] >= '0' && t[p] <= '9'; ++p) {
+    for (; p < e; ++p) {
+      const uint8_t c = at(p);
+      if (c < '0' || c > '9') break;
       any = true;
-      const uint64_t d = t[p] - '0';
+      const uint64_t d = c - '0';
       if (acc > (lim - d) / 10) ovf = true;
       else acc = acc * 10 + d;
     }
@@ -201,20 +193,22 @@ __global__ void parse_tokens_kernel(const uint8_t* __restrict__ t, int64_t L, co
     v = c0;
   } else if (c0 == 'B') {
     kd = TK_BUS;
-    const uint8_t c1 = n > 1 ? upper(t[s + 1]) : 0;
+    const uint8_t c1 = n > 1 ? upper(at(s + 1)) : 0;
     if (!(c1 == 'X' || c1 == 'U' || c1 == 'Z')) {
       // strtol(..., 2): optional sign, binary digits, saturating
       int64_t p = s + 1;
       bool neg = false;
-      if (p < e && (t[p] == '+' || t[p] == '-')) {
-        neg = t[p] == '-';
+      if (p < e && (at(p) == '+' || at(p) == '-')) {
+        neg = at(p) == '-';
         ++p;
       }
       uint64_t acc = 0;
       bool ovf = false;
       const uint64_t lim = neg ? (1ull << 63) : (1ull << 63) - 1;
-      for (; p < e && (t[p] == '0' || t[p] == '1'); ++p) {
-        const uint64_t d = t[p] - '0';
+      for (; p < e; ++p) {
+        const uint8_t c = at(p);
+        if (c != '0' && c != '1') break;
+        const uint64_t d = c - '0';
         if (ovf || acc > (lim - d) / 2) ovf = true;
         else acc = acc * 2 + d;
       }
@@ -225,7 +219,53 @@ __global__ void parse_tokens_kernel(const uint8_t* __restrict__ t, int64_t L, co
   kind[k] = kd;
   len[k] = (uint32_t)n;
   val[k] = v;
-  skey[k] = kd == TK_SCALAR ? short_id_key(t + s + 1, n - 1) : -1;  // a scalar's id
+  skey[k] = kd == TK_SCALAR ? short_id_key(at, s + 1, n - 1) : -1;  // a scalar's id
+}
+
+// One workgroup per text block (the blocks of the token-start count): the
+// block's BPB bytes and PT_HALO bytes after it are staged in LDS with 16-byte
+// loads, and the tokens that start in the block are parsed from there (a byte
+// outside the staged range is read from memory).  A token ends where the
+// whitespace before the next token starts (only whitespace separates tokens):
+// step back from the next start, usually over one separator; the last token
+// scans forward.
+constexpr int PT_HALO = 256;
+constexpr int PT_STAGE = BPB + PT_HALO;
+
+__global__ __launch_bounds__(TB) void parse_tokens_kernel(const uint8_t* __restrict__ t, int64_t L, int64_t t_bytes,
+                                                          int64_t base0, const uint64_t* __restrict__ boff,
+                                                          int64_t n_blk, const uint32_t* __restrict__ tok, int64_t K,
+                                                          uint8_t* __restrict__ kind, uint32_t* __restrict__ len,
+                                                          int64_t* __restrict__ val, int32_t* __restrict__ skey) {
+  __shared__ __align__(16) uint8_t st[PT_STAGE];
+  const int64_t b = blockIdx.x;
+  const int64_t base = base0 + b * BPB;
+  for (int i = threadIdx.x; i < PT_STAGE / 16; i += blockDim.x) {
+    const int64_t p = base + 16 * (int64_t)i;
+    // past the buffer: blanks, as the buffer's own padding
+    const uint4 q = p + 16 <= t_bytes ? *reinterpret_cast<const uint4*>(t + p)
+                                      : make_uint4(0x20202020u, 0x20202020u, 0x20202020u, 0x20202020u);
+    reinterpret_cast<uint4*>(st)[i] = q;
+  }
+  __syncthreads();
+  const int64_t k0 = (int64_t)boff[b];
+  const int64_t k1 = min(b + 1 < n_blk ? (int64_t)boff[b + 1] : K, K);
+  const auto at = [&](int64_t p) -> uint8_t {
+    const int64_t o = p - base;
+    return (o >= 0 && o < PT_STAGE) ? st[o] : t[p];
+  };
+  for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+    const int64_t s = tok[k];
+    int64_t e;
+    if (k + 1 < K) {
+      e = tok[k + 1];
+      while (e > s + 1 && is_ws(at(e - 1))) --e;
+    } else {
+      e = s + 1;
+      while (e < L && !is_ws(at(e))) ++e;
+    }
+    parse_one(at, k, s, e, kind, len, val, skey);
+  }
 }
 
 // start of the run of bus-looking tokens holding token k (k itself when the
@@ -316,7 +356,7 @@ __global__ void emit_changes_kernel(ChangeArgs a) {
   if (bus) {
     id = a.t + a.tok[k + 1];
     n = a.len[k + 1];
-    key = short_id_key(id, n);
+    key = short_id_key([id](int64_t p) { return id[p]; }, 0, n);
   } else {
     id = a.t + a.tok[k] + 1;
     n = (int64_t)a.len[k] - 1;
@@ -853,8 +893,9 @@ GTP_API gtp_status gtp_vcd_parse(gtp_vcd* v, const char* text, int64_t len, gtp_
   int64_t n_change = 0;
   int64_t t_max = 0;
   if (K > 0) {
-    parse_tokens_kernel<<<nblk(K), 256, 0, s>>>(S_.text.p, len, S_.tok.p, K, S_.kind.p, S_.len.p, S_.val.p,
-                                                S_.skey.p);
+    parse_tokens_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(S_.text.p, len, (int64_t)text_bytes, b0al,
+                                                                  S_.boff.p, n_blk, S_.tok.p, K, S_.kind.p,
+                                                                  S_.len.p, S_.val.p, S_.skey.p);
     VCHK(v, hipGetLastError());
     run_start_in_kernel<<<nblk(K), 256, 0, s>>>(S_.kind.p, K, S_.rs_in.p);
     VCHK(v, hipGetLastError());
