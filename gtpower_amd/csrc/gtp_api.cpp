@@ -122,6 +122,7 @@ struct gtp_ctx {
   int32_t tile_idx_tc = 0;
   // per-tile lane order of the full run's schedule (built with the tile index)
   DArr<int32_t> d_perm;
+  DArr<int32_t> d_perm_batch;   // the same per work tile of a batched run
   int64_t perm_tc = 0;
   int32_t perm_rg = 0, perm_lanes = 0;
   std::vector<int64_t> h_net_off;  // host copy of the event offsets
@@ -966,7 +967,7 @@ gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, int32_t n_work_tiles, RunPlan
       use_perm = false;
     } else {
       HIPCHK(x, launch_tile_order(os.d_order.p, n_sched, RG, block, x->d_gate_pin_off.p, x->d_pin_net.p,
-                                  x->d_tile_idx.p, n_tile, x->d_perm.p, s));
+                                  x->d_tile_idx.p, n_tile, n_tile, nullptr, nullptr, nullptr, x->d_perm.p, s));
       x->perm_tc = TC;
       x->perm_rg = RG;
       x->perm_lanes = block;
@@ -1488,6 +1489,25 @@ gtp_status run_load(gtp_ctx* x, MultiPlan& mp, const std::vector<RunItem>& items
     HIPCHK(x, launch_tile_index(d_voff.p + (int64_t)k * N, d_ev.p, N, items[k].n_tile, TC * T, nullptr, 0,
                                 d_tidx.p + td[k].tile_idx_base, s));
   HIPCHK(x, launch_glitch_annotate(d_voff.p, d_ev.p, d_chunk.p, d_chunk_net.p, n_chunk, (uint32_t)T, s));
+  // per-work-tile lane order, as in full runs (GTP_TILE_ORDER=0 or without
+  // the memory for it: schedule order)
+  const int64_t n_sched_b = (int64_t)os.order.size();
+  int32_t* perm_b = nullptr;
+  {
+    bool use = rp.RG <= TILE_ORDER_MAX_RANGE;
+    if (const char* e = std::getenv("GTP_TILE_ORDER"))
+      if (std::atoi(e) == 0) use = false;
+    if (use) {
+      if (x->d_perm_batch.alloc((size_t)n_wt * n_sched_b) != hipSuccess) {
+        (void)hipGetLastError();
+        x->d_perm_batch.release();
+      } else {
+        HIPCHK(x, launch_tile_order(os.d_order.p, n_sched_b, rp.RG, rp.block, x->d_gate_pin_off.p, x->d_pin_net.p,
+                                    d_tidx.p, 0, n_wt, d_wt_tile.p, d_wt_trace.p, d_td.p, x->d_perm_batch.p, s));
+        perm_b = x->d_perm_batch.p;
+      }
+    }
+  }
   rp.a.net_off = d_voff.p;
   rp.a.ev = d_ev.p;
   rp.a.tile_idx = d_tidx.p;
@@ -1497,6 +1517,7 @@ gtp_status run_load(gtp_ctx* x, MultiPlan& mp, const std::vector<RunItem>& items
     const int32_t W = std::min(mp.W_round, n_wt - r0);
     rp.a.tile_list = d_wt_tile.p + r0;
     rp.a.wt_trace = d_wt_trace.p + r0;
+    rp.a.perm = perm_b ? perm_b + (int64_t)r0 * n_sched_b : nullptr;
     rp.a.n_work_tiles = W;
     {
       const gtp_status st = launch_groups(x, rp, true);

@@ -162,9 +162,9 @@ struct PowerArgs {
   int32_t range_gates;           // gates per range; order[] is padded with -1 to whole ranges
   int32_t xcd_remap;             // 1: remap blockIdx so each XCD takes a contiguous block of work
   const int32_t* tile_list;      // tiles to evaluate (null: all n_tile tiles)
-  const int32_t* perm;           // [n_tile][n_sched] per tile, the schedule positions of each range by
-                                 // the gates' events in the tile, most first (single-trace full runs;
-                                 // null: schedule order)
+  const int32_t* perm;           // [n_work_tiles][n_sched] per work tile, the schedule positions of each
+                                 // range by the gates' events in the tile, most first (full and batched
+                                 // runs; null: schedule order)
   int32_t n_work_tiles;          // entries of tile_list, or n_tile
   const int32_t* wt_trace;       // batched runs: trace of each work tile (tile_list: its tile); null otherwise
   const TraceDesc* traces;       // batched runs: [n_trace]
@@ -211,14 +211,17 @@ hipError_t launch_pack(const int64_t* t, const uint8_t* v, uint32_t* out, int64_
 hipError_t launch_tile_index(const int64_t* net_off, const uint32_t* ev, int64_t n_net, int32_t n_tile,
                              int64_t tile_ticks, const int32_t* tile_list, int32_t n_list, int32_t* out,
                              hipStream_t s);
-// perm[tile][p] for a full run: within each range of range_gates schedule
+// perm[wt][p] per work tile: within each range of range_gates schedule
 // positions, the positions ordered by their gate's pin events in the tile
 // (from the tile index), most first, ties by position, padding last, and
 // dealt to the merge's waves of `lanes` lanes (a snake over 64-rank groups).
+// Full runs: work tile = tile (wt_trace null); batched runs: work tile wt is
+// tile wt_tile[wt] of trace wt_trace[wt] (tile index at its tile_idx_base).
 constexpr int32_t TILE_ORDER_MAX_RANGE = 4096;
 hipError_t launch_tile_order(const int32_t* order, int64_t n_sched, int32_t range_gates, int32_t lanes,
                              const int64_t* gate_pin_off, const int32_t* pin_net, const int32_t* tile_idx,
-                             int32_t n_tile, int32_t* perm, hipStream_t s);
+                             int32_t n_tile, int32_t n_wt, const int32_t* wt_tile, const int32_t* wt_trace,
+                             const TraceDesc* traces, int32_t* perm, hipStream_t s);
 hipError_t launch_energy(const EnergyArgs& a, hipStream_t s);
 hipError_t launch_power(int maxp, int64_t grid, int block, size_t lds, hipStream_t s, const PowerArgs& a,
                         int32_t range_begin, int32_t range_count, unsigned int* wq);

@@ -159,16 +159,28 @@ __global__ __launch_bounds__(256) void tile_order_kernel(const int32_t* __restri
                                                          const int64_t* __restrict__ gate_pin_off,
                                                          const int32_t* __restrict__ pin_net,
                                                          const int32_t* __restrict__ tile_idx, int32_t n_tile,
+                                                         int32_t n_wt, const int32_t* __restrict__ wt_tile,
+                                                         const int32_t* __restrict__ wt_trace,
+                                                         const TraceDesc* __restrict__ traces,
                                                          int32_t* __restrict__ perm) {
   __shared__ unsigned long long key[TO_MAX];
   const int64_t n_range = n_sched / range_gates;
-  const int64_t n_item = n_range * n_tile;
+  const int64_t n_item = n_range * n_wt;
   int n2 = 1;
   while (n2 < range_gates) n2 <<= 1;
-  const int64_t stride = (int64_t)n_tile + 1;
   for (int64_t item = blockIdx.x; item < n_item; item += gridDim.x) {
-    const int64_t range = item / n_tile;
-    const int tile = (int)(item - range * n_tile);
+    const int64_t range = item / n_wt;
+    const int wt = (int)(item - range * n_wt);
+    // the work tile's tile and tile index (batched runs: of its trace)
+    int tile = wt;
+    int64_t stride = (int64_t)n_tile + 1;
+    const int32_t* tidx = tile_idx;
+    if (wt_trace) {
+      const TraceDesc td = traces[wt_trace[wt]];
+      tile = wt_tile[wt];
+      stride = (int64_t)td.n_tile + 1;
+      tidx = tile_idx + td.tile_idx_base;
+    }
     const int64_t p0 = range * range_gates;
     for (int i = threadIdx.x; i < n2; i += blockDim.x) {
       long long ev = -1;  // padding: after every gate
@@ -178,7 +190,7 @@ __global__ __launch_bounds__(256) void tile_order_kernel(const int32_t* __restri
           ev = 0;
           for (int64_t q = gate_pin_off[g]; q < gate_pin_off[g + 1]; ++q) {
             const int32_t n = pin_net[q];
-            if (n >= 0) ev += tile_idx[(int64_t)n * stride + tile + 1] - tile_idx[(int64_t)n * stride + tile];
+            if (n >= 0) ev += tidx[(int64_t)n * stride + tile + 1] - tidx[(int64_t)n * stride + tile];
           }
         }
       } else {
@@ -216,7 +228,7 @@ __global__ __launch_bounds__(256) void tile_order_kernel(const int32_t* __restri
         if (pass & 1) wv = nw - 1 - wv;
         pos = pass * lanes + wv * 64 + (r & 63);
       }
-      perm[(int64_t)tile * n_sched + p0 + pos] =
+      perm[(int64_t)wt * n_sched + p0 + pos] =
           (int32_t)(p0 + (int64_t)(0xFFFFFFFFu - (uint32_t)(key[r] & 0xFFFFFFFFull)));
     }
     __syncthreads();
@@ -1318,7 +1330,7 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
 
   for (int64_t gs = gi_begin + threadIdx.x; gs < gi_end; gs += blockDim.x) {
     // the lane's schedule position: by this tile's order when there is one
-    const int64_t gi = a.perm ? (int64_t)a.perm[(int64_t)tile * a.n_sched + gs] : gs;
+    const int64_t gi = a.perm ? (int64_t)a.perm[(int64_t)wt * a.n_sched + gs] : gs;
     const int g = a.order[gi];
     if (g < 0) continue;
     const int64_t pb = a.gate_pin_off[g];
@@ -2030,13 +2042,15 @@ hipError_t launch_tile_index(const int64_t* net_off, const uint32_t* ev, int64_t
 
 hipError_t launch_tile_order(const int32_t* order, int64_t n_sched, int32_t range_gates, int32_t lanes,
                              const int64_t* gate_pin_off, const int32_t* pin_net, const int32_t* tile_idx,
-                             int32_t n_tile, int32_t* perm, hipStream_t s) {
-  if (n_sched <= 0 || n_tile <= 0) return hipSuccess;
+                             int32_t n_tile, int32_t n_wt, const int32_t* wt_tile, const int32_t* wt_trace,
+                             const TraceDesc* traces, int32_t* perm, hipStream_t s) {
+  if (n_sched <= 0 || n_wt <= 0) return hipSuccess;
   if (range_gates <= 0 || range_gates > TO_MAX || n_sched % range_gates != 0) return hipErrorInvalidValue;
   const int64_t n_range = n_sched / range_gates;
-  const int64_t n_item = n_range * n_tile;
+  const int64_t n_item = n_range * n_wt;
   tile_order_kernel<<<dim3((unsigned)std::min<int64_t>(n_item, 1 << 20)), dim3(256), 0, s>>>(
-      order, n_sched, range_gates, lanes, gate_pin_off, pin_net, tile_idx, n_tile, perm);
+      order, n_sched, range_gates, lanes, gate_pin_off, pin_net, tile_idx, n_tile, n_wt, wt_tile, wt_trace, traces,
+      perm);
   return hipGetLastError();
 }
 

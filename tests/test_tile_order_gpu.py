@@ -68,3 +68,27 @@ def test_tile_order_follows_a_new_trace():
             eng.set_design(d.library, d)
             eng.read_power_activities(t)
             same(r, eng.power(), "fresh context")
+
+
+@pytest.mark.parametrize("round_tiles", [0, 3])
+def test_tile_order_in_batched_runs(monkeypatch, round_tiles):
+    # per work tile (trace, tile), over rounds of a few work tiles
+    print(f"[order] batched run of 4 traces, round_tiles {round_tiles}", flush=True)
+    d = S.make_design(3000, dff_fraction=0.15, clock_ns=2.0, seed=85)
+    traces = [d.trace(c, toggle_p=p, seed=185 + k) for k, (c, p) in enumerate(
+        [(300, 0.1), (900, 0.25), (130, 0.3), (600, 0.15)])]
+    res = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("GTP_TILE_ORDER", flag)
+        with HipPower(tile_cycles=128, gates_per_wg=256, round_tiles=round_tiles) as eng:
+            eng.set_design(d.library, d)
+            eng.read_power_activities(traces[0])
+            res[flag] = eng.power_batch(traces)
+    monkeypatch.delenv("GTP_TILE_ORDER")
+    for k, (a, b) in enumerate(zip(res["1"], res["0"])):
+        same(a, b, f"batched trace {k}")
+    with HipPower(tile_cycles=128, gates_per_wg=256) as eng:
+        eng.set_design(d.library, d)
+        for k, tr in enumerate(traces):
+            eng.read_power_activities(tr)
+            same(res["1"][k], eng.power(), f"trace {k} alone")
