@@ -71,13 +71,17 @@ def copy_calibration(mib, reps=5):
     log(f"copy calibration: {reps} copies of {mib} MiB (read {mib} MiB + write {mib} MiB each)")
 
 
-def batch_leg(eng, design, w, n_trace, max_cycles, seed):
+def batch_leg(eng, design, w, n_trace, max_cycles, seed, schedule):
     """Batching: n_trace traces of the netlist with log-uniform lengths in
     [max_cycles / 20, max_cycles] and toggle p in [0.05, 0.3], each with its own
     seed, evaluated by one batched run (gtp_run_batch) and, for comparison, one
-    run per trace (gtp_set_trace + gtp_run)."""
+    run per trace (gtp_set_trace + gtp_run).  The batched run takes the exact
+    leg's schedule (autotuned on the same netlist; an engine of its own with
+    that schedule); each per-trace run takes the engine's default schedule for
+    its trace (as a caller without a tuning pass gets)."""
     import numpy as np
     from gtpower_amd import synthetic as S
+    from gtpower_amd.power import HipPower
     rng = np.random.default_rng(seed)
     lo = max(1, max_cycles // 20)
     lengths = np.exp(rng.uniform(np.log(lo), np.log(max_cycles), n_trace)).astype(int)
@@ -94,9 +98,13 @@ def batch_leg(eng, design, w, n_trace, max_cycles, seed):
     n_ev = int(sum(t.n_event for t in trs))
     log(f"batch: {n_trace} traces, {n_cyc} cycles ({lengths.min()}..{lengths.max()}), {n_ev} events, "
         f"generated in {gen_s:.1f} s")
-    eng.power_batch(trs, want=("cycle",))  # warm-up
-    b = eng.power_batch(trs, want=("cycle",))
-    batch_ms = b[0].kernel_ms
+    tc, rg = (int(x) for x in schedule)
+    log(f"batch: batched run with tile_cycles {tc}, gates_per_wg {rg}")
+    with HipPower(tile_cycles=tc, gates_per_wg=rg) as eb:
+        eb.set_design(design.library, design)
+        eb.power_batch(trs, want=("cycle",))  # warm-up
+        b = eb.power_batch(trs, want=("cycle",))
+        batch_ms = b[0].kernel_ms
     single_ms = 0.0
     for tr in trs:
         eng.read_power_activities(tr)
@@ -344,7 +352,8 @@ def main():
     # after the surrogate leg: the batch replaces the resident trace
     batch = None
     if args.batch_traces > 0:
-        batch = batch_leg(eng, design, w, args.batch_traces, args.batch_max_cycles, S.MASTER_SEED ^ 0xBA7C4)
+        batch = batch_leg(eng, design, w, args.batch_traces, args.batch_max_cycles, S.MASTER_SEED ^ 0xBA7C4,
+                          (sched.get("tile_cycles"), sched.get("gates_per_wg")))
     eng.close()
 
     vcd = None
