@@ -2,14 +2,18 @@
 //
 //   pack_events_kernel     int64 time + u8 value -> one 32-bit event word (device event store)
 //   tile_index_kernel      per (net, tile): first event at or after the tile start
+//   tile_order_kernel      per (range, tile): the range's gates ordered by their events in
+//                          the tile, dealt to the merge's waves (lane balance)
 //   energy_kernel          Liberty table interpolation per gate energy slot, tables staged in LDS
 //   power_tile_kernel<P>   fused per-gate event merge over one cycle tile: toggle and glitch
 //                          filtering, internal / switching / leakage evaluation, per-cycle
 //                          binning in LDS and per-gate sums, all int64 fixed point
 //   cycle_reduce_kernel    per-cycle sum over gate ranges -> FP64 waveforms
 //   gate_reduce_kernel     per-gate sum over tiles (int64) -> FP64 per-gate power
-//   glitch_annotate_kernel clock-cycle glitch flags, once per net event
-//   power_cell_kernel<NP>  the fused kernel specialised per single-output cell type without `when`
+//   glitch_annotate_kernel clock-cycle glitch flags, once per net event (glitch_annotate_tiles_kernel:
+//                          the events of listed tiles)
+//   power_cell_kernel<NP, KC>  the fused kernel specialised per single-output cell type without
+//                          `when`; KC >= 0: flip-flops whose clock edges the workgroup takes once
 //   total_*_kernel         FP64 design and class totals
 //
 // Reference counterparts: kernel1AllPowerCalculationTimeRangePartitionedByCycleUnit
