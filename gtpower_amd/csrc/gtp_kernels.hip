@@ -1181,12 +1181,20 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
   __shared__ long long ck_l[2];               // clock net: first event, events
   __shared__ int ck_s[8];                     // clock net, first edge index, first-event flag, edges, overflow
   __shared__ unsigned long long ck_t[4];      // scan scratch
+  __shared__ int4 ck_lk[CLKB ? 4 : 1];        // per non-clock state: leakage quanta and W, clock low and high
   const CellDev cd = a.cells[cell];
   const uint32_t clk = cd.clock_mask;
   const uint32_t arcm = cd.arc_mask;  // inputs that can be the related pin of an output event
   for (int i = threadIdx.x; i < NSTATE; i += blockDim.x) {
     s_leak[i] = a.leak[cd.leak_off + i];
     s_leakq[i] = a.leak_q[cd.leak_off + i];
+  }
+  if constexpr (CLKB) {
+    for (int j = threadIdx.x; j < 4; j += blockDim.x) {
+      const uint32_t s0 = ck_unpack<KC>((uint32_t)j, 0u), s1 = ck_unpack<KC>((uint32_t)j, 1u);
+      ck_lk[j] = make_int4((int32_t)a.leak_q[cd.leak_off + s0], (int32_t)a.leak_q[cd.leak_off + s1],
+                           __float_as_int(a.leak[cd.leak_off + s0]), __float_as_int(a.leak[cd.leak_off + s1]));
+    }
   }
   const int n_copy = a.bin_copies;
   long long* mybins = bins + (threadIdx.x % n_copy) * BW;
@@ -1564,9 +1572,10 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
         h_prev = H;
         const uint32_t sbe = (sb & ~(1u << KC)) | (bit << KC);
         const uint32_t sa = (sbe & ~(1u << ps)) | ((nv == 1 ? 1u : 0u) << ps);
-        const uint32_t sa0 = sa & ~(1u << KC), sa1 = sa | (1u << KC);
-        const int32_t nlo = (int32_t)s_leakq[sa0], nhi = (int32_t)s_leakq[sa1];
-        const float flo = s_leak[sa0], fhi = s_leak[sa1];
+        const uint32_t sa0 = sa & ~(1u << KC);
+        const int4 lk = ck_lk[ck_pack<KC>(sa)];
+        const int32_t nlo = lk.x, nhi = lk.y;
+        const float flo = __int_as_float(lk.z), fhi = __int_as_float(lk.w);
         const bool abit = (sa >> KC) & 1u;
         const long long lq_c = bit ? lq_hi : lq_lo, lq_new = abit ? nhi : nlo;
         const float lf_c = bit ? lf_hi : lf_lo, lf_new = abit ? fhi : flo;
