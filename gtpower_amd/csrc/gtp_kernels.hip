@@ -1172,8 +1172,9 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
   __shared__ long long s_item;
   constexpr int CKE = CLKB ? CK_SLOTS : 1;
   constexpr int CKC = CLKB ? CK_TILE_MAX + 2 : 1;
-  __shared__ uint32_t ck_w[CKE];              // time << 3 | not-a-rise << 2 | value
-  __shared__ uint32_t ck_h[CKE];              // clock-high ticks from the tile start to the edge
+  // per slot: clock-high ticks from the tile start to the edge << 32 | the
+  // word (time << 3 | not-a-rise << 2 | value)
+  __shared__ unsigned long long ck_wh[CKE];
   __shared__ unsigned long long ck_d[CKE];    // gates per non-clock state (16 bits each): differences, then prefix
   __shared__ uint16_t ck_cf[CKC];             // first slot in each tile cycle (the clamped column)
   __shared__ long long ck_q[4];               // summed edge energies of the item's gates, per edge kind
@@ -1262,8 +1263,8 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
             fx = 1;
           }
         }
-        ck_w[0] = (tstart << EV_SHIFT) | (l & 3u);
-        ck_h[0] = 0;
+        ck_wh[0] = (tstart << EV_SHIFT) | (l & 3u);
+        ck_d[0] = 0;
       } else {
         cnet = -1;
       }
@@ -1286,7 +1287,7 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
       const int64_t cnb = ck_l[0], ccn = ck_l[1];
       const int32_t fi = ck_s[1];
       const bool fx = ck_s[2] != 0;
-      const uint32_t w0 = ck_w[0];
+      const uint32_t w0 = (uint32_t)ck_wh[0];
       constexpr int NE_MAX = CK_SLOTS - 3;
       // edge k in slot k + 1; k = NE_MAX probes for one edge too many
       for (int k = threadIdx.x; k <= NE_MAX; k += blockDim.x) {
@@ -1306,8 +1307,8 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
         // TimeBasedPower.cc:420-431: after the net's first event (not at
         // time 0) the edge counts as not a rise
         const uint32_t nr = (k == 0 && fx) ? 1u : (r2 > 0 ? 0u : 1u);
-        ck_w[k + 1] = (t << EV_SHIFT) | (nr << 2) | (uint32_t)nv;
-        ck_h[k + 1] = pv == 1 ? t - pt : 0u;
+        ck_wh[k + 1] = (t << EV_SHIFT) | (nr << 2) | (uint32_t)nv;
+        ck_d[k + 1] = pv == 1 ? t - pt : 0u;  // clock-high ticks since the slot before, scanned below
         if (tx2 > 0) atomicAdd(&ck_n[(r2 > 0 ? 0 : 2) + (tx2 == 2 ? 1 : 0)], 1);
         atomicAdd(&ck_s[3], 1);
       }
@@ -1315,23 +1316,26 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
     __syncthreads();
     int ne = (cnet >= 0 && ck_s[4] == 0) ? ck_s[3] : -1;
     if (ne >= 0) {
-      block_scan_inplace(ck_h, ne, reinterpret_cast<uint32_t*>(ck_t));
+      block_scan_inplace(ck_d, ne, ck_t);
+      for (int sl = threadIdx.x; sl <= ne; sl += blockDim.x) ck_wh[sl] |= ck_d[sl] << 32;
+      __syncthreads();
       // first slot of each tile cycle; more than two edges in a cycle fall back
       for (int sl = 1 + (int)threadIdx.x; sl <= ne; sl += blockDim.x) {
         const auto col = [&](int q) {
-          return min((int)((int64_t)cycle_mul(ev_time(ck_w[q]), a.cyc_m, a.cyc_sh) - tc0), TC);
+          return min((int)((int64_t)cycle_mul(ev_time((uint32_t)ck_wh[q]), a.cyc_m, a.cyc_sh) - tc0), TC);
         };
         const int ce = col(sl);
         const int cp = sl == 1 ? -1 : col(sl - 1);
         for (int ci = cp + 1; ci <= ce; ++ci) ck_cf[ci] = (uint16_t)sl;
         if (sl + 2 <= ne && col(sl + 2) == ce) ck_s[4] = 1;
       }
-      const int last = ne >= 1 ? min((int)((int64_t)cycle_mul(ev_time(ck_w[ne]), a.cyc_m, a.cyc_sh) - tc0), TC) : -1;
+      const int last =
+          ne >= 1 ? min((int)((int64_t)cycle_mul(ev_time((uint32_t)ck_wh[ne]), a.cyc_m, a.cyc_sh) - tc0), TC) : -1;
       for (int ci = last + 1 + (int)threadIdx.x; ci <= TC + 1; ci += blockDim.x) ck_cf[ci] = (uint16_t)(ne + 1);
       for (int sl = threadIdx.x; sl <= ne + 1; sl += blockDim.x) ck_d[sl] = 0;
       if (threadIdx.x == 0) {
-        ck_w[ne + 1] = EV_INF;
-        ck_w[ne + 2] = EV_INF;
+        ck_wh[ne + 1] = EV_INF;
+        ck_wh[ne + 2] = EV_INF;
       }
       __syncthreads();
       if (ck_s[4] != 0) ne = -1;
@@ -1551,14 +1555,17 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
         int el = 0;
         if (bulk) {
           const bool tie = ps != OUT && KC < ps;
+          // the slots before, at and after the cycle's first: loaded together
           const int e = ck_cf[cr];
-          const uint32_t we0 = ck_w[e], we1 = ck_w[e + 1];
+          const unsigned long long s0 = ck_wh[e - 1], s1 = ck_wh[e], s2 = ck_wh[e + 1];
+          const uint32_t we0 = (uint32_t)s1, we1 = (uint32_t)s2;
           const bool c0 = ev_time(we0) < t || (ev_time(we0) == t && tie);
           const bool c1 = ev_time(we1) < t || (ev_time(we1) == t && tie);
           el = e - 1 + (c0 ? 1 : 0) + (c1 ? 1 : 0);
-          const uint32_t wl = ck_w[el];
+          const unsigned long long sl = c1 ? s2 : (c0 ? s1 : s0);
+          const uint32_t wl = (uint32_t)sl;
           bit = (wl & 3u) == 1u ? 1u : 0u;
-          H = ck_h[el] + (bit ? t - ev_time(wl) : 0u);
+          H = (uint32_t)(sl >> 32) + (bit ? t - ev_time(wl) : 0u);
           if (el > 0) {
             lw[KC] = wl;  // the latest consumed clock event, for the related-input search
             rflm = (rflm & ~(1u << KC)) | (((wl >> 2) & 1u) << KC);
@@ -1709,9 +1716,10 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
       // to the tile end, or to the latest event consumed (an edge included)
       uint32_t endp = max(lend, prev_t), H;
       if (bulk) {
-        const uint32_t wn = ck_w[ck_ne];
+        const unsigned long long sn = ck_wh[ck_ne];
+        const uint32_t wn = (uint32_t)sn;
         endp = max(endp, ev_time(wn));
-        H = ck_h[ck_ne] + ((wn & 3u) == 1u ? endp - ev_time(wn) : 0u);
+        H = (uint32_t)(sn >> 32) + ((wn & 3u) == 1u ? endp - ev_time(wn) : 0u);
         // the edges' internal energy and toggles, per edge kind
         // (the clock pin's own table at its edge, as for any input event)
         const float Er = E[2 * KC], Ef = E[2 * KC + 1];
@@ -1750,8 +1758,8 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
     if (ck_ne >= 0) {
       block_scan_inplace(ck_d, ck_ne, ck_t);
       for (int sl = 1 + (int)threadIdx.x; sl <= ck_ne; sl += blockDim.x) {
-        const uint32_t w = ck_w[sl];
-        const int pv = ev_val(ck_w[sl - 1]), nv = ev_val(w);
+        const uint32_t w = (uint32_t)ck_wh[sl];
+        const int pv = ev_val((uint32_t)ck_wh[sl - 1]), nv = ev_val(w);
         const uint32_t t = ev_time(w);
         const uint32_t c = cycle_mul(t, a.cyc_m, a.cyc_sh);
         const uint32_t cend = c * T + T;
