@@ -827,22 +827,30 @@ struct RunPlan {
   PowerArgs a{};
 };
 
-// Fixed-point quanta: powers of two with one term <= 2^30 quanta, so a term
-// converts through int32 and every sum is exact in int64 (a cycle would need
-// 2^33 terms, a gate's leakage 2^33 ticks of trace, to overflow).  Uploads
-// the leakage table in quanta.
+// Fixed-point quanta, powers of two.  Energy: one term <= 2^36 quanta (the
+// kernels take the coarse scale, 2^ENERGY_FINE_BITS quanta per unit, one term
+// <= 2^20 units, and convert through two exact int32 steps), so a term 1e-5 of
+// the design's largest still carries a rounding error below 1e-6 of itself;
+// the int64 sums stay exact while one cycle of one gate range, or one gate
+// over the trace, adds up less than 2^26 times the largest term.  Leakage: one
+// value <= 2^30 quanta, converted through int32 (a gate's leakage sum would
+// need 2^33 ticks of trace to overflow).  Uploads the leakage table in quanta.
 struct Quanta {
-  float inv_e = 0, inv_l = 0;
-  double q_e = 0, q_l = 0;
+  float inv_e = 0, inv_l = 0;   // kernel scales: coarse energy units, leakage quanta
+  double q_e = 0, q_l = 0;      // the quanta (J, W)
 };
 
-gtp_status make_quanta(gtp_ctx* x, int64_t t_max, Quanta* q) {
+gtp_status make_quanta(gtp_ctx* x, int64_t t_max, const std::vector<int64_t>& work, Quanta* q) {
+  // a gate's per-trace energy sum: at most its pin events x the largest term
+  const int64_t wmax = work.empty() ? 0 : *std::max_element(work.begin(), work.end());
+  if (wmax >= ((int64_t)1 << (62 - ENERGY_TERM_BITS)))
+    return x->fail(GTP_ERR_LIMIT, "gtp_run: a gate has 2^26 or more pin events in one trace (int64 energy sums)");
   double emax = std::max((double)x->energy_max * 1.000001, x->csw_max);
   if (!(emax > 0)) emax = 1e-30;
-  const int ke = (int)std::floor(30.0 - std::log2(emax));
+  const int ke = (int)std::floor((double)ENERGY_TERM_BITS - std::log2(emax));
   const double lmax = x->leak_max > 0 ? x->leak_max * 1.000001 : 1e-30;
   const int kl = (int)std::floor(30.0 - std::log2(lmax));
-  q->inv_e = std::ldexp(1.0f, ke);
+  q->inv_e = std::ldexp(1.0f, ke - ENERGY_FINE_BITS);
   q->inv_l = std::ldexp(1.0f, kl);
   q->q_e = std::ldexp(1.0, -ke);
   q->q_l = std::ldexp(1.0, -kl);
@@ -975,7 +983,7 @@ gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, int32_t n_work_tiles, RunPlan
   }
   Quanta qt;
   {
-    const gtp_status st = make_quanta(x, x->t_max, &qt);
+    const gtp_status st = make_quanta(x, x->t_max, x->gate_work, &qt);
     if (st != GTP_OK) return st;
   }
   // scratch
@@ -1317,7 +1325,7 @@ gtp_status multi_setup(gtp_ctx* x, int64_t T, double time_scale, double clk_peri
     if (st != GTP_OK) return st;
   }
   {
-    const gtp_status st = make_quanta(x, tmax_max, &mp->qt);
+    const gtp_status st = make_quanta(x, tmax_max, mp->work, &mp->qt);
     if (st != GTP_OK) return st;
   }
   // tile length: the option, else the autotuner's choice on the resident

@@ -12,9 +12,11 @@
 //
 // Results are accumulated as signed 64-bit fixed point: every FP32-evaluated
 // term x is added as rint(x / quantum), with a power-of-two quantum such that
-// one term is at most 2^30 quanta.  The sums are then exact integer sums:
-// order-free and bit-reproducible.  Per-gate leakage is the exact integer
-// sum of (state leakage in quanta) * (ticks in that state).
+// one energy term is at most 2^36 quanta (a term 1e-5 of the design's largest
+// is still resolved to 1e-6) and one leakage value at most 2^30 leakage
+// quanta.  The sums are then exact integer sums: order-free and
+// bit-reproducible.  Per-gate leakage is the exact integer sum of (state
+// leakage in quanta) * (ticks in that state).
 #pragma once
 #include <stdint.h>
 #include <stdlib.h>
@@ -65,6 +67,8 @@ constexpr int MAXP_SMALL = 4;                   // kernel variant for cells with
 constexpr int MAXP_LARGE = 6;                   // GTP_MAX_CELL_PINS
 constexpr int MAX_CELLS_LDS = 64;               // cell types staged in LDS
 constexpr int ARC_FRAC_BITS = 16;               // related-pin comparison grid: 2^-16 tick
+constexpr int ENERGY_FINE_BITS = 16;            // energy quanta per coarse unit of PowerArgs.inv_q_energy
+constexpr int ENERGY_TERM_BITS = 36;            // one energy term <= 2^36 quanta
 constexpr int32_t ARC_NONE = INT32_MIN;         // arc delays are int32 on that grid (< 2^15 ticks)
 // Grid cap of kernels that stride over per-event work: a dispatch counts its
 // work-items in 32 bits, so blocks x threads must stay below 2^32.
@@ -176,7 +180,7 @@ struct PowerArgs {
   int32_t n_leak;
   int32_t bin_copies;            // lane-interleaved copies of the per-cycle bins (specialised kernel)
   // fixed-point scales (powers of two): term / quantum
-  float inv_q_energy;
+  float inv_q_energy;            // coarse energy scale: one term <= 2^20 units, 2^ENERGY_FINE_BITS quanta each
   float inv_q_leak;
   // outputs
   // (partials are indexed by work tile: the tile itself, or its position in tile_list)

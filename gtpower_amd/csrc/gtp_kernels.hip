@@ -73,10 +73,21 @@ __device__ __forceinline__ void ff_add_prod(float& hi, float& lo, float a, float
 __device__ __forceinline__ long long quant(float x, float inv_q) {
   return __float2ll_rn(x * inv_q);
 }
-// One term in quanta: |x * inv_q| <= 2^30 by the choice of quantum, so a
-// 32-bit conversion is exact.
+// One leakage term in leakage quanta: |x * inv_q| <= 2^30 by the choice of
+// quantum, so a 32-bit conversion is exact.
 __device__ __forceinline__ long long qterm(float x, float inv_q) {
   return (long long)(int)rintf(x * inv_q);
+}
+// Energy quanta are 2^ENERGY_FINE_BITS times finer than the coarse scale the
+// kernels take (PowerArgs.inv_q_energy: one term <= 2^20 coarse units, so one
+// term <= 2^36 quanta).  rint(x * inv_q * 2^16) from two exact 32-bit
+// conversions: xs - rint(xs) is exact in FP32 and |.| <= 1/2, so its scaled
+// rounding is the low part.
+__device__ __forceinline__ long long qenergy(float x, float inv_q) {
+  const float xs = x * inv_q;
+  const float hi = rintf(xs);
+  const float lo = rintf((xs - hi) * (float)(1 << ENERGY_FINE_BITS));
+  return ((long long)(int)hi << ENERGY_FINE_BITS) + (long long)(int)lo;
 }
 
 // ---------------------------------------------------------------------------
@@ -434,6 +445,7 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
 
   const uint32_t T = (uint32_t)a.T;
   const float invT = a.inv_T;
+  const float inv_qf = a.inv_q_energy * (float)(1 << ENERGY_FINE_BITS);  // energy quanta (fine)
   const int64_t tc0 = (int64_t)tile * TC;
   const int tcn = (int)min((int64_t)TC, wk.n_cycle - tc0);
   const bool last_tile = tile == wk.n_tile - 1;
@@ -550,10 +562,10 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
 
     auto flush_pending = [&]() {
       if (pc >= 0 && pc < tcn) {
-        if (p_int != 0.f) atomicAdd((unsigned long long*)&bins[BIN_INT * TC + pc], (unsigned long long)quant(p_int, a.inv_q_energy));
-        if (p_gint != 0.f) atomicAdd((unsigned long long*)&bins[BIN_GINT * TC + pc], (unsigned long long)quant(p_gint, a.inv_q_energy));
-        if (p_sw != 0.f) atomicAdd((unsigned long long*)&bins[BIN_SW * TC + pc], (unsigned long long)quant(p_sw, a.inv_q_energy));
-        if (p_gsw != 0.f) atomicAdd((unsigned long long*)&bins[BIN_GSW * TC + pc], (unsigned long long)quant(p_gsw, a.inv_q_energy));
+        if (p_int != 0.f) atomicAdd((unsigned long long*)&bins[BIN_INT * TC + pc], (unsigned long long)quant(p_int, inv_qf));
+        if (p_gint != 0.f) atomicAdd((unsigned long long*)&bins[BIN_GINT * TC + pc], (unsigned long long)quant(p_gint, inv_qf));
+        if (p_sw != 0.f) atomicAdd((unsigned long long*)&bins[BIN_SW * TC + pc], (unsigned long long)quant(p_sw, inv_qf));
+        if (p_gsw != 0.f) atomicAdd((unsigned long long*)&bins[BIN_GSW * TC + pc], (unsigned long long)quant(p_gsw, inv_qf));
         if (p_ld != 0) atomicAdd((unsigned long long*)&bins[BIN_LD * TC + pc], (unsigned long long)p_ld);
         if (p_lw != 0.f) atomicAdd((unsigned long long*)&bins[BIN_LW * TC + pc], (unsigned long long)quant(p_lw, a.inv_q_leak));
       }
@@ -702,7 +714,7 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
     const int64_t plane = (int64_t)a.n_work_tiles * a.n_sched;
 #pragma unroll
     for (int k = 0; k < N_PG; ++k) {
-      const float iq = k == PG_LEAK ? a.inv_q_leak : a.inv_q_energy;  // powers of two: exact scaling
+      const float iq = k == PG_LEAK ? a.inv_q_leak : inv_qf;  // powers of two: exact scaling
       a.pg[k * plane + slot] = __float2ll_rn(acc_hi[k] * iq) + __float2ll_rn(acc_lo[k] * iq);
     }
     a.pg_tog[slot] = tog;
@@ -1646,13 +1658,13 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
       // switching: rising output edges
       {
         const float es = is_out ? 0.5f * (float)r2 * csw : 0.f;
-        const long long qs = qterm(gl ? ratio * es : es, inv_qe);
+        const long long qs = qenergy(gl ? ratio * es : es, inv_qe);
         acc[PG_SW] += qs;  // all switching; the glitch part is taken out after the loop
         acc[PG_GSW] += gl ? qs : 0;
         atomicAdd((unsigned long long*)&cb[(gl ? BIN_GSW : BIN_SW) * TCP], (unsigned long long)qs);
       }
       const float e = En * (0.5f * (float)tx2);
-      const long long qi = qterm(gl ? ratio * e : e, inv_qe);
+      const long long qi = qenergy(gl ? ratio * e : e, inv_qe);
       acc[PG_INT] += qi;  // all internal; the glitch part is taken out after the loop
       acc[PG_GINT] += gl ? qi : 0;
       atomicAdd((unsigned long long*)&cb[(gl ? BIN_GINT : BIN_INT) * TCP], (unsigned long long)qi);
@@ -1727,7 +1739,7 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
         for (int k = 0; k < 4; ++k) {
           const int tx = (k & 1) ? 2 : 1;
           const float En = (k >> 1) ? Ef : Er;
-          const long long q = qterm(En * (0.5f * (float)tx), inv_qe);
+          const long long q = qenergy(En * (0.5f * (float)tx), inv_qe);
           const int n = ck_n[k];
           acc[PG_INT] += (long long)n * q;
           tog += (uint32_t)(n * tx);
