@@ -339,7 +339,7 @@ def main():
         rs = eng.power_sliced(trace, max_events=budget, want=("cycle",))
         sl_s = time.perf_counter() - t0
         same = bool(np.array_equal(rs.cycle_power, r.cycle_power))
-        sliced = {"intervals": rs.schedule["split_target"], "event_budget": budget, "wall_ms": sl_s * 1e3,
+        sliced = {"intervals": rs.schedule["n_interval"], "event_budget": budget, "wall_ms": sl_s * 1e3,
                   "kernel_ms": rs.kernel_ms, "cycles_per_s": n_cycle / sl_s,
                   "identical_to_resident_run": same,
                   "note": "wall time includes slicing the host arrays and uploading every interval"}

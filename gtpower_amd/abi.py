@@ -38,9 +38,8 @@ p_u8 = C.POINTER(C.c_uint8)
 
 class GtpOptions(C.Structure):
     _fields_ = [("device", C.c_int32), ("tile_cycles", C.c_int32), ("block_lanes", C.c_int32),
-                ("gates_per_wg", C.c_int32), ("split_target", C.c_int32), ("autotune", C.c_int32),
-                ("max_event_num", C.c_int64), ("host_threads", C.c_int32), ("verbose", C.c_int32),
-                ("round_tiles", C.c_int32), ("pad_", C.c_int32)]
+                ("gates_per_wg", C.c_int32), ("autotune", C.c_int32), ("host_threads", C.c_int32),
+                ("max_event_num", C.c_int64), ("round_tiles", C.c_int32), ("pad_", C.c_int32)]
 
 
 class GtpTable(C.Structure):
@@ -83,9 +82,9 @@ class GtpResult(C.Structure):
 
 class GtpScheduleInfo(C.Structure):
     _fields_ = [("tile_cycles", C.c_int32), ("block_lanes", C.c_int32), ("gates_per_wg", C.c_int32),
-                ("split_target", C.c_int32), ("n_unit", C.c_int64), ("n_workgroup", C.c_int64),
-                ("n_tile", C.c_int64), ("n_event", C.c_int64), ("n_pin_event", C.c_int64),
-                ("autotune_ms", C.c_double)]
+                ("n_launch", C.c_int32), ("n_interval", C.c_int64), ("n_unit", C.c_int64),
+                ("n_workgroup", C.c_int64), ("n_tile", C.c_int64), ("n_event", C.c_int64),
+                ("n_pin_event", C.c_int64), ("autotune_ms", C.c_double)]
 
 
 class GtpMlpConfig(C.Structure):

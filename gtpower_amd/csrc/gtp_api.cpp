@@ -122,7 +122,6 @@ struct gtp_ctx {
   int32_t tile_idx_tc = 0;
   // per-tile lane order of the full run's schedule (built with the tile index)
   DArr<int32_t> d_perm;
-  DArr<int32_t> d_perm_batch;   // the same per work tile of a batched run
   int64_t perm_tc = 0;
   int32_t perm_rg = 0, perm_lanes = 0;
   std::vector<int64_t> h_net_off;  // host copy of the event offsets
@@ -1204,7 +1203,8 @@ gtp_status run_once(gtp_ctx* x, gtp_result* res, int64_t tc_req, int32_t rg_req)
   si.gates_per_wg = rp.RG;
   si.n_unit = (int64_t)rp.n_tile * x->G;
   si.n_workgroup = (int64_t)rp.n_tile * rp.n_range;
-  si.split_target = (int32_t)rp.os->launches.size();
+  si.n_launch = (int32_t)rp.os->launches.size();
+  si.n_interval = 1;
   si.n_tile = rp.n_tile;
   si.n_event = x->E;
   si.n_pin_event = x->P_ev;
@@ -1497,22 +1497,27 @@ gtp_status run_load(gtp_ctx* x, MultiPlan& mp, const std::vector<RunItem>& items
     HIPCHK(x, launch_tile_index(d_voff.p + (int64_t)k * N, d_ev.p, N, items[k].n_tile, TC * T, nullptr, 0,
                                 d_tidx.p + td[k].tile_idx_base, s));
   HIPCHK(x, launch_glitch_annotate(d_voff.p, d_ev.p, d_chunk.p, d_chunk_net.p, n_chunk, (uint32_t)T, s));
-  // per-work-tile lane order, as in full runs (GTP_TILE_ORDER=0 or without
-  // the memory for it: schedule order)
+  // per-work-tile lane order, as in full runs (GTP_TILE_ORDER=0, or when it
+  // would take more than half the device memory still free after the load's
+  // own buffers: schedule order).  Freed with the load.
   const int64_t n_sched_b = (int64_t)os.order.size();
   int32_t* perm_b = nullptr;
+  DArr<int32_t> d_perm;
   {
     bool use = rp.RG <= TILE_ORDER_MAX_RANGE;
     if (const char* e = std::getenv("GTP_TILE_ORDER"))
       if (std::atoi(e) == 0) use = false;
+    size_t free_b = 0, total_b = 0;
+    if (use && hipMemGetInfo(&free_b, &total_b) != hipSuccess) use = false;
+    if (use && (double)n_wt * (double)n_sched_b * 4.0 > 0.5 * (double)free_b) use = false;
     if (use) {
-      if (x->d_perm_batch.alloc((size_t)n_wt * n_sched_b) != hipSuccess) {
+      if (d_perm.alloc((size_t)n_wt * n_sched_b) != hipSuccess) {
         (void)hipGetLastError();
-        x->d_perm_batch.release();
+        d_perm.release();
       } else {
         HIPCHK(x, launch_tile_order(os.d_order.p, n_sched_b, rp.RG, rp.block, x->d_gate_pin_off.p, x->d_pin_net.p,
-                                    d_tidx.p, 0, n_wt, d_wt_tile.p, d_wt_trace.p, d_td.p, x->d_perm_batch.p, s));
-        perm_b = x->d_perm_batch.p;
+                                    d_tidx.p, 0, n_wt, d_wt_tile.p, d_wt_trace.p, d_td.p, d_perm.p, s));
+        perm_b = d_perm.p;
       }
     }
   }
@@ -1600,7 +1605,8 @@ void record_multi_schedule(gtp_ctx* x, const MultiPlan& mp, int64_t n_event) {
   si.gates_per_wg = mp.RG;
   si.n_unit = mp.total_tiles * x->G;
   si.n_workgroup = mp.total_tiles * mp.n_range;
-  si.split_target = (int32_t)mp.rp.os->launches.size();
+  si.n_launch = (int32_t)mp.rp.os->launches.size();
+  si.n_interval = 1;
   si.n_tile = mp.total_tiles;
   si.n_event = n_event;
   si.n_pin_event = std::accumulate(mp.work.begin(), mp.work.end(), (int64_t)0);
@@ -1652,13 +1658,16 @@ GTP_API gtp_status gtp_run_batch(gtp_ctx* x, int32_t n_trace, const gtp_trace* t
   const int64_t TC = mp.TC;
   std::vector<int32_t> n_tile(n_trace);
   for (int32_t k = 0; k < n_trace; ++k) n_tile[k] = (int32_t)((C[k] + TC - 1) / TC);
+  // loads under the budget, in 4-byte words: a trace's event words, its tile
+  // index (N x (tiles + 1)) and its per-net offsets and block maps (4 words a net)
   std::vector<std::pair<int32_t, int32_t>> loads;
+  auto load_words = [&](int32_t k) { return E[k] + N * ((int64_t)n_tile[k] + 1) + 4 * N; };
   for (int32_t k = 0; k < n_trace;) {
-    if (E[k] > mp.ev_budget)
+    if (load_words(k) > mp.ev_budget)
       return x->fail(GTP_ERR_OOM, "gtp_run_batch: trace " + std::to_string(k) + " alone exceeds the event budget");
     int64_t e = 0;
     int32_t j = k;
-    while (j < n_trace && e + E[j] <= mp.ev_budget) e += E[j++];
+    while (j < n_trace && e + load_words(j) <= mp.ev_budget) e += load_words(j++);
     loads.emplace_back(k, j);
     k = j;
   }
@@ -1729,7 +1738,13 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
   }
   const int64_t TC = mp.TC;
   const int32_t n_tile = (int32_t)((C + TC - 1) / TC);
-  const int64_t budget = max_events > 0 ? std::min(max_events, mp.ev_budget) : mp.ev_budget;
+  // event words of one interval: the budget less the interval load's tile
+  // index (the whole trace's tiles) and per-net offsets and block maps
+  const int64_t fixed_words = N * ((int64_t)n_tile + 1) + 4 * N;
+  const int64_t budget = max_events > 0 ? std::min(max_events, mp.ev_budget - fixed_words)
+                                        : mp.ev_budget - fixed_words;
+  if (budget <= 0)
+    return x->fail(GTP_ERR_OOM, "gtp_run_sliced: the tile index alone exceeds the device event budget");
   // The slice of net n for tiles [k0, k1): its events from the one before the
   // cycle group of the last event before the interval (so that group's
   // positions and the state before the interval come out as in a whole-trace
@@ -1831,7 +1846,7 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
   res->kernel_ms = kernel_ms_total;
   res->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
   record_multi_schedule(x, mp, E);
-  x->sched.split_target = (int32_t)ivals.size();
+  x->sched.n_interval = (int64_t)ivals.size();
   return GTP_OK;
 }
 

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define GTP_ABI_VERSION 1
+#define GTP_ABI_VERSION 2
 
 typedef int32_t gtp_status;
 enum {
@@ -71,11 +71,9 @@ typedef struct {
   int32_t tile_cycles;       /* cycles per work tile; 0 = chosen by the autotuner */
   int32_t block_lanes;       /* lanes (gates) per workgroup pass; 0 = autotuned */
   int32_t gates_per_wg;      /* gates a workgroup walks per tile; 0 = autotuned */
-  int32_t split_target;      /* target events per work unit for the density split; 0 = default */
   int32_t autotune;          /* 1 = time candidate schedules on the loaded trace */
-  int64_t max_event_num;     /* device event budget (-max_event_num); 0 = no limit */
   int32_t host_threads;      /* host worker threads (-multi_thread_number); 0 = all cores */
-  int32_t verbose;           /* 0 quiet, 1 timings to stderr */
+  int64_t max_event_num;     /* device event budget (-max_event_num); 0 = no limit */
   int32_t round_tiles;       /* batched runs: work tiles per round; 0 = sized by device memory */
   int32_t pad_;
 } gtp_options;
@@ -240,16 +238,24 @@ gtp_status gtp_run_batch(gtp_ctx* ctx, int32_t n_trace, const gtp_trace* traces,
  * gtp_run over the whole trace.  The resident trace of gtp_set_trace is not
  * touched.  Replaces the reference's multi-round interval loop
  * (power/cuda/CudaPower.cu:1261-1314, getTimeIntervals / renewGateWaveform
- * :1421-1599); the interval count is reported in gtp_schedule_info.split_target.
+ * :1421-1599); the interval count is reported in gtp_schedule_info.n_interval.
  * kernel_ms sums the intervals' device time; total_ms is the call's wall time
  * (host slicing and uploads included). */
 gtp_status gtp_run_sliced(gtp_ctx* ctx, const gtp_trace* trace, int64_t max_events, gtp_result* result);
 
-/* Schedule of the last run: tile length, lanes, workgroups, units. */
+/* Schedule of the last run. */
 typedef struct {
-  int32_t tile_cycles, block_lanes, gates_per_wg, split_target;
-  int64_t n_unit, n_workgroup, n_tile, n_event, n_pin_event;
-  double autotune_ms;
+  int32_t tile_cycles;   /* cycles per work tile */
+  int32_t block_lanes;   /* lanes (gates) per workgroup pass */
+  int32_t gates_per_wg;  /* gates of one work item (a gate range of one tile) */
+  int32_t n_launch;      /* power-kernel launches of one evaluation (one per cell kernel) */
+  int64_t n_interval;    /* gtp_run_sliced: cycle intervals loaded one after another; else 1 */
+  int64_t n_unit;        /* (gate, tile) pairs */
+  int64_t n_workgroup;   /* work items (gate range, tile) */
+  int64_t n_tile;        /* work tiles */
+  int64_t n_event;       /* net events */
+  int64_t n_pin_event;   /* pin events (net events x the pins reading them) */
+  double autotune_ms;    /* time the autotuner spent */
 } gtp_schedule_info;
 gtp_status gtp_get_schedule(const gtp_ctx* ctx, gtp_schedule_info* info);
 

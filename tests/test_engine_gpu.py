@@ -233,7 +233,7 @@ def test_sliced_equals_full_run(parts):
         a = eng.power()
         b = eng.power_sliced(tr, max_events=(tr.n_event + parts - 1) // parts + 4 * d.n_net if parts > 1 else 0)
     if parts > 1:
-        assert b.schedule["split_target"] >= min(parts, 2)
+        assert b.schedule["n_interval"] >= min(parts, 2)
     assert b.n_cycle == a.n_cycle
     assert np.array_equal(a.gate_toggle_x2, b.gate_toggle_x2)
     assert np.array_equal(a.gate_glitch, b.gate_glitch)
@@ -261,7 +261,7 @@ def test_sliced_generic_kernel_equals_full_run():
         eng.read_power_activities(tr)
         a = eng.power()
         b = eng.power_sliced(tr, max_events=tr.n_event // 7 + 4 * d.n_net)
-    assert b.schedule["split_target"] >= 2
+    assert b.schedule["n_interval"] >= 2
     for key in ("gate_toggle_x2", "gate_glitch", "cycle_power", "gate_power", "total"):
         assert np.array_equal(getattr(a, key), getattr(b, key)), key
 
@@ -422,7 +422,7 @@ def test_long_clock_period_uses_the_generic_kernel():
         full = eng.power()
         sl = eng.power_sliced(tr, max_events=tr.n_event // 3 + 4 * d.n_net)
         both = eng.power_batch([tr, tr])
-    assert sl.schedule["split_target"] > 1
+    assert sl.schedule["n_interval"] > 1
     for other, what in ((sl, "sliced"), (both[1], "batched")):
         assert np.array_equal(other.gate_glitch, full.gate_glitch), what
         assert np.array_equal(other.cycle_power, full.cycle_power), what
