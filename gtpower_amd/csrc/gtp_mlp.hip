@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <cstring>
 #include <numeric>
 #include <random>
@@ -515,6 +516,38 @@ inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b
 using namespace gtp;
 using namespace gtp::mlp;
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free inference (gtp_mlp_predict_noalloc): feature
+// rows, the two hidden layers' activations and the outputs of one chunk of
+// rows.  Every GPU the library runs on holds its own copy until the process
+// ends.  Host code never passes `scratch` itself to a kernel or a copy (in host
+// code the name is a host-side stand-in, not a device address): it uses the
+// address hipGetSymbolAddress returns on each device, kept in noalloc_base[].
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mu;
+unsigned char* noalloc_base[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on device dev, which must be current;
+// resolved once per device.
+hipError_t noalloc_resolve(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  if (!noalloc_base[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    noalloc_base[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = noalloc_base[dev];
+  return hipSuccess;
+}
+}  // namespace
+
 struct gtp_mlp {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -600,27 +633,32 @@ hipError_t refresh_bf16(gtp_mlp* m) {
 
 // forward pass over n rows of bit features already on the device
 // (drop_seed != 0: training-time input dropout)
-hipError_t forward(gtp_mlp* m, const uint32_t* X, int n, bool keep_z, float* out, uint32_t drop_seed = 0) {
+// (h1, h2: the bf16 activations of the two hidden layers, [n][H]; null: the
+// model's batch workspace)
+hipError_t forward(gtp_mlp* m, const uint32_t* X, int n, bool keep_z, float* out, uint32_t drop_seed = 0,
+                   bf16* h1 = nullptr, bf16* h2 = nullptr) {
   const int H = m->H;
+  if (!h1) h1 = m->h1;
+  if (!h2) h2 = m->h2;
   GemmArgs g{};
   g.M = n; g.N = H; g.K = m->K;
   g.A = X; g.lda = m->words;
   g.drop_seed = drop_seed;
   g.B = m->W1b; g.ldb = m->K;
   g.C = keep_z ? m->z1 : nullptr; g.ldc = H;
-  g.Cb = m->h1; g.ldcb = H;
+  g.Cb = h1; g.ldcb = H;
   g.bias = m->b1();
   hipError_t e = gemm<A_BITS_ROW, B_TRANS, EPI_BIAS_GELU>(g, m->stream);
   if (e != hipSuccess) return e;
   GemmArgs g2{};
   g2.M = n; g2.N = H; g2.K = H;
-  g2.A = m->h1; g2.lda = H;
+  g2.A = h1; g2.lda = H;
   g2.B = m->W2b; g2.ldb = H;
   g2.C = keep_z ? m->z2 : nullptr; g2.ldc = H;
-  g2.Cb = m->h2; g2.ldcb = H;
+  g2.Cb = h2; g2.ldcb = H;
   g2.bias = m->b2();
   if ((e = gemm<A_ROW, B_TRANS, EPI_BIAS_GELU>(g2, m->stream)) != hipSuccess) return e;
-  head_forward_kernel<<<nblk(n, 4), 256, 0, m->stream>>>(m->h2, m->w3(), m->b3(), n, H, out);
+  head_forward_kernel<<<nblk(n, 4), 256, 0, m->stream>>>(h2, m->w3(), m->b3(), n, H, out);
   return hipGetLastError();
 }
 
@@ -708,6 +746,12 @@ GTP_API gtp_status gtp_mlp_create(const gtp_mlp_config* cfg, gtp_mlp** out) {
   (void)hipMemcpy(m->M2, zero.data(), pb, hipMemcpyHostToDevice);
   (void)refresh_bf16(m);
   (void)hipStreamSynchronize(m->stream);
+  // the allocation-free inference's scratch on this device (a failure here is
+  // reported by that entry point, which resolves it again)
+  {
+    unsigned char* base = nullptr;
+    if (noalloc_resolve(dev, &base) != hipSuccess) (void)hipGetLastError();
+  }
   *out = m;
   return GTP_OK;
 }
@@ -888,6 +932,58 @@ GTP_API gtp_status gtp_mlp_predict(gtp_mlp* m, const uint32_t* bits, int64_t n, 
     MCHK(m, hipMemcpyAsync(m->Xb, bits + b * m->words, (size_t)c * m->words * 4, hipMemcpyHostToDevice, m->stream));
     MCHK(m, forward(m, m->Xb, c, false, m->out));
     MCHK(m, hipMemcpyAsync(out + b, m->out, (size_t)c * 4, hipMemcpyDeviceToHost, m->stream));
+    MCHK(m, hipStreamSynchronize(m->stream));
+  }
+  for (int64_t i = 0; i < n; ++i) out[i] = out[i] * m->y_std + m->y_mean;
+  return GTP_OK;
+}
+
+// Rows of one chunk of gtp_mlp_predict_noalloc and the scratch offsets of its
+// parts (256-byte aligned): feature words, layer-1 and layer-2 activations,
+// outputs.  0 rows: a 128-row chunk of this model does not fit the scratch.
+namespace {
+struct NoallocLayout {
+  int64_t rows = 0;
+  size_t x = 0, h1 = 0, h2 = 0, out = 0;
+};
+
+NoallocLayout noalloc_layout(const gtp_mlp* m) {
+  const auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t per_row = (size_t)m->words * 4 + (size_t)m->H * 4 + 4;
+  NoallocLayout L;
+  int64_t r = (int64_t)((NOALLOC_SCRATCH_BYTES - 4 * 256) / per_row) / 128 * 128;
+  for (; r >= 128; r -= 128) {
+    L.x = 0;
+    L.h1 = al(L.x + (size_t)r * m->words * 4);
+    L.h2 = al(L.h1 + (size_t)r * m->H * 2);
+    L.out = al(L.h2 + (size_t)r * m->H * 2);
+    if (L.out + (size_t)r * 4 <= NOALLOC_SCRATCH_BYTES) break;
+  }
+  L.rows = r >= 128 ? r : 0;
+  return L;
+}
+}  // namespace
+
+GTP_API gtp_status gtp_mlp_predict_noalloc(gtp_mlp* m, const uint32_t* bits, int64_t n, float* out) {
+  if (!m || !bits || !out || n <= 0) return GTP_ERR_ARG;
+  const NoallocLayout L = noalloc_layout(m);
+  if (L.rows == 0)
+    return m->fail(GTP_ERR_LIMIT, "gtp_mlp_predict_noalloc: n_in " + std::to_string(m->K) + " and n_hidden " +
+                                      std::to_string(m->H) + " need more than the " +
+                                      std::to_string(NOALLOC_SCRATCH_BYTES >> 20) +
+                                      " MiB scratch for a 128-row chunk");
+  MCHK(m, hipSetDevice(m->device));
+  unsigned char* base = nullptr;
+  MCHK(m, noalloc_resolve(m->device, &base));
+  uint32_t* X = reinterpret_cast<uint32_t*>(base + L.x);
+  bf16* h1 = reinterpret_cast<bf16*>(base + L.h1);
+  bf16* h2 = reinterpret_cast<bf16*>(base + L.h2);
+  float* o = reinterpret_cast<float*>(base + L.out);
+  for (int64_t b = 0; b < n; b += L.rows) {
+    const int c = (int)std::min<int64_t>(L.rows, n - b);
+    MCHK(m, hipMemcpyAsync(X, bits + b * m->words, (size_t)c * m->words * 4, hipMemcpyHostToDevice, m->stream));
+    MCHK(m, forward(m, X, c, false, o, 0, h1, h2));
+    MCHK(m, hipMemcpyAsync(out + b, o, (size_t)c * 4, hipMemcpyDeviceToHost, m->stream));
     MCHK(m, hipStreamSynchronize(m->stream));
   }
   for (int64_t i = 0; i < n; ++i) out[i] = out[i] * m->y_std + m->y_mean;

@@ -332,6 +332,17 @@ gtp_status gtp_mlp_train(gtp_mlp* m, const uint32_t* bits, const float* y, int64
                          const gtp_train_config* tc, float* epoch_loss);
 /* Predictions in target units for n rows of host feature bits. */
 gtp_status gtp_mlp_predict(gtp_mlp* m, const uint32_t* bits, int64_t n, float* out);
+/* gtp_mlp_predict without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * rows, hidden activations and outputs of each chunk of rows are staged in a
+ * fixed 16 MiB device array that the library holds on every GPU it runs on
+ * until the process ends; rows are processed in chunks of as many as fit it
+ * (10880 for n_in 4096, n_hidden 256).  GTP_ERR_LIMIT when a 128-row chunk of
+ * the model does not fit.  Results equal gtp_mlp_predict's.  Calls on one GPU
+ * share that array: calls that overlap on one GPU (from several threads or
+ * models) race and must be serialised by the caller. */
+gtp_status gtp_mlp_predict_noalloc(gtp_mlp* m, const uint32_t* bits, int64_t n, float* out);
 /* Features are per-cycle toggle bits of (net, kind) pairs, identified as
  * kind * n_net + net: kind GTP_FEAT_CHANGE sets the bit of a cycle in which
  * the net changes value, kind GTP_FEAT_RISE of one in which it changes to 1. */
