@@ -58,6 +58,39 @@ def algorithmic_bytes(design, trace):
     return per_cycle * C, per_cycle
 
 
+def merge_steps(design, trace):
+    """Pin-event steps of one evaluation per merge kernel: every pin event of
+    a gate, except the clock pin's of the flip-flops whose clock edges the
+    workgroup takes (single clock pin 0 of a 3-pin cell); cells are assigned to
+    the kernels as the engine assigns single-output cells without `when`."""
+    lib = design.library
+    cnt = np.diff(trace.net_event_offset)
+    pn = np.asarray(design.pin_net)
+    pcnt = np.where(pn >= 0, cnt[np.maximum(pn, 0)], 0)
+    cells = np.asarray(design.gate_cell)
+    gpo = np.asarray(design.gate_pin_offset)
+    full = {}
+    for c, k, rel, r, f, w in lib.ipowers:
+        full.setdefault(c, True)
+        full[c] = full[c] and w == (1 << (1 << lib.cell_n_pin[c])) - 1
+    out = {}
+    for c in range(len(lib.cells)):
+        gs = np.nonzero(cells == c)[0]
+        if len(gs) == 0:
+            continue
+        n, cm = int(lib.cell_n_pin[c]), int(lib.cell_clock_mask[c])
+        pins = gpo[gs][:, None] + np.arange(n)[None, :]
+        pe = pcnt[pins]
+        if lib.cell_n_pin[c] - lib.cell_n_in[c] != 1 or not full.get(c, True):
+            key = "power_tile_kernel"
+        elif n == 3 and cm == 1:
+            key, pe = "power_cell_kernel<3, 0>", pe[:, 1:]
+        else:
+            key = f"power_cell_kernel<{n}, -1>"
+        out[key] = out.get(key, 0) + int(pe.sum())
+    return out
+
+
 def copy_calibration(mib, reps=5):
     """Plain device-to-device copies of a known size (torch's elementwise copy
     kernel): the reference reading for the FETCH_SIZE / WRITE_SIZE counters."""
@@ -269,6 +302,8 @@ def main():
     ap.add_argument("--vcd-cycles", type=int, default=2000, help="cycles of the VCD ingest leg (0: off)")
     ap.add_argument("--sliced-parts", type=int, default=4, help="event budget 1/parts for the sliced leg (<2: off)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
+    ap.add_argument("--steps-json", default=None, metavar="FILE",
+                    help="write the pin-event steps of one evaluation per merge kernel (for counter summaries)")
     args = ap.parse_args()
 
     if args.copy_calibration:
@@ -293,6 +328,9 @@ def main():
     log(f"generated in {gen_s:.1f} s: {design.n_gate} gates, {design.n_pin} pins, {n_event} events, "
         f"{n_cycle} cycles")
     algo_bytes, bytes_per_cycle = algorithmic_bytes(design, trace)
+    if args.steps_json:
+        with open(args.steps_json, "w") as f:
+            json.dump(merge_steps(design, trace), f, indent=1)
 
     eng = HipPower(device=0, tile_cycles=args.tile_cycles, gates_per_wg=args.gates_per_wg)
     t0 = time.perf_counter()

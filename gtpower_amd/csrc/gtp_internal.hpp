@@ -82,8 +82,12 @@ inline int64_t stride_blocks(int64_t n_block) {
   return n_block < cap ? n_block : cap;
 }
 
-// Per-cycle accumulator components (LDS bins and partials).
-enum { BIN_INT = 0, BIN_GINT = 1, BIN_SW = 2, BIN_GSW = 3, BIN_LD = 4, BIN_LW = 5, N_BIN = 6 };
+// Per-cycle accumulator components (LDS bins and partials).  Leakage: BIN_LD
+// sums the leakage steps dL (quanta) taken in the cycle and BIN_LX sums
+// dL * (t - cycle start) (quanta x ticks); the work item's flush turns them into
+// S_c (leakage of the gates' states at the cycle start) and sum dL * (cycle end
+// - t), so that the cycle's leakage integral is S_c * len + BIN_LX, exactly.
+enum { BIN_INT = 0, BIN_GINT = 1, BIN_SW = 2, BIN_GSW = 3, BIN_LD = 4, BIN_LX = 5, N_BIN = 6 };
 // Per-gate partial components (float-float).
 enum { PG_INT = 0, PG_GINT = 1, PG_SW = 2, PG_GSW = 3, PG_LEAK = 4, N_PG = 5 };
 

@@ -410,6 +410,26 @@ __device__ __forceinline__ bool next_work_item(const PowerArgs& a, unsigned int*
   return k < per && w < n_work;
 }
 
+// Leakage bins of one work item, by one thread: on entry ld[ci] = sum of the
+// leakage steps dL (quanta) taken in tile cycle ci and lx[ci] = sum of dL * (t -
+// cycle start); on exit ld[ci] = S_c, the leakage of the gates' states at the
+// cycle start (base + the steps of earlier cycles of the tile), and lx[ci] =
+// sum of dL * (cycle end - t), the in-cycle part of the cycle's leakage
+// integral (quanta x ticks; the cycle end clamped to t_max).  The cycle's
+// integral is then S_c * len + lx[ci], exactly, in integers.
+__device__ void leak_flush(long long* ld, long long* lx, int TC, long long base, int64_t tc0, int64_t T, int64_t t_max,
+                           int64_t n_cycle) {
+  long long run = base;
+  for (int ci = 0; ci < TC; ++ci) {
+    const int64_t c = tc0 + ci;
+    const int64_t len = c < n_cycle ? min((c + 1) * T, t_max) - c * T : 0;
+    const long long d = ld[ci];
+    lx[ci] = d * (long long)len - lx[ci];
+    ld[ci] = run;
+    run += d;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // The fused power kernel
 // ---------------------------------------------------------------------------
@@ -556,8 +576,8 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
     for (int k = 0; k < N_PG; ++k) acc_hi[k] = acc_lo[k] = 0.f;
     uint32_t tog = 0, glc = 0;
     int pc = -1;  // pending cycle (tile-relative), -1 none
-    float p_int = 0.f, p_gint = 0.f, p_sw = 0.f, p_gsw = 0.f, p_lw = 0.f;
-    long long p_ld = 0;
+    float p_int = 0.f, p_gint = 0.f, p_sw = 0.f, p_gsw = 0.f;
+    long long p_ld = 0, p_lx = 0;
     uint32_t prev_t = tstart;
 
     auto flush_pending = [&]() {
@@ -567,10 +587,10 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
         if (p_sw != 0.f) atomicAdd((unsigned long long*)&bins[BIN_SW * TC + pc], (unsigned long long)quant(p_sw, inv_qf));
         if (p_gsw != 0.f) atomicAdd((unsigned long long*)&bins[BIN_GSW * TC + pc], (unsigned long long)quant(p_gsw, inv_qf));
         if (p_ld != 0) atomicAdd((unsigned long long*)&bins[BIN_LD * TC + pc], (unsigned long long)p_ld);
-        if (p_lw != 0.f) atomicAdd((unsigned long long*)&bins[BIN_LW * TC + pc], (unsigned long long)quant(p_lw, a.inv_q_leak));
+        if (p_lx != 0) atomicAdd((unsigned long long*)&bins[BIN_LX * TC + pc], (unsigned long long)p_lx);
       }
-      p_int = p_gint = p_sw = p_gsw = p_lw = 0.f;
-      p_ld = 0;
+      p_int = p_gint = p_sw = p_gsw = 0.f;
+      p_ld = p_lx = 0;
     };
 
     for (;;) {
@@ -599,10 +619,9 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
       const float Lb = s_leak[loff + sb];
       ff_add_prod(acc_hi[PG_LEAK], acc_lo[PG_LEAK], Lb, (float)(t - prev_t));
       if (sa != sb) {
-        const float La = s_leak[loff + sa];
-        p_ld += s_leakq[loff + sa] - s_leakq[loff + sb];
-        const uint32_t cend = min((c + 1) * T, t_max);
-        p_lw += (La - Lb) * ((float)(cend - t) * invT);
+        const long long dq = s_leakq[loff + sa] - s_leakq[loff + sb];
+        p_ld += dq;
+        p_lx += dq * (long long)(t - c * T);
       }
       // internal and switching energy of every pin changing at t (:845-883)
       float ri = 0.f, rg = 0.f, rs = 0.f, gs = 0.f;
@@ -725,15 +744,10 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
   if ((threadIdx.x & 63) == 0) atomicAdd((unsigned long long*)&s_base, (unsigned long long)base_acc);
   __syncthreads();
 
-  // flush: S_c = base + sum of leakage steps in earlier cycles of the tile
-  if (threadIdx.x == 0) {
-    long long run = s_base;
-    for (int ci = 0; ci < TC; ++ci) {
-      const long long d = bins[BIN_LD * TC + ci];
-      bins[BIN_LD * TC + ci] = run;
-      run += d;
-    }
-  }
+  // flush: S_c = base + sum of leakage steps in earlier cycles of the tile,
+  // and the in-cycle part of the leakage integral
+  if (threadIdx.x == 0)
+    leak_flush(bins + BIN_LD * TC, bins + BIN_LX * TC, TC, s_base, tc0, (int64_t)T, wk.t_max, wk.n_cycle);
   __syncthreads();
   long long* out = a.cyc_part + ((int64_t)wt * a.n_range + range) * N_BIN * TC;
   for (int i = threadIdx.x; i < N_BIN * TC; i += blockDim.x) out[i] = bins[i];
@@ -1170,7 +1184,6 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
   constexpr int NS = 6 * NIN + 2;
   constexpr int NSTATE = 1 << NP;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ float s_leak[NSTATE];
   __shared__ long long s_leakq[NSTATE];
   __shared__ long long s_base;
   const int TC = a.tile_cycles;
@@ -1194,19 +1207,15 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
   __shared__ long long ck_l[2];               // clock net: first event, events
   __shared__ int ck_s[8];                     // clock net, first edge index, first-event flag, edges, overflow
   __shared__ unsigned long long ck_t[4];      // scan scratch
-  __shared__ int4 ck_lk[CLKB ? 4 : 1];        // per non-clock state: leakage quanta and W, clock low and high
+  __shared__ int2 ck_lk[CLKB ? 4 : 1];        // per non-clock state: leakage quanta, clock low and high
   const CellDev cd = a.cells[cell];
   const uint32_t clk = cd.clock_mask;
   const uint32_t arcm = cd.arc_mask;  // inputs that can be the related pin of an output event
-  for (int i = threadIdx.x; i < NSTATE; i += blockDim.x) {
-    s_leak[i] = a.leak[cd.leak_off + i];
-    s_leakq[i] = a.leak_q[cd.leak_off + i];
-  }
+  for (int i = threadIdx.x; i < NSTATE; i += blockDim.x) s_leakq[i] = a.leak_q[cd.leak_off + i];
   if constexpr (CLKB) {
     for (int j = threadIdx.x; j < 4; j += blockDim.x) {
       const uint32_t s0 = ck_unpack<KC>((uint32_t)j, 0u), s1 = ck_unpack<KC>((uint32_t)j, 1u);
-      ck_lk[j] = make_int4((int32_t)a.leak_q[cd.leak_off + s0], (int32_t)a.leak_q[cd.leak_off + s1],
-                           __float_as_int(a.leak[cd.leak_off + s0]), __float_as_int(a.leak[cd.leak_off + s1]));
+      ck_lk[j] = make_int2((int32_t)a.leak_q[cd.leak_off + s0], (int32_t)a.leak_q[cd.leak_off + s1]);
     }
   }
   const int n_copy = a.bin_copies;
@@ -1483,13 +1492,11 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
     for (int k = 0; k < N_PG; ++k) acc[k] = 0;
     uint32_t tog = 0, glc = 0;
     uint32_t prev_t = tstart;
-    long long lq_cur = s_leakq[sb];  // leakage of the current state, in quanta and in W
-    float lf_cur = s_leak[sb];
+    long long lq_cur = s_leakq[sb];  // leakage of the current state, in quanta
     // CLKB: the leakage of the current non-clock state with the clock low and
     // high, the clock-high ticks from the tile start to prev_t, and whether
     // the clock events come from the item's edges (bulk)
     int32_t lq_lo = 0, lq_hi = 0;
-    float lf_lo = 0.f, lf_hi = 0.f;
     uint32_t h_prev = 0;
     bool bulk = false;
     if constexpr (CLKB) {
@@ -1502,8 +1509,6 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
       const uint32_t s0 = sb & ~(1u << KC), s1 = sb | (1u << KC);
       lq_lo = (int32_t)s_leakq[s0];
       lq_hi = (int32_t)s_leakq[s1];
-      lf_lo = s_leak[s0];
-      lf_hi = s_leak[s1];
     }
 
     // One pin event per step: the earliest pending event, the output pin's
@@ -1550,14 +1555,12 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
         acc[PG_LEAK] += (long long)(int)lq_cur * (long long)(t - prev_t);
         const uint32_t sa = (sb & ~(1u << ps)) | ((nv == 1 ? 1u : 0u) << ps);
         const long long lq_new = s_leakq[sa];
-        const float lf_new = s_leak[sa];
         // (zero when the state does not change)
-        atomicAdd((unsigned long long*)&cb[BIN_LD * TCP], (unsigned long long)(lq_new - lq_cur));
-        atomicAdd((unsigned long long*)&cb[BIN_LW * TCP],
-                  (unsigned long long)qterm((lf_new - lf_cur) * ((float)(min(cend, t_max) - t) * invT), inv_ql));
+        const long long dq = lq_new - lq_cur;
+        atomicAdd((unsigned long long*)&cb[BIN_LD * TCP], (unsigned long long)dq);
+        atomicAdd((unsigned long long*)&cb[BIN_LX * TCP], (unsigned long long)(dq * (long long)(t - cstart)));
         sb = sa;
         lq_cur = lq_new;
-        lf_cur = lf_new;
       } else {
         // the clock: its own events (not bulk), or the item's edges consumed
         // before this event: those before t, and those at t when the clock
@@ -1592,23 +1595,19 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
         const uint32_t sbe = (sb & ~(1u << KC)) | (bit << KC);
         const uint32_t sa = (sbe & ~(1u << ps)) | ((nv == 1 ? 1u : 0u) << ps);
         const uint32_t sa0 = sa & ~(1u << KC);
-        const int4 lk = ck_lk[ck_pack<KC>(sa)];
+        const int2 lk = ck_lk[ck_pack<KC>(sa)];
         const int32_t nlo = lk.x, nhi = lk.y;
-        const float flo = __int_as_float(lk.z), fhi = __int_as_float(lk.w);
         const bool abit = (sa >> KC) & 1u;
         const long long lq_c = bit ? lq_hi : lq_lo, lq_new = abit ? nhi : nlo;
-        const float lf_c = bit ? lf_hi : lf_lo, lf_new = abit ? fhi : flo;
-        atomicAdd((unsigned long long*)&cb[BIN_LD * TCP], (unsigned long long)(lq_new - lq_c));
-        atomicAdd((unsigned long long*)&cb[BIN_LW * TCP],
-                  (unsigned long long)qterm((lf_new - lf_c) * ((float)(min(cend, t_max) - t) * invT), inv_ql));
+        const long long dq = lq_new - lq_c;
+        atomicAdd((unsigned long long*)&cb[BIN_LD * TCP], (unsigned long long)dq);
+        atomicAdd((unsigned long long*)&cb[BIN_LX * TCP], (unsigned long long)(dq * (long long)(t - cstart)));
         // a new non-clock state counts from the next unconsumed edge on
         if (bulk && sa0 != (sbe & ~(1u << KC)))
           atomicAdd(&ck_d[el + 1], (1ull << (16 * ck_pack<KC>(sa))) - (1ull << (16 * ck_pack<KC>(sbe))));
         sb = sa;
         lq_lo = nlo;
         lq_hi = nhi;
-        lf_lo = flo;
-        lf_hi = fhi;
       }
 
       const int tx2 = toggle_x2(pv, nv);
@@ -1774,7 +1773,6 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
         const int pv = ev_val((uint32_t)ck_wh[sl - 1]), nv = ev_val(w);
         const uint32_t t = ev_time(w);
         const uint32_t c = cycle_mul(t, a.cyc_m, a.cyc_sh);
-        const uint32_t cend = c * T + T;
         const int cr = min((int)((int64_t)c - tc0), TC);
         const int tx2 = toggle_x2(pv, nv), r2 = rise_x2(pv, nv);
         if (tx2 > 0)
@@ -1783,17 +1781,17 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
         const uint32_t bo = pv == 1 ? 1u : 0u, bn = nv == 1 ? 1u : 0u;
         if (bo != bn) {
           const unsigned long long cnt = ck_d[sl];
-          const float fr = (float)(min(cend, t_max) - t) * invT;
-          long long ld = 0, lwq = 0;
+          long long ld = 0;
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             const long long n = (long long)((cnt >> (16 * j)) & 0xFFFFull);
             const uint32_t so = ck_unpack<KC>((uint32_t)j, bo), sn = ck_unpack<KC>((uint32_t)j, bn);
             ld += n * (s_leakq[sn] - s_leakq[so]);
-            lwq += n * qterm((s_leak[sn] - s_leak[so]) * fr, inv_ql);
           }
+          // the gates' steps dL each, and dL * (t - cycle start) in total
           atomicAdd((unsigned long long*)&bins[BIN_LD * TCP + cr], (unsigned long long)ld);
-          atomicAdd((unsigned long long*)&bins[BIN_LW * TCP + cr], (unsigned long long)lwq);
+          atomicAdd((unsigned long long*)&bins[BIN_LX * TCP + cr],
+                    (unsigned long long)(ld * (long long)(t - c * T)));
         }
       }
       __syncthreads();
@@ -1806,14 +1804,8 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
     bins[i] = v;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    long long run = s_base;
-    for (int ci = 0; ci < TC; ++ci) {
-      const long long d = bins[BIN_LD * TCP + ci];
-      bins[BIN_LD * TCP + ci] = run;
-      run += d;
-    }
-  }
+  if (threadIdx.x == 0)
+    leak_flush(bins + BIN_LD * TCP, bins + BIN_LX * TCP, TC, s_base, tc0, (int64_t)T, wk.t_max, wk.n_cycle);
   __syncthreads();
   long long* out = a.cyc_part + ((int64_t)wt * a.n_range + range) * N_BIN * TC;
   for (int i = threadIdx.x; i < N_BIN * TC; i += blockDim.x) {
@@ -1845,8 +1837,8 @@ __global__ void cycle_reduce_kernel(const long long* __restrict__ part, int32_t 
     for (int k = 0; k < N_BIN; ++k) s[k] += p[(int64_t)k * tile_cycles];
   }
   const int64_t cend = min((c + 1) * T, t_max);
-  const double frac = (double)(cend - c * T) / (double)T;
-  out[0 * n_cycle + c] = ((double)s[BIN_LD] * frac + (double)s[BIN_LW]) * q_leak;
+  const int64_t len = cend - c * T;
+  out[0 * n_cycle + c] = ((double)s[BIN_LD] * (double)len + (double)s[BIN_LX]) * (q_leak / (double)T);
   out[1 * n_cycle + c] = (double)(s[BIN_INT] + s[BIN_GINT]) * q_energy * inv_period;
   out[2 * n_cycle + c] = (double)s[BIN_GINT] * q_energy * inv_period;
   out[3 * n_cycle + c] = (double)(s[BIN_SW] + s[BIN_GSW]) * q_energy * inv_period;
@@ -1897,9 +1889,8 @@ __global__ __launch_bounds__(RL_CYCLES * RL_GROUPS) void cycle_reduce_list_kerne
 #pragma unroll
     for (int g = 0; g < RL_GROUPS; ++g) sum[k] += s_part[g][k][cl];
   }
-  const int64_t cend = min((c + 1) * T, t_max);
-  const double frac = (double)(cend - c * T) / (double)T;
-  out[0 * n_out + j] = ((double)sum[BIN_LD] * frac + (double)sum[BIN_LW]) * q_leak;
+  const int64_t len = min((c + 1) * T, t_max) - c * T;
+  out[0 * n_out + j] = ((double)sum[BIN_LD] * (double)len + (double)sum[BIN_LX]) * (q_leak / (double)T);
   out[1 * n_out + j] = (double)(sum[BIN_INT] + sum[BIN_GINT]) * q_energy * inv_period;
   out[2 * n_out + j] = (double)sum[BIN_GINT] * q_energy * inv_period;
   out[3 * n_out + j] = (double)(sum[BIN_SW] + sum[BIN_GSW]) * q_energy * inv_period;
@@ -1929,11 +1920,10 @@ __global__ void cycle_reduce_batch_kernel(const long long* __restrict__ part, in
 #pragma unroll
     for (int k = 0; k < N_BIN; ++k) s[k] += p[(int64_t)k * tile_cycles];
   }
-  const int64_t cend = min((c + 1) * T, td.t_max);
-  const double frac = (double)(cend - c * T) / (double)T;
+  const int64_t len = min((c + 1) * T, td.t_max) - c * T;
   double* o = out + out_base[tr];
   const int64_t C = td.n_cycle;
-  o[0 * C + c] = ((double)s[BIN_LD] * frac + (double)s[BIN_LW]) * q_leak;
+  o[0 * C + c] = ((double)s[BIN_LD] * (double)len + (double)s[BIN_LX]) * (q_leak / (double)T);
   o[1 * C + c] = (double)(s[BIN_INT] + s[BIN_GINT]) * q_energy * inv_period;
   o[2 * C + c] = (double)s[BIN_GINT] * q_energy * inv_period;
   o[3 * C + c] = (double)(s[BIN_SW] + s[BIN_GSW]) * q_energy * inv_period;

@@ -11,6 +11,13 @@ full glitch annotation (glitch_annotate_kernel) to the end of the last power
 kernel (power_cell_kernel / power_tile_kernel) before that run's
 cycle_reduce_kernel -- which is the interval the engine times with HIP events.
 
+`counters` reads `rocprofv3 --pmc` databases of one program (one per counter
+pass) and reports, per kernel and per evaluation, the instruction mix and the
+derived VALUBusy (100 SQ_ACTIVE_INST_VALU / CUs / GRBM_GUI_ACTIVE) and
+VALUUtilization (100 SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)); with
+--steps (a JSON of pin-event steps per evaluation by kernel name, written by
+bench.py --steps-json) also VALU instructions per 64-lane step.
+
 `pmc` reads two `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` databases of
 the same program, and two of plain device copies of a known size, and reports
 HBM traffic per evaluation window: the counters (KiB per dispatch) summed over
@@ -170,6 +177,61 @@ def pmc_summary(fetch_db, write_db, copy_fetch_db, copy_write_db, copy_mib):
     return lines, res
 
 
+def _counters_by_kernel(db):
+    """{kernel: {counter: summed value}} and {kernel: dispatches}; counters
+    with a GRBM_ prefix take the maximum over their instances per dispatch
+    (elapsed cycles), the others the sum."""
+    c = sqlite3.connect(db)
+    rows = c.execute("select dispatch_id, kernel_name, counter_name, sum(value), max(value) "
+                     "from counters_collection group by dispatch_id, counter_name").fetchall()
+    by, disp = {}, {}
+    for did, name, ctr, vsum, vmax in rows:
+        k = _short(name)
+        by.setdefault(k, {}).setdefault(ctr, 0.0)
+        by[k][ctr] += vmax if ctr.startswith("GRBM_") else vsum
+        disp.setdefault(k, set()).add(did)
+    return by, {k: len(v) for k, v in disp.items()}
+
+
+def counters_summary(dbs, steps=None, n_cu=256, n_xcd=8):
+    agg, ndisp = {}, {}
+    for db in dbs:
+        by, nd = _counters_by_kernel(db)
+        for k, d in by.items():
+            for ctr, v in d.items():
+                agg.setdefault(k, {})[ctr] = v   # one pass per counter set: later passes add counters
+            ndisp[k] = max(ndisp.get(k, 0), nd.get(k, 0))
+    n_eval = max(1, ndisp.get("gtp::glitch_annotate_kernel", 1))
+    lines = [f"evaluations: {n_eval} (glitch_annotate_kernel dispatches); values per evaluation", "",
+             f"{'kernel':34s} {'waves':>9s} {'VALU/wave':>9s} {'VALUBusy%':>9s} {'VALUUtil%':>9s} "
+             f"{'LDS/VALU':>8s} {'VMEM/VALU':>9s} {'SALU/VALU':>9s} {'wait/wave':>9s} {'L2hit%':>7s} "
+             f"{'steps':>10s} {'VALU/step':>9s}"]
+    res = {}
+    for k, d in sorted(agg.items(), key=lambda kv: -kv[1].get("SQ_INSTS_VALU", 0)):
+        if "SQ_INSTS_VALU" not in d:
+            continue
+        g = lambda ctr: d.get(ctr, float("nan")) / n_eval
+        waves, valu = g("SQ_WAVES"), g("SQ_INSTS_VALU")
+        # GRBM_GUI_ACTIVE of a dispatch is summed over the XCDs: elapsed cycles = / n_xcd
+        busy = 100.0 * g("SQ_ACTIVE_INST_VALU") / n_cu / (g("GRBM_GUI_ACTIVE") / n_xcd)
+        util = 100.0 * g("SQ_THREAD_CYCLES_VALU") / (64.0 * g("SQ_ACTIVE_INST_VALU"))
+        hit = 100.0 * g("TCC_HIT_sum") / (g("TCC_HIT_sum") + g("TCC_MISS_sum"))
+        st = (steps or {}).get(k.replace("gtp::", ""))
+        per_step = valu / (st / 64.0) if st else float("nan")
+        res[k] = dict(waves=waves, valu=valu, valu_busy=busy, valu_util=util, lds=g("SQ_INSTS_LDS"),
+                      vmem_rd=g("SQ_INSTS_VMEM_RD"), salu=g("SQ_INSTS_SALU"), wait_inst_any=g("SQ_WAIT_INST_ANY"),
+                      l2_hit=hit, steps=st, valu_per_step=per_step)
+        lines.append(f"{k.replace('gtp::', '')[:34]:34s} {waves:9.3g} {valu / waves:9.1f} {busy:9.1f} {util:9.1f} "
+                     f"{g('SQ_INSTS_LDS') / valu:8.3f} {g('SQ_INSTS_VMEM_RD') / valu:9.3f} "
+                     f"{g('SQ_INSTS_SALU') / valu:9.3f} {4 * g('SQ_WAIT_INST_ANY') / waves:9.3g} {hit:7.1f} "
+                     f"{(st or float('nan')):10.4g} {per_step:9.1f}")
+    lines += ["", "VALUBusy: 100 SQ_ACTIVE_INST_VALU (quad-cycles) / CUs / elapsed cycles (GRBM_GUI_ACTIVE / XCDs; "
+              "kernels profiled one at a time); "
+              "VALUUtil: active lanes per VALU cycle; wait/wave: SQ_WAIT_INST_ANY cycles per wave; "
+              "VALU/step: wave VALU instructions per 64 pin-event steps"]
+    return lines, res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="mode", required=True)
@@ -181,12 +243,22 @@ def main(argv=None):
     p.add_argument("--copy", nargs=2, required=True, metavar=("FETCH_DB", "WRITE_DB"))
     p.add_argument("--copy-mib", type=int, required=True)
     p.add_argument("--json")
-    for q in (t, p):
+    k = sub.add_parser("counters")
+    k.add_argument("dbs", nargs="+")
+    k.add_argument("--steps", help="JSON: pin-event steps per evaluation by kernel name")
+    k.add_argument("--json")
+    for q in (t, p, k):
         q.add_argument("--out")
         q.add_argument("--cmd", default="")
     a = ap.parse_args(argv)
     if a.mode == "trace":
         lines, _ = trace_summary(a.db)
+    elif a.mode == "counters":
+        steps = json.load(open(a.steps)) if a.steps else None
+        lines, res = counters_summary(a.dbs, steps)
+        if a.json:
+            with open(a.json, "w") as f:
+                json.dump(res, f, indent=1)
     else:
         lines, res = pmc_summary(a.fetch_db, a.write_db, a.copy[0], a.copy[1], a.copy_mib)
         if a.json:
