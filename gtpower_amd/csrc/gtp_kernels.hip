@@ -39,6 +39,21 @@ __device__ __forceinline__ int rise_x2(int p, int c) {
   if (c == 1) return p == 0 ? 2 : (p == 2 ? 1 : 0);
   return (c == 2 && p == 0) ? 1 : 0;
 }
+// The same two functions as 2-bit entries of a constant, entry 4 p + c
+// (checked against them for all 16 pairs by static_asserts below).
+constexpr uint32_t TOGGLE_X2_TABLE = 0x1a459298u;
+constexpr uint32_t RISE_X2_TABLE = 0x00040018u;
+__host__ __device__ constexpr int table_x2(uint32_t tab, int p, int c) { return (int)((tab >> (2 * (4 * p + c))) & 3u); }
+constexpr bool tables_match() {
+  for (int p = 0; p < 4; ++p)
+    for (int c = 0; c < 4; ++c) {
+      const int t = p == c ? 0 : ((p == 2 || c == 2) ? 1 : 2);
+      const int r = c == 1 ? (p == 0 ? 2 : (p == 2 ? 1 : 0)) : ((c == 2 && p == 0) ? 1 : 0);
+      if (table_x2(TOGGLE_X2_TABLE, p, c) != t || table_x2(RISE_X2_TABLE, p, c) != r) return false;
+    }
+  return true;
+}
+static_assert(tables_match(), "toggle / rise tables");
 
 // floor(t / T) for t < 2^29: a float estimate, then integer correction.
 __device__ __forceinline__ uint32_t cycle_of(uint32_t t, uint32_t T, float invT) {
@@ -84,10 +99,11 @@ __device__ __forceinline__ long long qterm(float x, float inv_q) {
 // conversions: xs - rint(xs) is exact in FP32 and |.| <= 1/2, so its scaled
 // rounding is the low part.
 __device__ __forceinline__ long long qenergy(float x, float inv_q) {
-  const float xs = x * inv_q;
-  const float hi = rintf(xs);
-  const float lo = rintf((xs - hi) * (float)(1 << ENERGY_FINE_BITS));
-  return ((long long)(int)hi << ENERGY_FINE_BITS) + (long long)(int)lo;
+  // x * inv_q * 2^16 is exact in FP64 (|.| <= 2^36); adding 1.5 * 2^52 rounds
+  // it to an integer held in the low mantissa bits (round to nearest even)
+  const double y = (double)x * (double)(inv_q * (float)(1 << ENERGY_FINE_BITS));
+  const double m = y + 6755399441055744.0;  // 1.5 * 2^52
+  return __double_as_longlong(m) - 0x4338000000000000ll;
 }
 
 // ---------------------------------------------------------------------------
@@ -1481,6 +1497,13 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
       cw[p] = rem[p] >= 1 ? c0 : EV_INF;
       nw[p] = rem[p] >= 2 ? nx : EV_INF;
     }
+    // an input whose net has no events is never the related input: no arcs
+#pragma unroll
+    for (int q = 0; q < NIN; ++q)
+      if (!((hasm >> q) & 1u)) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) D[q * 4 + k] = ARC_NONE;
+      }
 #pragma unroll
     for (int k = 0; k < NS; ++k) asm volatile("" : "+v"(E[k]));
 #pragma unroll
@@ -1610,8 +1633,8 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
         lq_hi = nhi;
       }
 
-      const int tx2 = toggle_x2(pv, nv);
-      const int r2 = rise_x2(pv, nv);
+      const int tx2 = table_x2(TOGGLE_X2_TABLE, pv, nv);
+      const int r2 = table_x2(RISE_X2_TABLE, pv, nv);
       const int rf = r2 > 0 ? 0 : 1;
       tog += (uint32_t)tx2;
       // glitch scaling (TimeBasedPower.cc:303-339): the width to the flagged
@@ -1645,7 +1668,9 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
         const float e23 = lo_rf ? E[2 * NIN + q * 4 + 3] : E[2 * NIN + q * 4 + 2];
         const float eq = hi_rf ? e23 : e01;
         const uint32_t ltq = ev_time(lw[q]);
-        const bool ok = ((hasm >> q) & 1u) && d != ARC_NONE && ltq >= cstart && ltq < cend;
+        // (an input without events has no arcs here; before its net's first
+        // event lw carries that event's time, a candidate only in this cycle)
+        const bool ok = d != ARC_NONE && ltq >= cstart && ltq < cend;
         // within one cycle t - lt < T < 2^15, so the grid difference fits 32 bits
         int32_t diff = (int32_t)((t - ltq) << ARC_FRAC_BITS) - d;
         diff = diff < 0 ? -diff : diff;
