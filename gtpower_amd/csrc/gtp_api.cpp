@@ -99,6 +99,9 @@ struct gtp_ctx {
   std::vector<double> arc_delay;
   double csw_max = 0;
   float energy_max = 0;
+  // the glitch width (s) beyond which every pin's glitch ratio is 1: the
+  // largest finite (s_r + s_f) / 2 (power/PowerUtils.cc:285-294)
+  double glitch_sat_s = 0;
   DArr<int32_t> d_gate_cell, d_gate_class;
   DArr<int64_t> d_gate_pin_off, d_gate_arc_off, d_gate_energy_off;
   DArr<int32_t> d_pin_net;
@@ -446,6 +449,7 @@ GTP_API gtp_status gtp_set_design(gtp_ctx* x, const gtp_library* L, const gtp_de
   const int64_t A = d->gate_arc_offset[G];
   if (A > 0 && !d->arc_delay) return x->fail(GTP_ERR_ARG, "gtp_design: arc_delay missing");
   std::vector<float> gk(P), csw(P, 0.f), slew(2 * P), cap(P);
+  double glitch_sat = 0;
   std::vector<int32_t> gclass(G);
   double csw_max = 0;
   for (int64_t g = 0; g < G; ++g) {
@@ -469,6 +473,7 @@ GTP_API gtp_status gtp_set_design(gtp_ctx* x, const gtp_library* L, const gtp_de
     // glitch scaling ratio min(1, (2 w / (s_r + s_f))^2) (power/PowerUtils.cc:285-294);
     // a zero slew sum gives ratio 1, an infinite one gives 0 (no glitch).
     gk[q] = sum == 0 ? INFINITY : (float)(2.0 / sum);
+    if (std::isfinite(sum)) glitch_sat = std::max(glitch_sat, 0.5 * sum);
     slew[2 * q] = (float)sr;
     slew[2 * q + 1] = (float)sf;
     cap[q] = (float)d->pin_load_cap[q];
@@ -482,6 +487,7 @@ GTP_API gtp_status gtp_set_design(gtp_ctx* x, const gtp_library* L, const gtp_de
   x->pin_net.assign(d->pin_net, d->pin_net + P);
   x->arc_delay.assign(d->arc_delay, d->arc_delay + A);
   x->csw_max = csw_max;
+  x->glitch_sat_s = glitch_sat;
   x->arc_q_scale = 0;
   // inputs with an arc to the output in some gate of each single-output cell
   // (the related-input search skips the others for the whole cell type)
@@ -554,13 +560,14 @@ namespace {
 
 // Checks one trace against the design; *T_out = ticks per cycle
 // (power/PowerUtils.cc:27-32).
-gtp_status validate_trace(gtp_ctx* x, const gtp_trace* tr, int64_t* T_out) {
+gtp_status validate_trace(gtp_ctx* x, const gtp_trace* tr, int64_t* T_out, bool long_ok = false) {
   if (!tr || !tr->net_event_offset || tr->n_net != x->N)
     return x->fail(GTP_ERR_ARG, "gtp_trace: n_net must equal the design's n_net");
   if (!(tr->time_scale > 0) || !(tr->clk_period > 0) || tr->t_max <= 0)
     return x->fail(GTP_ERR_ARG, "gtp_trace: time_scale, clk_period and t_max must be positive");
-  if (tr->t_max >= (int64_t)EV_TIME_INF)
-    return x->fail(GTP_ERR_UNSUPPORTED, "gtp_trace: t_max must be below 2^29 - 1 ticks");
+  if (!long_ok && tr->t_max >= (int64_t)EV_TIME_INF)
+    return x->fail(GTP_ERR_UNSUPPORTED,
+                   "gtp_trace: t_max must be below 2^29 - 1 ticks here (gtp_run_sliced takes longer traces)");
   const int64_t N = x->N;
   const int64_t* off = tr->net_event_offset;
   if (off[0] != 0) return x->fail(GTP_ERR_ARG, "gtp_trace: net_event_offset[0] != 0");
@@ -1292,9 +1299,11 @@ struct RunItem {
   const int64_t* off;        // [N + 1], off[0] == 0
   const int64_t* time;
   const uint8_t* value;
-  int64_t t_max, n_cycle;
+  int64_t t_max, n_cycle;    // relative to the load's origin (tile_base whole tiles into the trace)
   int32_t n_tile, tile_begin, tile_end, slot;
   const uint32_t* words = nullptr;  // the events already packed as event words (time / value unused)
+  int32_t tile_base = 0;     // the origin: tiles of the trace before it
+  int64_t out_cycles = 0;    // cycles of the trace's waveform (0: n_cycle)
 };
 
 // Schedule, fixed-point scales and scratch of a multi-trace run.
@@ -1457,7 +1466,9 @@ gtp_status run_load(gtp_ctx* x, MultiPlan& mp, const std::vector<RunItem>& items
     td[k].t_max = items[k].t_max;
     td[k].n_cycle = items[k].n_cycle;
     td[k].n_tile = items[k].n_tile;
-    td[k].pad_ = 0;
+    td[k].tile_base = items[k].tile_base;
+    td[k].cycle_base = (int64_t)items[k].tile_base * TC;
+    td[k].out_cycles = items[k].out_cycles > 0 ? items[k].out_cycles : items[k].n_cycle;
     tidx_total += N * ((int64_t)items[k].n_tile + 1);
   }
   DArr<int32_t> d_tidx, d_wt_tile, d_wt_trace, d_wt_begin, d_rslot;
@@ -1714,7 +1725,7 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
   const int64_t N = x->N, G = x->G;
   int64_t T = 0;
   {
-    const gtp_status st = validate_trace(x, tr, &T);
+    const gtp_status st = validate_trace(x, tr, &T, true);
     if (st != GTP_OK) return st;
   }
   const int64_t* off = tr->net_event_offset;
@@ -1745,6 +1756,28 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
                                         : mp.ev_budget - fixed_words;
   if (budget <= 0)
     return x->fail(GTP_ERR_OOM, "gtp_run_sliced: the tile index alone exceeds the device event budget");
+  // Each interval [k0, k1) is loaded with times relative to an origin of kb
+  // whole tiles, K cycles and one more before the interval; a trace may then be
+  // of any length, an interval (with those cycles) spanning below 2^29 - 1
+  // ticks.  K cycles exceed the widest glitch pulse whose scaling ratio is
+  // below 1, so the events of the slice before the interval (its last cycle
+  // group and the event before that group) can move to K cycles before it, and
+  // the two after it to K cycles after it, without changing any result: their
+  // cycle groups, their gaps within a cycle and the state they leave are kept,
+  // and every gap to the interval's events stays wider than K - 1 cycles
+  // (ratio 1, and no arrival in the same cycle).
+  const int64_t K = (int64_t)std::ceil(x->glitch_sat_s / tr->time_scale / (double)T) + 2;
+  auto origin_tile = [&](int32_t k0) {
+    return (int32_t)std::max<int64_t>(0, ((int64_t)k0 * TC - K - 1) / TC);
+  };
+  // relative span of an interval's load (ticks)
+  auto span = [&](int32_t k0, int32_t k1) {
+    const int64_t base = (int64_t)origin_tile(k0) * TC * T;
+    return (k1 >= n_tile ? tr->t_max : ((int64_t)k1 * TC + K + 2) * T) - base;
+  };
+  if (span(n_tile - 1, n_tile) >= (int64_t)EV_TIME_INF || span(0, 1) >= (int64_t)EV_TIME_INF)
+    return x->fail(GTP_ERR_UNSUPPORTED, "gtp_run_sliced: one tile of " + std::to_string(TC) + " cycles of " +
+                                            std::to_string(T) + " ticks spans 2^29 - 1 ticks or more");
   // The slice of net n for tiles [k0, k1): its events from the one before the
   // cycle group of the last event before the interval (so that group's
   // positions and the state before the interval come out as in a whole-trace
@@ -1782,6 +1815,9 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
     const int32_t step0 = (n_tile + guess - 1) / guess;
     for (int32_t k0 = 0; k0 < n_tile;) {
       int32_t step = std::max(1, std::min(step0, n_tile - k0));
+      while (step > 1 && span(k0, k0 + step) >= (int64_t)EV_TIME_INF) step = (step + 1) / 2;
+      if (span(k0, k0 + step) >= (int64_t)EV_TIME_INF)
+        return x->fail(GTP_ERR_UNSUPPORTED, "gtp_run_sliced: one tile spans 2^29 - 1 ticks or more");
       while (true) {
         const int64_t need = slice_bounds(k0, k0 + step, sb, se);
         if (need <= budget) break;
@@ -1820,9 +1856,41 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
     P.so[0] = 0;
     for (int64_t n = 0; n < N; ++n) P.so[n + 1] = P.so[n] + (P.se[n] - P.sb[n]);
     P.words.resize(std::max<int64_t>(P.so[N], 1));
+    const int64_t base = (int64_t)origin_tile(iv.first) * TC * T;
+    const int64_t c0 = (int64_t)iv.first * TC, t0 = c0 * T;               // interval start
+    const bool to_end = iv.second >= n_tile;
+    const int64_t c1 = (int64_t)iv.second * TC, t1 = c1 * T;              // interval end (not the last)
     parallel_for(N, x->threads, [&](int64_t n) {
       uint32_t* d = P.words.data() + P.so[n];
-      for (int64_t i = P.sb[n]; i < P.se[n]; ++i) *d++ = ((uint32_t)tm[i] << EV_SHIFT) | (uint32_t)(val[i] & 3u);
+      const int64_t a = P.sb[n], b = P.se[n];
+      const int64_t i0 = std::lower_bound(tm + a, tm + b, t0) - tm;
+      const int64_t i1 = to_end ? b : std::lower_bound(tm + i0, tm + b, t1) - tm;
+      // before the interval: the event before the group (a) at or after the
+      // start of cycle c0 - K - 1, the group (a + 1 .. i0 - 1, one cycle)
+      // moved to cycle c0 - K when it lies earlier
+      int64_t gshift = 0;
+      if (i0 > a) {
+        const int64_t gc = tm[i0 - 1] / T;
+        if (gc < c0 - K) gshift = (gc - (c0 - K)) * T;
+      }
+      const int64_t t_before = (c0 - K - 1) * T;
+      // after it: the first event (i1) moved to cycle c1 + K when later, the
+      // second (i1 + 1) kept in its cycle or put at the start of the next one
+      int64_t ashift = 0, ac_new = 0;
+      if (!to_end && i1 < b) {
+        const int64_t ac = tm[i1] / T;
+        if (ac > c1 + K) ashift = (ac - (c1 + K)) * T;
+        ac_new = ac - ashift / T;
+      }
+      for (int64_t i = a; i < b; ++i) {
+        int64_t t = tm[i];
+        if (i < i0) {
+          t = i > a ? t - gshift : std::max(t, t_before);
+        } else if (i >= i1) {
+          t = (i == i1 || tm[i] / T == tm[i1] / T) ? t - ashift : std::min(t, (ac_new + 1) * T);
+        }
+        *d++ = ((uint32_t)(t - base) << EV_SHIFT) | (uint32_t)(val[i] & 3u);
+      }
     });
   };
   prepare(pk[0], ivals[0]);
@@ -1831,8 +1899,18 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
     std::thread next;
     if (i + 1 < ivals.size()) next = std::thread([&, i] { prepare(pk[(i + 1) & 1], ivals[i + 1]); });
     JoinGuard join{next};
-    RunItem item{P.so.data(), nullptr, nullptr, tr->t_max, C, n_tile, ivals[i].first, ivals[i].second, 0};
+    // the load as a trace from its origin: kb tiles in, to the trace's end
+    // (the last interval) or just past the two events after the interval
+    const int32_t kb = origin_tile(ivals[i].first);
+    const int64_t base = (int64_t)kb * TC * T;
+    const int64_t tm_rel = ivals[i].second >= n_tile ? tr->t_max - base
+                                                     : ((int64_t)ivals[i].second * TC + K + 2) * T - base;
+    const int64_t c_rel = ivals[i].second >= n_tile ? C - (int64_t)kb * TC : (tm_rel + T - 1) / T;
+    RunItem item{P.so.data(), nullptr, nullptr, tm_rel, c_rel, n_tile - kb, ivals[i].first - kb,
+                 ivals[i].second - kb, 0};
     item.words = P.words.data();
+    item.tile_base = kb;
+    item.out_cycles = C;
     std::vector<RunItem> items{item};
     float ms = 0.f;
     const gtp_status st = run_load(x, mp, items, d_acc, d_cnt, d_cyc, slot_base, &ms);

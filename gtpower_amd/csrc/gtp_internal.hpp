@@ -123,12 +123,17 @@ struct TableDev {             // offsets into the staged float table data
   int32_t axis1, axis2, values;
 };
 
-// One trace of a batched run.
+// One trace of a multi-trace run (batched traces, or one interval of a sliced
+// run).  Times in its event words, its tiles and cycles are relative to its
+// load's origin, tile_base whole tiles into the trace (0 for a whole trace):
+// tile k here is tile k + tile_base of the trace, and its per-cycle values go
+// to cycle c + cycle_base of a waveform of out_cycles cycles.
 struct TraceDesc {
   int64_t net_base;        // first (virtual) net of the trace in the event store
   int64_t tile_idx_base;   // offset of the trace's [N][n_tile + 1] tile index
-  int64_t t_max, n_cycle;
-  int32_t n_tile, pad_;
+  int64_t t_max, n_cycle;  // relative to the origin
+  int64_t cycle_base, out_cycles;
+  int32_t n_tile, tile_base;
 };
 
 // Arguments of the power kernel.

@@ -366,7 +366,8 @@ __global__ void energy_kernel(EnergyArgs a) {
 // several traces (batched runs).
 // ---------------------------------------------------------------------------
 struct WorkTile {
-  int tile;                // tile within its trace
+  int tile;                // tile within its trace (its load: relative to the load's origin)
+  bool trace_start;        // the tile starts the trace (time 0: a time-0 event is the initial state)
   int tile_col;            // column of the tile index
   int32_t n_tile;          // tiles of its trace
   int64_t tile_stride;     // tile index row length
@@ -380,6 +381,7 @@ __device__ __forceinline__ WorkTile resolve_work_tile(const PowerArgs& a, int wt
   if (a.wt_trace) {
     const TraceDesc td = a.traces[a.wt_trace[wt]];
     k.tile = a.tile_list[wt];
+    k.trace_start = k.tile + td.tile_base == 0;
     k.tile_col = k.tile;
     k.n_tile = td.n_tile;
     k.tile_stride = (int64_t)td.n_tile + 1;
@@ -389,6 +391,7 @@ __device__ __forceinline__ WorkTile resolve_work_tile(const PowerArgs& a, int wt
     k.tidx = a.tile_idx + td.tile_idx_base;
   } else {
     k.tile = a.tile_list ? a.tile_list[wt] : wt;
+    k.trace_start = k.tile == 0;
     k.tile_col = a.tile_list ? wt : k.tile;
     k.n_tile = a.n_tile;
     k.tile_stride = a.tile_list ? (int64_t)a.n_work_tiles : (int64_t)a.n_tile + 1;
@@ -544,7 +547,7 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
           const bool isclk = (cd.clock_mask >> p) & 1u;
           if (i0 == 0) {
             const uint32_t e0 = evp[p][0];
-            if (tile == 0 && ev_time(e0) == 0) {
+            if (wk.trace_start && ev_time(e0) == 0) {
               // the reference consumes a time-0 event as the initial state
               // (power/TimeBasedPower.cc:782-787)
               cv[p] = ev_val(e0);
@@ -1293,7 +1296,7 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
         uint32_t l = w1;
         fi = i0;
         if (i0 == 0) {
-          if (tile == 0 && ev_time(w1) == 0) {
+          if (wk.trace_start && ev_time(w1) == 0) {
             fi = 1;  // the time-0 event is the initial state
           } else {
             l = 2u;  // unknown state before the net's first event
@@ -1459,7 +1462,7 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
       if (cn[p] > 0) {
         hasm |= 1u << p;
         if (i0[p] == 0) {
-          if (tile == 0 && ev_time(wp1[p]) == 0) {
+          if (wk.trace_start && ev_time(wp1[p]) == 0) {
             l = wp1[p];  // the time-0 event is the initial state (TimeBasedPower.cc:782-787)
             i = 1;
           } else {
@@ -1923,8 +1926,9 @@ __global__ __launch_bounds__(RL_CYCLES * RL_GROUPS) void cycle_reduce_list_kerne
 }
 
 // Batched runs: per-cycle outputs of the work tiles of one round.  Column
-// j = wt * tile_cycles + i is cycle tile_list[wt] * tile_cycles + i of trace
-// wt_trace[wt], written to out + out_base[trace] as [5][n_cycle of the trace].
+// j = wt * tile_cycles + i is cycle c = tile_list[wt] * tile_cycles + i of trace
+// wt_trace[wt] (relative to its load's origin), written to out + out_base[trace]
+// as [5][out_cycles], at column c + cycle_base.
 __global__ void cycle_reduce_batch_kernel(const long long* __restrict__ part, int32_t n_range,
                                           int32_t tile_cycles, const int32_t* __restrict__ tile_list,
                                           const int32_t* __restrict__ wt_trace, int32_t n_wt,
@@ -1938,7 +1942,7 @@ __global__ void cycle_reduce_batch_kernel(const long long* __restrict__ part, in
   const int tr = wt_trace[l];
   const TraceDesc td = traces[tr];
   const int64_t c = (int64_t)tile_list[l] * tile_cycles + ci;
-  if (c >= td.n_cycle) return;
+  if (c >= td.n_cycle) return;  // (relative cycle)
   long long s[N_BIN] = {0, 0, 0, 0, 0, 0};
   for (int r = 0; r < n_range; ++r) {
     const long long* p = part + ((l * n_range + r) * N_BIN) * tile_cycles + ci;
@@ -1946,8 +1950,8 @@ __global__ void cycle_reduce_batch_kernel(const long long* __restrict__ part, in
     for (int k = 0; k < N_BIN; ++k) s[k] += p[(int64_t)k * tile_cycles];
   }
   const int64_t len = min((c + 1) * T, td.t_max) - c * T;
-  double* o = out + out_base[tr];
-  const int64_t C = td.n_cycle;
+  double* o = out + out_base[tr] + td.cycle_base;
+  const int64_t C = td.out_cycles;
   o[0 * C + c] = ((double)s[BIN_LD] * (double)len + (double)s[BIN_LX]) * (q_leak / (double)T);
   o[1 * C + c] = (double)(s[BIN_INT] + s[BIN_GINT]) * q_energy * inv_period;
   o[2 * C + c] = (double)s[BIN_GINT] * q_energy * inv_period;

@@ -22,6 +22,10 @@ import numpy as np
 
 from . import abi
 
+# Event times at or beyond this many ticks do not fit the resident event store;
+# such traces are evaluated in intervals from host memory (gtp_run_sliced).
+LONG_TRACE_TICKS = (1 << 29) - 1
+
 
 @dataclass
 class PowerResult:
@@ -95,8 +99,13 @@ class HipPower:
         self.design = design
 
     def read_power_activities(self, trace):
-        tv = trace.as_abi()
-        self._check(self._lib.gtp_set_trace(self._h, C.byref(tv)), "gtp_set_trace")
+        """Load the trace's event store on the device; a trace of 2^29 - 1
+        ticks or more stays in host memory and power() evaluates it in
+        intervals (gtp_run_sliced)."""
+        self._long = trace.t_max >= LONG_TRACE_TICKS
+        if not self._long:
+            tv = trace.as_abi()
+            self._check(self._lib.gtp_set_trace(self._h, C.byref(tv)), "gtp_set_trace")
         self.trace = trace
 
     def read_vcd(self, vcd, clk_period):
@@ -106,6 +115,8 @@ class HipPower:
         self.trace = VcdClock(vcd.info, clk_period)
 
     def power(self, want=("gate", "cycle", "counts")):
+        if getattr(self, "_long", False):
+            return self.power_sliced(self.trace, 0, want)
         G = self.n_gate
         T = self.trace.ticks_per_cycle
         Cn = -(-self.trace.t_max // T)

@@ -389,7 +389,6 @@ def test_invalid_traces_are_rejected_and_the_context_stays_usable():
     cases["time past t_max"] = variant(tm=t)
     v = val.copy(); v[off[n_busy]] = 7
     cases["value above 3"] = variant(val=v)
-    cases["t_max beyond 2^29 - 1 ticks"] = variant(t_max=(1 << 29) + 5)
     cases["t_max zero"] = variant(t_max=0)
     with HipPower() as eng:
         eng.set_design(d.library, d)
