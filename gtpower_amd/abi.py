@@ -25,7 +25,8 @@ COMPONENTS = ("internal", "glitch_internal", "switching", "glitch_switching", "l
 WAVES = ("leakage", "internal", "glitch_internal", "switching", "glitch_switching")
 CLASS_COMB, CLASS_SEQ, CLASS_CLOCK, CLASS_MACRO, CLASS_PAD = range(5)
 AXIS_SLEW, AXIS_CAP = 0, 1
-MAX_CELL_PINS = 6
+MAX_CELL_PINS = 16
+MAX_WHEN_PINS = 6        # internal-power `when` tables: 2^6 states in a u64
 
 p_i32 = C.POINTER(C.c_int32)
 p_u32 = C.POINTER(C.c_uint32)

@@ -85,7 +85,8 @@ struct gtp_ctx {
   std::vector<HostCell> cells;
   std::vector<CellDev> cell_dev;
   std::vector<float> leak_f;     // concatenated per-cell leakage tables
-  double leak_max = 0;
+  double leak_max = 0;            // largest |leakage| of the cells the design uses
+  int leak_k = 0;                 // common leakage quantum 2^-leak_k W: leak_max <= 2^30 quanta
   DArr<CellDev> d_cells;
   DArr<float> d_leak;
   DArr<long long> d_leak_q;
@@ -192,7 +193,6 @@ gtp_status build_library(gtp_ctx* x, const gtp_library* L, std::vector<SlotDesc>
   if (!L || L->n_cell <= 0 || !L->cell_n_pin || !L->cell_n_in || !L->cell_class ||
       !L->cell_clock_mask || !L->cell_leak_offset || !L->cell_leak_default)
     return x->fail(GTP_ERR_ARG, "gtp_library: missing arrays");
-  if (L->n_cell > MAX_CELLS_LDS) return x->fail(GTP_ERR_LIMIT, "gtp_library: more than 64 cell types");
   if (L->n_table < 0 || (L->n_table > 0 && (!L->tables || !L->table_data)))
     return x->fail(GTP_ERR_ARG, "gtp_library: tables");
   // tables -> float data
@@ -205,7 +205,7 @@ gtp_status build_library(gtp_ctx* x, const gtp_library* L, std::vector<SlotDesc>
     data_len = std::max(data_len, (size_t)tb.axis1 + tb.n1);
     if (tb.n2) data_len = std::max(data_len, (size_t)tb.axis2 + tb.n2);
   }
-  if (data_len > 12000) return x->fail(GTP_ERR_LIMIT, "gtp_library: table data exceeds 12000 values (LDS staging)");
+  if (data_len > (size_t)INT32_MAX) return x->fail(GTP_ERR_LIMIT, "gtp_library: table data beyond 2^31 values");
   tdata.resize(std::max<size_t>(data_len, 1));
   for (size_t i = 0; i < data_len; ++i) tdata[i] = (float)L->table_data[i];
   tables.resize(L->n_table);
@@ -221,7 +221,8 @@ gtp_status build_library(gtp_ctx* x, const gtp_library* L, std::vector<SlotDesc>
     hc.cls = L->cell_class[c];
     hc.clock_mask = L->cell_clock_mask[c];
     if (hc.n_pin <= 0 || hc.n_pin > GTP_MAX_CELL_PINS || hc.n_in < 0 || hc.n_in > hc.n_pin)
-      return x->fail(GTP_ERR_LIMIT, "gtp_library: cell " + std::to_string(c) + " pin count outside 1..6");
+      return x->fail(GTP_ERR_LIMIT, "gtp_library: cell " + std::to_string(c) + " pin count outside 1.." +
+                                        std::to_string(GTP_MAX_CELL_PINS));
     if (hc.cls < 0 || hc.cls >= GTP_N_CLASS) return x->fail(GTP_ERR_ARG, "gtp_library: cell class");
     const int64_t o0 = L->cell_leak_offset[c], o1 = L->cell_leak_offset[c + 1];
     const int ns = 1 << hc.n_pin;
@@ -241,12 +242,16 @@ gtp_status build_library(gtp_ctx* x, const gtp_library* L, std::vector<SlotDesc>
       return x->fail(GTP_ERR_ARG, "gtp_library: ipower " + std::to_string(k));
     hc.groups[ip.pin].push_back(k);
     const uint64_t full = full_truth(hc.n_pin);
-    if ((ip.when_truth & full) != full) hc.sbits = hc.n_pin;
+    if ((ip.when_truth & full) != full) {
+      if (hc.n_pin > MAXP_LARGE)
+        return x->fail(GTP_ERR_UNSUPPORTED, "gtp_library: cell " + std::to_string(ip.cell) +
+                                                ": a `when` on internal power of a cell of more than 6 pins");
+      hc.sbits = hc.n_pin;
+    }
   }
   // slots
   x->cell_dev.assign(L->n_cell, CellDev{});
   x->leak_f.clear();
-  x->leak_max = 0;
   desc_off.assign(L->n_cell, 0);
   descs.clear();
   list.clear();
@@ -258,10 +263,7 @@ gtp_status build_library(gtp_ctx* x, const gtp_library* L, std::vector<SlotDesc>
     cd.clock_mask = hc.clock_mask;
     cd.state_bits = hc.sbits;
     cd.leak_off = (int32_t)x->leak_f.size();
-    for (float v : hc.leak) {
-      x->leak_f.push_back(v);
-      x->leak_max = std::max(x->leak_max, (double)std::fabs(v));
-    }
+    for (float v : hc.leak) x->leak_f.push_back(v);
     desc_off[c] = (int32_t)descs.size();
     const int sb = hc.sbits, nst = 1 << sb;
     int off = 0;
@@ -500,10 +502,40 @@ GTP_API gtp_status gtp_set_design(gtp_ctx* x, const gtp_library* L, const gtp_de
       for (int k = 0; k < 4; ++k)
         if (!std::isnan(ad[i * 4 + k])) cd.arc_mask |= 1u << i;
   }
+  // leakage quanta: a common quantum for the per-cycle sums over gates, from
+  // the largest state of the cells in use (2^30 quanta at most), and per cell
+  // a finer one for its per-gate sums (2^30 quanta for the cell's largest
+  // state, at most 31 bits finer), so that a cell of small leakage keeps its
+  // own relative precision beside cells of much larger leakage
+  std::vector<double> cmax(x->cells.size(), 0.0);
+  std::vector<char> used(x->cells.size(), 0);
+  for (int64_t g = 0; g < G; ++g) used[d->gate_cell[g]] = 1;
+  x->leak_max = 0;
+  for (size_t c = 0; c < x->cells.size(); ++c) {
+    for (float v : x->cells[c].leak) cmax[c] = std::max(cmax[c], (double)std::fabs(v));
+    if (used[c]) x->leak_max = std::max(x->leak_max, cmax[c]);
+  }
+  auto quantum_exp = [](double m) { return (int)std::floor(30.0 - std::log2(m * 1.000001)); };
+  x->leak_k = x->leak_max > 0 ? quantum_exp(x->leak_max) : 0;
+  std::vector<long long> lq(x->leak_f.size());
+  for (size_t c = 0; c < x->cells.size(); ++c) {
+    CellDev& cd = x->cell_dev[c];
+    const int kc = cmax[c] > 0 ? std::min(std::max(quantum_exp(cmax[c]), x->leak_k), x->leak_k + 31) : x->leak_k;
+    if (kc > 120 || x->leak_k < -120)
+      return x->fail(GTP_ERR_UNSUPPORTED, "gtp_design: leakage magnitudes outside the FP32 range");
+    cd.leak_shift = kc - x->leak_k;
+    const int ns = 1 << cd.n_pin;
+    for (int st = 0; st < ns; ++st) {
+      const double v = (double)x->leak_f[cd.leak_off + st];
+      const long long common = std::llrint(std::ldexp(v, x->leak_k));
+      const long long own = std::llrint(std::ldexp(v, kc));
+      lq[cd.leak_off + st] = (long long)((unsigned long long)common << 32 | (uint32_t)own);
+    }
+  }
   hipStream_t s = x->stream;
   HIPCHK(x, x->d_cells.upload(x->cell_dev.data(), x->cell_dev.size(), s));
   HIPCHK(x, x->d_leak.upload(x->leak_f.data(), x->leak_f.size(), s));
-  HIPCHK(x, x->d_leak_q.alloc(x->leak_f.size()));
+  HIPCHK(x, x->d_leak_q.upload(lq.data(), lq.size(), s));
   HIPCHK(x, x->d_gate_cell.upload(d->gate_cell, G, s));
   HIPCHK(x, x->d_gate_class.upload(gclass.data(), G, s));
   HIPCHK(x, x->d_gate_pin_off.upload(d->gate_pin_offset, G + 1, s));
@@ -783,18 +815,17 @@ void build_order(gtp_ctx* x, gtp_ctx::OrderSet& os, int32_t range_gates, const s
                  int64_t T) {
   const int nc = (int)x->cells.size();
   std::vector<std::vector<int32_t>> by_cell(nc);
-  std::vector<int32_t> generic;
-  int generic_maxp = 0;
+  // the generic kernel's gates by pin-count class: <= 4, <= 6, <= 8, <= 16 pins
+  std::vector<int32_t> generic[4];
   for (int64_t g = 0; g < x->G; ++g) {
     const HostCell& hc = x->cells[x->gate_cell[g]];
     // the specialised kernel compares arrivals on 32 bits: cycles below 2^15 ticks
     const bool fast = hc.sbits == 0 && hc.n_pin - hc.n_in == 1 && hc.n_pin >= 2 && hc.n_pin <= 6 &&
                       T < 32768;
     if (fast) by_cell[x->gate_cell[g]].push_back((int32_t)g);
-    else {
-      generic.push_back((int32_t)g);
-      generic_maxp = std::max(generic_maxp, hc.n_pin);
-    }
+    else
+      generic[hc.n_pin <= MAXP_SMALL ? 0 : hc.n_pin <= MAXP_LARGE ? 1 : hc.n_pin <= MAXP_WIDE ? 2 : 3].push_back(
+          (int32_t)g);
   }
   auto by_work = [&](int32_t p, int32_t q) {
     if (work[p] != work[q]) return work[p] > work[q];
@@ -813,7 +844,10 @@ void build_order(gtp_ctx* x, gtp_ctx::OrderSet& os, int32_t range_gates, const s
     os.launches.push_back(gtp_ctx::Launch{cell, np, rb, nr});
   };
   for (int c = 0; c < nc; ++c) add(by_cell[c], c, x->cells[c].n_pin);
-  add(generic, -1, generic_maxp <= MAXP_SMALL ? MAXP_SMALL : MAXP_LARGE);
+  add(generic[0], -1, MAXP_SMALL);
+  add(generic[1], -1, MAXP_LARGE);
+  add(generic[2], -1, MAXP_WIDE);
+  add(generic[3], -1, MAXP_HUGE);
   os.n_range = (int32_t)(os.order.size() / range_gates);
 }
 
@@ -854,8 +888,7 @@ gtp_status make_quanta(gtp_ctx* x, int64_t t_max, const std::vector<int64_t>& wo
   double emax = std::max((double)x->energy_max * 1.000001, x->csw_max);
   if (!(emax > 0)) emax = 1e-30;
   const int ke = (int)std::floor((double)ENERGY_TERM_BITS - std::log2(emax));
-  const double lmax = x->leak_max > 0 ? x->leak_max * 1.000001 : 1e-30;
-  const int kl = (int)std::floor(30.0 - std::log2(lmax));
+  const int kl = x->leak_k;
   q->inv_e = std::ldexp(1.0f, ke - ENERGY_FINE_BITS);
   q->inv_l = std::ldexp(1.0f, kl);
   q->q_e = std::ldexp(1.0, -ke);
@@ -864,10 +897,6 @@ gtp_status make_quanta(gtp_ctx* x, int64_t t_max, const std::vector<int64_t>& wo
     return x->fail(GTP_ERR_UNSUPPORTED, "gtp_run: energy or leakage magnitudes outside the FP32 range");
   if ((double)t_max * 2147483648.0 >= 9.2e18)
     return x->fail(GTP_ERR_LIMIT, "gtp_run: trace too long for the int64 leakage sums");
-  std::vector<long long> lq(x->leak_f.size());
-  for (size_t i = 0; i < lq.size(); ++i) lq[i] = std::llrint((double)x->leak_f[i] * (double)q->inv_l);
-  HIPCHK(x, hipMemcpyAsync(x->d_leak_q.p, lq.data(), lq.size() * sizeof(long long), hipMemcpyHostToDevice, x->stream));
-  HIPCHK(x, hipStreamSynchronize(x->stream));
   return GTP_OK;
 }
 
@@ -906,8 +935,12 @@ void fill_args(gtp_ctx* x, const gtp_ctx::OrderSet& os, const Quanta& qt, int64_
 
 gtp_status lds_sizes(gtp_ctx* x, const PowerArgs& a, RunPlan* rp) {
   const int64_t TC = a.tile_cycles;
-  rp->lds = (size_t)N_BIN * TC * 8 + (size_t)a.n_leak * 8 + (size_t)(a.n_leak + (a.n_leak & 1)) * 4 +
-            (size_t)a.n_cell * sizeof(CellDev);
+  // the generic kernel stages the leakage tables when they are small (else it
+  // reads them from device memory)
+  rp->a.leak_lds = a.n_leak <= LEAK_LDS_MAX ? 1 : 0;
+  const size_t n_lds_leak = rp->a.leak_lds ? (size_t)a.n_leak : 0;
+  rp->lds = (size_t)N_BIN * TC * 8 + n_lds_leak * 8 + (n_lds_leak + (n_lds_leak & 1)) * 4 +
+            (size_t)(a.n_cell <= MAX_CELLS_LDS ? a.n_cell : 0) * sizeof(CellDev);
   // the specialised kernel's bins (one spare column per row): up to 4
   // lane-interleaved copies, as many as fit 32 KB (measured: more copies of
   // longer tiles cost occupancy and gain nothing), at least one
@@ -1181,8 +1214,8 @@ gtp_status run_once(gtp_ctx* x, gtp_result* res, int64_t tc_req, int32_t rg_req)
   HIPCHK(x, launch_cycle_reduce(rp.os->d_cyc_part.p, rp.n_range, (int32_t)rp.TC, x->C, x->T, x->t_max, rp.q_e, rp.q_l,
                                 1.0 / x->period, x->d_cycle.p, s));
   HIPCHK(x, launch_gate_reduce(rp.os->d_pg.p, rp.os->d_pg_tog.p, rp.os->d_pg_glc.p, rp.os->d_order.p, rp.n_sched, rp.n_tile, x->G,
-                               rp.q_e / ((double)x->t_max * x->scale), rp.q_l / (double)x->t_max, x->d_gate.p,
-                               x->d_gtog.p, x->d_gglc.p, s));
+                               rp.q_e / ((double)x->t_max * x->scale), rp.q_l / (double)x->t_max,
+                               x->d_gate_cell.p, x->d_cells.p, x->d_gate.p, x->d_gtog.p, x->d_gglc.p, s));
   HIPCHK(x, launch_total(x->d_gate.p, x->d_gate_class.p, x->G, x->d_total_part.p, x->d_total.p, s));
   if (res->cycle_power)
     HIPCHK(x, hipMemcpyAsync(res->cycle_power, x->d_cycle.p, sizeof(double) * GTP_N_WAVE * x->C, hipMemcpyDeviceToHost, s));
@@ -1586,7 +1619,8 @@ gtp_status finish_slot(gtp_ctx* x, const MultiPlan& mp, int32_t slot, const DArr
   HIPCHK(x, x->d_total_part.alloc((size_t)TOTAL_BLOCKS * 30));
   const double tm = (double)t_max;
   HIPCHK(x, launch_gate_finalize((const long long*)d_acc.p + (int64_t)slot * N_PG * G, G,
-                                 mp.qt.q_e / (tm * mp.time_scale), mp.qt.q_l / tm, x->d_gate.p, s));
+                                 mp.qt.q_e / (tm * mp.time_scale), mp.qt.q_l / tm, x->d_gate_cell.p, x->d_cells.p,
+                                 x->d_gate.p, s));
   HIPCHK(x, launch_total(x->d_gate.p, x->d_gate_class.p, G, x->d_total_part.p, x->d_total.p, s));
   if (res.gate_power)
     HIPCHK(x, hipMemcpyAsync(res.gate_power, x->d_gate.p, sizeof(double) * GTP_N_COMPONENT * G,

@@ -63,9 +63,13 @@ constexpr uint32_t EV_GLITCH = 4u;              // the event carries a glitch pu
 constexpr uint32_t EV_INF = 0xFFFFFFF8u;       // time field 2^29 - 1, glitch bit and value clear
 constexpr uint32_t EV_TIME_INF = 0x1FFFFFFFu;
 constexpr int64_t EV_PAD = 4;                   // words of padding after the event array
-constexpr int MAXP_SMALL = 4;                   // kernel variant for cells with <= 4 pins
-constexpr int MAXP_LARGE = 6;                   // GTP_MAX_CELL_PINS
-constexpr int MAX_CELLS_LDS = 64;               // cell types staged in LDS
+constexpr int MAXP_SMALL = 4;                   // generic kernel variant for cells with <= 4 pins
+constexpr int MAXP_LARGE = 6;                   // ... <= 6 pins (conditional internal power: up to here)
+constexpr int MAXP_WIDE = 8;                    // ... <= 8 pins (the widest sky130 cells)
+constexpr int MAXP_HUGE = 16;                   // ... <= 16 pins (GTP_MAX_CELL_PINS)
+constexpr int LEAK_LDS_MAX = 4096;              // leakage table entries the generic kernel stages in LDS
+constexpr int MAX_CELLS_LDS = 64;               // generic kernel: cell types staged in LDS (more: read in place)
+constexpr int TABLE_LDS_MAX = 12000;            // energy kernel: table values staged in LDS (more: read in place)
 constexpr int ARC_FRAC_BITS = 16;               // related-pin comparison grid: 2^-16 tick
 constexpr int ENERGY_FINE_BITS = 16;            // energy quanta per coarse unit of PowerArgs.inv_q_energy
 constexpr int ENERGY_TERM_BITS = 36;            // one energy term <= 2^36 quanta
@@ -98,8 +102,9 @@ struct CellDev {
   int32_t state_bits;        // n_pin when some internal power is `when`-conditioned, else 0
   int32_t leak_off;          // into the LDS leakage table (2^n_pin floats)
   int32_t n_slot;            // energy slots per gate of this cell
-  int32_t slot_off[6];       // energy slot base of each pin
+  int32_t slot_off[16];      // energy slot base of each pin
   uint32_t arc_mask;         // bit q: some gate of this cell has an arc from input q (single output)
+  int32_t leak_shift;        // per-gate leakage sums of this cell count quanta of 2^-leak_shift common quanta
 };
 
 // One energy slot of a cell: which table to interpolate at which pin's slew.
@@ -185,8 +190,12 @@ struct PowerArgs {
   const CellDev* cells;
   int32_t n_cell;
   const float* leak;             // [n_leak] leakage per gate state (W)
-  const long long* leak_q;       // [n_leak] the same in leakage quanta
+  // [n_leak] the same in leakage quanta: the common quantum (per-cycle sums
+  // over gates) << 32 | the cell's own quantum (per-gate sums, leak_shift
+  // bits finer; at most 2^30 for the cell's largest state)
+  const long long* leak_q;
   int32_t n_leak;
+  int32_t leak_lds;              // generic kernel: 1 = leakage tables staged in LDS (n_leak <= LEAK_LDS_MAX)
   int32_t bin_copies;            // lane-interleaved copies of the per-cycle bins (specialised kernel)
   // fixed-point scales (powers of two): term / quantum
   float inv_q_energy;            // coarse energy scale: one term <= 2^20 units, 2^ENERGY_FINE_BITS quanta each
@@ -254,13 +263,14 @@ hipError_t launch_gate_accumulate(const long long* pg, const uint32_t* tog, cons
                                   const int32_t* round_trace, int32_t n_round_trace, int64_t n_gate, long long* acc,
                                   unsigned long long* acc_cnt, hipStream_t s);
 hipError_t launch_gate_finalize(const long long* acc, int64_t n_gate, double norm_energy, double norm_leak,
-                                double* gate_power, hipStream_t s);
+                                const int32_t* gate_cell, const CellDev* cells, double* gate_power, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long* p, int64_t n, hipStream_t s);
 hipError_t launch_zero_u32(unsigned int* p, int64_t n, hipStream_t s);
 hipError_t launch_gate_reduce(const long long* pg, const uint32_t* tog, const uint32_t* glc,
                               const int32_t* order, int64_t n_sched, int32_t n_tile, int64_t n_gate,
-                              double norm_energy, double norm_leak, double* gate_power,
-                              unsigned long long* gate_tog, unsigned long long* gate_glitch, hipStream_t s);
+                              double norm_energy, double norm_leak, const int32_t* gate_cell,
+                              const CellDev* cells, double* gate_power, unsigned long long* gate_tog,
+                              unsigned long long* gate_glitch, hipStream_t s);
 constexpr int TOTAL_BLOCKS = 256;
 hipError_t launch_chunk_net(const int64_t* chunk_off, int64_t n_net, int32_t* chunk_net, hipStream_t s);
 hipError_t launch_glitch_annotate(const int64_t* net_off, uint32_t* ev, const int64_t* chunk_off,

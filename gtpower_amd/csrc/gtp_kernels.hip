@@ -316,9 +316,12 @@ __device__ float interp_f(const float* D, const TableDev& tb, float slew, float 
 
 
 __global__ void energy_kernel(EnergyArgs a) {
-  extern __shared__ float s_tab[];
-  for (int i = threadIdx.x; i < a.n_table_data; i += blockDim.x) s_tab[i] = a.table_data[i];
+  extern __shared__ float s_tab_lds[];
+  const bool staged = a.n_table_data <= TABLE_LDS_MAX;
+  if (staged)
+    for (int i = threadIdx.x; i < a.n_table_data; i += blockDim.x) s_tab_lds[i] = a.table_data[i];
   __syncthreads();
+  const float* s_tab = staged ? s_tab_lds : a.table_data;
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   float local_max = 0.f;
   if (g < a.n_gate) {
@@ -458,17 +461,24 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int TC = a.tile_cycles;
   long long* bins = reinterpret_cast<long long*>(smem_raw);
-  long long* s_leakq = bins + (size_t)N_BIN * TC;
-  float* s_leak = reinterpret_cast<float*>(s_leakq + a.n_leak);
-  CellDev* s_cells = reinterpret_cast<CellDev*>(s_leak + a.n_leak + (a.n_leak & 1));
+  // leakage tables (all cells) in LDS when they are small, else read in place
+  const int n_ll = a.leak_lds ? a.n_leak : 0;
+  long long* lds_leakq = bins + (size_t)N_BIN * TC;
+  float* lds_leak = reinterpret_cast<float*>(lds_leakq + n_ll);
+  // cell records in LDS when there are few cell types, else read in place
+  const int n_lc = a.n_cell <= MAX_CELLS_LDS ? a.n_cell : 0;
+  CellDev* lds_cells = reinterpret_cast<CellDev*>(lds_leak + n_ll + (n_ll & 1));
   __shared__ long long s_base;
   __shared__ long long s_item;
 
-  for (int i = threadIdx.x; i < a.n_leak; i += blockDim.x) {
-    s_leak[i] = a.leak[i];
-    s_leakq[i] = a.leak_q[i];
+  for (int i = threadIdx.x; i < n_ll; i += blockDim.x) {
+    lds_leak[i] = a.leak[i];
+    lds_leakq[i] = a.leak_q[i];
   }
-  for (int i = threadIdx.x; i < a.n_cell; i += blockDim.x) s_cells[i] = a.cells[i];
+  const long long* s_leakq = a.leak_lds ? lds_leakq : a.leak_q;
+  const float* s_leak = a.leak_lds ? lds_leak : a.leak;
+  for (int i = threadIdx.x; i < n_lc; i += blockDim.x) lds_cells[i] = a.cells[i];
+  const CellDev* s_cells = n_lc ? lds_cells : a.cells;
 
   const int64_t n_work = (int64_t)a.n_work_tiles * range_count;
   int64_t w = 0;
@@ -588,7 +598,7 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
       if (p < n && cv[p] == 1) sb |= 1u << p;
     }
     const int loff = cd.leak_off;
-    base_acc += s_leakq[loff + sb];
+    base_acc += s_leakq[loff + sb] >> 32;
 
     float acc_hi[N_PG], acc_lo[N_PG];
 #pragma unroll
@@ -638,7 +648,7 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
       const float Lb = s_leak[loff + sb];
       ff_add_prod(acc_hi[PG_LEAK], acc_lo[PG_LEAK], Lb, (float)(t - prev_t));
       if (sa != sb) {
-        const long long dq = s_leakq[loff + sa] - s_leakq[loff + sb];
+        const long long dq = (s_leakq[loff + sa] >> 32) - (s_leakq[loff + sb] >> 32);
         p_ld += dq;
         p_lx += dq * (long long)(t - c * T);
       }
@@ -646,7 +656,11 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
       float ri = 0.f, rg = 0.f, rs = 0.f, gs = 0.f;
       bool Fi[MAXP];
       int ji[MAXP];
-#pragma unroll
+      // fully unrolled up to 8 pins; the 16-pin variant steps its pins in a
+      // loop (the compiler cannot unroll it fully, and its partial unrolling
+      // of this loop gave wrong glitch flags against the oracle)
+      constexpr int UNROLL_PINS = MAXP <= MAXP_WIDE ? MAXP : 1;
+#pragma unroll UNROLL_PINS
       for (int p = 0; p < MAXP; ++p) {
         Fi[p] = false;
         ji[p] = -1;
@@ -750,9 +764,10 @@ __global__ __launch_bounds__(256) void power_tile_kernel(PowerArgs a, int32_t ra
     if (lend > prev_t) ff_add_prod(acc_hi[PG_LEAK], acc_lo[PG_LEAK], s_leak[loff + sb], (float)(lend - prev_t));
     const int64_t slot = (int64_t)wt * a.n_sched + gi;
     const int64_t plane = (int64_t)a.n_work_tiles * a.n_sched;
+    const float iql = ldexpf(a.inv_q_leak, cd.leak_shift);  // the cell's own leakage quanta
 #pragma unroll
     for (int k = 0; k < N_PG; ++k) {
-      const float iq = k == PG_LEAK ? a.inv_q_leak : inv_qf;  // powers of two: exact scaling
+      const float iq = k == PG_LEAK ? iql : inv_qf;  // powers of two: exact scaling
       a.pg[k * plane + slot] = __float2ll_rn(acc_hi[k] * iq) + __float2ll_rn(acc_lo[k] * iq);
     }
     a.pg_tog[slot] = tog;
@@ -1226,7 +1241,7 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
   __shared__ long long ck_l[2];               // clock net: first event, events
   __shared__ int ck_s[8];                     // clock net, first edge index, first-event flag, edges, overflow
   __shared__ unsigned long long ck_t[4];      // scan scratch
-  __shared__ int2 ck_lk[CLKB ? 4 : 1];        // per non-clock state: leakage quanta, clock low and high
+  __shared__ longlong2 ck_lk[CLKB ? 4 : 1];   // per non-clock state: leakage quanta, clock low and high
   const CellDev cd = a.cells[cell];
   const uint32_t clk = cd.clock_mask;
   const uint32_t arcm = cd.arc_mask;  // inputs that can be the related pin of an output event
@@ -1234,7 +1249,7 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
   if constexpr (CLKB) {
     for (int j = threadIdx.x; j < 4; j += blockDim.x) {
       const uint32_t s0 = ck_unpack<KC>((uint32_t)j, 0u), s1 = ck_unpack<KC>((uint32_t)j, 1u);
-      ck_lk[j] = make_int2((int32_t)a.leak_q[cd.leak_off + s0], (int32_t)a.leak_q[cd.leak_off + s1]);
+      ck_lk[j] = make_longlong2(a.leak_q[cd.leak_off + s0], a.leak_q[cd.leak_off + s1]);
     }
   }
   const int n_copy = a.bin_copies;
@@ -1511,18 +1526,19 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
     for (int k = 0; k < NS; ++k) asm volatile("" : "+v"(E[k]));
 #pragma unroll
     for (int k = 0; k < 4 * NIN; ++k) asm volatile("" : "+v"(D[k]));
-    base_acc += s_leakq[sb];
+    base_acc += s_leakq[sb] >> 32;
 
     long long acc[N_PG];     // per-gate sums in quanta (leakage: quanta * ticks)
 #pragma unroll
     for (int k = 0; k < N_PG; ++k) acc[k] = 0;
     uint32_t tog = 0, glc = 0;
     uint32_t prev_t = tstart;
-    long long lq_cur = s_leakq[sb];  // leakage of the current state, in quanta
+    // leakage of the current state: common quanta << 32 | the cell's own quanta
+    long long lq_cur = s_leakq[sb];
     // CLKB: the leakage of the current non-clock state with the clock low and
     // high, the clock-high ticks from the tile start to prev_t, and whether
     // the clock events come from the item's edges (bulk)
-    int32_t lq_lo = 0, lq_hi = 0;
+    long long lq_lo = 0, lq_hi = 0;
     uint32_t h_prev = 0;
     bool bulk = false;
     if constexpr (CLKB) {
@@ -1533,8 +1549,8 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
         atomicAdd(&ck_d[0], 1ull << (16 * ck_pack<KC>(sb)));
       }
       const uint32_t s0 = sb & ~(1u << KC), s1 = sb | (1u << KC);
-      lq_lo = (int32_t)s_leakq[s0];
-      lq_hi = (int32_t)s_leakq[s1];
+      lq_lo = s_leakq[s0];
+      lq_hi = s_leakq[s1];
     }
 
     // One pin event per step: the earliest pending event, the output pin's
@@ -1582,7 +1598,7 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
         const uint32_t sa = (sb & ~(1u << ps)) | ((nv == 1 ? 1u : 0u) << ps);
         const long long lq_new = s_leakq[sa];
         // (zero when the state does not change)
-        const long long dq = lq_new - lq_cur;
+        const long long dq = (lq_new >> 32) - (lq_cur >> 32);
         atomicAdd((unsigned long long*)&cb[BIN_LD * TCP], (unsigned long long)dq);
         atomicAdd((unsigned long long*)&cb[BIN_LX * TCP], (unsigned long long)(dq * (long long)(t - cstart)));
         sb = sa;
@@ -1616,16 +1632,16 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
           H = h_prev + (bit ? t - prev_t : 0u);
         }
         const uint32_t dH = H - h_prev, dt = t - prev_t;
-        acc[PG_LEAK] += (long long)lq_hi * (long long)dH + (long long)lq_lo * (long long)(dt - dH);
+        acc[PG_LEAK] += (long long)(int)lq_hi * (long long)dH + (long long)(int)lq_lo * (long long)(dt - dH);
         h_prev = H;
         const uint32_t sbe = (sb & ~(1u << KC)) | (bit << KC);
         const uint32_t sa = (sbe & ~(1u << ps)) | ((nv == 1 ? 1u : 0u) << ps);
         const uint32_t sa0 = sa & ~(1u << KC);
-        const int2 lk = ck_lk[ck_pack<KC>(sa)];
-        const int32_t nlo = lk.x, nhi = lk.y;
+        const longlong2 lk = ck_lk[ck_pack<KC>(sa)];
+        const long long nlo = lk.x, nhi = lk.y;
         const bool abit = (sa >> KC) & 1u;
         const long long lq_c = bit ? lq_hi : lq_lo, lq_new = abit ? nhi : nlo;
-        const long long dq = lq_new - lq_c;
+        const long long dq = (lq_new >> 32) - (lq_c >> 32);
         atomicAdd((unsigned long long*)&cb[BIN_LD * TCP], (unsigned long long)dq);
         atomicAdd((unsigned long long*)&cb[BIN_LX * TCP], (unsigned long long)(dq * (long long)(t - cstart)));
         // a new non-clock state counts from the next unconsumed edge on
@@ -1776,7 +1792,7 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
         H = h_prev + (((sb >> KC) & 1u) ? endp - prev_t : 0u);
       }
       const uint32_t dH = H - h_prev, dt = endp - prev_t;
-      acc[PG_LEAK] += (long long)lq_hi * (long long)dH + (long long)lq_lo * (long long)(dt - dH);
+      acc[PG_LEAK] += (long long)(int)lq_hi * (long long)dH + (long long)(int)lq_lo * (long long)(dt - dH);
     }
     acc[PG_SW] -= acc[PG_GSW];
     acc[PG_INT] -= acc[PG_GINT];
@@ -1814,7 +1830,7 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
           for (int j = 0; j < 4; ++j) {
             const long long n = (long long)((cnt >> (16 * j)) & 0xFFFFull);
             const uint32_t so = ck_unpack<KC>((uint32_t)j, bo), sn = ck_unpack<KC>((uint32_t)j, bn);
-            ld += n * (s_leakq[sn] - s_leakq[so]);
+            ld += n * ((s_leakq[sn] >> 32) - (s_leakq[so] >> 32));
           }
           // the gates' steps dL each, and dL * (t - cycle start) in total
           atomicAdd((unsigned long long*)&bins[BIN_LD * TCP + cr], (unsigned long long)ld);
@@ -1846,6 +1862,8 @@ void power_cell_kernel(PowerArgs a, int32_t range_begin, int32_t range_count, in
 
 template __global__ void power_tile_kernel<MAXP_SMALL>(PowerArgs, int32_t, int32_t, unsigned int*);
 template __global__ void power_tile_kernel<MAXP_LARGE>(PowerArgs, int32_t, int32_t, unsigned int*);
+template __global__ void power_tile_kernel<MAXP_WIDE>(PowerArgs, int32_t, int32_t, unsigned int*);
+template __global__ void power_tile_kernel<MAXP_HUGE>(PowerArgs, int32_t, int32_t, unsigned int*);
 
 // ---------------------------------------------------------------------------
 // Reductions
@@ -1992,10 +2010,13 @@ __global__ void gate_accumulate_kernel(const long long* __restrict__ pg, const u
 
 // Batched runs: per-gate power of one trace from its exact sums.
 __global__ void gate_finalize_kernel(const long long* __restrict__ acc, int64_t n_gate, double norm_energy,
-                                     double norm_leak, double* __restrict__ gate_power) {
+                                     double norm_leak, const int32_t* __restrict__ gate_cell,
+                                     const CellDev* __restrict__ cells, double* __restrict__ gate_power) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)N_PG * n_gate) return;
-  gate_power[i] = (double)acc[i] * (i / n_gate == PG_LEAK ? norm_leak : norm_energy);
+  // leakage sums count the gate's cell's own quanta
+  const double norm = i / n_gate == PG_LEAK ? ldexp(norm_leak, -cells[gate_cell[i % n_gate]].leak_shift) : norm_energy;
+  gate_power[i] = (double)acc[i] * norm;
 }
 
 __global__ void fill_u64_kernel(unsigned long long* __restrict__ p, int64_t n) {
@@ -2011,7 +2032,8 @@ __global__ void fill_u32_zero_kernel(unsigned int* __restrict__ p, int64_t n) {
 __global__ void gate_reduce_kernel(const long long* __restrict__ pg, const uint32_t* __restrict__ tog,
                                    const uint32_t* __restrict__ glc, const int32_t* __restrict__ order,
                                    int64_t n_sched, int32_t n_tile, int64_t n_gate, double norm_energy,
-                                   double norm_leak, double* __restrict__ gate_power,
+                                   double norm_leak, const int32_t* __restrict__ gate_cell,
+                                   const CellDev* __restrict__ cells, double* __restrict__ gate_power,
                                    unsigned long long* __restrict__ gate_tog,
                                    unsigned long long* __restrict__ gate_glitch) {
   const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2019,11 +2041,13 @@ __global__ void gate_reduce_kernel(const long long* __restrict__ pg, const uint3
   const int g = order[gi];
   if (g < 0) return;
   const int64_t plane = (int64_t)n_tile * n_sched;
+  // leakage sums count the gate's cell's own quanta
+  const double norm_leak_g = ldexp(norm_leak, -cells[gate_cell[g]].leak_shift);
 #pragma unroll
   for (int k = 0; k < N_PG; ++k) {
     long long v = 0;
     for (int t = 0; t < n_tile; ++t) v += pg[k * plane + (int64_t)t * n_sched + gi];
-    gate_power[(int64_t)k * n_gate + g] = (double)v * (k == PG_LEAK ? norm_leak : norm_energy);
+    gate_power[(int64_t)k * n_gate + g] = (double)v * (k == PG_LEAK ? norm_leak_g : norm_energy);
   }
   unsigned long long tg = 0, gg = 0;
   for (int t = 0; t < n_tile; ++t) {
@@ -2109,7 +2133,8 @@ hipError_t launch_tile_order(const int32_t* order, int64_t n_sched, int32_t rang
 hipError_t launch_energy(const EnergyArgs& a, hipStream_t s) {
   if (a.n_gate <= 0) return hipSuccess;
   const int64_t blocks = (a.n_gate + 255) / 256;
-  energy_kernel<<<dim3((unsigned)blocks), dim3(256), sizeof(float) * a.n_table_data, s>>>(a);
+  const size_t lds = a.n_table_data <= TABLE_LDS_MAX ? sizeof(float) * a.n_table_data : 0;
+  energy_kernel<<<dim3((unsigned)blocks), dim3(256), lds, s>>>(a);
   return hipGetLastError();
 }
 
@@ -2118,8 +2143,14 @@ hipError_t launch_power(int maxp, int64_t grid, int block, size_t lds, hipStream
   if (grid <= 0) return hipSuccess;
   if (maxp <= MAXP_SMALL)
     power_tile_kernel<MAXP_SMALL><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count, wq);
-  else
+  else if (maxp <= MAXP_LARGE)
     power_tile_kernel<MAXP_LARGE><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count, wq);
+  else if (maxp <= MAXP_WIDE)
+    power_tile_kernel<MAXP_WIDE><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count, wq);
+  else if (maxp <= MAXP_HUGE)
+    power_tile_kernel<MAXP_HUGE><<<dim3((unsigned)grid), dim3(block), lds, s>>>(a, range_begin, range_count, wq);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
@@ -2165,11 +2196,11 @@ hipError_t launch_gate_accumulate(const long long* pg, const uint32_t* tog, cons
 }
 
 hipError_t launch_gate_finalize(const long long* acc, int64_t n_gate, double norm_energy, double norm_leak,
-                                double* gate_power, hipStream_t s) {
+                                const int32_t* gate_cell, const CellDev* cells, double* gate_power, hipStream_t s) {
   const int64_t n = (int64_t)N_PG * n_gate;
   if (n <= 0) return hipSuccess;
   gate_finalize_kernel<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(acc, n_gate, norm_energy, norm_leak,
-                                                                              gate_power);
+                                                                              gate_cell, cells, gate_power);
   return hipGetLastError();
 }
 
@@ -2187,12 +2218,13 @@ hipError_t launch_zero_u32(unsigned int* p, int64_t n, hipStream_t s) {
 
 hipError_t launch_gate_reduce(const long long* pg, const uint32_t* tog, const uint32_t* glc,
                               const int32_t* order, int64_t n_sched, int32_t n_tile, int64_t n_gate,
-                              double norm_energy, double norm_leak, double* gate_power,
-                              unsigned long long* gate_tog, unsigned long long* gate_glitch, hipStream_t s) {
+                              double norm_energy, double norm_leak, const int32_t* gate_cell,
+                              const CellDev* cells, double* gate_power, unsigned long long* gate_tog,
+                              unsigned long long* gate_glitch, hipStream_t s) {
   if (n_sched <= 0) return hipSuccess;
   gate_reduce_kernel<<<dim3((unsigned)((n_sched + 255) / 256)), dim3(256), 0, s>>>(
-      pg, tog, glc, order, n_sched, n_tile, n_gate, norm_energy, norm_leak, gate_power, gate_tog,
-      gate_glitch);
+      pg, tog, glc, order, n_sched, n_tile, n_gate, norm_energy, norm_leak, gate_cell, cells, gate_power,
+      gate_tog, gate_glitch);
   return hipGetLastError();
 }
 

@@ -164,10 +164,15 @@ def conjunction_literals(tree, negated=False):
 
 
 def when_truth(when, pin_names):
-    """Truth table of a `when` string over gate states (bit p = pin p is 1)."""
+    """Truth table of a `when` string over gate states (bit p = pin p is 1).
+    Cells of more than abi.MAX_WHEN_PINS pins take unconditional groups only:
+    all 64 bits set (no sky130 cell of 7 or 8 pins conditions its internal
+    power)."""
     n = len(pin_names)
     if not when:
-        return (1 << (1 << n)) - 1
+        return (1 << (1 << min(n, abi.MAX_WHEN_PINS))) - 1
+    if n > abi.MAX_WHEN_PINS:
+        raise ValueError(f"`when` on internal power of a {n}-pin cell (at most {abi.MAX_WHEN_PINS} pins)")
     tree = parse_expr(when)
     idx = {name: k for k, name in enumerate(pin_names)}
     mask = 0
@@ -288,8 +293,9 @@ class CellLibrary:
                 leak=leak, leak_default=leak_default, clock_mask=clock_mask,
                 cls=abi.CLASS_SEQ if is_ff else abi.CLASS_COMB, arcs=arcs, func=func,
                 is_ff=is_ff,
-                ff_clock=pidx[c["ff"]["clocked_on"]] if is_ff else -1,
-                ff_data=pidx[c["ff"]["next_state"]] if is_ff else -1))
+                # a plain pin name, else -1 (`!CLK_N`, `(D&!SCE) | (SCD&SCE)`)
+                ff_clock=pidx.get(c["ff"]["clocked_on"], -1) if is_ff else -1,
+                ff_data=pidx.get(c["ff"]["next_state"], -1) if is_ff else -1))
         self.ipowers = ipowers
         self._finish()
 

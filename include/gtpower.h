@@ -93,7 +93,9 @@ typedef struct {
  * (reference: power/TimeBasedPower.cc:373-378, :489-502).
  * when_truth bit s is set when the condition holds for gate state index s,
  * the state index being sum over cell pins p of (value_p == 1) << p
- * (reference: power/PowerUtils.cc:39-48, :155-187). */
+ * (reference: power/PowerUtils.cc:39-48, :155-187).  Cells of more than 6
+ * pins take unconditional groups only (when_truth = all ones); a `when` there
+ * is GTP_ERR_UNSUPPORTED. */
 typedef struct {
   int32_t cell;
   int32_t pin;           /* cell pin index: inputs first, then outputs */
@@ -106,8 +108,10 @@ typedef struct {
 
 /* The cell library, flattened.  Pins of a cell are ordered inputs first, then
  * outputs, as power/TimeBasedPower.cc:223-256 orders them.  Cells may have at
- * most GTP_MAX_CELL_PINS pins. */
-#define GTP_MAX_CELL_PINS 6
+ * most GTP_MAX_CELL_PINS pins, the reference's limit for state-indexed
+ * leakage and internal power (util/GlobalConfig.hh:42-43); its cells of more
+ * pins are counted as fallbacks (power/TimeBasedPower.cc:100-102). */
+#define GTP_MAX_CELL_PINS 16
 typedef struct {
   int32_t n_cell;
   const int32_t* cell_n_pin;        /* [n_cell] */

@@ -166,8 +166,10 @@ struct Ctx {
   std::vector<CellInfo> cells;
 };
 
+// Cells of more than 6 pins carry unconditional groups only (when_truth all
+// ones), so a state index of 64 or more holds exactly when every bit is set.
 inline bool when_holds(const gtp_internal_power& ip, uint32_t state) {
-  return (ip.when_truth >> state) & 1ull;
+  return state < 64 ? ((ip.when_truth >> state) & 1ull) != 0 : ip.when_truth == ~0ull;
 }
 
 // power(rf, op_cond, slew, cap) of one internal_power group
