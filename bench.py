@@ -6,19 +6,22 @@ primary-input toggle p = 0.15, events timed by the NLDM delays (glitches
 included).  One step = one full evaluation of per-gate and per-cycle power of
 the whole trace (gtp_run) with the event store resident in HBM.
 
-Surrogate leg (on by default): an MLP over 4096 per-cycle toggle bits
-(4096 -> 256 -> 256 -> 1, GELU, BF16 matrix-core kernels, FP32 accumulation)
-is trained on the exact per-cycle power of the first 50k cycles of that trace,
-then run on a fresh 100k-cycle trace of the same netlist (new seed), with the
-exact engine recomputing 1% of its cycles as the judge (fallback to the exact
-answer above 2% NRMSE).  Its throughput, error and speedup over the exact path
-are reported in the "surrogate" object.
+Surrogate leg (on by default): a linear toggle-energy base term (per-net
+weights from the exact run's per-gate power) plus an MLP over 4096 per-cycle
+toggle bits (4096 -> 256 -> 256 -> 1, GELU, BF16 matrix-core kernels, FP32
+accumulation) for what the base leaves, trained on the exact per-cycle power
+of the first 50k cycles of that trace, then run on ten fresh 100k-cycle traces
+of the same netlist (new seeds: 1M cycles it never saw), the exact engine
+recomputing 0.5% of each window's cycles as the judge (fallback to the exact
+answer above 2% NRMSE or above 0.9x the judged cycles' own spread).  Its
+throughput, pooled error and speedup over the exact path are reported in the
+"surrogate" object.
 
 Prints one JSON line (stdout); progress goes to stderr.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload exact_200k]
                     [--gates G] [--cycles C] [--cpu-sample-cycles S] [--no-cpu]
-                    [--no-surrogate] [--surrogate-cycles C]
+                    [--no-surrogate] [--surrogate-cycles C] [--surrogate-windows K]
     python bench.py --copy-calibration MiB   # plain device copies, for counter scaling
 """
 import argparse
@@ -217,64 +220,98 @@ def vcd_leg(design, w, n_cycle, steps, cpu):
     return res
 
 
-def surrogate_leg(eng, design, exact_cycle_power, w, n_fresh, steps, exact_cps, exact_device_cps, judge_tile=64):
-    """Train on the first 50k exact cycles, infer on a fresh trace, judge 1%."""
+def surrogate_leg(eng, design, exact_result, w, n_fresh, n_windows, exact_wall_cps, exact_device_cps, judge_tile=64):
+    """Train on the first 50k exact cycles of the loaded trace, then evaluate
+    n_windows fresh traces of n_fresh cycles each (new seeds: cycles the
+    network never saw), each judged on 0.5% of its cycles; the full exact answer
+    of every window is computed as well, untimed, for the pooled NRMSE."""
     import numpy as np
     from gtpower_amd import synthetic as S
     from gtpower_amd.surrogate import MlpSurrogate, SurrogatePower, cycle_total, nrmse
 
-    n_train = min(50000, exact_cycle_power.shape[1])
-    mask = np.zeros(exact_cycle_power.shape[1], bool)
+    n_train = min(50000, exact_result.cycle_power.shape[1])
+    mask = np.zeros(exact_result.cycle_power.shape[1], bool)
     mask[:n_train] = True
-
-    class _Exact:
-        cycle_power = exact_cycle_power
-
-    sp = SurrogatePower(MlpSurrogate(4096, 256, seed=3), threshold=0.02, judge_fraction=0.01,
+    # the judge recomputes 0.5% of the cycles (8 tiles of 64 cycles per
+    # 100k-cycle window, plus tile 0): ~500 cycles pin the NRMSE to about 3%
+    # of itself, well inside the 10% margin of the relative bar
+    sp = SurrogatePower(MlpSurrogate(4096, 256, seed=3), threshold=0.02, judge_fraction=0.005,
                         judge_tile=judge_tile)
     t0 = time.perf_counter()
-    sp.fit(eng, train_mask=mask, exact=_Exact)
+    sp.fit(eng, train_mask=mask, exact=exact_result)
     train_s = time.perf_counter() - t0
     log(f"surrogate trained on {n_train} exact cycles in {train_s:.1f} s "
         f"(best validation loss {float(np.nanmin(sp.train_loss[:, 1])):.4f} at epoch "
-        f"{int(np.nanargmin(sp.train_loss[:, 1]))}, {int((sp.feats >= design.n_net).sum())} rise features)")
+        f"{int(np.nanargmin(sp.train_loss[:, 1]))}; base term over {len(sp.base_nets)} nets, "
+        f"{int((sp.feats >= design.n_net).sum())} rise features)")
     eng.trace = None
     tk = dict(w["trace"])
     tk["n_cycle"] = n_fresh
     tseed = S.MASTER_SEED ^ (w["seed"] * 0x9E3779B1)
-    t0 = time.perf_counter()
-    fresh = design.trace(seed=tseed + 1001, **tk)
-    log(f"fresh trace: {fresh.n_cycle} cycles, {fresh.n_event} events in {time.perf_counter() - t0:.1f} s")
-    eng.read_power_activities(fresh)
-    fresh.close()
-    outs = [sp.evaluate(eng) for _ in range(max(1, steps))]
-    out = outs[-1]
-    surr_ms = sum(o.surrogate_ms for o in outs) / len(outs)
-    judge_ms = sum(o.judge_ms for o in outs) / len(outs)
-    exact = eng.power(want=("cycle",))
-    y = cycle_total(exact.cycle_power)
-    pred, _ = sp.mlp.predict_trace(eng, sp.feats)
-    full = nrmse(pred, y)
-    r2 = 1.0 - float(np.mean((pred - y) ** 2) / y.var())
-    cps = y.size / ((surr_ms + judge_ms) * 1e-3)
-    flop = 2.0 * (4096 * 256 + 256 * 256 + 256) * y.size
-    log(f"surrogate: {surr_ms:.3f} ms + judge {judge_ms:.3f} ms for {y.size} cycles -> {cps:.4g} cycles/s; "
-        f"judged nrmse {out.nrmse:.4f}, full nrmse {full:.4f}, r2 {r2:.3f}, source {out.source}")
+    se = sy = sy2 = 0.0
+    n_tot = 0
+    wall = surr = judge = exact_ms = 0.0
+    windows = []
+    for k in range(n_windows):
+        t0 = time.perf_counter()
+        fresh = design.trace(seed=tseed + 1001 + k, **tk)
+        gen_s = time.perf_counter() - t0
+        eng.read_power_activities(fresh)
+        fresh.close()
+        if k == 0:
+            sp.evaluate(eng)  # warm-up (first-call allocations), untimed
+        out = sp.evaluate(eng)
+        ex = eng.power(want=("cycle",))
+        y = cycle_total(ex.cycle_power)
+        err = out.cycle_total - y
+        se += float(np.sum(err * err))
+        sy += float(y.sum())
+        sy2 += float(np.sum(y * y))
+        n_tot += y.size
+        wall += out.wall_ms
+        surr += out.surrogate_ms
+        judge += out.judge_ms
+        exact_ms += ex.kernel_ms
+        windows.append({"nrmse_full": nrmse(out.cycle_total, y), "constant_predictor_nrmse": float(y.std() / y.mean()),
+                        "nrmse_judged": out.nrmse, "judged_spread": out.spread, "source": out.source,
+                        "wall_ms": out.wall_ms})
+        log(f"surrogate window {k}: {y.size} fresh cycles (generated in {gen_s:.1f} s): {out.wall_ms:.2f} ms wall "
+            f"(device: surrogate {out.surrogate_ms:.3f} ms, judge {out.judge_ms:.3f} ms, overlapped), judged nrmse "
+            f"{out.nrmse:.4f} vs spread {out.spread:.4f}, full nrmse {windows[-1]['nrmse_full']:.4f} vs constant "
+            f"{windows[-1]['constant_predictor_nrmse']:.4f}, source {out.source}")
+    mean = sy / n_tot
+    full = float(np.sqrt(se / n_tot) / mean)
+    const = float(np.sqrt(max(sy2 / n_tot - mean * mean, 0.0)) / mean)
+    cps = n_tot / (wall * 1e-3)
+    dev_cps = n_tot / (max(surr, judge) * 1e-3)
+    flop = 2.0 * (4096 * 256 + 256 * 256 + 256) * n_tot
+    log(f"surrogate: {n_tot} fresh cycles in {n_windows} windows, {cps:.4g} cycles/s wall ({cps / exact_wall_cps:.1f}x "
+        f"the exact path's wall-clock rate), pooled nrmse {full:.4f} vs constant predictor {const:.4f} "
+        f"({full / const:.3f}x)")
     sp.mlp.close()
     return {
-        "cycles_per_s": cps, "speedup_vs_exact": cps / exact_cps,
-        # both in device time (the surrogate's figure is device time; the exact
-        # path's value is wall-clock per step)
-        "speedup_vs_exact_device_time": cps / exact_device_cps, "source": out.source,
-        "nrmse_judged": out.nrmse, "nrmse_full": full, "r2": r2,
-        "constant_predictor_nrmse": float(y.std() / y.mean()),
-        "nrmse_range": float(np.sqrt(np.mean((pred - y) ** 2)) / (y.max() - y.min())),
-        "judged_cycles": out.judged_cycles, "judge_tile": sp.judge_tile, "train_cycles": n_train, "infer_cycles": int(y.size),
-        "surrogate_ms": surr_ms, "judge_ms": judge_ms, "exact_ms_fresh": exact.kernel_ms,
+        "cycles_per_s": cps,
+        # wall-clock against wall-clock: evaluate() (base term + network on
+        # one stream, the exact judge on the engine's streams, host work
+        # included) vs the exact path's timed steps
+        "speedup_vs_exact": cps / exact_wall_cps,
+        "speedup_vs_exact_device_time": dev_cps / exact_device_cps,
+        # the exact engine's first (cold) evaluation of each same fresh window,
+        # device time: what evaluating these windows exactly would have cost
+        "speedup_vs_exact_same_windows": exact_ms / wall,
+        "fresh_cycles": n_tot, "windows": n_windows, "window_cycles": n_fresh,
+        "nrmse_full": full, "constant_predictor_nrmse": const, "nrmse_vs_constant": full / const,
+        "sources": sorted(set(x["source"] for x in windows)),
+        "per_window": windows,
+        "judge_fraction": sp.judge_fraction, "judge_tile": sp.judge_tile, "rel_gain_bar": sp.rel_gain,
+        "train_cycles": n_train, "base_nets": int(len(sp.base_nets)),
+        "surrogate_ms_per_window": surr / n_windows, "judge_ms_per_window": judge / n_windows,
+        "wall_ms_per_window": wall / n_windows, "exact_kernel_ms_per_window": exact_ms / n_windows,
         "train_s": round(train_s, 2), "features": 4096, "hidden": [256, 256], "dtype": "bf16 (fp32 accumulate)",
-        "mfma": {"achieved": flop / (surr_ms * 1e-3) / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
-                 "frac": flop / (surr_ms * 1e-3) / 1e12 / 2500.0,
-                 "note": "features + inference window; FLOP = 2 (4096*256 + 256*256 + 256) per cycle"},
+        "mfma": {"achieved": flop / (surr * 1e-3) / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
+                 "frac": flop / (surr * 1e-3) / 1e12 / 2500.0,
+                 "note": "network FLOP over the surrogate's device time (base term, features and inference); "
+                         "FLOP = 2 (4096*256 + 256*256 + 256) per cycle"},
     }
 
 
@@ -299,6 +336,8 @@ def main():
     ap.add_argument("--batch-max-cycles", type=int, default=2000)
     ap.add_argument("--surrogate-cycles", type=int, default=100000)
     ap.add_argument("--judge-tile", type=int, default=64, help="cycles per tile the exact judge recomputes")
+    ap.add_argument("--surrogate-windows", type=int, default=10,
+                    help="fresh traces the surrogate evaluates (of --surrogate-cycles cycles each)")
     ap.add_argument("--vcd-cycles", type=int, default=2000, help="cycles of the VCD ingest leg (0: off)")
     ap.add_argument("--sliced-parts", type=int, default=4, help="event budget 1/parts for the sliced leg (<2: off)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
@@ -387,9 +426,10 @@ def main():
 
     surrogate = None
     if not args.no_surrogate:
-        surrogate = surrogate_leg(eng, design, r.cycle_power, w,
+        surrogate = surrogate_leg(eng, design, r, w,
                                   min(args.surrogate_cycles, args.cycles or args.surrogate_cycles),
-                                  3, value, n_cycle / (avg_kernel_ms * 1e-3), judge_tile=args.judge_tile)
+                                  args.surrogate_windows, value, n_cycle / (avg_kernel_ms * 1e-3),
+                                  judge_tile=args.judge_tile)
     # after the surrogate leg: the batch replaces the resident trace
     batch = None
     if args.batch_traces > 0:

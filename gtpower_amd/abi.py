@@ -177,6 +177,9 @@ def load_engine():
     lib.gtp_mlp_predict_trace.argtypes = [C.c_void_p, C.c_void_p, p_i32, C.c_int64, C.c_int64, p_f32, p_f64]
     lib.gtp_feature_correlation.argtypes = [C.c_void_p, p_f64, p_u8, p_f32]
     lib.gtp_toggle_features.argtypes = [C.c_void_p, C.c_int32, p_i32, C.c_int64, C.c_int64, p_u32]
+    lib.gtp_net_event_counts.argtypes = [C.c_void_p, p_i64]
+    lib.gtp_net_weight_sum.argtypes = [C.c_void_p, C.c_int32, p_i32, p_f64, C.c_int64, C.c_int64, p_f64,
+                                       C.POINTER(C.c_double)]
     lib.gtp_mlp_gemm_test.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                       C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, p_f32]
     lib.gtp_vcd_create.argtypes = [C.c_int32, C.POINTER(C.c_void_p)]
@@ -192,7 +195,8 @@ def load_engine():
                  "gtp_get_schedule", "gtp_reference_cycles_per_thread", "gtp_write_cell_power",
                  "gtp_write_per_cycle", "gtp_run_tiles", "gtp_autotune", "gtp_mlp_create", "gtp_mlp_get_params",
                  "gtp_mlp_set_params", "gtp_mlp_gradients", "gtp_mlp_train", "gtp_mlp_predict",
-                 "gtp_mlp_predict_trace", "gtp_feature_correlation", "gtp_toggle_features",
+                 "gtp_mlp_predict_trace", "gtp_feature_correlation", "gtp_toggle_features", "gtp_net_weight_sum",
+                 "gtp_net_event_counts",
                  "gtp_mlp_gemm_test", "gtp_vcd_create", "gtp_vcd_parse", "gtp_vcd_var", "gtp_vcd_events",
                  "gtp_set_trace_vcd"):
         getattr(lib, name).restype = C.c_int32

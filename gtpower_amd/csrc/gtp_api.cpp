@@ -77,6 +77,11 @@ struct gtp_ctx {
   hipStream_t stream = nullptr;
   static constexpr int N_SIDE = 3;
   hipStream_t side[N_SIDE] = {nullptr, nullptr, nullptr};  // concurrent per-cell launches
+  // feature work (weighted toggle sums) beside the power runs; created on
+  // first use, after the run streams, so that it does not take one of the
+  // hardware queues the run streams would otherwise get to themselves
+  hipStream_t aux = nullptr;
+  gtp::AuxBuffers aux_buf;
   hipEvent_t fork = nullptr, join[N_SIDE] = {nullptr, nullptr, nullptr};
   std::string err;
   int threads = 0;
@@ -117,6 +122,8 @@ struct gtp_ctx {
   int64_t E = 0, T = 0, C = 0, t_max = 0, P_ev = 0;
   double scale = 0, period = 0;
   std::vector<int64_t> gate_work;
+  int64_t gate_work_max = 0;
+  std::vector<int32_t> order_tmp[2];  // build_order's scratch, kept between rebuilds
   DArr<int64_t> d_net_off;
   DArr<uint32_t> d_ev;
   DArr<int64_t> d_chunk_off;     // annotation blocks per net (prefix)
@@ -398,6 +405,10 @@ GTP_API void gtp_destroy(gtp_ctx* x) {
   if (x->stream) {
     (void)hipStreamSynchronize(x->stream);
     (void)hipStreamDestroy(x->stream);
+  }
+  if (x->aux) {
+    (void)hipStreamSynchronize(x->aux);
+    (void)hipStreamDestroy(x->aux);
   }
   for (int k = 0; k < gtp_ctx::N_SIDE; ++k) {
     if (x->side[k]) {
@@ -727,6 +738,7 @@ gtp_status finish_trace(gtp_ctx* x) {
   }
   // per-gate work (pin events) for the schedule
   x->gate_work.assign(x->G, 0);
+  x->gate_work_max = 0;
   int64_t pev = 0;
   for (int64_t g = 0; g < x->G; ++g) {
     int64_t w = 0;
@@ -735,6 +747,7 @@ gtp_status finish_trace(gtp_ctx* x) {
       if (net >= 0) w += off[net + 1] - off[net];
     }
     x->gate_work[g] = w;
+    x->gate_work_max = std::max(x->gate_work_max, w);
     pev += w;
   }
   x->P_ev = pev;
@@ -812,42 +825,75 @@ gtp_status ensure_arc_q(gtp_ctx* x, double time_scale) {
 // without `when` conditions, one generic group), each group sorted by work (pin
 // events), heaviest first, and padded with -1 to whole ranges.
 void build_order(gtp_ctx* x, gtp_ctx::OrderSet& os, int32_t range_gates, const std::vector<int64_t>& work,
-                 int64_t T) {
+                 int64_t T, bool coarse = false) {
   const int nc = (int)x->cells.size();
-  std::vector<std::vector<int32_t>> by_cell(nc);
-  // the generic kernel's gates by pin-count class: <= 4, <= 6, <= 8, <= 16 pins
-  std::vector<int32_t> generic[4];
-  for (int64_t g = 0; g < x->G; ++g) {
-    const HostCell& hc = x->cells[x->gate_cell[g]];
+  const int nl = nc + 4;
+  // per cell: its own list (specialised kernel) or a generic pin-count class
+  // (<= 4, <= 6, <= 8, <= 16 pins): list nc + class
+  std::vector<int32_t> list_of(nc);
+  for (int c = 0; c < nc; ++c) {
+    const HostCell& hc = x->cells[c];
     // the specialised kernel compares arrivals on 32 bits: cycles below 2^15 ticks
     const bool fast = hc.sbits == 0 && hc.n_pin - hc.n_in == 1 && hc.n_pin >= 2 && hc.n_pin <= 6 &&
                       T < 32768;
-    if (fast) by_cell[x->gate_cell[g]].push_back((int32_t)g);
-    else
-      generic[hc.n_pin <= MAXP_SMALL ? 0 : hc.n_pin <= MAXP_LARGE ? 1 : hc.n_pin <= MAXP_WIDE ? 2 : 3].push_back(
-          (int32_t)g);
+    list_of[c] = fast ? c
+                      : nc + (hc.n_pin <= MAXP_SMALL ? 0 : hc.n_pin <= MAXP_LARGE ? 1 : hc.n_pin <= MAXP_WIDE ? 2 : 3);
+  }
+  // the lists side by side in one array, gate order within each (the
+  // context keeps the arrays, so a rebuild touches no new memory)
+  std::vector<int64_t> loff(nl + 1, 0);
+  for (int64_t g = 0; g < x->G; ++g) ++loff[list_of[x->gate_cell[g]] + 1];
+  for (int l = 0; l < nl; ++l) loff[l + 1] += loff[l];
+  std::vector<int32_t>& flat = x->order_tmp[0];
+  std::vector<int32_t>& sorted = x->order_tmp[1];
+  flat.resize(x->G);
+  sorted.resize(x->G);
+  {
+    std::vector<int64_t> pos(loff.begin(), loff.end() - 1);
+    for (int64_t g = 0; g < x->G; ++g) flat[pos[list_of[x->gate_cell[g]]]++] = (int32_t)g;
   }
   auto by_work = [&](int32_t p, int32_t q) {
     if (work[p] != work[q]) return work[p] > work[q];
     return p < q;
   };
+  // coarse: gates bucketed by work in quarter octaves (a counting sort, gate
+  // order kept within a bucket) -- the order of runs over a few listed tiles,
+  // rebuilt for each new trace, where a full sort would cost more host time
+  // than the run itself
+  auto bucket = [&](int32_t g) {
+    if (work[g] <= 0) return 0;
+    const uint64_t v = (uint64_t)work[g] + 1;
+    const int e = 63 - __builtin_clzll(v);
+    const int frac = (int)(e >= 2 ? (v >> (e - 2)) & 3u : (v << (2 - e)) & 3u);
+    return std::min(255, 1 + 4 * e + frac);
+  };
   os.order.clear();
   os.launches.clear();
   os.range_gates = range_gates;
-  auto add = [&](std::vector<int32_t>& v, int cell, int np) {
-    if (v.empty()) return;
-    std::sort(v.begin(), v.end(), by_work);
-    const int32_t nr = (int32_t)((v.size() + range_gates - 1) / range_gates);
+  auto add = [&](int l, int cell, int np) {
+    const int64_t b = loff[l], e = loff[l + 1];
+    if (e == b) return;
+    int32_t* v = sorted.data() + b;
+    if (coarse) {
+      int64_t cnt[257] = {0};
+      for (int64_t i = b; i < e; ++i) ++cnt[256 - bucket(flat[i])];   // larger work first
+      for (int k = 1; k <= 256; ++k) cnt[k] += cnt[k - 1];
+      for (int64_t i = b; i < e; ++i) v[cnt[256 - bucket(flat[i]) - 1]++] = flat[i];
+    } else {
+      std::copy(flat.begin() + b, flat.begin() + e, v);
+      std::sort(v, v + (e - b), by_work);
+    }
+    const int32_t nr = (int32_t)((e - b + range_gates - 1) / range_gates);
     const int32_t rb = (int32_t)(os.order.size() / range_gates);
-    os.order.insert(os.order.end(), v.begin(), v.end());
+    os.order.insert(os.order.end(), v, v + (e - b));
     os.order.resize((size_t)(rb + nr) * range_gates, -1);
     os.launches.push_back(gtp_ctx::Launch{cell, np, rb, nr});
   };
-  for (int c = 0; c < nc; ++c) add(by_cell[c], c, x->cells[c].n_pin);
-  add(generic[0], -1, MAXP_SMALL);
-  add(generic[1], -1, MAXP_LARGE);
-  add(generic[2], -1, MAXP_WIDE);
-  add(generic[3], -1, MAXP_HUGE);
+  for (int c = 0; c < nc; ++c) add(c, c, x->cells[c].n_pin);
+  add(nc + 0, -1, MAXP_SMALL);
+  add(nc + 1, -1, MAXP_LARGE);
+  add(nc + 2, -1, MAXP_WIDE);
+  add(nc + 3, -1, MAXP_HUGE);
   os.n_range = (int32_t)(os.order.size() / range_gates);
 }
 
@@ -882,7 +928,8 @@ struct Quanta {
 
 gtp_status make_quanta(gtp_ctx* x, int64_t t_max, const std::vector<int64_t>& work, Quanta* q) {
   // a gate's per-trace energy sum: at most its pin events x the largest term
-  const int64_t wmax = work.empty() ? 0 : *std::max_element(work.begin(), work.end());
+  const int64_t wmax = &work == &x->gate_work ? x->gate_work_max
+                                              : (work.empty() ? 0 : *std::max_element(work.begin(), work.end()));
   if (wmax >= ((int64_t)1 << (62 - ENERGY_TERM_BITS)))
     return x->fail(GTP_ERR_LIMIT, "gtp_run: a gate has 2^26 or more pin events in one trace (int64 energy sums)");
   double emax = std::max((double)x->energy_max * 1.000001, x->csw_max);
@@ -986,7 +1033,7 @@ gtp_status prepare_run(gtp_ctx* x, int64_t tc_req, int32_t n_work_tiles, RunPlan
   bool rebuilt = false;
   if (os.order.empty() || os.range_gates != RG || os.order_T != x->T) {
     os.order_T = x->T;
-    build_order(x, os, RG, x->gate_work, x->T);
+    build_order(x, os, RG, x->gate_work, x->T, subset);
     HIPCHK(x, os.d_order.upload(os.order.data(), os.order.size(), s));
     rebuilt = true;
   }
@@ -1253,6 +1300,13 @@ gtp_status run_once(gtp_ctx* x, gtp_result* res, int64_t tc_req, int32_t rg_req)
 }
 }  // namespace
 
+GTP_API gtp_status gtp_net_event_counts(gtp_ctx* x, int64_t* counts) {
+  if (!x || !counts) return GTP_ERR_ARG;
+  if (!x->have_trace) return x->fail(GTP_ERR_STATE, "gtp_net_event_counts: call gtp_set_trace first");
+  for (int64_t n = 0; n < x->N; ++n) counts[n] = x->h_net_off[n + 1] - x->h_net_off[n];
+  return GTP_OK;
+}
+
 GTP_API gtp_status gtp_run_tiles(gtp_ctx* x, int32_t tile_cycles, int32_t n_list, const int32_t* tiles,
                                  double* cycle_power, double* kernel_ms) {
   if (!x) return GTP_ERR_ARG;
@@ -1297,11 +1351,20 @@ GTP_API gtp_status gtp_run_tiles(gtp_ctx* x, int32_t tile_cycles, int32_t n_list
   return GTP_OK;
 }
 
+gtp::AuxBuffers* gtp::aux_buffers(gtp_ctx* x) { return x ? &x->aux_buf : nullptr; }
+
+hipStream_t gtp::aux_stream(gtp_ctx* x) {
+  if (x && !x->aux && (hipSetDevice(x->device) != hipSuccess || hipStreamCreate(&x->aux) != hipSuccess))
+    x->aux = nullptr;
+  return x ? x->aux : nullptr;
+}
+
 int32_t gtp::trace_dev(gtp_ctx* x, TraceDev* out) {
   if (!x || !out) return GTP_ERR_ARG;
   if (!x->have_trace) return x->fail(GTP_ERR_STATE, "call gtp_set_trace first");
   out->device = x->device;
   out->stream = x->stream;
+  out->aux_stream = x->aux;
   out->n_net = x->N;
   out->n_cycle = x->C;
   out->T = x->T;

@@ -51,6 +51,12 @@ struct DArr {
     n = count;
     return hipSuccess;
   }
+  // at least count elements, keeping a larger buffer (no free, so no
+  // implicit device synchronisation, when the size does not grow)
+  hipError_t reserve(size_t count) {
+    if (p && count <= n) return hipSuccess;
+    return alloc(count);
+  }
   hipError_t upload(const T* src, size_t count, hipStream_t s) {
     hipError_t e = alloc(count);
     if (e != hipSuccess || count == 0) return e;
@@ -302,6 +308,7 @@ inline void cycle_magic(int64_t T, uint32_t* m, int32_t* sh) {
 struct TraceDev {
   int device;
   hipStream_t stream;
+  hipStream_t aux_stream;        // the context's stream for feature work (nullptr until first used)
   int64_t n_net, n_cycle, T, t_max;
   const int64_t* net_off;        // device [N+1]
   const uint32_t* ev;            // device [E]
@@ -318,6 +325,14 @@ namespace gtp {
 // Fills *out from a context with a loaded trace; 0 on success, else a
 // negative status (the context's last error says why).
 int32_t trace_dev(gtp_ctx* ctx, TraceDev* out);
+// Device buffers of the context's weighted toggle sums, kept between calls.
+struct AuxBuffers {
+  DArr<int32_t> sel, wq, idx;
+  DArr<unsigned long long> out;
+};
+AuxBuffers* aux_buffers(gtp_ctx* ctx);
+// The context's auxiliary stream, created on first use (nullptr on failure).
+hipStream_t aux_stream(gtp_ctx* ctx);
 // A parsed VCD's per-(id, bit) event store on its device (gtp_vcd.hip), for
 // gtp_set_trace_vcd: words as above, padded by EV_PAD words.
 struct VcdView {

@@ -417,23 +417,97 @@ __global__ __launch_bounds__(256) void feature_bits_kernel(const int64_t* __rest
     for (int j = 0; j < 32; ++j) s_off[j + 1] += s_off[j];
   }
   __syncthreads();
-  const int total = s_off[32];
   const uint32_t rise_mask = s_rise;
   const int64_t cbase = c0 + cs;
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    int lo = 0, hi = 32;  // last j with s_off[j] <= e
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (s_off[mid] <= e) lo = mid; else hi = mid;
+  // one feature per wave at a time, its events in the window read by the 64
+  // lanes together, four loads in flight per lane
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, n_wave = blockDim.x >> 6;
+  for (int j = wave; j < 32; j += n_wave) {
+    const uint32_t* e = ev + s_beg[j];
+    const int cnt = s_off[j + 1] - s_off[j];
+    const bool rise = (rise_mask >> j) & 1u;
+    for (int i = lane; i < cnt; i += 4 * 64) {
+      uint32_t wd[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) wd[u] = i + u * 64 < cnt ? e[i + u * 64] : 0u;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (i + u * 64 >= cnt) break;
+        if (rise && (wd[u] & 3u) != 1u) continue;
+        const uint32_t t = wd[u] >> EV_SHIFT;
+        const int64_t c = (int64_t)(((uint64_t)cyc_m * (uint64_t)t) >> cyc_sh) - cbase;
+        atomicOr(&bits[c], 1u << j);
+      }
     }
-    const uint32_t wd = ev[s_beg[lo] + (e - s_off[lo])];
-    if (((rise_mask >> lo) & 1u) && (wd & 3u) != 1u) continue;
-    const uint32_t t = wd >> EV_SHIFT;
-    const int64_t c = (int64_t)(((uint64_t)cyc_m * (uint64_t)t) >> cyc_sh) - cbase;
-    atomicOr(&bits[c], 1u << lo);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < nr; i += blockDim.x) X[(cs + i) * words + w] = bits[i];
+}
+
+constexpr int WS_CYCLES = 2048;  // cycles per weighted-sum window
+constexpr int WS_NETS = 256;     // selected nets per weighted-sum block
+
+// First event index (>= 1) of each selected net at or after the start of
+// each window: idx[k * (n_win + 1) + j] for window j of net sel[k], j = n_win
+// the end of the range; one thread per (net, window), so the searches run in
+// parallel rather than one after another in the summing waves.
+__global__ void net_window_start_kernel(const int64_t* __restrict__ net_off, const uint32_t* __restrict__ ev,
+                                        const int32_t* __restrict__ sel, int32_t n_sel, int32_t n_win, int64_t T,
+                                        int64_t c0, int64_t n_cyc, int32_t* __restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n_sel * (n_win + 1)) return;
+  const int64_t k = i / (n_win + 1);
+  const int j = (int)(i - k * (n_win + 1));
+  const int32_t n = sel[k];
+  const int64_t b = net_off[n], cnt = net_off[n + 1] - b;
+  const int64_t t = (c0 + min((int64_t)j * WS_CYCLES, n_cyc)) * T;
+  const uint32_t key = t >= (int64_t)EV_TIME_INF ? EV_INF : (uint32_t)(t << EV_SHIFT);
+  idx[i] = (int32_t)max((int64_t)1, lower_bound_ev(ev + b, cnt, key));
+}
+
+// Weighted toggle sums: out[c] += sum over the selected nets k of wq[k] x
+// (value changes of sel[k] in cycle c0 + c; event index >= 1).  Block (g, j):
+// nets [g * WS_NETS, +WS_NETS) over window j, one net per wave at a time (its
+// events in the window, from the window index, read by the 64 lanes
+// together), int64 LDS bins, one global atomic per nonzero bin.
+__global__ __launch_bounds__(256) void net_weight_sum_kernel(const int64_t* __restrict__ net_off,
+                                                             const uint32_t* __restrict__ ev,
+                                                             const int32_t* __restrict__ sel,
+                                                             const int32_t* __restrict__ wq,
+                                                             const int32_t* __restrict__ idx, int32_t n_sel,
+                                                             int32_t n_win, uint32_t cyc_m, int cyc_sh, int64_t c0,
+                                                             int64_t n_cyc, unsigned long long* __restrict__ out) {
+  __shared__ long long bins[WS_CYCLES];
+  const int j = blockIdx.y;
+  const int64_t cs = (int64_t)j * WS_CYCLES;
+  const int nr = (int)min((int64_t)WS_CYCLES, n_cyc - cs);
+  for (int i = threadIdx.x; i < nr; i += blockDim.x) bins[i] = 0;
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, n_wave = blockDim.x >> 6;
+  const int64_t cbase = c0 + cs;
+  const int k_end = min(n_sel, (int)(blockIdx.x + 1) * WS_NETS);
+  for (int k = (int)blockIdx.x * WS_NETS + wave; k < k_end; k += n_wave) {
+    const long long w = wq[k];
+    const uint32_t* e = ev + net_off[sel[k]];
+    const int32_t i0 = idx[(int64_t)k * (n_win + 1) + j], i1 = idx[(int64_t)k * (n_win + 1) + j + 1];
+    // four loads in flight per lane before their events are binned (the
+    // loop is bound by load latency, not by the LDS atomics)
+    for (int32_t i = i0 + lane; i < i1; i += 4 * 64) {
+      uint32_t wd[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) wd[u] = i + u * 64 < i1 ? e[i + u * 64] : 0u;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (i + u * 64 >= i1) break;
+        const uint32_t t = wd[u] >> EV_SHIFT;
+        const int64_t c = (int64_t)(((uint64_t)cyc_m * (uint64_t)t) >> cyc_sh) - cbase;
+        atomicAdd((unsigned long long*)&bins[c], (unsigned long long)w);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nr; i += blockDim.x)
+    if (bins[i] != 0) atomicAdd(&out[cs + i], (unsigned long long)bins[i]);
 }
 
 // Per net over the cycles with mask[c] != 0 (all when mask is null): the
@@ -565,6 +639,12 @@ struct gtp_mlp {
   float *yb = nullptr, *z1 = nullptr, *z2 = nullptr, *out = nullptr, *dout = nullptr, *lossp = nullptr;
   bf16 *h1 = nullptr, *h2 = nullptr, *dz1 = nullptr, *dz2 = nullptr;
   int32_t* idx = nullptr;
+  // predictions over a loaded trace: kept between calls (no free, so no
+  // implicit device synchronisation while another thread's runs are queued)
+  uint32_t* tr_X = nullptr;
+  float* tr_out = nullptr;
+  int32_t* tr_net = nullptr;
+  int64_t tr_X_n = 0, tr_out_n = 0;
 
   float* W1() { return P; }
   float* b1() { return P + (int64_t)H * K; }
@@ -761,7 +841,8 @@ GTP_API void gtp_mlp_destroy(gtp_mlp* m) {
   (void)hipSetDevice(m->device);
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   m->release_batch();
-  for (void* p : {(void*)m->P, (void*)m->G, (void*)m->M1, (void*)m->M2, (void*)m->W1b, (void*)m->W2b})
+  for (void* p : {(void*)m->P, (void*)m->G, (void*)m->M1, (void*)m->M2, (void*)m->W1b, (void*)m->W2b,
+                  (void*)m->tr_X, (void*)m->tr_out, (void*)m->tr_net})
     if (p) (void)hipFree(p);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
@@ -999,13 +1080,23 @@ namespace {
 // of each feature's events.
 struct FeatureSel {
   int32_t* d_net = nullptr;
+  bool own = false;
   ~FeatureSel() {
-    if (d_net) (void)hipFree(d_net);
+    if (own && d_net) (void)hipFree(d_net);
   }
   hipError_t build(const TraceDev& t, int K, const int32_t* nets, hipStream_t s) {
     (void)t;
     hipError_t e = hipMalloc((void**)&d_net, (size_t)K * 4);
+    own = e == hipSuccess;
     if (e == hipSuccess) e = hipMemcpyAsync(d_net, nets, (size_t)K * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
+  }
+  // the same into a caller-kept device buffer of K entries
+  hipError_t use(int32_t* buf, int K, const int32_t* nets, hipStream_t s) {
+    d_net = buf;
+    own = false;
+    hipError_t e = hipMemcpyAsync(d_net, nets, (size_t)K * 4, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     return e;
   }
@@ -1114,6 +1205,82 @@ GTP_API gtp_status gtp_toggle_features(gtp_ctx* ctx, int32_t n_sel, const int32_
   return e == hipSuccess ? GTP_OK : (e == hipErrorOutOfMemory ? GTP_ERR_OOM : GTP_ERR_DEVICE);
 }
 
+// Weighted toggle sums of the selected nets over cycles [cycle_begin,
+// +n_cycle) of the loaded trace, on the context's auxiliary stream.  The
+// weights are held as int32 multiples of a power-of-two quantum (the largest
+// |weight| is 2^20 quanta), summed exactly in int64.
+GTP_API gtp_status gtp_net_weight_sum(gtp_ctx* ctx, int32_t n_sel, const int32_t* nets, const double* weights,
+                                      int64_t cycle_begin, int64_t n_cycle, double* out, double* ms) {
+  TraceDev t;
+  gtp_status st = gtp::trace_dev(ctx, &t);
+  if (st != GTP_OK) return st;
+  if (n_sel <= 0 || !nets || !weights || !out || cycle_begin < 0 || n_cycle <= 0 ||
+      cycle_begin + n_cycle > t.n_cycle)
+    return GTP_ERR_ARG;
+  double wmax = 0;
+  for (int32_t k = 0; k < n_sel; ++k) {
+    if (nets[k] < 0 || nets[k] >= t.n_net || !std::isfinite(weights[k])) return GTP_ERR_ARG;
+    // the window index holds event positions within a net in 32 bits
+    if (t.host_net_off[nets[k] + 1] - t.host_net_off[nets[k]] >= INT32_MAX) return GTP_ERR_LIMIT;
+    wmax = std::max(wmax, std::fabs(weights[k]));
+  }
+  if ((n_cycle + WS_CYCLES - 1) / WS_CYCLES > 65535) return GTP_ERR_LIMIT;  // windows on the grid's y axis
+  if (!(wmax > 0)) {
+    for (int64_t c = 0; c < n_cycle; ++c) out[c] = 0.0;
+    if (ms) *ms = 0.0;
+    return GTP_OK;
+  }
+  const int kq = (int)std::floor(20.0 - std::log2(wmax * 1.000001));
+  std::vector<int32_t> wq(n_sel);
+  for (int32_t k = 0; k < n_sel; ++k) wq[k] = (int32_t)std::llrint(std::ldexp(weights[k], kq));
+  if (hipSetDevice(t.device) != hipSuccess) return GTP_ERR_DEVICE;
+  hipStream_t s = gtp::aux_stream(ctx);
+  if (!s) return GTP_ERR_DEVICE;
+  const int32_t n_win = (int32_t)((n_cycle + WS_CYCLES - 1) / WS_CYCLES);
+  gtp::AuxBuffers& ab = *gtp::aux_buffers(ctx);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = ab.sel.reserve(n_sel);
+  if (e == hipSuccess) e = ab.wq.reserve(n_sel);
+  if (e == hipSuccess) e = ab.idx.reserve((size_t)n_sel * (n_win + 1));
+  if (e == hipSuccess) e = ab.out.reserve(n_cycle);
+  int32_t *d_sel = ab.sel.p, *d_wq = ab.wq.p, *d_idx = ab.idx.p;
+  unsigned long long* d_out = ab.out.p;
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_sel, nets, (size_t)n_sel * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_wq, wq.data(), (size_t)n_sel * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipEventRecord(e0, s);
+  if (e == hipSuccess) {
+    fill_u32_kernel<<<dim3((unsigned)((2 * n_cycle + 255) / 256)), 256, 0, s>>>((uint32_t*)d_out, 2 * n_cycle, 0u);
+    uint32_t m = 0;
+    int32_t sh = 0;
+    cycle_magic(t.T, &m, &sh);
+    const int64_t n_idx = (int64_t)n_sel * (n_win + 1);
+    net_window_start_kernel<<<dim3((unsigned)((n_idx + 255) / 256)), 256, 0, s>>>(t.net_off, t.ev, d_sel, n_sel, n_win,
+                                                                                  t.T, cycle_begin, n_cycle, d_idx);
+    const dim3 grid((unsigned)((n_sel + WS_NETS - 1) / WS_NETS), (unsigned)n_win);
+    net_weight_sum_kernel<<<grid, 256, 0, s>>>(t.net_off, t.ev, d_sel, d_wq, d_idx, n_sel, n_win, m, sh, cycle_begin,
+                                              n_cycle, d_out);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
+  std::vector<long long> acc;
+  if (e == hipSuccess) {
+    acc.resize(n_cycle);
+    e = hipMemcpyAsync(acc.data(), d_out, (size_t)n_cycle * 8, hipMemcpyDeviceToHost, s);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  float tms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&tms, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return e == hipErrorOutOfMemory ? GTP_ERR_OOM : GTP_ERR_DEVICE;
+  const double q = std::ldexp(1.0, -kq);
+  for (int64_t c = 0; c < n_cycle; ++c) out[c] = (double)acc[c] * q;
+  if (ms) *ms = tms;
+  return GTP_OK;
+}
+
 // Surrogate inference straight from the resident event store: toggle bits of
 // the selected nets (device), then the MLP.  `ms` receives the device time of
 // features + inference.
@@ -1127,12 +1294,23 @@ GTP_API gtp_status gtp_mlp_predict_trace(gtp_mlp* m, gtp_ctx* ctx, const int32_t
     return m->fail(GTP_ERR_ARG, "gtp_mlp_predict_trace: bad arguments");
   MCHK(m, hipSetDevice(m->device));
   const int words = m->words;
+  auto grow = [](auto*& p, int64_t& have, int64_t need, size_t elem) {
+    if (p && need <= have) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    have = 0;
+    hipError_t e = hipMalloc((void**)&p, (size_t)need * elem);
+    if (e == hipSuccess) have = need;
+    return e;
+  };
+  int64_t net_n = m->tr_net ? m->K : 0;
+  if (!m->tr_net) MCHK(m, grow(m->tr_net, net_n, m->K, 4));
   FeatureSel fs;
-  MCHK(m, fs.build(t, m->K, nets, m->stream));
-  uint32_t* d_X = nullptr;
-  float* d_out = nullptr;
-  hipError_t e = hipMalloc((void**)&d_X, (size_t)n_cycle * words * 4);
-  if (e == hipSuccess) e = hipMalloc((void**)&d_out, (size_t)n_cycle * 4);
+  MCHK(m, fs.use(m->tr_net, m->K, nets, m->stream));
+  hipError_t e = grow(m->tr_X, m->tr_X_n, n_cycle * words, 4);
+  if (e == hipSuccess) e = grow(m->tr_out, m->tr_out_n, n_cycle, 4);
+  uint32_t* d_X = m->tr_X;
+  float* d_out = m->tr_out;
   const int64_t chunk = 1 << 18;
   if (e == hipSuccess) e = m->reserve((int)std::min<int64_t>(n_cycle, chunk));
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1154,8 +1332,6 @@ GTP_API gtp_status gtp_mlp_predict_trace(gtp_mlp* m, gtp_ctx* ctx, const int32_t
   if (e == hipSuccess) e = hipEventElapsedTime(&tms, e0, e1);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  if (d_X) (void)hipFree(d_X);
-  if (d_out) (void)hipFree(d_out);
   if (e != hipSuccess) return m->fail(GTP_ERR_DEVICE, std::string("gtp_mlp_predict_trace: ") + hipGetErrorString(e));
   for (int64_t i = 0; i < n_cycle; ++i) out[i] = out[i] * m->y_std + m->y_mean;
   if (ms) *ms = tms;

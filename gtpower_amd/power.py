@@ -238,6 +238,26 @@ class HipPower:
                                                   int(n), abi.ptr(bits, C.c_uint32)), "gtp_toggle_features")
         return bits
 
+    def net_event_counts(self):
+        """Events per net of the loaded trace (int64 [n_net])."""
+        out = np.zeros(self.design.n_net, np.int64)
+        self._check(self._lib.gtp_net_event_counts(self._h, abi.ptr(out, C.c_int64)), "gtp_net_event_counts")
+        return out
+
+    def net_weight_sum(self, nets, weights, cycle_begin=0, n_cycle=None):
+        """Per cycle, sum over k of weights[k] x value changes of nets[k]
+        (device, exact to 2^-20 of the largest weight); returns (sums, ms)."""
+        nets = np.ascontiguousarray(nets, dtype=np.int32)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        assert nets.shape == w.shape
+        n = self.n_cycle - cycle_begin if n_cycle is None else n_cycle
+        out = np.zeros(n, np.float64)
+        ms = C.c_double()
+        self._check(self._lib.gtp_net_weight_sum(self._h, len(nets), abi.ptr(nets, C.c_int32), abi.ptr(w, C.c_double),
+                                                 int(cycle_begin), int(n), abi.ptr(out, C.c_double), C.byref(ms)),
+                    "gtp_net_weight_sum")
+        return out, ms.value
+
     def print_cell_res(self, path, gate_names=None):
         """Write cuda_cell_power.txt (power/cuda/CudaPower.cu:3134-3165)."""
         res = self.last

@@ -364,6 +364,18 @@ gtp_status gtp_feature_correlation(gtp_ctx* ctx, const double* y, const uint8_t*
  * [cycle_begin, +n_cycle) into bits[n_cycle][n_sel / 32]. */
 gtp_status gtp_toggle_features(gtp_ctx* ctx, int32_t n_sel, const int32_t* feats, int64_t cycle_begin,
                                int64_t n_cycle, uint32_t* bits);
+/* Events per net of the trace loaded in ctx (counts[n_net]; the first event
+ * of a net is its initial value). */
+gtp_status gtp_net_event_counts(gtp_ctx* ctx, int64_t* counts);
+/* Weighted toggle sums of the trace loaded in ctx: out[c] = sum over k of
+ * weights[k] x (value changes of net nets[k] in cycle cycle_begin + c), for
+ * n_cycle cycles (the surrogate's linear base term; no counterpart in the
+ * reference).  Weights are held to 2^-20 of the largest |weight| and summed
+ * exactly.  Runs on the context's auxiliary stream, so it may overlap a
+ * power run of the same context issued from another host thread; ms (may be
+ * NULL) receives the device time. */
+gtp_status gtp_net_weight_sum(gtp_ctx* ctx, int32_t n_sel, const int32_t* nets, const double* weights,
+                              int64_t cycle_begin, int64_t n_cycle, double* out, double* ms);
 /* The surrogate's matrix-core GEMM on host arrays, C[M][N] = A[M][K] B[K][N]
  * in FP32: a_mode 0 A row-major bf16, 1 A stored [K][M] bf16, 2 A bits
  * [M][K/32]; b_mode 0 B row-major bf16, 1 B stored [N][K] bf16, 2 B bits
