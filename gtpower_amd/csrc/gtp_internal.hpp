@@ -64,6 +64,36 @@ struct DArr {
   }
 };
 
+// Page-locked host memory (hipHostMalloc), grow-only: host -> device copies
+// from it run as DMA at PCIe rate, without the runtime's staging through its
+// own pinned buffer.
+template <typename T>
+struct Pinned {
+  T* p = nullptr;
+  size_t n = 0;
+  Pinned() = default;
+  Pinned(const Pinned&) = delete;
+  Pinned& operator=(const Pinned&) = delete;
+  ~Pinned() { release(); }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t reserve(size_t count) {
+    if (p && count <= n) return hipSuccess;
+    release();
+    if (count == 0) return hipSuccess;
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&p), count * sizeof(T), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
+    n = count;
+    return hipSuccess;
+  }
+};
+
 constexpr uint32_t EV_SHIFT = 3;                // time bits start here
 constexpr uint32_t EV_GLITCH = 4u;              // the event carries a glitch pulse width
 constexpr uint32_t EV_INF = 0xFFFFFFF8u;       // time field 2^29 - 1, glitch bit and value clear
