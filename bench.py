@@ -44,6 +44,57 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """The host CPU the baselines ran on: its model (from /proc/cpuinfo) and
+    the processors visible to this process (nproc)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        nproc = os.cpu_count()
+    return {"cpu_model": model, "nproc": nproc}
+
+
+_H2D_PROBE = """
+import sys, torch
+n = int(sys.argv[1]) << 20
+reps = int(sys.argv[2])
+h = torch.empty(n, dtype=torch.uint8, pin_memory=True)
+h.fill_(1)
+d = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+d.copy_(h, non_blocking=True)
+torch.cuda.synchronize()
+e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+e0.record()
+for _ in range(reps):
+    d.copy_(h, non_blocking=True)
+e1.record()
+torch.cuda.synchronize()
+print(n * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+"""
+
+
+def pinned_h2d_gbs(mib=1024, reps=5):
+    """Host-to-device copy rate from page-locked memory (GB/s), the bound of
+    any upload of host data.  Measured in a child process: torch brings its
+    own HIP runtime, which cannot share this process with the engine's."""
+    import subprocess
+    out = subprocess.run([sys.executable, "-c", _H2D_PROBE, str(mib), str(reps)], capture_output=True, text=True,
+                         timeout=300)
+    try:
+        return float(out.stdout.strip().splitlines()[-1])
+    except (ValueError, IndexError):
+        log(f"pinned H2D probe failed: {out.stderr.strip()[-300:]}")
+        return None
+
+
 def algorithmic_bytes(design, trace):
     """Compulsory bytes of one evaluation, per the per-cycle unit
         B/cycle = 12 E/C + 40 + (24 P + 48 G + 16 sum(n_out n_in) + LUT bytes) / C
@@ -107,52 +158,59 @@ def copy_calibration(mib, reps=5):
     log(f"copy calibration: {reps} copies of {mib} MiB (read {mib} MiB + write {mib} MiB each)")
 
 
-def batch_leg(eng, design, w, n_trace, max_cycles, seed, schedule):
+def batch_leg(eng, design, w, n_trace, min_cycles, max_cycles, seed, schedule):
     """Batching: n_trace traces of the netlist with log-uniform lengths in
-    [max_cycles / 20, max_cycles] and toggle p in [0.05, 0.3], each with its own
-    seed, evaluated by one batched run (gtp_run_batch) and, for comparison, one
-    run per trace (gtp_set_trace + gtp_run).  The batched run takes the exact
-    leg's schedule (autotuned on the same netlist; an engine of its own with
-    that schedule); each per-trace run takes the engine's default schedule for
-    its trace (as a caller without a tuning pass gets)."""
-    import numpy as np
-    from gtpower_amd import synthetic as S
+    [min_cycles, max_cycles] and toggle p in [0.05, 0.3], each with its own
+    seed, handed one at a time to a streamed batch (gtp_batch_begin / add /
+    end: each trace is packed onto the device as it arrives and freed right
+    after, so the host holds one trace at a time) and, for comparison, run
+    one by one (gtp_set_trace + gtp_run, kernel time).  The batch takes the
+    exact leg's schedule (autotuned on the same netlist; an engine of its own
+    with that schedule); each per-trace run takes the engine's default
+    schedule for its trace (as a caller without a tuning pass gets)."""
     from gtpower_amd.power import HipPower
     rng = np.random.default_rng(seed)
-    lo = max(1, max_cycles // 20)
-    lengths = np.exp(rng.uniform(np.log(lo), np.log(max_cycles), n_trace)).astype(int)
+    lengths = np.exp(rng.uniform(np.log(min_cycles), np.log(max_cycles), n_trace)).astype(int)
     ps = rng.uniform(0.05, 0.3, n_trace)
     tk = dict(w["trace"])
-    t0 = time.perf_counter()
-    trs = []
-    for i in range(n_trace):
-        tk["n_cycle"] = int(lengths[i])
-        tk["toggle_p"] = float(ps[i])
-        trs.append(design.trace(seed=seed + 7919 * (i + 1), **tk))
-    gen_s = time.perf_counter() - t0
-    n_cyc = int(sum(t.n_cycle for t in trs))
-    n_ev = int(sum(t.n_event for t in trs))
-    log(f"batch: {n_trace} traces, {n_cyc} cycles ({lengths.min()}..{lengths.max()}), {n_ev} events, "
-        f"generated in {gen_s:.1f} s")
     tc, rg = (int(x) for x in schedule)
-    log(f"batch: batched run with tile_cycles {tc}, gates_per_wg {rg}")
+    log(f"batch: {n_trace} traces of {lengths.min()}..{lengths.max()} cycles ({int(lengths.sum())} in all), "
+        f"streamed; batched run with tile_cycles {tc}, gates_per_wg {rg}")
+    stats = {"events": 0, "gen_s": 0.0, "single_ms": 0.0, "max_events": 0}
+
+    def traces():
+        for i in range(n_trace):
+            tk["n_cycle"] = int(lengths[i])
+            tk["toggle_p"] = float(ps[i])
+            t0 = time.perf_counter()
+            tr = design.trace(seed=seed + 7919 * (i + 1), **tk)
+            stats["gen_s"] += time.perf_counter() - t0
+            stats["events"] += tr.n_event
+            stats["max_events"] = max(stats["max_events"], tr.n_event)
+            eng.read_power_activities(tr)
+            stats["single_ms"] += eng.power(want=("cycle",)).kernel_ms
+            eng.trace = None
+            yield tr
+
     with HipPower(tile_cycles=tc, gates_per_wg=rg) as eb:
         eb.set_design(design.library, design)
-        eb.power_batch(trs, want=("cycle",))  # warm-up
-        b = eb.power_batch(trs, want=("cycle",))
-        batch_ms = b[0].kernel_ms
-    single_ms = 0.0
-    for tr in trs:
-        eng.read_power_activities(tr)
-        single_ms += eng.power(want=("cycle",)).kernel_ms
-    for tr in trs:
-        tr.close()
-    eng.trace = None
-    log(f"batch: one batched run {batch_ms:.2f} ms vs one run per trace {single_ms:.2f} ms")
-    return {"n_trace": n_trace, "cycles": n_cyc, "events": n_ev, "min_cycles": int(lengths.min()),
-            "max_cycles": int(lengths.max()), "batch_ms": batch_ms, "per_trace_runs_ms": single_ms,
+        t0 = time.perf_counter()
+        res, batch_ms = eb.power_batch_stream(traces(), want=("cycle",), release=lambda t: t.close())
+        wall_s = time.perf_counter() - t0
+        sched = res[0].schedule
+    n_cyc = int(lengths.sum())
+    single_ms = stats["single_ms"]
+    log(f"batch: {stats['events']} events (largest trace {stats['max_events']}), generated in "
+        f"{stats['gen_s']:.1f} s; streamed batch {batch_ms:.1f} ms of kernels in {sched['n_interval']} load(s) vs "
+        f"one run per trace {single_ms:.1f} ms")
+    return {"n_trace": n_trace, "cycles": n_cyc, "events": stats["events"], "min_cycles": int(lengths.min()),
+            "max_cycles": int(lengths.max()), "length_range": [min_cycles, max_cycles],
+            "largest_trace_events": stats["max_events"], "loads": sched["n_interval"],
+            "batch_ms": batch_ms, "per_trace_runs_ms": single_ms,
             "cycles_per_s": n_cyc / (batch_ms * 1e-3), "speedup_vs_per_trace_runs": single_ms / batch_ms,
-            "tile_cycles": b[0].schedule.get("tile_cycles"), "gates_per_wg": b[0].schedule.get("gates_per_wg")}
+            "tile_cycles": sched.get("tile_cycles"), "gates_per_wg": sched.get("gates_per_wg"),
+            "note": "streamed: each trace generated, run alone for comparison, handed to the batch and freed; "
+                    "batch_ms is the batch's device time"}
 
 
 def vcd_leg(design, w, n_cycle, steps, cpu):
@@ -215,7 +273,8 @@ def vcd_leg(design, w, n_cycle, steps, cpu):
         assert len(ref["times"]) == info["n_event"]
         res["cpu_baseline"] = {"value": body / cpu_s / 1e9, "unit": "GB/s of text", "cores": 1, "kind": "port",
                                "sample": f"the same {len(text) / 1e6:.0f} MB text, CPU restatement of the "
-                                         "reference reader and per-bit store build (single thread)"}
+                                         "reference reader and per-bit store build (single thread)",
+                               **host_cpu()}
         log(f"vcd: cpu reader {cpu_s:.2f} s ({body / cpu_s / 1e6:.1f} MB/s)")
     return res
 
@@ -339,7 +398,9 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--batch-traces", type=int, default=64)
-    ap.add_argument("--batch-max-cycles", type=int, default=2000)
+    ap.add_argument("--batch-min-cycles", type=int, default=1000)
+    ap.add_argument("--batch-max-cycles", type=int, default=20000,
+                    help="longest batch trace (the host generator holds a whole trace: ~26 B per event)")
     ap.add_argument("--surrogate-cycles", type=int, default=100000)
     ap.add_argument("--judge-tile", type=int, default=64, help="cycles per tile the exact judge recomputes")
     ap.add_argument("--surrogate-windows", type=int, default=10,
@@ -418,16 +479,25 @@ def main():
         # budget of 1/parts of it (a trace larger than the device's memory
         # runs this way): host slicing, PCIe uploads and kernels timed
         budget = n_event // args.sliced_parts + 4 * design.n_net
+        h2d = pinned_h2d_gbs()
+        # the first call sets up the page-locked staging ring; the second is timed
+        eng.power_sliced(trace, max_events=budget, want=("cycle",))
         t0 = time.perf_counter()
         rs = eng.power_sliced(trace, max_events=budget, want=("cycle",))
         sl_s = time.perf_counter() - t0
         same = bool(np.array_equal(rs.cycle_power, r.cycle_power))
+        words_gb = 4.0 * n_event / 1e9
+        bound_ms = 1.5 * words_gb / h2d * 1e3 + rs.kernel_ms if h2d else None
         sliced = {"intervals": rs.schedule["n_interval"], "event_budget": budget, "wall_ms": sl_s * 1e3,
                   "kernel_ms": rs.kernel_ms, "cycles_per_s": n_cycle / sl_s,
-                  "identical_to_resident_run": same,
-                  "note": "wall time includes slicing the host arrays and uploading every interval"}
+                  "identical_to_resident_run": same, "pinned_h2d_gb_per_s": h2d, "event_word_gb": words_gb,
+                  "upload_bound_ms": bound_ms,
+                  "note": "wall time of the second call: slicing the host arrays, packing event words into "
+                          "page-locked staging, uploading and evaluating every interval; upload_bound_ms = "
+                          "1.5 x the words over the pinned copy rate + kernel time"}
         log(f"sliced: {sliced['intervals']} intervals, {sl_s * 1e3:.0f} ms wall ({rs.kernel_ms:.1f} ms kernels), "
-            f"{n_cycle / sl_s:.3g} cycles/s, identical to the resident run: {same}")
+            f"{n_cycle / sl_s:.3g} cycles/s, pinned H2D {h2d} GB/s (bound {bound_ms} ms), "
+            f"identical to the resident run: {same}")
     trace.close()
 
     surrogate = None
@@ -439,8 +509,8 @@ def main():
     # after the surrogate leg: the batch replaces the resident trace
     batch = None
     if args.batch_traces > 0:
-        batch = batch_leg(eng, design, w, args.batch_traces, args.batch_max_cycles, S.MASTER_SEED ^ 0xBA7C4,
-                          (sched.get("tile_cycles"), sched.get("gates_per_wg")))
+        batch = batch_leg(eng, design, w, args.batch_traces, args.batch_min_cycles, args.batch_max_cycles,
+                          S.MASTER_SEED ^ 0xBA7C4, (sched.get("tile_cycles"), sched.get("gates_per_wg")))
     eng.close()
 
     vcd = None
@@ -462,7 +532,8 @@ def main():
         ref = oracle.run(design.library, design, strace, threads=threads)
         cpu = {"value": sample / ref["wall_s"], "unit": "cycles/s", "cores": threads, "kind": "port",
                "sample": f"first {sample} of {n_cycle} cycles, all {design.n_gate} gates "
-                         f"({strace.n_event} events), FP64 restatement of the reference CPU path"}
+                         f"({strace.n_event} events), FP64 restatement of the reference CPU path",
+               **host_cpu()}
         log(f"cpu baseline {cpu['value']:.4g} cycles/s ({ref['wall_s']:.2f} s)")
 
     out = {

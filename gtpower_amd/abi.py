@@ -152,6 +152,9 @@ def load_engine():
     lib.gtp_run.argtypes = [C.c_void_p, C.POINTER(GtpResult)]
     lib.gtp_run_batch.argtypes = [C.c_void_p, C.c_int32, C.POINTER(GtpTrace), C.POINTER(GtpResult)]
     lib.gtp_run_sliced.argtypes = [C.c_void_p, C.POINTER(GtpTrace), C.c_int64, C.POINTER(GtpResult)]
+    lib.gtp_batch_begin.argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_void_p)]
+    lib.gtp_batch_add.argtypes = [C.c_void_p, C.POINTER(GtpTrace), C.POINTER(GtpResult)]
+    lib.gtp_batch_end.argtypes = [C.c_void_p, p_f64]
     lib.gtp_get_schedule.argtypes = [C.c_void_p, C.POINTER(GtpScheduleInfo)]
     lib.gtp_reference_cycles_per_thread.argtypes = [C.c_int64, p_i64, C.c_int64, C.c_int64,
                                                     C.c_int64, p_i64]

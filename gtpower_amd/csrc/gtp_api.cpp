@@ -17,6 +17,7 @@
 #include <numeric>
 #include <string>
 #include <thread>
+#include <sched.h>
 #include <vector>
 
 #include "gtp_internal.hpp"
@@ -30,10 +31,23 @@ namespace {
 
 thread_local std::string g_create_error;
 
+// Host worker threads when none are asked for: OMP_NUM_THREADS when set (a
+// job's CPU share), else the processors this process may run on.
+int default_threads() {
+  if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+    const int v = std::atoi(e);
+    if (v > 0) return v;
+  }
+  cpu_set_t cs;
+  CPU_ZERO(&cs);
+  if (sched_getaffinity(0, sizeof(cs), &cs) == 0 && CPU_COUNT(&cs) > 0) return CPU_COUNT(&cs);
+  return (int)std::max(1u, std::thread::hardware_concurrency());
+}
+
 template <class F>
 void parallel_for(int64_t n, int threads, F f) {
   if (n <= 0) return;
-  if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+  if (threads <= 0) threads = default_threads();
   threads = (int)std::min<int64_t>(threads, std::max<int64_t>(1, n / 1024));
   if (threads <= 1) {
     for (int64_t i = 0; i < n; ++i) f(i);
@@ -173,6 +187,18 @@ struct gtp_ctx {
   DArr<double> d_cycle, d_gate, d_total, d_total_part;
   DArr<unsigned long long> d_gtog, d_gglc;
   DArr<unsigned int> d_maxbits;
+
+  // page-locked staging of packed event words for loads from host memory
+  // (sliced runs): a ring of buffers, each refilled once its copy to the
+  // device has completed, so that packing the next part overlaps the copy of
+  // the last one; the copies run on their own stream (created on first use)
+  static constexpr int N_STAGE = 4;
+  static constexpr int64_t STAGE_WORDS = (int64_t)1 << 25;  // 128 MiB a buffer
+  Pinned<uint32_t> stage[N_STAGE];
+  hipEvent_t stage_ev[N_STAGE] = {nullptr, nullptr, nullptr, nullptr};
+  bool stage_pending[N_STAGE] = {false, false, false, false};
+  int stage_next = 0;
+  hipStream_t up = nullptr;
 
   gtp_status fail(gtp_status st, const std::string& m) {
     err = m;
@@ -390,7 +416,7 @@ GTP_API gtp_status gtp_create(const gtp_options* opt, gtp_ctx** out) {
     }
   }
   x->threads = x->opt.host_threads > 0 ? x->opt.host_threads
-                                       : (int)std::max(1u, std::thread::hardware_concurrency());
+                                       : default_threads();
   {
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
@@ -419,6 +445,12 @@ GTP_API void gtp_destroy(gtp_ctx* x) {
     if (x->join[k]) (void)hipEventDestroy(x->join[k]);
   }
   if (x->fork) (void)hipEventDestroy(x->fork);
+  if (x->up) {
+    (void)hipStreamSynchronize(x->up);
+    (void)hipStreamDestroy(x->up);
+  }
+  for (int k = 0; k < gtp_ctx::N_STAGE; ++k)
+    if (x->stage_ev[k]) (void)hipEventDestroy(x->stage_ev[k]);
   delete x;
 }
 
@@ -602,8 +634,16 @@ GTP_API gtp_status gtp_set_design(gtp_ctx* x, const gtp_library* L, const gtp_de
 // ---------------------------------------------------------------------------
 namespace {
 
-// Checks one trace against the design; *T_out = ticks per cycle
-// (power/PowerUtils.cc:27-32).
+// Ticks per cycle, ceil(period / time scale) (power/PowerUtils.cc:27-32),
+// exact when the ratio is an integer up to rounding.
+int64_t ticks_per_cycle(double clk_period, double time_scale) {
+  const double ratio = clk_period / time_scale;
+  int64_t T = (int64_t)std::ceil(ratio);
+  if (std::fabs(ratio - std::round(ratio)) < 1e-9 * std::max(1.0, ratio)) T = (int64_t)std::llround(ratio);
+  return T;
+}
+
+// Checks one trace against the design; *T_out = ticks per cycle.
 gtp_status validate_trace(gtp_ctx* x, const gtp_trace* tr, int64_t* T_out, bool long_ok = false) {
   if (!tr || !tr->net_event_offset || tr->n_net != x->N)
     return x->fail(GTP_ERR_ARG, "gtp_trace: n_net must equal the design's n_net");
@@ -635,9 +675,7 @@ gtp_status validate_trace(gtp_ctx* x, const gtp_trace* tr, int64_t* T_out, bool 
   if (bad.load() >= 0)
     return x->fail(GTP_ERR_ARG, "gtp_trace: events of net " + std::to_string(bad.load()) +
                                     " are not strictly increasing in [0, t_max] or carry a value > 3");
-  const double ratio = tr->clk_period / tr->time_scale;
-  int64_t T = (int64_t)std::ceil(ratio);
-  if (std::fabs(ratio - std::round(ratio)) < 1e-9 * std::max(1.0, ratio)) T = (int64_t)std::llround(ratio);
+  const int64_t T = ticks_per_cycle(tr->clk_period, tr->time_scale);
   if (T < 1 || T >= (int64_t)EV_TIME_INF) return x->fail(GTP_ERR_UNSUPPORTED, "gtp_trace: ticks per cycle out of range");
   *T_out = T;
   return GTP_OK;
@@ -1404,6 +1442,8 @@ struct RunItem {
   int64_t t_max, n_cycle;    // relative to the load's origin (tile_base whole tiles into the trace)
   int32_t n_tile, tile_begin, tile_end, slot;
   const uint32_t* words = nullptr;  // the events already packed as event words (time / value unused)
+  uint32_t* dev_words = nullptr;    // first item only: every item's words already on the device, in item
+                                    // order from here (EV_PAD padded)
   int32_t tile_base = 0;     // the origin: tiles of the trace before it
   int64_t out_cycles = 0;    // cycles of the trace's waveform (0: n_cycle)
 };
@@ -1419,6 +1459,14 @@ struct MultiPlan {
   double time_scale = 0, clk_period = 0;
   std::vector<int64_t> work;
 };
+
+// Tile length of multi-trace runs: the option, else the autotuner's choice
+// on the resident trace (gtp_autotune), else 128.
+int64_t multi_tile_cycles(const gtp_ctx* x, int64_t T) {
+  int64_t TC = x->opt.tile_cycles > 0 ? x->opt.tile_cycles : (x->tuned_tc > 0 ? x->tuned_tc : 128);
+  if (T < (1 << 24)) TC = std::min<int64_t>(TC, std::max<int64_t>(1, ((1 << 24) - 1) / T));
+  return std::max<int64_t>(1, std::min<int64_t>(TC, 4096));
+}
 
 // work[g] = pin events of gate g over the run (orders the gates)
 // cycles[k]: cycles of trace k (sizes the rounds)
@@ -1439,11 +1487,7 @@ gtp_status multi_setup(gtp_ctx* x, int64_t T, double time_scale, double clk_peri
     const gtp_status st = make_quanta(x, tmax_max, mp->work, &mp->qt);
     if (st != GTP_OK) return st;
   }
-  // tile length: the option, else the autotuner's choice on the resident
-  // trace (gtp_autotune), else 128
-  int64_t TC = x->opt.tile_cycles > 0 ? x->opt.tile_cycles : (x->tuned_tc > 0 ? x->tuned_tc : 128);
-  if (T < (1 << 24)) TC = std::min<int64_t>(TC, std::max<int64_t>(1, ((1 << 24) - 1) / T));
-  TC = std::max<int64_t>(1, std::min<int64_t>(TC, 4096));
+  const int64_t TC = multi_tile_cycles(x, T);
   mp->TC = TC;
   int64_t total_tiles = 0;
   for (int64_t c : cycles) total_tiles += (c + TC - 1) / TC;
@@ -1528,8 +1572,10 @@ gtp_status run_load(gtp_ctx* x, MultiPlan& mp, const std::vector<RunItem>& items
   DArr<int64_t> d_voff, d_chunk;
   DArr<uint32_t> d_ev;
   HIPCHK(x, d_voff.upload(voff.data(), voff.size(), s));
-  HIPCHK(x, d_ev.alloc(E_load + EV_PAD));
-  {
+  uint32_t* ev_p = items[0].dev_words;  // the load's words already on the device, items in order
+  if (!ev_p) {
+    HIPCHK(x, d_ev.alloc(E_load + EV_PAD));
+    ev_p = d_ev.p;
     const int64_t CH = 1 << 26;
     DArr<int64_t> st;
     DArr<uint8_t> sv;
@@ -1538,7 +1584,7 @@ gtp_status run_load(gtp_ctx* x, MultiPlan& mp, const std::vector<RunItem>& items
     int64_t base = 0;
     for (int32_t k = 0; k < nt; ++k) {
       if (items[k].words) {
-        HIPCHK(x, hipMemcpyAsync(d_ev.p + base, items[k].words, sizeof(uint32_t) * E[k], hipMemcpyHostToDevice, s));
+        HIPCHK(x, hipMemcpyAsync(ev_p + base, items[k].words, sizeof(uint32_t) * E[k], hipMemcpyHostToDevice, s));
         base += E[k];
         continue;
       }
@@ -1546,7 +1592,7 @@ gtp_status run_load(gtp_ctx* x, MultiPlan& mp, const std::vector<RunItem>& items
         const int64_t n = std::min(CH, E[k] - b0);
         HIPCHK(x, hipMemcpyAsync(st.p, items[k].time + b0, n * sizeof(int64_t), hipMemcpyHostToDevice, s));
         HIPCHK(x, hipMemcpyAsync(sv.p, items[k].value + b0, n, hipMemcpyHostToDevice, s));
-        HIPCHK(x, launch_pack(st.p, sv.p, d_ev.p + base + b0, n, s));
+        HIPCHK(x, launch_pack(st.p, sv.p, ev_p + base + b0, n, s));
       }
       base += E[k];
     }
@@ -1607,9 +1653,9 @@ gtp_status run_load(gtp_ctx* x, MultiPlan& mp, const std::vector<RunItem>& items
   } evg{e0, e1};
   HIPCHK(x, hipEventRecord(e0, s));
   for (int32_t k = 0; k < nt; ++k)
-    HIPCHK(x, launch_tile_index(d_voff.p + (int64_t)k * N, d_ev.p, N, items[k].n_tile, TC * T, nullptr, 0,
+    HIPCHK(x, launch_tile_index(d_voff.p + (int64_t)k * N, ev_p, N, items[k].n_tile, TC * T, nullptr, 0,
                                 d_tidx.p + td[k].tile_idx_base, s));
-  HIPCHK(x, launch_glitch_annotate(d_voff.p, d_ev.p, d_chunk.p, d_chunk_net.p, n_chunk, (uint32_t)T, s));
+  HIPCHK(x, launch_glitch_annotate(d_voff.p, ev_p, d_chunk.p, d_chunk_net.p, n_chunk, (uint32_t)T, s));
   // per-work-tile lane order, as in full runs (GTP_TILE_ORDER=0, or when it
   // would take more than half the device memory still free after the load's
   // own buffers: schedule order).  Freed with the load.
@@ -1635,7 +1681,7 @@ gtp_status run_load(gtp_ctx* x, MultiPlan& mp, const std::vector<RunItem>& items
     }
   }
   rp.a.net_off = d_voff.p;
-  rp.a.ev = d_ev.p;
+  rp.a.ev = ev_p;
   rp.a.tile_idx = d_tidx.p;
   rp.a.traces = d_td.p;
   const int32_t n_range = mp.n_range;
@@ -1709,6 +1755,60 @@ gtp_status finish_slot(gtp_ctx* x, const MultiPlan& mp, int32_t slot, const DArr
   if (res.total) std::memcpy(res.total, tot, sizeof(double) * 5);
   if (res.class_total) std::memcpy(res.class_total, tot + 5, sizeof(double) * 25);
   res.n_cycle = n_cycle;
+  return GTP_OK;
+}
+
+// Creates the staging ring and its stream on first use.
+gtp_status staging_init(gtp_ctx* x) {
+  if (!x->up) HIPCHK(x, hipStreamCreate(&x->up));
+  for (int k = 0; k < gtp_ctx::N_STAGE; ++k) {
+    if (!x->stage_ev[k]) HIPCHK(x, hipEventCreateWithFlags(&x->stage_ev[k], hipEventDisableTiming));
+    HIPCHK(x, x->stage[k].reserve((size_t)gtp_ctx::STAGE_WORDS));
+  }
+  return GTP_OK;
+}
+
+// Packs the event words of one load into dst (device) through the staging
+// ring: nets [0, N) with word offsets so[N + 1], pack_net(n, out) writing net
+// n's so[n + 1] - so[n] words.  Nets are taken in runs that fill a staging
+// buffer; each run is packed by the host threads and copied on the staging
+// stream while the next run is packed.  done (recorded on the staging stream
+// after the last copy) tells a consumer stream when dst is complete.  Runs on
+// any host thread (it sets the context's device itself).
+template <class F>
+gtp_status staged_upload(gtp_ctx* x, const int64_t* so, int64_t N, uint32_t* dst, F pack_net, hipEvent_t done) {
+  HIPCHK(x, hipSetDevice(x->device));
+  const int64_t cap = gtp_ctx::STAGE_WORDS;
+  std::vector<uint32_t> big;
+  for (int64_t n0 = 0; n0 < N;) {
+    // the longest run of nets from n0 whose words fit one buffer
+    int64_t n1 = std::upper_bound(so + n0 + 1, so + N + 1, so[n0] + cap) - so - 1;
+    if (n1 <= n0) {
+      // one net larger than a buffer: packed whole and copied from pageable memory
+      big.resize((size_t)(so[n0 + 1] - so[n0]));
+      pack_net(n0, big.data());
+      HIPCHK(x, hipMemcpyAsync(dst + so[n0], big.data(), big.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                               x->up));
+      HIPCHK(x, hipStreamSynchronize(x->up));
+      n0 += 1;
+      continue;
+    }
+    const int b = x->stage_next;
+    x->stage_next = (b + 1) % gtp_ctx::N_STAGE;
+    if (x->stage_pending[b]) HIPCHK(x, hipEventSynchronize(x->stage_ev[b]));
+    x->stage_pending[b] = false;
+    uint32_t* h = x->stage[b].p;
+    const int64_t base = so[n0];
+    parallel_for(n1 - n0, x->threads, [&](int64_t i) { pack_net(n0 + i, h + (so[n0 + i] - base)); });
+    if (so[n1] > base) {
+      HIPCHK(x, hipMemcpyAsync(dst + base, h, (size_t)(so[n1] - base) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                               x->up));
+      HIPCHK(x, hipEventRecord(x->stage_ev[b], x->up));
+      x->stage_pending[b] = true;
+    }
+    n0 = n1;
+  }
+  HIPCHK(x, hipEventRecord(done, x->up));
   return GTP_OK;
 }
 
@@ -1913,6 +2013,7 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
   };
   // intervals: whole tiles, halved until the slice fits the budget
   std::vector<std::pair<int32_t, int32_t>> ivals;
+  int64_t max_words = 0;  // events of the largest interval's load
   {
     const int32_t guess = (int32_t)std::max<int64_t>(1, std::min<int64_t>(n_tile, (E + budget - 1) / std::max<int64_t>(budget, 1) * 2));
     const int32_t step0 = (n_tile + guess - 1) / guess;
@@ -1923,7 +2024,10 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
         return x->fail(GTP_ERR_UNSUPPORTED, "gtp_run_sliced: one tile spans 2^29 - 1 ticks or more");
       while (true) {
         const int64_t need = slice_bounds(k0, k0 + step, sb, se);
-        if (need <= budget) break;
+        if (need <= budget) {
+          max_words = std::max(max_words, need);
+          break;
+        }
         if (step == 1)
           return x->fail(GTP_ERR_OOM, "gtp_run_sliced: one tile of " + std::to_string(TC) + " cycles needs " +
                                           std::to_string(need) + " events, over the budget");
@@ -1942,83 +2046,131 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
   HIPCHK(x, launch_zero_u64(d_acc.p, (int64_t)N_PG * G, s));
   HIPCHK(x, launch_zero_u64(d_cnt.p, (int64_t)2 * G, s));
   float kernel_ms_total = 0.f;
-  // One interval's slices packed on the host as event words (4 B per event
-  // uploaded); two buffers alternate, so that the next interval is sliced and
-  // packed on a helper thread while the current one is uploaded and evaluated.
-  struct Packed {
+  // Each interval's slices are packed on the host straight into event words
+  // (4 B per event uploaded) through the page-locked staging ring, into one of
+  // two device buffers: a helper thread slices, packs and uploads the next
+  // interval while the current one is evaluated (one buffer when two do not
+  // fit the device: then one interval after the other).
+  struct Slices {
     std::vector<int64_t> sb, se, so;
-    std::vector<uint32_t> words;
   };
-  Packed pk[2];
+  Slices sl[2];
   const uint8_t* val = tr->event_value;
-  auto prepare = [&](Packed& P, const std::pair<int32_t, int32_t>& iv) {
+  auto bounds = [&](Slices& P, const std::pair<int32_t, int32_t>& iv) {
     P.sb.resize(N);
     P.se.resize(N);
     P.so.resize(N + 1);
     slice_bounds(iv.first, iv.second, P.sb, P.se);
     P.so[0] = 0;
     for (int64_t n = 0; n < N; ++n) P.so[n + 1] = P.so[n] + (P.se[n] - P.sb[n]);
-    P.words.resize(std::max<int64_t>(P.so[N], 1));
+  };
+  // net n's slice of interval iv as event words, times relative to the origin
+  auto pack_net = [&](const Slices& P, const std::pair<int32_t, int32_t>& iv, int64_t n, uint32_t* d) {
     const int64_t base = (int64_t)origin_tile(iv.first) * TC * T;
     const int64_t c0 = (int64_t)iv.first * TC, t0 = c0 * T;               // interval start
     const bool to_end = iv.second >= n_tile;
     const int64_t c1 = (int64_t)iv.second * TC, t1 = c1 * T;              // interval end (not the last)
-    parallel_for(N, x->threads, [&](int64_t n) {
-      uint32_t* d = P.words.data() + P.so[n];
-      const int64_t a = P.sb[n], b = P.se[n];
-      const int64_t i0 = std::lower_bound(tm + a, tm + b, t0) - tm;
-      const int64_t i1 = to_end ? b : std::lower_bound(tm + i0, tm + b, t1) - tm;
-      // before the interval: the event before the group (a) at or after the
-      // start of cycle c0 - K - 1, the group (a + 1 .. i0 - 1, one cycle)
-      // moved to cycle c0 - K when it lies earlier
-      int64_t gshift = 0;
-      if (i0 > a) {
-        const int64_t gc = tm[i0 - 1] / T;
-        if (gc < c0 - K) gshift = (gc - (c0 - K)) * T;
-      }
-      const int64_t t_before = (c0 - K - 1) * T;
-      // after it: the first event (i1) moved to cycle c1 + K when later, the
-      // second (i1 + 1) kept in its cycle or put at the start of the next one
-      int64_t ashift = 0, ac_new = 0;
-      if (!to_end && i1 < b) {
-        const int64_t ac = tm[i1] / T;
-        if (ac > c1 + K) ashift = (ac - (c1 + K)) * T;
-        ac_new = ac - ashift / T;
-      }
-      for (int64_t i = a; i < b; ++i) {
-        int64_t t = tm[i];
-        if (i < i0) {
-          t = i > a ? t - gshift : std::max(t, t_before);
-        } else if (i >= i1) {
-          t = (i == i1 || tm[i] / T == tm[i1] / T) ? t - ashift : std::min(t, (ac_new + 1) * T);
-        }
-        *d++ = ((uint32_t)(t - base) << EV_SHIFT) | (uint32_t)(val[i] & 3u);
-      }
-    });
+    const int64_t a = P.sb[n], b = P.se[n];
+    const int64_t i0 = std::lower_bound(tm + a, tm + b, t0) - tm;
+    const int64_t i1 = to_end ? b : std::lower_bound(tm + i0, tm + b, t1) - tm;
+    // before the interval: the event before the group (a) at or after the
+    // start of cycle c0 - K - 1, the group (a + 1 .. i0 - 1, one cycle)
+    // moved to cycle c0 - K when it lies earlier
+    int64_t gshift = 0;
+    if (i0 > a) {
+      const int64_t gc = tm[i0 - 1] / T;
+      if (gc < c0 - K) gshift = (gc - (c0 - K)) * T;
+    }
+    const int64_t t_before = (c0 - K - 1) * T;
+    // after it: the first event (i1) moved to cycle c1 + K when later, the
+    // second (i1 + 1) kept in its cycle or put at the start of the next one
+    int64_t ashift = 0, ac_new = 0;
+    if (!to_end && i1 < b) {
+      const int64_t ac = tm[i1] / T;
+      if (ac > c1 + K) ashift = (ac - (c1 + K)) * T;
+      ac_new = ac - ashift / T;
+    }
+    for (int64_t i = a; i < i0; ++i) {
+      const int64_t t = i > a ? tm[i] - gshift : std::max(tm[i], t_before);
+      *d++ = ((uint32_t)(t - base) << EV_SHIFT) | (uint32_t)(val[i] & 3u);
+    }
+    // the interval's own events (the bulk): times relative to the origin
+    for (int64_t i = i0; i < i1; ++i) *d++ = ((uint32_t)(tm[i] - base) << EV_SHIFT) | (uint32_t)(val[i] & 3u);
+    for (int64_t i = i1; i < b; ++i) {
+      const int64_t t = (i == i1 || tm[i] / T == tm[i1] / T) ? tm[i] - ashift : std::min(tm[i], (ac_new + 1) * T);
+      *d++ = ((uint32_t)(t - base) << EV_SHIFT) | (uint32_t)(val[i] & 3u);
+    }
   };
-  prepare(pk[0], ivals[0]);
+  // device buffers sized for the largest interval; the second one only when
+  // it takes at most half the memory still free (the loads' tile indexes and
+  // block maps need the rest)
+  {
+    const gtp_status st = staging_init(x);
+    if (st != GTP_OK) return st;
+  }
+  DArr<uint32_t> d_words[2];
+  HIPCHK(x, d_words[0].alloc(max_words + EV_PAD));
+  int n_buf = 1;
+  if (ivals.size() > 1) {
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(x, hipMemGetInfo(&free_b, &total_b));
+    if ((double)(max_words + EV_PAD) * 4.0 <= 0.5 * (double)free_b) {
+      if (d_words[1].alloc(max_words + EV_PAD) == hipSuccess) n_buf = 2;
+      else (void)hipGetLastError();
+    }
+  }
+  hipEvent_t ready[2] = {nullptr, nullptr};
+  HIPCHK(x, hipEventCreateWithFlags(&ready[0], hipEventDisableTiming));
+  HIPCHK(x, hipEventCreateWithFlags(&ready[1], hipEventDisableTiming));
+  struct EvG2 {
+    hipEvent_t* e;
+    ~EvG2() {
+      (void)hipEventDestroy(e[0]);
+      (void)hipEventDestroy(e[1]);
+    }
+  } evg2{ready};
+  auto prepare = [&](size_t i) {
+    Slices& P = sl[i & 1];
+    bounds(P, ivals[i]);
+    const auto iv = ivals[i];
+    return staged_upload(x, P.so.data(), N, d_words[i % n_buf].p,
+                         [&](int64_t n, uint32_t* d) { pack_net(P, iv, n, d); }, ready[i & 1]);
+  };
+  {
+    const gtp_status st = prepare(0);
+    if (st != GTP_OK) return st;
+  }
   for (size_t i = 0; i < ivals.size(); ++i) {
-    Packed& P = pk[i & 1];
+    Slices& P = sl[i & 1];
     std::thread next;
-    if (i + 1 < ivals.size()) next = std::thread([&, i] { prepare(pk[(i + 1) & 1], ivals[i + 1]); });
-    JoinGuard join{next};
-    // the load as a trace from its origin: kb tiles in, to the trace's end
-    // (the last interval) or just past the two events after the interval
-    const int32_t kb = origin_tile(ivals[i].first);
-    const int64_t base = (int64_t)kb * TC * T;
-    const int64_t tm_rel = ivals[i].second >= n_tile ? tr->t_max - base
-                                                     : ((int64_t)ivals[i].second * TC + K + 2) * T - base;
-    const int64_t c_rel = ivals[i].second >= n_tile ? C - (int64_t)kb * TC : (tm_rel + T - 1) / T;
-    RunItem item{P.so.data(), nullptr, nullptr, tm_rel, c_rel, n_tile - kb, ivals[i].first - kb,
-                 ivals[i].second - kb, 0};
-    item.words = P.words.data();
-    item.tile_base = kb;
-    item.out_cycles = C;
-    std::vector<RunItem> items{item};
-    float ms = 0.f;
-    const gtp_status st = run_load(x, mp, items, d_acc, d_cnt, d_cyc, slot_base, &ms);
-    if (st != GTP_OK) return st;  // (the helper thread is joined first)
-    kernel_ms_total += ms;
+    gtp_status next_st = GTP_OK;
+    if (i + 1 < ivals.size() && n_buf == 2) next = std::thread([&, i] { next_st = prepare(i + 1); });
+    {
+      JoinGuard join{next};
+      // the load as a trace from its origin: kb tiles in, to the trace's end
+      // (the last interval) or just past the two events after the interval
+      const int32_t kb = origin_tile(ivals[i].first);
+      const int64_t base = (int64_t)kb * TC * T;
+      const int64_t tm_rel = ivals[i].second >= n_tile ? tr->t_max - base
+                                                       : ((int64_t)ivals[i].second * TC + K + 2) * T - base;
+      const int64_t c_rel = ivals[i].second >= n_tile ? C - (int64_t)kb * TC : (tm_rel + T - 1) / T;
+      RunItem item{P.so.data(), nullptr, nullptr, tm_rel, c_rel, n_tile - kb, ivals[i].first - kb,
+                   ivals[i].second - kb, 0};
+      item.dev_words = d_words[i % n_buf].p;
+      item.tile_base = kb;
+      item.out_cycles = C;
+      std::vector<RunItem> items{item};
+      HIPCHK(x, hipStreamWaitEvent(s, ready[i & 1], 0));
+      float ms = 0.f;
+      const gtp_status st = run_load(x, mp, items, d_acc, d_cnt, d_cyc, slot_base, &ms);
+      if (st != GTP_OK) return st;  // (the helper thread is joined first)
+      kernel_ms_total += ms;
+    }
+    if (next_st != GTP_OK) return next_st;
+    if (i + 1 < ivals.size() && n_buf == 1) {
+      const gtp_status st = prepare(i + 1);
+      if (st != GTP_OK) return st;
+    }
   }
   {
     const gtp_status st = finish_slot(x, mp, 0, d_acc, d_cnt, d_cyc, 0, tr->t_max, C, *res);
@@ -2029,6 +2181,234 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
   record_multi_schedule(x, mp, E);
   x->sched.n_interval = (int64_t)ivals.size();
   return GTP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Streamed batch: traces handed over one at a time; each is packed into event
+// words on the device on arrival (through the staging ring), so the caller
+// holds only the trace it is adding.  Loads are evaluated when the next trace
+// does not fit the event budget, and at the end.
+// ---------------------------------------------------------------------------
+struct gtp_batch {
+  gtp_ctx* x = nullptr;
+  int64_t T = 0, TC = 0, budget = 0;
+  double time_scale = 0, clk_period = 0;
+  gtp_status st = GTP_OK;          // first error; later calls return it
+  double kernel_ms = 0;
+  int32_t n_added = 0, n_load = 0;
+  int64_t n_event = 0;
+  // the load being filled
+  DArr<uint32_t> d_words;
+  int64_t words_used = 0, load_words = 0;
+  hipEvent_t ready = nullptr;      // the load's last upload done
+  struct Pend {
+    std::vector<int64_t> off;      // the trace's net offsets
+    int64_t t_max = 0, C = 0;
+    int32_t n_tile = 0;
+    gtp_result* res = nullptr;
+  };
+  std::vector<Pend> pend;
+  std::vector<int64_t> work;       // pin events per gate over the load
+};
+
+namespace {
+
+gtp_status batch_flush(gtp_batch* b) {
+  gtp_ctx* x = b->x;
+  if (b->pend.empty()) return GTP_OK;
+  hipStream_t s = x->stream;
+  const int64_t G = x->G;
+  const int32_t nt = (int32_t)b->pend.size();
+  std::vector<int64_t> cyc(nt);
+  int64_t tmax_max = 0;
+  for (int32_t k = 0; k < nt; ++k) {
+    cyc[k] = b->pend[k].C;
+    tmax_max = std::max(tmax_max, b->pend[k].t_max);
+  }
+  MultiPlan mp;
+  {
+    const gtp_status st = multi_setup(x, b->T, b->time_scale, b->clk_period, tmax_max, cyc, std::move(b->work), &mp);
+    if (st != GTP_OK) return st;
+  }
+  if (mp.TC != b->TC) return x->fail(GTP_ERR_STATE, "gtp_batch: the tile length changed during the batch");
+  std::vector<int64_t> slot_base(nt + 1, 0);
+  for (int32_t k = 0; k < nt; ++k) slot_base[k + 1] = slot_base[k] + (int64_t)GTP_N_WAVE * cyc[k];
+  DArr<unsigned long long> d_acc, d_cnt;
+  DArr<double> d_cyc;
+  HIPCHK(x, d_acc.alloc((size_t)nt * N_PG * G + 1));
+  HIPCHK(x, d_cnt.alloc((size_t)nt * 2 * G + 1));
+  HIPCHK(x, d_cyc.alloc(std::max<int64_t>(slot_base[nt], 1)));
+  HIPCHK(x, launch_zero_u64(d_acc.p, (int64_t)nt * N_PG * G, s));
+  HIPCHK(x, launch_zero_u64(d_cnt.p, (int64_t)nt * 2 * G, s));
+  std::vector<RunItem> items;
+  for (int32_t k = 0; k < nt; ++k) {
+    const auto& P = b->pend[k];
+    items.push_back(RunItem{P.off.data(), nullptr, nullptr, P.t_max, P.C, P.n_tile, 0, P.n_tile, k});
+  }
+  items[0].dev_words = b->d_words.p;
+  HIPCHK(x, hipStreamWaitEvent(s, b->ready, 0));
+  float ms = 0.f;
+  {
+    const gtp_status st = run_load(x, mp, items, d_acc, d_cnt, d_cyc, slot_base, &ms);
+    if (st != GTP_OK) return st;
+  }
+  for (int32_t k = 0; k < nt; ++k) {
+    gtp_result& r = *b->pend[k].res;
+    const gtp_status st = finish_slot(x, mp, k, d_acc, d_cnt, d_cyc, slot_base[k], b->pend[k].t_max, b->pend[k].C, r);
+    if (st != GTP_OK) return st;
+    r.kernel_ms = ms;  // the load's device time
+    r.total_ms = ms;
+  }
+  record_multi_schedule(x, mp, b->n_event);
+  b->kernel_ms += ms;
+  b->n_load += 1;
+  b->pend.clear();
+  b->work.assign(G, 0);
+  b->words_used = 0;
+  b->load_words = 0;
+  return GTP_OK;
+}
+
+gtp_status batch_add(gtp_batch* b, const gtp_trace* tr, gtp_result* res) {
+  gtp_ctx* x = b->x;
+  const int64_t N = x->N, G = x->G;
+  int64_t Tk = 0;
+  {
+    const gtp_status st = validate_trace(x, tr, &Tk, true);
+    if (st != GTP_OK) return st;
+  }
+  if (Tk != b->T || tr->time_scale != b->time_scale || tr->clk_period != b->clk_period)
+    return x->fail(GTP_ERR_UNSUPPORTED, "gtp_batch_add: traces must share time_scale and clk_period");
+  const int64_t* off = tr->net_event_offset;
+  const int64_t E = off[N];
+  const int64_t C = (tr->t_max + b->T - 1) / b->T;
+  const int32_t n_tile = (int32_t)((C + b->TC - 1) / b->TC);
+  const int64_t words = E + N * ((int64_t)n_tile + 1) + 4 * N;
+  b->n_event += E;
+  if (tr->t_max >= (int64_t)EV_TIME_INF || words > b->budget) {
+    // longer than one event word's time span, or larger than one load: by
+    // itself, in cycle intervals
+    {
+      const gtp_status st = batch_flush(b);
+      if (st != GTP_OK) return st;
+    }
+    const gtp_status st = gtp_run_sliced(x, tr, b->budget, res);
+    if (st != GTP_OK) return st;
+    b->kernel_ms += res->kernel_ms;
+    b->n_load += (int32_t)x->sched.n_interval;
+    return GTP_OK;
+  }
+  if (b->load_words + words > b->budget) {
+    const gtp_status st = batch_flush(b);
+    if (st != GTP_OK) return st;
+  }
+  // room for the words: grown by doubling, the words so far copied over
+  const int64_t need = b->words_used + E + EV_PAD;
+  if ((int64_t)b->d_words.n < need) {
+    HIPCHK(x, hipStreamSynchronize(x->up));
+    DArr<uint32_t> nw;
+    HIPCHK(x, nw.alloc((size_t)std::max<int64_t>(need, std::min<int64_t>(2 * (int64_t)b->d_words.n, b->budget + EV_PAD))));
+    if (b->words_used > 0)
+      HIPCHK(x, hipMemcpyAsync(nw.p, b->d_words.p, sizeof(uint32_t) * b->words_used, hipMemcpyDeviceToDevice, x->up));
+    HIPCHK(x, hipStreamSynchronize(x->up));
+    std::swap(nw.p, b->d_words.p);
+    std::swap(nw.n, b->d_words.n);
+  }
+  const int64_t* tm = tr->event_time;
+  const uint8_t* val = tr->event_value;
+  {
+    const gtp_status st = staged_upload(x, off, N, b->d_words.p + b->words_used,
+                                        [&](int64_t n, uint32_t* d) {
+                                          for (int64_t i = off[n]; i < off[n + 1]; ++i)
+                                            *d++ = ((uint32_t)tm[i] << EV_SHIFT) | (uint32_t)(val[i] & 3u);
+                                        },
+                                        b->ready);
+    if (st != GTP_OK) return st;
+  }
+  // the staged copies read the caller's arrays only through the staging
+  // buffers, which are filled before staged_upload returns
+  b->words_used += E;
+  b->load_words += words;
+  gtp_batch::Pend P;
+  P.off.assign(off, off + N + 1);
+  P.t_max = tr->t_max;
+  P.C = C;
+  P.n_tile = n_tile;
+  P.res = res;
+  b->pend.push_back(std::move(P));
+  parallel_for(G, x->threads, [&](int64_t g) {
+    int64_t w = 0;
+    for (int64_t q = x->gate_pin_off[g]; q < x->gate_pin_off[g + 1]; ++q) {
+      const int32_t net = x->pin_net[q];
+      if (net >= 0) w += off[net + 1] - off[net];
+    }
+    b->work[g] += w;
+  });
+  return GTP_OK;
+}
+
+}  // namespace
+
+GTP_API gtp_status gtp_batch_begin(gtp_ctx* x, double time_scale, double clk_period, gtp_batch** out) {
+  if (!x || !out) return GTP_ERR_ARG;
+  *out = nullptr;
+  if (!x->have_design) return x->fail(GTP_ERR_STATE, "gtp_batch_begin: call gtp_set_design first");
+  if (!(time_scale > 0) || !(clk_period > 0))
+    return x->fail(GTP_ERR_ARG, "gtp_batch_begin: time_scale and clk_period must be positive");
+  HIPCHK(x, hipSetDevice(x->device));
+  auto* b = new gtp_batch();
+  b->x = x;
+  b->time_scale = time_scale;
+  b->clk_period = clk_period;
+  b->T = ticks_per_cycle(clk_period, time_scale);
+  if (b->T < 1 || b->T >= (int64_t)EV_TIME_INF) {
+    delete b;
+    return x->fail(GTP_ERR_UNSUPPORTED, "gtp_batch_begin: ticks per cycle out of range");
+  }
+  b->TC = multi_tile_cycles(x, b->T);
+  size_t free_b = 0, total_b = 0;
+  gtp_status st = GTP_OK;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) st = x->fail(GTP_ERR_DEVICE, "gtp_batch_begin: hipMemGetInfo");
+  b->budget = x->opt.max_event_num > 0 ? x->opt.max_event_num : (int64_t)((double)free_b * 0.45 / 4.0);
+  if (st == GTP_OK) st = staging_init(x);
+  if (st == GTP_OK && hipEventCreateWithFlags(&b->ready, hipEventDisableTiming) != hipSuccess)
+    st = x->fail(GTP_ERR_DEVICE, "gtp_batch_begin: hipEventCreate");
+  if (st != GTP_OK) {
+    if (b->ready) (void)hipEventDestroy(b->ready);
+    delete b;
+    return st;
+  }
+  b->work.assign(x->G, 0);
+  *out = b;
+  return GTP_OK;
+}
+
+GTP_API gtp_status gtp_batch_add(gtp_batch* b, const gtp_trace* trace, gtp_result* result) {
+  if (!b) return GTP_ERR_ARG;
+  if (b->st != GTP_OK) return b->st;
+  if (!trace || !result) return b->st = b->x->fail(GTP_ERR_ARG, "gtp_batch_add: no trace or result");
+  gtp_ctx* x = b->x;
+  if (hipSetDevice(x->device) != hipSuccess) return b->st = x->fail(GTP_ERR_DEVICE, "gtp_batch_add: hipSetDevice");
+  b->st = batch_add(b, trace, result);
+  if (b->st == GTP_OK) b->n_added += 1;
+  return b->st;
+}
+
+GTP_API gtp_status gtp_batch_end(gtp_batch* b, double* kernel_ms) {
+  if (!b) return GTP_ERR_ARG;
+  gtp_ctx* x = b->x;
+  gtp_status st = b->st;
+  if (st == GTP_OK) st = hipSetDevice(x->device) == hipSuccess ? batch_flush(b)
+                                                               : x->fail(GTP_ERR_DEVICE, "gtp_batch_end: hipSetDevice");
+  if (st == GTP_OK) {
+    if (kernel_ms) *kernel_ms = b->kernel_ms;
+    x->sched.n_interval = b->n_load;
+    x->sched.n_event = b->n_event;
+  }
+  (void)hipStreamSynchronize(x->up);
+  if (b->ready) (void)hipEventDestroy(b->ready);
+  delete b;
+  return st;
 }
 
 GTP_API gtp_status gtp_get_schedule(const gtp_ctx* x, gtp_schedule_info* info) {

@@ -177,6 +177,56 @@ class HipPower:
                                    r.total_ms, sched))
         return res
 
+    def power_batch_stream(self, traces, want=("gate", "cycle", "counts"), release=None):
+        """Evaluate traces handed over one at a time (gtp_batch_begin / add /
+        end): `traces` may be a generator, each trace is packed onto the
+        device when it is added and `release(trace)` (if given) is called right
+        after, so that only one trace need be held at a time.  Returns (one
+        PowerResult per trace, the batch's device time in ms)."""
+        G = self.n_gate
+        h = C.c_void_p()
+        outs, rs = [], []
+        it = iter(traces)
+        first = next(it, None)
+        if first is None:
+            return [], 0.0
+        self._check(self._lib.gtp_batch_begin(self._h, first.time_scale, first.clk_period, C.byref(h)),
+                    "gtp_batch_begin")
+        ended = False
+        try:
+            tr = first
+            while tr is not None:
+                v = tr.as_abi()
+                Cn = -(-tr.t_max // tr.ticks_per_cycle)
+                o = dict(gp=np.zeros(5 * G) if "gate" in want else None,
+                         cp=np.zeros(5 * Cn) if "cycle" in want else None,
+                         tot=np.zeros(5), cls=np.zeros(25),
+                         tg=np.zeros(G, np.uint64) if "counts" in want else None,
+                         gl=np.zeros(G, np.uint64) if "counts" in want else None, C=Cn)
+                r = abi.GtpResult(abi.ptr(o["gp"], C.c_double), abi.ptr(o["cp"], C.c_double),
+                                  abi.ptr(o["tot"], C.c_double), abi.ptr(o["cls"], C.c_double),
+                                  abi.ptr(o["tg"], C.c_uint64), abi.ptr(o["gl"], C.c_uint64), 0, 0.0, 0.0)
+                outs.append(o)
+                rs.append(r)
+                self._check(self._lib.gtp_batch_add(h, C.byref(v), C.byref(r)), "gtp_batch_add")
+                if release is not None:
+                    release(tr)
+                tr = next(it, None)
+            ms = C.c_double(0.0)
+            ended = True
+            self._check(self._lib.gtp_batch_end(h, C.byref(ms)), "gtp_batch_end")
+        finally:
+            if not ended:
+                self._lib.gtp_batch_end(h, None)
+        si = abi.GtpScheduleInfo()
+        self._check(self._lib.gtp_get_schedule(self._h, C.byref(si)), "gtp_get_schedule")
+        sched = {k: getattr(si, k) for k, _ in abi.GtpScheduleInfo._fields_}
+        res = [PowerResult(o["gp"].reshape(5, G) if o["gp"] is not None else None,
+                           o["cp"].reshape(5, o["C"]) if o["cp"] is not None else None,
+                           o["tot"], o["cls"].reshape(5, 5), o["tg"], o["gl"], r.n_cycle, r.kernel_ms,
+                           r.total_ms, sched) for o, r in zip(outs, rs)]
+        return res, ms.value
+
     def power_sliced(self, trace, max_events=0, want=("gate", "cycle", "counts")):
         """Evaluate a host trace in cycle intervals whose device event store
         stays within max_events (gtp_run_sliced): for traces larger than

@@ -247,6 +247,29 @@ gtp_status gtp_run_batch(gtp_ctx* ctx, int32_t n_trace, const gtp_trace* traces,
  * (host slicing and uploads included). */
 gtp_status gtp_run_sliced(gtp_ctx* ctx, const gtp_trace* trace, int64_t max_events, gtp_result* result);
 
+/* Streamed batch: traces handed over one at a time, so that the caller holds
+ * only the trace it is adding (a batch larger than host memory can be read
+ * trace by trace from its files).  gtp_batch_add packs the trace into event
+ * words on the device (through page-locked staging) before it returns, after
+ * which the caller may free the trace; its result is filled when the load it
+ * joined is evaluated -- when the next trace does not fit the event budget
+ * (max_event_num, else device memory), or in gtp_batch_end -- so result
+ * pointers must stay valid until gtp_batch_end returns.  A trace of 2^29 - 1
+ * ticks or more, or larger than one load, is evaluated by itself in cycle
+ * intervals (as gtp_run_sliced).  Every result equals gtp_run's for its trace;
+ * a result's kernel_ms is the device time of the load it was evaluated in.
+ * After an error, later calls return it; gtp_batch_end frees the batch in
+ * every case and returns the first error, or fills *kernel_ms (may be NULL)
+ * with the device time of the whole batch.  The resident trace of
+ * gtp_set_trace is not touched.  Replaces the reference's input rounds
+ * (power/cuda/CudaPower.cu:1261-1314, the reader's bounded buffers
+ * power/VcdReader.cc:292-300) for many traces; the batch itself is a new
+ * method. */
+typedef struct gtp_batch gtp_batch;
+gtp_status gtp_batch_begin(gtp_ctx* ctx, double time_scale, double clk_period, gtp_batch** out);
+gtp_status gtp_batch_add(gtp_batch* batch, const gtp_trace* trace, gtp_result* result);
+gtp_status gtp_batch_end(gtp_batch* batch, double* kernel_ms);
+
 /* Schedule of the last run. */
 typedef struct {
   int32_t tile_cycles;   /* cycles per work tile */
