@@ -215,7 +215,8 @@ def load_synth():
     global _SYNTH
     if _SYNTH is not None:
         return _SYNTH
-    if not SYNTH_SO.exists():
-        raise RuntimeError(f"generator not built: {SYNTH_SO} is missing (run gtpower_amd/build.py)")
-    _SYNTH = C.CDLL(str(SYNTH_SO))
+    path = Path(os.environ.get("GTPOWER_SYNTH_SO", SYNTH_SO))
+    if not path.exists():
+        raise RuntimeError(f"generator not built: {path} is missing (run gtpower_amd/build.py)")
+    _SYNTH = C.CDLL(str(path))
     return _SYNTH

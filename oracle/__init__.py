@@ -5,13 +5,15 @@ __graft_entry__.smoke()) and the CPU baseline of bench.py.  Product code never
 imports this package.
 """
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
 from gtpower_amd import abi
 
-SO = Path(__file__).resolve().parent / "liboracle.so"
+SO = Path(os.environ.get("GTPOWER_ORACLE_SO", Path(__file__).resolve().parent / "liboracle.so"))
+SO_F32 = Path(__file__).resolve().parent / "liboracle_f32.so"
 
 
 class OracleResult(C.Structure):
@@ -21,6 +23,24 @@ class OracleResult(C.Structure):
 
 
 _LIB = None
+_LIB_F32 = None
+
+
+def lib_f32():
+    """The power path built with FP32 values (energy terms, per-gate sums and
+    per-cycle waveforms in float, as the reference's CPU path): the
+    FP32-versus-FP64 error-budget diagnostic, never the oracle."""
+    global _LIB_F32
+    if _LIB_F32 is None:
+        if not SO_F32.exists():
+            raise RuntimeError(f"FP32 oracle build missing: {SO_F32} (python -m gtpower_amd.build --oracle)")
+        L = C.CDLL(str(SO_F32))
+        L.oracle_run.argtypes = [C.POINTER(abi.GtpLibrary), C.POINTER(abi.GtpDesign),
+                                 C.POINTER(abi.GtpTrace), C.c_int32, C.c_int64, C.c_int64,
+                                 C.POINTER(OracleResult)]
+        L.oracle_run.restype = C.c_int32
+        _LIB_F32 = L
+    return _LIB_F32
 
 
 def lib():
@@ -62,8 +82,9 @@ def lib():
     return _LIB
 
 
-def run(library, design, trace, threads=0, gate_begin=0, gate_end=-1):
-    """Evaluate the design on the trace; returns a dict of numpy arrays."""
+def run(library, design, trace, threads=0, gate_begin=0, gate_end=-1, fp32=False):
+    """Evaluate the design on the trace; returns a dict of numpy arrays.
+    fp32=True runs the FP32 diagnostic build instead of the FP64 oracle."""
     G = design.n_gate if hasattr(design, "n_gate") else design.as_abi().n_gate
     T = trace.ticks_per_cycle
     Cn = -(-trace.t_max // T)
@@ -75,7 +96,8 @@ def run(library, design, trace, threads=0, gate_begin=0, gate_end=-1):
                      abi.ptr(out["gate_toggle_x2"], C.c_uint64), abi.ptr(out["gate_glitch"], C.c_uint64),
                      0, 0.0)
     lv, dv, tv = library.as_abi(), design.as_abi(), trace.as_abi()
-    st = lib().oracle_run(C.byref(lv), C.byref(dv), C.byref(tv), threads, gate_begin, gate_end, C.byref(r))
+    L = lib_f32() if fp32 else lib()
+    st = L.oracle_run(C.byref(lv), C.byref(dv), C.byref(tv), threads, gate_begin, gate_end, C.byref(r))
     if st != 0:
         raise abi.GtpError(st, "oracle_run")
     assert r.n_cycle == Cn, (r.n_cycle, Cn)

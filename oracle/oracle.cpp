@@ -15,6 +15,12 @@
 //   * a per-cycle contribution whose cycle index is >= n_cycle (an event at
 //     t_max when t_max is a multiple of T) is dropped instead of written one
 //     past the end (power/TimeBasedPower.cc:690-699 writes index C).
+//
+// Built with -DORACLE_REAL=float (liboracle_f32.so) the energy terms, the
+// per-gate sums and the per-thread per-cycle waveforms are FP32, as the
+// reference's PowerVal / EnergyVal (util/Types.hh:3-18); table interpolation
+// stays FP64 inside with an FP32 result, as OpenSTA's findValue.  That build
+// measures the FP32-versus-FP64 error budget; the FP64 build is the oracle.
 #include "oracle.h"
 
 #include <algorithm>
@@ -25,8 +31,13 @@
 #include <thread>
 #include <vector>
 
+#ifndef ORACLE_REAL
+#define ORACLE_REAL double
+#endif
+
 namespace {
 
+typedef ORACLE_REAL real;  // energy and power values (FP64 oracle, FP32 diagnostic)
 constexpr int RISE = 0, FALL = 1;
 constexpr int ARC_FRAC_BITS = 16;
 
@@ -192,10 +203,11 @@ struct GateScratch {
 };
 
 struct Acc {
-  std::vector<double> cyc, cyc_abs;  // [5][C]
+  std::vector<real> cyc;       // [5][C]
+  std::vector<double> cyc_abs;  // [5][C]
 };
 
-void eval_gate(const Ctx& x, int64_t g, GateScratch& s, Acc& acc, double* gp, double* ga,
+void eval_gate(const Ctx& x, int64_t g, GateScratch& s, Acc& acc, real* gp, double* ga,
                uint64_t* tog, uint64_t* gl) {
   const gtp_design* d = x.d;
   const gtp_trace* tr = x.tr;
@@ -205,9 +217,10 @@ void eval_gate(const Ctx& x, int64_t g, GateScratch& s, Acc& acc, double* gp, do
   const int64_t pb = d->gate_pin_offset[g];
   const int64_t ab = d->gate_arc_offset[g];
   const int64_t T = x.T, C = x.C;
-  const double scale = x.scale, period = x.period;
+  const double scale = x.scale, period_d = x.period;
   const int64_t t_max = tr->t_max;
-  const double denom = (double)t_max * scale;  // vcd_.timeMax() * vcd_.timeScale()
+  const double denom_d = (double)t_max * scale;  // vcd_.timeMax() * vcd_.timeScale()
+  const real period = (real)period_d, denom = (real)denom_d;
 
   const int64_t* tt[GTP_MAX_CELL_PINS];
   const uint8_t* vv[GTP_MAX_CELL_PINS];
@@ -232,16 +245,16 @@ void eval_gate(const Ctx& x, int64_t g, GateScratch& s, Acc& acc, double* gp, do
     if (!((ci.clock_mask >> p) & 1u) && ne[p] > 0) glitch_flags(tt[p], ne[p], T, s.flags[p].data());
   }
   // single switching energies C*V^2 of output pins (TimeBasedPower.cc:749-760)
-  double sw[GTP_MAX_CELL_PINS];
+  real sw[GTP_MAX_CELL_PINS];
   for (int p = 0; p < n; ++p)
-    sw[p] = (p >= nin) ? d->pin_load_cap[pb + p] * d->pin_voltage[pb + p] * d->pin_voltage[pb + p] : 0.0;
-  auto leak_of = [&](const std::vector<int>& st) {
+    sw[p] = (p >= nin) ? (real)(d->pin_load_cap[pb + p] * d->pin_voltage[pb + p] * d->pin_voltage[pb + p]) : (real)0;
+  auto leak_of = [&](const std::vector<int>& st) -> real {
     // power/PowerUtils.cc:67-90 findLeakageVal with getPinStatesIndex (:39-48)
-    if (!ci.leak) return ci.leak_default;
+    if (!ci.leak) return (real)ci.leak_default;
     uint32_t idx = 0;
     for (int p = 0; p < n; ++p)
       if (st[p] == 1) idx |= 1u << p;
-    return ci.leak[idx];
+    return (real)ci.leak[idx];
   };
   auto state_index = [&](const std::vector<int>& st) {
     uint32_t idx = 0;
@@ -267,17 +280,17 @@ void eval_gate(const Ctx& x, int64_t g, GateScratch& s, Acc& acc, double* gp, do
   }
   s.cur = s.prev;
   int64_t prev_time = 0;
-  double* cyc = acc.cyc.data();
+  real* cyc = acc.cyc.data();
   double* cab = acc.cyc_abs.data();
 
   // per-cycle leakage split over [t0, t1) (TimeBasedPower.cc:677-701, integer ticks)
-  auto leak_split = [&](double L, int64_t t0, int64_t t1) {
-    if (L == 0.0 || t1 <= t0) return;
+  auto leak_split = [&](real L, int64_t t0, int64_t t1) {
+    if (L == (real)0 || t1 <= t0) return;
     int64_t cidx = t0 / T;
     int64_t left = t0;
     while (cidx < C && cidx * T < t1) {
       int64_t right = std::min((cidx + 1) * T, t1);
-      double v = L * (double)(right - left) / (double)T;
+      real v = L * (real)(right - left) / (real)T;
       cyc[GTP_WAVE_LEAKAGE * C + cidx] += v;
       cab[GTP_WAVE_LEAKAGE * C + cidx] += std::fabs(v);
       left = right;
@@ -301,9 +314,9 @@ void eval_gate(const Ctx& x, int64_t g, GateScratch& s, Acc& acc, double* gp, do
       }
     }
     // leakage by the previous state (:831-843)
-    double L = leak_of(s.prev);
+    real L = leak_of(s.prev);
     leak_split(L, prev_time, cur_time);
-    gp[GTP_LEAKAGE] += L * (double)(cur_time - prev_time) / (double)t_max;
+    gp[GTP_LEAKAGE] += L * (real)(cur_time - prev_time) / (real)t_max;
     ga[GTP_LEAKAGE] += std::fabs(L) * (double)(cur_time - prev_time) / (double)t_max;
     const int64_t cidx = cur_time / T;
     const bool in_range = cidx < C;
@@ -313,14 +326,15 @@ void eval_gate(const Ctx& x, int64_t g, GateScratch& s, Acc& acc, double* gp, do
       if (!s.trig[p]) continue;
       const int pv = s.prev[p], cv = s.cur[p];
       const int tx2 = toggle_x2(pv, cv);
-      const double n_toggle = 0.5 * tx2;
+      const real n_toggle = (real)0.5 * (real)tx2;
       const int64_t ei = s.frontier[p] - 1;
-      const double ratio = glitch_ratio(tt[p], ne[p], s.flags[p].data(), ei, scale,
+      const real ratio = (real)glitch_ratio(tt[p], ne[p], s.flags[p].data(), ei, scale,
                                         d->pin_slew[2 * (pb + p)], d->pin_slew[2 * (pb + p) + 1]);
       *tog += (uint64_t)tx2;
       if (ratio > 0) *gl += 1;
       const int rf = edge(pv, cv);
-      double E = 0.0, M = 0.0;
+      real E = 0;
+      double M = 0.0;
       if (p < nin) {
         // findInputInternalVal (:341-381)
         double slew = d->pin_slew[2 * (pb + p) + rf];
@@ -328,7 +342,7 @@ void eval_gate(const Ctx& x, int64_t g, GateScratch& s, Acc& acc, double* gp, do
           for (int k : ci.groups[p]) {
             const gtp_internal_power& ip = x.lib->ipowers[k];
             if (when_holds(ip, prev_idx)) {
-              E = group_power(x, ip, rf, slew, 0.0, &M);
+              E = (real)group_power(x, ip, rf, slew, 0.0, &M);
               break;
             }
           }
@@ -361,7 +375,8 @@ void eval_gate(const Ctx& x, int64_t g, GateScratch& s, Acc& acc, double* gp, do
         if (related < 0) {
           // getDefaultOutputPinInternalEnergyVal (:511-555)
           int n_table = 0;
-          double tot = 0.0, totm = 0.0;
+          real tot = 0;
+          double totm = 0.0;
           for (int i = 0; i < nin; ++i) {
             for (int frf = 0; frf < 2; ++frf) {
               double slew = d->pin_slew[2 * (pb + i) + frf];
@@ -370,14 +385,14 @@ void eval_gate(const Ctx& x, int64_t g, GateScratch& s, Acc& acc, double* gp, do
                 const gtp_internal_power& ip = x.lib->ipowers[k];
                 if (ip.related_pin != i) continue;
                 double m = 0;
-                tot += group_power(x, ip, to_rf, slew, cap, &m);
+                tot += (real)group_power(x, ip, to_rf, slew, cap, &m);
                 totm += m;
                 ++n_table;
               }
             }
           }
           if (n_table) {
-            E = tot / n_table;
+            E = tot / (real)n_table;
             M = totm / n_table;
           }
         } else {
@@ -386,7 +401,7 @@ void eval_gate(const Ctx& x, int64_t g, GateScratch& s, Acc& acc, double* gp, do
             for (int k : ci.groups[p]) {
               const gtp_internal_power& ip = x.lib->ipowers[k];
               if (ip.related_pin == related && when_holds(ip, prev_idx)) {
-                E = group_power(x, ip, to_rf, slew, cap, &M);
+                E = (real)group_power(x, ip, to_rf, slew, cap, &M);
                 break;
               }
             }
@@ -395,23 +410,23 @@ void eval_gate(const Ctx& x, int64_t g, GateScratch& s, Acc& acc, double* gp, do
         // switching (:856-869)
         int rx2 = rise_x2(pv, cv);
         if (rx2 > 0) {
-          double Es = 0.5 * rx2 * sw[p];
+          real Es = (real)0.5 * (real)rx2 * sw[p];
           if (ratio > 0) {
-            double e = ratio * Es;
+            real e = ratio * Es;
             gp[GTP_GLITCH_SWITCHING] += e / denom;
-            ga[GTP_GLITCH_SWITCHING] += std::fabs(e) / denom;
+            ga[GTP_GLITCH_SWITCHING] += std::fabs((double)e) / denom_d;
             if (in_range) {
               cyc[GTP_WAVE_SWITCHING * C + cidx] += e / period;
               cyc[GTP_WAVE_GLITCH_SWITCHING * C + cidx] += e / period;
-              cab[GTP_WAVE_SWITCHING * C + cidx] += std::fabs(e) / period;
-              cab[GTP_WAVE_GLITCH_SWITCHING * C + cidx] += std::fabs(e) / period;
+              cab[GTP_WAVE_SWITCHING * C + cidx] += std::fabs((double)e) / period_d;
+              cab[GTP_WAVE_GLITCH_SWITCHING * C + cidx] += std::fabs((double)e) / period_d;
             }
           } else {
             gp[GTP_SWITCHING] += Es / denom;
-            ga[GTP_SWITCHING] += std::fabs(Es) / denom;
+            ga[GTP_SWITCHING] += std::fabs((double)Es) / denom_d;
             if (in_range) {
               cyc[GTP_WAVE_SWITCHING * C + cidx] += Es / period;
-              cab[GTP_WAVE_SWITCHING * C + cidx] += std::fabs(Es) / period;
+              cab[GTP_WAVE_SWITCHING * C + cidx] += std::fabs((double)Es) / period_d;
             }
           }
         }
@@ -419,21 +434,22 @@ void eval_gate(const Ctx& x, int64_t g, GateScratch& s, Acc& acc, double* gp, do
       E *= n_toggle;
       M *= n_toggle;
       if (ratio > 0) {
-        double e = ratio * E, m = ratio * M;
+        real e = ratio * E;
+        double m = (double)ratio * M;
         gp[GTP_GLITCH_INTERNAL] += e / denom;
-        ga[GTP_GLITCH_INTERNAL] += m / denom;
+        ga[GTP_GLITCH_INTERNAL] += m / denom_d;
         if (in_range) {
           cyc[GTP_WAVE_INTERNAL * C + cidx] += e / period;
           cyc[GTP_WAVE_GLITCH_INTERNAL * C + cidx] += e / period;
-          cab[GTP_WAVE_INTERNAL * C + cidx] += m / period;
-          cab[GTP_WAVE_GLITCH_INTERNAL * C + cidx] += m / period;
+          cab[GTP_WAVE_INTERNAL * C + cidx] += m / period_d;
+          cab[GTP_WAVE_GLITCH_INTERNAL * C + cidx] += m / period_d;
         }
       } else {
         gp[GTP_INTERNAL] += E / denom;
-        ga[GTP_INTERNAL] += M / denom;
+        ga[GTP_INTERNAL] += M / denom_d;
         if (in_range) {
           cyc[GTP_WAVE_INTERNAL * C + cidx] += E / period;
-          cab[GTP_WAVE_INTERNAL * C + cidx] += M / period;
+          cab[GTP_WAVE_INTERNAL * C + cidx] += M / period_d;
         }
       }
     }
@@ -442,9 +458,9 @@ void eval_gate(const Ctx& x, int64_t g, GateScratch& s, Acc& acc, double* gp, do
     s.prev = s.cur;
   }
   // final leakage (:892-897)
-  double L = leak_of(s.prev);
+  real L = leak_of(s.prev);
   leak_split(L, prev_time, t_max);
-  gp[GTP_LEAKAGE] += L * (double)(t_max - prev_time) / (double)t_max;
+  gp[GTP_LEAKAGE] += L * (real)(t_max - prev_time) / (real)t_max;
   ga[GTP_LEAKAGE] += std::fabs(L) * (double)(t_max - prev_time) / (double)t_max;
 }
 
@@ -529,16 +545,17 @@ int32_t oracle_run(const gtp_library* lib, const gtp_design* d, const gtp_trace*
   for (int t = 0; t < nth; ++t) {
     pool.emplace_back([&, t] {
       Acc& acc = accs[t];
-      acc.cyc.assign((size_t)5 * C, 0.0);
+      acc.cyc.assign((size_t)5 * C, (real)0);
       acc.cyc_abs.assign((size_t)5 * C, 0.0);
       GateScratch s;
       int64_t b = gate_begin + ng * t / nth, e = gate_begin + ng * (t + 1) / nth;
       for (int64_t g = b; g < e; ++g) {
-        double p5[5] = {0, 0, 0, 0, 0}, a5[5] = {0, 0, 0, 0, 0};
+        real p5[5] = {0, 0, 0, 0, 0};
+        double a5[5] = {0, 0, 0, 0, 0};
         uint64_t tg = 0, gg = 0;
         eval_gate(x, g, s, acc, p5, a5, &tg, &gg);
         for (int k = 0; k < 5; ++k) {
-          gp[(size_t)k * G + g] = p5[k];
+          gp[(size_t)k * G + g] = (double)p5[k];
           ga[(size_t)k * G + g] = a5[k];
         }
         tog[g] = tg;
@@ -548,12 +565,14 @@ int32_t oracle_run(const gtp_library* lib, const gtp_design* d, const gtp_trace*
   }
   for (auto& th : pool) th.join();
   // merge per-thread waveforms in thread order (Power.cc:322-329)
-  std::vector<double> cyc((size_t)5 * C, 0.0), cab((size_t)5 * C, 0.0);
+  std::vector<real> cyc_r((size_t)5 * C, (real)0);
+  std::vector<double> cab((size_t)5 * C, 0.0);
   for (int t = 0; t < nth; ++t)
-    for (size_t k = 0; k < cyc.size(); ++k) {
-      cyc[k] += accs[t].cyc[k];
+    for (size_t k = 0; k < cyc_r.size(); ++k) {
+      cyc_r[k] += accs[t].cyc[k];
       cab[k] += accs[t].cyc_abs[k];
     }
+  std::vector<double> cyc(cyc_r.begin(), cyc_r.end());
   auto t1 = std::chrono::steady_clock::now();
   out->wall_s = std::chrono::duration<double>(t1 - t0).count();
   if (out->gate_power) std::memcpy(out->gate_power, gp.data(), sizeof(double) * 5 * G);
