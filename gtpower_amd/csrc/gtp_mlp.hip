@@ -265,18 +265,48 @@ __global__ void loss_kernel(const float* __restrict__ out, const float* __restri
 }
 
 // dW3[n] = sum_m h2[m][n] dout[m]; db3 = sum_m dout[m]   (thread per n)
-__global__ void head_grad_kernel(const bf16* __restrict__ h2, const float* __restrict__ dout, int M, int H,
-                                 float* __restrict__ dw3, float* __restrict__ db3) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n < H) {
-    float s = 0.f;
-    for (int m = 0; m < M; ++m) s += bf2f(h2[(int64_t)m * H + n]) * dout[m];
-    dw3[n] = s;
+// Column sums over the rows of a [M][H] bf16 matrix, each row weighted by
+// w[m] (w null: 1): out[n] = sum_m x[m][n] w[m].  A block takes 32 columns
+// with 8 row groups (thread (g, c) sums rows g, g + 8, ... of column c, the
+// loads of a row group coalesced over 32 columns), then adds the 8 partials
+// in a fixed order: deterministic, and the M loads of a column no longer run
+// one after another in one thread.  With sum1 != null, the extra block
+// (blockIdx.x == H / 32) also writes sum_m w[m] there.
+constexpr int CR_COLS = 32, CR_GROUPS = 8;
+__global__ __launch_bounds__(CR_COLS * CR_GROUPS) void col_reduce_kernel(const bf16* __restrict__ x,
+                                                                          const float* __restrict__ w, int M,
+                                                                          int H, float* __restrict__ out,
+                                                                          float* __restrict__ sum1) {
+  __shared__ float part[CR_GROUPS][CR_COLS];
+  const int c = threadIdx.x % CR_COLS, g = threadIdx.x / CR_COLS;
+  const int n = blockIdx.x * CR_COLS + c;
+  const bool extra = sum1 && (int)blockIdx.x == H / CR_COLS;
+  float s = 0.f;
+  if (extra) {
+    for (int m = threadIdx.x; m < M; m += CR_COLS * CR_GROUPS) s += w[m];
+  } else if (n < H) {
+#pragma unroll 4
+    for (int m = g; m < M; m += CR_GROUPS) s += bf2f(x[(int64_t)m * H + n]) * (w ? w[m] : 1.f);
   }
-  if (n == 0) {
-    float s = 0.f;
-    for (int m = 0; m < M; ++m) s += dout[m];
-    db3[0] = s;
+  if (extra) {
+    // a block sum of the per-thread partials, in a fixed order
+    __shared__ float all[CR_COLS * CR_GROUPS];
+    all[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = CR_COLS * CR_GROUPS / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) all[threadIdx.x] += all[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) sum1[0] = all[0];
+    return;
+  }
+  part[g][c] = s;
+  __syncthreads();
+  if (g == 0 && n < H) {
+    float t = 0.f;
+#pragma unroll
+    for (int k = 0; k < CR_GROUPS; ++k) t += part[k][c];
+    out[n] = t;
   }
 }
 
@@ -287,15 +317,6 @@ __global__ void head_back_kernel(const float* __restrict__ dout, const float* __
   if (i >= (int64_t)M * H) return;
   const int m = (int)(i / H), n = (int)(i - (int64_t)m * H);
   dz2[i] = f2bf(dout[m] * w3[n] * gelu_grad(z2[i]));
-}
-
-// db[n] = sum_m dz[m][n]   (thread per n)
-__global__ void colsum_kernel(const bf16* __restrict__ dz, int M, int H, float* __restrict__ db) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= H) return;
-  float s = 0.f;
-  for (int m = 0; m < M; ++m) s += bf2f(dz[(int64_t)m * H + n]);
-  db[n] = s;
 }
 
 // Adam with decoupled weight decay (AdamW): p -= lr (m^ / (sqrt(v^) + eps) + wd p)
@@ -749,7 +770,9 @@ hipError_t backward(gtp_mlp* m, int n, uint32_t drop_seed = 0) {
   hipError_t e = forward(m, m->Xb, n, true, m->out, drop_seed);
   if (e != hipSuccess) return e;
   loss_kernel<<<nblk(n), 256, 0, m->stream>>>(m->out, m->yb, n, m->dout, m->lossp);
-  head_grad_kernel<<<nblk(H), 256, 0, m->stream>>>(m->h2, m->dout, n, H, m->g(4), m->g(5));
+  // dw3 = h2^T dout, db3 = sum dout
+  col_reduce_kernel<<<(H + CR_COLS - 1) / CR_COLS + 1, CR_COLS * CR_GROUPS, 0, m->stream>>>(m->h2, m->dout, n, H,
+                                                                                          m->g(4), m->g(5));
   head_back_kernel<<<nblk((int64_t)n * H), 256, 0, m->stream>>>(m->dout, m->w3(), m->z2, n, H, m->dz2);
   // dW2 = dz2^T h1
   GemmArgs a{};
@@ -758,7 +781,8 @@ hipError_t backward(gtp_mlp* m, int n, uint32_t drop_seed = 0) {
   a.B = m->h1; a.ldb = H;
   a.C = m->g(2); a.ldc = H;
   if ((e = gemm<A_COL, B_ROW, EPI_F32>(a, m->stream)) != hipSuccess) return e;
-  colsum_kernel<<<nblk(H), 256, 0, m->stream>>>(m->dz2, n, H, m->g(3));
+  col_reduce_kernel<<<(H + CR_COLS - 1) / CR_COLS, CR_COLS * CR_GROUPS, 0, m->stream>>>(m->dz2, nullptr, n, H,
+                                                                                      m->g(3), nullptr);
   // dz1 = (dz2 W2) * GELU'(z1)
   GemmArgs b{};
   b.M = n; b.N = H; b.K = H;
@@ -767,7 +791,8 @@ hipError_t backward(gtp_mlp* m, int n, uint32_t drop_seed = 0) {
   b.Cb = m->dz1; b.ldcb = H;
   b.zin = m->z1; b.ldz = H;
   if ((e = gemm<A_ROW, B_ROW, EPI_GELU_BACK>(b, m->stream)) != hipSuccess) return e;
-  colsum_kernel<<<nblk(H), 256, 0, m->stream>>>(m->dz1, n, H, m->g(1));
+  col_reduce_kernel<<<(H + CR_COLS - 1) / CR_COLS, CR_COLS * CR_GROUPS, 0, m->stream>>>(m->dz1, nullptr, n, H,
+                                                                                      m->g(1), nullptr);
   // dW1 = dz1^T X
   GemmArgs c{};
   c.M = H; c.N = m->K; c.K = n;
