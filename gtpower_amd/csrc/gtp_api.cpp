@@ -2111,7 +2111,10 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
   DArr<uint32_t> d_words[2];
   HIPCHK(x, d_words[0].alloc(max_words + EV_PAD));
   int n_buf = 1;
-  if (ivals.size() > 1) {
+  bool two = ivals.size() > 1;
+  if (const char* e = std::getenv("GTP_SLICE_BUFFERS"))
+    if (std::atoi(e) == 1) two = false;
+  if (two) {
     size_t free_b = 0, total_b = 0;
     HIPCHK(x, hipMemGetInfo(&free_b, &total_b));
     if ((double)(max_words + EV_PAD) * 4.0 <= 0.5 * (double)free_b) {

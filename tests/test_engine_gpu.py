@@ -217,14 +217,18 @@ def test_batch_equals_individual_runs(opts):
         assert np.array_equal(a.total, b.total), k
 
 
-@pytest.mark.parametrize("parts", [1, 3, 11])
-def test_sliced_equals_full_run(parts):
+@pytest.mark.parametrize("parts,buffers", [(1, 2), (3, 2), (11, 2), (11, 1)])
+def test_sliced_equals_full_run(parts, buffers, monkeypatch):
     """gtp_run_sliced with an event budget that forces about `parts`
     intervals gives exactly what one gtp_run over the whole trace gives:
     counts and every power value bit-identical (the slices carry the events
     that the state, glitch flags and related-pin choice at their edges need;
-    per-gate sums are exact integers over intervals)."""
-    print(f"[sliced] 2000 gates x 1500 cycles, ~{parts} intervals", flush=True)
+    per-gate sums are exact integers over intervals).  With two device
+    buffers the next interval is packed and uploaded while one runs; with one
+    they take turns."""
+    if buffers == 1:
+        monkeypatch.setenv("GTP_SLICE_BUFFERS", "1")
+    print(f"[sliced] 2000 gates x 1500 cycles, ~{parts} intervals, {buffers} buffer(s)", flush=True)
     d = S.make_design(2000, dff_fraction=0.15, clock_ns=2.0, seed=45)
     tr = d.trace(1500, toggle_p=0.2, seed=46)
     with HipPower(tile_cycles=64) as eng:
