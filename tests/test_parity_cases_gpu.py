@@ -90,6 +90,29 @@ def test_small_workload_full_size():
     assert res.gate_glitch.sum() > 0
 
 
+@pytest.mark.parametrize("name,gates,cycles,autotune", [("exact_200k", 20_000, 3_000, 1),
+                                                           ("skew_1m", 50_000, 2_000, 1)])
+def test_scaled_workloads(name, gates, cycles, autotune):
+    """The exact-at-scale and skewed-activity workloads with their generators,
+    mixes and activity models, at 1/10 and 1/20 of their gate counts (the
+    oracle must finish in seconds), with the schedule the autotuner picks on
+    them, against the oracle; then in intervals (gtp_run_sliced) bit-identical
+    to the resident run."""
+    print(f"[parity] {name} scaled to {gates} gates x {cycles} cycles", flush=True)
+    d, tr = S.workload(name, scale_gates=gates, scale_cycles=cycles)
+    with HipPower(autotune=autotune) as eng:
+        eng.set_design(d.library, d)
+        eng.read_power_activities(tr)
+        res = eng.power()
+        sl = eng.power_sliced(tr, max_events=tr.n_event // 5 + 4 * d.n_net)
+    ref = oracle.run(d.library, d, tr, threads=0)
+    compare(res, ref, name)
+    assert res.gate_glitch.sum() > 0
+    assert sl.schedule["n_interval"] >= 2
+    assert np.array_equal(sl.cycle_power, res.cycle_power)
+    assert np.array_equal(sl.gate_power, res.gate_power)
+
+
 def _float6(v):
     """A float32 value printed as an ostream prints it (6 significant digits)."""
     return float("%g" % float(np.float32(v)))
