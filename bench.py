@@ -279,11 +279,13 @@ def vcd_leg(design, w, n_cycle, steps, cpu):
     return res
 
 
-def surrogate_leg(eng, design, exact_result, w, n_fresh, n_windows, exact_wall_cps, exact_device_cps, judge_tile=64):
+def surrogate_leg(eng, design, exact_result, w, n_fresh, n_windows, exact_wall_cps, exact_device_cps, judge_tile=64,
+                  judge_fraction=0.005):
     """Train on the first 50k exact cycles of the loaded trace, then evaluate
     n_windows fresh traces of n_fresh cycles each (new seeds: cycles the
-    network never saw), each judged on 0.5% of its cycles; the full exact answer
-    of every window is computed as well, untimed, for the pooled NRMSE."""
+    network never saw), each judged on judge_fraction of its cycles; the full
+    exact answer of every window is computed as well, untimed, for the pooled
+    NRMSE."""
     import numpy as np
     from gtpower_amd import synthetic as S
     from gtpower_amd.surrogate import MlpSurrogate, SurrogatePower, cycle_total, nrmse
@@ -294,7 +296,7 @@ def surrogate_leg(eng, design, exact_result, w, n_fresh, n_windows, exact_wall_c
     # the judge recomputes 0.5% of the cycles (8 tiles of 64 cycles per
     # 100k-cycle window, plus tile 0): ~500 cycles pin the NRMSE to about 3%
     # of itself, well inside the 10% margin of the relative bar
-    sp = SurrogatePower(MlpSurrogate(4096, 256, seed=3), threshold=0.02, judge_fraction=0.005,
+    sp = SurrogatePower(MlpSurrogate(4096, 256, seed=3), threshold=0.02, judge_fraction=judge_fraction,
                         judge_tile=judge_tile)
     t0 = time.perf_counter()
     sp.fit(eng, train_mask=mask, exact=exact_result)
@@ -403,6 +405,8 @@ def main():
                     help="longest batch trace (the host generator holds a whole trace: ~26 B per event)")
     ap.add_argument("--surrogate-cycles", type=int, default=100000)
     ap.add_argument("--judge-tile", type=int, default=64, help="cycles per tile the exact judge recomputes")
+    ap.add_argument("--judge-fraction", type=float, default=0.005,
+                    help="fraction of each window's tiles the exact judge recomputes (plus tile 0)")
     ap.add_argument("--surrogate-windows", type=int, default=10,
                     help="fresh traces the surrogate evaluates (of --surrogate-cycles cycles each)")
     ap.add_argument("--vcd-cycles", type=int, default=2000, help="cycles of the VCD ingest leg (0: off)")
@@ -505,7 +509,7 @@ def main():
         surrogate = surrogate_leg(eng, design, r, w,
                                   min(args.surrogate_cycles, args.cycles or args.surrogate_cycles),
                                   args.surrogate_windows, value, n_cycle / (avg_kernel_ms * 1e-3),
-                                  judge_tile=args.judge_tile)
+                                  judge_tile=args.judge_tile, judge_fraction=args.judge_fraction)
     # after the surrogate leg: the batch replaces the resident trace
     batch = None
     if args.batch_traces > 0:
