@@ -189,16 +189,18 @@ struct gtp_ctx {
   DArr<unsigned int> d_maxbits;
 
   // page-locked staging of packed event words for loads from host memory
-  // (sliced runs): a ring of buffers, each refilled once its copy to the
-  // device has completed, so that packing the next part overlaps the copy of
-  // the last one; the copies run on their own stream (created on first use)
+  // (sliced runs, streamed batches): a ring of buffers, each refilled once its
+  // copy to the device has completed, so that packing the next part overlaps
+  // the copy of the last one.  The copies go on the run stream: a stream of
+  // their own would take one of the process's few hardware queues and share
+  // it with the run streams or the surrogate's feature work, which then
+  // serialise (the surrogate's overlapped evaluation slowed by ~10%)
   static constexpr int N_STAGE = 4;
   static constexpr int64_t STAGE_WORDS = (int64_t)1 << 25;  // 128 MiB a buffer
   Pinned<uint32_t> stage[N_STAGE];
   hipEvent_t stage_ev[N_STAGE] = {nullptr, nullptr, nullptr, nullptr};
   bool stage_pending[N_STAGE] = {false, false, false, false};
   int stage_next = 0;
-  hipStream_t up = nullptr;
 
   gtp_status fail(gtp_status st, const std::string& m) {
     err = m;
@@ -445,10 +447,6 @@ GTP_API void gtp_destroy(gtp_ctx* x) {
     if (x->join[k]) (void)hipEventDestroy(x->join[k]);
   }
   if (x->fork) (void)hipEventDestroy(x->fork);
-  if (x->up) {
-    (void)hipStreamSynchronize(x->up);
-    (void)hipStreamDestroy(x->up);
-  }
   for (int k = 0; k < gtp_ctx::N_STAGE; ++k)
     if (x->stage_ev[k]) (void)hipEventDestroy(x->stage_ev[k]);
   delete x;
@@ -1760,7 +1758,6 @@ gtp_status finish_slot(gtp_ctx* x, const MultiPlan& mp, int32_t slot, const DArr
 
 // Creates the staging ring and its stream on first use.
 gtp_status staging_init(gtp_ctx* x) {
-  if (!x->up) HIPCHK(x, hipStreamCreate(&x->up));
   for (int k = 0; k < gtp_ctx::N_STAGE; ++k) {
     if (!x->stage_ev[k]) HIPCHK(x, hipEventCreateWithFlags(&x->stage_ev[k], hipEventDisableTiming));
     HIPCHK(x, x->stage[k].reserve((size_t)gtp_ctx::STAGE_WORDS));
@@ -1788,8 +1785,8 @@ gtp_status staged_upload(gtp_ctx* x, const int64_t* so, int64_t N, uint32_t* dst
       big.resize((size_t)(so[n0 + 1] - so[n0]));
       pack_net(n0, big.data());
       HIPCHK(x, hipMemcpyAsync(dst + so[n0], big.data(), big.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
-                               x->up));
-      HIPCHK(x, hipStreamSynchronize(x->up));
+                               x->stream));
+      HIPCHK(x, hipStreamSynchronize(x->stream));
       n0 += 1;
       continue;
     }
@@ -1802,13 +1799,13 @@ gtp_status staged_upload(gtp_ctx* x, const int64_t* so, int64_t N, uint32_t* dst
     parallel_for(n1 - n0, x->threads, [&](int64_t i) { pack_net(n0 + i, h + (so[n0 + i] - base)); });
     if (so[n1] > base) {
       HIPCHK(x, hipMemcpyAsync(dst + base, h, (size_t)(so[n1] - base) * sizeof(uint32_t), hipMemcpyHostToDevice,
-                               x->up));
-      HIPCHK(x, hipEventRecord(x->stage_ev[b], x->up));
+                               x->stream));
+      HIPCHK(x, hipEventRecord(x->stage_ev[b], x->stream));
       x->stage_pending[b] = true;
     }
     n0 = n1;
   }
-  HIPCHK(x, hipEventRecord(done, x->up));
+  HIPCHK(x, hipEventRecord(done, x->stream));
   return GTP_OK;
 }
 
@@ -2308,12 +2305,12 @@ gtp_status batch_add(gtp_batch* b, const gtp_trace* tr, gtp_result* res) {
   // room for the words: grown by doubling, the words so far copied over
   const int64_t need = b->words_used + E + EV_PAD;
   if ((int64_t)b->d_words.n < need) {
-    HIPCHK(x, hipStreamSynchronize(x->up));
+    HIPCHK(x, hipStreamSynchronize(x->stream));
     DArr<uint32_t> nw;
     HIPCHK(x, nw.alloc((size_t)std::max<int64_t>(need, std::min<int64_t>(2 * (int64_t)b->d_words.n, b->budget + EV_PAD))));
     if (b->words_used > 0)
-      HIPCHK(x, hipMemcpyAsync(nw.p, b->d_words.p, sizeof(uint32_t) * b->words_used, hipMemcpyDeviceToDevice, x->up));
-    HIPCHK(x, hipStreamSynchronize(x->up));
+      HIPCHK(x, hipMemcpyAsync(nw.p, b->d_words.p, sizeof(uint32_t) * b->words_used, hipMemcpyDeviceToDevice, x->stream));
+    HIPCHK(x, hipStreamSynchronize(x->stream));
     std::swap(nw.p, b->d_words.p);
     std::swap(nw.n, b->d_words.n);
   }
@@ -2408,7 +2405,7 @@ GTP_API gtp_status gtp_batch_end(gtp_batch* b, double* kernel_ms) {
     x->sched.n_interval = b->n_load;
     x->sched.n_event = b->n_event;
   }
-  (void)hipStreamSynchronize(x->up);
+  (void)hipStreamSynchronize(x->stream);
   if (b->ready) (void)hipEventDestroy(b->ready);
   delete b;
   return st;
