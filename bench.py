@@ -405,6 +405,9 @@ def main():
                     help="longest batch trace (the host generator holds a whole trace: ~26 B per event)")
     ap.add_argument("--surrogate-cycles", type=int, default=100000)
     ap.add_argument("--judge-tile", type=int, default=64, help="cycles per tile the exact judge recomputes")
+    ap.add_argument("--judge-probe", type=int, default=0, metavar="N",
+                    help="time the exact judge (gtp_run_tiles) alone on N random tiles of --judge-tile cycles of "
+                         "the bench trace, against the full run's cost per cycle")
     ap.add_argument("--judge-fraction", type=float, default=0.005,
                     help="fraction of each window's tiles the exact judge recomputes (plus tile 0)")
     ap.add_argument("--surrogate-windows", type=int, default=10,
@@ -477,6 +480,20 @@ def main():
     if tf.exists() and args.gates is None and args.cycles is None:
         traffic = json.loads(tf.read_text())["traffic_bytes"]
         traffic_src = str(tf.relative_to(REPO))
+    judge_probe = None
+    if args.judge_probe > 0:
+        # the judge's cost per cycle beside the full run's: N random tiles,
+        # the trace already annotated by the timed runs
+        rng = np.random.default_rng(S.MASTER_SEED)
+        n_tile = -(-n_cycle // args.judge_tile)
+        tiles = np.sort(rng.choice(n_tile, min(args.judge_probe, n_tile), replace=False)).astype(np.int32)
+        jms = [eng.power_tiles(args.judge_tile, tiles)[2] for _ in range(4)][1:]
+        jm = float(np.median(jms))
+        jc = len(tiles) * args.judge_tile
+        judge_probe = {"tiles": len(tiles), "tile_cycles": args.judge_tile, "cycles": jc, "device_ms": jm,
+                       "us_per_cycle": jm * 1e3 / jc, "full_run_us_per_cycle": avg_kernel_ms * 1e3 / n_cycle}
+        log(f"judge probe: {len(tiles)} tiles x {args.judge_tile} cycles in {jm:.3f} ms = "
+            f"{jm * 1e3 / jc:.3f} us/cycle (full run {avg_kernel_ms * 1e3 / n_cycle:.3f} us/cycle)")
     sliced = None
     if args.sliced_parts > 1:
         # the same trace from host memory in cycle intervals under an event
@@ -570,6 +587,7 @@ def main():
         "batch": batch,
         "vcd_ingest": vcd,
         "sliced": sliced,
+        "judge_probe": judge_probe,
     }
     print(json.dumps(out), flush=True)
 

@@ -18,6 +18,12 @@ VALUUtilization (100 SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)); with
 --steps (a JSON of pin-event steps per evaluation by kernel name, written by
 bench.py --steps-json) also VALU instructions per 64-lane step.
 
+`tiles` reads one `rocprofv3 --pmc` database and reports the kernels of the
+last tile-subset run (gtp_run_tiles, the surrogate's judge: from its
+tile_index_kernel to its cycle_reduce_list_kernel) per dispatch, and their
+wave VALU instructions per judged cycle beside a full run's (--full-valu
+instructions per --full-cycles cycles).
+
 `pmc` reads two `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` databases of
 the same program, and two of plain device copies of a known size, and reports
 HBM traffic per evaluation window: the counters (KiB per dispatch) summed over
@@ -232,6 +238,40 @@ def counters_summary(dbs, steps=None, n_cu=256, n_xcd=8):
     return lines, res
 
 
+def tiles_summary(db, judged_cycles, full_valu, full_cycles):
+    c = sqlite3.connect(db)
+    rows = c.execute("select dispatch_id, kernel_name, counter_name, sum(value) from counters_collection "
+                     "group by dispatch_id, counter_name").fetchall()
+    by, names = {}, {}
+    for did, name, ctr, v in rows:
+        by.setdefault(did, {})[ctr] = v
+        names[did] = name
+    ids = sorted(by)
+    ends = [i for i in ids if "cycle_reduce_list_kernel" in names[i]]
+    if not ends:
+        return ["no tile-subset run in the database"], {}
+    end = ends[-1]
+    start = max(i for i in ids if i < end and "tile_index_kernel" in names[i])
+    lines = [f"{'kernel':34s} {'waves':>7s} {'VALU/wave':>10s} {'SALU/wave':>10s} {'wait/wave':>10s} "
+             f"{'cycles/wave':>12s}"]
+    merge_valu = 0.0
+    for i in ids:
+        if i < start or i > end:
+            continue
+        d = by[i]
+        w = max(d.get("SQ_WAVES", 0.0), 1.0)
+        if any(k in names[i] for k in WINDOW_KERNELS):
+            merge_valu += d.get("SQ_INSTS_VALU", 0.0)
+        lines.append(f"{_short(names[i]).replace('gtp::', '')[:34]:34s} {w:7.0f} "
+                     f"{d.get('SQ_INSTS_VALU', 0) / w:10.0f} {d.get('SQ_INSTS_SALU', 0) / w:10.0f} "
+                     f"{d.get('SQ_WAIT_INST_ANY', 0) / w:10.0f} {d.get('SQ_WAVE_CYCLES', 0) / w:12.0f}")
+    per_k = merge_valu / judged_cycles * 1000.0
+    full_k = full_valu / full_cycles * 1000.0
+    lines += ["", f"power kernels: {merge_valu:.3g} wave VALU instructions for {judged_cycles} judged cycles = "
+                  f"{per_k:.3g} per 1000 cycles; a full run: {full_k:.3g} per 1000 cycles; ratio {per_k / full_k:.2f}"]
+    return lines, {"merge_valu": merge_valu, "per_1000_cycles": per_k, "full_per_1000_cycles": full_k}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="mode", required=True)
@@ -247,12 +287,19 @@ def main(argv=None):
     k.add_argument("dbs", nargs="+")
     k.add_argument("--steps", help="JSON: pin-event steps per evaluation by kernel name")
     k.add_argument("--json")
-    for q in (t, p, k):
+    j = sub.add_parser("tiles")
+    j.add_argument("db")
+    j.add_argument("--judged-cycles", type=int, required=True)
+    j.add_argument("--full-valu", type=float, required=True)
+    j.add_argument("--full-cycles", type=int, required=True)
+    for q in (t, p, k, j):
         q.add_argument("--out")
         q.add_argument("--cmd", default="")
     a = ap.parse_args(argv)
     if a.mode == "trace":
         lines, _ = trace_summary(a.db)
+    elif a.mode == "tiles":
+        lines, _ = tiles_summary(a.db, a.judged_cycles, a.full_valu, a.full_cycles)
     elif a.mode == "counters":
         steps = json.load(open(a.steps)) if a.steps else None
         lines, res = counters_summary(a.dbs, steps)
