@@ -2191,6 +2191,7 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
 // ---------------------------------------------------------------------------
 struct gtp_batch {
   gtp_ctx* x = nullptr;
+  int64_t G = 0, N = 0;            // the design the batch began with
   int64_t T = 0, TC = 0, budget = 0;
   double time_scale = 0, clk_period = 0;
   gtp_status st = GTP_OK;          // first error; later calls return it
@@ -2216,6 +2217,8 @@ namespace {
 gtp_status batch_flush(gtp_batch* b) {
   gtp_ctx* x = b->x;
   if (b->pend.empty()) return GTP_OK;
+  if (x->G != b->G || x->N != b->N)
+    return x->fail(GTP_ERR_STATE, "gtp_batch: the design changed during the batch");
   hipStream_t s = x->stream;
   const int64_t G = x->G;
   const int32_t nt = (int32_t)b->pend.size();
@@ -2271,6 +2274,8 @@ gtp_status batch_flush(gtp_batch* b) {
 
 gtp_status batch_add(gtp_batch* b, const gtp_trace* tr, gtp_result* res) {
   gtp_ctx* x = b->x;
+  if (x->G != b->G || x->N != b->N)
+    return x->fail(GTP_ERR_STATE, "gtp_batch_add: the design changed during the batch");
   const int64_t N = x->N, G = x->G;
   int64_t Tk = 0;
   {
@@ -2358,6 +2363,8 @@ GTP_API gtp_status gtp_batch_begin(gtp_ctx* x, double time_scale, double clk_per
   HIPCHK(x, hipSetDevice(x->device));
   auto* b = new gtp_batch();
   b->x = x;
+  b->G = x->G;
+  b->N = x->N;
   b->time_scale = time_scale;
   b->clk_period = clk_period;
   b->T = ticks_per_cycle(clk_period, time_scale);

@@ -260,7 +260,8 @@ gtp_status gtp_run_sliced(gtp_ctx* ctx, const gtp_trace* trace, int64_t max_even
  * a result's kernel_ms is the device time of the load it was evaluated in.
  * After an error, later calls return it; gtp_batch_end frees the batch in
  * every case and returns the first error, or fills *kernel_ms (may be NULL)
- * with the device time of the whole batch.  The resident trace of
+ * with the device time of the whole batch.  The design must stay the one
+ * loaded at gtp_batch_begin (GTP_ERR_STATE otherwise); the resident trace of
  * gtp_set_trace is not touched.  Replaces the reference's input rounds
  * (power/cuda/CudaPower.cu:1261-1314, the reader's bounded buffers
  * power/VcdReader.cc:292-300) for many traces; the batch itself is a new

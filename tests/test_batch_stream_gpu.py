@@ -95,3 +95,21 @@ def test_stream_rejects_mixed_clocks_and_frees():
         res, _ = eng.power_batch_stream([t1])
         eng.read_power_activities(t1)
         _same(eng.power(), res[0], 0)
+
+
+def test_stream_refuses_a_design_change():
+    """Loading another design while a streamed batch is open makes the next
+    add fail (the batch's per-gate sums are sized for the first design)."""
+    d = S.make_design(200, dff_fraction=0.1, clock_ns=10.0, seed=44)
+    d2 = S.make_design(300, dff_fraction=0.1, clock_ns=10.0, seed=45)
+    t1 = d.trace(50, toggle_p=0.2, seed=1)
+    with HipPower() as eng:
+        eng.set_design(d.library, d)
+
+        def gen():
+            yield t1
+            eng.set_design(d2.library, d2)
+            yield d2.trace(50, toggle_p=0.2, seed=2)
+
+        with pytest.raises(abi.GtpError, match="design changed"):
+            eng.power_batch_stream(gen())
