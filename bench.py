@@ -250,7 +250,23 @@ def vcd_leg(design, w, n_cycle, steps, cpu):
     with VcdText(device=0) as v:
         v.parse(text)  # warmup: the reader's scratch memory is allocated once
         infos = [v.parse(text) for _ in range(max(1, steps))]
+        one = v.events()
+        # the same text in four parts (what a text above the part size gets):
+        # the store must not change
+        part = len(text) // 4 + 1
+        os.environ["GTP_VCD_PART_BYTES"] = str(part)
+        try:
+            v.parse(text)
+            pinfos = [v.parse(text) for _ in range(max(1, steps))]
+            four = v.events()
+        finally:
+            del os.environ["GTP_VCD_PART_BYTES"]
     info = infos[-1]
+    same = all(np.array_equal(a, b) for a, b in zip(one, four))
+    del one, four
+    assert same, "vcd: the text parsed in parts gave another event store"
+    p_ms = sum(i["device_ms"] for i in pinfos) / len(pinfos)
+    log(f"vcd: in {pinfos[-1]['n_part']} parts of {part / 1e6:.0f} MB: {p_ms:.2f} ms, the same store")
     dev_ms = sum(i["device_ms"] for i in infos) / len(infos)
     tot_ms = sum(i["total_ms"] for i in infos) / len(infos)
     body = info["body_bytes"]
@@ -264,7 +280,9 @@ def vcd_leg(design, w, n_cycle, steps, cpu):
            "text_gb_per_s": body / (dev_ms * 1e-3) / 1e9, "events_per_s": info["n_event"] / (dev_ms * 1e-3),
            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": gbs / HBM_PEAK_GBS,
-                        "note": "algorithmic bytes = text read once + 4 B per event word + 8 B per stream offset"}}
+                        "note": "algorithmic bytes = text read once + 4 B per event word + 8 B per stream offset"},
+           "in_parts": {"n_part": pinfos[-1]["n_part"], "part_bytes": part, "device_ms": p_ms,
+                        "same_store": same}}
     if cpu:
         import oracle
         t0 = time.perf_counter()
@@ -277,6 +295,42 @@ def vcd_leg(design, w, n_cycle, steps, cpu):
                                **host_cpu()}
         log(f"vcd: cpu reader {cpu_s:.2f} s ({body / cpu_s / 1e6:.1f} MB/s)")
     return res
+
+
+def vcd_big_leg(design, w, n_cycle):
+    """A VCD text above 4 GiB: the netlist's trace over n_cycle cycles with
+    every net a scalar variable, parsed on the device in parts of the default
+    size; every stream must equal the trace's events (the text's round trip
+    is exact for scalar nets)."""
+    import numpy as np
+    from gtpower_amd import synthetic as S
+    from gtpower_amd.vcd import VcdText
+
+    tk = dict(w["trace"])
+    tk["n_cycle"] = n_cycle
+    tseed = S.MASTER_SEED ^ (w["seed"] * 0x9E3779B1)
+    t0 = time.perf_counter()
+    tr = design.trace(seed=tseed + 2005, **tk)
+    text = S.vcd_text(tr)
+    gen_s = time.perf_counter() - t0
+    log(f"vcd big: {len(text) / 2**30:.2f} GiB of VCD text ({n_cycle} cycles, {tr.n_event} events) in {gen_s:.1f} s")
+    with VcdText(device=0) as v:
+        infos = [v.parse(text) for _ in range(2)]  # the first allocates the reader's scratch memory
+        off, t, val = v.events()
+    info = infos[-1]
+    same = (np.array_equal(off, tr.net_event_offset) and np.array_equal(t, tr.event_time)
+            and np.array_equal(val, tr.event_value))
+    del off, t, val
+    tr.close()
+    assert same, "vcd big: the streams differ from the trace"
+    body = info["body_bytes"]
+    log(f"vcd big: {info['n_part']} parts, device parse {info['device_ms']:.1f} ms "
+        f"({body / (info['device_ms'] * 1e-3) / 1e9:.1f} GB/s of text), {info['total_ms']:.0f} ms with the upload; "
+        "streams equal to the trace")
+    return {"text_bytes": len(text), "body_bytes": body, "cycles": n_cycle, "n_event": info["n_event"],
+            "n_stream": info["n_stream"], "n_part": info["n_part"], "device_ms": info["device_ms"],
+            "total_ms_with_upload": info["total_ms"], "text_gb_per_s": body / (info["device_ms"] * 1e-3) / 1e9,
+            "streams_equal_trace": same}
 
 
 def surrogate_leg(eng, design, exact_result, w, n_fresh, n_windows, exact_wall_cps, exact_device_cps, judge_tile=64,
@@ -413,6 +467,8 @@ def main():
     ap.add_argument("--surrogate-windows", type=int, default=10,
                     help="fresh traces the surrogate evaluates (of --surrogate-cycles cycles each)")
     ap.add_argument("--vcd-cycles", type=int, default=2000, help="cycles of the VCD ingest leg (0: off)")
+    ap.add_argument("--vcd-big-cycles", type=int, default=0,
+                    help="cycles of a VCD text of scalar nets parsed in parts (20000: 4.6 GiB; 0: off)")
     ap.add_argument("--sliced-parts", type=int, default=4, help="event budget 1/parts for the sliced leg (<2: off)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
     ap.add_argument("--steps-json", default=None, metavar="FILE",
@@ -537,6 +593,9 @@ def main():
     vcd = None
     if args.vcd_cycles > 0:
         vcd = vcd_leg(design, w, min(args.vcd_cycles, n_cycle), 3, not args.no_cpu)
+    if args.vcd_big_cycles > 0:
+        vcd = dict(vcd or {})
+        vcd["big_text"] = vcd_big_leg(design, w, args.vcd_big_cycles)
 
     cpu = None
     if not args.no_cpu:

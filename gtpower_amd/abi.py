@@ -103,7 +103,7 @@ class GtpVcdInfo(C.Structure):
     _fields_ = [("n_var", C.c_int64), ("n_id", C.c_int64), ("n_stream", C.c_int64), ("n_event", C.c_int64),
                 ("t_max", C.c_int64), ("time_scale", C.c_double), ("n_token", C.c_int64),
                 ("n_change", C.c_int64), ("body_bytes", C.c_int64), ("device_ms", C.c_double),
-                ("total_ms", C.c_double)]
+                ("total_ms", C.c_double), ("n_part", C.c_int64)]
 
 
 class GtpVcdVarInfo(C.Structure):

@@ -33,6 +33,12 @@
 //                    scattered to the streams of its id's bits as 32-bit event
 //                    words (time << 3 | value)
 //
+// A text longer than the part size is parsed in parts cut at whitespace (the
+// reference reads its text as one stream of tokens, so the result does not
+// depend on the cuts): each part runs the whole pipeline with the time in
+// force and every id's last kept change carried in, and the parts' stores are
+// appended per stream at the end.
+//
 // Streams are numbered per unique id in declaration order, bits 0..w-1 (w =
 // the widest declaration of the id); bit b of a bus value is (value >> b) & 1.
 // Value codes: 0 -> 0, 1 -> 1, X -> 2, Z -> 3 and U -> 2 (the reference's
@@ -42,7 +48,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -338,6 +346,7 @@ struct ChangeArgs {
   uint64_t mask;
   const uint8_t* idc;
   const int64_t* ido;
+  int64_t t_carry;          // time of the changes before the first time token
   uint32_t* ch_uid;
   uint64_t* ch_pack;   // bus << 63 | time << 32 | (scalar: value letter, bus: change index)
   uint64_t* ch_val;    // bus value of bus changes
@@ -365,7 +374,7 @@ __global__ void emit_changes_kernel(ChangeArgs a) {
   const int32_t u = lookup_id(id, n, key, a.direct, a.tab, a.mask, a.idc, a.ido);
   if (u < 0) set_err(a.err, GTP_VCD_ERR_ID, k);
   const int32_t tk = a.ti[k];
-  const int64_t time = tk >= 0 ? a.val[tk] : 0;
+  const int64_t time = tk >= 0 ? a.val[tk] : a.t_carry;
   if (time < 0 || time >= (int64_t)EV_TIME_INF) set_err(a.err, GTP_VCD_ERR_RANGE, k);
   a.ch_uid[c] = u < 0 ? 0u : (uint32_t)u;
   const uint64_t tt = (uint64_t)(time < 0 ? 0 : min(time, (int64_t)EV_TIME_INF));
@@ -386,11 +395,29 @@ __global__ void unpack_kernel(const uint64_t* __restrict__ s_pack, int64_t n, co
   s_val[p] = bus ? ch_val[(uint32_t)w] : (w & 0xffffffffu);
 }
 
+// The last kept change of every id in the text parts read so far (a text is
+// parsed in parts cut at whitespace): whether there was one, its kind, value
+// and time.  All zero before the first part.
+struct Carry {
+  uint8_t* has;
+  uint8_t* bus;
+  uint64_t* val;
+  uint32_t* time;
+};
+
+// the reference's repeat rule against the last kept change (power/Vcd.cc:
+// 203-232): a scalar compares its character with the last value's ('\0' for
+// a bus value), a bus value compares with the last one's (0 for a scalar)
+__device__ __forceinline__ bool differs_from(bool bus, uint64_t v, bool last_bus, uint64_t last_v) {
+  return bus ? v != (last_bus ? last_v : 0ull) : v != (last_bus ? 0ull : last_v);
+}
+
 // segment bounds per id and the parallel keep rule: a change equal to the
 // previous change of its id is dropped (exact when the id's changes are all
-// scalar or all bus); ids mixing both are flagged for the sequential pass
+// scalar or all bus); ids mixing both are flagged for the sequential pass.
+// The first change of an id in a part compares with the carried one.
 __global__ void keep_kernel(const uint32_t* __restrict__ key, int64_t n, const uint64_t* __restrict__ s_val,
-                            const uint8_t* __restrict__ s_bus, int64_t* __restrict__ seg_start,
+                            const uint8_t* __restrict__ s_bus, Carry c, int64_t* __restrict__ seg_start,
                             int64_t* __restrict__ seg_end, uint32_t* __restrict__ mixed,
                             uint32_t* __restrict__ keep) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -404,6 +431,8 @@ __global__ void keep_kernel(const uint32_t* __restrict__ key, int64_t n, const u
   if (!first) {
     if (s_bus[p] != s_bus[p - 1]) mixed[u] = 1;
     k = s_val[p] != s_val[p - 1];
+  } else if (c.has[u]) {
+    k = differs_from(s_bus[p] != 0, s_val[p], c.bus[u] != 0, c.val[u]);
   }
   keep[p] = k ? 1u : 0u;
 }
@@ -414,19 +443,21 @@ __global__ void keep_kernel(const uint32_t* __restrict__ key, int64_t n, const u
 // scalar) (power/Vcd.cc:203-232), one thread per id
 __global__ void keep_mixed_kernel(const uint32_t* __restrict__ mixed, int64_t n_id,
                                   const int64_t* __restrict__ seg_start, const int64_t* __restrict__ seg_end,
-                                  const uint64_t* __restrict__ s_val, const uint8_t* __restrict__ s_bus,
+                                  const uint64_t* __restrict__ s_val, const uint8_t* __restrict__ s_bus, Carry c,
                                   uint32_t* __restrict__ keep) {
   const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= n_id || !mixed[u]) return;
-  uint64_t last_char = 0, last_bus = 0;
+  const bool had = c.has[u] != 0;
+  bool last_is_bus = had && c.bus[u] != 0;
+  uint64_t last_v = had ? c.val[u] : 0;
   for (int64_t p = seg_start[u]; p < seg_end[u]; ++p) {
     const bool bus = s_bus[p] != 0;
     const uint64_t v = s_val[p];
-    const bool k = p == seg_start[u] || (bus ? v != last_bus : v != last_char);
+    const bool k = (p == seg_start[u] && !had) || differs_from(bus, v, last_is_bus, last_v);
     keep[p] = k ? 1u : 0u;
     if (k) {
-      last_char = bus ? 0 : v;
-      last_bus = bus ? v : 0;
+      last_is_bus = bus;
+      last_v = v;
     }
   }
 }
@@ -439,29 +470,41 @@ __global__ void order_key_kernel(const uint32_t* __restrict__ key, int64_t n, co
   out[p] = keep[p] ? (((uint64_t)key[p] << 32) | ((uint64_t)s_time[p] + 1)) : 0ull;
 }
 
-// a kept change no later than an earlier kept change of its id
+// a kept change no later than an earlier kept change of its id (in this
+// part, or the carried one of the parts before)
 __global__ void order_check_kernel(const uint32_t* __restrict__ key, int64_t n, const uint32_t* __restrict__ keep,
                                    const uint64_t* __restrict__ okey, const uint64_t* __restrict__ omax,
-                                   DevErr* err) {
+                                   const uint32_t* __restrict__ s_time, Carry c, DevErr* err) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n || !keep[p]) return;
-  const uint64_t prev = omax[p];  // exclusive max of earlier keys
-  if ((prev >> 32) == key[p] && (prev & 0xffffffffull) >= (okey[p] & 0xffffffffull))
+  const uint32_t u = key[p];
+  // exclusive max of earlier keys; 0 (the scan's identity; kept keys hold
+  // time + 1 >= 1) when the part has none
+  const uint64_t prev = omax[p];
+  if ((prev >> 32) == u && (prev & 0xffffffffull) != 0) {
+    if ((prev & 0xffffffffull) >= (okey[p] & 0xffffffffull)) set_err(err, GTP_VCD_ERR_ORDER, p);
+  } else if (c.has[u] && s_time[p] <= c.time[u]) {
     set_err(err, GTP_VCD_ERR_ORDER, p);
+  }
 }
 
-// events per stream: the id's kept changes, plus the initial value when its
-// first change is after 0
+// the initial value goes in when an id's first change in the text is after 0
+__device__ __forceinline__ int initial_event(const Carry& c, uint32_t u, uint32_t first_time) {
+  return (!c.has[u] && first_time != 0) ? 1 : 0;
+}
+
+// events per stream in this part: the id's kept changes, plus the initial
+// value
 __global__ void stream_count_kernel(const int32_t* __restrict__ stream_uid, int64_t S,
                                     const int64_t* __restrict__ seg_start, const int64_t* __restrict__ seg_end,
-                                    const uint32_t* __restrict__ ks, const uint32_t* __restrict__ s_time,
+                                    const uint32_t* __restrict__ ks, const uint32_t* __restrict__ s_time, Carry cr,
                                     int64_t* __restrict__ cnt) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
   const int32_t u = stream_uid[s];
   const int64_t a = seg_start[u];
   int64_t c = 0;
-  if (a >= 0) c = (int64_t)(ks[seg_end[u]] - ks[a]) + (s_time[a] != 0 ? 1 : 0);
+  if (a >= 0) c = (int64_t)(ks[seg_end[u]] - ks[a]) + initial_event(cr, (uint32_t)u, s_time[a]);
   cnt[s] = c;
 }
 
@@ -477,6 +520,7 @@ struct ScatterArgs {
   const int32_t* uid_width;
   const int64_t* stream_base;
   const int64_t* stream_off;
+  Carry carry;
   uint32_t* ev;
 };
 
@@ -491,7 +535,7 @@ __global__ void scatter_kernel(ScatterArgs a) {
   if (p >= a.n || !a.keep[p]) return;
   const uint32_t u = a.key[p];
   const int64_t s0 = a.seg_start[u];
-  const int init = a.s_time[s0] != 0 ? 1 : 0;
+  const int init = initial_event(a.carry, u, a.s_time[s0]);
   const int64_t rank = (int64_t)(a.ks[p] - a.ks[s0]) + init;
   const uint32_t tw = a.s_time[p] << EV_SHIFT;
   const bool bus = a.s_bus[p] != 0;
@@ -508,6 +552,42 @@ __global__ void scatter_kernel(ScatterArgs a) {
   }
 }
 
+// the carried change of every id kept in this part: its last kept change
+__global__ void carry_kernel(const uint32_t* __restrict__ key, int64_t n, const uint32_t* __restrict__ keep,
+                             const uint32_t* __restrict__ ks, const int64_t* __restrict__ seg_end,
+                             const uint8_t* __restrict__ s_bus, const uint64_t* __restrict__ s_val,
+                             const uint32_t* __restrict__ s_time, Carry c) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n || !keep[p]) return;
+  const uint32_t u = key[p];
+  if (ks[p] + 1 != ks[seg_end[u]]) return;
+  c.has[u] = 1;
+  c.bus[u] = s_bus[p];
+  c.val[u] = s_val[p];
+  c.time[u] = s_time[p];
+}
+
+// per-stream event counts of one part added up
+__global__ void add_counts_kernel(const int64_t* __restrict__ off, int64_t S, int64_t* __restrict__ total) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < S) total[s] += off[s + 1] - off[s];
+}
+
+// one part's store appended to the final one: a wave per stream copies the
+// stream's words behind those of the parts before (done[s] of them)
+__global__ __launch_bounds__(256) void append_part_kernel(const uint32_t* __restrict__ part_ev,
+                                                          const int64_t* __restrict__ part_off, int64_t S,
+                                                          const int64_t* __restrict__ off, int64_t* __restrict__ done,
+                                                          uint32_t* __restrict__ ev) {
+  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (s >= S) return;
+  const int64_t a = part_off[s], n = part_off[s + 1] - a;
+  const int64_t d = off[s] + done[s];
+  for (int64_t i = lane; i < n; i += 64) ev[d + i] = part_ev[a + i];
+  if (lane == 0) done[s] += n;
+}
+
 __global__ void fill_words_kernel(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -517,6 +597,10 @@ __global__ void fill_i64_kernel(int64_t* __restrict__ p, int64_t n, int64_t v) {
   if (i < n) p[i] = v;
 }
 __global__ void fill_u32_kernel(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void fill_u8_kernel(uint8_t* __restrict__ p, int64_t n, uint8_t v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
@@ -565,6 +649,11 @@ struct gtp_vcd {
     DArr<uint64_t> bcnt, boff, ch_val, ch_pack, s_pack, s_val, okey, omax;
     DArr<uint32_t> tok, len, is_change, chpos, ch_uid, key, s_time, keep, ks, mixed;
     DArr<unsigned long long> err;
+    // carried between the parts of one text
+    DArr<uint8_t> c_has, c_bus;
+    DArr<uint64_t> c_val;
+    DArr<uint32_t> c_time;
+    DArr<int64_t> tot, done;
   } sc;
   std::vector<int64_t> h_off;
   gtp_vcd_info info{};
@@ -764,11 +853,295 @@ hipError_t put(DArr<T>& a, const T* src, size_t n, hipStream_t s) {
 }
 }  // namespace
 
+namespace {
+
+// Text parts: a text longer than the part size is parsed in parts cut at
+// whitespace, each through the whole device pipeline, with the time, the
+// last kept change of every id and the error positions carried from part to
+// part; each part's event store is appended per stream at the end.
+constexpr int64_t VCD_PART_BYTES = int64_t(1) << 30;
+
+int64_t part_bytes() {
+  int64_t n = VCD_PART_BYTES;
+  if (const char* e = std::getenv("GTP_VCD_PART_BYTES")) {
+    const long long x = std::atoll(e);
+    if (x > 0) n = std::min<int64_t>(x, int64_t(1) << 31);
+  }
+  return n;
+}
+
+// A part may end before the whitespace byte w (w > b0) unless the body
+// token before w looks like a bus value: that token takes the next one as
+// its id, and which tokens are ids follows from runs of such tokens.
+bool good_cut(const char* t, int64_t b0, int64_t w) {
+  if (!host_ws((unsigned char)t[w])) return false;
+  int64_t q = w - 1;
+  while (q >= b0 && host_ws((unsigned char)t[q])) --q;
+  if (q < b0) return true;
+  int64_t s0 = q;
+  while (s0 > b0 && !host_ws((unsigned char)t[s0 - 1])) --s0;
+  return t[s0] != 'b' && t[s0] != 'B';
+}
+
+// part bounds [0 = c_0, c_1, ..., c_n = len): each cut at the good cut
+// nearest below start + size, else the first one above it
+std::vector<int64_t> plan_parts(const char* t, int64_t len, int64_t b0, int64_t size) {
+  std::vector<int64_t> c{0};
+  int64_t start = 0;
+  while (len - start > size) {
+    const int64_t lo = std::max(start, b0) + 1;
+    const int64_t w = std::min(std::max(start + size, lo), len - 1);
+    int64_t cut = -1;
+    for (int64_t x = w; x >= lo; --x)
+      if (good_cut(t, b0, x)) {
+        cut = x;
+        break;
+      }
+    for (int64_t x = w + 1; cut < 0 && x < len; ++x)
+      if (good_cut(t, b0, x)) cut = x;
+    if (cut < 0) break;
+    c.push_back(cut);
+    start = cut;
+  }
+  c.push_back(len);
+  return c;
+}
+
+struct PartIn {
+  const char* t;
+  int64_t len, b0;      // the part's bytes; its body starts at b0
+  bool cut;             // the text ends inside the part's last token
+  int64_t t_carry;      // the time in force at the part's start
+  bool carry_out;       // record the part's last kept change of every id
+  int64_t tok_before, chg_before;  // tokens and changes of the parts before
+  DArr<uint32_t>* ev;   // the part's event store
+  DArr<int64_t>* off;   // [n_stream + 1]
+  bool pad;             // EV_PAD words of EV_INF after the store
+};
+struct PartOut {
+  int64_t K = 0, n_change = 0, E = 0;
+  bool has_time = false;
+  int64_t t_last = 0;   // the value of the part's last time token
+  float ms = 0;
+};
+
+const unsigned long long kNoErr = ~0ull;
+
+gtp_status parse_part(gtp_vcd* v, const PartIn& in, PartOut* out, uint64_t tsz, const Carry& carry) {
+  hipStream_t s = v->stream;
+  gtp_vcd::Scratch& S_ = v->sc;
+  const int64_t U = (int64_t)v->ids.size();
+  const int64_t S = v->stream_base[U];
+  const int64_t len = in.len, b0 = in.b0;
+  const int64_t b0al = b0 & ~(int64_t)15;
+  const int64_t n_blk = std::max<int64_t>(1, (len - b0al + BPB - 1) / BPB);
+  const size_t text_bytes = (size_t)(b0al + n_blk * BPB + 16);
+  VCHK(v, grow(S_.text, text_bytes));
+  VCHK(v, hipMemsetAsync(S_.text.p + len, ' ', text_bytes - (size_t)len, s));
+  VCHK(v, hipMemcpyAsync(S_.text.p, in.t, (size_t)len, hipMemcpyHostToDevice, s));
+  VCHK(v, put(S_.err, &kNoErr, 1, s));
+  DevErr* d_err = reinterpret_cast<DevErr*>(S_.err.p);
+  // upper bounds of tokens and changes: a token and its whitespace take 2 bytes
+  const int64_t K_max = (len - b0) / 2 + 2;
+  VCHK(v, grow(S_.bcnt, n_blk));
+  VCHK(v, grow(S_.boff, n_blk));
+  VCHK(v, grow(S_.tok, K_max));
+  VCHK(v, grow(S_.kind, K_max));
+  VCHK(v, grow(S_.len, K_max));
+  VCHK(v, grow(S_.val, K_max));
+  VCHK(v, grow(S_.skey, K_max));
+  VCHK(v, grow(S_.rs_in, K_max));
+  VCHK(v, grow(S_.rs, K_max));
+  VCHK(v, grow(S_.ti_in, K_max));
+  VCHK(v, grow(S_.ti, K_max));
+  VCHK(v, grow(S_.is_change, K_max + 1));
+  VCHK(v, grow(S_.chpos, K_max + 1));
+  VCHK(v, grow(S_.seg_start, std::max<int64_t>(U, 1)));
+  VCHK(v, grow(S_.seg_end, std::max<int64_t>(U, 1)));
+  VCHK(v, grow(S_.mixed, std::max<int64_t>(U, 1)));
+  VCHK(v, grow(S_.scnt, S + 1));
+  VCHK(v, in.off->alloc(S + 1));
+  auto tmp = [&](size_t need) -> hipError_t { return grow(S_.tmp, std::max<size_t>(need, 1)); };
+  hipEvent_t ev0, ev1;
+  VCHK(v, hipEventCreate(&ev0));
+  VCHK(v, hipEventCreate(&ev1));
+  struct EvGuard {
+    hipEvent_t a, b;
+    ~EvGuard() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } evg{ev0, ev1};
+  VCHK(v, hipEventRecord(ev0, s));
+
+  // token starts
+  count_starts_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(S_.text.p, b0al, b0, len, S_.bcnt.p);
+  VCHK(v, hipGetLastError());
+  {
+    size_t need = 0;
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, S_.bcnt.p, S_.boff.p, n_blk, s));
+    VCHK(v, tmp(need));
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(S_.tmp.p, need, S_.bcnt.p, S_.boff.p, n_blk, s));
+  }
+  uint64_t last_off = 0, last_cnt = 0;
+  VCHK(v, hipMemcpyAsync(&last_off, S_.boff.p + n_blk - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  VCHK(v, hipMemcpyAsync(&last_cnt, S_.bcnt.p + n_blk - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  VCHK(v, hipStreamSynchronize(s));
+  int64_t K = (int64_t)(last_off + last_cnt);
+  if (in.cut && K > 0) --K;
+  if (K > K_max) return v->fail(GTP_ERR_DEVICE, "gtp_vcd_parse: token count above its bound");
+  write_starts_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(S_.text.p, b0al, b0, len, S_.boff.p, S_.tok.p, K);
+  VCHK(v, hipGetLastError());
+
+  // tokens
+  int64_t n_change = 0;
+  if (K > 0) {
+    parse_tokens_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(S_.text.p, len, (int64_t)text_bytes, b0al,
+                                                                  S_.boff.p, n_blk, S_.tok.p, K, S_.kind.p,
+                                                                  S_.len.p, S_.val.p, S_.skey.p);
+    VCHK(v, hipGetLastError());
+    run_start_in_kernel<<<nblk(K), 256, 0, s>>>(S_.kind.p, K, S_.rs_in.p);
+    VCHK(v, hipGetLastError());
+    size_t need = 0;
+    VCHK(v, hipcub::DeviceScan::InclusiveScan(nullptr, need, S_.rs_in.p, S_.rs.p, MaxI32(), K, s));
+    VCHK(v, tmp(need));
+    VCHK(v, hipcub::DeviceScan::InclusiveScan(S_.tmp.p, need, S_.rs_in.p, S_.rs.p, MaxI32(), K, s));
+    time_in_kernel<<<nblk(K), 256, 0, s>>>(S_.kind.p, S_.rs.p, K, S_.ti_in.p, S_.is_change.p, d_err);
+    VCHK(v, hipGetLastError());
+    VCHK(v, hipcub::DeviceScan::InclusiveScan(nullptr, need, S_.ti_in.p, S_.ti.p, MaxI32(), K, s));
+    VCHK(v, tmp(need));
+    VCHK(v, hipcub::DeviceScan::InclusiveScan(S_.tmp.p, need, S_.ti_in.p, S_.ti.p, MaxI32(), K, s));
+    fill_u32_kernel<<<1, 1, 0, s>>>(S_.is_change.p + K, 1, 0u);
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, S_.is_change.p, S_.chpos.p, K + 1, s));
+    VCHK(v, tmp(need));
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(S_.tmp.p, need, S_.is_change.p, S_.chpos.p, K + 1, s));
+    uint32_t nc = 0;
+    int32_t tlast = -1;
+    VCHK(v, hipMemcpyAsync(&nc, S_.chpos.p + K, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    VCHK(v, hipMemcpyAsync(&tlast, S_.ti.p + K - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    VCHK(v, hipStreamSynchronize(s));
+    n_change = nc;
+    if (tlast >= 0) {
+      VCHK(v, hipMemcpyAsync(&out->t_last, S_.val.p + tlast, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+      VCHK(v, hipStreamSynchronize(s));
+      out->has_time = true;
+    }
+  }
+
+  // changes in text order, grouped by id (stable), payload gathered once
+  const int64_t NC1 = std::max<int64_t>(n_change, 1);
+  VCHK(v, grow(S_.ch_uid, NC1));
+  VCHK(v, grow(S_.key, NC1));
+  VCHK(v, grow(S_.ch_pack, NC1));
+  VCHK(v, grow(S_.s_pack, NC1));
+  VCHK(v, grow(S_.ch_val, NC1));
+  VCHK(v, grow(S_.s_time, NC1));
+  VCHK(v, grow(S_.s_val, NC1));
+  VCHK(v, grow(S_.s_bus, NC1));
+  VCHK(v, grow(S_.keep, NC1 + 1));
+  VCHK(v, grow(S_.ks, NC1 + 1));
+  VCHK(v, grow(S_.okey, NC1));
+  VCHK(v, grow(S_.omax, NC1));
+  const int64_t U1 = std::max<int64_t>(U, 1);
+  fill_i64_kernel<<<nblk(U1), 256, 0, s>>>(S_.seg_start.p, U1, -1);
+  fill_i64_kernel<<<nblk(U1), 256, 0, s>>>(S_.seg_end.p, U1, 0);
+  fill_u32_kernel<<<nblk(U1), 256, 0, s>>>(S_.mixed.p, U1, 0u);
+  VCHK(v, hipGetLastError());
+  if (n_change > 0) {
+    ChangeArgs ca{S_.text.p, S_.tok.p, S_.kind.p, S_.len.p, S_.val.p, S_.skey.p, S_.ti.p, S_.chpos.p,
+                  S_.is_change.p, K, S_.direct.p, S_.tab.p, tsz - 1, S_.idc.p, S_.ido.p, in.t_carry,
+                  S_.ch_uid.p, S_.ch_pack.p, S_.ch_val.p, d_err};
+    emit_changes_kernel<<<nblk(K), 256, 0, s>>>(ca);
+    VCHK(v, hipGetLastError());
+    int bits = 1;
+    while (bits < 32 && (1ll << bits) < U) ++bits;
+    size_t need = 0;
+    VCHK(v, hipcub::DeviceRadixSort::SortPairs(nullptr, need, S_.ch_uid.p, S_.key.p, S_.ch_pack.p, S_.s_pack.p,
+                                               n_change, 0, bits, s));
+    VCHK(v, tmp(need));
+    VCHK(v, hipcub::DeviceRadixSort::SortPairs(S_.tmp.p, need, S_.ch_uid.p, S_.key.p, S_.ch_pack.p, S_.s_pack.p,
+                                               n_change, 0, bits, s));
+    unpack_kernel<<<nblk(n_change), 256, 0, s>>>(S_.s_pack.p, n_change, S_.ch_val.p, S_.s_time.p, S_.s_val.p,
+                                                 S_.s_bus.p);
+    VCHK(v, hipGetLastError());
+    keep_kernel<<<nblk(n_change), 256, 0, s>>>(S_.key.p, n_change, S_.s_val.p, S_.s_bus.p, carry, S_.seg_start.p,
+                                               S_.seg_end.p, S_.mixed.p, S_.keep.p);
+    VCHK(v, hipGetLastError());
+    if (U > 0) {
+      keep_mixed_kernel<<<nblk(U), 256, 0, s>>>(S_.mixed.p, U, S_.seg_start.p, S_.seg_end.p, S_.s_val.p,
+                                                S_.s_bus.p, carry, S_.keep.p);
+      VCHK(v, hipGetLastError());
+    }
+    order_key_kernel<<<nblk(n_change), 256, 0, s>>>(S_.key.p, n_change, S_.keep.p, S_.s_time.p, S_.okey.p);
+    VCHK(v, hipGetLastError());
+    VCHK(v, hipcub::DeviceScan::ExclusiveScan(nullptr, need, S_.okey.p, S_.omax.p, MaxU64(), (uint64_t)0,
+                                              n_change, s));
+    VCHK(v, tmp(need));
+    VCHK(v, hipcub::DeviceScan::ExclusiveScan(S_.tmp.p, need, S_.okey.p, S_.omax.p, MaxU64(), (uint64_t)0,
+                                              n_change, s));
+    order_check_kernel<<<nblk(n_change), 256, 0, s>>>(S_.key.p, n_change, S_.keep.p, S_.okey.p, S_.omax.p,
+                                                      S_.s_time.p, carry, d_err);
+    VCHK(v, hipGetLastError());
+  }
+  fill_u32_kernel<<<1, 1, 0, s>>>(S_.keep.p + n_change, 1, 0u);
+  {
+    size_t need = 0;
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, S_.keep.p, S_.ks.p, n_change + 1, s));
+    VCHK(v, tmp(need));
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(S_.tmp.p, need, S_.keep.p, S_.ks.p, n_change + 1, s));
+  }
+  // streams
+  if (S > 0) {
+    stream_count_kernel<<<nblk(S), 256, 0, s>>>(S_.stream_uid.p, S, S_.seg_start.p, S_.seg_end.p, S_.ks.p,
+                                                S_.s_time.p, carry, S_.scnt.p);
+    VCHK(v, hipGetLastError());
+  }
+  fill_i64_kernel<<<1, 1, 0, s>>>(S_.scnt.p + S, 1, 0);
+  {
+    size_t need = 0;
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, S_.scnt.p, in.off->p, S + 1, s));
+    VCHK(v, tmp(need));
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(S_.tmp.p, need, S_.scnt.p, in.off->p, S + 1, s));
+  }
+  unsigned long long he = 0;
+  int64_t E = 0;
+  VCHK(v, hipMemcpyAsync(&he, S_.err.p, sizeof(he), hipMemcpyDeviceToHost, s));
+  VCHK(v, hipMemcpyAsync(&E, in.off->p + S, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  VCHK(v, hipStreamSynchronize(s));
+  if (he != ~0ull) {
+    const int code = (int)(he & 0xff);
+    const int64_t item = (int64_t)(he >> 8) + (code == GTP_VCD_ERR_ORDER ? in.chg_before : in.tok_before);
+    return v->fail(GTP_ERR_ARG, std::string("vcd: ") + err_text(code) + " (item " + std::to_string(item) +
+                                    " of the value-change section)");
+  }
+  const int64_t pad = in.pad ? EV_PAD : 0;
+  VCHK(v, grow(*in.ev, std::max<int64_t>(E + pad, 1)));
+  if (pad) fill_words_kernel<<<1, 64, 0, s>>>(in.ev->p + E, EV_PAD, EV_INF);
+  if (n_change > 0) {
+    ScatterArgs sa{S_.key.p, n_change, S_.keep.p, S_.ks.p, S_.seg_start.p, S_.s_time.p, S_.s_val.p, S_.s_bus.p,
+                   S_.uid_width.p, S_.stream_base.p, in.off->p, carry, in.ev->p};
+    scatter_kernel<<<nblk(n_change), 256, 0, s>>>(sa);
+    VCHK(v, hipGetLastError());
+    if (in.carry_out) {
+      carry_kernel<<<nblk(n_change), 256, 0, s>>>(S_.key.p, n_change, S_.keep.p, S_.ks.p, S_.seg_end.p, S_.s_bus.p,
+                                                  S_.s_val.p, S_.s_time.p, carry);
+      VCHK(v, hipGetLastError());
+    }
+  }
+  VCHK(v, hipEventRecord(ev1, s));
+  VCHK(v, hipStreamSynchronize(s));
+  VCHK(v, hipEventElapsedTime(&out->ms, ev0, ev1));
+  out->K = K;
+  out->n_change = n_change;
+  out->E = E;
+  return GTP_OK;
+}
+
+}  // namespace
+
 GTP_API gtp_status gtp_vcd_parse(gtp_vcd* v, const char* text, int64_t len, gtp_vcd_info* info) {
   if (!v) return GTP_ERR_ARG;
   if (!text || len < 0) return v->fail(GTP_ERR_ARG, "gtp_vcd_parse: bad text");
-  if (len >= (int64_t)UINT32_MAX - 64)
-    return v->fail(GTP_ERR_UNSUPPORTED, "gtp_vcd_parse: texts of 4 GiB or more are read in several parts");
   v->parsed = false;
   v->error.clear();
   VCHK(v, hipSetDevice(v->device));
@@ -781,6 +1154,11 @@ GTP_API gtp_status gtp_vcd_parse(gtp_vcd* v, const char* text, int64_t len, gtp_
   }
   const int64_t U = (int64_t)v->ids.size();
   if (U >= (int64_t)INT32_MAX) return v->fail(GTP_ERR_LIMIT, "gtp_vcd_parse: too many ids");
+  const std::vector<int64_t> bounds = plan_parts(text, len, b0, part_bytes());
+  const int64_t n_part = (int64_t)bounds.size() - 1;
+  for (int64_t i = 0; i < n_part; ++i)
+    if (bounds[i + 1] - bounds[i] >= (int64_t)UINT32_MAX - 64 - BPB)
+      return v->fail(GTP_ERR_UNSUPPORTED, "gtp_vcd_parse: no place to cut the text into parts under 4 GiB");
   v->stream_base.assign(U + 1, 0);
   for (int64_t u = 0; u < U; ++u) v->stream_base[u + 1] = v->stream_base[u] + v->uid_width[u];
   const int64_t S = v->stream_base[U];
@@ -817,16 +1195,8 @@ GTP_API gtp_status gtp_vcd_parse(gtp_vcd* v, const char* text, int64_t len, gtp_
     idc += id;
     ido[u + 1] = (int64_t)idc.size();
   }
-  // the body's last token is dropped when the text ends inside it
-  const bool cut = len > b0 && !host_ws((unsigned char)text[len - 1]);
 
   const auto t0 = std::chrono::steady_clock::now();
-  const int64_t b0al = b0 & ~(int64_t)15;
-  const int64_t n_blk = std::max<int64_t>(1, (len - b0al + BPB - 1) / BPB);
-  const size_t text_bytes = (size_t)(b0al + n_blk * BPB + 16);
-  VCHK(v, grow(S_.text, text_bytes));
-  VCHK(v, hipMemsetAsync(S_.text.p + len, ' ', text_bytes - (size_t)len, s));
-  VCHK(v, hipMemcpyAsync(S_.text.p, text, (size_t)len, hipMemcpyHostToDevice, s));
   VCHK(v, put(S_.tab, tab.data(), tab.size(), s));
   VCHK(v, put(S_.direct, direct.data(), direct.size(), s));
   VCHK(v, put(S_.ido, ido.data(), ido.size(), s));
@@ -834,205 +1204,113 @@ GTP_API gtp_status gtp_vcd_parse(gtp_vcd* v, const char* text, int64_t len, gtp_
   VCHK(v, put(S_.stream_uid, stream_uid.data(), stream_uid.size(), s));
   VCHK(v, put(S_.uid_width, v->uid_width.data(), v->uid_width.size(), s));
   VCHK(v, put(S_.stream_base, v->stream_base.data(), v->stream_base.size(), s));
-  const unsigned long long no_err = ~0ull;  // alive until the stream synchronizes below
-  VCHK(v, put(S_.err, &no_err, 1, s));
-  DevErr* d_err = reinterpret_cast<DevErr*>(S_.err.p);
-  // upper bounds of tokens and changes: a token and its whitespace take 2 bytes
-  const int64_t K_max = (len - b0) / 2 + 2;
-  VCHK(v, grow(S_.bcnt, n_blk));
-  VCHK(v, grow(S_.boff, n_blk));
-  VCHK(v, grow(S_.tok, K_max));
-  VCHK(v, grow(S_.kind, K_max));
-  VCHK(v, grow(S_.len, K_max));
-  VCHK(v, grow(S_.val, K_max));
-  VCHK(v, grow(S_.skey, K_max));
-  VCHK(v, grow(S_.rs_in, K_max));
-  VCHK(v, grow(S_.rs, K_max));
-  VCHK(v, grow(S_.ti_in, K_max));
-  VCHK(v, grow(S_.ti, K_max));
-  VCHK(v, grow(S_.is_change, K_max + 1));
-  VCHK(v, grow(S_.chpos, K_max + 1));
-  VCHK(v, grow(S_.seg_start, std::max<int64_t>(U, 1)));
-  VCHK(v, grow(S_.seg_end, std::max<int64_t>(U, 1)));
-  VCHK(v, grow(S_.mixed, std::max<int64_t>(U, 1)));
-  VCHK(v, grow(S_.scnt, S + 1));
-  VCHK(v, v->d_off.alloc(S + 1));
-  auto tmp = [&](size_t need) -> hipError_t { return grow(S_.tmp, std::max<size_t>(need, 1)); };
-  hipEvent_t ev0, ev1;
-  VCHK(v, hipEventCreate(&ev0));
-  VCHK(v, hipEventCreate(&ev1));
-  struct EvGuard {
-    hipEvent_t a, b;
-    ~EvGuard() {
-      (void)hipEventDestroy(a);
-      (void)hipEventDestroy(b);
-    }
-  } evg{ev0, ev1};
-  VCHK(v, hipEventRecord(ev0, s));
-
-  // token starts
-  count_starts_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(S_.text.p, b0al, b0, len, S_.bcnt.p);
-  VCHK(v, hipGetLastError());
-  {
-    size_t need = 0;
-    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, S_.bcnt.p, S_.boff.p, n_blk, s));
-    VCHK(v, tmp(need));
-    VCHK(v, hipcub::DeviceScan::ExclusiveSum(S_.tmp.p, need, S_.bcnt.p, S_.boff.p, n_blk, s));
-  }
-  uint64_t last_off = 0, last_cnt = 0;
-  VCHK(v, hipMemcpyAsync(&last_off, S_.boff.p + n_blk - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  VCHK(v, hipMemcpyAsync(&last_cnt, S_.bcnt.p + n_blk - 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  VCHK(v, hipStreamSynchronize(s));
-  int64_t K = (int64_t)(last_off + last_cnt);
-  if (cut && K > 0) --K;
-  if (K > K_max) return v->fail(GTP_ERR_DEVICE, "gtp_vcd_parse: token count above its bound");
-  write_starts_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(S_.text.p, b0al, b0, len, S_.boff.p, S_.tok.p, K);
-  VCHK(v, hipGetLastError());
-
-  // tokens
-  int64_t n_change = 0;
-  int64_t t_max = 0;
-  if (K > 0) {
-    parse_tokens_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(S_.text.p, len, (int64_t)text_bytes, b0al,
-                                                                  S_.boff.p, n_blk, S_.tok.p, K, S_.kind.p,
-                                                                  S_.len.p, S_.val.p, S_.skey.p);
-    VCHK(v, hipGetLastError());
-    run_start_in_kernel<<<nblk(K), 256, 0, s>>>(S_.kind.p, K, S_.rs_in.p);
-    VCHK(v, hipGetLastError());
-    size_t need = 0;
-    VCHK(v, hipcub::DeviceScan::InclusiveScan(nullptr, need, S_.rs_in.p, S_.rs.p, MaxI32(), K, s));
-    VCHK(v, tmp(need));
-    VCHK(v, hipcub::DeviceScan::InclusiveScan(S_.tmp.p, need, S_.rs_in.p, S_.rs.p, MaxI32(), K, s));
-    time_in_kernel<<<nblk(K), 256, 0, s>>>(S_.kind.p, S_.rs.p, K, S_.ti_in.p, S_.is_change.p, d_err);
-    VCHK(v, hipGetLastError());
-    VCHK(v, hipcub::DeviceScan::InclusiveScan(nullptr, need, S_.ti_in.p, S_.ti.p, MaxI32(), K, s));
-    VCHK(v, tmp(need));
-    VCHK(v, hipcub::DeviceScan::InclusiveScan(S_.tmp.p, need, S_.ti_in.p, S_.ti.p, MaxI32(), K, s));
-    fill_u32_kernel<<<1, 1, 0, s>>>(S_.is_change.p + K, 1, 0u);
-    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, S_.is_change.p, S_.chpos.p, K + 1, s));
-    VCHK(v, tmp(need));
-    VCHK(v, hipcub::DeviceScan::ExclusiveSum(S_.tmp.p, need, S_.is_change.p, S_.chpos.p, K + 1, s));
-    uint32_t nc = 0;
-    int32_t tlast = -1;
-    VCHK(v, hipMemcpyAsync(&nc, S_.chpos.p + K, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    VCHK(v, hipMemcpyAsync(&tlast, S_.ti.p + K - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    VCHK(v, hipStreamSynchronize(s));
-    n_change = nc;
-    if (tlast >= 0) {
-      VCHK(v, hipMemcpyAsync(&t_max, S_.val.p + tlast, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-      VCHK(v, hipStreamSynchronize(s));
-    }
-  }
-
-  // changes in text order, grouped by id (stable), payload gathered once
-  const int64_t NC1 = std::max<int64_t>(n_change, 1);
-  VCHK(v, grow(S_.ch_uid, NC1));
-  VCHK(v, grow(S_.key, NC1));
-  VCHK(v, grow(S_.ch_pack, NC1));
-  VCHK(v, grow(S_.s_pack, NC1));
-  VCHK(v, grow(S_.ch_val, NC1));
-  VCHK(v, grow(S_.s_time, NC1));
-  VCHK(v, grow(S_.s_val, NC1));
-  VCHK(v, grow(S_.s_bus, NC1));
-  VCHK(v, grow(S_.keep, NC1 + 1));
-  VCHK(v, grow(S_.ks, NC1 + 1));
-  VCHK(v, grow(S_.okey, NC1));
-  VCHK(v, grow(S_.omax, NC1));
   const int64_t U1 = std::max<int64_t>(U, 1);
-  fill_i64_kernel<<<nblk(U1), 256, 0, s>>>(S_.seg_start.p, U1, -1);
-  fill_i64_kernel<<<nblk(U1), 256, 0, s>>>(S_.seg_end.p, U1, 0);
-  fill_u32_kernel<<<nblk(U1), 256, 0, s>>>(S_.mixed.p, U1, 0u);
+  VCHK(v, grow(S_.c_has, U1));
+  VCHK(v, grow(S_.c_bus, U1));
+  VCHK(v, grow(S_.c_val, U1));
+  VCHK(v, grow(S_.c_time, U1));
+  fill_u8_kernel<<<nblk(U1), 256, 0, s>>>(S_.c_has.p, U1, 0);
   VCHK(v, hipGetLastError());
-  if (n_change > 0) {
-    ChangeArgs ca{S_.text.p, S_.tok.p, S_.kind.p, S_.len.p, S_.val.p, S_.skey.p, S_.ti.p, S_.chpos.p,
-                  S_.is_change.p, K, S_.direct.p, S_.tab.p, tsz - 1, S_.idc.p, S_.ido.p, S_.ch_uid.p,
-                  S_.ch_pack.p, S_.ch_val.p, d_err};
-    emit_changes_kernel<<<nblk(K), 256, 0, s>>>(ca);
+  const Carry carry{S_.c_has.p, S_.c_bus.p, S_.c_val.p, S_.c_time.p};
+
+  // one part: its store is the result; several: each part's store is kept
+  // until the per-stream totals are known
+  std::vector<std::unique_ptr<DArr<uint32_t>>> part_ev;
+  std::vector<std::unique_ptr<DArr<int64_t>>> part_off;
+  int64_t t_now = 0, n_token = 0, n_change = 0;
+  double ms = 0;
+  if (n_part > 1) {
+    VCHK(v, grow(S_.tot, S + 1));
+    VCHK(v, grow(S_.done, S + 1));
+    fill_i64_kernel<<<nblk(S + 1), 256, 0, s>>>(S_.tot.p, S + 1, 0);
+    fill_i64_kernel<<<nblk(S + 1), 256, 0, s>>>(S_.done.p, S + 1, 0);
     VCHK(v, hipGetLastError());
-    int bits = 1;
-    while (bits < 32 && (1ll << bits) < U) ++bits;
-    size_t need = 0;
-    VCHK(v, hipcub::DeviceRadixSort::SortPairs(nullptr, need, S_.ch_uid.p, S_.key.p, S_.ch_pack.p, S_.s_pack.p,
-                                               n_change, 0, bits, s));
-    VCHK(v, tmp(need));
-    VCHK(v, hipcub::DeviceRadixSort::SortPairs(S_.tmp.p, need, S_.ch_uid.p, S_.key.p, S_.ch_pack.p, S_.s_pack.p,
-                                               n_change, 0, bits, s));
-    unpack_kernel<<<nblk(n_change), 256, 0, s>>>(S_.s_pack.p, n_change, S_.ch_val.p, S_.s_time.p, S_.s_val.p,
-                                                 S_.s_bus.p);
-    VCHK(v, hipGetLastError());
-    keep_kernel<<<nblk(n_change), 256, 0, s>>>(S_.key.p, n_change, S_.s_val.p, S_.s_bus.p, S_.seg_start.p,
-                                               S_.seg_end.p, S_.mixed.p, S_.keep.p);
-    VCHK(v, hipGetLastError());
-    if (U > 0) {
-      keep_mixed_kernel<<<nblk(U), 256, 0, s>>>(S_.mixed.p, U, S_.seg_start.p, S_.seg_end.p, S_.s_val.p,
-                                                S_.s_bus.p, S_.keep.p);
+  }
+  for (int64_t i = 0; i < n_part; ++i) {
+    const bool last = i + 1 == n_part;
+    PartIn in{};
+    in.t = text + bounds[i];
+    in.len = bounds[i + 1] - bounds[i];
+    in.b0 = i == 0 ? b0 : 0;
+    in.cut = last && in.len > in.b0 && !host_ws((unsigned char)in.t[in.len - 1]);
+    in.t_carry = t_now;
+    in.carry_out = !last;
+    in.tok_before = n_token;
+    in.chg_before = n_change;
+    if (n_part == 1) {
+      in.ev = &v->d_ev;
+      in.off = &v->d_off;
+      in.pad = true;
+    } else {
+      part_ev.emplace_back(new DArr<uint32_t>());
+      part_off.emplace_back(new DArr<int64_t>());
+      in.ev = part_ev.back().get();
+      in.off = part_off.back().get();
+      in.pad = false;
+    }
+    PartOut o;
+    const gtp_status st = parse_part(v, in, &o, tsz, carry);
+    if (st != GTP_OK) return st;
+    if (o.has_time) t_now = o.t_last;
+    n_token += o.K;
+    n_change += o.n_change;
+    ms += o.ms;
+    if (n_part > 1 && S > 0) {
+      add_counts_kernel<<<nblk(S), 256, 0, s>>>(in.off->p, S, S_.tot.p);
       VCHK(v, hipGetLastError());
     }
-    order_key_kernel<<<nblk(n_change), 256, 0, s>>>(S_.key.p, n_change, S_.keep.p, S_.s_time.p, S_.okey.p);
-    VCHK(v, hipGetLastError());
-    VCHK(v, hipcub::DeviceScan::ExclusiveScan(nullptr, need, S_.okey.p, S_.omax.p, MaxU64(), (uint64_t)0,
-                                              n_change, s));
-    VCHK(v, tmp(need));
-    VCHK(v, hipcub::DeviceScan::ExclusiveScan(S_.tmp.p, need, S_.okey.p, S_.omax.p, MaxU64(), (uint64_t)0,
-                                              n_change, s));
-    order_check_kernel<<<nblk(n_change), 256, 0, s>>>(S_.key.p, n_change, S_.keep.p, S_.okey.p, S_.omax.p, d_err);
-    VCHK(v, hipGetLastError());
   }
-  fill_u32_kernel<<<1, 1, 0, s>>>(S_.keep.p + n_change, 1, 0u);
-  {
+  if (n_part > 1) {
+    hipEvent_t ea, eb;
+    VCHK(v, hipEventCreate(&ea));
+    VCHK(v, hipEventCreate(&eb));
+    struct EvGuard {
+      hipEvent_t a, b;
+      ~EvGuard() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+      }
+    } evg{ea, eb};
+    VCHK(v, v->d_off.alloc(S + 1));
+    VCHK(v, hipEventRecord(ea, s));
     size_t need = 0;
-    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, S_.keep.p, S_.ks.p, n_change + 1, s));
-    VCHK(v, tmp(need));
-    VCHK(v, hipcub::DeviceScan::ExclusiveSum(S_.tmp.p, need, S_.keep.p, S_.ks.p, n_change + 1, s));
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, S_.tot.p, v->d_off.p, S + 1, s));
+    VCHK(v, grow(S_.tmp, std::max<size_t>(need, 1)));
+    VCHK(v, hipcub::DeviceScan::ExclusiveSum(S_.tmp.p, need, S_.tot.p, v->d_off.p, S + 1, s));
+    int64_t E = 0;
+    VCHK(v, hipMemcpyAsync(&E, v->d_off.p + S, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    VCHK(v, hipStreamSynchronize(s));
+    VCHK(v, grow(v->d_ev, E + EV_PAD));
+    fill_words_kernel<<<1, 64, 0, s>>>(v->d_ev.p + E, EV_PAD, EV_INF);
+    if (S > 0)
+      for (int64_t i = 0; i < n_part; ++i) {
+        append_part_kernel<<<nblk(S, 4), 256, 0, s>>>(part_ev[i]->p, part_off[i]->p, S, v->d_off.p, S_.done.p,
+                                                      v->d_ev.p);
+        VCHK(v, hipGetLastError());
+      }
+    VCHK(v, hipEventRecord(eb, s));
+    VCHK(v, hipStreamSynchronize(s));
+    float am = 0;
+    VCHK(v, hipEventElapsedTime(&am, ea, eb));
+    ms += am;
   }
-  // streams
-  if (S > 0) {
-    stream_count_kernel<<<nblk(S), 256, 0, s>>>(S_.stream_uid.p, S, S_.seg_start.p, S_.seg_end.p, S_.ks.p,
-                                                S_.s_time.p, S_.scnt.p);
-    VCHK(v, hipGetLastError());
-  }
-  fill_i64_kernel<<<1, 1, 0, s>>>(S_.scnt.p + S, 1, 0);
-  {
-    size_t need = 0;
-    VCHK(v, hipcub::DeviceScan::ExclusiveSum(nullptr, need, S_.scnt.p, v->d_off.p, S + 1, s));
-    VCHK(v, tmp(need));
-    VCHK(v, hipcub::DeviceScan::ExclusiveSum(S_.tmp.p, need, S_.scnt.p, v->d_off.p, S + 1, s));
-  }
-  unsigned long long he = 0;
-  int64_t E = 0;
-  VCHK(v, hipMemcpyAsync(&he, S_.err.p, sizeof(he), hipMemcpyDeviceToHost, s));
-  VCHK(v, hipMemcpyAsync(&E, v->d_off.p + S, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  VCHK(v, hipStreamSynchronize(s));
-  if (he != ~0ull)
-    return v->fail(GTP_ERR_ARG, std::string("vcd: ") + err_text((int)(he & 0xff)) + " (item " +
-                                    std::to_string(he >> 8) + " of the value-change section)");
-  VCHK(v, grow(v->d_ev, E + EV_PAD));
-  fill_words_kernel<<<1, 64, 0, s>>>(v->d_ev.p + E, EV_PAD, EV_INF);
-  if (n_change > 0) {
-    ScatterArgs sa{S_.key.p, n_change, S_.keep.p, S_.ks.p, S_.seg_start.p, S_.s_time.p, S_.s_val.p, S_.s_bus.p,
-                   S_.uid_width.p, S_.stream_base.p, v->d_off.p, v->d_ev.p};
-    scatter_kernel<<<nblk(n_change), 256, 0, s>>>(sa);
-    VCHK(v, hipGetLastError());
-  }
-  VCHK(v, hipEventRecord(ev1, s));
   v->h_off.resize(S + 1);
   VCHK(v, hipMemcpyAsync(v->h_off.data(), v->d_off.p, sizeof(int64_t) * (S + 1), hipMemcpyDeviceToHost, s));
   VCHK(v, hipStreamSynchronize(s));
-  float ms = 0;
-  VCHK(v, hipEventElapsedTime(&ms, ev0, ev1));
   const auto t1 = std::chrono::steady_clock::now();
   v->info = gtp_vcd_info{};
   v->info.n_var = (int64_t)v->vars.size();
   v->info.n_id = U;
   v->info.n_stream = S;
-  v->info.n_event = E;
-  v->info.t_max = t_max;
+  v->info.n_event = v->h_off[S];
+  v->info.t_max = t_now;
   v->info.time_scale = v->time_scale;
-  v->info.n_token = K;
+  v->info.n_token = n_token;
   v->info.n_change = n_change;
   v->info.body_bytes = len - b0;
   v->info.device_ms = ms;
   v->info.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  v->info.n_part = n_part;
   v->parsed = true;
   if (info) *info = v->info;
   return GTP_OK;

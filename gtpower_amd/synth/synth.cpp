@@ -737,7 +737,7 @@ int32_t gtps_vcd_write(const int64_t* off, const int64_t* time, const uint8_t* v
     };
     static const char vch[4] = {'0', '1', 'x', 'z'};
     std::string txt;
-    txt.reserve((size_t)(off[n_net] * 9 + vars.size() * 40 + 4096));
+    txt.reserve((size_t)(vars.size() * 40 + 4096));
     txt += "$date\n\tsynthetic\n$end\n$version\n\tgtpower_amd synthetic trace\n$end\n$timescale\n\t";
     txt += timescale;
     txt += "\n$end\n$scope module top $end\n";
@@ -751,34 +751,34 @@ int32_t gtps_vcd_write(const int64_t* off, const int64_t* time, const uint8_t* v
       }
     }
     txt += "$upscope $end\n$enddefinitions $end\n";
-    // all events by (time, net)
     const int64_t E = off[n_net];
-    std::vector<uint64_t> key((size_t)E);
+    int64_t t_hi = -1;
+    bool small = true;  // times in [0, 2^32), non-decreasing per net
     for (int64_t n = 0; n < n_net; ++n)
-      for (int64_t e = off[n]; e < off[n + 1]; ++e) key[e] = ((uint64_t)time[e] << 32) | (uint64_t)n;
-    std::sort(key.begin(), key.end());
-    std::vector<uint8_t> cur(n_net, 2);
-    std::vector<int64_t> pos(n_net);
-    for (int64_t n = 0; n < n_net; ++n) pos[n] = off[n];
-    std::vector<uint8_t> touched(vars.size(), 0);
-    std::vector<int64_t> tl;
-    char buf[80];
-    int64_t i = 0;
-    bool first_time = true;
-    while (i < E) {
-      const int64_t t = (int64_t)(key[i] >> 32);
+      for (int64_t e = off[n]; e < off[n + 1]; ++e) {
+        small = small && time[e] >= 0 && time[e] < (int64_t(1) << 32) && (e == off[n] || time[e] >= time[e - 1]);
+        t_hi = std::max(t_hi, time[e]);
+      }
+    std::vector<std::string> ids(vars.size());
+    for (size_t v = 0; v < vars.size(); ++v) ids[v] = id_of((int64_t)v);
+    // the changes of one time: "#t", the scalars in net order, then each bus
+    // touched once with its bits' current values
+    auto write_time = [&](std::string& o, int64_t t, const uint32_t* on, const uint8_t* ov, int64_t b, int64_t e,
+                          std::vector<uint8_t>& cur, std::vector<uint8_t>& touched, std::vector<int64_t>& tl,
+                          bool first) {
+      char buf[32];
       std::snprintf(buf, sizeof(buf), "#%lld\n", (long long)t);
-      txt += buf;
-      if (first_time) txt += "$dumpvars\n";
+      o += buf;
+      if (first) o += "$dumpvars\n";
       tl.clear();
-      for (; i < E && (int64_t)(key[i] >> 32) == t; ++i) {
-        const int64_t n = (int64_t)(key[i] & 0xffffffffull);
-        cur[n] = value[pos[n]++];
+      for (int64_t i = b; i < e; ++i) {
+        const int64_t n = on[i];
+        cur[n] = ov[i];
         const int64_t v = var_of[n];
         if (vars[v].second == 1) {
-          txt.push_back(vch[cur[n] & 3]);
-          txt += id_of(v);
-          txt.push_back('\n');
+          o.push_back(vch[cur[n] & 3]);
+          o += ids[v];
+          o.push_back('\n');
         } else if (!touched[v]) {
           touched[v] = 1;
           tl.push_back(v);
@@ -786,26 +786,121 @@ int32_t gtps_vcd_write(const int64_t* off, const int64_t* time, const uint8_t* v
       }
       for (int64_t v : tl) {
         touched[v] = 0;
-        txt.push_back('b');
-        for (int32_t k = vars[v].second - 1; k >= 0; --k) txt.push_back(vch[cur[vars[v].first + k] & 3]);
-        txt.push_back(' ');
-        txt += id_of(v);
-        txt.push_back('\n');
+        o.push_back('b');
+        for (int32_t k = vars[v].second - 1; k >= 0; --k) o.push_back(vch[cur[vars[v].first + k] & 3]);
+        o.push_back(' ');
+        o += ids[v];
+        o.push_back('\n');
       }
-      if (first_time) txt += "$end\n";
-      first_time = false;
+      if (first) o += "$end\n";
+    };
+    // the value changes as texts of consecutive time ranges, concatenated
+    std::vector<std::string> part;
+    std::vector<uint32_t> on((size_t)E);
+    std::vector<uint8_t> ov((size_t)E);
+    if (E == 0) {
+      // no value changes
+    } else if (small && t_hi + 2 <= std::max<int64_t>(int64_t(1) << 24, 2 * E)) {
+      // counting sort by time, per time range in parallel: the events of
+      // range k of every net (a binary search in its sorted times) are
+      // counted, placed and written by one task
+      int threads = (int)std::max(1u, std::thread::hardware_concurrency());
+      if (const char* env = std::getenv("OMP_NUM_THREADS")) threads = std::max(1, std::atoi(env));
+      const int64_t R = std::max<int64_t>(1, std::min<int64_t>(t_hi + 1, 128));
+      auto lo_of = [&](int64_t k) { return (t_hi + 1) * k / R; };
+      auto span = [&](int64_t n, int64_t lo, int64_t hi, int64_t& e0, int64_t& e1) {
+        e0 = std::lower_bound(time + off[n], time + off[n + 1], lo) - time;
+        e1 = std::lower_bound(time + e0, time + off[n + 1], hi) - time;
+      };
+      std::vector<int64_t> at((size_t)(t_hi + 2), 0);
+      parallel_for(R, threads, [&](int64_t k) {
+        const int64_t lo = lo_of(k), hi = lo_of(k + 1);
+        int64_t e0, e1;
+        for (int64_t n = 0; n < n_net; ++n) {
+          span(n, lo, hi, e0, e1);
+          for (int64_t e = e0; e < e1; ++e) ++at[(size_t)time[e] + 1];
+        }
+      });
+      for (int64_t t = 0; t <= t_hi; ++t) at[(size_t)t + 1] += at[(size_t)t];
+      parallel_for(R, threads, [&](int64_t k) {
+        const int64_t lo = lo_of(k), hi = lo_of(k + 1);
+        int64_t e0, e1;
+        for (int64_t n = 0; n < n_net; ++n) {
+          span(n, lo, hi, e0, e1);
+          for (int64_t e = e0; e < e1; ++e) {
+            const int64_t s = at[(size_t)time[e]]++;
+            on[(size_t)s] = (uint32_t)n;
+            ov[(size_t)s] = value[e];
+          }
+        }
+      });
+      // at[t] is now the end of time t's events, at[t - 1] their start
+      int64_t t_first = 0;
+      while (t_first <= t_hi && at[(size_t)t_first] == 0) ++t_first;
+      part.resize((size_t)R);
+      parallel_for(R, threads, [&](int64_t k) {
+        const int64_t lo = lo_of(k), hi = lo_of(k + 1);
+        std::vector<uint8_t> cur(n_net, 2), touched(vars.size(), 0);
+        std::vector<int64_t> tl;
+        // bus bits enter the range with the value of their last earlier event
+        for (const auto& x : vars)
+          if (x.second > 1)
+            for (int32_t q = 0; q < x.second; ++q) {
+              const int64_t n = x.first + q;
+              const int64_t e = std::lower_bound(time + off[n], time + off[n + 1], lo) - time;
+              if (e > off[n]) cur[n] = value[e - 1];
+            }
+        const int64_t b0 = lo ? at[(size_t)lo - 1] : 0;
+        part[k].reserve((size_t)(at[(size_t)hi - 1] - b0) * 6 + 64);
+        for (int64_t t = lo; t < hi; ++t) {
+          const int64_t b = t ? at[(size_t)t - 1] : 0, e = at[(size_t)t];
+          if (e > b) write_time(part[k], t, on.data(), ov.data(), b, e, cur, touched, tl, t == t_first);
+        }
+      });
+    } else {
+      std::vector<uint64_t> key((size_t)E);
+      for (int64_t n = 0; n < n_net; ++n)
+        for (int64_t e = off[n]; e < off[n + 1]; ++e) key[e] = ((uint64_t)time[e] << 32) | (uint64_t)n;
+      std::sort(key.begin(), key.end());
+      std::vector<int64_t> pos(off, off + n_net);
+      for (int64_t i = 0; i < E; ++i) {
+        const int64_t n = (int64_t)(key[i] & 0xffffffffull);
+        on[i] = (uint32_t)n;
+        ov[i] = value[pos[n]++];
+      }
+      std::vector<uint8_t> cur(n_net, 2), touched(vars.size(), 0);
+      std::vector<int64_t> tl;
+      part.resize(1);
+      part[0].reserve((size_t)E * 6 + 64);
+      for (int64_t b = 0; b < E;) {
+        int64_t e = b + 1;
+        while (e < E && (key[e] >> 32) == (key[b] >> 32)) ++e;
+        write_time(part[0], (int64_t)(key[b] >> 32), on.data(), ov.data(), b, e, cur, touched, tl, b == 0);
+        b = e;
+      }
     }
-    std::snprintf(buf, sizeof(buf), "#%lld\n", (long long)t_max);
-    txt += buf;
-    char* m = (char*)std::malloc(txt.size() + 1);
+    {
+      char buf[32];
+      std::snprintf(buf, sizeof(buf), "#%lld\n", (long long)t_max);
+      part.emplace_back(buf);
+    }
+    size_t total = txt.size();
+    for (const auto& x : part) total += x.size();
+    char* m = (char*)std::malloc(total + 1);
     if (!m) {
       g_err = "gtps_vcd_write: out of memory";
       return GTP_ERR_OOM;
     }
+    size_t at_byte = txt.size();
     std::memcpy(m, txt.data(), txt.size());
-    m[txt.size()] = 0;
+    for (auto& x : part) {
+      std::memcpy(m + at_byte, x.data(), x.size());
+      at_byte += x.size();
+      std::string().swap(x);
+    }
+    m[total] = 0;
     *out = m;
-    *out_len = (int64_t)txt.size();
+    *out_len = (int64_t)total;
     return GTP_OK;
   } catch (const std::exception& e) {
     g_err = std::string("gtps_vcd_write: ") + e.what();

@@ -186,7 +186,8 @@ def vcd_text(trace, buses=()):
     if st != 0:
         raise RuntimeError(lib.gtps_last_error().decode() if hasattr(lib, "gtps_last_error") else "gtps_vcd_write")
     try:
-        return C.string_at(out.value, n.value)
+        # (string_at takes its size as a C int: texts of 2 GiB or more would be cut)
+        return bytes(memoryview((C.c_char * n.value).from_address(out.value)))
     finally:
         lib.gtps_free(out)
 

@@ -432,6 +432,7 @@ typedef struct {
   int64_t body_bytes;   /* bytes of the value-change section */
   double device_ms;     /* device time of the parse (text resident on the device) */
   double total_ms;      /* wall time including the text upload */
+  int64_t n_part;       /* parts the text was parsed in (cut at whitespace) */
 } gtp_vcd_info;
 typedef struct {
   const char* name;     /* scope path joined by '/', as the reference names it */
@@ -443,7 +444,13 @@ typedef struct {
 gtp_status gtp_vcd_create(int32_t device, gtp_vcd** out);
 void gtp_vcd_destroy(gtp_vcd* v);
 const char* gtp_vcd_last_error(const gtp_vcd* v);
-/* Parse text[0, len) (under 4 GiB); the result stays on the device. */
+/* Parse text[0, len); the result stays on the device.  A text longer than
+ * the part size (1 GiB; GTP_VCD_PART_BYTES in the environment sets another)
+ * is parsed in parts cut at whitespace (not after a token that starts with
+ * 'b' or 'B', which may take the next token as its id): each part goes
+ * through the device pipeline with the time and every id's last kept change
+ * carried over, and the parts' stores are appended per stream at the end
+ * (device memory: the parts' stores and the final store at once). */
 gtp_status gtp_vcd_parse(gtp_vcd* v, const char* text, int64_t len, gtp_vcd_info* info);
 /* Declared variable i; the strings live until the next parse. */
 gtp_status gtp_vcd_var(const gtp_vcd* v, int64_t i, gtp_vcd_var_info* out);

@@ -39,14 +39,14 @@ def test_struct_layouts_match_header(tmp_path):
     import ctypes as C
     src = tmp_path / "probe.c"
     structs = ["gtp_options", "gtp_table", "gtp_internal_power", "gtp_library", "gtp_design", "gtp_trace",
-               "gtp_result", "gtp_schedule_info"]
+               "gtp_result", "gtp_schedule_info", "gtp_vcd_info", "gtp_vcd_var_info"]
     body = "\n".join(f'printf("%zu\\n", sizeof({s}));' for s in structs)
     src.write_text(f'#include <stdio.h>\n#include "gtpower.h"\nint main(void){{{body} return 0;}}\n')
     exe = tmp_path / "probe"
     subprocess.run(["gcc", "-I", str(REPO / "include"), str(src), "-o", str(exe)], check=True)
     sizes = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     mirrors = [abi.GtpOptions, abi.GtpTable, abi.GtpInternalPower, abi.GtpLibrary, abi.GtpDesign, abi.GtpTrace,
-               abi.GtpResult, abi.GtpScheduleInfo]
+               abi.GtpResult, abi.GtpScheduleInfo, abi.GtpVcdInfo, abi.GtpVcdVarInfo]
     assert sizes == [C.sizeof(m) for m in mirrors]
 
 
