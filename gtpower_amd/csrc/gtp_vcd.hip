@@ -181,13 +181,15 @@ __device__ __forceinline__ void parse_one(At at, int64_t k, int64_t s, int64_t e
     }
     uint64_t acc = 0;
     bool any = false, ovf = false;
-    const uint64_t lim = neg ? (1ull << 63) : (1ull << 63) - 1;
+    // 10 acc + d > lim  <=>  acc > lim / 10 or (acc == lim / 10 and d > lim % 10)
+    const uint64_t q = 922337203685477580ull;  // (2^63 - 1) / 10 = 2^63 / 10
+    const uint64_t r = neg ? 8u : 7u;
     for (; p < e; ++p) {
       const uint8_t c = at(p);
       if (c < '0' || c > '9') break;
       any = true;
       const uint64_t d = c - '0';
-      if (acc > (lim - d) / 10) ovf = true;
+      if (acc > q || (acc == q && d > r)) ovf = true;
       else acc = acc * 10 + d;
     }
     if (!any || ovf) {
@@ -212,12 +214,14 @@ __device__ __forceinline__ void parse_one(At at, int64_t k, int64_t s, int64_t e
       }
       uint64_t acc = 0;
       bool ovf = false;
-      const uint64_t lim = neg ? (1ull << 63) : (1ull << 63) - 1;
+      // 2 acc + d > lim, as above: lim / 2 = 2^62 - 1 or 2^62, lim % 2 = 1 or 0
+      const uint64_t q = neg ? (1ull << 62) : (1ull << 62) - 1;
+      const uint64_t r = neg ? 0u : 1u;
       for (; p < e; ++p) {
         const uint8_t c = at(p);
         if (c != '0' && c != '1') break;
         const uint64_t d = c - '0';
-        if (ovf || acc > (lim - d) / 2) ovf = true;
+        if (ovf || acc > q || (acc == q && d > r)) ovf = true;
         else acc = acc * 2 + d;
       }
       if (ovf) v = neg ? INT64_MIN : INT64_MAX;
@@ -258,21 +262,26 @@ __global__ __launch_bounds__(TB) void parse_tokens_kernel(const uint8_t* __restr
   __syncthreads();
   const int64_t k0 = (int64_t)boff[b];
   const int64_t k1 = min(b + 1 < n_blk ? (int64_t)boff[b + 1] : K, K);
-  const auto at = [&](int64_t p) -> uint8_t {
-    const int64_t o = p - base;
-    return (o >= 0 && o < PT_STAGE) ? st[o] : t[p];
-  };
+  // two readers, so that the staged bytes are read as LDS (one reader that
+  // picks LDS or memory per byte compiles to generic loads)
+  const auto in_stage = [base](int64_t p) -> uint8_t { return st[p - base]; };
+  const auto in_memory = [t](int64_t p) -> uint8_t { return t[p]; };
   for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
     const int64_t s = tok[k];
     int64_t e;
     if (k + 1 < K) {
       e = tok[k + 1];
-      while (e > s + 1 && is_ws(at(e - 1))) --e;
+      if (e - base <= PT_STAGE) {
+        while (e > s + 1 && is_ws(in_stage(e - 1))) --e;
+      } else {
+        while (e > s + 1 && is_ws(in_memory(e - 1))) --e;
+      }
     } else {
       e = s + 1;
-      while (e < L && !is_ws(at(e))) ++e;
+      while (e < L && !is_ws(in_memory(e))) ++e;
     }
-    parse_one(at, k, s, e, kind, len, val, skey);
+    if (e - base <= PT_STAGE) parse_one(in_stage, k, s, e, kind, len, val, skey);
+    else parse_one(in_memory, k, s, e, kind, len, val, skey);
   }
 }
 
@@ -337,6 +346,7 @@ struct ChangeArgs {
   const uint32_t* len;
   const int64_t* val;
   const int32_t* skey;      // short id key of scalar tokens
+  const int32_t* ti_in;     // k for a time token that is not a bus value's id, else -1
   const int32_t* ti;        // inclusive max-scan of time tokens
   const uint32_t* chpos;    // exclusive scan of is_change
   const uint32_t* is_change;
@@ -350,13 +360,19 @@ struct ChangeArgs {
   uint32_t* ch_uid;
   uint64_t* ch_pack;   // bus << 63 | time << 32 | (scalar: value letter, bus: change index)
   uint64_t* ch_val;    // bus value of bus changes
+  uint32_t* flags;     // [0]: set when a time token is below the time before it
   DevErr* err;
 };
 
 // one change per scalar token or (bus token, id token) pair, in text order
 __global__ void emit_changes_kernel(ChangeArgs a) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= a.K || !a.is_change[k]) return;
+  if (k >= a.K) return;
+  if (a.ti_in[k] == k) {
+    const int32_t pk = k > 0 ? a.ti[k - 1] : -1;
+    if (a.val[k] < (pk >= 0 ? a.val[pk] : a.t_carry)) a.flags[0] = 1u;
+  }
+  if (!a.is_change[k]) return;
   const uint32_t c = a.chpos[k];
   const bool bus = a.kind[k] == TK_BUS;
   const uint8_t* id;
@@ -416,10 +432,13 @@ __device__ __forceinline__ bool differs_from(bool bus, uint64_t v, bool last_bus
 // previous change of its id is dropped (exact when the id's changes are all
 // scalar or all bus); ids mixing both are flagged for the sequential pass.
 // The first change of an id in a part compares with the carried one.
+// flags[1] is set when two changes of an id have the same time, or the
+// first has the carried time or an earlier one: only then (or when a time
+// token goes back, flags[0]) can kept times fail to increase.
 __global__ void keep_kernel(const uint32_t* __restrict__ key, int64_t n, const uint64_t* __restrict__ s_val,
-                            const uint8_t* __restrict__ s_bus, Carry c, int64_t* __restrict__ seg_start,
-                            int64_t* __restrict__ seg_end, uint32_t* __restrict__ mixed,
-                            uint32_t* __restrict__ keep) {
+                            const uint8_t* __restrict__ s_bus, const uint32_t* __restrict__ s_time, Carry c,
+                            int64_t* __restrict__ seg_start, int64_t* __restrict__ seg_end,
+                            uint32_t* __restrict__ mixed, uint32_t* __restrict__ keep, uint32_t* __restrict__ flags) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const uint32_t u = key[p];
@@ -431,8 +450,10 @@ __global__ void keep_kernel(const uint32_t* __restrict__ key, int64_t n, const u
   if (!first) {
     if (s_bus[p] != s_bus[p - 1]) mixed[u] = 1;
     k = s_val[p] != s_val[p - 1];
+    if (s_time[p] == s_time[p - 1]) flags[1] = 1u;
   } else if (c.has[u]) {
     k = differs_from(s_bus[p] != 0, s_val[p], c.bus[u] != 0, c.val[u]);
+    if (s_time[p] <= c.time[u]) flags[1] = 1u;
   }
   keep[p] = k ? 1u : 0u;
 }
@@ -654,6 +675,7 @@ struct gtp_vcd {
     DArr<uint64_t> c_val;
     DArr<uint32_t> c_time;
     DArr<int64_t> tot, done;
+    DArr<uint32_t> flags;  // a time token went back; two changes of an id at one time
   } sc;
   std::vector<int64_t> h_off;
   gtp_vcd_info info{};
@@ -922,6 +944,7 @@ struct PartOut {
   int64_t K = 0, n_change = 0, E = 0;
   bool has_time = false;
   int64_t t_last = 0;   // the value of the part's last time token
+  bool order_checked = false;  // the kept times needed the full order check
   float ms = 0;
 };
 
@@ -1046,11 +1069,13 @@ gtp_status parse_part(gtp_vcd* v, const PartIn& in, PartOut* out, uint64_t tsz, 
   fill_i64_kernel<<<nblk(U1), 256, 0, s>>>(S_.seg_start.p, U1, -1);
   fill_i64_kernel<<<nblk(U1), 256, 0, s>>>(S_.seg_end.p, U1, 0);
   fill_u32_kernel<<<nblk(U1), 256, 0, s>>>(S_.mixed.p, U1, 0u);
+  VCHK(v, grow(S_.flags, 2));
+  fill_u32_kernel<<<1, 2, 0, s>>>(S_.flags.p, 2, 0u);
   VCHK(v, hipGetLastError());
   if (n_change > 0) {
-    ChangeArgs ca{S_.text.p, S_.tok.p, S_.kind.p, S_.len.p, S_.val.p, S_.skey.p, S_.ti.p, S_.chpos.p,
-                  S_.is_change.p, K, S_.direct.p, S_.tab.p, tsz - 1, S_.idc.p, S_.ido.p, in.t_carry,
-                  S_.ch_uid.p, S_.ch_pack.p, S_.ch_val.p, d_err};
+    ChangeArgs ca{S_.text.p, S_.tok.p, S_.kind.p, S_.len.p, S_.val.p, S_.skey.p, S_.ti_in.p, S_.ti.p,
+                  S_.chpos.p, S_.is_change.p, K, S_.direct.p, S_.tab.p, tsz - 1, S_.idc.p, S_.ido.p,
+                  in.t_carry, S_.ch_uid.p, S_.ch_pack.p, S_.ch_val.p, S_.flags.p, d_err};
     emit_changes_kernel<<<nblk(K), 256, 0, s>>>(ca);
     VCHK(v, hipGetLastError());
     int bits = 1;
@@ -1064,24 +1089,14 @@ gtp_status parse_part(gtp_vcd* v, const PartIn& in, PartOut* out, uint64_t tsz, 
     unpack_kernel<<<nblk(n_change), 256, 0, s>>>(S_.s_pack.p, n_change, S_.ch_val.p, S_.s_time.p, S_.s_val.p,
                                                  S_.s_bus.p);
     VCHK(v, hipGetLastError());
-    keep_kernel<<<nblk(n_change), 256, 0, s>>>(S_.key.p, n_change, S_.s_val.p, S_.s_bus.p, carry, S_.seg_start.p,
-                                               S_.seg_end.p, S_.mixed.p, S_.keep.p);
+    keep_kernel<<<nblk(n_change), 256, 0, s>>>(S_.key.p, n_change, S_.s_val.p, S_.s_bus.p, S_.s_time.p, carry,
+                                               S_.seg_start.p, S_.seg_end.p, S_.mixed.p, S_.keep.p, S_.flags.p);
     VCHK(v, hipGetLastError());
     if (U > 0) {
       keep_mixed_kernel<<<nblk(U), 256, 0, s>>>(S_.mixed.p, U, S_.seg_start.p, S_.seg_end.p, S_.s_val.p,
                                                 S_.s_bus.p, carry, S_.keep.p);
       VCHK(v, hipGetLastError());
     }
-    order_key_kernel<<<nblk(n_change), 256, 0, s>>>(S_.key.p, n_change, S_.keep.p, S_.s_time.p, S_.okey.p);
-    VCHK(v, hipGetLastError());
-    VCHK(v, hipcub::DeviceScan::ExclusiveScan(nullptr, need, S_.okey.p, S_.omax.p, MaxU64(), (uint64_t)0,
-                                              n_change, s));
-    VCHK(v, tmp(need));
-    VCHK(v, hipcub::DeviceScan::ExclusiveScan(S_.tmp.p, need, S_.okey.p, S_.omax.p, MaxU64(), (uint64_t)0,
-                                              n_change, s));
-    order_check_kernel<<<nblk(n_change), 256, 0, s>>>(S_.key.p, n_change, S_.keep.p, S_.okey.p, S_.omax.p,
-                                                      S_.s_time.p, carry, d_err);
-    VCHK(v, hipGetLastError());
   }
   fill_u32_kernel<<<1, 1, 0, s>>>(S_.keep.p + n_change, 1, 0u);
   {
@@ -1105,9 +1120,32 @@ gtp_status parse_part(gtp_vcd* v, const PartIn& in, PartOut* out, uint64_t tsz, 
   }
   unsigned long long he = 0;
   int64_t E = 0;
+  uint32_t fl[2] = {0, 0};
   VCHK(v, hipMemcpyAsync(&he, S_.err.p, sizeof(he), hipMemcpyDeviceToHost, s));
   VCHK(v, hipMemcpyAsync(&E, in.off->p + S, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  VCHK(v, hipMemcpyAsync(fl, S_.flags.p, sizeof(fl), hipMemcpyDeviceToHost, s));
   VCHK(v, hipStreamSynchronize(s));
+  // Kept times of an id strictly increasing: when the time tokens never go
+  // back (from the carried time on), the changes of an id are in time order,
+  // so only two changes at one time (or a first one at the carried time) can
+  // break it; otherwise every kept change is checked against the largest
+  // earlier kept (id, time) key.
+  out->order_checked = n_change > 0 && (fl[0] || fl[1]);
+  if (out->order_checked) {
+    size_t need = 0;
+    order_key_kernel<<<nblk(n_change), 256, 0, s>>>(S_.key.p, n_change, S_.keep.p, S_.s_time.p, S_.okey.p);
+    VCHK(v, hipGetLastError());
+    VCHK(v, hipcub::DeviceScan::ExclusiveScan(nullptr, need, S_.okey.p, S_.omax.p, MaxU64(), (uint64_t)0,
+                                              n_change, s));
+    VCHK(v, tmp(need));
+    VCHK(v, hipcub::DeviceScan::ExclusiveScan(S_.tmp.p, need, S_.okey.p, S_.omax.p, MaxU64(), (uint64_t)0,
+                                              n_change, s));
+    order_check_kernel<<<nblk(n_change), 256, 0, s>>>(S_.key.p, n_change, S_.keep.p, S_.okey.p, S_.omax.p,
+                                                      S_.s_time.p, carry, d_err);
+    VCHK(v, hipGetLastError());
+    VCHK(v, hipMemcpyAsync(&he, S_.err.p, sizeof(he), hipMemcpyDeviceToHost, s));
+    VCHK(v, hipStreamSynchronize(s));
+  }
   if (he != ~0ull) {
     const int code = (int)(he & 0xff);
     const int64_t item = (int64_t)(he >> 8) + (code == GTP_VCD_ERR_ORDER ? in.chg_before : in.tok_before);

@@ -80,6 +80,10 @@ def test_text_edges(reader):
     same_as_oracle(reader, V.BASIC.replace(b"\n", b"\t \n"), "tabs")
     same_as_oracle(reader, V.BASIC.rstrip(b"\n"), "no final newline")
     same_as_oracle(reader, V.QUIRKS + b"\n", "quirks with a final newline")
+    # a repeat at the same time is dropped (the order check runs and passes);
+    # the time goes back, but no id's kept times do
+    same_as_oracle(reader, V._HEAD + b"1!\n#5\n0!\n0!\n#9\n", "a repeat at one time")
+    same_as_oracle(reader, V._HEAD + b"1!\n#5\n0!\n#3\nb11 \"\n#9\n", "time back, per id forward")
 
 
 @pytest.mark.parametrize("with_buses", [False, True])

@@ -115,6 +115,7 @@ _HEAD = b"$timescale 1ps $end\n$var wire 1 ! a $end\n$var wire 2 \" b [1:0] $end
 ERRORS = [
     ("unknown_id", _HEAD + b"1!\n#5\n1?\n#6\n"),
     ("time_backwards", _HEAD + b"1!\n#5\n0!\n#3\n1!\n#9\n"),
+    ("same_time", _HEAD + b"1!\n#5\n0!\n1!\n#9\n"),  # two kept changes of ! at 5
     ("time_range", _HEAD + b"1!\n#600000000\n0!\n#600000001\n"),
     ("time_syntax", _HEAD + b"1!\n#abc\n0!\n#9\n"),
     ("bus_without_id", _HEAD + b"1!\n#5\nb10\n"),
