@@ -181,16 +181,28 @@ __device__ __forceinline__ void parse_one(At at, int64_t k, int64_t s, int64_t e
     }
     uint64_t acc = 0;
     bool any = false, ovf = false;
-    // 10 acc + d > lim  <=>  acc > lim / 10 or (acc == lim / 10 and d > lim % 10)
-    const uint64_t q = 922337203685477580ull;  // (2^63 - 1) / 10 = 2^63 / 10
-    const uint64_t r = neg ? 8u : 7u;
-    for (; p < e; ++p) {
-      const uint8_t c = at(p);
-      if (c < '0' || c > '9') break;
-      any = true;
-      const uint64_t d = c - '0';
-      if (acc > q || (acc == q && d > r)) ovf = true;
-      else acc = acc * 10 + d;
+    if (e - p <= 9) {
+      // at most 9 digits: no overflow, 32-bit arithmetic
+      uint32_t a32 = 0;
+      for (; p < e; ++p) {
+        const uint8_t c = at(p);
+        if (c < '0' || c > '9') break;
+        any = true;
+        a32 = a32 * 10u + (uint32_t)(c - '0');
+      }
+      acc = a32;
+    } else {
+      // 10 acc + d > lim  <=>  acc > lim / 10 or (acc == lim / 10 and d > lim % 10)
+      const uint64_t q = 922337203685477580ull;  // (2^63 - 1) / 10 = 2^63 / 10
+      const uint64_t r = neg ? 8u : 7u;
+      for (; p < e; ++p) {
+        const uint8_t c = at(p);
+        if (c < '0' || c > '9') break;
+        any = true;
+        const uint64_t d = c - '0';
+        if (acc > q || (acc == q && d > r)) ovf = true;
+        else acc = acc * 10 + d;
+      }
     }
     if (!any || ovf) {
       kd = TK_TIME_BAD;
@@ -214,15 +226,26 @@ __device__ __forceinline__ void parse_one(At at, int64_t k, int64_t s, int64_t e
       }
       uint64_t acc = 0;
       bool ovf = false;
-      // 2 acc + d > lim, as above: lim / 2 = 2^62 - 1 or 2^62, lim % 2 = 1 or 0
-      const uint64_t q = neg ? (1ull << 62) : (1ull << 62) - 1;
-      const uint64_t r = neg ? 0u : 1u;
-      for (; p < e; ++p) {
-        const uint8_t c = at(p);
-        if (c != '0' && c != '1') break;
-        const uint64_t d = c - '0';
-        if (ovf || acc > q || (acc == q && d > r)) ovf = true;
-        else acc = acc * 2 + d;
+      if (e - p <= 31) {
+        // at most 31 digits: no overflow, 32-bit arithmetic
+        uint32_t a32 = 0;
+        for (; p < e; ++p) {
+          const uint8_t c = at(p);
+          if (c != '0' && c != '1') break;
+          a32 = (a32 << 1) | (uint32_t)(c - '0');
+        }
+        acc = a32;
+      } else {
+        // 2 acc + d > lim, as above: lim / 2 = 2^62 - 1 or 2^62, lim % 2 = 1 or 0
+        const uint64_t q = neg ? (1ull << 62) : (1ull << 62) - 1;
+        const uint64_t r = neg ? 0u : 1u;
+        for (; p < e; ++p) {
+          const uint8_t c = at(p);
+          if (c != '0' && c != '1') break;
+          const uint64_t d = c - '0';
+          if (ovf || acc > q || (acc == q && d > r)) ovf = true;
+          else acc = acc * 2 + d;
+        }
       }
       if (ovf) v = neg ? INT64_MIN : INT64_MAX;
       else v = neg ? (int64_t)(0 - acc) : (int64_t)acc;

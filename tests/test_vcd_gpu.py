@@ -84,6 +84,11 @@ def test_text_edges(reader):
     # the time goes back, but no id's kept times do
     same_as_oracle(reader, V._HEAD + b"1!\n#5\n0!\n0!\n#9\n", "a repeat at one time")
     same_as_oracle(reader, V._HEAD + b"1!\n#5\n0!\n#3\nb11 \"\n#9\n", "time back, per id forward")
+    # digits past 32-bit arithmetic: a 40-digit and a saturating 70-digit bus
+    # value, times of 9, 10 and 12 digits (the last two after the changes)
+    same_as_oracle(reader, b"$timescale 1ps $end\n$var wire 64 ! w [63:0] $end\n$var wire 1 # a $end\n"
+                           b"$enddefinitions $end\n#0\nb" + b"1" * 40 + b" !\n1#\n#7\nb" + b"10" * 35 +
+                           b" !\n0#\n#123456789\nb101 !\n#1234567890\n#123456789012\n", "long digit strings")
 
 
 @pytest.mark.parametrize("with_buses", [False, True])
