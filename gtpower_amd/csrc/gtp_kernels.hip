@@ -2036,26 +2036,43 @@ __global__ void gate_reduce_kernel(const long long* __restrict__ pg, const uint3
                                    const CellDev* __restrict__ cells, double* __restrict__ gate_power,
                                    unsigned long long* __restrict__ gate_tog,
                                    unsigned long long* __restrict__ gate_glitch) {
+  // one thread per (schedule position, array): the N_PG sums, then toggles and
+  // glitches (blockIdx.y), each over the tiles with four independent
+  // accumulators (integer sums: the order does not matter)
   const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gi >= n_sched) return;
   const int g = order[gi];
   if (g < 0) return;
+  const int k = (int)blockIdx.y;
   const int64_t plane = (int64_t)n_tile * n_sched;
-  // leakage sums count the gate's cell's own quanta
-  const double norm_leak_g = ldexp(norm_leak, -cells[gate_cell[g]].leak_shift);
-#pragma unroll
-  for (int k = 0; k < N_PG; ++k) {
-    long long v = 0;
-    for (int t = 0; t < n_tile; ++t) v += pg[k * plane + (int64_t)t * n_sched + gi];
-    gate_power[(int64_t)k * n_gate + g] = (double)v * (k == PG_LEAK ? norm_leak_g : norm_energy);
+  if (k < N_PG) {
+    const long long* p = pg + k * plane + gi;
+    long long v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+    int t = 0;
+    for (; t + 4 <= n_tile; t += 4) {
+      v0 += p[(int64_t)t * n_sched];
+      v1 += p[(int64_t)(t + 1) * n_sched];
+      v2 += p[(int64_t)(t + 2) * n_sched];
+      v3 += p[(int64_t)(t + 3) * n_sched];
+    }
+    for (; t < n_tile; ++t) v0 += p[(int64_t)t * n_sched];
+    const long long v = (v0 + v1) + (v2 + v3);
+    // leakage sums count the gate's cell's own quanta
+    const double norm = k == PG_LEAK ? ldexp(norm_leak, -cells[gate_cell[g]].leak_shift) : norm_energy;
+    gate_power[(int64_t)k * n_gate + g] = (double)v * norm;
+  } else {
+    const uint32_t* p = (k == N_PG ? tog : glc) + gi;
+    unsigned long long v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+    int t = 0;
+    for (; t + 4 <= n_tile; t += 4) {
+      v0 += p[(int64_t)t * n_sched];
+      v1 += p[(int64_t)(t + 1) * n_sched];
+      v2 += p[(int64_t)(t + 2) * n_sched];
+      v3 += p[(int64_t)(t + 3) * n_sched];
+    }
+    for (; t < n_tile; ++t) v0 += p[(int64_t)t * n_sched];
+    (k == N_PG ? gate_tog : gate_glitch)[g] = (v0 + v1) + (v2 + v3);
   }
-  unsigned long long tg = 0, gg = 0;
-  for (int t = 0; t < n_tile; ++t) {
-    tg += tog[(int64_t)t * n_sched + gi];
-    gg += glc[(int64_t)t * n_sched + gi];
-  }
-  gate_tog[g] = tg;
-  gate_glitch[g] = gg;
 }
 
 // FP64 design and class totals: per-block partials, then one fixed-order pass.
@@ -2222,7 +2239,7 @@ hipError_t launch_gate_reduce(const long long* pg, const uint32_t* tog, const ui
                               const CellDev* cells, double* gate_power, unsigned long long* gate_tog,
                               unsigned long long* gate_glitch, hipStream_t s) {
   if (n_sched <= 0) return hipSuccess;
-  gate_reduce_kernel<<<dim3((unsigned)((n_sched + 255) / 256)), dim3(256), 0, s>>>(
+  gate_reduce_kernel<<<dim3((unsigned)((n_sched + 255) / 256), N_PG + 2), dim3(256), 0, s>>>(
       pg, tog, glc, order, n_sched, n_tile, n_gate, norm_energy, norm_leak, gate_cell, cells, gate_power,
       gate_tog, gate_glitch);
   return hipGetLastError();
