@@ -52,6 +52,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -671,6 +672,14 @@ struct gtp_vcd {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string error;
+  // text uploads: on a stream of their own through two page-locked staging
+  // buffers, into two device text buffers that parts take in turn (part
+  // i + 1 is uploaded while part i is parsed)
+  hipStream_t up = nullptr;
+  Pinned<uint8_t> stage[2];
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};  // the last copy out of a staging buffer
+  hipEvent_t text_ev[2] = {nullptr, nullptr};   // a text buffer's upload
+  DArr<uint8_t> textb[2];
   // header
   struct Var {
     std::string name, id;
@@ -687,7 +696,7 @@ struct gtp_vcd {
   DArr<int64_t> d_off;              // [n_stream + 1]
   // scratch, kept between parses (grown, never shrunk)
   struct Scratch {
-    DArr<uint8_t> text, idc, kind, s_bus, tmp;
+    DArr<uint8_t> idc, kind, s_bus, tmp;
     DArr<int32_t> tab, direct, stream_uid, uid_width, rs_in, rs, ti_in, ti, skey;
     DArr<int64_t> ido, stream_base, val, seg_start, seg_end, scnt;
     DArr<uint64_t> bcnt, boff, ch_val, ch_pack, s_pack, s_val, okey, omax;
@@ -877,8 +886,17 @@ GTP_API gtp_status gtp_vcd_create(int32_t device, gtp_vcd** out) {
 GTP_API void gtp_vcd_destroy(gtp_vcd* v) {
   if (!v) return;
   (void)hipSetDevice(v->device);
+  if (v->stream) (void)hipStreamSynchronize(v->stream);
+  if (v->up) (void)hipStreamSynchronize(v->up);
   v->d_ev.release();
   v->d_off.release();
+  for (int k = 0; k < 2; ++k) {
+    if (v->stage_ev[k]) (void)hipEventDestroy(v->stage_ev[k]);
+    if (v->text_ev[k]) (void)hipEventDestroy(v->text_ev[k]);
+    v->stage[k].release();
+    v->textb[k].release();
+  }
+  if (v->up) (void)hipStreamDestroy(v->up);
   if (v->stream) (void)hipStreamDestroy(v->stream);
   delete v;
 }
@@ -895,6 +913,55 @@ hipError_t put(DArr<T>& a, const T* src, size_t n, hipStream_t s) {
   hipError_t e = grow(a, std::max<size_t>(n, 1));
   if (e != hipSuccess || n == 0) return e;
   return hipMemcpyAsync(a.p, src, n * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+constexpr int64_t VCD_STAGE_BYTES = int64_t(64) << 20;
+
+// n bytes from src to dst, by up to 8 host threads
+void host_copy(uint8_t* dst, const char* src, int64_t n) {
+  const int th = n >= (int64_t(8) << 20) ? 8 : 1;
+  if (th == 1) {
+    std::memcpy(dst, src, (size_t)n);
+    return;
+  }
+  const int64_t per = (n + th - 1) / th;
+  std::vector<std::thread> pool;
+  for (int k = 0; k < th; ++k) {
+    const int64_t a = k * per, b = std::min(n, a + per);
+    if (a < b) pool.emplace_back([=] { std::memcpy(dst + a, src + a, (size_t)(b - a)); });
+  }
+  for (std::thread& t : pool) t.join();
+}
+
+// The device bytes a part takes: whole 4 KB blocks from its body's 16-byte
+// aligned start, and 16 more, blank past its text.
+size_t part_text_bytes(int64_t len, int64_t b0) {
+  const int64_t b0al = b0 & ~(int64_t)15;
+  const int64_t n_blk = std::max<int64_t>(1, (len - b0al + BPB - 1) / BPB);
+  return (size_t)(b0al + n_blk * BPB + 16);
+}
+
+// A part's text [0, len) into the device buffer dst (text_bytes long, blank
+// past len) on the upload stream: 64 MB pieces copied by the host threads
+// into the two staging buffers in turn (a buffer is refilled once its last
+// copy to the device has ended), each copied on to the device; `done` is
+// recorded after the last.
+hipError_t upload_text(gtp_vcd* v, uint8_t* dst, const char* src, int64_t len, size_t text_bytes, hipEvent_t done) {
+  hipError_t e;
+  for (int k = 0; k < 2; ++k)
+    if ((e = v->stage[k].reserve((size_t)VCD_STAGE_BYTES)) != hipSuccess) return e;
+  int64_t k = 0;
+  for (int64_t off = 0; off < len; off += VCD_STAGE_BYTES, ++k) {
+    const int j = (int)(k & 1);
+    if ((e = hipEventSynchronize(v->stage_ev[j])) != hipSuccess) return e;
+    const int64_t n = std::min(VCD_STAGE_BYTES, len - off);
+    host_copy(v->stage[j].p, src + off, n);
+    if ((e = hipMemcpyAsync(dst + off, v->stage[j].p, (size_t)n, hipMemcpyHostToDevice, v->up)) != hipSuccess)
+      return e;
+    if ((e = hipEventRecord(v->stage_ev[j], v->up)) != hipSuccess) return e;
+  }
+  if ((e = hipMemsetAsync(dst + len, ' ', text_bytes - (size_t)len, v->up)) != hipSuccess) return e;
+  return hipEventRecord(done, v->up);
 }
 }  // namespace
 
@@ -955,6 +1022,8 @@ std::vector<int64_t> plan_parts(const char* t, int64_t len, int64_t b0, int64_t 
 struct PartIn {
   const char* t;
   int64_t len, b0;      // the part's bytes; its body starts at b0
+  const uint8_t* dev_text;  // the part on the device (part_text_bytes long), uploaded when text_ready is
+  hipEvent_t text_ready;
   bool cut;             // the text ends inside the part's last token
   int64_t t_carry;      // the time in force at the part's start
   bool carry_out;       // record the part's last kept change of every id
@@ -981,10 +1050,9 @@ gtp_status parse_part(gtp_vcd* v, const PartIn& in, PartOut* out, uint64_t tsz, 
   const int64_t len = in.len, b0 = in.b0;
   const int64_t b0al = b0 & ~(int64_t)15;
   const int64_t n_blk = std::max<int64_t>(1, (len - b0al + BPB - 1) / BPB);
-  const size_t text_bytes = (size_t)(b0al + n_blk * BPB + 16);
-  VCHK(v, grow(S_.text, text_bytes));
-  VCHK(v, hipMemsetAsync(S_.text.p + len, ' ', text_bytes - (size_t)len, s));
-  VCHK(v, hipMemcpyAsync(S_.text.p, in.t, (size_t)len, hipMemcpyHostToDevice, s));
+  const size_t text_bytes = part_text_bytes(len, b0);
+  const uint8_t* const text_d = in.dev_text;
+  VCHK(v, hipStreamWaitEvent(s, in.text_ready, 0));
   VCHK(v, put(S_.err, &kNoErr, 1, s));
   DevErr* d_err = reinterpret_cast<DevErr*>(S_.err.p);
   // upper bounds of tokens and changes: a token and its whitespace take 2 bytes
@@ -1021,7 +1089,7 @@ gtp_status parse_part(gtp_vcd* v, const PartIn& in, PartOut* out, uint64_t tsz, 
   VCHK(v, hipEventRecord(ev0, s));
 
   // token starts
-  count_starts_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(S_.text.p, b0al, b0, len, S_.bcnt.p);
+  count_starts_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(text_d, b0al, b0, len, S_.bcnt.p);
   VCHK(v, hipGetLastError());
   {
     size_t need = 0;
@@ -1036,13 +1104,13 @@ gtp_status parse_part(gtp_vcd* v, const PartIn& in, PartOut* out, uint64_t tsz, 
   int64_t K = (int64_t)(last_off + last_cnt);
   if (in.cut && K > 0) --K;
   if (K > K_max) return v->fail(GTP_ERR_DEVICE, "gtp_vcd_parse: token count above its bound");
-  write_starts_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(S_.text.p, b0al, b0, len, S_.boff.p, S_.tok.p, K);
+  write_starts_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(text_d, b0al, b0, len, S_.boff.p, S_.tok.p, K);
   VCHK(v, hipGetLastError());
 
   // tokens
   int64_t n_change = 0;
   if (K > 0) {
-    parse_tokens_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(S_.text.p, len, (int64_t)text_bytes, b0al,
+    parse_tokens_kernel<<<dim3((unsigned)n_blk), dim3(TB), 0, s>>>(text_d, len, (int64_t)text_bytes, b0al,
                                                                   S_.boff.p, n_blk, S_.tok.p, K, S_.kind.p,
                                                                   S_.len.p, S_.val.p, S_.skey.p);
     VCHK(v, hipGetLastError());
@@ -1096,7 +1164,7 @@ gtp_status parse_part(gtp_vcd* v, const PartIn& in, PartOut* out, uint64_t tsz, 
   fill_u32_kernel<<<1, 2, 0, s>>>(S_.flags.p, 2, 0u);
   VCHK(v, hipGetLastError());
   if (n_change > 0) {
-    ChangeArgs ca{S_.text.p, S_.tok.p, S_.kind.p, S_.len.p, S_.val.p, S_.skey.p, S_.ti_in.p, S_.ti.p,
+    ChangeArgs ca{text_d, S_.tok.p, S_.kind.p, S_.len.p, S_.val.p, S_.skey.p, S_.ti_in.p, S_.ti.p,
                   S_.chpos.p, S_.is_change.p, K, S_.direct.p, S_.tab.p, tsz - 1, S_.idc.p, S_.ido.p,
                   in.t_carry, S_.ch_uid.p, S_.ch_pack.p, S_.ch_val.p, S_.flags.p, d_err};
     emit_changes_kernel<<<nblk(K), 256, 0, s>>>(ca);
@@ -1287,9 +1355,39 @@ GTP_API gtp_status gtp_vcd_parse(gtp_vcd* v, const char* text, int64_t len, gtp_
     fill_i64_kernel<<<nblk(S + 1), 256, 0, s>>>(S_.done.p, S + 1, 0);
     VCHK(v, hipGetLastError());
   }
+  // part 0's text is uploaded here, part i + 1's by a helper thread while
+  // part i is parsed
+  if (!v->up) {
+    VCHK(v, hipStreamCreate(&v->up));
+    for (int k = 0; k < 2; ++k) {
+      VCHK(v, hipEventCreateWithFlags(&v->stage_ev[k], hipEventDisableTiming));
+      VCHK(v, hipEventCreateWithFlags(&v->text_ev[k], hipEventDisableTiming));
+    }
+  }
+  const auto part_len = [&](int64_t i) { return bounds[i + 1] - bounds[i]; };
+  {
+    const size_t tb0 = part_text_bytes(part_len(0), b0);
+    VCHK(v, grow(v->textb[0], tb0));
+    VCHK(v, upload_text(v, v->textb[0].p, text, part_len(0), tb0, v->text_ev[0]));
+  }
   for (int64_t i = 0; i < n_part; ++i) {
     const bool last = i + 1 == n_part;
+    const int j = (int)(i & 1);
+    std::thread helper;
+    hipError_t up_err = hipSuccess;
+    if (!last) {
+      // buffer 1 - j held part i - 1, whose parse has ended
+      const int jn = 1 - j;
+      const size_t tbn = part_text_bytes(part_len(i + 1), 0);
+      VCHK(v, grow(v->textb[jn], tbn));
+      helper = std::thread([v, jn, tbn, src = text + bounds[i + 1], n = part_len(i + 1), &up_err] {
+        up_err = hipSetDevice(v->device);
+        if (up_err == hipSuccess) up_err = upload_text(v, v->textb[jn].p, src, n, tbn, v->text_ev[jn]);
+      });
+    }
     PartIn in{};
+    in.dev_text = v->textb[j].p;
+    in.text_ready = v->text_ev[j];
     in.t = text + bounds[i];
     in.len = bounds[i + 1] - bounds[i];
     in.b0 = i == 0 ? b0 : 0;
@@ -1311,7 +1409,10 @@ GTP_API gtp_status gtp_vcd_parse(gtp_vcd* v, const char* text, int64_t len, gtp_
     }
     PartOut o;
     const gtp_status st = parse_part(v, in, &o, tsz, carry);
+    if (helper.joinable()) helper.join();
     if (st != GTP_OK) return st;
+    if (up_err != hipSuccess)
+      return v->fail(GTP_ERR_DEVICE, std::string("vcd: text upload: ") + hipGetErrorString(up_err));
     if (o.has_time) t_now = o.t_last;
     n_token += o.K;
     n_change += o.n_change;
