@@ -255,7 +255,8 @@ __device__ __forceinline__ void parse_one(At at, int64_t k, int64_t s, int64_t e
   kind[k] = kd;
   len[k] = (uint32_t)n;
   val[k] = v;
-  skey[k] = kd == TK_SCALAR ? short_id_key(at, s + 1, n - 1) : -1;  // a scalar's id
+  // a scalar's id, else the whole token (the id when it follows a bus value)
+  skey[k] = kd == TK_SCALAR ? short_id_key(at, s + 1, n - 1) : short_id_key(at, s, n);
 }
 
 // One workgroup per text block (the blocks of the token-start count): the
@@ -369,7 +370,7 @@ struct ChangeArgs {
   const uint8_t* kind;
   const uint32_t* len;
   const int64_t* val;
-  const int32_t* skey;      // short id key of scalar tokens
+  const int32_t* skey;      // short id key of a scalar's id part, else of the token
   const int32_t* ti_in;     // k for a time token that is not a bus value's id, else -1
   const int32_t* ti;        // inclusive max-scan of time tokens
   const uint32_t* chpos;    // exclusive scan of is_change
@@ -403,9 +404,16 @@ __global__ void emit_changes_kernel(ChangeArgs a) {
   int64_t n;
   int32_t key;
   if (bus) {
-    id = a.t + a.tok[k + 1];
-    n = a.len[k + 1];
-    key = short_id_key([id](int64_t p) { return id[p]; }, 0, n);
+    // the next token's own key, unless it looks like a scalar change (its
+    // key is then that of its id part)
+    key = a.kind[k + 1] != TK_SCALAR ? a.skey[k + 1] : -1;
+    id = nullptr;
+    n = 0;
+    if (key < 0) {
+      id = a.t + a.tok[k + 1];
+      n = a.len[k + 1];
+      key = short_id_key([id](int64_t p) { return id[p]; }, 0, n);
+    }
   } else {
     id = a.t + a.tok[k] + 1;
     n = (int64_t)a.len[k] - 1;

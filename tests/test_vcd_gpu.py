@@ -89,6 +89,11 @@ def test_text_edges(reader):
     same_as_oracle(reader, b"$timescale 1ps $end\n$var wire 64 ! w [63:0] $end\n$var wire 1 # a $end\n"
                            b"$enddefinitions $end\n#0\nb" + b"1" * 40 + b" !\n1#\n#7\nb" + b"10" * 35 +
                            b" !\n0#\n#123456789\nb101 !\n#1234567890\n#123456789012\n", "long digit strings")
+    # bus values whose ids look like scalar changes ("x", "1a"), beside a
+    # scalar change "1a" of id "a"
+    same_as_oracle(reader, b"$timescale 1ps $end\n$var wire 2 x v [1:0] $end\n$var wire 2 1a w [1:0] $end\n"
+                           b"$var wire 1 a s $end\n$enddefinitions $end\n#0\nb10 x\nb01 1a\n1a\n#5\nb11 x\n"
+                           b"b11 1a\n0a\n#9\n", "bus ids that look like scalar changes")
 
 
 @pytest.mark.parametrize("with_buses", [False, True])
