@@ -184,12 +184,16 @@ def batch_leg(eng, design, w, n_trace, min_cycles, max_cycles, seed, schedule):
             tk["toggle_p"] = float(ps[i])
             t0 = time.perf_counter()
             tr = design.trace(seed=seed + 7919 * (i + 1), **tk)
-            stats["gen_s"] += time.perf_counter() - t0
+            gen_s = time.perf_counter() - t0
+            stats["gen_s"] += gen_s
             stats["events"] += tr.n_event
             stats["max_events"] = max(stats["max_events"], tr.n_event)
             eng.read_power_activities(tr)
-            stats["single_ms"] += eng.power(want=("cycle",)).kernel_ms
+            single = eng.power(want=("cycle",)).kernel_ms
+            stats["single_ms"] += single
             eng.trace = None
+            log(f"batch: trace {i}: {int(lengths[i])} cycles, p {ps[i]:.3f}, {tr.n_event} events "
+                f"(generated in {gen_s:.1f} s), alone {single:.1f} ms")
             yield tr
 
     with HipPower(tile_cycles=tc, gates_per_wg=rg) as eb:

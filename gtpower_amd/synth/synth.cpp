@@ -23,6 +23,8 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <malloc.h>
+#include <memory>
 #include <new>
 #include <string>
 #include <thread>
@@ -200,8 +202,10 @@ struct gtps_trace {
   int64_t N = 0, t_max = 0;
   double time_scale = 0, clk_period = 0;
   std::vector<int64_t> off;
-  std::vector<int64_t> time;
-  std::vector<uint8_t> value;
+  // left uninitialised until the nets are copied in, so that pages are taken
+  // only as the per-net waves are freed (peak ~10 B per event, not 18)
+  std::unique_ptr<int64_t[]> time;
+  std::unique_ptr<uint8_t[]> value;
 };
 
 extern "C" {
@@ -662,14 +666,21 @@ int32_t gtps_trace_create(const gtps_design* d, const gtps_trace_params* p, gtps
     tr->off[0] = 0;
     for (int64_t n = 0; n < N; ++n) tr->off[n + 1] = tr->off[n] + (int64_t)w[n].t.size();
     const int64_t E = tr->off[N];
-    tr->time.resize(E);
-    tr->value.resize(E);
-    parallel_for(N, threads, [&](int64_t n) {
-      std::copy(w[n].t.begin(), w[n].t.end(), tr->time.begin() + tr->off[n]);
-      std::copy(w[n].v.begin(), w[n].v.end(), tr->value.begin() + tr->off[n]);
-      std::vector<int64_t>().swap(w[n].t);
-      std::vector<uint8_t>().swap(w[n].v);
-    });
+    tr->time.reset(new int64_t[(size_t)std::max<int64_t>(E, 1)]);
+    tr->value.reset(new uint8_t[(size_t)std::max<int64_t>(E, 1)]);
+    // in 64 groups of nets, the freed waves handed back to the system after
+    // each (most are small heap blocks, which free() alone keeps)
+    for (int64_t gq = 0; gq < 64; ++gq) {
+      const int64_t n0 = N * gq / 64, n1 = N * (gq + 1) / 64;
+      parallel_for(n1 - n0, threads, [&](int64_t i) {
+        const int64_t n = n0 + i;
+        std::copy(w[n].t.begin(), w[n].t.end(), tr->time.get() + tr->off[n]);
+        std::copy(w[n].v.begin(), w[n].v.end(), tr->value.get() + tr->off[n]);
+        std::vector<int64_t>().swap(w[n].t);
+        std::vector<uint8_t>().swap(w[n].v);
+      });
+      malloc_trim(0);
+    }
     *out = tr;
     return GTP_OK;
   } catch (const std::bad_alloc&) {
@@ -683,8 +694,8 @@ void gtps_trace_free(gtps_trace* t) { delete t; }
 void gtps_trace_view(const gtps_trace* t, gtp_trace* v) {
   v->n_net = t->N;
   v->net_event_offset = t->off.data();
-  v->event_time = t->time.data();
-  v->event_value = t->value.data();
+  v->event_time = t->time.get();
+  v->event_value = t->value.get();
   v->t_max = t->t_max;
   v->time_scale = t->time_scale;
   v->clk_period = t->clk_period;
