@@ -1365,12 +1365,10 @@ GTP_API gtp_status gtp_vcd_parse(gtp_vcd* v, const char* text, int64_t len, gtp_
   }
   // part 0's text is uploaded here, part i + 1's by a helper thread while
   // part i is parsed
-  if (!v->up) {
-    VCHK(v, hipStreamCreate(&v->up));
-    for (int k = 0; k < 2; ++k) {
-      VCHK(v, hipEventCreateWithFlags(&v->stage_ev[k], hipEventDisableTiming));
-      VCHK(v, hipEventCreateWithFlags(&v->text_ev[k], hipEventDisableTiming));
-    }
+  if (!v->up) VCHK(v, hipStreamCreate(&v->up));
+  for (int k = 0; k < 2; ++k) {
+    if (!v->stage_ev[k]) VCHK(v, hipEventCreateWithFlags(&v->stage_ev[k], hipEventDisableTiming));
+    if (!v->text_ev[k]) VCHK(v, hipEventCreateWithFlags(&v->text_ev[k], hipEventDisableTiming));
   }
   const auto part_len = [&](int64_t i) { return bounds[i + 1] - bounds[i]; };
   {
