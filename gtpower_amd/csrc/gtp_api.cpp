@@ -4,6 +4,7 @@
 // flattening of gates, pins, tables and delays (initGateData :1601-1816), the
 // LUT construction (:2347-2624), the event store (:1874-2130), the schedule
 // (:2718-2849), the launches (:2851-2957) and the result merge (:2959-3132).
+#include <emmintrin.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,6 +43,30 @@ int default_threads() {
   CPU_ZERO(&cs);
   if (sched_getaffinity(0, sizeof(cs), &cs) == 0 && CPU_COUNT(&cs) > 0) return CPU_COUNT(&cs);
   return (int)std::max(1u, std::thread::hardware_concurrency());
+}
+
+// Event words (t - base) << EV_SHIFT | (v & 3) of n events, written with
+// stores that bypass the cache (the destination lines are not read first:
+// 13 instead of 17 bytes of host memory traffic per event), four at a time;
+// the fence makes them visible before the caller hands the words on.
+void pack_words_nt(uint32_t* d, const int64_t* tm, const uint8_t* val, int64_t n, int64_t base) {
+  int64_t i = 0;
+  for (; i < n && ((uintptr_t)(d + i) & 15u) != 0; ++i)
+    d[i] = ((uint32_t)(tm[i] - base) << EV_SHIFT) | (uint32_t)(val[i] & 3u);
+  const __m128i vb = _mm_set1_epi64x(base), m3 = _mm_set1_epi32(3), z = _mm_setzero_si128();
+  for (; i + 4 <= n; i += 4) {
+    const __m128i a = _mm_sub_epi64(_mm_loadu_si128(reinterpret_cast<const __m128i*>(tm + i)), vb);
+    const __m128i b = _mm_sub_epi64(_mm_loadu_si128(reinterpret_cast<const __m128i*>(tm + i + 2)), vb);
+    // the low 32 bits of the four differences
+    const __m128i lo = _mm_unpacklo_epi64(_mm_shuffle_epi32(a, _MM_SHUFFLE(2, 0, 2, 0)),
+                                          _mm_shuffle_epi32(b, _MM_SHUFFLE(2, 0, 2, 0)));
+    int32_t v4;
+    std::memcpy(&v4, val + i, 4);
+    const __m128i vv = _mm_and_si128(_mm_unpacklo_epi16(_mm_unpacklo_epi8(_mm_cvtsi32_si128(v4), z), z), m3);
+    _mm_stream_si128(reinterpret_cast<__m128i*>(d + i), _mm_or_si128(_mm_slli_epi32(lo, EV_SHIFT), vv));
+  }
+  for (; i < n; ++i) d[i] = ((uint32_t)(tm[i] - base) << EV_SHIFT) | (uint32_t)(val[i] & 3u);
+  _mm_sfence();
 }
 
 template <class F>
@@ -731,9 +756,8 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
       buf[which].resize(n);
       uint32_t* d = buf[which].data();
       parallel_for((n + 65535) / 65536, x->threads, [&](int64_t blk) {
-        const int64_t i1 = std::min(n, (blk + 1) * 65536);
-        for (int64_t i = blk * 65536; i < i1; ++i)
-          d[i] = ((uint32_t)tm[b + i] << EV_SHIFT) | (uint32_t)(val[b + i] & 3u);
+        const int64_t i0 = blk * 65536, i1 = std::min(n, (blk + 1) * 65536);
+        pack_words_nt(d + i0, tm + b + i0, val + b + i0, i1 - i0, 0);
       });
     };
     if (E > 0) pack(0, 0);
@@ -2121,7 +2145,8 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
       *d++ = ((uint32_t)(t - base) << EV_SHIFT) | (uint32_t)(val[i] & 3u);
     }
     // the interval's own events (the bulk): times relative to the origin
-    for (int64_t i = i0; i < i1; ++i) *d++ = ((uint32_t)(tm[i] - base) << EV_SHIFT) | (uint32_t)(val[i] & 3u);
+    pack_words_nt(d, tm + i0, val + i0, i1 - i0, base);
+    d += i1 - i0;
     for (int64_t i = i1; i < b; ++i) {
       const int64_t t = (i == i1 || tm[i] / T == tm[i1] / T) ? tm[i] - ashift : std::min(tm[i], (ac_new + 1) * T);
       *d++ = ((uint32_t)(t - base) << EV_SHIFT) | (uint32_t)(val[i] & 3u);
@@ -2353,8 +2378,7 @@ gtp_status batch_add(gtp_batch* b, const gtp_trace* tr, gtp_result* res) {
   {
     const gtp_status st = staged_upload(x, off, N, b->d_words.p + b->words_used,
                                         [&](int64_t n, uint32_t* d) {
-                                          for (int64_t i = off[n]; i < off[n + 1]; ++i)
-                                            *d++ = ((uint32_t)tm[i] << EV_SHIFT) | (uint32_t)(val[i] & 3u);
+                                          pack_words_nt(d, tm + off[n], val + off[n], off[n + 1] - off[n], 0);
                                         },
                                         b->ready);
     if (st != GTP_OK) return st;
