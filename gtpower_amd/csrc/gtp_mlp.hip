@@ -96,10 +96,20 @@ __device__ __forceinline__ uint4 expand8(uint32_t b, uint32_t one = 0x3F80u) {
   return v;
 }
 
+// The operand words one thread moves from global memory into the LDS tiles
+// for one BK step (which fields are used depends on the layouts).
+struct TileRegs {
+  uint4 a0, a1, b0, b1;
+  uint32_t aw, bw;
+};
+
 template <int AM, int BMODE, int EPI>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
-  __shared__ __align__(16) bf16 As[BM * LDSK];
-  __shared__ __align__(16) bf16 Bs[BN * LDSK];
+  // two LDS buffers per operand: the step being multiplied reads one while the
+  // next step's operands, loaded into registers before the multiply, go into
+  // the other (one barrier per step)
+  __shared__ __align__(16) bf16 As[2][BM * LDSK];
+  __shared__ __align__(16) bf16 Bs[2][BN * LDSK];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -110,99 +120,135 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  for (int k0 = 0; k0 < g.K; k0 += BK) {
-    // ---- A tile -> As[m][k]
+  // global -> registers for the step at k0
+  const auto fetch = [&](int k0) {
+    TileRegs r{};
     if (AM == A_ROW) {
       const bf16* A = (const bf16*)g.A;
-      const int r = tid >> 1, ks = (tid & 1) * 16;
-      uint4 v0 = {0, 0, 0, 0}, v1 = {0, 0, 0, 0};
-      if (m0 + r < g.M) {
-        const uint4* src = (const uint4*)(A + (int64_t)(m0 + r) * g.lda + k0 + ks);
-        v0 = src[0];
-        v1 = src[1];
+      const int row = tid >> 1, ks = (tid & 1) * 16;
+      if (m0 + row < g.M) {
+        const uint4* src = (const uint4*)(A + (int64_t)(m0 + row) * g.lda + k0 + ks);
+        r.a0 = src[0];
+        r.a1 = src[1];
       }
-      uint4* dst = (uint4*)(As + r * LDSK + ks);
-      dst[0] = v0;
-      dst[1] = v1;
     } else if (AM == A_COL) {
       // A[m][k] = S[k][m], S row-major with stride lda
       // (M is a multiple of 16 in this mode: whole 16-row groups or none)
       const bf16* S = (const bf16*)g.A;
       const int kk = tid >> 3, ms = (tid & 7) * 16;
+      if (m0 + ms < g.M) {
+        const uint4* src = (const uint4*)(S + (int64_t)(k0 + kk) * g.lda + m0 + ms);
+        r.a0 = src[0];
+        r.a1 = src[1];
+      }
+    } else {
+      // bit-packed rows: A[m][k] = bit k of row m
+      const uint32_t* X = (const uint32_t*)g.A;
+      const int row = tid >> 1;
+      r.aw = (m0 + row < g.M) ? X[(int64_t)(m0 + row) * g.lda + (k0 >> 5)] : 0u;
+    }
+    if (BMODE == B_TRANS) {
+      // B[k][n] = W[n][k], W row-major with stride ldb
+      const bf16* W = (const bf16*)g.B;
+      const int row = tid >> 1, ks = (tid & 1) * 16;
+      const uint4* src = (const uint4*)(W + (int64_t)(n0 + row) * g.ldb + k0 + ks);
+      r.b0 = src[0];
+      r.b1 = src[1];
+    } else if (BMODE == B_ROW) {
+      const bf16* Bm = (const bf16*)g.B;
+      const int kk = tid >> 3, ns = (tid & 7) * 16;
+      const uint4* src = (const uint4*)(Bm + (int64_t)(k0 + kk) * g.ldb + n0 + ns);
+      r.b0 = src[0];
+      r.b1 = src[1];
+    } else {
+      // B[k][n] = bit n of bit-packed row k
+      const uint32_t* X = (const uint32_t*)g.B;
+      if (tid < BN) r.bw = X[(int64_t)(k0 + (tid >> 2)) * g.ldb + (n0 >> 5) + (tid & 3)];
+    }
+    return r;
+  };
+  // registers -> LDS buffer buf (As[m][k], Bs[n][k])
+  const auto stash = [&](const TileRegs& r, int k0, int buf) {
+    bf16* as = As[buf];
+    bf16* bs = Bs[buf];
+    if (AM == A_ROW) {
+      const int row = tid >> 1, ks = (tid & 1) * 16;
+      uint4* dst = (uint4*)(as + row * LDSK + ks);
+      dst[0] = r.a0;
+      dst[1] = r.a1;
+    } else if (AM == A_COL) {
+      const int kk = tid >> 3, ms = (tid & 7) * 16;
       union {
         uint4 u[2];
         bf16 h[16];
       } v;
-      v.u[0] = v.u[1] = uint4{0, 0, 0, 0};
-      if (m0 + ms < g.M) {
-        const uint4* src = (const uint4*)(S + (int64_t)(k0 + kk) * g.lda + m0 + ms);
-        v.u[0] = src[0];
-        v.u[1] = src[1];
-      }
+      v.u[0] = r.a0;
+      v.u[1] = r.a1;
 #pragma unroll
-      for (int j = 0; j < 16; ++j) As[(ms + j) * LDSK + kk] = v.h[j];
+      for (int j = 0; j < 16; ++j) as[(ms + j) * LDSK + kk] = v.h[j];
     } else {
-      // bit-packed rows: A[m][k] = bit k of row m
-      const uint32_t* X = (const uint32_t*)g.A;
-      const int r = tid >> 1, h = tid & 1;
-      uint32_t w = (m0 + r < g.M) ? X[(int64_t)(m0 + r) * g.lda + (k0 >> 5)] : 0u;
-      uint32_t one = 0x3F80u;
+      const int row = tid >> 1, h = tid & 1;
+      uint32_t w = r.aw, one = 0x3F80u;
       if (g.drop_seed) {
-        w &= keep_mask(g.drop_seed, (uint32_t)(m0 + r), (uint32_t)(k0 >> 5));
+        w &= keep_mask(g.drop_seed, (uint32_t)(m0 + row), (uint32_t)(k0 >> 5));
         one = 0x4000u;
       }
-      uint4* dst = (uint4*)(As + r * LDSK + h * 16);
+      uint4* dst = (uint4*)(as + row * LDSK + h * 16);
       dst[0] = expand8((w >> (h * 16)) & 0xFFu, one);
       dst[1] = expand8((w >> (h * 16 + 8)) & 0xFFu, one);
     }
-    // ---- B tile -> Bs[n][k]
     if (BMODE == B_TRANS) {
-      // B[k][n] = W[n][k], W row-major with stride ldb
-      const bf16* W = (const bf16*)g.B;
-      const int r = tid >> 1, ks = (tid & 1) * 16;
-      const uint4* src = (const uint4*)(W + (int64_t)(n0 + r) * g.ldb + k0 + ks);
-      uint4* dst = (uint4*)(Bs + r * LDSK + ks);
-      dst[0] = src[0];
-      dst[1] = src[1];
+      const int row = tid >> 1, ks = (tid & 1) * 16;
+      uint4* dst = (uint4*)(bs + row * LDSK + ks);
+      dst[0] = r.b0;
+      dst[1] = r.b1;
     } else if (BMODE == B_ROW) {
-      const bf16* Bm = (const bf16*)g.B;
       const int kk = tid >> 3, ns = (tid & 7) * 16;
       union {
         uint4 u[2];
         bf16 h[16];
       } v;
-      const uint4* src = (const uint4*)(Bm + (int64_t)(k0 + kk) * g.ldb + n0 + ns);
-      v.u[0] = src[0];
-      v.u[1] = src[1];
+      v.u[0] = r.b0;
+      v.u[1] = r.b1;
 #pragma unroll
-      for (int j = 0; j < 16; ++j) Bs[(ns + j) * LDSK + kk] = v.h[j];
+      for (int j = 0; j < 16; ++j) bs[(ns + j) * LDSK + kk] = v.h[j];
     } else {
-      // B[k][n] = bit n of bit-packed row k
-      const uint32_t* X = (const uint32_t*)g.B;
       if (tid < BN) {
         const int kk = tid >> 2, wi = tid & 3;
-        uint32_t w = X[(int64_t)(k0 + kk) * g.ldb + (n0 >> 5) + wi];
+        uint32_t w = r.bw;
         bf16 one = (bf16)1.0f;
         if (g.drop_seed) {
           w &= keep_mask(g.drop_seed, (uint32_t)(k0 + kk), (uint32_t)((n0 >> 5) + wi));
           one = (bf16)2.0f;
         }
 #pragma unroll
-        for (int b = 0; b < 32; ++b) Bs[(wi * 32 + b) * LDSK + kk] = ((w >> b) & 1u) ? one : (bf16)0.0f;
+        for (int b = 0; b < 32; ++b) bs[(wi * 32 + b) * LDSK + kk] = ((w >> b) & 1u) ? one : (bf16)0.0f;
       }
     }
-    __syncthreads();
+  };
+
+  const int nk = g.K / BK;
+  stash(fetch(0), 0, 0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    // the next step's operands, in flight during the multiply (the last step
+    // loads its own again, unused)
+    const TileRegs r = fetch((more ? kt + 1 : kt) * BK);
     // ---- 4 x 4 MFMA 16x16x32 per wave
     bf16x8 af[4], bfr[4];
     const int fr = lane & 15, fk = (lane >> 4) * 8;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) af[i] = *(const bf16x8*)(As + (wm * 64 + i * 16 + fr) * LDSK + fk);
+    for (int i = 0; i < 4; ++i) af[i] = *(const bf16x8*)(As[cur] + (wm * 64 + i * 16 + fr) * LDSK + fk);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) bfr[j] = *(const bf16x8*)(Bs + (wn * 64 + j * 16 + fr) * LDSK + fk);
+    for (int j = 0; j < 4; ++j) bfr[j] = *(const bf16x8*)(Bs[cur] + (wn * 64 + j * 16 + fr) * LDSK + fk);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    // the other buffer was last read in the previous step, before its barrier
+    if (more) stash(r, (kt + 1) * BK, cur ^ 1);
     __syncthreads();
   }
   // ---- epilogue: lane holds D[(lane/16)*4 + e][lane%16] of each 16x16 tile
