@@ -18,6 +18,7 @@
 // bit-reproducible.  Per-gate leakage is the exact integer sum of (state
 // leakage in quanta) * (ticks in that state).
 #pragma once
+#include <vector>
 #include <stdint.h>
 #include <stdlib.h>
 
@@ -359,6 +360,10 @@ int32_t trace_dev(gtp_ctx* ctx, TraceDev* out);
 struct AuxBuffers {
   DArr<int32_t> sel, wq, idx;
   DArr<unsigned long long> out;
+  // what sel and wq hold (uploaded to sel_at), so that a repeated selection
+  // is not uploaded again
+  std::vector<int32_t> sel_h, wq_h;
+  const int32_t* sel_at = nullptr;
 };
 AuxBuffers* aux_buffers(gtp_ctx* ctx);
 // The context's auxiliary stream, created on first use (nullptr on failure).

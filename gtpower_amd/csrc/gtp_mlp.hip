@@ -711,6 +711,7 @@ struct gtp_mlp {
   uint32_t* tr_X = nullptr;
   float* tr_out = nullptr;
   int32_t* tr_net = nullptr;
+  std::vector<int32_t> tr_net_h;  // what tr_net holds
   int64_t tr_X_n = 0, tr_out_n = 0;
 
   float* W1() { return P; }
@@ -1318,8 +1319,19 @@ GTP_API gtp_status gtp_net_weight_sum(gtp_ctx* ctx, int32_t n_sel, const int32_t
   unsigned long long* d_out = ab.out.p;
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
-  if (e == hipSuccess) e = hipMemcpyAsync(d_sel, nets, (size_t)n_sel * 4, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(d_wq, wq.data(), (size_t)n_sel * 4, hipMemcpyHostToDevice, s);
+  // the selection and its weights: uploaded unless the buffers hold them
+  const bool same = e == hipSuccess && ab.sel_at == d_sel && ab.sel_h.size() == (size_t)n_sel &&
+                    std::equal(nets, nets + n_sel, ab.sel_h.begin()) && std::equal(wq.begin(), wq.end(), ab.wq_h.begin());
+  if (!same) {
+    ab.sel_at = nullptr;
+    if (e == hipSuccess) e = hipMemcpyAsync(d_sel, nets, (size_t)n_sel * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_wq, wq.data(), (size_t)n_sel * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+      ab.sel_h.assign(nets, nets + n_sel);
+      ab.wq_h = wq;
+      ab.sel_at = d_sel;
+    }
+  }
   if (e == hipSuccess) e = hipEventRecord(e0, s);
   if (e == hipSuccess) {
     fill_u32_kernel<<<dim3((unsigned)((2 * n_cycle + 255) / 256)), 256, 0, s>>>((uint32_t*)d_out, 2 * n_cycle, 0u);
@@ -1377,7 +1389,13 @@ GTP_API gtp_status gtp_mlp_predict_trace(gtp_mlp* m, gtp_ctx* ctx, const int32_t
   int64_t net_n = m->tr_net ? m->K : 0;
   if (!m->tr_net) MCHK(m, grow(m->tr_net, net_n, m->K, 4));
   FeatureSel fs;
-  MCHK(m, fs.use(m->tr_net, m->K, nets, m->stream));
+  if (m->tr_net_h.size() == (size_t)m->K && std::equal(nets, nets + m->K, m->tr_net_h.begin())) {
+    fs.d_net = m->tr_net;  // the same selection as the last call: already on the device
+  } else {
+    m->tr_net_h.clear();
+    MCHK(m, fs.use(m->tr_net, m->K, nets, m->stream));
+    m->tr_net_h.assign(nets, nets + m->K);
+  }
   hipError_t e = grow(m->tr_X, m->tr_X_n, n_cycle * words, 4);
   if (e == hipSuccess) e = grow(m->tr_out, m->tr_out_n, n_cycle, 4);
   uint32_t* d_X = m->tr_X;
