@@ -69,11 +69,13 @@ void pack_words_nt(uint32_t* d, const int64_t* tm, const uint8_t* val, int64_t n
   _mm_sfence();
 }
 
+// f(i) for i in [0, n) on up to `threads` host threads, each given at least
+// `grain` items (1024 for light items; a few for items that are whole nets)
 template <class F>
-void parallel_for(int64_t n, int threads, F f) {
+void parallel_for(int64_t n, int threads, F f, int64_t grain = 1024) {
   if (n <= 0) return;
   if (threads <= 0) threads = default_threads();
-  threads = (int)std::min<int64_t>(threads, std::max<int64_t>(1, n / 1024));
+  threads = (int)std::min<int64_t>(threads, std::max<int64_t>(1, n / std::max<int64_t>(grain, 1)));
   if (threads <= 1) {
     for (int64_t i = 0; i < n; ++i) f(i);
     return;
@@ -214,12 +216,14 @@ struct gtp_ctx {
   DArr<unsigned int> d_maxbits;
 
   // page-locked staging of packed event words for loads from host memory
-  // (sliced runs, streamed batches): a ring of buffers, each refilled once its
-  // copy to the device has completed, so that packing the next part overlaps
-  // the copy of the last one.  The copies go on the run stream: a stream of
-  // their own would take one of the process's few hardware queues and share
-  // it with the run streams or the surrogate's feature work, which then
-  // serialise (the surrogate's overlapped evaluation slowed by ~10%)
+  // (large traces, sliced runs, streamed batches): a ring of buffers, each
+  // refilled once its copy to the device has completed, so that packing the
+  // next part overlaps the copy of the last one.  The copies go on the
+  // auxiliary stream (aux), so that they do not wait behind the kernels of
+  // the interval being evaluated on the run stream; no stream is added for
+  // them (a stream of their own took one of the process's few hardware queues,
+  // and the surrogate's overlapped evaluation, whose base term uses aux, then
+  // slowed by ~10%)
   static constexpr int N_STAGE = 4;
   static constexpr int64_t STAGE_WORDS = (int64_t)1 << 25;  // 128 MiB a buffer
   Pinned<uint32_t> stage[N_STAGE];
@@ -698,7 +702,7 @@ gtp_status validate_trace(gtp_ctx* x, const gtp_trace* tr, int64_t* T_out, bool 
       }
       prev = t;
     }
-  });
+  }, std::max<int64_t>(1, (int64_t)(65536.0 * (double)N / (double)std::max<int64_t>(E, 1))));  // >= ~64k events a thread
   if (bad.load() >= 0)
     return x->fail(GTP_ERR_ARG, "gtp_trace: events of net " + std::to_string(bad.load()) +
                                     " are not strictly increasing in [0, t_max] or carry a value > 3");
@@ -715,6 +719,12 @@ gtp_status ensure_arc_q(gtp_ctx* x, double time_scale);
 
 namespace {
 gtp_status finish_trace(gtp_ctx* x);
+gtp_status staging_init(gtp_ctx* x);
+template <class F>
+gtp_status staged_upload(gtp_ctx* x, const int64_t* so, int64_t N, uint32_t* dst, F pack_net, hipEvent_t done);
+// traces of at least this many events are uploaded through the page-locked
+// staging ring (smaller ones from a pageable buffer, without the ring's 512 MiB)
+constexpr int64_t STAGED_MIN_EVENTS = (int64_t)1 << 25;
 }  // namespace
 
 GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
@@ -743,14 +753,37 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
   x->h_net_off.assign(off, off + N + 1);
   x->glitch_valid = false;
   HIPCHK(x, x->d_ev.alloc(E + EV_PAD));
+  const int64_t* tm = tr->event_time;
+  const uint8_t* val = tr->event_value;
+  if (E >= STAGED_MIN_EVENTS) {
+    // packed by the host threads into the page-locked ring, a run of nets
+    // per buffer, each buffer copied while the next is packed (4 B per event
+    // uploaded instead of 9)
+    {
+      const gtp_status st = staging_init(x);
+      if (st != GTP_OK) return st;
+    }
+    hipEvent_t done = nullptr;
+    HIPCHK(x, hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    struct EvG {
+      hipEvent_t e;
+      ~EvG() { (void)hipEventDestroy(e); }
+    } evg{done};
+    const gtp_status st = staged_upload(x, off, N, x->d_ev.p,
+                                        [&](int64_t n, uint32_t* d) {
+                                          pack_words_nt(d, tm + off[n], val + off[n], off[n + 1] - off[n], 0);
+                                        },
+                                        done);
+    if (st != GTP_OK) return st;
+    HIPCHK(x, hipEventSynchronize(done));
+    return finish_trace(x);
+  }
   // The event words are packed on the host a chunk at a time (4 B per event
   // uploaded instead of 9), the next chunk on a helper thread while the
   // current one is copied; two buffers alternate.
   {
     const int64_t CH = 1 << 26;
     std::vector<uint32_t> buf[2];
-    const int64_t* tm = tr->event_time;
-    const uint8_t* val = tr->event_value;
     auto pack = [&](int which, int64_t b) {
       const int64_t n = std::min(CH, E - b);
       buf[which].resize(n);
@@ -758,7 +791,7 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
       parallel_for((n + 65535) / 65536, x->threads, [&](int64_t blk) {
         const int64_t i0 = blk * 65536, i1 = std::min(n, (blk + 1) * 65536);
         pack_words_nt(d + i0, tm + b + i0, val + b + i0, i1 - i0, 0);
-      });
+      }, 1);
     };
     if (E > 0) pack(0, 0);
     int k = 0;
@@ -1828,6 +1861,8 @@ gtp_status staging_init(gtp_ctx* x) {
 template <class F>
 gtp_status staged_upload(gtp_ctx* x, const int64_t* so, int64_t N, uint32_t* dst, F pack_net, hipEvent_t done) {
   HIPCHK(x, hipSetDevice(x->device));
+  const hipStream_t us = gtp::aux_stream(x);
+  if (!us) return x->fail(GTP_ERR_DEVICE, "staging: no upload stream");
   const int64_t cap = gtp_ctx::STAGE_WORDS;
   std::vector<uint32_t> big;
   for (int64_t n0 = 0; n0 < N;) {
@@ -1837,9 +1872,8 @@ gtp_status staged_upload(gtp_ctx* x, const int64_t* so, int64_t N, uint32_t* dst
       // one net larger than a buffer: packed whole and copied from pageable memory
       big.resize((size_t)(so[n0 + 1] - so[n0]));
       pack_net(n0, big.data());
-      HIPCHK(x, hipMemcpyAsync(dst + so[n0], big.data(), big.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
-                               x->stream));
-      HIPCHK(x, hipStreamSynchronize(x->stream));
+      HIPCHK(x, hipMemcpyAsync(dst + so[n0], big.data(), big.size() * sizeof(uint32_t), hipMemcpyHostToDevice, us));
+      HIPCHK(x, hipStreamSynchronize(us));
       n0 += 1;
       continue;
     }
@@ -1849,16 +1883,16 @@ gtp_status staged_upload(gtp_ctx* x, const int64_t* so, int64_t N, uint32_t* dst
     x->stage_pending[b] = false;
     uint32_t* h = x->stage[b].p;
     const int64_t base = so[n0];
-    parallel_for(n1 - n0, x->threads, [&](int64_t i) { pack_net(n0 + i, h + (so[n0 + i] - base)); });
+    // (a net is thousands of events: one per thread is enough)
+    parallel_for(n1 - n0, x->threads, [&](int64_t i) { pack_net(n0 + i, h + (so[n0 + i] - base)); }, 1);
     if (so[n1] > base) {
-      HIPCHK(x, hipMemcpyAsync(dst + base, h, (size_t)(so[n1] - base) * sizeof(uint32_t), hipMemcpyHostToDevice,
-                               x->stream));
-      HIPCHK(x, hipEventRecord(x->stage_ev[b], x->stream));
+      HIPCHK(x, hipMemcpyAsync(dst + base, h, (size_t)(so[n1] - base) * sizeof(uint32_t), hipMemcpyHostToDevice, us));
+      HIPCHK(x, hipEventRecord(x->stage_ev[b], us));
       x->stage_pending[b] = true;
     }
     n0 = n1;
   }
-  HIPCHK(x, hipEventRecord(done, x->stream));
+  HIPCHK(x, hipEventRecord(done, us));
   return GTP_OK;
 }
 
@@ -2036,16 +2070,58 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
   // positions and the state before the interval come out as in a whole-trace
   // run; event 0 keeps index 0 when the slice starts there) to the second
   // event at or after the interval's end (the flag of the first one).
-  std::vector<int64_t> sb(N), se(N);
-  auto slice_bounds = [&](int32_t k0, int32_t k1, std::vector<int64_t>& lo, std::vector<int64_t>& hi) {
+  // The first event at or after time t among events [lo, b) of a net (b when
+  // none): a guess interpolated in time, then a galloping search around it,
+  // so that a lookup costs a few cache lines instead of a binary search over
+  // the whole net.  Times are strictly increasing (validated).
+  auto seek = [&](int64_t lo, int64_t b, int64_t t) -> int64_t {
+    if (lo >= b || tm[lo] >= t) return lo;
+    if (tm[b - 1] < t) return b;
+    int64_t L = lo, R = b - 1;  // tm[L] < t <= tm[R]
+    const double f = (double)(t - tm[lo]) / ((double)(tm[b - 1] - tm[lo]) + 1.0);
+    const int64_t g = std::min(b - 1, std::max(lo + 1, lo + (int64_t)(f * (double)(b - 1 - lo))));
+    if (tm[g] >= t) {
+      R = g;
+      for (int64_t step = 1;; step *= 2) {
+        const int64_t c = R - step;
+        if (c <= L) break;
+        if (tm[c] < t) {
+          L = c;
+          break;
+        }
+        R = c;
+      }
+    } else {
+      L = g;
+      for (int64_t step = 1;; step *= 2) {
+        const int64_t c = L + step;
+        if (c >= R) break;
+        if (tm[c] >= t) {
+          R = c;
+          break;
+        }
+        L = c;
+      }
+    }
+    while (R - L > 1) {
+      const int64_t m = L + (R - L) / 2;
+      if (tm[m] < t) L = m;
+      else R = m;
+    }
+    return R;
+  };
+  // The slices of tiles [k0, k1): per net the first event at or after the
+  // interval start (i0) and end (i1), and the slice [lo, hi) loaded.  `from`,
+  // when given, holds each net's i0 (the i1 of the interval before).
+  auto slice_bounds = [&](int32_t k0, int32_t k1, const int64_t* from, std::vector<int64_t>& lo,
+                          std::vector<int64_t>& hi, std::vector<int64_t>& I0, std::vector<int64_t>& I1) {
     const int64_t t0 = (int64_t)k0 * TC * T;
     const int64_t t1 = k1 >= n_tile ? INT64_MAX : (int64_t)k1 * TC * T;
-    int64_t total = 0;
     std::atomic<int64_t> tot{0};
     parallel_for(N, x->threads, [&](int64_t n) {
       const int64_t a = off[n], b = off[n + 1];
-      const int64_t i0 = std::lower_bound(tm + a, tm + b, t0) - tm;
-      const int64_t i1 = t1 == INT64_MAX ? b : std::lower_bound(tm + i0, tm + b, t1) - tm;
+      const int64_t i0 = from ? from[n] : seek(a, b, t0);
+      const int64_t i1 = t1 == INT64_MAX ? b : seek(i0, b, t1);
       int64_t st = a;
       if (i0 - a >= 2) {
         const int64_t cs = (tm[i0 - 1] / T) * T;
@@ -2056,15 +2132,17 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
       }
       lo[n] = st;
       hi[n] = std::min(b, i1 + 2);
+      I0[n] = i0;
+      I1[n] = i1;
       tot += hi[n] - lo[n];
-    });
-    total = tot.load();
-    return total;
+    }, 64);
+    return tot.load();
   };
   // intervals: whole tiles, halved until the slice fits the budget
   std::vector<std::pair<int32_t, int32_t>> ivals;
   int64_t max_words = 0;  // events of the largest interval's load
   {
+    std::vector<int64_t> sb(N), se(N), i0v(N), i1v(N), next(off, off + N);  // next: i0 of the next interval
     const int32_t guess = (int32_t)std::max<int64_t>(1, std::min<int64_t>(n_tile, (E + budget - 1) / std::max<int64_t>(budget, 1) * 2));
     const int32_t step0 = (n_tile + guess - 1) / guess;
     for (int32_t k0 = 0; k0 < n_tile;) {
@@ -2073,7 +2151,7 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
       if (span(k0, k0 + step) >= (int64_t)EV_TIME_INF)
         return x->fail(GTP_ERR_UNSUPPORTED, "gtp_run_sliced: one tile spans 2^29 - 1 ticks or more");
       while (true) {
-        const int64_t need = slice_bounds(k0, k0 + step, sb, se);
+        const int64_t need = slice_bounds(k0, k0 + step, next.data(), sb, se, i0v, i1v);
         if (need <= budget) {
           max_words = std::max(max_words, need);
           break;
@@ -2084,6 +2162,7 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
         step = (step + 1) / 2;
       }
       ivals.emplace_back(k0, k0 + step);
+      next.swap(i1v);
       k0 += step;
     }
   }
@@ -2102,15 +2181,21 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
   // interval while the current one is evaluated (one buffer when two do not
   // fit the device: then one interval after the other).
   struct Slices {
-    std::vector<int64_t> sb, se, so;
+    std::vector<int64_t> sb, se, so, i0, i1;
   };
   Slices sl[2];
   const uint8_t* val = tr->event_value;
+  // each net's i1 of the interval prepared last, the i0 of the next one (the
+  // intervals are prepared in order)
+  std::vector<int64_t> next_i0(off, off + N);
   auto bounds = [&](Slices& P, const std::pair<int32_t, int32_t>& iv) {
     P.sb.resize(N);
     P.se.resize(N);
     P.so.resize(N + 1);
-    slice_bounds(iv.first, iv.second, P.sb, P.se);
+    P.i0.resize(N);
+    P.i1.resize(N);
+    slice_bounds(iv.first, iv.second, next_i0.data(), P.sb, P.se, P.i0, P.i1);
+    std::copy(P.i1.begin(), P.i1.end(), next_i0.begin());
     P.so[0] = 0;
     for (int64_t n = 0; n < N; ++n) P.so[n + 1] = P.so[n] + (P.se[n] - P.sb[n]);
   };
@@ -2121,8 +2206,8 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
     const bool to_end = iv.second >= n_tile;
     const int64_t c1 = (int64_t)iv.second * TC, t1 = c1 * T;              // interval end (not the last)
     const int64_t a = P.sb[n], b = P.se[n];
-    const int64_t i0 = std::lower_bound(tm + a, tm + b, t0) - tm;
-    const int64_t i1 = to_end ? b : std::lower_bound(tm + i0, tm + b, t1) - tm;
+    const int64_t i0 = P.i0[n];
+    const int64_t i1 = to_end ? b : P.i1[n];
     // before the interval: the event before the group (a) at or after the
     // start of cycle c0 - K - 1, the group (a + 1 .. i0 - 1, one cycle)
     // moved to cycle c0 - K when it lies earlier
@@ -2365,6 +2450,7 @@ gtp_status batch_add(gtp_batch* b, const gtp_trace* tr, gtp_result* res) {
   const int64_t need = b->words_used + E + EV_PAD;
   if ((int64_t)b->d_words.n < need) {
     HIPCHK(x, hipStreamSynchronize(x->stream));
+    if (x->aux) HIPCHK(x, hipStreamSynchronize(x->aux));  // the staged copies into the old buffer
     DArr<uint32_t> nw;
     HIPCHK(x, nw.alloc((size_t)std::max<int64_t>(need, std::min<int64_t>(2 * (int64_t)b->d_words.n, b->budget + EV_PAD))));
     if (b->words_used > 0)
