@@ -71,6 +71,57 @@ void pack_words_nt(uint32_t* d, const int64_t* tm, const uint8_t* val, int64_t n
 
 // f(i) for i in [0, n) on up to `threads` host threads, each given at least
 // `grain` items (1024 for light items; a few for items that are whole nets)
+// pack_words_nt that also checks the events: times strictly increasing from
+// t_prev (the time of the net's event before tm[0], -1 when none) and at most
+// t_max, values at most 3.  Returns false when one is not.
+bool pack_words_checked(uint32_t* d, const int64_t* tm, const uint8_t* val, int64_t n, int64_t base, int64_t t_prev,
+                        int64_t t_max) {
+  if (n <= 0) return true;
+  bool ok = tm[0] > t_prev;
+  int64_t i = 0;
+  for (; i < n && ((uintptr_t)(d + i) & 15u) != 0; ++i) {
+    ok &= (i == 0 || tm[i] > tm[i - 1]) && tm[i] <= t_max && val[i] <= 3;
+    d[i] = ((uint32_t)(tm[i] - base) << EV_SHIFT) | (uint32_t)(val[i] & 3u);
+  }
+  // sign bits: t - t_prev - 1 < 0 (not increasing) or t_max - t < 0
+  __m128i bad = _mm_setzero_si128();
+  uint32_t vor = 0;
+  const __m128i vb = _mm_set1_epi64x(base), one = _mm_set1_epi64x(1), tmx = _mm_set1_epi64x(t_max);
+  const __m128i m3 = _mm_set1_epi32(3), z = _mm_setzero_si128();
+  if (i == 0 && i + 4 <= n) {  // the first quad has no word before it in tm: checked on its own
+    ok &= tm[1] > tm[0] && tm[2] > tm[1] && tm[3] > tm[2];
+  }
+  for (; i + 4 <= n; i += 4) {
+    const __m128i ta = _mm_loadu_si128(reinterpret_cast<const __m128i*>(tm + i));
+    const __m128i tb = _mm_loadu_si128(reinterpret_cast<const __m128i*>(tm + i + 2));
+    if (i > 0) {
+      const __m128i pa = _mm_loadu_si128(reinterpret_cast<const __m128i*>(tm + i - 1));
+      const __m128i pb = _mm_loadu_si128(reinterpret_cast<const __m128i*>(tm + i + 1));
+      bad = _mm_or_si128(bad, _mm_sub_epi64(_mm_sub_epi64(ta, pa), one));
+      bad = _mm_or_si128(bad, _mm_sub_epi64(_mm_sub_epi64(tb, pb), one));
+    }
+    bad = _mm_or_si128(bad, _mm_sub_epi64(tmx, ta));
+    bad = _mm_or_si128(bad, _mm_sub_epi64(tmx, tb));
+    const __m128i a = _mm_sub_epi64(ta, vb), b = _mm_sub_epi64(tb, vb);
+    const __m128i lo = _mm_unpacklo_epi64(_mm_shuffle_epi32(a, _MM_SHUFFLE(2, 0, 2, 0)),
+                                          _mm_shuffle_epi32(b, _MM_SHUFFLE(2, 0, 2, 0)));
+    uint32_t v4;
+    std::memcpy(&v4, val + i, 4);
+    vor |= v4;
+    const __m128i vv = _mm_and_si128(_mm_unpacklo_epi16(_mm_unpacklo_epi8(_mm_cvtsi32_si128((int)v4), z), z), m3);
+    _mm_stream_si128(reinterpret_cast<__m128i*>(d + i), _mm_or_si128(_mm_slli_epi32(lo, EV_SHIFT), vv));
+  }
+  for (; i < n; ++i) {
+    ok &= (i == 0 || tm[i] > tm[i - 1]) && tm[i] <= t_max && val[i] <= 3;
+    d[i] = ((uint32_t)(tm[i] - base) << EV_SHIFT) | (uint32_t)(val[i] & 3u);
+  }
+  _mm_sfence();
+  // the sign bit of each 64-bit lane: bytes 7 and 15
+  ok &= (_mm_movemask_epi8(bad) & 0x8080) == 0;
+  ok &= (vor & 0xFCFCFCFCu) == 0;
+  return ok;
+}
+
 template <class F>
 void parallel_for(int64_t n, int threads, F f, int64_t grain = 1024) {
   if (n <= 0) return;
@@ -675,7 +726,10 @@ int64_t ticks_per_cycle(double clk_period, double time_scale) {
 }
 
 // Checks one trace against the design; *T_out = ticks per cycle.
-gtp_status validate_trace(gtp_ctx* x, const gtp_trace* tr, int64_t* T_out, bool long_ok = false) {
+// scan_events = false leaves the per-event checks to the caller (a sliced
+// run checks each event as it packs it)
+gtp_status validate_trace(gtp_ctx* x, const gtp_trace* tr, int64_t* T_out, bool long_ok = false,
+                          bool scan_events = true) {
   if (!tr || !tr->net_event_offset || tr->n_net != x->N)
     return x->fail(GTP_ERR_ARG, "gtp_trace: n_net must equal the design's n_net");
   if (!(tr->time_scale > 0) || !(tr->clk_period > 0) || tr->t_max <= 0)
@@ -691,7 +745,7 @@ gtp_status validate_trace(gtp_ctx* x, const gtp_trace* tr, int64_t* T_out, bool 
   const int64_t E = off[N];
   if (E > 0 && (!tr->event_time || !tr->event_value)) return x->fail(GTP_ERR_ARG, "gtp_trace: event arrays missing");
   std::atomic<int64_t> bad{-1};
-  parallel_for(N, x->threads, [&](int64_t n) {
+  if (scan_events) parallel_for(N, x->threads, [&](int64_t n) {
     int64_t prev = -1;
     for (int64_t e = off[n]; e < off[n + 1]; ++e) {
       const int64_t t = tr->event_time[e];
@@ -2012,7 +2066,8 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
   const int64_t N = x->N, G = x->G;
   int64_t T = 0;
   {
-    const gtp_status st = validate_trace(x, tr, &T, true);
+    // (each event is checked as it is packed, below)
+    const gtp_status st = validate_trace(x, tr, &T, true, false);
     if (st != GTP_OK) return st;
   }
   const int64_t* off = tr->net_event_offset;
@@ -2199,6 +2254,8 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
     P.so[0] = 0;
     for (int64_t n = 0; n < N; ++n) P.so[n + 1] = P.so[n] + (P.se[n] - P.sb[n]);
   };
+  // the first net found with events out of order, past t_max or above 3
+  std::atomic<int64_t> bad_net{-1};
   // net n's slice of interval iv as event words, times relative to the origin
   auto pack_net = [&](const Slices& P, const std::pair<int32_t, int32_t>& iv, int64_t n, uint32_t* d) {
     const int64_t base = (int64_t)origin_tile(iv.first) * TC * T;
@@ -2229,8 +2286,13 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
       const int64_t t = i > a ? tm[i] - gshift : std::max(tm[i], t_before);
       *d++ = ((uint32_t)(t - base) << EV_SHIFT) | (uint32_t)(val[i] & 3u);
     }
-    // the interval's own events (the bulk): times relative to the origin
-    pack_words_nt(d, tm + i0, val + i0, i1 - i0, base);
+    // the interval's own events (the bulk): times relative to the origin.
+    // The bulks of the intervals partition each net's events, so checking
+    // each bulk and the event before it checks the whole trace.
+    if (!pack_words_checked(d, tm + i0, val + i0, i1 - i0, base, i0 > off[n] ? tm[i0 - 1] : -1, tr->t_max)) {
+      int64_t expect = -1;
+      bad_net.compare_exchange_strong(expect, n);
+    }
     d += i1 - i0;
     for (int64_t i = i1; i < b; ++i) {
       const int64_t t = (i == i1 || tm[i] / T == tm[i1] / T) ? tm[i] - ashift : std::min(tm[i], (ac_new + 1) * T);
@@ -2272,8 +2334,12 @@ GTP_API gtp_status gtp_run_sliced(gtp_ctx* x, const gtp_trace* tr, int64_t max_e
     Slices& P = sl[i & 1];
     bounds(P, ivals[i]);
     const auto iv = ivals[i];
-    return staged_upload(x, P.so.data(), N, d_words[i % n_buf].p,
-                         [&](int64_t n, uint32_t* d) { pack_net(P, iv, n, d); }, ready[i & 1]);
+    const gtp_status st = staged_upload(x, P.so.data(), N, d_words[i % n_buf].p,
+                                        [&](int64_t n, uint32_t* d) { pack_net(P, iv, n, d); }, ready[i & 1]);
+    if (st == GTP_OK && bad_net.load() >= 0)
+      return x->fail(GTP_ERR_ARG, "gtp_trace: events of net " + std::to_string(bad_net.load()) +
+                                      " are not strictly increasing in [0, t_max] or carry a value > 3");
+    return st;
   };
   {
     const gtp_status st = prepare(0);
