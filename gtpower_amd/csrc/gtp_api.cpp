@@ -777,8 +777,12 @@ gtp_status staging_init(gtp_ctx* x);
 template <class F>
 gtp_status staged_upload(gtp_ctx* x, const int64_t* so, int64_t N, uint32_t* dst, F pack_net, hipEvent_t done);
 // traces of at least this many events are uploaded through the page-locked
-// staging ring (smaller ones from a pageable buffer, without the ring's 512 MiB)
-constexpr int64_t STAGED_MIN_EVENTS = (int64_t)1 << 25;
+// staging ring (smaller ones from a pageable buffer, without the ring's
+// 512 MiB); GTP_STAGED_MIN_EVENTS sets another threshold
+int64_t staged_min_events() {
+  if (const char* e = std::getenv("GTP_STAGED_MIN_EVENTS")) return std::max<long long>(0, std::atoll(e));
+  return (int64_t)1 << 25;
+}
 }  // namespace
 
 GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
@@ -787,8 +791,10 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
   HIPCHK(x, hipSetDevice(x->device));
   x->have_trace = false;
   int64_t T = 0;
+  // a trace for the staging ring has its events checked as they are packed
+  const bool staged = tr && tr->net_event_offset && tr->n_net == x->N && tr->net_event_offset[x->N] >= staged_min_events();
   {
-    const gtp_status st = validate_trace(x, tr, &T);
+    const gtp_status st = validate_trace(x, tr, &T, false, !staged);
     if (st != GTP_OK) return st;
   }
   const int64_t N = x->N;
@@ -809,7 +815,7 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
   HIPCHK(x, x->d_ev.alloc(E + EV_PAD));
   const int64_t* tm = tr->event_time;
   const uint8_t* val = tr->event_value;
-  if (E >= STAGED_MIN_EVENTS) {
+  if (staged) {
     // packed by the host threads into the page-locked ring, a run of nets
     // per buffer, each buffer copied while the next is packed (4 B per event
     // uploaded instead of 9)
@@ -823,13 +829,21 @@ GTP_API gtp_status gtp_set_trace(gtp_ctx* x, const gtp_trace* tr) {
       hipEvent_t e;
       ~EvG() { (void)hipEventDestroy(e); }
     } evg{done};
+    std::atomic<int64_t> bad{-1};
     const gtp_status st = staged_upload(x, off, N, x->d_ev.p,
                                         [&](int64_t n, uint32_t* d) {
-                                          pack_words_nt(d, tm + off[n], val + off[n], off[n + 1] - off[n], 0);
+                                          if (!pack_words_checked(d, tm + off[n], val + off[n], off[n + 1] - off[n], 0,
+                                                                  -1, tr->t_max)) {
+                                            int64_t expect = -1;
+                                            bad.compare_exchange_strong(expect, n);
+                                          }
                                         },
                                         done);
     if (st != GTP_OK) return st;
     HIPCHK(x, hipEventSynchronize(done));
+    if (bad.load() >= 0)
+      return x->fail(GTP_ERR_ARG, "gtp_trace: events of net " + std::to_string(bad.load()) +
+                                      " are not strictly increasing in [0, t_max] or carry a value > 3");
     return finish_trace(x);
   }
   // The event words are packed on the host a chunk at a time (4 B per event

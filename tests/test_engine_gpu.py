@@ -367,10 +367,15 @@ def test_silent_trace_and_a_single_partial_cycle():
     assert res.n_cycle == 1
 
 
-def test_invalid_traces_are_rejected_and_the_context_stays_usable():
+@pytest.mark.parametrize("staged", [False, True])
+def test_invalid_traces_are_rejected_and_the_context_stays_usable(monkeypatch, staged):
     """Each malformed trace is refused with an error status (never a crash),
-    by the resident and the sliced entry points; a valid trace works after."""
-    print("[errors] malformed traces", flush=True)
+    by the resident and the sliced entry points; a valid trace works after.
+    staged: the resident load goes through the page-locked staging ring,
+    which checks each event as it packs it (large traces take that path)."""
+    if staged:
+        monkeypatch.setenv("GTP_STAGED_MIN_EVENTS", "1")
+    print(f"[errors] malformed traces (staged={staged})", flush=True)
     d = S.make_design(60, dff_fraction=0.1, clock_ns=1.0, seed=51)
     tr = d.trace(10, toggle_p=0.3, seed=52)
     T = tr.ticks_per_cycle
