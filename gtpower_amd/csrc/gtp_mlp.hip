@@ -545,18 +545,32 @@ __global__ __launch_bounds__(256) void net_weight_sum_kernel(const int64_t* __re
                                                              int32_t n_win, uint32_t cyc_m, int cyc_sh, int64_t c0,
                                                              int64_t n_cyc, unsigned long long* __restrict__ out) {
   __shared__ long long bins[WS_CYCLES];
+  // the block's nets: weight, first event and the window's event range,
+  // loaded by all threads at once (one dependent chain for the block instead
+  // of one per net and wave)
+  __shared__ long long s_w[WS_NETS];
+  __shared__ int64_t s_base[WS_NETS];
+  __shared__ int32_t s_i0[WS_NETS], s_i1[WS_NETS];
   const int j = blockIdx.y;
   const int64_t cs = (int64_t)j * WS_CYCLES;
   const int nr = (int)min((int64_t)WS_CYCLES, n_cyc - cs);
+  const int k0 = (int)blockIdx.x * WS_NETS;
+  const int nk = min(n_sel - k0, WS_NETS);
+  for (int q = threadIdx.x; q < nk; q += blockDim.x) {
+    const int k = k0 + q;
+    s_w[q] = wq[k];
+    s_base[q] = net_off[sel[k]];
+    s_i0[q] = idx[(int64_t)k * (n_win + 1) + j];
+    s_i1[q] = idx[(int64_t)k * (n_win + 1) + j + 1];
+  }
   for (int i = threadIdx.x; i < nr; i += blockDim.x) bins[i] = 0;
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, n_wave = blockDim.x >> 6;
   const int64_t cbase = c0 + cs;
-  const int k_end = min(n_sel, (int)(blockIdx.x + 1) * WS_NETS);
-  for (int k = (int)blockIdx.x * WS_NETS + wave; k < k_end; k += n_wave) {
-    const long long w = wq[k];
-    const uint32_t* e = ev + net_off[sel[k]];
-    const int32_t i0 = idx[(int64_t)k * (n_win + 1) + j], i1 = idx[(int64_t)k * (n_win + 1) + j + 1];
+  for (int q = wave; q < nk; q += n_wave) {
+    const long long w = s_w[q];
+    const uint32_t* e = ev + s_base[q];
+    const int32_t i0 = s_i0[q], i1 = s_i1[q];
     // four loads in flight per lane before their events are binned (the
     // loop is bound by load latency, not by the LDS atomics)
     for (int32_t i = i0 + lane; i < i1; i += 4 * 64) {
