@@ -2498,7 +2498,8 @@ gtp_status batch_add(gtp_batch* b, const gtp_trace* tr, gtp_result* res) {
   const int64_t N = x->N, G = x->G;
   int64_t Tk = 0;
   {
-    const gtp_status st = validate_trace(x, tr, &Tk, true);
+    // (each event is checked as it is packed, below, or by the sliced run)
+    const gtp_status st = validate_trace(x, tr, &Tk, true, false);
     if (st != GTP_OK) return st;
   }
   if (Tk != b->T || tr->time_scale != b->time_scale || tr->clk_period != b->clk_period)
@@ -2508,7 +2509,6 @@ gtp_status batch_add(gtp_batch* b, const gtp_trace* tr, gtp_result* res) {
   const int64_t C = (tr->t_max + b->T - 1) / b->T;
   const int32_t n_tile = (int32_t)((C + b->TC - 1) / b->TC);
   const int64_t words = E + N * ((int64_t)n_tile + 1) + 4 * N;
-  b->n_event += E;
   if (tr->t_max >= (int64_t)EV_TIME_INF || words > b->budget) {
     // longer than one event word's time span, or larger than one load: by
     // itself, in cycle intervals
@@ -2518,6 +2518,7 @@ gtp_status batch_add(gtp_batch* b, const gtp_trace* tr, gtp_result* res) {
     }
     const gtp_status st = gtp_run_sliced(x, tr, b->budget, res);
     if (st != GTP_OK) return st;
+    b->n_event += E;
     b->kernel_ms += res->kernel_ms;
     b->n_load += (int32_t)x->sched.n_interval;
     return GTP_OK;
@@ -2542,13 +2543,24 @@ gtp_status batch_add(gtp_batch* b, const gtp_trace* tr, gtp_result* res) {
   const int64_t* tm = tr->event_time;
   const uint8_t* val = tr->event_value;
   {
+    // a trace with events out of order, past t_max or above 3 is refused; its
+    // words lie past words_used, where the next trace's go
+    std::atomic<int64_t> bad{-1};
     const gtp_status st = staged_upload(x, off, N, b->d_words.p + b->words_used,
                                         [&](int64_t n, uint32_t* d) {
-                                          pack_words_nt(d, tm + off[n], val + off[n], off[n + 1] - off[n], 0);
+                                          if (!pack_words_checked(d, tm + off[n], val + off[n], off[n + 1] - off[n], 0,
+                                                                  -1, tr->t_max)) {
+                                            int64_t expect = -1;
+                                            bad.compare_exchange_strong(expect, n);
+                                          }
                                         },
                                         b->ready);
     if (st != GTP_OK) return st;
+    if (bad.load() >= 0)
+      return x->fail(GTP_ERR_ARG, "gtp_trace: events of net " + std::to_string(bad.load()) +
+                                      " are not strictly increasing in [0, t_max] or carry a value > 3");
   }
+  b->n_event += E;
   // the staged copies read the caller's arrays only through the staging
   // buffers, which are filled before staged_upload returns
   b->words_used += E;

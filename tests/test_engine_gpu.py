@@ -370,7 +370,8 @@ def test_silent_trace_and_a_single_partial_cycle():
 @pytest.mark.parametrize("staged", [False, True])
 def test_invalid_traces_are_rejected_and_the_context_stays_usable(monkeypatch, staged):
     """Each malformed trace is refused with an error status (never a crash),
-    by the resident and the sliced entry points; a valid trace works after.
+    by the resident, sliced and streamed-batch entry points; a valid trace
+    works after.
     staged: the resident load goes through the page-locked staging ring,
     which checks each event as it packs it (large traces take that path)."""
     if staged:
@@ -406,8 +407,13 @@ def test_invalid_traces_are_rejected_and_the_context_stays_usable(monkeypatch, s
                 eng.read_power_activities(bad)
             with pytest.raises(GtpError):
                 eng.power_sliced(bad, max_events=10 ** 6)
+            with pytest.raises(GtpError):
+                eng.power_batch_stream([tr, bad])
         eng.read_power_activities(tr)
         res = eng.power()
+        streamed, _ = eng.power_batch_stream([tr, tr])
+        for r in streamed:
+            assert np.array_equal(r.cycle_power, res.cycle_power) and np.array_equal(r.gate_power, res.gate_power)
     ref = oracle.run(d.library, d, tr, threads=0)
     compare(res, ref, "after rejected traces")
 
