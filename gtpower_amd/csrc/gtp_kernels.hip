@@ -2037,8 +2037,8 @@ __global__ void gate_reduce_kernel(const long long* __restrict__ pg, const uint3
                                    unsigned long long* __restrict__ gate_tog,
                                    unsigned long long* __restrict__ gate_glitch) {
   // one thread per (schedule position, array): the N_PG sums, then toggles and
-  // glitches (blockIdx.y), each over the tiles with four independent
-  // accumulators (integer sums: the order does not matter)
+  // glitches (blockIdx.y), each over the tiles, 16 loads in flight into four
+  // independent accumulators (integer sums: the order does not matter)
   const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gi >= n_sched) return;
   const int g = order[gi];
@@ -2049,11 +2049,17 @@ __global__ void gate_reduce_kernel(const long long* __restrict__ pg, const uint3
     const long long* p = pg + k * plane + gi;
     long long v0 = 0, v1 = 0, v2 = 0, v3 = 0;
     int t = 0;
-    for (; t + 4 <= n_tile; t += 4) {
-      v0 += p[(int64_t)t * n_sched];
-      v1 += p[(int64_t)(t + 1) * n_sched];
-      v2 += p[(int64_t)(t + 2) * n_sched];
-      v3 += p[(int64_t)(t + 3) * n_sched];
+    for (; t + 16 <= n_tile; t += 16) {
+      long long x[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) x[u] = p[(int64_t)(t + u) * n_sched];
+#pragma unroll
+      for (int u = 0; u < 16; u += 4) {
+        v0 += x[u];
+        v1 += x[u + 1];
+        v2 += x[u + 2];
+        v3 += x[u + 3];
+      }
     }
     for (; t < n_tile; ++t) v0 += p[(int64_t)t * n_sched];
     const long long v = (v0 + v1) + (v2 + v3);
@@ -2064,11 +2070,17 @@ __global__ void gate_reduce_kernel(const long long* __restrict__ pg, const uint3
     const uint32_t* p = (k == N_PG ? tog : glc) + gi;
     unsigned long long v0 = 0, v1 = 0, v2 = 0, v3 = 0;
     int t = 0;
-    for (; t + 4 <= n_tile; t += 4) {
-      v0 += p[(int64_t)t * n_sched];
-      v1 += p[(int64_t)(t + 1) * n_sched];
-      v2 += p[(int64_t)(t + 2) * n_sched];
-      v3 += p[(int64_t)(t + 3) * n_sched];
+    for (; t + 16 <= n_tile; t += 16) {
+      uint32_t x[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) x[u] = p[(int64_t)(t + u) * n_sched];
+#pragma unroll
+      for (int u = 0; u < 16; u += 4) {
+        v0 += x[u];
+        v1 += x[u + 1];
+        v2 += x[u + 2];
+        v3 += x[u + 3];
+      }
     }
     for (; t < n_tile; ++t) v0 += p[(int64_t)t * n_sched];
     (k == N_PG ? gate_tog : gate_glitch)[g] = (v0 + v1) + (v2 + v3);
